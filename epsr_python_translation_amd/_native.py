@@ -1,0 +1,119 @@
+"""Build and load libepsr_mi355x.so (the HIP kernels + C ABI) for gfx950.
+
+The library is built in-tree next to this file.  Loading never falls back to anything:
+if the shared library is missing or fails to load, the GPU entry points raise.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import pathlib
+import shutil
+import subprocess
+
+HERE = pathlib.Path(__file__).resolve().parent
+CSRC = HERE / "csrc"
+LIB_PATH = HERE / "libepsr_mi355x.so"
+INCLUDE = HERE.parent / "include"
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    # the reference's float32 contract: no contraction to FMA, correctly rounded
+    # division and square root, no fast-math
+    "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt", "-fno-fast-math",
+    # keep packed-FP32 instructions out of the generated code
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def sources():
+    return sorted(CSRC.glob("*.hip"))
+
+
+def build(verbose: bool = False) -> pathlib.Path:
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    tmp = LIB_PATH.with_suffix(".so.tmp")
+    cmd = [hipcc, *HIPCC_FLAGS, f"-I{INCLUDE}", "-o", str(tmp), *map(str, sources())]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    env = dict(os.environ)
+    r = subprocess.run(cmd, capture_output=True, text=True, env=env)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed ({r.returncode}):\n{r.stderr[-6000:]}")
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+_LIB = None
+
+
+def lib():
+    """The loaded library; raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() (or _native.build())")
+        _LIB = C.CDLL(str(LIB_PATH))
+        _declare(_LIB)
+    return _LIB
+
+
+vp = C.c_void_p
+ip = C.POINTER(C.c_int)
+i32p = C.POINTER(C.c_int32)
+fp = C.POINTER(C.c_float)
+dp = C.POINTER(C.c_double)
+
+
+class BoxDesc(C.Structure):
+    _fields_ = [("nmol", C.c_int), ("natom", C.c_int), ("ntype", C.c_int), ("npairs", C.c_int),
+                ("nmoltype", C.c_int), ("itype", ip), ("naf", ip), ("nal", ip), ("mtype", ip),
+                ("npairref", ip), ("n1extr", ip), ("n2extr", ip), ("cfrac", fp),
+                ("natomr", C.c_float), ("sizefactor", C.c_float), ("dmax", C.c_float),
+                ("cellst", C.c_float)]
+
+
+class Coords(C.Structure):
+    _fields_ = [("xa", vp), ("ya", vp), ("za", vp), ("xadd", vp), ("yadd", vp), ("zadd", vp)]
+
+
+# name -> (restype, argtypes); every symbol declared in include/epsr_mi355x.h
+SIGNATURES = {
+    "epsr_ctx_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+    "epsr_ctx_destroy": (None, [vp]),
+    "epsr_last_error": (C.c_char_p, [vp]),
+    "epsr_ctx_set_stream": (C.c_int, [vp, vp]),
+    "epsr_ctx_synchronize": (C.c_int, [vp]),
+    "epsr_ctx_last_kernel_ms": (C.c_double, [vp, C.c_char_p]),
+    "epsr_ctx_enable_timing": (C.c_int, [vp, C.c_int]),
+    "epsr_box_create": (C.c_int, [vp, C.POINTER(BoxDesc), C.POINTER(vp)]),
+    "epsr_box_destroy": (None, [vp]),
+    "epsr_ran1_stream": (C.c_int, [i32p, C.c_int, fp]),
+    "epsr_atogen": (C.c_int, [vp, vp, C.POINTER(Coords), C.c_int, C.c_int, vp, vp, vp, ip, i32p, vp, vp, vp, vp]),
+    "epsr_gensq": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp, C.c_int, vp, vp, C.c_int, C.c_int]),
+    "epsr_stog": (C.c_int, [vp, C.c_float, vp, vp, C.c_int, vp, vp, C.c_int, vp, vp, C.c_int, C.c_int, vp,
+                            C.c_int]),
+    "epsr_foqols": (C.c_int, [vp, C.c_int, vp, C.c_int, vp, vp, vp, C.c_float, C.c_float, C.c_float, C.c_float,
+                              C.c_float, C.c_float, vp, C.c_int]),
+}
+
+
+def _declare(L):
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def header_symbols():
+    """Function names declared in include/*.h (for the export test)."""
+    import re
+
+    names = set()
+    for h in INCLUDE.glob("*.h"):
+        text = h.read_text()
+        for m in re.finditer(r"^\s*(?:[A-Za-z_][\w\s\*]*?)\b(epsr_\w+)\s*\(", text, re.M):
+            names.add(m.group(1))
+    return names

@@ -1,0 +1,325 @@
+// atogen: pair-distance histograms of one box (the reference's
+// src/epsr26/library/ato_manipulation_rev4.py:3640-3899).
+//
+// Work split
+//  * host: the origin selection, which is a serial ran1 walk over the molecules
+//    (:3735-3779); O(nmol) integer work, no pair touched.
+//  * GPU: absolute atom positions, then the O(N_origin * N) inter-molecular pair loop
+//    as a tiled all-pairs kernel: one origin atom per lane, partner atoms staged
+//    through LDS 256 at a time (one float4 per atom), per-workgroup uint32 histograms
+//    in LDS, merged into the global counts with atomics.  Intra-molecular pairs run
+//    in a small kernel of their own.  A last kernel turns counts into the normalised
+//    float32 wav of the reference.
+//  * counts are exact integers; the reference's float32 accumulation of equal
+//    power-of-two increments equals increment * min(count, 2^24), which the
+//    normalisation kernel reproduces.
+#include "common.h"
+
+#define ATOGEN_BLOCK 256
+#define ATOGEN_TILE 256
+
+// absolute positions x2 = xa + xadd (one fp32 add, as :3794-3796), type in .w
+__global__ void atogen_positions(const float *__restrict__ xa, const float *__restrict__ ya,
+                                 const float *__restrict__ za, const float *__restrict__ xadd,
+                                 const float *__restrict__ yadd, const float *__restrict__ zadd,
+                                 const int *__restrict__ amol, const int *__restrict__ itype,
+                                 float4 *__restrict__ P, int natom)
+{
+    int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= natom) return;
+    int m = amol[a];
+    P[a] = make_float4(xa[a] + xadd[m], ya[a] + yadd[m], za[a] + zadd[m], __int_as_float(itype[a]));
+}
+
+// intramolecular pairs (:3698-3724): one thread per molecule
+__global__ void atogen_intra(const float *__restrict__ xa, const float *__restrict__ ya,
+                             const float *__restrict__ za, const int *__restrict__ naf,
+                             const int *__restrict__ nal, const int *__restrict__ itype,
+                             const int *__restrict__ npairref, int ntype, int nmol, float sizefactor,
+                             float stepcl, int npt1, int ldw, uint32_t *__restrict__ counts)
+{
+    int im = blockIdx.x * blockDim.x + threadIdx.x;
+    if (im >= nmol) return;
+    int a0 = naf[im], a1 = nal[im];
+    for (int ia = a0; ia < a1; ++ia) {
+        float x1 = xa[ia], y1 = ya[ia], z1 = za[ia];
+        int it = itype[ia];
+        for (int ja = ia + 1; ja <= a1; ++ja) {
+            int icref = npairref[it * ntype + itype[ja]];
+            float xx = dev_mic((x1 - xa[ja]) * sizefactor);
+            float yy = dev_mic((y1 - ya[ja]) * sizefactor);
+            float zz = dev_mic((z1 - za[ja]) * sizefactor);
+            float dtest = sqrtf(xx * xx + yy * yy + zz * zz);
+            int iz1 = dev_nint(dtest / stepcl) + 2;
+            if (iz1 > 1 && iz1 <= npt1) atomicAdd(&counts[(size_t)icref * ldw + iz1], 1u);
+        }
+    }
+}
+
+// intermolecular pairs (:3780-3837).
+// origin atoms: oatoms[0..norigin); for origin atom ia the partners are the atoms
+// ja >= lo (lo = nal(mol(ia))+1 when every molecule is an origin, else 0) whose
+// molecule differs from mol(ia).  blockIdx.x: block of 256 origin atoms;
+// blockIdx.y: slice of the partner range.
+__global__ void __launch_bounds__(ATOGEN_BLOCK)
+atogen_inter(const float4 *__restrict__ P, const int *__restrict__ amol, const int *__restrict__ nal,
+             const int *__restrict__ oatoms, int norigin, int full, int natom,
+             const int *__restrict__ npairref_g, int ntype, int npairs, int npt1, int ldw,
+             float stepcl, float dmax, const float *__restrict__ rminex_abs, int slice,
+             uint32_t *__restrict__ counts, uint32_t *__restrict__ potex_counts)
+{
+    extern __shared__ uint32_t smem[];
+    const int ldh = npt1 + 1;
+    uint32_t *hist = smem;                                    // [npairs*ldh]
+    uint32_t *spot = hist + (size_t)npairs * ldh;             // [npairs]
+    int *snp = (int *)(spot + npairs);                        // [ntype*ntype]
+    float *srm = (float *)(snp + ntype * ntype);              // [npairs]
+    float4 *tileP = (float4 *)(((uintptr_t)(srm + npairs) + 15) & ~(uintptr_t)15);
+    int *tileM = (int *)(tileP + ATOGEN_TILE);
+
+    const int tid = threadIdx.x;
+    for (int i = tid; i < npairs * ldh; i += ATOGEN_BLOCK) hist[i] = 0u;
+    for (int i = tid; i < npairs; i += ATOGEN_BLOCK) {
+        spot[i] = 0u;
+        srm[i] = rminex_abs[i];
+    }
+    for (int i = tid; i < ntype * ntype; i += ATOGEN_BLOCK) snp[i] = npairref_g[i];
+
+    const int oi = blockIdx.x * ATOGEN_BLOCK + tid;
+    const bool active = oi < norigin;
+    float4 p1 = make_float4(0.f, 0.f, 0.f, 0.f);
+    int mymol = -1, lo = natom;
+    int it = 0;
+    if (active) {
+        int ia = oatoms[oi];
+        p1 = P[ia];
+        mymol = amol[ia];
+        lo = full ? nal[mymol] + 1 : 0;
+        it = __float_as_int(p1.w);
+    }
+    // partner range handled by this workgroup
+    int blo = natom;
+    {
+        // smallest lo of the block (full mode): the block's first origin atom has it
+        int first = blockIdx.x * ATOGEN_BLOCK;
+        if (full) blo = nal[amol[oatoms[first]]] + 1;
+        else blo = 0;
+    }
+    const int span = natom - blo;
+    const int per = (span + gridDim.y - 1) / gridDim.y;
+    const int p0 = blo + blockIdx.y * per;
+    const int p1e = min(natom, p0 + per);
+    __syncthreads();
+
+    for (int t0 = p0; t0 < p1e; t0 += ATOGEN_TILE) {
+        int n = min(ATOGEN_TILE, p1e - t0);
+        __syncthreads();
+        if (tid < n) {
+            tileP[tid] = P[t0 + tid];
+            tileM[tid] = amol[t0 + tid];
+        }
+        __syncthreads();
+        if (!active) continue;
+        for (int k = 0; k < n; ++k) {
+            int ja = t0 + k;
+            float4 p2 = tileP[k];
+            int m2 = tileM[k];
+            if (ja < lo || m2 == mymol) continue;
+            float xx = dev_mic(p1.x - p2.x);
+            float yy = dev_mic(p1.y - p2.y);
+            float zz = dev_mic(p1.z - p2.z);
+            float dtest = sqrtf(xx * xx + yy * yy + zz * zz);
+            int iz1 = dev_nint(dtest / stepcl) + 2;
+            int icref = snp[it * ntype + __float_as_int(p2.w)];
+            if (iz1 > 1 && iz1 <= npt1) atomicAdd(&hist[icref * ldh + iz1], 1u);
+            if (dtest * dmax < srm[icref]) atomicAdd(&spot[icref], 1u);
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < npairs * ldh; i += ATOGEN_BLOCK) {
+        uint32_t v = hist[i];
+        if (v) {
+            int ic = i / ldh, iz = i - ic * ldh;
+            atomicAdd(&counts[((size_t)npairs + ic) * ldw + iz], v);
+        }
+    }
+    for (int i = tid; i < npairs; i += ATOGEN_BLOCK)
+        if (spot[i]) atomicAdd(&potex_counts[i], spot[i]);
+}
+
+// normalisation (:3859-3897).  norm[ic] = cprod, norm[npairs+ic] = cprod2 (host
+// computed in float32 exactly as the reference); potex follows cprod2.
+__global__ void atogen_normalise(const uint32_t *__restrict__ counts, const uint32_t *__restrict__ pcounts,
+                                 const float *__restrict__ norm, const float *__restrict__ cfrac_jt,
+                                 const float *__restrict__ ycoh, int npairs, int npt1, int ldw,
+                                 float addwav, float *__restrict__ wav, float *__restrict__ potex,
+                                 float *__restrict__ potexmax)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int total = 2 * npairs * npt1;
+    if (i < npairs) {
+        uint32_t pc = pcounts[i];
+        float v = (float)(pc > 16777216u ? 16777216u : pc);
+        float c2 = norm[npairs + i];
+        if (c2 > 0.0f) v = v * cfrac_jt[i] / c2;
+        potex[i] = v;
+        potexmax[i] = 0.0f;
+    }
+    if (i >= total) return;
+    int k = i / (npairs * npt1);
+    int r = i - k * npairs * npt1;
+    int ic = r / npt1;
+    int iz = r - ic * npt1 + 1;
+    size_t o = ((size_t)k * npairs + ic) * ldw + iz;
+    uint32_t c = counts[o];
+    float inc = k == 0 ? 2.0f : addwav;
+    float v = inc * (float)(c > 16777216u ? 16777216u : c);
+    float d = norm[k * npairs + ic];
+    if (d > 0.0f) v = v / d;
+    else v = (k == 0) ? 0.0f : ycoh[iz];
+    wav[o] = v;
+}
+
+extern "C" int epsr_atogen(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *c, int npt1, int ldw,
+                           const float *ycoh, const int *irefpcof, const float *rminex, int *ngrsamples,
+                           int32_t *ran1_state, float *wav, float *potex, float *potexmax, uint32_t *counts_out)
+{
+    if (!ctx) return EPSR_ERR_ARG;
+    ARG_CHECK(ctx, box && c && ycoh && irefpcof && rminex && ngrsamples && ran1_state && wav && potex && potexmax,
+              "epsr_atogen: null argument");
+    ARG_CHECK(ctx, npt1 >= 2 && ldw >= npt1 + 1, "epsr_atogen: need npt1 >= 2 and ldw >= npt1+1 (npt1=%d ldw=%d)", npt1, ldw);
+    HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    const int nmol = box->nmol, natom = box->natom, npairs = box->npairs, ntype = box->ntype;
+    hipStream_t st = ctx->stream;
+
+    // ---- origin selection on the host (:3735-3779), ran1 drawn for every candidate
+    int ngr = *ngrsamples;
+    if (ngr <= 0) ngr = 1000;
+    if (ngr > nmol) ngr = nmol;
+    float probaccept = (float)ngr / (float)nmol;
+    std::vector<int> molsampled(box->nmoltype, 0);
+    std::vector<float> molfraction(box->nmoltype, 0.0f);
+    std::vector<float> csampled(ntype, 0.0f);
+    for (int im = 0; im < nmol; ++im) molfraction[box->h_mtype[im]] = molfraction[box->h_mtype[im]] + 1.0f;
+    for (int t = 0; t < box->nmoltype; ++t) molfraction[t] = molfraction[t] / (float)nmol;
+    int im2 = nmol - 1;
+    bool full = false;
+    float addwav = 1.0f;
+    if (probaccept >= 1.0f) {
+        im2 = nmol - 2;
+        full = true;
+        addwav = 2.0f;
+        for (int t = 0; t < ntype; ++t)
+            csampled[t] = csampled[t] + (float)box->h_type_count[(size_t)(nmol - 1) * ntype + t];
+    }
+    Ran1 rng;
+    ran1_load(rng, ran1_state);
+    std::vector<int> oatoms;
+    oatoms.reserve(full ? natom : (size_t)ngr * box->max_atoms_per_mol * 2);
+    for (int im = 0; im <= im2; ++im) {
+        int mt = box->h_mtype[im];
+        float r = ran1_next(rng);
+        if (r < probaccept || molsampled[mt] < 1 || molfraction[mt] < probaccept) {
+            molsampled[mt] += 1;
+            for (int t = 0; t < ntype; ++t)
+                csampled[t] = csampled[t] + (float)box->h_type_count[(size_t)im * ntype + t];
+            for (int a = box->h_naf[im]; a <= box->h_nal[im]; ++a) oatoms.push_back(a);
+        }
+    }
+    ran1_store(rng, ran1_state);
+    *ngrsamples = ngr;
+
+    // ---- normalisation constants (:3862-3869), float32 as the reference
+    std::vector<float> hnorm(2 * npairs), hcfj(npairs);
+    for (int ic = 0; ic < npairs; ++ic) {
+        int it = box->h_n1extr[ic], jt = box->h_n2extr[ic];
+        float cprod = box->h_cfrac[it] * box->h_cfrac[jt] * box->natomr;
+        float cprod2 = csampled[it] * box->h_cfrac[jt];
+        if (it != jt) {
+            cprod = cprod * 2.0f;
+            cprod2 = cprod2 + csampled[jt] * box->h_cfrac[it];
+        }
+        hnorm[ic] = cprod;
+        hnorm[npairs + ic] = cprod2;
+        hcfj[ic] = box->h_cfrac[jt];
+    }
+
+    // ---- device scratch layout
+    const int norigin = (int)oatoms.size();
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
+    size_t oP = take(sizeof(float4) * natom);
+    size_t oO = take(sizeof(int) * (norigin > 0 ? norigin : 1));
+    size_t oC = counts_out ? 0 : take(sizeof(uint32_t) * 2 * (size_t)npairs * ldw);
+    size_t oPC = take(sizeof(uint32_t) * npairs);
+    size_t oN = take(sizeof(float) * 2 * npairs);
+    size_t oF = take(sizeof(float) * npairs);
+    size_t oR = take(sizeof(float) * npairs);
+    char *scr = (char *)ctx_scratch(ctx, off);
+    if (!scr) return EPSR_ERR_HIP;
+    float4 *dP = (float4 *)(scr + oP);
+    int *dO = (int *)(scr + oO);
+    uint32_t *dC = counts_out ? counts_out : (uint32_t *)(scr + oC);
+    uint32_t *dPC = (uint32_t *)(scr + oPC);
+    float *dN = (float *)(scr + oN), *dF = (float *)(scr + oF), *dR = (float *)(scr + oR);
+
+    // staging of the small host arrays through pinned memory, one copy
+    size_t hb = sizeof(int) * (size_t)norigin + sizeof(float) * 4 * npairs;
+    char *pin = (char *)ctx_pinned(ctx, hb);
+    if (!pin) return EPSR_ERR_HIP;
+    HIP_CHECK(ctx, hipStreamSynchronize(st));   // the pinned buffer may still feed a previous copy
+    memcpy(pin, oatoms.data(), sizeof(int) * (size_t)norigin);
+    float *pf = (float *)(pin + sizeof(int) * (size_t)norigin);
+    memcpy(pf, hnorm.data(), sizeof(float) * 2 * npairs);
+    memcpy(pf + 2 * npairs, hcfj.data(), sizeof(float) * npairs);
+    if (norigin) HIP_CHECK(ctx, hipMemcpyAsync(dO, pin, sizeof(int) * (size_t)norigin, hipMemcpyHostToDevice, st));
+    HIP_CHECK(ctx, hipMemcpyAsync(dN, pf, sizeof(float) * 2 * npairs, hipMemcpyHostToDevice, st));
+    HIP_CHECK(ctx, hipMemcpyAsync(dF, pf + 2 * npairs, sizeof(float) * npairs, hipMemcpyHostToDevice, st));
+
+    // |rminex(irefpcof(ic))| per pair, gathered on the host side of the stream
+    {
+        std::vector<int> hir(npairs);
+        HIP_CHECK(ctx, hipMemcpyAsync(hir.data(), irefpcof, sizeof(int) * npairs, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(ctx, hipStreamSynchronize(st));
+        int maxref = 0;
+        for (int v : hir) maxref = std::max(maxref, v + 1);
+        std::vector<float> hrm(maxref);
+        HIP_CHECK(ctx, hipMemcpyAsync(hrm.data(), rminex, sizeof(float) * maxref, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(ctx, hipStreamSynchronize(st));
+        float *pr = pf + 3 * npairs;
+        for (int ic = 0; ic < npairs; ++ic) pr[ic] = 1.0f * fabsf(hrm[hir[ic]]);
+        HIP_CHECK(ctx, hipMemcpyAsync(dR, pr, sizeof(float) * npairs, hipMemcpyHostToDevice, st));
+    }
+
+    HIP_CHECK(ctx, hipMemsetAsync(dC, 0, sizeof(uint32_t) * 2 * (size_t)npairs * ldw, st));
+    HIP_CHECK(ctx, hipMemsetAsync(dPC, 0, sizeof(uint32_t) * npairs, st));
+    const float stepcl = box->cellst / box->dmax;
+
+    atogen_positions<<<ceil_div(natom, 256), 256, 0, st>>>(c->xa, c->ya, c->za, c->xadd, c->yadd, c->zadd,
+                                                            box->d_amol, box->d_itype, dP, natom);
+    atogen_intra<<<ceil_div(nmol, 128), 128, 0, st>>>(c->xa, c->ya, c->za, box->d_naf, box->d_nal, box->d_itype,
+                                                       box->d_npairref, ntype, nmol, box->sizefactor, stepcl, npt1,
+                                                       ldw, dC);
+    if (norigin > 0) {
+        size_t lds = sizeof(uint32_t) * ((size_t)npairs * (npt1 + 1) + npairs) + sizeof(int) * ntype * ntype +
+                     sizeof(float) * npairs + 16 + ATOGEN_TILE * (sizeof(float4) + sizeof(int));
+        if (lds > 160 * 1024)
+            return set_error(ctx, EPSR_ERR_CAPACITY,
+                             "epsr_atogen: histogram of %d pairs x %d bins does not fit in LDS", npairs, npt1);
+        int gx = ceil_div(norigin, ATOGEN_BLOCK);
+        // partner slices so that the launch has a few workgroups per CU
+        int gy = std::max(1, std::min(64, ceil_div(1024, gx)));
+        gy = std::min(gy, std::max(1, natom / ATOGEN_TILE));
+        timer_begin(ctx, "atogen_inter");
+        hipLaunchKernelGGL(atogen_inter, dim3(gx, gy), dim3(ATOGEN_BLOCK), lds, st, dP, box->d_amol, box->d_nal, dO,
+                           norigin, full ? 1 : 0, natom, box->d_npairref, ntype, npairs, npt1, ldw, stepcl, box->dmax,
+                           dR, 0, dC, dPC);
+        timer_end(ctx, "atogen_inter");
+    }
+    int total = 2 * npairs * npt1;
+    atogen_normalise<<<ceil_div(std::max(total, npairs), 256), 256, 0, st>>>(dC, dPC, dN, dF, ycoh, npairs, npt1, ldw,
+                                                                              addwav, wav, potex, potexmax);
+    HIP_CHECK(ctx, hipGetLastError());
+    HIP_CHECK(ctx, hipStreamSynchronize(st));
+    return EPSR_OK;
+}

@@ -1,0 +1,118 @@
+// Internal definitions shared by the libepsr_mi355x.so translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/epsr_mi355x.h"
+
+// ------------------------------------------------------------------ context
+
+struct KernelTimer {
+    hipEvent_t start = nullptr, stop = nullptr;
+    bool pending = false;
+    double last_ms = -1.0;
+};
+
+struct epsr_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool timing = false;
+    std::map<std::string, KernelTimer> timers;
+    // growable device scratch, reused across calls
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    // growable pinned host staging buffer
+    void *pinned = nullptr;
+    size_t pinned_bytes = 0;
+};
+
+struct epsr_box {
+    epsr_ctx *ctx = nullptr;
+    int nmol = 0, natom = 0, ntype = 0, npairs = 0, nmoltype = 0;
+    float natomr = 0.f, sizefactor = 1.f, dmax = 1.f, cellst = 0.f;
+    // host copies
+    std::vector<int> h_itype, h_naf, h_nal, h_mtype, h_npairref, h_n1extr, h_n2extr;
+    std::vector<float> h_cfrac;
+    std::vector<int> h_type_count;   // [nmol*ntype] atoms of each type per molecule
+    int max_atoms_per_mol = 0;
+    // device copies
+    int *d_itype = nullptr, *d_naf = nullptr, *d_nal = nullptr, *d_mtype = nullptr;
+    int *d_npairref = nullptr, *d_amol = nullptr;
+};
+
+// ------------------------------------------------------------------ errors
+
+int set_error(epsr_ctx *ctx, int code, const char *fmt, ...);
+
+#define HIP_CHECK(ctx, expr)                                                             \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return set_error((ctx), EPSR_ERR_HIP, "%s failed: %s (%s:%d)", #expr,         \
+                             hipGetErrorString(e_), __FILE__, __LINE__);                  \
+    } while (0)
+
+#define ARG_CHECK(ctx, cond, ...)                                                        \
+    do {                                                                                 \
+        if (!(cond)) return set_error((ctx), EPSR_ERR_ARG, __VA_ARGS__);                  \
+    } while (0)
+
+// scratch management (device); returns nullptr and sets the error on failure
+void *ctx_scratch(epsr_ctx *ctx, size_t bytes);
+void *ctx_pinned(epsr_ctx *ctx, size_t bytes);
+
+// kernel timing with HIP events on the launch stream
+void timer_begin(epsr_ctx *ctx, const char *family);
+void timer_end(epsr_ctx *ctx, const char *family);
+
+// ------------------------------------------------------------------ host ran1
+
+struct Ran1 {
+    int32_t mcall, idum, iy, iv[32];
+};
+// src/epsr26/library/auxiliary_routines.py:324-385 (Park-Miller + Bays-Durham shuffle)
+float ran1_next(Ran1 &s);
+void ran1_load(Ran1 &s, const int32_t *a);
+void ran1_store(const Ran1 &s, int32_t *a);
+
+// ------------------------------------------------------------------ device arithmetic
+// The reference's float32 contract (src/epsr26/fortran/fmath.py).  All translation
+// units are compiled with -ffp-contract=off and correctly rounded division and
+// square root, so a*b+c is two roundings and x/y, sqrtf(x) are correctly rounded.
+
+// NINT: round half away from zero (fmath.py:222-239); |x| < 2^31 on this path.
+__device__ __forceinline__ int dev_nint(float x)
+{
+    float t = truncf(x);
+    float f = x - t;   // exact
+    int ti = (int)t;
+    ti += f >= 0.5f ? 1 : 0;
+    ti -= f <= -0.5f ? 1 : 0;
+    return ti;
+}
+
+__device__ __forceinline__ float dev_anint(float x)
+{
+    // branch-free: t is integral, so (t + up) - dn is exact
+    float t = truncf(x);
+    float f = x - t;
+    float up = f >= 0.5f ? 1.0f : 0.0f;
+    float dn = f <= -0.5f ? 1.0f : 0.0f;
+    return (t + up) - dn;
+}
+
+// minimagec, cubic branch (src/epsr26/library/minimagec.py:48-62)
+__device__ __forceinline__ float dev_mic(float x) { return x - dev_anint(x); }
+
+// IFIX (fmath.py:252-260) for arguments known to lie in [0, 2^31)
+__device__ __forceinline__ int dev_ifix_pos(float x) { return (int)truncf(x); }
+
+inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }

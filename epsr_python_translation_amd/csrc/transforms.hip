@@ -1,0 +1,335 @@
+// r <-> Q transforms: gensq (histograms -> partial S(Q)) and stog (S(Q) -> g(r)).
+//
+// Two modes each:
+//  mode 0  one lane per output element, summing in the reference's order in float32
+//          (gensq is then bit-identical to the reference; stog is bit-identical to
+//          the reference evaluated with a correctly rounded sinf);
+//  mode 1  the same contraction as an FP32 MFMA GEMM (v_mfma_f32_16x16x4_f32) with
+//          FP32 accumulation; the right-hand operand is assembled on the fly.
+#include "common.h"
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// ------------------------------------------------------------------ gensq, mode 0
+// src/epsr26/programs/epsr_standalone_rev3.py:4730-4802.  One thread per (Q point i,
+// column): columns 0..npairs-1 are the intramolecular sums, npairs..2npairs-1 the
+// intermolecular ones.
+__global__ void gensq_exact(int nq, int npairs, int npt1, int ldw, const float *__restrict__ wav,
+                            const float *__restrict__ ycoh, const float *__restrict__ foqols, int ldf,
+                            const float *__restrict__ natomsphere, float *__restrict__ sofq, int ldq)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int ncol = 2 * npairs;
+    if (t >= nq * ncol) return;
+    int i = t / ncol + 1;
+    int col = t - (i - 1) * ncol;
+    const float *fq = foqols + (size_t)i * ldf;
+    if (col < npairs) {
+        const float *w1 = wav + (size_t)col * ldw;
+        float sum1 = 0.0f;
+        for (int j = 2; j <= npt1; ++j) {
+            float a = w1[j];
+            if (fabsf(a) > 0.0f) sum1 = sum1 + a * fq[j];
+        }
+        sofq[(size_t)col * ldq + i] = sum1;
+    } else {
+        int ic = col - npairs;
+        const float *w2 = wav + ((size_t)npairs + ic) * ldw;
+        float nas = natomsphere ? natomsphere[ic] : 1.0f;
+        float sum2 = 0.0f, sum3 = 0.0f;
+        for (int j = 2; j <= npt1; ++j) {
+            float w = w2[j];
+            float add2 = (w - ycoh[j]) * nas;
+            sum2 = sum2 + add2 * fq[j];
+            sum3 = sum3 + w;
+        }
+        if (sum3 <= 0.0f) sum2 = 0.0f;
+        sofq[((size_t)npairs + ic) * ldq + i] = sum2;
+    }
+}
+
+// ------------------------------------------------------------------ MFMA GEMM core
+// One wave computes a 16 x 16 tile of C = A . B over k in [k0, k1), A row-major
+// (element (m,k) at A[m*lda + k]) and B given by a functor b(k, n).  Lane l holds
+// A(m0 + l%16, k + l/16) and B(k + l/16, n0 + l%16); the result element
+// (m0 + 4*(l/16) + v, n0 + l%16) is in acc[v].
+template <class BF>
+__device__ __forceinline__ floatx4 mfma_tile_16x16(const float *__restrict__ A, int lda, int M, int m0, int n0,
+                                                   int k0, int k1, BF b)
+{
+    const int lane = threadIdx.x & 63;
+    const int r = lane & 15, kq = lane >> 4;
+    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+    const int m = m0 + r;
+    const bool mrow = m < M;
+    for (int k = k0; k < k1; k += 4) {
+        int kk = k + kq;
+        float a = (mrow && kk < k1) ? A[(size_t)m * lda + kk] : 0.0f;
+        float bv = (kk < k1) ? b(kk, n0 + r) : 0.0f;
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+// gensq, mode 1.  B(j, col): intramolecular wav for col < npairs, (wav2 - ycoh) *
+// natomsphere for the intermolecular columns; rows j = 2..npt1.  grid.x: 16-row
+// tiles of Q points, grid.y: 16-column tiles; one wave per workgroup.
+__global__ void __launch_bounds__(64) gensq_mfma(int nq, int npairs, int npt1, int ldw, const float *__restrict__ wav,
+                                                 const float *__restrict__ ycoh, const float *__restrict__ foqols,
+                                                 int ldf, const float *__restrict__ natomsphere,
+                                                 const float *__restrict__ colsum, float *__restrict__ sofq, int ldq)
+{
+    const int ncol = 2 * npairs;
+    const int m0 = blockIdx.x * 16;   // Q rows, 0-based: Q index = m + 1
+    const int n0 = blockIdx.y * 16;
+    auto bfun = [&](int j, int col) -> float {
+        if (col >= ncol) return 0.0f;
+        if (col < npairs) return wav[(size_t)col * ldw + j];
+        int ic = col - npairs;
+        float nas = natomsphere ? natomsphere[ic] : 1.0f;
+        return (wav[((size_t)npairs + ic) * ldw + j] - ycoh[j]) * nas;
+    };
+    // A rows start at foqols + ldf (Q index 1); columns j are direct indices
+    floatx4 acc = mfma_tile_16x16(foqols + ldf, ldf, nq, m0, n0, 2, npt1 + 1, bfun);
+    const int lane = threadIdx.x & 63;
+    const int col = n0 + (lane & 15);
+    if (col >= ncol) return;
+    bool zero = col >= npairs && colsum[col - npairs] <= 0.0f;
+    for (int v = 0; v < 4; ++v) {
+        int m = m0 + 4 * (lane >> 4) + v;
+        if (m < nq) sofq[(size_t)col * ldq + m + 1] = zero ? 0.0f : acc[v];
+    }
+}
+
+// sum3 of gensq: sum over j of the intermolecular histogram (sequential, exact)
+__global__ void gensq_colsum(int npairs, int npt1, int ldw, const float *__restrict__ wav, float *__restrict__ out)
+{
+    int ic = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ic >= npairs) return;
+    const float *w2 = wav + ((size_t)npairs + ic) * ldw;
+    float s = 0.0f;
+    for (int j = 2; j <= npt1; ++j) s = s + w2[j];
+    out[ic] = s;
+}
+
+extern "C" int epsr_gensq(epsr_ctx *ctx, int nq, int npairs, int npt1, int ldw, const float *wav, const float *ycoh,
+                          const float *foqols, int ldf, const float *natomsphere, float *sofq, int ldq, int mode)
+{
+    if (!ctx) return EPSR_ERR_ARG;
+    ARG_CHECK(ctx, wav && ycoh && foqols && sofq, "epsr_gensq: null argument");
+    ARG_CHECK(ctx, nq >= 1 && npairs >= 1 && npt1 >= 2 && ldw > npt1 && ldf > npt1 && ldq > nq,
+              "epsr_gensq: bad sizes nq=%d npairs=%d npt1=%d ldw=%d ldf=%d ldq=%d", nq, npairs, npt1, ldw, ldf, ldq);
+    ARG_CHECK(ctx, mode == 0 || mode == 1, "epsr_gensq: mode must be 0 or 1");
+    HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    timer_begin(ctx, "gensq");
+    if (mode == 0) {
+        int n = nq * 2 * npairs;
+        gensq_exact<<<ceil_div(n, 64), 64, 0, st>>>(nq, npairs, npt1, ldw, wav, ycoh, foqols, ldf, natomsphere, sofq, ldq);
+    } else {
+        float *cs = (float *)ctx_scratch(ctx, sizeof(float) * npairs);
+        if (!cs) return EPSR_ERR_HIP;
+        gensq_colsum<<<ceil_div(npairs, 64), 64, 0, st>>>(npairs, npt1, ldw, wav, cs);
+        dim3 grid(ceil_div(nq, 16), ceil_div(2 * npairs, 16));
+        gensq_mfma<<<grid, 64, 0, st>>>(nq, npairs, npt1, ldw, wav, ycoh, foqols, ldf, natomsphere, cs, sofq, ldq);
+    }
+    timer_end(ctx, "gensq");
+    HIP_CHECK(ctx, hipGetLastError());
+    HIP_CHECK(ctx, hipStreamSynchronize(st));
+    return EPSR_OK;
+}
+
+// ------------------------------------------------------------------ stog
+// correctly rounded single-precision sine: double evaluation, one rounding
+__device__ __forceinline__ float cr_sinf(float x) { return (float)sin((double)x); }
+
+// ato_manipulation_rev4.py:4114-4124
+__device__ __forceinline__ float dev_lorch(float x, float pi)
+{
+    if (x == 0.0f) return 1.0f;
+    if (x < pi) return cr_sinf(x) / x;
+    return 0.0f;
+}
+
+// ato_manipulation_rev4.py:4066-4111, one thread per (r point j, column in)
+__global__ void stog_exact(float sfac, const float *__restrict__ q, const float *__restrict__ sofq, int ldq,
+                           const float *__restrict__ r, float *__restrict__ gofr, int ldg,
+                           const int *__restrict__ nqmin, const int *__restrict__ nqmax, int nr, int ncols,
+                           const float *__restrict__ xwt)
+{
+    const float pi = 3.14159265358979323846f;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nr * ncols) return;
+    int in = t / nr;
+    int j = t - in * nr + 1;
+    float rr = r[j];
+    float qmax = pi / q[nqmax[in]];
+    const float *sq = sofq + (size_t)in * ldq;
+    float sumi = 0.0f;
+    for (int i = nqmin[in]; i <= nqmax[in]; ++i) {
+        float qq = q[i];
+        float prod;
+        if (rr > 0.0f) prod = qq * cr_sinf(qq * rr) * dev_lorch(qmax * qq, pi);
+        else prod = qq * qq * dev_lorch(qmax * qq, pi);
+        sumi = sumi + sq[i] * prod;
+    }
+    gofr[(size_t)in * ldg + j] = rr > 0.0f ? xwt[in] + sfac * sumi / rr : xwt[in] + sfac * sumi;
+}
+
+// B(i, in) = sofq(i, in) * lorch(pi q_i / q_nqmax(in)) on [nqmin, nqmax], else 0
+__global__ void stog_prepare_b(const float *__restrict__ q, const float *__restrict__ sofq, int ldq,
+                               const int *__restrict__ nqmin, const int *__restrict__ nqmax, int nqpad, int ncols,
+                               float *__restrict__ B)
+{
+    const float pi = 3.14159265358979323846f;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nqpad * ncols) return;
+    int in = t / nqpad;
+    int i = t - in * nqpad;
+    float v = 0.0f;
+    if (i >= nqmin[in] && i <= nqmax[in]) {
+        float qmax = pi / q[nqmax[in]];
+        v = sofq[(size_t)in * ldq + i] * dev_lorch(qmax * q[i], pi);
+    }
+    B[(size_t)in * nqpad + i] = v;
+}
+
+// stog, mode 1: A(j, i) = q_i sin(q_i r_j) (q_i^2 at r = 0) generated in registers.
+__global__ void __launch_bounds__(64) stog_mfma(float sfac, const float *__restrict__ q, int nqpad,
+                                                const float *__restrict__ B, const float *__restrict__ r,
+                                                float *__restrict__ gofr, int ldg, int nr, int ncols,
+                                                const float *__restrict__ xwt)
+{
+    const int lane = threadIdx.x & 63;
+    const int m0 = blockIdx.x * 16, n0 = blockIdx.y * 16;
+    const int rl = lane & 15, kq = lane >> 4;
+    const int jrow = m0 + rl + 1;   // r index (1-based)
+    const float rr = jrow <= nr ? r[jrow] : 0.0f;
+    const int ncol = n0 + rl;
+    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int k = 0; k < nqpad; k += 4) {
+        int i = k + kq;
+        float qq = i < nqpad ? q[i] : 0.0f;
+        float a = 0.0f;
+        if (jrow <= nr && i < nqpad && i >= 1) a = rr > 0.0f ? qq * sinf(qq * rr) : qq * qq;
+        float bv = (ncol < ncols && i < nqpad) ? B[(size_t)ncol * nqpad + i] : 0.0f;
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv, acc, 0, 0, 0);
+    }
+    if (ncol >= ncols) return;
+    for (int v = 0; v < 4; ++v) {
+        int j = m0 + 4 * kq + v + 1;
+        if (j > nr) continue;
+        float rj = r[j];
+        gofr[(size_t)ncol * ldg + j] = rj > 0.0f ? xwt[ncol] + sfac * acc[v] / rj : xwt[ncol] + sfac * acc[v];
+    }
+}
+
+extern "C" int epsr_stog(epsr_ctx *ctx, float rho, const float *q, const float *sofq, int ldq, const float *r,
+                         float *gofr, int ldg, const int *nqmin, const int *nqmax, int nr, int ncols, const float *xwt,
+                         int mode)
+{
+    if (!ctx) return EPSR_ERR_ARG;
+    ARG_CHECK(ctx, q && sofq && r && gofr && nqmin && nqmax && xwt, "epsr_stog: null argument");
+    ARG_CHECK(ctx, nr >= 1 && ncols >= 1 && ldg > nr, "epsr_stog: bad sizes nr=%d ncols=%d ldg=%d", nr, ncols, ldg);
+    ARG_CHECK(ctx, mode == 0 || mode == 1, "epsr_stog: mode must be 0 or 1");
+    HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    // qstep = q(4) - q(3); sfac = 0.5*qstep/pi/pi/rho (:4076-4077)
+    float qh[5];
+    HIP_CHECK(ctx, hipMemcpyAsync(qh, q, sizeof(float) * 5, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(ctx, hipStreamSynchronize(st));
+    const float pi = 3.14159265358979323846f;
+    float qstep = qh[4] - qh[3];
+    float sfac = 0.5f * qstep / pi / pi / rho;
+    timer_begin(ctx, "stog");
+    if (mode == 0) {
+        int n = nr * ncols;
+        stog_exact<<<ceil_div(n, 64), 64, 0, st>>>(sfac, q, sofq, ldq, r, gofr, ldg, nqmin, nqmax, nr, ncols, xwt);
+    } else {
+        // the Q range used is 1..ldq-1 (nqmax <= ldq-1 is the caller's contract)
+        int nqpad = ldq;
+        float *B = (float *)ctx_scratch(ctx, sizeof(float) * (size_t)nqpad * ncols);
+        if (!B) return EPSR_ERR_HIP;
+        stog_prepare_b<<<ceil_div(nqpad * ncols, 256), 256, 0, st>>>(q, sofq, ldq, nqmin, nqmax, nqpad, ncols, B);
+        dim3 grid(ceil_div(nr, 16), ceil_div(ncols, 16));
+        stog_mfma<<<grid, 64, 0, st>>>(sfac, q, nqpad, B, r, gofr, ldg, nr, ncols, xwt);
+    }
+    timer_end(ctx, "stog");
+    HIP_CHECK(ctx, hipGetLastError());
+    HIP_CHECK(ctx, hipStreamSynchronize(st));
+    return EPSR_OK;
+}
+
+// ------------------------------------------------------------------ foqols
+// j1qr (src/epsr26/programs/epsr_standalone_rev3.py:3319-3329): real*8 result
+__device__ __forceinline__ double dev_j1qr(float q, float r)
+{
+    if (q > 0.0f) {
+        double qr = (double)(q * r);
+        return (sin(qr) - qr * cos(qr)) / (double)(q * q * q);
+    }
+    return (double)(r * r * r / 3.0f);
+}
+
+__device__ __forceinline__ float cr_expf(float x) { return (float)exp((double)x); }
+__device__ __forceinline__ float cr_cosf(float x) { return (float)cos((double)x); }
+
+// pofr (:3389-3399)
+__device__ __forceinline__ float dev_pofr(float delr)
+{
+    if (delr <= 0.0f) return 1.0f;
+    float delr3 = 3.0f / (delr * delr * delr);
+    return delr3 * (cr_sinf(delr) - delr * cr_cosf(delr));
+}
+
+// the transform matrix of inita (:3035-3071), one thread per (Q point i, r point j);
+// r2[j] are the bin edges of the serial r1/r2 recurrence (r2[0] = 0).
+__global__ void foqols_kernel(int nq, const float *__restrict__ q, int npt1, const float *__restrict__ x,
+                              const float *__restrict__ r2, const float *__restrict__ ycoh, float trfac, float sigres,
+                              float sigrsq, float lorchlim, float lw, float lwq, float *__restrict__ foqols, int ldf)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nq * npt1) return;
+    int i = t / npt1 + 1;
+    int j = t - (i - 1) * npt1 + 1;
+    float qq = q[i];
+    float rr = x[j];
+    float r1 = j == 1 ? 0.0f : r2[j - 1];
+    double j1 = dev_j1qr(qq, r1), j2 = dev_j1qr(qq, r2[j]);
+    float fo = ycoh[j] > 0.0f ? (float)((double)trfac * (j2 - j1) / (double)ycoh[j]) : 0.0f;
+    float qsig = sigres + qq * sigrsq;
+    float qrs = qsig * rr;
+    float qr2 = 0.5f * qrs * qrs;
+    float sums = 0.0f;
+    if (qr2 < 30.0f) {
+        sums = cr_expf(-qr2);
+        if (lw > 0.0f || lwq > 0.0f) {
+            float bwidth = rr * lorchlim * (lw + lwq * qq);
+            sums = sums * dev_pofr(bwidth);
+        }
+    }
+    foqols[(size_t)i * ldf + j] = fo * sums;
+}
+
+extern "C" int epsr_foqols(epsr_ctx *ctx, int nq, const float *q, int npt1, const float *x, const float *r2,
+                           const float *ycoh, float rho, float fwhm, float fwhmq, float lorchqwidth,
+                           float lorchqwidthq, float rmaxgr, float *foqols, int ldf)
+{
+    if (!ctx) return EPSR_ERR_ARG;
+    ARG_CHECK(ctx, q && x && r2 && ycoh && foqols, "epsr_foqols: null argument");
+    ARG_CHECK(ctx, nq >= 1 && npt1 >= 1 && ldf > npt1, "epsr_foqols: bad sizes");
+    HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    const float pi = 3.14159265358979323846f;
+    // scalar constants of inita (:3029-3039), float32 as the reference
+    float sqrt2ln2 = sqrtf(2.0f * (float)log(2.0));
+    float sigres = 0.5f * fwhm / sqrt2ln2;
+    float sigrsq = 0.5f * fwhmq / sqrt2ln2;
+    float trfac = 4.0f * pi * rho;
+    float lorchlim = 4.4934f / rmaxgr;
+    int n = nq * npt1;
+    foqols_kernel<<<ceil_div(n, 256), 256, 0, ctx->stream>>>(nq, q, npt1, x, r2, ycoh, trfac, sigres, sigrsq,
+                                                              lorchlim, lorchqwidth, lorchqwidthq, foqols, ldf);
+    HIP_CHECK(ctx, hipGetLastError());
+    HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return EPSR_OK;
+}

@@ -1,0 +1,149 @@
+"""gensq and stog on the GPU against the oracle.
+
+gensq mode 0 is bit-identical to the reference's sequential float32 sums; stog mode 0 is
+bit-identical to the reference evaluated with a correctly rounded sinf (oracle 'cr'
+flavour) and within 1e-5 of sum|terms| of the glibc flavour.  The MFMA modes (FP32
+inputs, FP32 accumulation in a different order) are held to 1e-5 relative to the sum of
+|terms| of each output, and their error against an FP64 evaluation is reported."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import pyoracle as po
+from tests import helpers as H
+
+F32 = np.float32
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from epsr_python_translation_amd import device as D
+    return D.Context(0)
+
+
+def _gensq_inputs(name):
+    s, g = H.system(name), H.grid(name)
+    r = H.oracle_atogen(name, ngrsamples=0, idum=-77)
+    fo, ldf = H.foqols(name)
+    return s, g, r["wav"], r["ldw"], fo, ldf
+
+
+def _oracle_gensq(s, g, wav, ldw, fo, ldf):
+    ldq = g.nq + 1
+    out = np.zeros(2 * s.npairs * ldq, F32)
+    po.lib().ora_gensq(g.nq, s.npairs, g.npt1, ldw, po.Pf(wav), po.Pf(g.ycoh), po.Pf(fo), ldf,
+                       po.Pi(s.n2extr), None, po.Pf(out), ldq)
+    return out, ldq
+
+
+def _abs_terms_gensq(s, g, wav, ldw, fo, ldf):
+    w = wav.reshape(2, s.npairs, ldw)[:, :, 2:g.npt1 + 1].astype(np.float64)
+    w[1] -= g.ycoh[None, 2:g.npt1 + 1]
+    F = fo.reshape(g.nq + 1, ldf)[1:, 2:g.npt1 + 1].astype(np.float64)
+    return np.abs(w) @ np.abs(F).T, w @ F.T       # [2, npairs, nq]
+
+
+@pytest.mark.parametrize("name", ["tiny_11_10_0", "c0", "c1"])
+def test_gensq_exact_mode_bit_identical(ctx, name):
+    from epsr_python_translation_amd import device as D
+    print(f"gensq mode0 {name}", flush=True)
+    s, g, wav, ldw, fo, ldf = _gensq_inputs(name)
+    ref, ldq = _oracle_gensq(s, g, wav, ldw, fo, ldf)
+    sofq = ctx.zeros(2 * s.npairs * ldq)
+    D.gensq(ctx, g.nq, s.npairs, g.npt1, ldw, ctx.tensor(wav), ctx.tensor(g.ycoh), ctx.tensor(fo), ldf, sofq, ldq,
+            mode=0)
+    got = sofq.cpu().numpy()
+    assert np.array_equal(got.view(np.int32), ref.view(np.int32))
+
+
+@pytest.mark.parametrize("name", ["c0", "c1"])
+def test_gensq_mfma_mode_tolerance(ctx, name):
+    from epsr_python_translation_amd import device as D
+    print(f"gensq mode1 {name}", flush=True)
+    s, g, wav, ldw, fo, ldf = _gensq_inputs(name)
+    ref, ldq = _oracle_gensq(s, g, wav, ldw, fo, ldf)
+    sofq = ctx.zeros(2 * s.npairs * ldq)
+    D.gensq(ctx, g.nq, s.npairs, g.npt1, ldw, ctx.tensor(wav), ctx.tensor(g.ycoh), ctx.tensor(fo), ldf, sofq, ldq,
+            mode=1)
+    got = sofq.cpu().numpy().reshape(2, s.npairs, ldq)[:, :, 1:]
+    ref = ref.reshape(2, s.npairs, ldq)[:, :, 1:]
+    absterms, f64 = _abs_terms_gensq(s, g, wav, ldw, fo, ldf)
+    scale = np.maximum(absterms, 1e-30)
+    err = np.abs(got - ref) / scale
+    err64 = np.abs(got - f64) / scale
+    print(f"gensq mfma {name}: max |gpu-oracle|/sum|t| = {err.max():.2e}, vs fp64 {err64.max():.2e}")
+    assert err.max() <= 1e-5
+
+
+def _stog_case(name):
+    s, g, wav, ldw, fo, ldf = _gensq_inputs(name)
+    sofq, ldq = _oracle_gensq(s, g, wav, ldw, fo, ldf)
+    ncols = s.npairs
+    sq = np.ascontiguousarray(sofq.reshape(2, s.npairs, ldq)[1].reshape(-1))   # inter partials
+    nqmin = np.full(ncols, 2, np.int32)
+    nqmax = np.full(ncols, g.nq, np.int32)
+    xwt = np.linspace(0.0, 0.1, ncols).astype(F32)
+    nr = g.npt1
+    r = g.x[:nr + 1].copy()
+    return s, g, sq, ldq, r, nr, ncols, nqmin, nqmax, xwt
+
+
+def _oracle_stog(flavour, s, g, sq, ldq, r, nr, ncols, nqmin, nqmax, xwt):
+    ldg = nr + 1
+    out = np.zeros(ncols * ldg, F32)
+    po.lib(flavour).ora_stog(F32(s.rho), po.Pf(g.q), po.Pf(sq), ldq, po.Pf(r), po.Pf(out), ldg, po.Pi(nqmin),
+                             po.Pi(nqmax), nr, ncols, po.Pf(xwt))
+    return out, ldg
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_stog(ctx, mode):
+    from epsr_python_translation_amd import device as D
+    print(f"stog mode{mode} c0", flush=True)
+    s, g, sq, ldq, r, nr, ncols, nqmin, nqmax, xwt = _stog_case("c0")
+    ref_cr, ldg = _oracle_stog("cr", s, g, sq, ldq, r, nr, ncols, nqmin, nqmax, xwt)
+    ref_gl, _ = _oracle_stog("glibc", s, g, sq, ldq, r, nr, ncols, nqmin, nqmax, xwt)
+    gofr = ctx.zeros(ncols * ldg)
+    D.stog(ctx, s.rho, ctx.tensor(g.q), ctx.tensor(sq), ldq, ctx.tensor(r), gofr, ldg, ctx.tensor(nqmin),
+           ctx.tensor(nqmax), nr, ncols, ctx.tensor(xwt), mode=mode)
+    got = gofr.cpu().numpy().reshape(ncols, ldg)[:, 1:]
+    ref_cr = ref_cr.reshape(ncols, ldg)[:, 1:]
+    ref_gl = ref_gl.reshape(ncols, ldg)[:, 1:]
+    # scale: xwt + sfac * sum|terms| / r, evaluated in float64
+    q = g.q.astype(np.float64)
+    rr = r[1:].astype(np.float64)
+    qs = q[None, 2:g.nq + 1] * np.abs(np.sin(q[None, 2:g.nq + 1] * rr[:, None]))
+    qs = np.where(rr[:, None] > 0, qs / np.maximum(rr[:, None], 1e-30), q[None, 2:g.nq + 1] ** 2)
+    sfac = 0.5 * (q[4] - q[3]) / np.pi / np.pi / float(s.rho)
+    scale = np.abs(xwt[:, None]) + sfac * (np.abs(sq.reshape(ncols, ldq)[:, 2:g.nq + 1]) @ qs.T)
+    err_cr = np.abs(got - ref_cr) / scale
+    err_gl = np.abs(got - ref_gl) / scale
+    print(f"stog mode{mode}: vs cr-oracle {err_cr.max():.2e}, vs glibc-oracle {err_gl.max():.2e}")
+    if mode == 0:
+        mism = np.count_nonzero(got.view(np.int32) != ref_cr.view(np.int32))
+        assert mism <= got.size * 1e-4, mism
+    assert err_gl.max() <= 1e-5
+    assert err_cr.max() <= 1e-5
+
+
+@pytest.mark.parametrize("name", ["c0", "c1"])
+def test_foqols_gpu_against_oracle(ctx, name):
+    """GPU foqols (double j1qr differences, correctly rounded expf) vs the oracle's glibc
+    build: element-wise within 2 ulp of the value (1 ulp from the double sin/cos of two
+    libms, 1 from expf)."""
+    from epsr_python_translation_amd import device as D
+    print(f"foqols {name}", flush=True)
+    s, g = H.system(name), H.grid(name)
+    for kw in ({}, {"fwhm": 0.02, "fwhmq": 0.01, "lorchqwidth": 0.1}):
+        ref, ldf = H.foqols(name, "glibc", **kw)
+        got, _ = D.foqols(ctx, g, s.rho, s.rmaxgr, **kw)
+        got = got.cpu().numpy()
+        a = ref.reshape(g.nq + 1, ldf)[1:, 1:]
+        b = got.reshape(g.nq + 1, ldf)[1:, 1:]
+        ulp = np.spacing(np.maximum(np.abs(a), np.float32(1e-30))).astype(np.float64)
+        err = np.abs(a.astype(np.float64) - b) / ulp
+        frac_exact = np.mean(a.view(np.int32) == b.view(np.int32))
+        print(f"foqols {name} {kw}: max {err.max():.1f} ulp, bit-identical fraction {frac_exact:.6f}")
+        assert err.max() <= 2.0
