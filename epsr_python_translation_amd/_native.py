@@ -79,6 +79,21 @@ class Coords(C.Structure):
     _fields_ = [("xa", vp), ("ya", vp), ("za", vp), ("xadd", vp), ("yadd", vp), ("zadd", vp)]
 
 
+class Tables(C.Structure):
+    _fields_ = [("ld", C.c_int), ("npt2", C.c_int), ("xstar", vp), ("xavs", vp), ("xavs1", vp), ("ecoul", vp),
+                ("ecoul1", vp)]
+
+
+class MCParams(C.Structure):
+    _fields_ = [("simt", C.c_float), ("stepr", C.c_float), ("stepm", C.c_float), ("rejrate", C.c_float),
+                ("ireset", C.c_int)]
+
+
+class MCStats(C.Structure):
+    _fields_ = [("moves", C.c_int), ("accepted", C.c_int), ("exact_resolutions", C.c_int), ("retries", C.c_int),
+                ("ms_pairs", C.c_double), ("ms_resolve", C.c_double)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/epsr_mi355x.h
 SIGNATURES = {
     "epsr_ctx_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
@@ -95,6 +110,15 @@ SIGNATURES = {
     "epsr_gensq": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp, C.c_int, vp, vp, C.c_int, C.c_int]),
     "epsr_stog": (C.c_int, [vp, C.c_float, vp, vp, C.c_int, vp, vp, C.c_int, vp, vp, C.c_int, C.c_int, vp,
                             C.c_int]),
+    "epsr_ledger_create": (C.c_int, [vp, vp, C.POINTER(vp)]),
+    "epsr_ledger_destroy": (None, [vp]),
+    "epsr_cells_init": (C.c_int, [vp, vp, vp, C.POINTER(Coords), C.c_int, C.c_float]),
+    "epsr_fmolec1": (C.c_int, [vp, vp, C.POINTER(Coords), C.POINTER(Tables), vp, vp, dp, dp, fp, fp, ip, ip]),
+    "epsr_ledger_export": (C.c_int, [vp, vp, C.c_int, ip, ip, ip, fp, ip]),
+    "epsr_en_change": (C.c_int, [vp, vp, C.POINTER(Coords), C.POINTER(Tables), vp, C.c_int, vp, fp]),
+    "epsr_en_mole_pair": (C.c_int, [vp, vp, C.POINTER(Coords), C.POINTER(Tables), C.c_int, vp, vp, vp]),
+    "epsr_molfit": (C.c_int, [vp, vp, C.POINTER(Coords), C.POINTER(Tables), vp, C.POINTER(MCParams), fp, fp, ip, ip,
+                              vp, i32p, fp, C.POINTER(MCStats)]),
     "epsr_foqols": (C.c_int, [vp, C.c_int, vp, C.c_int, vp, vp, vp, C.c_float, C.c_float, C.c_float, C.c_float,
                               C.c_float, C.c_float, vp, C.c_int]),
 }

@@ -31,27 +31,49 @@ __global__ void atogen_positions(const float *__restrict__ xa, const float *__re
     P[a] = make_float4(xa[a] + xadd[m], ya[a] + yadd[m], za[a] + zadd[m], __int_as_float(itype[a]));
 }
 
-// intramolecular pairs (:3698-3724): one thread per molecule
+// intramolecular pairs (:3698-3724): one thread per molecule.  Rigid molecules put
+// every copy of a bond in the same bin, so the counts go through a per-workgroup LDS
+// histogram (or straight to global memory when it does not fit).
 __global__ void atogen_intra(const float *__restrict__ xa, const float *__restrict__ ya,
                              const float *__restrict__ za, const int *__restrict__ naf,
                              const int *__restrict__ nal, const int *__restrict__ itype,
-                             const int *__restrict__ npairref, int ntype, int nmol, float sizefactor,
-                             float stepcl, int npt1, int ldw, uint32_t *__restrict__ counts)
+                             const int *__restrict__ npairref, int ntype, int npairs, int nmol, float sizefactor,
+                             float stepcl, int npt1, int ldw, int use_lds, uint32_t *__restrict__ counts)
 {
+    extern __shared__ uint32_t hist[];
+    const int ldh = npt1 + 1;
+    if (use_lds) {
+        for (int i = threadIdx.x; i < npairs * ldh; i += blockDim.x) hist[i] = 0u;
+        __syncthreads();
+    }
     int im = blockIdx.x * blockDim.x + threadIdx.x;
-    if (im >= nmol) return;
-    int a0 = naf[im], a1 = nal[im];
-    for (int ia = a0; ia < a1; ++ia) {
-        float x1 = xa[ia], y1 = ya[ia], z1 = za[ia];
-        int it = itype[ia];
-        for (int ja = ia + 1; ja <= a1; ++ja) {
-            int icref = npairref[it * ntype + itype[ja]];
-            float xx = dev_mic((x1 - xa[ja]) * sizefactor);
-            float yy = dev_mic((y1 - ya[ja]) * sizefactor);
-            float zz = dev_mic((z1 - za[ja]) * sizefactor);
-            float dtest = sqrtf(xx * xx + yy * yy + zz * zz);
-            int iz1 = dev_nint(dtest / stepcl) + 2;
-            if (iz1 > 1 && iz1 <= npt1) atomicAdd(&counts[(size_t)icref * ldw + iz1], 1u);
+    if (im < nmol) {
+        int a0 = naf[im], a1 = nal[im];
+        for (int ia = a0; ia < a1; ++ia) {
+            float x1 = xa[ia], y1 = ya[ia], z1 = za[ia];
+            int it = itype[ia];
+            for (int ja = ia + 1; ja <= a1; ++ja) {
+                int icref = npairref[it * ntype + itype[ja]];
+                float xx = dev_mic((x1 - xa[ja]) * sizefactor);
+                float yy = dev_mic((y1 - ya[ja]) * sizefactor);
+                float zz = dev_mic((z1 - za[ja]) * sizefactor);
+                float dtest = sqrtf(xx * xx + yy * yy + zz * zz);
+                int iz1 = dev_nint(dtest / stepcl) + 2;
+                if (iz1 > 1 && iz1 <= npt1) {
+                    if (use_lds) atomicAdd(&hist[icref * ldh + iz1], 1u);
+                    else atomicAdd(&counts[(size_t)icref * ldw + iz1], 1u);
+                }
+            }
+        }
+    }
+    if (use_lds) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < npairs * ldh; i += blockDim.x) {
+            uint32_t v = hist[i];
+            if (v) {
+                int ic = i / ldh, iz = i - ic * ldh;
+                atomicAdd(&counts[(size_t)ic * ldw + iz], v);
+            }
         }
     }
 }
@@ -297,9 +319,14 @@ extern "C" int epsr_atogen(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
 
     atogen_positions<<<ceil_div(natom, 256), 256, 0, st>>>(c->xa, c->ya, c->za, c->xadd, c->yadd, c->zadd,
                                                             box->d_amol, box->d_itype, dP, natom);
-    atogen_intra<<<ceil_div(nmol, 128), 128, 0, st>>>(c->xa, c->ya, c->za, box->d_naf, box->d_nal, box->d_itype,
-                                                       box->d_npairref, ntype, nmol, box->sizefactor, stepcl, npt1,
-                                                       ldw, dC);
+    {
+        size_t hbytes = sizeof(uint32_t) * (size_t)npairs * (npt1 + 1);
+        int use_lds = hbytes <= 96 * 1024;
+        const int blk = 256;
+        atogen_intra<<<ceil_div(nmol, blk), blk, use_lds ? hbytes : 0, st>>>(
+            c->xa, c->ya, c->za, box->d_naf, box->d_nal, box->d_itype, box->d_npairref, ntype, npairs, nmol,
+            box->sizefactor, stepcl, npt1, ldw, use_lds, dC);
+    }
     if (norigin > 0) {
         size_t lds = sizeof(uint32_t) * ((size_t)npairs * (npt1 + 1) + npairs) + sizeof(int) * ntype * ntype +
                      sizeof(float) * npairs + 16 + ATOGEN_TILE * (sizeof(float4) + sizeof(int));

@@ -1,0 +1,1305 @@
+// Cell lists, the pair-energy ledger (fmolec1), en_change, en_mole_pair and the MC
+// cycle (molfit) on the GPU.
+//
+// Reference: src/epsr26/library/cellneighbourlist.py:23-330 (cells),
+// src/epsr26/library/ergpairreference.py:63-213 (ledger),
+// src/epsr26/library/ato_manipulation_rev4.py:2374-2528 (fmolec1),
+// src/epsr26/library/molfit.py:51-541, 671-759, 805-919, 1250-1279, 1527-1578.
+//
+// The MC cycle visits molecules in a fixed chain order (cells, then molecules in a
+// cell) and every acceptance depends on the one before.  The GPU version splits it:
+//   A  trial moves of every molecule at once (their ran1 offsets are known up front:
+//      3 for the rotation, 3 for the translation, 1 for the Metropolis test);
+//   B  for every ledger slot (molecule m, partner n) the pair energies the chain can
+//      need: E(m_trial, n_current) and, when n moves earlier in the chain, also
+//      E(m_trial, n_trial) -- massively parallel, one lane per slot;
+//   C  a single workgroup walks the chain: for move k it selects, slot by slot, the
+//      variants implied by the earlier decisions, sums them in the reference's
+//      order (sequential float32) and applies the Metropolis test;
+//   D  the accepted trial positions and the final ledger values are written back.
+// Because every pair energy and every sum is the reference's float32 expression, the
+// result is the reference's (with a correctly rounded cosf/expf).
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "common.h"
+
+struct epsr_ledger {
+    epsr_ctx *ctx = nullptr;
+    const epsr_box *box = nullptr;
+    int nmol = 0;
+    // cells (host)
+    int na = 0, nb = 0, nc = 0, ncell = 0, ncellneighb = 0, maxcellno = 0, nnbrefmax = 0;
+    std::vector<int> nmolsincell, molsincell, molcellno, iaoff, iboff, icoff;
+    std::vector<int> order, pos;
+    bool cells_ready = false;
+    // ledger (device)
+    int nself = 0, npairent = 0, nslots = 0, nentries = 0;
+    int *d_rowstart = nullptr, *d_cnt = nullptr, *d_order = nullptr, *d_pos = nullptr;
+    int *d_partner = nullptr, *d_entry = nullptr, *d_mirror = nullptr, *d_slotmol = nullptr;
+    float *d_ergpair = nullptr;
+    size_t cap_slots = 0, cap_entries = 0;
+    bool ledger_ready = false;
+    std::vector<int> h_rowstart, h_cnt;
+};
+
+template <class T>
+static int dev_alloc(epsr_ctx *ctx, T **p, size_t n)
+{
+    if (*p) hipFree(*p);
+    *p = nullptr;
+    HIP_CHECK(ctx, hipMalloc((void **)p, sizeof(T) * (n > 0 ? n : 1)));
+    return EPSR_OK;
+}
+
+extern "C" int epsr_ledger_create(epsr_ctx *ctx, const epsr_box *box, epsr_ledger **out)
+{
+    if (!ctx || !box || !out) return set_error(ctx, EPSR_ERR_ARG, "epsr_ledger_create: null argument");
+    epsr_ledger *l = new epsr_ledger();
+    l->ctx = ctx;
+    l->box = box;
+    l->nmol = box->nmol;
+    int rc;
+    if ((rc = dev_alloc(ctx, &l->d_order, box->nmol)) || (rc = dev_alloc(ctx, &l->d_pos, box->nmol)) ||
+        (rc = dev_alloc(ctx, &l->d_rowstart, box->nmol + 1)) || (rc = dev_alloc(ctx, &l->d_cnt, box->nmol))) {
+        epsr_ledger_destroy(l);
+        return rc;
+    }
+    *out = l;
+    return EPSR_OK;
+}
+
+extern "C" void epsr_ledger_destroy(epsr_ledger *l)
+{
+    if (!l) return;
+    hipFree(l->d_rowstart); hipFree(l->d_cnt); hipFree(l->d_order); hipFree(l->d_pos);
+    hipFree(l->d_partner); hipFree(l->d_entry); hipFree(l->d_mirror); hipFree(l->d_slotmol);
+    hipFree(l->d_ergpair);
+    delete l;
+}
+
+// ------------------------------------------------------------------ host arithmetic
+static int h_nint(float xf)
+{
+    double x = (double)xf;
+    if (!(x >= -9.2233720368547758e18 && x < 9.2233720368547758e18)) return 0;
+    double t = std::trunc(x), f = x - t;
+    int64_t ti = (int64_t)t;
+    if (f >= 0.5) ti += 1;
+    else if (f <= -0.5) ti -= 1;
+    return (int)(int32_t)(uint32_t)(uint64_t)ti;
+}
+
+static int h_ifix(float xf)
+{
+    double x = (double)xf;
+    if (!(x > -2147483649.0 && x < 2147483648.0)) return (int)0x80000000u;
+    return (int)std::trunc(x);
+}
+
+static int fold(int ix, int n)
+{
+    while (ix < 0) ix += n;
+    while (ix >= n) ix -= n;
+    return ix;
+}
+
+// cellneighbourlist.py:309-330, cubic lattice in box fractions
+static int h_cellcoord(float x, int n)
+{
+    float x1 = x - (float)h_nint(x);
+    int ix = h_ifix((x1 + 0.5f) * (float)n);
+    return fold(ix, n);
+}
+
+extern "C" int epsr_cells_init(epsr_ctx *ctx, const epsr_box *box, epsr_ledger *l, const epsr_coords_t *c,
+                               int nmolcellin, float rmax)
+{
+    if (!ctx) return EPSR_ERR_ARG;
+    ARG_CHECK(ctx, box && l && c && nmolcellin > 0, "epsr_cells_init: bad argument");
+    HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    const int nmol = box->nmol;
+    std::vector<float> xadd(nmol), yadd(nmol), zadd(nmol);
+    hipStream_t st = ctx->stream;
+    HIP_CHECK(ctx, hipMemcpyAsync(xadd.data(), c->xadd, sizeof(float) * nmol, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(ctx, hipMemcpyAsync(yadd.data(), c->yadd, sizeof(float) * nmol, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(ctx, hipMemcpyAsync(zadd.data(), c->zadd, sizeof(float) * nmol, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(ctx, hipStreamSynchronize(st));
+    // initialise_cellneighbourlist (:23-96)
+    int nmolcell = std::min(nmolcellin, nmol);
+    int ncelldiv = h_nint(powf((float)nmol / (float)nmolcell, 1.0f / 3.0f));
+    if (ncelldiv < 1) ncelldiv = 1;
+    float cellsize = 2.0f * 1.0f / (float)ncelldiv;
+    l->na = l->nb = l->nc = 2 * h_ifix(1.0f / cellsize) + 1;
+    l->ncell = l->na * l->nb * l->nc;
+    l->nmolsincell.assign(l->ncell, 0);
+    l->molcellno.assign(nmol, 0);
+    l->maxcellno = 0;
+    for (int im = 0; im < nmol; ++im) {
+        // atomcellno (:317-330): fc = z/clatz, fb = (y - fc*claty)/blaty, fa = ...
+        float fc = zadd[im] / 1.0f;
+        float fb = (yadd[im] - fc * 0.0f) / 1.0f;
+        float fa = (xadd[im] - fb * 0.0f - fc * 0.0f) / 1.0f;
+        int ia = h_cellcoord(fa, l->na), ib = h_cellcoord(fb, l->nb), ic = h_cellcoord(fc, l->nc);
+        int cc = ia + l->na * (ib + l->nb * ic);
+        l->molcellno[im] = cc;
+        l->nmolsincell[cc] += 1;
+        l->maxcellno = std::max(l->maxcellno, l->nmolsincell[cc]);
+    }
+    l->molsincell.assign((size_t)l->ncell * std::max(1, l->maxcellno), 0);
+    std::fill(l->nmolsincell.begin(), l->nmolsincell.end(), 0);
+    for (int im = 0; im < nmol; ++im) {
+        int cc = l->molcellno[im];
+        l->molsincell[(size_t)cc * l->maxcellno + l->nmolsincell[cc]++] = im;
+    }
+    // makecellneighbourlist (:102-194)
+    {
+        float cella = 1.0f / (float)l->na, cellb = 1.0f / (float)l->nb, cellc = 1.0f / (float)l->nc;
+        float cella2 = 0.5f * cella, cellb2 = 0.5f * cellb, cellc2 = 0.5f * cellc;
+        float xc[8], yc[8], zc[8];
+        int nf = 0;
+        for (int ic = -1; ic <= 1; ic += 2)
+            for (int ib = -1; ib <= 1; ib += 2)
+                for (int ia = -1; ia <= 1; ia += 2) {
+                    float cx = (float)ic * cellc2 * 0.0f, cy = (float)ic * cellc2 * 0.0f, cz = (float)ic * cellc2 * 1.0f;
+                    float bx = (float)ib * cellb2 * 0.0f, by = (float)ib * cellb2 * 1.0f, bz = (float)ib * cellb2 * 0.0f;
+                    float ax = (float)ia * cella2 * 1.0f, ay = (float)ia * cella2 * 0.0f, az = (float)ia * cella2 * 0.0f;
+                    xc[nf] = ax + bx + cx; yc[nf] = ay + by + cy; zc[nf] = az + bz + cz;
+                    nf++;
+                }
+        int nra = (l->na - 1) / 2, nrb = (l->nb - 1) / 2, nrc = (l->nc - 1) / 2;
+        float rmaxsq = rmax * rmax;
+        l->iaoff.clear(); l->iboff.clear(); l->icoff.clear();
+        for (int ic = -nrc; ic <= nrc; ++ic) {
+            float cx = (float)ic * cellc * 0.0f, cy = (float)ic * cellc * 0.0f, cz = (float)ic * cellc * 1.0f;
+            for (int ib = -nrb; ib <= nrb; ++ib) {
+                float bx = (float)ib * cellb * 0.0f, by = (float)ib * cellb * 1.0f, bz = (float)ib * cellb * 0.0f;
+                for (int ia = -nra; ia <= nra; ++ia) {
+                    float ax = (float)ia * cella * 1.0f, ay = (float)ia * cella * 0.0f, az = (float)ia * cella * 0.0f;
+                    float xdc = ax + bx + cx, ydc = ay + by + cy, zdc = az + bz + cz;
+                    bool found = false;
+                    for (int i1 = 0; i1 < 8 && !found; ++i1)
+                        for (int i2 = 0; i2 < 8 && !found; ++i2) {
+                            float xd = xdc + xc[i1] - xc[i2], yd = ydc + yc[i1] - yc[i2], zd = zdc + zc[i1] - zc[i2];
+                            float rsq = xd * xd + yd * yd + zd * zd;
+                            found = (rsq < rmaxsq) || (ia == 0 && ib == 0 && ic == 0);
+                        }
+                    if (found) {
+                        l->iaoff.push_back(ia); l->iboff.push_back(ib); l->icoff.push_back(ic);
+                    }
+                }
+            }
+        }
+        l->ncellneighb = (int)l->iaoff.size();
+    }
+    // chain order of molfit (cells in order, molecules in cell order) and nnbrefmax
+    l->order.clear();
+    l->order.reserve(nmol);
+    for (int cc = 0; cc < l->ncell; ++cc)
+        for (int i = 0; i < l->nmolsincell[cc]; ++i) l->order.push_back(l->molsincell[(size_t)cc * l->maxcellno + i]);
+    l->pos.assign(nmol, 0);
+    for (int k = 0; k < nmol; ++k) l->pos[l->order[k]] = k;
+    l->nnbrefmax = 0;
+    for (int cc = 0; cc < l->ncell; ++cc) {
+        int icc = cc / (l->na * l->nb), ib = (cc - icc * l->na * l->nb) / l->na, ia = cc - l->na * (ib + l->nb * icc);
+        int n = 0;
+        for (int k = 0; k < l->ncellneighb; ++k) {
+            int a = fold(ia + l->iaoff[k], l->na), b = fold(ib + l->iboff[k], l->nb), q = fold(icc + l->icoff[k], l->nc);
+            n += l->nmolsincell[a + l->na * (b + l->nb * q)];
+        }
+        l->nnbrefmax = std::max(l->nnbrefmax, n);
+    }
+    HIP_CHECK(ctx, hipMemcpy(l->d_order, l->order.data(), sizeof(int) * nmol, hipMemcpyHostToDevice));
+    HIP_CHECK(ctx, hipMemcpy(l->d_pos, l->pos.data(), sizeof(int) * nmol, hipMemcpyHostToDevice));
+    l->cells_ready = true;
+    l->ledger_ready = false;
+    return EPSR_OK;
+}
+
+// the molecules of the neighbour cells of cell cc, in the order of
+// make_neighbour_list_from_cells (cellneighbourlist.py:199-219)
+static void cell_nbref(const epsr_ledger *l, int cc, std::vector<int> &out)
+{
+    out.clear();
+    int icc = cc / (l->na * l->nb), ib = (cc - icc * l->na * l->nb) / l->na, ia = cc - l->na * (ib + l->nb * icc);
+    for (int k = 0; k < l->ncellneighb; ++k) {
+        int a = fold(ia + l->iaoff[k], l->na), b = fold(ib + l->iboff[k], l->nb), q = fold(icc + l->icoff[k], l->nc);
+        int cref = a + l->na * (b + l->nb * q);
+        for (int i = 0; i < l->nmolsincell[cref]; ++i) out.push_back(l->molsincell[(size_t)cref * l->maxcellno + i]);
+    }
+}
+
+// ------------------------------------------------------------------ pair energy
+// Packed tables: T[ic*nte + iz] = (xavs, xavs1, ecoul, ecoul1)(iz, ic) for iz < nte;
+// beyond nte every table entry is +0, where the reference's lookup also gives +0.
+struct PairTab {
+    const float4 *T;
+    const float *xs;
+    const int *npr;
+    int nte, npt2, ntype;
+    float stepcl;
+};
+
+// en_atom_inter_scale (molfit.py:1250-1279) with scalefac = 1
+__device__ __forceinline__ float atom_pair(float x1, float y1, float z1, float x2, float y2, float z2, int ic,
+                                           const PairTab &t)
+{
+    float xd = dev_mic(x2 - x1), yd = dev_mic(y2 - y1), zd = dev_mic(z2 - z1);
+    float dsq = xd * xd + yd * yd + zd * zd;
+    float dtest = sqrtf(dsq);
+    float rat = dtest / t.stepcl;
+    float sume = 0.0f;
+    if (rat < 2.0e9f) {
+        int iz = (int)truncf(rat) + 2;
+        if (iz <= t.npt2 && iz < t.nte) {
+            float4 v = t.T[ic * t.nte + iz];
+            float dtdif = dtest - t.xs[iz];
+            sume = v.x + v.y * dtdif;
+            sume = sume + v.z + v.w * dtdif;
+        }
+    }
+    return sume;
+}
+
+// en_mole_pair (molfit.py:1527-1578): sequential sum, molecule 1 outer, molecule 2 inner
+__device__ __forceinline__ float mol_pair(const float4 *__restrict__ P1, int na1, const float4 *__restrict__ P2,
+                                          int nb2, const PairTab &t)
+{
+    float sume = 0.0f;
+    for (int ia = 0; ia < na1; ++ia) {
+        float4 p = P1[ia];
+        int it = __float_as_int(p.w) * t.ntype;
+        for (int ja = 0; ja < nb2; ++ja) {
+            float4 q = P2[ja];
+            sume = sume + atom_pair(p.x, p.y, p.z, q.x, q.y, q.z, t.npr[it + __float_as_int(q.w)], t);
+        }
+    }
+    return sume;
+}
+
+__global__ void pack_tables(const float *__restrict__ xavs, const float *__restrict__ xavs1,
+                            const float *__restrict__ ecoul, const float *__restrict__ ecoul1, int ld, int npairs,
+                            int nte, float4 *__restrict__ T)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npairs * nte) return;
+    int ic = t / nte, iz = t - ic * nte;
+    size_t o = (size_t)ic * ld + iz;
+    T[t] = make_float4(xavs[o], xavs1[o], ecoul[o], ecoul1[o]);
+}
+
+__global__ void last_nonzero(const float *__restrict__ a, const float *__restrict__ b, const float *__restrict__ c,
+                             const float *__restrict__ d, int ld, int npairs, int lim, int *out)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npairs * ld) return;
+    int iz = t % ld;
+    if (iz > lim) return;
+    if (__float_as_int(a[t]) | __float_as_int(b[t]) | __float_as_int(c[t]) | __float_as_int(d[t])) atomicMax(out, iz);
+}
+
+// absolute atom positions (x = xa + xadd, one fp32 add) with the type in .w
+__global__ void abs_positions(const float *__restrict__ xa, const float *__restrict__ ya, const float *__restrict__ za,
+                              const float *__restrict__ xadd, const float *__restrict__ yadd,
+                              const float *__restrict__ zadd, const int *__restrict__ amol,
+                              const int *__restrict__ itype, float4 *__restrict__ P, int natom)
+{
+    int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= natom) return;
+    int m = amol[a];
+    P[a] = make_float4(xa[a] + xadd[m], ya[a] + yadd[m], za[a] + zadd[m], __int_as_float(itype[a]));
+}
+
+// Tables prepared for a call: packed float4 rows up to the last non-zero bin.
+struct PreparedTables {
+    PairTab tab;
+    size_t lds_bytes;
+    bool in_lds;
+};
+
+static int prepare_tables(epsr_ctx *ctx, const epsr_box *box, const epsr_tables_t *tb, float4 *dT, int *dscratch_int,
+                          PreparedTables &pt)
+{
+    hipStream_t st = ctx->stream;
+    const int np = box->npairs, ld = tb->ld;
+    int lim = std::min(tb->npt2, ld - 1);
+    HIP_CHECK(ctx, hipMemsetAsync(dscratch_int, 0, sizeof(int), st));
+    last_nonzero<<<ceil_div((long)np * ld, 256), 256, 0, st>>>(tb->xavs, tb->xavs1, tb->ecoul, tb->ecoul1, ld, np, lim,
+                                                               dscratch_int);
+    int last = 0;
+    HIP_CHECK(ctx, hipMemcpyAsync(&last, dscratch_int, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(ctx, hipStreamSynchronize(st));
+    int nte = std::max(2, last + 1);
+    pack_tables<<<ceil_div((long)np * nte, 256), 256, 0, st>>>(tb->xavs, tb->xavs1, tb->ecoul, tb->ecoul1, ld, np, nte,
+                                                               dT);
+    pt.tab.T = dT;
+    pt.tab.xs = tb->xstar;
+    pt.tab.npr = box->d_npairref;
+    pt.tab.nte = nte;
+    pt.tab.npt2 = tb->npt2;
+    pt.tab.ntype = box->ntype;
+    pt.tab.stepcl = box->cellst / box->dmax;
+    pt.lds_bytes = sizeof(float4) * (size_t)np * nte + sizeof(float) * nte + sizeof(int) * box->ntype * box->ntype + 64;
+    pt.in_lds = pt.lds_bytes <= 150 * 1024;
+    return EPSR_OK;
+}
+
+// copy the tables into LDS; returns a PairTab pointing at LDS
+__device__ PairTab tables_to_lds(const PairTab &g, int npairs, char *smem)
+{
+    float4 *T = (float4 *)smem;
+    int nT = npairs * g.nte;
+    float *xs = (float *)(T + nT);
+    int *npr = (int *)(xs + g.nte);
+    for (int i = threadIdx.x; i < nT; i += blockDim.x) T[i] = g.T[i];
+    for (int i = threadIdx.x; i < g.nte; i += blockDim.x) xs[i] = g.xs[i];
+    for (int i = threadIdx.x; i < g.ntype * g.ntype; i += blockDim.x) npr[i] = g.npr[i];
+    __syncthreads();
+    PairTab t = g;
+    t.T = T;
+    t.xs = xs;
+    t.npr = npr;
+    return t;
+}
+
+// ------------------------------------------------------------------ fmolec1 kernels
+// F1: for every chain position k (molecule im) and every position inn >= 1 of its cell's
+// neighbour list with jm > im: E = en_mole_pair(im, jm) (ato_manipulation_rev4.py:2454-2462).
+// Items are (k, inn) flattened with rowoff (int64 prefix over k); persistent blocks.
+template <bool LDS>
+__global__ void fmolec1_pairs(const long long *__restrict__ rowoff, int nmol, const int *__restrict__ order,
+                              const int *__restrict__ mcell, const int *__restrict__ cstart,
+                              const int *__restrict__ nbflat, const float4 *__restrict__ P,
+                              const int *__restrict__ naf, const int *__restrict__ nal, PairTab gt, int npairs,
+                              float *__restrict__ E, int *__restrict__ flag)
+{
+    extern __shared__ char smem[];
+    PairTab t = LDS ? tables_to_lds(gt, npairs, smem) : gt;
+    const long long total = rowoff[nmol];
+    for (long long item = (long long)blockIdx.x * blockDim.x + threadIdx.x; item < total;
+         item += (long long)gridDim.x * blockDim.x) {
+        // binary search of the chain position
+        int lo = 0, hi = nmol - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (rowoff[mid] <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        int k = lo;
+        int inn = (int)(item - rowoff[k]);
+        int im = order[k];
+        int jm = nbflat[cstart[mcell[im]] + inn];
+        float e = 0.0f;
+        int f = 0;
+        if (inn >= 1 && jm > im) {
+            e = mol_pair(P + naf[im], nal[im] - naf[im] + 1, P + naf[jm], nal[jm] - naf[jm] + 1, t);
+            f = e != 0.0f;
+        }
+        E[item] = e;
+        flag[item] = f;
+    }
+}
+
+__global__ void fmolec1_compact(const long long *__restrict__ rowoff, int nmol, const int *__restrict__ order,
+                                const int *__restrict__ mcell, const int *__restrict__ cstart,
+                                const int *__restrict__ nbflat, const float *__restrict__ E,
+                                const int *__restrict__ flag, const int *__restrict__ idx, int *__restrict__ pim,
+                                int *__restrict__ pjm, float *__restrict__ pE)
+{
+    const long long total = rowoff[nmol];
+    for (long long item = (long long)blockIdx.x * blockDim.x + threadIdx.x; item < total;
+         item += (long long)gridDim.x * blockDim.x) {
+        if (!flag[item]) continue;
+        int lo = 0, hi = nmol - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (rowoff[mid] <= item) lo = mid;
+            else hi = mid - 1;
+        }
+        int im = order[lo];
+        int inn = (int)(item - rowoff[lo]);
+        int p = idx[item];
+        pim[p] = im;
+        pjm[p] = nbflat[cstart[mcell[im]] + inn];
+        pE[p] = E[item];
+    }
+}
+
+// records for the per-molecule lists: self record r < nself (key = molecule), pair p
+// gives records nself + 2p (key = first molecule) and nself + 2p + 1 (key = second)
+__global__ void fmolec1_records(int nmol, const float *__restrict__ eintra, const int *__restrict__ sidx, int nself,
+                                int npair, const int *__restrict__ pim, const int *__restrict__ pjm,
+                                int *__restrict__ key, int *__restrict__ val)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nmol && eintra[t] != 0.0f) {
+        int r = sidx[t];
+        key[r] = t;
+        val[r] = r;
+    }
+    if (t < npair) {
+        int r = nself + 2 * t;
+        key[r] = pim[t];
+        val[r] = r;
+        key[r + 1] = pjm[t];
+        val[r + 1] = r + 1;
+    }
+}
+
+__global__ void count_keys(const int *__restrict__ key, int n, int *__restrict__ cnt)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) atomicAdd(&cnt[key[t]], 1);
+}
+
+__global__ void fmolec1_rows(const int *__restrict__ skey, const int *__restrict__ sval, int nrec, int nself,
+                             const int *__restrict__ pim, const int *__restrict__ pjm, const float *__restrict__ pE,
+                             const float *__restrict__ eintra, const int *__restrict__ rowstart,
+                             int *__restrict__ partner, int *__restrict__ entry, int *__restrict__ slotmol,
+                             int *__restrict__ sortedpos, float *__restrict__ ergpair)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrec) return;
+    int m = skey[i], r = sval[i];
+    slotmol[i] = m;
+    sortedpos[r] = i;
+    if (r < nself) {
+        partner[i] = m;
+        entry[i] = r;
+        ergpair[r] = eintra[m];
+    } else {
+        int p = (r - nself) >> 1, side = (r - nself) & 1;
+        partner[i] = side ? pim[p] : pjm[p];
+        entry[i] = nself + p;
+        if (!side) ergpair[nself + p] = pE[p];
+    }
+}
+
+__global__ void fmolec1_mirror(const int *__restrict__ sval, int nrec, int nself, const int *__restrict__ sortedpos,
+                               int *__restrict__ mirror)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrec) return;
+    int r = sval[i];
+    mirror[i] = r < nself ? -1 : sortedpos[nself + (((r - nself) >> 1) << 1) + (1 - ((r - nself) & 1))];
+}
+
+__global__ void flag_nonzero(const float *__restrict__ a, int n, int *__restrict__ f)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) f[t] = a[t] != 0.0f;
+}
+
+__global__ void sum_f64(const float *__restrict__ a, int n, double *out)
+{
+    __shared__ double s[256];
+    double v = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v += (double)a[i];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(out, s[0]);
+}
+
+// grid size for persistent loops
+static int persistent_blocks(size_t lds_bytes, long long items, int block)
+{
+    int per_cu = lds_bytes > 80 * 1024 ? 1 : (lds_bytes > 50 * 1024 ? 2 : 4);
+    long long want = (items + block - 1) / block;
+    return (int)std::max(1LL, std::min<long long>(want, 256LL * per_cu));
+}
+
+struct Scratch {
+    char *base = nullptr;
+    size_t off = 0;
+    template <class T>
+    T *take(size_t n)
+    {
+        size_t o = off;
+        off += (sizeof(T) * (n > 0 ? n : 1) + 255) & ~(size_t)255;
+        return (T *)(base + o);
+    }
+};
+
+static int hipcub_scan_int(epsr_ctx *ctx, const int *in, int *out, int n)
+{
+    size_t tb = 0;
+    HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, n, ctx->stream));
+    void *tmp = nullptr;
+    HIP_CHECK(ctx, hipMallocAsync(&tmp, tb + 16, ctx->stream));
+    HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, tb, in, out, n, ctx->stream));
+    HIP_CHECK(ctx, hipFreeAsync(tmp, ctx->stream));
+    return EPSR_OK;
+}
+
+extern "C" int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *c, const epsr_tables_t *tb,
+                            epsr_ledger *l, const float *eintra, double *ftest64, double *ftestb64, float *ftest,
+                            float *ftestb, int *nnbrefmax_out, int *nergpair_out)
+{
+    if (!ctx) return EPSR_ERR_ARG;
+    ARG_CHECK(ctx, box && c && tb && l && eintra, "epsr_fmolec1: null argument");
+    ARG_CHECK(ctx, l->cells_ready, "epsr_fmolec1: call epsr_cells_init first");
+    HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int nmol = box->nmol;
+    // host: the neighbour list of every cell, flattened, and the row offsets of the
+    // (chain position, neighbour position) items
+    std::vector<int> cstart(l->ncell + 1, 0), nbflat, tmp;
+    for (int cc = 0; cc < l->ncell; ++cc) {
+        cstart[cc] = (int)nbflat.size();
+        if (l->nmolsincell[cc] > 0) {
+            cell_nbref(l, cc, tmp);
+            nbflat.insert(nbflat.end(), tmp.begin(), tmp.end());
+        }
+    }
+    cstart[l->ncell] = (int)nbflat.size();
+    std::vector<long long> rowoff(nmol + 1, 0);
+    for (int k = 0; k < nmol; ++k) {
+        int cc = l->molcellno[l->order[k]];
+        rowoff[k + 1] = rowoff[k] + (cstart[cc + 1] - cstart[cc]);
+    }
+    const long long total = rowoff[nmol];
+    ARG_CHECK(ctx, total < (1LL << 31), "epsr_fmolec1: %lld candidate pairs exceed the 2^31 item limit", total);
+
+    Scratch S;
+    size_t need = 0;
+    {
+        Scratch q;
+        q.base = nullptr;
+        q.take<float4>(box->natom); q.take<float4>((size_t)box->npairs * std::max(2, std::min(tb->ld, tb->npt2 + 1)));
+        q.take<int>(4); q.take<int>(l->ncell + 1); q.take<int>(nbflat.size()); q.take<long long>(nmol + 1);
+        q.take<int>(nmol); q.take<float>(total); q.take<int>(total); q.take<int>(total + 1);
+        q.take<int>(nmol); q.take<int>(nmol + 1);
+        need = q.off;
+    }
+    S.base = (char *)ctx_scratch(ctx, need);
+    if (!S.base) return EPSR_ERR_HIP;
+    float4 *dP = S.take<float4>(box->natom);
+    float4 *dT = S.take<float4>((size_t)box->npairs * std::max(2, std::min(tb->ld, tb->npt2 + 1)));
+    int *dtmp = S.take<int>(4);
+    int *dcstart = S.take<int>(l->ncell + 1);
+    int *dnb = S.take<int>(nbflat.size());
+    long long *drow = S.take<long long>(nmol + 1);
+    int *dmcell = S.take<int>(nmol);
+    float *dE = S.take<float>(total);
+    int *dflag = S.take<int>(total);
+    int *didx = S.take<int>(total + 1);
+    int *dsflag = S.take<int>(nmol);
+    int *dsidx = S.take<int>(nmol + 1);
+
+    HIP_CHECK(ctx, hipMemcpyAsync(dcstart, cstart.data(), sizeof(int) * (l->ncell + 1), hipMemcpyHostToDevice, st));
+    HIP_CHECK(ctx, hipMemcpyAsync(dnb, nbflat.data(), sizeof(int) * nbflat.size(), hipMemcpyHostToDevice, st));
+    HIP_CHECK(ctx, hipMemcpyAsync(drow, rowoff.data(), sizeof(long long) * (nmol + 1), hipMemcpyHostToDevice, st));
+    HIP_CHECK(ctx, hipMemcpyAsync(dmcell, l->molcellno.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, st));
+
+    PreparedTables pt;
+    int rc = prepare_tables(ctx, box, tb, dT, dtmp, pt);
+    if (rc) return rc;
+    abs_positions<<<ceil_div(box->natom, 256), 256, 0, st>>>(c->xa, c->ya, c->za, c->xadd, c->yadd, c->zadd,
+                                                             box->d_amol, box->d_itype, dP, box->natom);
+    {
+        const int B = 256;
+        timer_begin(ctx, "fmolec1_pair");
+        if (pt.in_lds) {
+            int g = persistent_blocks(pt.lds_bytes, total, B);
+            fmolec1_pairs<true><<<g, B, pt.lds_bytes, st>>>(drow, nmol, l->d_order, dmcell, dcstart, dnb, dP,
+                                                            box->d_naf, box->d_nal, pt.tab, box->npairs, dE, dflag);
+        } else {
+            int g = persistent_blocks(0, total, B);
+            fmolec1_pairs<false><<<g, B, 0, st>>>(drow, nmol, l->d_order, dmcell, dcstart, dnb, dP, box->d_naf,
+                                                  box->d_nal, pt.tab, box->npairs, dE, dflag);
+        }
+        timer_end(ctx, "fmolec1_pair");
+    }
+    // global entry numbers: self entries first (molecule order, eintra != 0), then the
+    // stored pairs in (chain position, neighbour position) order
+    HIP_CHECK(ctx, hipMemsetAsync(didx + total, 0, sizeof(int), st));
+    if ((rc = hipcub_scan_int(ctx, dflag, didx, (int)total))) return rc;
+    flag_nonzero<<<ceil_div(nmol, 256), 256, 0, st>>>(eintra, nmol, dsflag);
+    if ((rc = hipcub_scan_int(ctx, dsflag, dsidx, nmol))) return rc;
+    int hlast[2], hflag[2];
+    HIP_CHECK(ctx, hipMemcpyAsync(&hlast[0], didx + total - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(ctx, hipMemcpyAsync(&hflag[0], dflag + total - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(ctx, hipMemcpyAsync(&hlast[1], dsidx + nmol - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(ctx, hipMemcpyAsync(&hflag[1], dsflag + nmol - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(ctx, hipStreamSynchronize(st));
+    const int npair = hlast[0] + hflag[0];
+    const int nself = hlast[1] + hflag[1];
+    const int nrec = nself + 2 * npair;
+    const int nent = nself + npair;
+
+    // pair arrays and the sort buffers (separate allocations: they outlive the scratch)
+    int *pim = nullptr, *pjm = nullptr, *key = nullptr, *val = nullptr, *skey = nullptr, *sval = nullptr,
+        *spos = nullptr;
+    float *pE = nullptr;
+    HIP_CHECK(ctx, hipMallocAsync((void **)&pim, sizeof(int) * (npair + 1), st));
+    HIP_CHECK(ctx, hipMallocAsync((void **)&pjm, sizeof(int) * (npair + 1), st));
+    HIP_CHECK(ctx, hipMallocAsync((void **)&pE, sizeof(float) * (npair + 1), st));
+    HIP_CHECK(ctx, hipMallocAsync((void **)&key, sizeof(int) * (nrec + 1), st));
+    HIP_CHECK(ctx, hipMallocAsync((void **)&val, sizeof(int) * (nrec + 1), st));
+    HIP_CHECK(ctx, hipMallocAsync((void **)&skey, sizeof(int) * (nrec + 1), st));
+    HIP_CHECK(ctx, hipMallocAsync((void **)&sval, sizeof(int) * (nrec + 1), st));
+    HIP_CHECK(ctx, hipMallocAsync((void **)&spos, sizeof(int) * (nrec + 1), st));
+    {
+        int g = persistent_blocks(0, total, 256);
+        fmolec1_compact<<<g, 256, 0, st>>>(drow, nmol, l->d_order, dmcell, dcstart, dnb, dE, dflag, didx, pim, pjm, pE);
+    }
+    fmolec1_records<<<ceil_div(std::max(nmol, npair), 256), 256, 0, st>>>(nmol, eintra, dsidx, nself, npair, pim, pjm,
+                                                                           key, val);
+    {
+        int bits = 1;
+        while ((1 << bits) < nmol) ++bits;
+        size_t tbytes = 0;
+        HIP_CHECK(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, key, skey, val, sval, nrec, 0, bits, st));
+        void *tmpb = nullptr;
+        HIP_CHECK(ctx, hipMallocAsync(&tmpb, tbytes + 16, st));
+        HIP_CHECK(ctx, hipcub::DeviceRadixSort::SortPairs(tmpb, tbytes, key, skey, val, sval, nrec, 0, bits, st));
+        HIP_CHECK(ctx, hipFreeAsync(tmpb, st));
+    }
+    // rows
+    if (l->cap_slots < (size_t)nrec + 1) {
+        if ((rc = dev_alloc(ctx, &l->d_partner, nrec + 1)) || (rc = dev_alloc(ctx, &l->d_entry, nrec + 1)) ||
+            (rc = dev_alloc(ctx, &l->d_mirror, nrec + 1)) || (rc = dev_alloc(ctx, &l->d_slotmol, nrec + 1)))
+            return rc;
+        l->cap_slots = (size_t)nrec + 1;
+    }
+    if (l->cap_entries < (size_t)nent + 1) {
+        if ((rc = dev_alloc(ctx, &l->d_ergpair, nent + 1))) return rc;
+        l->cap_entries = (size_t)nent + 1;
+    }
+    HIP_CHECK(ctx, hipMemsetAsync(l->d_cnt, 0, sizeof(int) * nmol, st));
+    count_keys<<<ceil_div(nrec, 256), 256, 0, st>>>(skey, nrec, l->d_cnt);
+    if ((rc = hipcub_scan_int(ctx, l->d_cnt, l->d_rowstart, nmol))) return rc;
+    fmolec1_rows<<<ceil_div(nrec, 256), 256, 0, st>>>(skey, sval, nrec, nself, pim, pjm, pE, eintra, l->d_rowstart,
+                                                      l->d_partner, l->d_entry, l->d_slotmol, spos, l->d_ergpair);
+    fmolec1_mirror<<<ceil_div(nrec, 256), 256, 0, st>>>(sval, nrec, nself, spos, l->d_mirror);
+    // FP64 totals
+    double *dsum = nullptr;
+    HIP_CHECK(ctx, hipMallocAsync((void **)&dsum, 2 * sizeof(double), st));
+    HIP_CHECK(ctx, hipMemsetAsync(dsum, 0, 2 * sizeof(double), st));
+    sum_f64<<<64, 256, 0, st>>>(pE, npair, dsum);
+    sum_f64<<<16, 256, 0, st>>>(eintra, nmol, dsum + 1);
+    double hs[2];
+    HIP_CHECK(ctx, hipMemcpyAsync(hs, dsum, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+    l->h_cnt.resize(nmol);
+    l->h_rowstart.resize(nmol + 1);
+    HIP_CHECK(ctx, hipMemcpyAsync(l->h_cnt.data(), l->d_cnt, sizeof(int) * nmol, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(ctx, hipMemcpyAsync(l->h_rowstart.data(), l->d_rowstart, sizeof(int) * nmol, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(ctx, hipFreeAsync(dsum, st));
+    HIP_CHECK(ctx, hipFreeAsync(pim, st)); HIP_CHECK(ctx, hipFreeAsync(pjm, st)); HIP_CHECK(ctx, hipFreeAsync(pE, st));
+    HIP_CHECK(ctx, hipFreeAsync(key, st)); HIP_CHECK(ctx, hipFreeAsync(val, st));
+    HIP_CHECK(ctx, hipFreeAsync(skey, st)); HIP_CHECK(ctx, hipFreeAsync(sval, st)); HIP_CHECK(ctx, hipFreeAsync(spos, st));
+    HIP_CHECK(ctx, hipGetLastError());
+    HIP_CHECK(ctx, hipStreamSynchronize(st));
+    l->h_rowstart[nmol] = nrec;
+    int maxcnt = 0;
+    for (int v : l->h_cnt) maxcnt = std::max(maxcnt, v);
+    // set_ergpair caps every list at nnbrefmax; the cells bound a list by the molecules of
+    // the neighbour cells, which is nnbrefmax, so the cap can never bind
+    ARG_CHECK(ctx, maxcnt <= l->nnbrefmax, "epsr_fmolec1: a ledger list of %d exceeds nnbrefmax %d", maxcnt,
+              l->nnbrefmax);
+    l->nself = nself;
+    l->npairent = npair;
+    l->nslots = nrec;
+    l->nentries = nent;
+    l->ledger_ready = true;
+    if (ftest64) *ftest64 = hs[0] / (double)nmol;
+    if (ftestb64) *ftestb64 = hs[1] / (double)nmol;
+    if (ftest) *ftest = (float)(hs[0] / (double)nmol);
+    if (ftestb) *ftestb = (float)(hs[1] / (double)nmol);
+    if (nnbrefmax_out) *nnbrefmax_out = l->nnbrefmax;
+    if (nergpair_out) *nergpair_out = nent;
+    return EPSR_OK;
+}
+
+extern "C" int epsr_ledger_export(epsr_ctx *ctx, const epsr_ledger *l, int ldl, int *nergpairm, int *molergpair,
+                                  int *ergpairref, float *ergpair, int *order)
+{
+    if (!ctx) return EPSR_ERR_ARG;
+    ARG_CHECK(ctx, l && l->ledger_ready, "epsr_ledger_export: no ledger built");
+    HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    const int nmol = l->nmol;
+    std::vector<int> partner(l->nslots), entry(l->nslots);
+    HIP_CHECK(ctx, hipMemcpy(partner.data(), l->d_partner, sizeof(int) * l->nslots, hipMemcpyDeviceToHost));
+    HIP_CHECK(ctx, hipMemcpy(entry.data(), l->d_entry, sizeof(int) * l->nslots, hipMemcpyDeviceToHost));
+    if (ergpair) HIP_CHECK(ctx, hipMemcpy(ergpair, l->d_ergpair, sizeof(float) * l->nentries, hipMemcpyDeviceToHost));
+    for (int m = 0; m < nmol; ++m) {
+        int n = l->h_cnt[m];
+        ARG_CHECK(ctx, n <= ldl, "epsr_ledger_export: ldl %d < list length %d", ldl, n);
+        if (nergpairm) nergpairm[m] = n;
+        for (int s = 0; s < n; ++s) {
+            if (molergpair) molergpair[(size_t)m * ldl + s] = partner[l->h_rowstart[m] + s];
+            if (ergpairref) ergpairref[(size_t)m * ldl + s] = entry[l->h_rowstart[m] + s];
+        }
+    }
+    if (order) std::copy(l->order.begin(), l->order.end(), order);
+    return EPSR_OK;
+}
+
+// ------------------------------------------------------------------ en_change / en_mole_pair
+// en_change(im) at the current coordinates: one workgroup; lane s computes the ledger
+// slot s >= 1, lane 0 then forms the sequential float32 difference (molfit.py:846-890).
+__global__ void en_change_kernel(int im, const int *__restrict__ rowstart, const int *__restrict__ cnt,
+                                 const int *__restrict__ partner, const int *__restrict__ entry,
+                                 const float *__restrict__ ergpair, const float4 *__restrict__ P,
+                                 const int *__restrict__ naf, const int *__restrict__ nal, PairTab t,
+                                 float *__restrict__ ergnew_out, float *__restrict__ fdiff_out, float *__restrict__ work)
+{
+    const int row = rowstart[im], c = cnt[im];
+    const float4 *P1 = P + naf[im];
+    const int n1 = nal[im] - naf[im] + 1;
+    for (int s = threadIdx.x; s < c; s += blockDim.x) {
+        float old = ergpair[entry[row + s]];
+        float v = old;
+        if (s >= 1) {
+            int n = partner[row + s];
+            v = 0.0f + mol_pair(P1, n1, P + naf[n], nal[n] - naf[n] + 1, t);
+        }
+        work[2 * s] = v;
+        work[2 * s + 1] = old;
+        if (ergnew_out) ergnew_out[s] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float fdiff = 0.0f;
+        for (int s = 1; s < c; ++s) fdiff = fdiff + work[2 * s] - work[2 * s + 1];
+        *fdiff_out = fdiff;
+    }
+}
+
+extern "C" int epsr_en_change(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *c, const epsr_tables_t *tb,
+                              const epsr_ledger *l, int im, float *ergnew, float *fdiff)
+{
+    if (!ctx) return EPSR_ERR_ARG;
+    ARG_CHECK(ctx, box && c && tb && l && fdiff, "epsr_en_change: null argument");
+    ARG_CHECK(ctx, l->ledger_ready, "epsr_en_change: no ledger (call epsr_fmolec1)");
+    ARG_CHECK(ctx, im >= 0 && im < box->nmol, "epsr_en_change: molecule %d out of range", im);
+    HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    Scratch S;
+    Scratch q;
+    q.take<float4>(box->natom); q.take<float4>((size_t)box->npairs * std::max(2, std::min(tb->ld, tb->npt2 + 1)));
+    q.take<int>(4); q.take<float>(2 * (size_t)l->nnbrefmax + 2); q.take<float>(1);
+    S.base = (char *)ctx_scratch(ctx, q.off);
+    if (!S.base) return EPSR_ERR_HIP;
+    float4 *dP = S.take<float4>(box->natom);
+    float4 *dT = S.take<float4>((size_t)box->npairs * std::max(2, std::min(tb->ld, tb->npt2 + 1)));
+    int *dtmp = S.take<int>(4);
+    float *work = S.take<float>(2 * (size_t)l->nnbrefmax + 2);
+    float *dfd = S.take<float>(1);
+    PreparedTables pt;
+    int rc = prepare_tables(ctx, box, tb, dT, dtmp, pt);
+    if (rc) return rc;
+    abs_positions<<<ceil_div(box->natom, 256), 256, 0, st>>>(c->xa, c->ya, c->za, c->xadd, c->yadd, c->zadd,
+                                                             box->d_amol, box->d_itype, dP, box->natom);
+    en_change_kernel<<<1, 256, 0, st>>>(im, l->d_rowstart, l->d_cnt, l->d_partner, l->d_entry, l->d_ergpair, dP,
+                                       box->d_naf, box->d_nal, pt.tab, ergnew, dfd, work);
+    HIP_CHECK(ctx, hipMemcpyAsync(fdiff, dfd, sizeof(float), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(ctx, hipGetLastError());
+    HIP_CHECK(ctx, hipStreamSynchronize(st));
+    return EPSR_OK;
+}
+
+__global__ void en_mole_pair_kernel(int npair, const int *__restrict__ im, const int *__restrict__ jm,
+                                    const float4 *__restrict__ P, const int *__restrict__ naf,
+                                    const int *__restrict__ nal, PairTab t, float *__restrict__ out)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npair) return;
+    int a = im[k], b = jm[k];
+    out[k] = mol_pair(P + naf[a], nal[a] - naf[a] + 1, P + naf[b], nal[b] - naf[b] + 1, t);
+}
+
+extern "C" int epsr_en_mole_pair(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *c, const epsr_tables_t *tb,
+                                 int npair, const int *im, const int *jm, float *energy)
+{
+    if (!ctx) return EPSR_ERR_ARG;
+    ARG_CHECK(ctx, box && c && tb && im && jm && energy && npair >= 0, "epsr_en_mole_pair: bad argument");
+    HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    if (npair == 0) return EPSR_OK;
+    hipStream_t st = ctx->stream;
+    Scratch S, q;
+    q.take<float4>(box->natom); q.take<float4>((size_t)box->npairs * std::max(2, std::min(tb->ld, tb->npt2 + 1)));
+    q.take<int>(4);
+    S.base = (char *)ctx_scratch(ctx, q.off);
+    if (!S.base) return EPSR_ERR_HIP;
+    float4 *dP = S.take<float4>(box->natom);
+    float4 *dT = S.take<float4>((size_t)box->npairs * std::max(2, std::min(tb->ld, tb->npt2 + 1)));
+    int *dtmp = S.take<int>(4);
+    PreparedTables pt;
+    int rc = prepare_tables(ctx, box, tb, dT, dtmp, pt);
+    if (rc) return rc;
+    abs_positions<<<ceil_div(box->natom, 256), 256, 0, st>>>(c->xa, c->ya, c->za, c->xadd, c->yadd, c->zadd,
+                                                             box->d_amol, box->d_itype, dP, box->natom);
+    en_mole_pair_kernel<<<ceil_div(npair, 128), 128, 0, st>>>(npair, im, jm, dP, box->d_naf, box->d_nal, pt.tab, energy);
+    HIP_CHECK(ctx, hipGetLastError());
+    HIP_CHECK(ctx, hipStreamSynchronize(st));
+    return EPSR_OK;
+}
+
+// ------------------------------------------------------------------ molfit
+__device__ __forceinline__ float cr_cos(float x) { return (float)cos((double)x); }
+__device__ __forceinline__ float cr_exp(float x) { return (float)exp((double)x); }
+
+// A: trial moves.  rotatm (molfit.py:671-711) about z, y, x with correctly rounded
+// cosf; movem (:732-759) with the translation triplet at toff (the host has resolved
+// any step doubling).  nochange is reported so the host can resolve it.
+__global__ void mc_trial(int nmol, const int *__restrict__ order, const int *__restrict__ naf,
+                         const int *__restrict__ nal, const float *__restrict__ xa, const float *__restrict__ ya,
+                         const float *__restrict__ za, const float *__restrict__ xadd, const float *__restrict__ yadd,
+                         const float *__restrict__ zadd, const float *__restrict__ stream, const int *__restrict__ roff,
+                         const int *__restrict__ toff, const float *__restrict__ rstep, const float *__restrict__ tstep,
+                         float *__restrict__ txa, float *__restrict__ tya, float *__restrict__ tza,
+                         float *__restrict__ txadd, float *__restrict__ tyadd, float *__restrict__ tzadd,
+                         int *__restrict__ nochange)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nmol) return;
+    const float pi = 3.14159265358979323846f;
+    int m = order[k];
+    int a0 = naf[m], a1 = nal[m];
+    for (int a = a0; a <= a1; ++a) {
+        txa[a] = xa[a];
+        tya[a] = ya[a];
+        tza[a] = za[a];
+    }
+    if (roff[k] >= 0 && a1 > a0) {
+        float st = rstep[k];
+        for (int axis = 0; axis < 3; ++axis) {
+            float phi = pi * st * (2.0f * stream[roff[k] + axis] - 1.0f);
+            float cphi = cr_cos(phi);
+            float sphi = copysignf(fabsf(sqrtf(1.0f - cphi * cphi)), phi);
+            for (int a = a0; a <= a1; ++a) {
+                float x = txa[a], y = tya[a], z = tza[a];
+                if (axis == 0) {
+                    txa[a] = x * cphi - y * sphi;
+                    tya[a] = x * sphi + y * cphi;
+                } else if (axis == 1) {
+                    tza[a] = z * cphi - x * sphi;
+                    txa[a] = z * sphi + x * cphi;
+                } else {
+                    tya[a] = y * cphi - z * sphi;
+                    tza[a] = y * sphi + z * cphi;
+                }
+            }
+        }
+    }
+    float x = xadd[m], y = yadd[m], z = zadd[m];
+    float nx = x, ny = y, nz = z;
+    if (toff[k] >= 0) {
+        float st = tstep[k];
+        nz = z + st * (2.0f * stream[toff[k]] - 1.0f);
+        ny = y + st * (2.0f * stream[toff[k] + 1] - 1.0f);
+        nx = x + st * (2.0f * stream[toff[k] + 2] - 1.0f);
+        if (nx == x && ny == y && nz == z) atomicAdd(nochange, 1);
+    }
+    txadd[m] = nx;
+    tyadd[m] = ny;
+    tzadd[m] = nz;
+}
+
+// B: the pair energies a slot can need.  For slot i (molecule m, partner n, slot >= 1):
+// A = E(m_trial, n_current), Bv = E(m_trial, n_trial) when n moves before m, and the
+// ledger value at the start of the cycle.
+template <bool LDS>
+__global__ void mc_pairs(int nslots, const int *__restrict__ slotmol, const int *__restrict__ rowstart,
+                         const int *__restrict__ partner, const int *__restrict__ entry, const int *__restrict__ pos,
+                         const float *__restrict__ ergpair, const float4 *__restrict__ Pold,
+                         const float4 *__restrict__ Pnew, const int *__restrict__ naf, const int *__restrict__ nal,
+                         PairTab gt, int npairs, float *__restrict__ A, float *__restrict__ Bv, float *__restrict__ L)
+{
+    extern __shared__ char smem[];
+    PairTab t = LDS ? tables_to_lds(gt, npairs, smem) : gt;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nslots; i += gridDim.x * blockDim.x) {
+        int m = slotmol[i];
+        L[i] = ergpair[entry[i]];
+        if (i == rowstart[m]) continue;
+        int n = partner[i];
+        const float4 *P1 = Pnew + naf[m];
+        int n1 = nal[m] - naf[m] + 1, n2 = nal[n] - naf[n] + 1;
+        A[i] = mol_pair(P1, n1, Pold + naf[n], n2, t);
+        if (pos[n] < pos[m]) Bv[i] = mol_pair(P1, n1, Pnew + naf[n], n2, t);
+    }
+}
+
+// C: the Metropolis chain (molfit.py:397-414), one workgroup.  acc bits by chain
+// position live in LDS; lane 0 sums each move in the reference's order.
+__global__ void __launch_bounds__(256)
+mc_resolve(int nmol, const int *__restrict__ order, const int *__restrict__ pos, const int *__restrict__ rowstart,
+           const int *__restrict__ cnt, const int *__restrict__ partner, const int *__restrict__ mirror,
+           const float *__restrict__ A, const float *__restrict__ Bv, const float *__restrict__ L,
+           const float *__restrict__ stream, const int *__restrict__ moff, float temp, int maxc,
+           unsigned *__restrict__ acc_out, float *__restrict__ fdiff_out, float *__restrict__ res)
+{
+    extern __shared__ unsigned sm[];
+    const int nw = (nmol + 31) >> 5;
+    unsigned *bits = sm;
+    float *term = (float *)(bits + nw);   // [2*maxc]
+    for (int i = threadIdx.x; i < nw; i += blockDim.x) bits[i] = 0u;
+    float fadd = 0.0f;
+    int nacc = 0;
+    __syncthreads();
+    for (int k = 0; k < nmol; ++k) {
+        const int m = order[k];
+        const int row = rowstart[m], c = cnt[m];
+        for (int s = 1 + threadIdx.x; s < c; s += blockDim.x) {
+            int i = row + s;
+            int n = partner[i];
+            int pn = pos[n];
+            bool a = pn < k && ((bits[pn >> 5] >> (pn & 31)) & 1u);
+            int mi = mirror[i];
+            bool wrote = a && mi != rowstart[n];      // n's move rewrote this entry
+            term[2 * s] = a ? Bv[i] : A[i];
+            term[2 * s + 1] = wrote ? A[mi] : L[i];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float fdiff = 0.0f;
+            for (int s = 1; s < c; ++s) fdiff = fdiff + term[2 * s] - term[2 * s + 1];
+            float ediff = (fdiff + 0.0f) * temp;
+            float etest;
+            if (ediff <= 0.0f) etest = 10.0f;
+            else if (ediff < 20.0f) etest = cr_exp(-ediff);
+            else etest = 0.0f;
+            bool accept = stream[moff[k]] < etest;
+            if (accept) {
+                bits[k >> 5] |= 1u << (k & 31);
+                fadd = fadd + fdiff;
+                nacc++;
+            }
+            fdiff_out[k] = fdiff;
+        }
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < nw; i += blockDim.x) acc_out[i] = bits[i];
+    if (threadIdx.x == 0) {
+        res[0] = fadd;
+        res[1] = (float)nacc;
+    }
+}
+
+// D: write back positions, counters and the ledger
+__global__ void mc_commit_mols(int nmol, const int *__restrict__ order, const int *__restrict__ naf,
+                               const int *__restrict__ nal, const int *__restrict__ mtype,
+                               const unsigned *__restrict__ accb, const int *__restrict__ roff,
+                               const int *__restrict__ toff, const float *__restrict__ txa,
+                               const float *__restrict__ tya, const float *__restrict__ tza,
+                               const float *__restrict__ txadd, const float *__restrict__ tyadd,
+                               const float *__restrict__ tzadd, float *__restrict__ xa, float *__restrict__ ya,
+                               float *__restrict__ za, float *__restrict__ xadd, float *__restrict__ yadd,
+                               float *__restrict__ zadd, int *__restrict__ nstepreject, int count_rot,
+                               int *__restrict__ rej)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nmol) return;
+    int m = order[k];
+    bool acc = (accb[k >> 5] >> (k & 31)) & 1u;
+    if (acc) {
+        for (int a = naf[m]; a <= nal[m]; ++a) {
+            xa[a] = txa[a];
+            ya[a] = tya[a];
+            za[a] = tza[a];
+        }
+        xadd[m] = txadd[m];
+        yadd[m] = tyadd[m];
+        zadd[m] = tzadd[m];
+        nstepreject[m] = 0;
+    } else {
+        nstepreject[m] += 1;
+        int mt = mtype[m];
+        if (count_rot) atomicAdd(&rej[2 * mt], 1);
+        if (toff[k] >= 0) atomicAdd(&rej[2 * mt + 1], 1);
+    }
+}
+
+// each pair entry is finalised by its slot in the row of the molecule that moves later
+__global__ void mc_commit_ledger(int nslots, const int *__restrict__ slotmol, const int *__restrict__ rowstart,
+                                 const int *__restrict__ partner, const int *__restrict__ entry,
+                                 const int *__restrict__ mirror, const int *__restrict__ pos,
+                                 const unsigned *__restrict__ accb, const float *__restrict__ A,
+                                 const float *__restrict__ Bv, float *__restrict__ ergpair)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nslots) return;
+    int mi = mirror[i];
+    if (mi < 0) return;
+    int m = slotmol[i], n = partner[i];
+    int pm = pos[m], pn = pos[n];
+    if (pm < pn) return;
+    bool am = (accb[pm >> 5] >> (pm & 31)) & 1u;
+    bool an = (accb[pn >> 5] >> (pn & 31)) & 1u;
+    bool wm = am && i != rowstart[m];
+    bool wn = an && mi != rowstart[n];
+    if (wm) ergpair[entry[i]] = an ? Bv[i] : A[i];
+    else if (wn) ergpair[entry[i]] = A[mi];
+}
+
+static float adapt_ratio(float rejrate, int att, int rej)
+{
+    float rat;
+    if (rej > 0) {
+        rat = rejrate * (float)att / (float)rej;
+        if (rat > 1.1f) rat = 1.1f;
+        if (rat < 0.9f) rat = 0.9f;
+    } else {
+        rat = att > 0 ? 1.1f : 1.0f;
+    }
+    return rat;
+}
+
+extern "C" int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *c, const epsr_tables_t *tb,
+                           epsr_ledger *l, const epsr_mc_params_t *p, float *wholerot, float *wholetrans, int *nwhole,
+                           int *nmovem, int *nstepreject, int32_t *ran1_state, float *fadds, epsr_mc_stats_t *stats)
+{
+    if (!ctx) return EPSR_ERR_ARG;
+    ARG_CHECK(ctx, box && c && tb && l && p && wholerot && wholetrans && nwhole && nmovem && nstepreject &&
+                       ran1_state && fadds,
+              "epsr_molfit: null argument");
+    ARG_CHECK(ctx, l->ledger_ready, "epsr_molfit: no ledger (call epsr_fmolec1)");
+    HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int nmol = box->nmol, natom = box->natom, nmt = box->nmoltype;
+    const float temp = 1.0f / p->simt;
+    float trstepmax = 1.0f / box->natomr;
+    trstepmax = powf(trstepmax, 1.0f / 3.0f);
+    float trstepmin = 0.0001f * trstepmax;
+    if (trstepmin < 1.0e-6f) trstepmin = 1.0e-6f;
+    const float rotstepmax = 1.0f, rotstepmin = 1.0e-5f;
+
+    // host: per-move ran1 offsets and step sizes in chain order (molfit.py:367-403)
+    std::vector<int> roff(nmol), toff(nmol), moff(nmol);
+    std::vector<float> rstep(nmol), tstep(nmol);
+    std::vector<int> att(2 * nmt, 0);
+    std::vector<float> wr(wholerot, wholerot + nmt), wt(wholetrans, wholetrans + nmt);
+    int cursor = 0;
+    for (int k = 0; k < nmol; ++k) {
+        int m = l->order[k], mt = box->h_mtype[m];
+        int nat = box->h_nal[m] - box->h_naf[m] + 1;
+        roff[k] = -1;
+        toff[k] = -1;
+        if (p->stepr > 0.0f && nat > 1) {
+            if (wr[mt] <= 0.0f) wr[mt] = rotstepmin;
+            att[2 * mt] += 1;
+            roff[k] = cursor;
+            rstep[k] = wr[mt];
+            cursor += 3;
+        } else {
+            wr[mt] = 0.0f;
+            if (p->stepr > 0.0f) roff[k] = -3;   // no rotation for a single atom; counted only if stepr > 0
+        }
+        if (p->stepm > 0.0f) {
+            if (wt[mt] <= 0.0f) wt[mt] = trstepmin;
+            att[2 * mt + 1] += 1;
+            toff[k] = cursor;
+            tstep[k] = wt[mt];
+            cursor += 3;
+        } else {
+            wt[mt] = 0.0f;
+        }
+        moff[k] = cursor;
+        cursor += 1;
+    }
+    int total = cursor;
+    Ran1 rng;
+    ran1_load(rng, ran1_state);
+    Ran1 rng0 = rng;
+    std::vector<float> hstream(total);
+    for (int i = 0; i < total; ++i) hstream[i] = ran1_next(rng);
+
+    // device buffers
+    const int nslots = l->nslots;
+    Scratch S, q;
+    auto layout = [&](Scratch &x) {
+        x.take<float4>(natom); x.take<float4>(natom);
+        x.take<float4>((size_t)box->npairs * std::max(2, std::min(tb->ld, tb->npt2 + 1)));
+        x.take<int>(4);
+        x.take<float>(total + 8); x.take<int>(nmol); x.take<int>(nmol); x.take<int>(nmol);
+        x.take<float>(nmol); x.take<float>(nmol);
+        x.take<float>(natom); x.take<float>(natom); x.take<float>(natom);
+        x.take<float>(nmol); x.take<float>(nmol); x.take<float>(nmol);
+        x.take<float>(nslots); x.take<float>(nslots); x.take<float>(nslots);
+        x.take<unsigned>((nmol + 31) / 32); x.take<float>(nmol); x.take<float>(4); x.take<int>(2 * nmt + 2);
+    };
+    layout(q);
+    S.base = (char *)ctx_scratch(ctx, q.off);
+    if (!S.base) return EPSR_ERR_HIP;
+    float4 *Pold = S.take<float4>(natom), *Pnew = S.take<float4>(natom);
+    float4 *dT = S.take<float4>((size_t)box->npairs * std::max(2, std::min(tb->ld, tb->npt2 + 1)));
+    int *dtmp = S.take<int>(4);
+    float *dstream = S.take<float>(total + 8);
+    int *droff = S.take<int>(nmol), *dtoff = S.take<int>(nmol), *dmoff = S.take<int>(nmol);
+    float *drstep = S.take<float>(nmol), *dtstep = S.take<float>(nmol);
+    float *txa = S.take<float>(natom), *tya = S.take<float>(natom), *tza = S.take<float>(natom);
+    float *txadd = S.take<float>(nmol), *tyadd = S.take<float>(nmol), *tzadd = S.take<float>(nmol);
+    float *dA = S.take<float>(nslots), *dB = S.take<float>(nslots), *dL = S.take<float>(nslots);
+    unsigned *dacc = S.take<unsigned>((nmol + 31) / 32);
+    float *dfdiff = S.take<float>(nmol);
+    float *dres = S.take<float>(4);
+    int *drej = S.take<int>(2 * nmt + 2);
+
+    PreparedTables pt;
+    int rc = prepare_tables(ctx, box, tb, dT, dtmp, pt);
+    if (rc) return rc;
+    HIP_CHECK(ctx, hipMemcpyAsync(dstream, hstream.data(), sizeof(float) * total, hipMemcpyHostToDevice, st));
+    HIP_CHECK(ctx, hipMemcpyAsync(droff, roff.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, st));
+    HIP_CHECK(ctx, hipMemcpyAsync(dtoff, toff.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, st));
+    HIP_CHECK(ctx, hipMemcpyAsync(dmoff, moff.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, st));
+    HIP_CHECK(ctx, hipMemcpyAsync(drstep, rstep.data(), sizeof(float) * nmol, hipMemcpyHostToDevice, st));
+    HIP_CHECK(ctx, hipMemcpyAsync(dtstep, tstep.data(), sizeof(float) * nmol, hipMemcpyHostToDevice, st));
+    HIP_CHECK(ctx, hipMemsetAsync(dtmp + 1, 0, sizeof(int), st));
+    mc_trial<<<ceil_div(nmol, 128), 128, 0, st>>>(nmol, l->d_order, box->d_naf, box->d_nal, c->xa, c->ya, c->za, c->xadd,
+                                                  c->yadd, c->zadd, dstream, droff, dtoff, drstep, dtstep, txa, tya,
+                                                  tza, txadd, tyadd, tzadd, dtmp + 1);
+    int nochange = 0;
+    HIP_CHECK(ctx, hipMemcpyAsync(&nochange, dtmp + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(ctx, hipStreamSynchronize(st));
+    int retries = 0;
+    if (nochange) {
+        // movem doubles its step until the centre moves (molfit.py:747-758); the doubled
+        // step persists for the molecule type.  Resolve the chain on the host from the
+        // current centres, then redo the trial moves.
+        std::vector<float> cx(nmol), cy(nmol), cz(nmol);
+        HIP_CHECK(ctx, hipMemcpy(cx.data(), c->xadd, sizeof(float) * nmol, hipMemcpyDeviceToHost));
+        HIP_CHECK(ctx, hipMemcpy(cy.data(), c->yadd, sizeof(float) * nmol, hipMemcpyDeviceToHost));
+        HIP_CHECK(ctx, hipMemcpy(cz.data(), c->zadd, sizeof(float) * nmol, hipMemcpyDeviceToHost));
+        rng = rng0;
+        hstream.clear();
+        auto draw = [&](int idx) {
+            while ((int)hstream.size() <= idx) hstream.push_back(ran1_next(rng));
+            return hstream[idx];
+        };
+        wr.assign(wholerot, wholerot + nmt);
+        wt.assign(wholetrans, wholetrans + nmt);
+        cursor = 0;
+        for (int k = 0; k < nmol; ++k) {
+            int m = l->order[k], mt = box->h_mtype[m];
+            int nat = box->h_nal[m] - box->h_naf[m] + 1;
+            if (p->stepr > 0.0f && nat > 1) {
+                if (wr[mt] <= 0.0f) wr[mt] = rotstepmin;
+                roff[k] = cursor;
+                rstep[k] = wr[mt];
+                cursor += 3;
+            } else {
+                wr[mt] = 0.0f;
+                roff[k] = p->stepr > 0.0f ? -3 : -1;
+            }
+            if (p->stepm > 0.0f) {
+                if (wt[mt] <= 0.0f) wt[mt] = trstepmin;
+                float s = wt[mt];
+                while (true) {
+                    float nz = cz[m] + s * (2.0f * draw(cursor) - 1.0f);
+                    float ny = cy[m] + s * (2.0f * draw(cursor + 1) - 1.0f);
+                    float nx = cx[m] + s * (2.0f * draw(cursor + 2) - 1.0f);
+                    if (!(nx == cx[m] && ny == cy[m] && nz == cz[m])) break;
+                    s = s * 2.0f;
+                    cursor += 3;
+                    retries++;
+                }
+                toff[k] = cursor;
+                tstep[k] = s;
+                wt[mt] = s;
+                cursor += 3;
+            } else {
+                wt[mt] = 0.0f;
+                toff[k] = -1;
+            }
+            moff[k] = cursor;
+            draw(cursor);
+            cursor += 1;
+        }
+        total = cursor;
+        hstream.resize(total);
+        void *dstream2 = nullptr;
+        HIP_CHECK(ctx, hipMallocAsync(&dstream2, sizeof(float) * (total + 8), st));
+        HIP_CHECK(ctx, hipMemcpyAsync(dstream2, hstream.data(), sizeof(float) * total, hipMemcpyHostToDevice, st));
+        HIP_CHECK(ctx, hipMemcpyAsync(droff, roff.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, st));
+        HIP_CHECK(ctx, hipMemcpyAsync(dtoff, toff.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, st));
+        HIP_CHECK(ctx, hipMemcpyAsync(dmoff, moff.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, st));
+        HIP_CHECK(ctx, hipMemcpyAsync(drstep, rstep.data(), sizeof(float) * nmol, hipMemcpyHostToDevice, st));
+        HIP_CHECK(ctx, hipMemcpyAsync(dtstep, tstep.data(), sizeof(float) * nmol, hipMemcpyHostToDevice, st));
+        mc_trial<<<ceil_div(nmol, 128), 128, 0, st>>>(nmol, l->d_order, box->d_naf, box->d_nal, c->xa, c->ya, c->za,
+                                                      c->xadd, c->yadd, c->zadd, (const float *)dstream2, droff, dtoff,
+                                                      drstep, dtstep, txa, tya, tza, txadd, tyadd, tzadd, dtmp + 2);
+        HIP_CHECK(ctx, hipStreamSynchronize(st));
+        dstream = (float *)dstream2;   // used by mc_resolve below; freed at the end
+    }
+    // trial and current absolute positions
+    abs_positions<<<ceil_div(natom, 256), 256, 0, st>>>(c->xa, c->ya, c->za, c->xadd, c->yadd, c->zadd, box->d_amol,
+                                                        box->d_itype, Pold, natom);
+    abs_positions<<<ceil_div(natom, 256), 256, 0, st>>>(txa, tya, tza, txadd, tyadd, tzadd, box->d_amol, box->d_itype,
+                                                        Pnew, natom);
+    timer_begin(ctx, "mc_pair");
+    if (pt.in_lds) {
+        int g = persistent_blocks(pt.lds_bytes, nslots, 256);
+        mc_pairs<true><<<g, 256, pt.lds_bytes, st>>>(nslots, l->d_slotmol, l->d_rowstart, l->d_partner, l->d_entry,
+                                                     l->d_pos, l->d_ergpair, Pold, Pnew, box->d_naf, box->d_nal,
+                                                     pt.tab, box->npairs, dA, dB, dL);
+    } else {
+        int g = persistent_blocks(0, nslots, 256);
+        mc_pairs<false><<<g, 256, 0, st>>>(nslots, l->d_slotmol, l->d_rowstart, l->d_partner, l->d_entry, l->d_pos,
+                                           l->d_ergpair, Pold, Pnew, box->d_naf, box->d_nal, pt.tab, box->npairs, dA,
+                                           dB, dL);
+    }
+    timer_end(ctx, "mc_pair");
+    int maxc = 1;
+    for (int v : l->h_cnt) maxc = std::max(maxc, v);
+    size_t lds = sizeof(unsigned) * ((nmol + 31) / 32) + sizeof(float) * 2 * (size_t)maxc + 64;
+    ARG_CHECK(ctx, lds <= 160 * 1024, "epsr_molfit: %d molecules x %d ledger slots exceed LDS", nmol, maxc);
+    timer_begin(ctx, "mc_resolve");
+    mc_resolve<<<1, 256, lds, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner, l->d_mirror,
+                                    dA, dB, dL, dstream, dmoff, temp, maxc, dacc, dfdiff, dres);
+    timer_end(ctx, "mc_resolve");
+    HIP_CHECK(ctx, hipMemsetAsync(drej, 0, sizeof(int) * 2 * nmt, st));
+    mc_commit_ledger<<<ceil_div(nslots, 256), 256, 0, st>>>(nslots, l->d_slotmol, l->d_rowstart, l->d_partner,
+                                                            l->d_entry, l->d_mirror, l->d_pos, dacc, dA, dB,
+                                                            l->d_ergpair);
+    mc_commit_mols<<<ceil_div(nmol, 128), 128, 0, st>>>(nmol, l->d_order, box->d_naf, box->d_nal, box->d_mtype, dacc,
+                                                        droff, dtoff, txa, tya, tza, txadd, tyadd, tzadd, c->xa, c->ya,
+                                                        c->za, c->xadd, c->yadd, c->zadd, nstepreject,
+                                                        p->stepr > 0.0f ? 1 : 0, drej);
+    float hres[2];
+    std::vector<int> rej(2 * nmt);
+    HIP_CHECK(ctx, hipMemcpyAsync(hres, dres, sizeof(float) * 2, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(ctx, hipMemcpyAsync(rej.data(), drej, sizeof(int) * 2 * nmt, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(ctx, hipGetLastError());
+    HIP_CHECK(ctx, hipStreamSynchronize(st));
+    if (nochange) HIP_CHECK(ctx, hipFree(dstream));
+    // counters and the step-size adaptation (molfit.py:465-536)
+    for (int mt = 0; mt < nmt; ++mt) {
+        nwhole[2 * mt] = att[2 * mt];
+        nwhole[2 * mt + 1] = rej[2 * mt];
+        nmovem[2 * mt] = att[2 * mt + 1];
+        nmovem[2 * mt + 1] = rej[2 * mt + 1];
+    }
+    if (p->ireset != 1) {
+        for (int mt = 0; mt < nmt; ++mt) {
+            float rat = adapt_ratio(p->rejrate, nwhole[2 * mt], nwhole[2 * mt + 1]);
+            wr[mt] = wr[mt] * rat;
+            if (wr[mt] > rotstepmax) wr[mt] = rotstepmax;
+            if (wr[mt] < rotstepmin && wr[mt] != 0.0f) wr[mt] = rotstepmin;
+            rat = adapt_ratio(p->rejrate, nmovem[2 * mt], nmovem[2 * mt + 1]);
+            wt[mt] = wt[mt] * rat;
+            if (wt[mt] > trstepmax) wt[mt] = trstepmax;
+            if (wt[mt] < trstepmin && wt[mt] != 0.0f) wt[mt] = trstepmin;
+        }
+    }
+    std::copy(wr.begin(), wr.end(), wholerot);
+    std::copy(wt.begin(), wt.end(), wholetrans);
+    *fadds = *fadds + hres[0] / (float)nmol;
+    ran1_store(rng, ran1_state);
+    if (stats) {
+        stats->moves = nmol;
+        stats->accepted = (int)hres[1];
+        stats->exact_resolutions = nmol;
+        stats->retries = retries;
+        stats->ms_pairs = ctx->timing ? epsr_ctx_last_kernel_ms(ctx, "mc_pair") : -1.0;
+        stats->ms_resolve = ctx->timing ? epsr_ctx_last_kernel_ms(ctx, "mc_resolve") : -1.0;
+    }
+    return EPSR_OK;
+}

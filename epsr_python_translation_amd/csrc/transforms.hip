@@ -101,15 +101,23 @@ __global__ void __launch_bounds__(64) gensq_mfma(int nq, int npairs, int npt1, i
     }
 }
 
-// sum3 of gensq: sum over j of the intermolecular histogram (sequential, exact)
+// sum3 of gensq (:4785-4793) is only tested for <= 0.  The intermolecular histogram is
+// non-negative, so the sequential float32 sum is <= 0 exactly when every term is zero,
+// which a parallel maximum decides in any order.  One workgroup per pair.
 __global__ void gensq_colsum(int npairs, int npt1, int ldw, const float *__restrict__ wav, float *__restrict__ out)
 {
-    int ic = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ic >= npairs) return;
+    __shared__ float s[256];
+    int ic = blockIdx.x;
     const float *w2 = wav + ((size_t)npairs + ic) * ldw;
-    float s = 0.0f;
-    for (int j = 2; j <= npt1; ++j) s = s + w2[j];
-    out[ic] = s;
+    float m = 0.0f;
+    for (int j = 2 + threadIdx.x; j <= npt1; j += blockDim.x) m = fmaxf(m, w2[j]);
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] = fmaxf(s[threadIdx.x], s[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[ic] = s[0];
 }
 
 extern "C" int epsr_gensq(epsr_ctx *ctx, int nq, int npairs, int npt1, int ldw, const float *wav, const float *ycoh,
@@ -129,7 +137,7 @@ extern "C" int epsr_gensq(epsr_ctx *ctx, int nq, int npairs, int npt1, int ldw, 
     } else {
         float *cs = (float *)ctx_scratch(ctx, sizeof(float) * npairs);
         if (!cs) return EPSR_ERR_HIP;
-        gensq_colsum<<<ceil_div(npairs, 64), 64, 0, st>>>(npairs, npt1, ldw, wav, cs);
+        gensq_colsum<<<npairs, 256, 0, st>>>(npairs, npt1, ldw, wav, cs);
         dim3 grid(ceil_div(nq, 16), ceil_div(2 * npairs, 16));
         gensq_mfma<<<grid, 64, 0, st>>>(nq, npairs, npt1, ldw, wav, ycoh, foqols, ldf, natomsphere, cs, sofq, ldq);
     }

@@ -200,8 +200,136 @@ def foqols(ctx: Context, grid, rho, rmaxgr, fwhm=0.0, fwhmq=0.0, lorchqwidth=0.0
     """GPU foqols for a tables.Grid; returns (tensor [(nq+1)*ldf], ldf)."""
     ldf = grid.npt1 + 1
     out = ctx.zeros((grid.nq + 1) * ldf)
-    ctx.check(ctx.lib.epsr_foqols(ctx.h, int(grid.nq), _ptr(ctx.tensor(grid.q)), int(grid.npt1),
-                                  _ptr(ctx.tensor(grid.x)), _ptr(ctx.tensor(grid.r2)), _ptr(ctx.tensor(grid.ycoh)),
+    # the inputs stay referenced until the call returns (the caching allocator would
+    # otherwise hand a freed temporary to the next one)
+    q, x, r2, ycoh = (ctx.tensor(a) for a in (grid.q, grid.x, grid.r2, grid.ycoh))
+    ctx.check(ctx.lib.epsr_foqols(ctx.h, int(grid.nq), _ptr(q), int(grid.npt1), _ptr(x), _ptr(r2), _ptr(ycoh),
                                   F32(rho), F32(fwhm), F32(fwhmq), F32(lorchqwidth), F32(lorchqwidthq), F32(rmaxgr),
                                   _ptr(out), int(ldf)))
     return out, ldf
+
+
+class DeviceTables:
+    """Potential tables resident in HBM (float32 [npairs*ld] each, 1-based bins)."""
+
+    def __init__(self, ctx: Context, ld, npt2, xstar, xavs, xavs1, ecoul, ecoul1):
+        self.ctx = ctx
+        self.ld = int(ld)
+        self.npt2 = int(npt2)
+        self.xstar = xstar if torch.is_tensor(xstar) else ctx.tensor(np.asarray(xstar, F32))
+        self.xavs = xavs if torch.is_tensor(xavs) else ctx.tensor(np.asarray(xavs, F32))
+        self.xavs1 = xavs1 if torch.is_tensor(xavs1) else ctx.tensor(np.asarray(xavs1, F32))
+        self.ecoul = ecoul if torch.is_tensor(ecoul) else ctx.tensor(np.asarray(ecoul, F32))
+        self.ecoul1 = ecoul1 if torch.is_tensor(ecoul1) else ctx.tensor(np.asarray(ecoul1, F32))
+
+    def struct(self):
+        return _native.Tables(self.ld, self.npt2, _ptr(self.xstar), _ptr(self.xavs), _ptr(self.xavs1),
+                              _ptr(self.ecoul), _ptr(self.ecoul1))
+
+
+class Ledger:
+    """Cell lists + pair-energy ledger resident on the GPU (epsr_ledger)."""
+
+    def __init__(self, ctx: Context, box: DeviceBox):
+        self.ctx = ctx
+        self.box = box
+        h = C.c_void_p()
+        ctx.check(ctx.lib.epsr_ledger_create(ctx.h, box.h, C.byref(h)))
+        self.h = h
+        self.nnbrefmax = 0
+        self.nergpair = 0
+
+    def cells_init(self, nmolcellin, rmax):
+        c = self.box.coords()
+        self.ctx.check(self.ctx.lib.epsr_cells_init(self.ctx.h, self.box.h, self.h, C.byref(c), int(nmolcellin),
+                                                    F32(rmax)))
+
+    def fmolec1(self, tables: DeviceTables, eintra):
+        """Returns (ftest, ftestb) per molecule in FP64."""
+        c = self.box.coords()
+        t = tables.struct()
+        ei = eintra if torch.is_tensor(eintra) else self.ctx.tensor(np.asarray(eintra, F32))
+        f64 = C.c_double()
+        fb64 = C.c_double()
+        nn = C.c_int()
+        ne = C.c_int()
+        self.ctx.check(self.ctx.lib.epsr_fmolec1(self.ctx.h, self.box.h, C.byref(c), C.byref(t), self.h, _ptr(ei),
+                                                 C.byref(f64), C.byref(fb64), None, None, C.byref(nn), C.byref(ne)))
+        self.nnbrefmax = nn.value
+        self.nergpair = ne.value
+        return f64.value, fb64.value
+
+    def export(self):
+        nmol = self.box.nmol
+        ld = max(1, self.nnbrefmax)
+        cnt = np.zeros(nmol, np.int32)
+        mol = np.zeros(nmol * ld, np.int32)
+        ref = np.zeros(nmol * ld, np.int32)
+        erg = np.zeros(max(1, self.nergpair), F32)
+        order = np.zeros(nmol, np.int32)
+        P = lambda a, t=C.c_int: a.ctypes.data_as(C.POINTER(t))
+        self.ctx.check(self.ctx.lib.epsr_ledger_export(self.ctx.h, self.h, ld, P(cnt), P(mol), P(ref),
+                                                       P(erg, C.c_float), P(order)))
+        return dict(nergpairm=cnt, molergpair=mol.reshape(nmol, ld), ergpairref=ref.reshape(nmol, ld),
+                    ergpair=erg[:self.nergpair], order=order)
+
+    def en_change(self, tables: DeviceTables, im, ergnew=None):
+        c = self.box.coords()
+        t = tables.struct()
+        fd = C.c_float()
+        self.ctx.check(self.ctx.lib.epsr_en_change(self.ctx.h, self.box.h, C.byref(c), C.byref(t), self.h, int(im),
+                                                   _ptr(ergnew), C.byref(fd)))
+        return F32(fd.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.epsr_ledger_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def en_mole_pair(ctx: Context, box: DeviceBox, tables: DeviceTables, im, jm):
+    im_t = ctx.tensor(np.asarray(im, np.int32))
+    jm_t = ctx.tensor(np.asarray(jm, np.int32))
+    out = ctx.empty(im_t.numel())
+    c = box.coords()
+    t = tables.struct()
+    ctx.check(ctx.lib.epsr_en_mole_pair(ctx.h, box.h, C.byref(c), C.byref(t), int(im_t.numel()), _ptr(im_t),
+                                        _ptr(jm_t), _ptr(out)))
+    return out
+
+
+class MCState:
+    """Step sizes and counters of the MC cycle (clustercom.wholerot/wholetrans,
+    nwhole/nmovem, nstepreject) plus the Metropolis parameters."""
+
+    def __init__(self, ctx: Context, nmol, nmoltype, simt, stepr=1.0, stepm=1.0, rejrate=0.75, ireset=0,
+                 wholerot=None, wholetrans=None):
+        self.params = _native.MCParams(F32(simt), F32(stepr), F32(stepm), F32(rejrate), int(ireset))
+        self.wholerot = np.ascontiguousarray(np.zeros(nmoltype, F32) if wholerot is None else wholerot, dtype=F32)
+        self.wholetrans = np.ascontiguousarray(np.zeros(nmoltype, F32) if wholetrans is None else wholetrans,
+                                               dtype=F32)
+        self.nwhole = np.zeros(2 * nmoltype, np.int32)
+        self.nmovem = np.zeros(2 * nmoltype, np.int32)
+        self.nstepreject = ctx.zeros(nmol, torch.int32)
+        self.fadds = F32(0.0)
+        self.stats = _native.MCStats()
+
+
+def molfit(ctx: Context, box: DeviceBox, tables: DeviceTables, ledger: Ledger, mc: MCState, ran1_state):
+    """One MC cycle on the GPU; updates coordinates, ledger, mc and ran1_state in place."""
+    c = box.coords()
+    t = tables.struct()
+    fad = C.c_float(float(mc.fadds))
+    P = lambda a, ty=C.c_int: a.ctypes.data_as(C.POINTER(ty))
+    ctx.check(ctx.lib.epsr_molfit(ctx.h, box.h, C.byref(c), C.byref(t), ledger.h, C.byref(mc.params),
+                                  P(mc.wholerot, C.c_float), P(mc.wholetrans, C.c_float), P(mc.nwhole),
+                                  P(mc.nmovem), _ptr(mc.nstepreject), P(ran1_state, C.c_int32), C.byref(fad),
+                                  C.byref(mc.stats)))
+    mc.fadds = F32(fad.value)
+    return mc

@@ -114,3 +114,13 @@ def make_grid(system, mpz=MPZ):
     q[1:] = F32(system.qstep) * np.arange(system.nq, dtype=F32)
     return Grid(x=x, xstar=xstar, ycoh=ycoh, r2=r2_chain(x, npt1), q=q, npt=npt, npt1=npt1,
                 npt2=npt2, nptmin=nptmin, nptmax=nptmax, nq=system.nq)
+
+
+def sepmax(system, grid):
+    """Neighbour-list range of the refinement loop, in box fractions:
+    xstar(nptmax) + maxmolsize + 3/dmax (src/epsr26/programs/epsr_standalone_rev3.py:1259)."""
+    nat = system.nal - system.naf + 1
+    r = np.sqrt(system.xa.astype(np.float64) ** 2 + system.ya.astype(np.float64) ** 2 +
+                system.za.astype(np.float64) ** 2)
+    maxmolsize = F32(2.0 * r.max()) if nat.max() > 1 else F32(0.0)
+    return F32(F32(grid.xstar[grid.nptmax] + maxmolsize) + F32(F32(3.0) / F32(system.dmax)))

@@ -116,6 +116,79 @@ int epsr_stog(epsr_ctx *ctx, float rho, const float *q, const float *sofq, int l
               const float *r, float *gofr, int ldg, const int *nqmin, const int *nqmax,
               int nr, int ncols, const float *xwt, int mode);
 
+/* ---- potential tables, dev arrays [npairs*ld] (table[ic*ld+iz] = reference
+ * table(iz, ic+1)) and xstar dev [ld]: potential_par.xavs/xavs1 (empirical potential
+ * and gradient), potential_par.ecoul/ecoul1 (reference potential and gradient),
+ * epsr16.xstar; npt2 is the last usable index (src/epsr26/common/potential_par.py:22-86). */
+typedef struct {
+    int ld, npt2;
+    const float *xstar;
+    const float *xavs, *xavs1, *ecoul, *ecoul1;
+} epsr_tables_t;
+
+/* ---- cell lists + pair-energy ledger, device resident ----
+ * epsr_ledger_create allocates the object; epsr_cells_init replaces
+ * cellneighbourlist.initialise_cellneighbourlist(nmolcellin, xadd, yadd, zadd, nmol, rmax)
+ * (src/epsr26/library/cellneighbourlist.py:23-96, makecellneighbourlist :102-194) for a
+ * cubic box, reading the centres from coords (dev).  rmax in box fractions. */
+int epsr_ledger_create(epsr_ctx *ctx, const epsr_box *box, epsr_ledger **out);
+void epsr_ledger_destroy(epsr_ledger *l);
+int epsr_cells_init(epsr_ctx *ctx, const epsr_box *box, epsr_ledger *l, const epsr_coords_t *coords,
+                    int nmolcellin, float rmax);
+
+/* ---- fmolec1: replaces ato_manipulation_rev4.fmolec1(ierr, num_threads, ftest, ftestb,
+ * ftethered, fcoordconstraints) (src/epsr26/library/ato_manipulation_rev4.py:2374-2528;
+ * Fortran :2344) with the ledger routines initialise_ergpair/set_ergpair
+ * (src/epsr26/library/ergpairreference.py:63-126): builds the ledger from the cells of
+ * the last epsr_cells_init.  eintra dev [nmol]: intramolecular energy of each
+ * molecule (molfit.en_mole_intra).  Outputs (host): ftest and ftestb per molecule
+ * accumulated in FP64 (ftest64, ftestb64) and rounded to float (ftest, ftestb);
+ * nnbrefmax and the number of stored entries (nergpair). */
+int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *coords, const epsr_tables_t *tables,
+                 epsr_ledger *l, const float *eintra, double *ftest64, double *ftestb64, float *ftest,
+                 float *ftestb, int *nnbrefmax, int *nergpair);
+
+/* copy the ledger to host in the reference's layout: nergpairm [nmol],
+ * molergpair/ergpairref [nmol*ldl] (row im, slot k at im*ldl+k), ergpair [nergpair];
+ * also the chain order of the MC cycle (molecules by cell) in order [nmol]. */
+int epsr_ledger_export(epsr_ctx *ctx, const epsr_ledger *l, int ldl, int *nergpairm, int *molergpair,
+                       int *ergpairref, float *ergpair, int *order);
+
+/* ---- en_change: replaces molfit.en_change(im) (src/epsr26/library/molfit.py:805-919)
+ * after make_neighbour_list_from_mols(im) (ergpairreference.py:165-188): energy change of
+ * molecule im at its current coordinates against the ledger.  ergnew (dev, optional)
+ * receives the ledger-slot energies (slot 0 = stored self energy). */
+int epsr_en_change(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *coords, const epsr_tables_t *tables,
+                   const epsr_ledger *l, int im, float *ergnew, float *fdiff);
+
+/* ---- en_mole_pair: replaces molfit.en_mole_pair(im, jm) (molfit.py:1527-1578); npair
+ * pairs (dev im[npair], jm[npair]) evaluated at once into energy (dev [npair]). */
+int epsr_en_mole_pair(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *coords, const epsr_tables_t *tables,
+                      int npair, const int *im, const int *jm, float *energy);
+
+/* ---- molfit: replaces molfit.molfit(fadds, faddbs, nisum, intra, rotfreq, ireset)
+ * (src/epsr26/library/molfit.py:51-541) for the whole-molecule branch (nisum < intra,
+ * no internal rotational groups, no tethered molecules): one MC cycle over the cells,
+ * rotatm + movem (:671-759) trial moves, en_change, Metropolis, ledger write-back and the
+ * step-size adaptation.  Host arrays: wholerot/wholetrans [nmoltype] (in/out),
+ * nwhole/nmovem [2*nmoltype] (out: attempts, rejections), nstepreject dev [nmol]
+ * (in/out).  fadds in/out.  Trial moves are evaluated in parallel for every molecule
+ * and both states of each earlier neighbour; the serial Metropolis chain then resolves
+ * them in the reference's order (bit-identical to a correctly rounded libm). */
+typedef struct {
+    float simt, stepr, stepm, rejrate;
+    int ireset;
+} epsr_mc_params_t;
+
+typedef struct {
+    int moves, accepted, exact_resolutions, retries;
+    double ms_pairs, ms_resolve;
+} epsr_mc_stats_t;
+
+int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *coords, const epsr_tables_t *tables,
+                epsr_ledger *l, const epsr_mc_params_t *p, float *wholerot, float *wholetrans, int *nwhole,
+                int *nmovem, int *nstepreject, int32_t *ran1_state, float *fadds, epsr_mc_stats_t *stats);
+
 /* ---- foqols: the r -> Q transform matrix built by inita
  * (src/epsr26/programs/epsr_standalone_rev3.py:3025-3071, j1qr :3319-3329, pofr :3389-3399).
  * q dev [nq+1], x dev [npt1+1], r2 dev [npt1+1] (bin edges of the serial r1/r2

@@ -110,3 +110,43 @@ def oracle_atogen(name, ngrsamples=0, idum=-12345, flavour="glibc", rminex=None,
                                      po.Pi(irefpcof), po.Pf(rminex), ngrsamples, C.byref(rng), po.Pf(wav),
                                      po.Pf(potex), po.Pf(potexmax), po.Pf(cs), None)
     return dict(wav=wav, potex=potex, ngr=ngr, ran1=rng.as_array(), ldw=ldw, csampled=cs)
+
+
+def oracle_cells(name, xadd=None, yadd=None, zadd=None):
+    s, g = system(name), grid(name)
+    cl = po.Cells()
+    xa = np.ascontiguousarray(s.xadd if xadd is None else xadd, dtype=F32)
+    ya = np.ascontiguousarray(s.yadd if yadd is None else yadd, dtype=F32)
+    za = np.ascontiguousarray(s.zadd if zadd is None else zadd, dtype=F32)
+    from epsr_python_translation_amd import tables as T
+    po.lib().ora_cells_init(C.byref(cl), s.nmolcell, po.Pf(xa), po.Pf(ya), po.Pf(za), s.nmol, T.sepmax(s, g))
+    return cl
+
+
+def ledger_dict(led, nmol):
+    M = led.nnbrefmax
+    cnt = np.ctypeslib.as_array(led.nergpairm, shape=(nmol,)).copy()
+    mol = np.ctypeslib.as_array(led.molergpair, shape=(nmol * M,)).reshape(nmol, M).copy()
+    ref = np.ctypeslib.as_array(led.ergpairref, shape=(nmol * M,)).reshape(nmol, M).copy()
+    erg = np.ctypeslib.as_array(led.ergpair, shape=(max(1, led.nergpair),))[:led.nergpair].copy()
+    return dict(nergpairm=cnt, molergpair=mol, ergpairref=ref, ergpair=erg, nnbrefmax=M)
+
+
+def oracle_fmolec1(name, flavour="glibc", ob=None, cells=None):
+    s = system(name)
+    t = oracle_tables(name)
+    ob = ob or po.OracleBox(s)
+    cl = cells or oracle_cells(name, ob.xadd, ob.yadd, ob.zadd)
+    led = po.Ledger()
+    ft, fb = C.c_float(), C.c_float()
+    f64 = C.c_double()
+    tt = ctables(t)
+    ei = np.ascontiguousarray(s.eintra, dtype=F32)
+    po.lib(flavour).ora_fmolec1(C.byref(ob.box), C.byref(ob.centres), C.byref(tt), C.byref(cl), po.Pf(ei),
+                                C.byref(led), C.byref(ft), C.byref(fb), C.byref(f64))
+    return dict(ledger=led, cells=cl, ob=ob, tables=t, ctab=tt, ftest=ft.value, ftestb=fb.value, ftest64=f64.value)
+
+
+def initial_steps(s):
+    """Initial whole-molecule rotation (fraction of pi) and translation (box fraction) steps."""
+    return (np.full(s.nmoltype, 0.05, F32), np.full(s.nmoltype, F32(0.1) / F32(s.dmax), F32))

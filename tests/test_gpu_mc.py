@@ -1,0 +1,174 @@
+"""Ledger (fmolec1), en_change, en_mole_pair and the MC cycle (molfit) on the GPU against
+the oracle.
+
+The GPU evaluates cosf/expf correctly rounded; the oracle's 'cr' build does the same, so
+ledgers, coordinates, counters, step sizes and the ran1 state must agree bit for bit.
+Against the glibc build (the reference's own libm calls) the chains agree until the first
+transcendental that glibc rounds differently; that comparison is reported."""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import pyoracle as po
+from tests import helpers as H
+
+F32 = np.float32
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from epsr_python_translation_amd import device as D
+    return D.Context(0)
+
+
+def _gpu_setup(ctx, name, coords=None):
+    from epsr_python_translation_amd import device as D
+    from epsr_python_translation_amd import tables as T
+    s, g = H.system(name), H.grid(name)
+    t = H.oracle_tables(name)
+    box = D.DeviceBox.from_system(ctx, s)
+    if coords:
+        for k, v in coords.items():
+            getattr(box, k).copy_(torch.as_tensor(v))
+    tab = D.DeviceTables(ctx, t["ld"], t["npt2"], t["xstar"], t["xavs"], t["xavs1"], t["ecoul"], t["ecoul1"])
+    led = D.Ledger(ctx, box)
+    led.cells_init(s.nmolcell, T.sepmax(s, g))
+    f64, fb64 = led.fmolec1(tab, s.eintra)
+    return box, tab, led, f64, fb64
+
+
+@pytest.mark.parametrize("name", ["tiny_12_6_2", "c0", "c1"])
+def test_fmolec1_ledger_bit_exact(ctx, name):
+    print(f"fmolec1 parity {name}", flush=True)
+    s = H.system(name)
+    ref = H.oracle_fmolec1(name, "cr")
+    rd = H.ledger_dict(ref["ledger"], s.nmol)
+    box, tab, led, f64, fb64 = _gpu_setup(ctx, name)
+    gd = led.export()
+    assert led.nnbrefmax == rd["nnbrefmax"]
+    np.testing.assert_array_equal(gd["nergpairm"], rd["nergpairm"])
+    for m in range(s.nmol):
+        n = rd["nergpairm"][m]
+        assert np.array_equal(gd["molergpair"][m, :n], rd["molergpair"][m, :n]), m
+        assert np.array_equal(gd["ergpairref"][m, :n], rd["ergpairref"][m, :n]), m
+    assert np.array_equal(gd["ergpair"].view(np.int32), rd["ergpair"].view(np.int32))
+    # the chain order equals the oracle's cell order
+    cl = ref["cells"]
+    cnt = np.ctypeslib.as_array(cl.nmolsincell, shape=(cl.ncell,))
+    mic = np.ctypeslib.as_array(cl.molsincell, shape=(cl.ncell * cl.maxcellno,)).reshape(cl.ncell, cl.maxcellno)
+    order = np.concatenate([mic[c, :cnt[c]] for c in range(cl.ncell)])
+    np.testing.assert_array_equal(gd["order"], order)
+    # totals: FP64 on the GPU against the FP64 sum of the same entries; the reference's
+    # float32 running sum is within its own rounding of it
+    assert abs(f64 - ref["ftest64"]) <= 1e-12 * max(1.0, abs(ref["ftest64"]))
+    rel32 = abs(f64 - ref["ftest"]) / max(1e-30, abs(f64))
+    print(f"fmolec1 {name}: ftest fp64 {f64:.9g}, reference fp32 {ref['ftest']:.9g} (rel {rel32:.2e})")
+    assert rel32 < 1e-4
+    assert abs(fb64 - float(np.sum(s.eintra, dtype=np.float64)) / s.nmol) < 1e-12
+
+
+def test_en_change_and_en_mole_pair_bit_exact(ctx):
+    from epsr_python_translation_amd import device as D
+    name = "c0"
+    s = H.system(name)
+    ref = H.oracle_fmolec1(name, "cr")
+    box, tab, led, _, _ = _gpu_setup(ctx, name)
+    L = po.lib("cr")
+    ob = ref["ob"]
+    ergnew_ref = np.zeros(max(1, ref["ledger"].nnbrefmax), F32)
+    ergnew_gpu = ctx.zeros(max(1, led.nnbrefmax))
+    nn = C.c_int()
+    rng = np.random.default_rng(7)
+    for im in rng.choice(s.nmol, 12, replace=False):
+        im = int(im)
+        dx = F32(rng.uniform(-0.01, 0.01))
+        ob.xadd[im] = F32(ob.xadd[im] + dx)
+        box.xadd[im] = float(ob.xadd[im])
+        fr = L.ora_en_change(C.byref(ob.box), C.byref(ob.centres), C.byref(ref["ctab"]), C.byref(ref["ledger"]), im,
+                             po.Pf(ergnew_ref), C.byref(nn))
+        fg = led.en_change(tab, im, ergnew_gpu)
+        torch.cuda.synchronize()
+        assert F32(fr).view(np.int32) == F32(fg).view(np.int32), (im, fr, fg)
+        eg = ergnew_gpu.cpu().numpy()[:nn.value]
+        assert np.array_equal(eg.view(np.int32), ergnew_ref[:nn.value].view(np.int32))
+    ii = rng.integers(0, s.nmol, 64)
+    jj = rng.integers(0, s.nmol, 64)
+    ok = ii != jj
+    ii, jj = ii[ok], jj[ok]
+    got = D.en_mole_pair(ctx, box, tab, ii, jj).cpu().numpy()
+    want = np.array([L.ora_en_mole_pair(C.byref(ob.box), C.byref(ob.centres), C.byref(ref["ctab"]), int(a), int(b))
+                     for a, b in zip(ii, jj)], F32)
+    assert np.array_equal(got.view(np.int32), want.view(np.int32))
+
+
+def _run_oracle_molfit(name, flavour, nsweep, idum):
+    s = H.system(name)
+    ref = H.oracle_fmolec1(name, flavour)
+    ob = ref["ob"]
+    wr, wt = H.initial_steps(s)
+    nwh = np.zeros(2 * s.nmoltype, np.int32)
+    nmv = np.zeros(2 * s.nmoltype, np.int32)
+    nsr = np.zeros(s.nmol, np.int32)
+    mc = po.MCState(po.Pf(wr), po.Pf(wt), po.Pi(nwh), po.Pi(nmv), po.Pi(nsr), s.simt, F32(1), F32(1), F32(0.75), 0)
+    rng = po.Ran1.seeded(idum)
+    fadds = C.c_float(0.0)
+    nacc = C.c_int()
+    for _ in range(nsweep):
+        po.lib(flavour).ora_molfit(C.byref(ob.box), po.Pf(ob.xa), po.Pf(ob.ya), po.Pf(ob.za), C.byref(ob.centres),
+                                   C.byref(ref["ctab"]), C.byref(ref["cells"]), C.byref(ref["ledger"]), C.byref(mc),
+                                   C.byref(rng), C.byref(fadds), C.byref(nacc))
+    return dict(ob=ob, ledger=H.ledger_dict(ref["ledger"], s.nmol), wr=wr, wt=wt, nwhole=nwh, nmovem=nmv, nsr=nsr,
+                ran1=rng.as_array(), fadds=F32(fadds.value))
+
+
+@pytest.mark.parametrize("name,nsweep", [("tiny_13_12_3", 3), ("c0", 2), ("c1", 1)])
+def test_molfit_bit_exact_with_correctly_rounded_libm(ctx, name, nsweep):
+    from epsr_python_translation_amd import device as D
+    print(f"molfit parity {name} sweeps={nsweep}", flush=True)
+    s = H.system(name)
+    idum = -2718
+    ref = _run_oracle_molfit(name, "cr", nsweep, idum)
+    box, tab, led, _, _ = _gpu_setup(ctx, name)
+    wr, wt = H.initial_steps(s)
+    mc = D.MCState(ctx, s.nmol, s.nmoltype, s.simt, wholerot=wr, wholetrans=wt)
+    st = D.ran1_state_array(idum)
+    for _ in range(nsweep):
+        D.molfit(ctx, box, tab, led, mc, st)
+        print(f"  sweep: accepted {mc.stats.accepted}/{mc.stats.moves}", flush=True)
+    torch.cuda.synchronize()
+    co = box.get_coords()
+    ob = ref["ob"]
+    for k in ("xa", "ya", "za", "xadd", "yadd", "zadd"):
+        assert np.array_equal(co[k].view(np.int32), getattr(ob, k).view(np.int32)), k
+    np.testing.assert_array_equal(st, ref["ran1"])
+    assert np.array_equal(mc.wholerot.view(np.int32), ref["wr"].view(np.int32))
+    assert np.array_equal(mc.wholetrans.view(np.int32), ref["wt"].view(np.int32))
+    np.testing.assert_array_equal(mc.nwhole, ref["nwhole"])
+    np.testing.assert_array_equal(mc.nmovem, ref["nmovem"])
+    np.testing.assert_array_equal(mc.nstepreject.cpu().numpy(), ref["nsr"])
+    assert F32(mc.fadds).view(np.int32) == ref["fadds"].view(np.int32)
+    gd = led.export()
+    assert np.array_equal(gd["ergpair"].view(np.int32), ref["ledger"]["ergpair"].view(np.int32))
+
+
+def test_molfit_against_glibc_libm_chain(ctx):
+    """Same chain with the reference's glibc cosf/expf: identical until the first value
+    glibc rounds differently from the correctly rounded result; report the agreement."""
+    from epsr_python_translation_amd import device as D
+    name = "c0"
+    s = H.system(name)
+    ref = _run_oracle_molfit(name, "glibc", 1, -99)
+    box, tab, led, _, _ = _gpu_setup(ctx, name)
+    wr, wt = H.initial_steps(s)
+    mc = D.MCState(ctx, s.nmol, s.nmoltype, s.simt, wholerot=wr, wholetrans=wt)
+    st = D.ran1_state_array(-99)
+    D.molfit(ctx, box, tab, led, mc, st)
+    co = box.get_coords()
+    same = np.mean(co["xadd"] == ref["ob"].xadd)
+    print(f"molfit vs glibc chain: {same * 100:.1f}% of centres identical after one sweep")
+    np.testing.assert_array_equal(st, ref["ran1"])      # RNG consumption does not depend on libm
+    assert same > 0.5
