@@ -119,6 +119,12 @@ SIGNATURES = {
     "epsr_en_mole_pair": (C.c_int, [vp, vp, C.POINTER(Coords), C.POINTER(Tables), C.c_int, vp, vp, vp]),
     "epsr_molfit": (C.c_int, [vp, vp, C.POINTER(Coords), C.POINTER(Tables), vp, C.POINTER(MCParams), fp, fp, ip, ip,
                               vp, i32p, fp, C.POINTER(MCStats)]),
+    "epsr_truncation": (C.c_int, [vp, C.c_int, vp, C.c_float, C.c_float, vp, vp]),
+    "epsr_generate_rp": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_int]),
+    "epsr_generate_ep": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp, C.c_int, ip, C.c_int, fp, C.c_int, vp, vp, vp, vp,
+                                   vp, vp, C.c_int, fp, fp]),
+    "epsr_calc_pwrexpq": (C.c_int, [vp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float, vp,
+                                    vp, C.c_float, ip]),
     "epsr_foqols": (C.c_int, [vp, C.c_int, vp, C.c_int, vp, vp, vp, C.c_float, C.c_float, C.c_float, C.c_float,
                               C.c_float, C.c_float, vp, C.c_int]),
 }

@@ -83,19 +83,53 @@ __global__ void atogen_intra(const float *__restrict__ xa, const float *__restri
 // ja >= lo (lo = nal(mol(ia))+1 when every molecule is an origin, else 0) whose
 // molecule differs from mol(ia).  blockIdx.x: block of 256 origin atoms;
 // blockIdx.y: slice of the partner range.
+//  * bin index: nint(fl(sqrtf(dsq)) / stepcl) is needed exactly.  The hardware square
+//    root (within 1 ulp) times the rounded reciprocal of stepcl is within 4.5 ulp of the
+//    correctly rounded quotient q; NINT(q) can only differ from NINT of the approximation
+//    when q lies within that distance of a half-integer, and those pairs (about 1e-6 * q
+//    of them) take the exact path: correctly rounded sqrtf and division.
+//  * SMALL: every coordinate lies in [-0.74, 0.74] (checked on the device before the
+//    launch), so differences lie in (-1.5, 1.5) and NINT(x) is -1, 0 or 1.
+//  * POTEX: close-contact counting only when some rminex is non-zero.
+__global__ void abs_max_coord(const float4 *__restrict__ P, int n, unsigned *__restrict__ out)
+{
+    __shared__ float s[256];
+    float m = 0.0f;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float4 p = P[i];
+        m = fmaxf(m, fmaxf(fabsf(p.x), fmaxf(fabsf(p.y), fabsf(p.z))));
+    }
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] = fmaxf(s[threadIdx.x], s[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(out, __float_as_uint(s[0]));   // non-negative floats order as uints
+}
+
+__device__ __forceinline__ float mic_small(float x)
+{
+    float a = x >= 0.5f ? 1.0f : 0.0f;
+    float b = x <= -0.5f ? 1.0f : 0.0f;
+    return x - (a - b);
+}
+
+template <bool POTEX>
 __global__ void __launch_bounds__(ATOGEN_BLOCK)
-atogen_inter(const float4 *__restrict__ P, const int *__restrict__ amol, const int *__restrict__ nal,
-             const int *__restrict__ oatoms, int norigin, int full, int natom,
-             const int *__restrict__ npairref_g, int ntype, int npairs, int npt1, int ldw,
-             float stepcl, float dmax, const float *__restrict__ rminex_abs, int slice,
-             uint32_t *__restrict__ counts, uint32_t *__restrict__ potex_counts)
+atogen_inter_fast(const float4 *__restrict__ P, const int *__restrict__ amol, const int *__restrict__ nal,
+                  const int *__restrict__ oatoms, int norigin, int full, int natom,
+                  const int *__restrict__ npairref_g, int ntype, int npairs, int npt1, int ldw, float stepcl,
+                  float inv_stepcl, float dmax, const float *__restrict__ rminex_abs,
+                  const unsigned *__restrict__ maxcoord, uint32_t *__restrict__ counts,
+                  uint32_t *__restrict__ potex_counts)
 {
     extern __shared__ uint32_t smem[];
     const int ldh = npt1 + 1;
-    uint32_t *hist = smem;                                    // [npairs*ldh]
-    uint32_t *spot = hist + (size_t)npairs * ldh;             // [npairs]
-    int *snp = (int *)(spot + npairs);                        // [ntype*ntype]
-    float *srm = (float *)(snp + ntype * ntype);              // [npairs]
+    uint32_t *hist = smem;
+    uint32_t *spot = hist + (size_t)npairs * ldh;
+    int *snp = (int *)(spot + npairs);
+    float *srm = (float *)(snp + ntype * ntype);
     float4 *tileP = (float4 *)(((uintptr_t)(srm + npairs) + 15) & ~(uintptr_t)15);
     int *tileM = (int *)(tileP + ATOGEN_TILE);
 
@@ -106,31 +140,26 @@ atogen_inter(const float4 *__restrict__ P, const int *__restrict__ amol, const i
         srm[i] = rminex_abs[i];
     }
     for (int i = tid; i < ntype * ntype; i += ATOGEN_BLOCK) snp[i] = npairref_g[i];
+    const bool small = __uint_as_float(*maxcoord) <= 0.74f;
 
     const int oi = blockIdx.x * ATOGEN_BLOCK + tid;
     const bool active = oi < norigin;
     float4 p1 = make_float4(0.f, 0.f, 0.f, 0.f);
-    int mymol = -1, lo = natom;
-    int it = 0;
+    int mymol = -1, lo = natom, rowoff = 0;
     if (active) {
         int ia = oatoms[oi];
         p1 = P[ia];
         mymol = amol[ia];
         lo = full ? nal[mymol] + 1 : 0;
-        it = __float_as_int(p1.w);
+        rowoff = __float_as_int(p1.w) * ntype;
     }
-    // partner range handled by this workgroup
-    int blo = natom;
-    {
-        // smallest lo of the block (full mode): the block's first origin atom has it
-        int first = blockIdx.x * ATOGEN_BLOCK;
-        if (full) blo = nal[amol[oatoms[first]]] + 1;
-        else blo = 0;
-    }
+    const int first = blockIdx.x * ATOGEN_BLOCK;
+    const int blo = full ? nal[amol[oatoms[first]]] + 1 : 0;
     const int span = natom - blo;
     const int per = (span + gridDim.y - 1) / gridDim.y;
     const int p0 = blo + blockIdx.y * per;
     const int p1e = min(natom, p0 + per);
+    const float amb = 1.2e-6f;    // 10 ulp of q; the approximation is within 3 ulp
     __syncthreads();
 
     for (int t0 = p0; t0 < p1e; t0 += ATOGEN_TILE) {
@@ -142,19 +171,37 @@ atogen_inter(const float4 *__restrict__ P, const int *__restrict__ amol, const i
         }
         __syncthreads();
         if (!active) continue;
-        for (int k = 0; k < n; ++k) {
-            int ja = t0 + k;
+        const int kstart = max(0, lo - t0);
+#pragma unroll 4
+        for (int k = kstart; k < n; ++k) {
             float4 p2 = tileP[k];
-            int m2 = tileM[k];
-            if (ja < lo || m2 == mymol) continue;
-            float xx = dev_mic(p1.x - p2.x);
-            float yy = dev_mic(p1.y - p2.y);
-            float zz = dev_mic(p1.z - p2.z);
-            float dtest = sqrtf(xx * xx + yy * yy + zz * zz);
-            int iz1 = dev_nint(dtest / stepcl) + 2;
-            int icref = snp[it * ntype + __float_as_int(p2.w)];
+            if (tileM[k] == mymol) continue;
+            float xx = p1.x - p2.x, yy = p1.y - p2.y, zz = p1.z - p2.z;
+            if (small) {
+                xx = mic_small(xx);
+                yy = mic_small(yy);
+                zz = mic_small(zz);
+            } else {
+                xx = dev_mic(xx);
+                yy = dev_mic(yy);
+                zz = dev_mic(zz);
+            }
+            float dsq = xx * xx + yy * yy + zz * zz;
+            float qa = __builtin_amdgcn_sqrtf(dsq) * inv_stepcl;
+            float tq = truncf(qa);
+            float fr = qa - tq;
+            int iz1;
+            float dtest = 0.0f;
+            if (fabsf(fr - 0.5f) > amb * qa + 1.0e-30f) {
+                iz1 = (int)tq + (fr > 0.5f ? 1 : 0) + 2;
+                if (POTEX) dtest = sqrtf(dsq);
+            } else {
+                dtest = sqrtf(dsq);
+                iz1 = dev_nint(dtest / stepcl) + 2;
+            }
+            int icref = snp[rowoff + __float_as_int(p2.w)];
             if (iz1 > 1 && iz1 <= npt1) atomicAdd(&hist[icref * ldh + iz1], 1u);
-            if (dtest * dmax < srm[icref]) atomicAdd(&spot[icref], 1u);
+            if (POTEX && dtest * dmax < srm[icref]) atomicAdd(&spot[icref], 1u);
         }
     }
     __syncthreads();
@@ -165,8 +212,9 @@ atogen_inter(const float4 *__restrict__ P, const int *__restrict__ amol, const i
             atomicAdd(&counts[((size_t)npairs + ic) * ldw + iz], v);
         }
     }
-    for (int i = tid; i < npairs; i += ATOGEN_BLOCK)
-        if (spot[i]) atomicAdd(&potex_counts[i], spot[i]);
+    if (POTEX)
+        for (int i = tid; i < npairs; i += ATOGEN_BLOCK)
+            if (spot[i]) atomicAdd(&potex_counts[i], spot[i]);
 }
 
 // normalisation (:3859-3897).  norm[ic] = cprod, norm[npairs+ic] = cprod2 (host
@@ -277,6 +325,7 @@ extern "C" int epsr_atogen(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
     size_t oN = take(sizeof(float) * 2 * npairs);
     size_t oF = take(sizeof(float) * npairs);
     size_t oR = take(sizeof(float) * npairs);
+    size_t oMX = take(sizeof(unsigned));
     char *scr = (char *)ctx_scratch(ctx, off);
     if (!scr) return EPSR_ERR_HIP;
     float4 *dP = (float4 *)(scr + oP);
@@ -284,6 +333,7 @@ extern "C" int epsr_atogen(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
     uint32_t *dC = counts_out ? counts_out : (uint32_t *)(scr + oC);
     uint32_t *dPC = (uint32_t *)(scr + oPC);
     float *dN = (float *)(scr + oN), *dF = (float *)(scr + oF), *dR = (float *)(scr + oR);
+    unsigned *dMX = (unsigned *)(scr + oMX);
 
     // staging of the small host arrays through pinned memory, one copy
     size_t hb = sizeof(int) * (size_t)norigin + sizeof(float) * 4 * npairs;
@@ -337,10 +387,20 @@ extern "C" int epsr_atogen(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
         // partner slices so that the launch has a few workgroups per CU
         int gy = std::max(1, std::min(64, ceil_div(1024, gx)));
         gy = std::min(gy, std::max(1, natom / ATOGEN_TILE));
+        HIP_CHECK(ctx, hipMemsetAsync(dMX, 0, sizeof(unsigned), st));
+        abs_max_coord<<<64, 256, 0, st>>>(dP, natom, dMX);
+        bool potex_on = false;
+        for (int ic = 0; ic < npairs; ++ic) potex_on = potex_on || pf[3 * npairs + ic] > 0.0f;
+        const float inv_stepcl = 1.0f / stepcl;
         timer_begin(ctx, "atogen_inter");
-        hipLaunchKernelGGL(atogen_inter, dim3(gx, gy), dim3(ATOGEN_BLOCK), lds, st, dP, box->d_amol, box->d_nal, dO,
-                           norigin, full ? 1 : 0, natom, box->d_npairref, ntype, npairs, npt1, ldw, stepcl, box->dmax,
-                           dR, 0, dC, dPC);
+        if (potex_on)
+            hipLaunchKernelGGL(atogen_inter_fast<true>, dim3(gx, gy), dim3(ATOGEN_BLOCK), lds, st, dP, box->d_amol,
+                               box->d_nal, dO, norigin, full ? 1 : 0, natom, box->d_npairref, ntype, npairs, npt1, ldw,
+                               stepcl, inv_stepcl, box->dmax, dR, dMX, dC, dPC);
+        else
+            hipLaunchKernelGGL(atogen_inter_fast<false>, dim3(gx, gy), dim3(ATOGEN_BLOCK), lds, st, dP, box->d_amol,
+                               box->d_nal, dO, norigin, full ? 1 : 0, natom, box->d_npairref, ntype, npairs, npt1, ldw,
+                               stepcl, inv_stepcl, box->dmax, dR, dMX, dC, dPC);
         timer_end(ctx, "atogen_inter");
     }
     int total = 2 * npairs * npt1;

@@ -1,0 +1,140 @@
+"""One box's refinement-loop state resident on the GPU, and the steps of the loop.
+
+A :class:`Refinement` holds everything the hot path reads and writes, in HBM:
+coordinates, radial/Q grids, transform matrix, truncation functions, reference and
+empirical potential tables, cell lists + pair-energy ledger, MC step sizes and counters,
+histograms and partial structure factors.  Its methods are the steps of EPSR's
+iteration (src/epsr26/programs/epsr_standalone_rev3.py:1235-1400): fmolec1, molfit
+sweeps, atogen, gensq, stog, generate_rp/generate_ep.  Only the ran1 state and a few
+scalars live on the host.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _native
+from . import device as D
+from . import tables as T
+
+F32 = np.float32
+
+
+class Refinement:
+    def __init__(self, ctx: D.Context, system, idum: int = -1, timing: bool = False):
+        self.ctx = ctx
+        self.s = s = system
+        self.g = g = T.make_grid(s)
+        self.box = D.DeviceBox.from_system(ctx, s)
+        self.ran1 = D.ran1_state_array(idum)
+        # grids on the device
+        self.x = ctx.tensor(g.x)
+        self.xstar = ctx.tensor(g.xstar)
+        self.ycoh = ctx.tensor(g.ycoh)
+        self.q = ctx.tensor(g.q)
+        self.ld = g.ld
+        self.ldw = g.npt1 + 1
+        self.ldq = g.nq + 1
+        npairs = s.npairs
+        self.irefpcof_h = np.arange(npairs, dtype=np.int32)
+        self.irefpcof = ctx.tensor(self.irefpcof_h)
+        self.rminex = ctx.zeros(npairs)
+        self.pcoefp = np.ascontiguousarray(s.pcoefp.astype(F32).copy())
+        # tables
+        z = lambda: ctx.zeros(npairs * self.ld)
+        self.wfac = ctx.zeros(self.ld)
+        self.wfacc = ctx.zeros(self.ld)
+        self.epot, self.ecoul, self.epot1, self.ecoul1 = z(), z(), z(), z()
+        self.xavs, self.xavs1, self.xavtot, self.xavtot1 = z(), z(), z(), z()
+        self.foqols = None
+        self.ldf = g.npt1 + 1
+        self.wav = ctx.zeros(2 * npairs * self.ldw)
+        self.potex = ctx.zeros(npairs)
+        self.potexmax = ctx.zeros(npairs)
+        self.sofq = ctx.zeros(2 * npairs * self.ldq)
+        self.eintra = ctx.tensor(s.eintra.astype(F32))
+        self.ledger = D.Ledger(ctx, self.box)
+        wr = np.full(s.nmoltype, 0.05, F32)
+        wt = np.full(s.nmoltype, F32(0.1) / F32(s.dmax), F32)
+        self.mc = D.MCState(ctx, s.nmol, s.nmoltype, s.simt, rejrate=s.rejrate, wholerot=wr, wholetrans=wt)
+        self.pressfac = F32(1.0)
+
+    # ---------------------------------------------------------------- setup (inita/initpt)
+    def build_tables(self):
+        """truncation functions, foqols, generate_rp and generate_ep (initpt + inita)."""
+        ctx, s, g = self.ctx, self.s, self.g
+        ctx.check(ctx.lib.epsr_truncation(ctx.h, g.npt, D._ptr(self.x), F32(s.rminpt), F32(s.rmaxpt),
+                                          D._ptr(self.wfac), D._ptr(self.wfacc)))
+        self.foqols, self.ldf = D.foqols(ctx, g, s.rho, s.rmaxgr)
+        self.generate_rp()
+        self.generate_ep()
+        return self
+
+    def generate_rp(self):
+        ctx, s, g = self.ctx, self.s, self.g
+        self._rp_in = [ctx.tensor(a.astype(F32)) for a in (s.ecor, s.dcor, s.qcor)]
+        p = _native_rp_params(s, self._rp_in)
+        ctx.check(ctx.lib.epsr_generate_rp(ctx.h, g.npt, g.npt2, D._ptr(self.x), D._ptr(self.xstar),
+                                           D._ptr(self.wfac), D._ptr(self.wfacc), C.byref(p), D._ptr(self.epot),
+                                           D._ptr(self.ecoul), D._ptr(self.epot1), D._ptr(self.ecoul1), self.ld))
+
+    def generate_ep(self, absolute_energy=10.0, erequnit=0.0, ereqstep=0.0):
+        ctx, s, g = self.ctx, self.s, self.g
+        p = EPParams(F32(s.pdmin), F32(s.pdstep), F32(s.rmaxpt), F32(s.psigma2), F32(s.rho), F32(0.0),
+                     F32(s.dmax), F32(absolute_energy), F32(erequnit), F32(ereqstep))
+        pf, ea = C.c_float(), C.c_float()
+        ir = self.irefpcof_h
+        ctx.check(ctx.lib.epsr_generate_ep(ctx.h, g.npt, g.npt2, D._ptr(self.x), D._ptr(self.xstar),
+                                           D._ptr(self.wfac), s.npairs, ir.ctypes.data_as(C.POINTER(C.c_int)),
+                                           s.nterms, self.pcoefp.ctypes.data_as(C.POINTER(C.c_float)),
+                                           s.nterms + 1, C.byref(p), D._ptr(self.ecoul), D._ptr(self.xavs),
+                                           D._ptr(self.xavs1), D._ptr(self.xavtot), D._ptr(self.xavtot1), self.ld,
+                                           C.byref(pf), C.byref(ea)))
+        self.pressfac = F32(pf.value)
+        self.energabs = F32(ea.value)
+
+    @property
+    def tables(self):
+        return D.DeviceTables(self.ctx, self.ld, self.g.npt2, self.xstar, self.xavs, self.xavs1, self.ecoul,
+                              self.ecoul1)
+
+    # ---------------------------------------------------------------- the loop
+    def cells_and_ledger(self):
+        """initialise_cellneighbourlist + fmolec1 (:1259-1281).  Returns (ftest, ftestb)."""
+        self.ledger.cells_init(self.s.nmolcell, T.sepmax(self.s, self.g))
+        return self.ledger.fmolec1(self.tables, self.eintra)
+
+    def molfit(self, tables=None):
+        D.molfit(self.ctx, self.box, tables or self.tables, self.ledger, self.mc, self.ran1)
+        return self.mc
+
+    def atogen(self, ngrsamples=0):
+        _, _, _, ngr = D.atogen(self.ctx, self.box, self.g.npt1, self.ldw, self.ycoh, self.irefpcof, self.rminex,
+                                ngrsamples, self.ran1, wav=self.wav, potex=self.potex, potexmax=self.potexmax)
+        return ngr
+
+    def gensq(self, mode=1):
+        D.gensq(self.ctx, self.g.nq, self.s.npairs, self.g.npt1, self.ldw, self.wav, self.ycoh, self.foqols,
+                self.ldf, self.sofq, self.ldq, mode=mode)
+        return self.sofq
+
+
+class EPParams(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("pdmin", "pdstep", "pdmax", "psigma2", "rho", "rbroad", "dmax",
+                                         "absolute_energy", "erequnit", "ereqstep")]
+
+
+class RPParams(C.Structure):
+    _fields_ = [("npairs", C.c_int), ("ecor", C.c_void_p), ("dcor", C.c_void_p), ("qcor", C.c_void_p),
+                ("qradius_i", C.c_void_p), ("qradius_j", C.c_void_p), ("pcoefex", C.c_void_p),
+                ("rminex", C.c_void_p), ("power", C.c_float), ("refpotfac", C.c_float), ("expecf", C.c_float),
+                ("expecfpwr", C.c_float)]
+
+
+def _native_rp_params(s, tensors, pcoefex=None, rminex=None, qradius_i=None, qradius_j=None):
+    ecor, dcor, qcor = tensors
+    return RPParams(s.npairs, D._ptr(ecor), D._ptr(dcor), D._ptr(qcor), D._ptr(qradius_i), D._ptr(qradius_j),
+                    D._ptr(pcoefex), D._ptr(rminex), F32(s.power), F32(1.0), F32(0.3), F32(0.0))

@@ -189,6 +189,46 @@ int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *coords,
                 epsr_ledger *l, const epsr_mc_params_t *p, float *wholerot, float *wholetrans, int *nwhole,
                 int *nmovem, int *nstepreject, int32_t *ran1_state, float *fadds, epsr_mc_stats_t *stats);
 
+/* ---- truncation functions of initpt (src/epsr26/programs/epsr_standalone_rev3.py:3755-3767):
+ * wfac = truncate(x, rminpt, rmaxpt), wfacc = truncatec(x, rmaxpt)
+ * (src/epsr26/library/ato_manipulation_rev4.py:693-727); x, wfac, wfacc dev [npt+1]. */
+int epsr_truncation(epsr_ctx *ctx, int npt, const float *x, float rminpt, float rmaxpt, float *wfac, float *wfacc);
+
+/* ---- generate_rp: replaces epsr_standalone_rev3.generate_rp() (:5675-5872) for n-6
+ * Lennard-Jones + Coulomb between charge spheres (chsph, ato_manipulation_rev4.py:589-653)
+ * + the exponential repulsive core; per-pair parameters dev [npairs]
+ * (molecom.ecor/dcor/qcor, qradius of the two types, potential_par.pcoefex/rminex). */
+typedef struct {
+    int npairs;
+    const float *ecor, *dcor, *qcor;
+    const float *qradius_i, *qradius_j;   /* NULL: point charges */
+    const float *pcoefex, *rminex;        /* NULL: no repulsive core */
+    float power, refpotfac, expecf, expecfpwr;
+} epsr_rp_params_t;
+int epsr_generate_rp(epsr_ctx *ctx, int npt, int npt2, const float *x, const float *xstar, const float *wfac,
+                     const float *wfacc, const epsr_rp_params_t *p, float *epot, float *ecoul, float *epot1,
+                     float *ecoul1, int ld);
+
+/* ---- generate_ep: replaces epsr_standalone_rev3.generate_ep(changepotfac=False)
+ * (:5875-6023) with calc_pwrexpr_grad (:5229-5310), expand (:4924-4939), abs_energy_ep
+ * (ato_manipulation_rev4.py:4127-4157) and multiply_pot_coeffs (:6493-6510) for the
+ * Poisson basis.  pcoefp host [nref*ldp] (row irefp, coefficient m at irefp*ldp+m,
+ * m = 1..nterms), multiplied in place by the returned pressfac; irefpcof host [npairs];
+ * x, xstar, wfac, ecoul dev; xavs/xavs1/xavtot/xavtot1 dev [npairs*ld] in/out. */
+typedef struct {
+    float pdmin, pdstep, pdmax, psigma2, rho, rbroad, dmax, absolute_energy, erequnit, ereqstep;
+} epsr_ep_params_t;
+int epsr_generate_ep(epsr_ctx *ctx, int npt, int npt2, const float *x, const float *xstar, const float *wfac,
+                     int npairs, const int *irefpcof, int nterms, float *pcoefp, int ldp, const epsr_ep_params_t *p,
+                     const float *ecoul, float *xavs, float *xavs1, float *xavtot, float *xavtot1, int ld,
+                     float *pressfac, float *energabs);
+
+/* ---- calc_pwrexpq: the Q-space Poisson basis of the EP fit
+ * (epsr_standalone_rev3.py:5100-5153); q dev [lpt+1], expr dev [lpt*nterms+1] with
+ * expr[(i-1)*nterms+m] = basis m at Q point i; ncfirst host out. */
+int epsr_calc_pwrexpq(epsr_ctx *ctx, int lpt, int nterms, float dmin, float dstep, float dmax, float sigmab,
+                      float rho, const float *q, float *expr, float rmin, int *ncfirst);
+
 /* ---- foqols: the r -> Q transform matrix built by inita
  * (src/epsr26/programs/epsr_standalone_rev3.py:3025-3071, j1qr :3319-3329, pofr :3389-3399).
  * q dev [nq+1], x dev [npt1+1], r2 dev [npt1+1] (bin edges of the serial r1/r2

@@ -130,9 +130,11 @@ def test_stog(ctx, mode):
 
 @pytest.mark.parametrize("name", ["c0", "c1"])
 def test_foqols_gpu_against_oracle(ctx, name):
-    """GPU foqols (double j1qr differences, correctly rounded expf) vs the oracle's glibc
-    build: element-wise within 2 ulp of the value (1 ulp from the double sin/cos of two
-    libms, 1 from expf)."""
+    """GPU foqols (double j1qr differences, correctly rounded expf/sinf/cosf) vs the
+    oracle's glibc build.  Without resolution broadening the two agree bit for bit; with
+    it, pofr(x) = 3 (sin x - x cos x)/x^3 cancels for small x and one ulp of libm
+    difference grows by ~1/x^2, so the bar is 1e-5 of each Q row's largest element (the
+    scale at which foqols weights the histograms in gensq)."""
     from epsr_python_translation_amd import device as D
     print(f"foqols {name}", flush=True)
     s, g = H.system(name), H.grid(name)
@@ -145,5 +147,10 @@ def test_foqols_gpu_against_oracle(ctx, name):
         ulp = np.spacing(np.maximum(np.abs(a), np.float32(1e-30))).astype(np.float64)
         err = np.abs(a.astype(np.float64) - b) / ulp
         frac_exact = np.mean(a.view(np.int32) == b.view(np.int32))
-        print(f"foqols {name} {kw}: max {err.max():.1f} ulp, bit-identical fraction {frac_exact:.6f}")
-        assert err.max() <= 2.0
+        rowscale = np.abs(a).max(axis=1, keepdims=True).astype(np.float64)
+        rel = (np.abs(a.astype(np.float64) - b) / np.maximum(rowscale, 1e-30)).max()
+        print(f"foqols {name} {kw}: max {err.max():.1f} ulp, max/rowscale {rel:.2e}, "
+              f"bit-identical fraction {frac_exact:.6f}")
+        if not kw:
+            assert frac_exact == 1.0
+        assert rel <= 1e-5
