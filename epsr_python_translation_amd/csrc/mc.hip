@@ -41,6 +41,7 @@ struct epsr_ledger {
     int *d_rowstart = nullptr, *d_cnt = nullptr, *d_order = nullptr, *d_pos = nullptr;
     int *d_partner = nullptr, *d_entry = nullptr, *d_mirror = nullptr, *d_slotmol = nullptr;
     float *d_ergpair = nullptr;
+    int *d_koff = nullptr, *d_kcnt = nullptr;   // rows in chain order: offset and length
     size_t cap_slots = 0, cap_entries = 0;
     bool ledger_ready = false;
     std::vector<int> h_rowstart, h_cnt;
@@ -64,7 +65,8 @@ extern "C" int epsr_ledger_create(epsr_ctx *ctx, const epsr_box *box, epsr_ledge
     l->nmol = box->nmol;
     int rc;
     if ((rc = dev_alloc(ctx, &l->d_order, box->nmol)) || (rc = dev_alloc(ctx, &l->d_pos, box->nmol)) ||
-        (rc = dev_alloc(ctx, &l->d_rowstart, box->nmol + 1)) || (rc = dev_alloc(ctx, &l->d_cnt, box->nmol))) {
+        (rc = dev_alloc(ctx, &l->d_rowstart, box->nmol + 1)) || (rc = dev_alloc(ctx, &l->d_cnt, box->nmol)) ||
+        (rc = dev_alloc(ctx, &l->d_koff, box->nmol + 1)) || (rc = dev_alloc(ctx, &l->d_kcnt, box->nmol))) {
         epsr_ledger_destroy(l);
         return rc;
     }
@@ -78,6 +80,8 @@ extern "C" void epsr_ledger_destroy(epsr_ledger *l)
     hipFree(l->d_rowstart); hipFree(l->d_cnt); hipFree(l->d_order); hipFree(l->d_pos);
     hipFree(l->d_partner); hipFree(l->d_entry); hipFree(l->d_mirror); hipFree(l->d_slotmol);
     hipFree(l->d_ergpair);
+    hipFree(l->d_koff);
+    hipFree(l->d_kcnt);
     delete l;
 }
 
@@ -704,6 +708,15 @@ extern "C" int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coord
     // the neighbour cells, which is nnbrefmax, so the cap can never bind
     ARG_CHECK(ctx, maxcnt <= l->nnbrefmax, "epsr_fmolec1: a ledger list of %d exceeds nnbrefmax %d", maxcnt,
               l->nnbrefmax);
+    {
+        std::vector<int> koff(nmol + 1, 0), kcnt(nmol);
+        for (int k = 0; k < nmol; ++k) {
+            kcnt[k] = l->h_cnt[l->order[k]];
+            koff[k + 1] = koff[k] + kcnt[k];
+        }
+        HIP_CHECK(ctx, hipMemcpy(l->d_koff, koff.data(), sizeof(int) * (nmol + 1), hipMemcpyHostToDevice));
+        HIP_CHECK(ctx, hipMemcpy(l->d_kcnt, kcnt.data(), sizeof(int) * nmol, hipMemcpyHostToDevice));
+    }
     l->nself = nself;
     l->npairent = npair;
     l->nslots = nrec;
@@ -1040,6 +1053,263 @@ __global__ void mc_commit_ledger(int nslots, const int *__restrict__ slotmol, co
     else if (wn) ergpair[entry[i]] = A[mi];
 }
 
+// ------------------------------------------------------------------ fast resolution
+// The reference decides move k from fdiff_k, the float32 sequential sum over its slots
+// s >= 1 of (+ergnew_s, -ergold_s).  Every slot has two variants: partner not yet moved
+// (or rejected) and partner accepted earlier in the chain.  Let e0_s = A - L (first
+// variant) and delta_s = (Bv - L') - e0_s with L' the ledger value the partner's move
+// wrote.  Then, in exact arithmetic, fdiff_k = base_k + sum of delta_s over earlier
+// accepted partners, with base_k = sum of e0_s.  A move is decided from this value
+// when the decision is the same for every float32 result within the a-priori rounding
+// bound of the reference's sequential sum; otherwise the exact sequential float32 sum
+// is formed and used (same result as the reference in every case).
+//   bound_k = u * (1 + 1e-3) * sum_l |t_l| (N - l + 1)  (recursive summation, the
+//             largest variant of each term t_l, N terms)
+//           + u * D_k * (npb_k + 2)                       (float32 pushes of the deltas)
+// with u = 2^-24, D_k = sum |delta_s| and npb_k the number of earlier partners.
+
+// one wave per molecule: per-move base, bound and the chain position's push data
+__global__ void __launch_bounds__(64)
+mc_prepare(int nmol, const int *__restrict__ pos, const int *__restrict__ rowstart, const int *__restrict__ cnt,
+           const int *__restrict__ partner, const int *__restrict__ mirror, const float *__restrict__ A,
+           const float *__restrict__ Bv, const float *__restrict__ L, float *__restrict__ delta,
+           double *__restrict__ base_k, double *__restrict__ bound_k)
+{
+    const int m = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int row = rowstart[m], c = cnt[m];
+    const int N = 2 * (c - 1);
+    const int pm = pos[m];
+    double b = 0.0, w = 0.0, dsum = 0.0;
+    int npb = 0;
+    for (int s = 1 + lane; s < c; s += 64) {
+        int i = row + s;
+        int n = partner[i];
+        bool before = pos[n] < pm;
+        float a = A[i], l = L[i];
+        double e0 = (double)a - (double)l;
+        float maxnew = fabsf(a), maxold = fabsf(l);
+        float dl = 0.0f;
+        if (before) {
+            int mi = mirror[i];
+            float lw = mi != rowstart[n] ? A[mi] : l;
+            float bv = Bv[i];
+            double e1 = (double)bv - (double)lw;
+            dl = (float)(e1 - e0);
+            maxnew = fmaxf(maxnew, fabsf(bv));
+            maxold = fmaxf(maxold, fabsf(lw));
+            dsum += fabs((double)dl);
+            npb++;
+        }
+        delta[i] = dl;
+        b += e0;
+        w += (double)maxnew * (double)(N - 2 * s + 2) + (double)maxold * (double)(N - 2 * s + 1);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        b += __shfl_down(b, off);
+        w += __shfl_down(w, off);
+        dsum += __shfl_down(dsum, off);
+        npb += __shfl_down(npb, off);
+    }
+    if (lane == 0) {
+        const double u = 5.9604644775390625e-08;   // 2^-24
+        base_k[pm] = b;
+        bound_k[pm] = u * (1.001 * w + dsum * (double)(npb + 2)) + 1e-12 * fabs(b) + 1e-30;
+    }
+}
+
+// push lists in chain order: for slot i' of the molecule at chain position k, the
+// later partner's chain position (or -1) and the delta of the partner's slot
+__global__ void mc_push_lists(int nslots, const int *__restrict__ slotmol, const int *__restrict__ rowstart,
+                              const int *__restrict__ partner, const int *__restrict__ mirror,
+                              const int *__restrict__ pos, const int *__restrict__ koff, const float *__restrict__ delta,
+                              int *__restrict__ kpt, float *__restrict__ kpv)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nslots) return;
+    int n = slotmol[i];
+    int pn = pos[n];
+    int o = koff[pn] + (i - rowstart[n]);
+    int tgt = -1;
+    float v = 0.0f;
+    int mi = mirror[i];
+    if (mi >= 0) {
+        int m = partner[i];
+        int pm = pos[m];
+        if (pm > pn && mi != rowstart[m]) {
+            tgt = pm;
+            v = delta[mi];
+        }
+    }
+    kpt[o] = tgt;
+    kpv[o] = v;
+}
+
+__device__ __forceinline__ float etest_of(float ediff)
+{
+    if (ediff <= 0.0f) return 10.0f;
+    if (ediff < 20.0f) return (float)exp(-(double)ediff);
+    return 0.0f;
+}
+
+// float32 rounding towards -inf / +inf of a double
+__device__ __forceinline__ float f_down(double x)
+{
+    float f = (float)x;
+    return (double)f > x ? nextafterf(f, -INFINITY) : f;
+}
+__device__ __forceinline__ float f_up(double x)
+{
+    float f = (float)x;
+    return (double)f < x ? nextafterf(f, INFINITY) : f;
+}
+
+#define RES_BLOCK 256
+#define RES_J 2
+#define RES_D 8
+
+// the chain: one workgroup, a software pipeline RES_D moves deep for the push rows
+__global__ void __launch_bounds__(RES_BLOCK)
+mc_resolve_fast(int nmol, const int *__restrict__ order, const int *__restrict__ pos,
+                const int *__restrict__ rowstart, const int *__restrict__ cnt, const int *__restrict__ partner,
+                const int *__restrict__ mirror, const float *__restrict__ A, const float *__restrict__ Bv,
+                const float *__restrict__ L, const int *__restrict__ koff, const int *__restrict__ kcnt,
+                const int *__restrict__ kpt, const float *__restrict__ kpv, const double *__restrict__ base_k,
+                const double *__restrict__ bound_k, const float *__restrict__ stream, const int *__restrict__ moff,
+                float temp, unsigned *__restrict__ acc_out, int *__restrict__ nexact_out)
+{
+    extern __shared__ unsigned sm[];
+    const int nw = (nmol + 31) >> 5;
+    unsigned *bits = sm;
+    float *accde = (float *)(bits + nw);
+    __shared__ int sdec;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < nw; i += RES_BLOCK) bits[i] = 0u;
+    for (int i = tid; i < nmol; i += RES_BLOCK) accde[i] = 0.0f;
+    int nexact = 0;
+
+    int s_off[RES_D], s_c[RES_D];
+    double s_base[RES_D], s_bnd[RES_D];
+    float s_r[RES_D];
+    int s_pt[RES_D][RES_J];
+    float s_pv[RES_D][RES_J];
+    auto load = [&](int d, int k) {
+        int off = koff[k], c = kcnt[k];
+        s_off[d] = off;
+        s_c[d] = c;
+        s_base[d] = base_k[k];
+        s_bnd[d] = bound_k[k];
+        s_r[d] = stream[moff[k]];
+#pragma unroll
+        for (int j = 0; j < RES_J; ++j) {
+            int s = tid + j * RES_BLOCK;
+            s_pt[d][j] = s < c ? kpt[off + s] : -1;
+            s_pv[d][j] = s < c ? kpv[off + s] : 0.0f;
+        }
+    };
+#pragma unroll
+    for (int d = 0; d < RES_D; ++d)
+        if (d < nmol) load(d, d);
+    __syncthreads();
+    for (int k0 = 0; k0 < nmol; k0 += RES_D) {
+#pragma unroll
+        for (int d = 0; d < RES_D; ++d) {
+            const int k = k0 + d;
+            if (k >= nmol) break;
+            // interval of fdiff, then of ediff = fl(fl(fdiff + 0) * temp), then of etest
+            double v = s_base[d] + (double)accde[k];
+            double bnd = s_bnd[d];
+            float flo = f_down(v - bnd), fhi = f_up(v + bnd);
+            float elo = flo * temp, ehi = fhi * temp;
+            float tlo = etest_of(ehi), thi = etest_of(elo);
+            float r = s_r[d];
+            int dec;   // 1 accept, 0 reject, -1 undecided
+            if (r < tlo) dec = 1;
+            else if (r >= thi) dec = 0;
+            else dec = -1;
+            if (dec < 0) {
+                // exact: the reference's sequential float32 sum with the earlier decisions
+                if (tid == 0) {
+                    const int m = order[k];
+                    const int row = rowstart[m], c = cnt[m];
+                    float fdiff = 0.0f;
+                    for (int s = 1; s < c; ++s) {
+                        int i = row + s;
+                        int n = partner[i];
+                        int pn = pos[n];
+                        bool a = pn < k && ((bits[pn >> 5] >> (pn & 31)) & 1u);
+                        int mi = mirror[i];
+                        float en = a ? Bv[i] : A[i];
+                        float eo = (a && mi != rowstart[n]) ? A[mi] : L[i];
+                        fdiff = fdiff + en - eo;
+                    }
+                    float ediff = (fdiff + 0.0f) * temp;
+                    sdec = r < etest_of(ediff) ? 1 : 0;
+                }
+                __syncthreads();
+                dec = sdec;
+                nexact++;
+            }
+            if (dec) {
+                if (tid == 0) bits[k >> 5] |= 1u << (k & 31);
+#pragma unroll
+                for (int j = 0; j < RES_J; ++j) {
+                    int t = s_pt[d][j];
+                    if (t >= 0) atomicAdd(&accde[t], s_pv[d][j]);
+                }
+            }
+            __syncthreads();
+            if (k + RES_D < nmol) load(d, k + RES_D);
+        }
+    }
+    for (int i = tid; i < nw; i += RES_BLOCK) acc_out[i] = bits[i];
+    if (tid == 0) *nexact_out = nexact;
+}
+
+// exact fdiff of every accepted move (for fadd), one lane per chain position
+__global__ void mc_fdiff_accepted(int nmol, const int *__restrict__ order, const int *__restrict__ pos,
+                                  const int *__restrict__ rowstart, const int *__restrict__ cnt,
+                                  const int *__restrict__ partner, const int *__restrict__ mirror,
+                                  const float *__restrict__ A, const float *__restrict__ Bv,
+                                  const float *__restrict__ L, const unsigned *__restrict__ bits,
+                                  float *__restrict__ fdiff_out)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nmol) return;
+    if (!((bits[k >> 5] >> (k & 31)) & 1u)) return;
+    const int m = order[k];
+    const int row = rowstart[m], c = cnt[m];
+    float fdiff = 0.0f;
+    for (int s = 1; s < c; ++s) {
+        int i = row + s;
+        int n = partner[i];
+        int pn = pos[n];
+        bool a = pn < k && ((bits[pn >> 5] >> (pn & 31)) & 1u);
+        int mi = mirror[i];
+        float en = a ? Bv[i] : A[i];
+        float eo = (a && mi != rowstart[n]) ? A[mi] : L[i];
+        fdiff = fdiff + en - eo;
+    }
+    fdiff_out[k] = fdiff;
+}
+
+// fadd = fadd + fdiff over the accepted moves in chain order (molfit.py:405), and the
+// number of accepted moves
+__global__ void mc_fadd(int nmol, const unsigned *__restrict__ bits, const float *__restrict__ fdiff,
+                        float *__restrict__ res)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float fadd = 0.0f;
+    int nacc = 0;
+    for (int k = 0; k < nmol; ++k)
+        if ((bits[k >> 5] >> (k & 31)) & 1u) {
+            fadd = fadd + fdiff[k];
+            nacc++;
+        }
+    res[0] = fadd;
+    res[1] = (float)nacc;
+}
+
 static float adapt_ratio(float rejrate, int att, int rej)
 {
     float rat;
@@ -1125,6 +1395,8 @@ extern "C" int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
         x.take<float>(nmol); x.take<float>(nmol); x.take<float>(nmol);
         x.take<float>(nslots); x.take<float>(nslots); x.take<float>(nslots);
         x.take<unsigned>((nmol + 31) / 32); x.take<float>(nmol); x.take<float>(4); x.take<int>(2 * nmt + 2);
+        x.take<float>(nslots); x.take<double>(nmol); x.take<double>(nmol); x.take<int>(nslots); x.take<float>(nslots);
+        x.take<int>(4);
     };
     layout(q);
     S.base = (char *)ctx_scratch(ctx, q.off);
@@ -1142,6 +1414,11 @@ extern "C" int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
     float *dfdiff = S.take<float>(nmol);
     float *dres = S.take<float>(4);
     int *drej = S.take<int>(2 * nmt + 2);
+    float *ddelta = S.take<float>(nslots);
+    double *dbase = S.take<double>(nmol), *dbound = S.take<double>(nmol);
+    int *dkpt = S.take<int>(nslots);
+    float *dkpv = S.take<float>(nslots);
+    int *dnex = S.take<int>(4);
 
     PreparedTables pt;
     int rc = prepare_tables(ctx, box, tb, dT, dtmp, pt);
@@ -1249,12 +1526,31 @@ extern "C" int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
     timer_end(ctx, "mc_pair");
     int maxc = 1;
     for (int v : l->h_cnt) maxc = std::max(maxc, v);
-    size_t lds = sizeof(unsigned) * ((nmol + 31) / 32) + sizeof(float) * 2 * (size_t)maxc + 64;
-    ARG_CHECK(ctx, lds <= 160 * 1024, "epsr_molfit: %d molecules x %d ledger slots exceed LDS", nmol, maxc);
-    timer_begin(ctx, "mc_resolve");
-    mc_resolve<<<1, 256, lds, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner, l->d_mirror,
-                                    dA, dB, dL, dstream, dmoff, temp, maxc, dacc, dfdiff, dres);
-    timer_end(ctx, "mc_resolve");
+    const size_t lds_fast = sizeof(unsigned) * ((nmol + 31) / 32) + sizeof(float) * (size_t)nmol + 64;
+    const bool fast = maxc <= RES_J * RES_BLOCK && lds_fast <= 150 * 1024;
+    int nexact = nmol;
+    if (fast) {
+        mc_prepare<<<nmol, 64, 0, st>>>(nmol, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner, l->d_mirror, dA, dB, dL,
+                                        ddelta, dbase, dbound);
+        mc_push_lists<<<ceil_div(nslots, 256), 256, 0, st>>>(nslots, l->d_slotmol, l->d_rowstart, l->d_partner,
+                                                             l->d_mirror, l->d_pos, l->d_koff, ddelta, dkpt, dkpv);
+        timer_begin(ctx, "mc_resolve");
+        mc_resolve_fast<<<1, RES_BLOCK, lds_fast, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt,
+                                                        l->d_partner, l->d_mirror, dA, dB, dL, l->d_koff, l->d_kcnt,
+                                                        dkpt, dkpv, dbase, dbound, dstream, dmoff, temp, dacc, dnex);
+        timer_end(ctx, "mc_resolve");
+        mc_fdiff_accepted<<<ceil_div(nmol, 128), 128, 0, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt,
+                                                               l->d_partner, l->d_mirror, dA, dB, dL, dacc, dfdiff);
+        mc_fadd<<<1, 64, 0, st>>>(nmol, dacc, dfdiff, dres);
+        HIP_CHECK(ctx, hipMemcpyAsync(&nexact, dnex, sizeof(int), hipMemcpyDeviceToHost, st));
+    } else {
+        size_t lds = sizeof(unsigned) * ((nmol + 31) / 32) + sizeof(float) * 2 * (size_t)maxc + 64;
+        ARG_CHECK(ctx, lds <= 160 * 1024, "epsr_molfit: %d molecules x %d ledger slots exceed LDS", nmol, maxc);
+        timer_begin(ctx, "mc_resolve");
+        mc_resolve<<<1, 256, lds, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner,
+                                        l->d_mirror, dA, dB, dL, dstream, dmoff, temp, maxc, dacc, dfdiff, dres);
+        timer_end(ctx, "mc_resolve");
+    }
     HIP_CHECK(ctx, hipMemsetAsync(drej, 0, sizeof(int) * 2 * nmt, st));
     mc_commit_ledger<<<ceil_div(nslots, 256), 256, 0, st>>>(nslots, l->d_slotmol, l->d_rowstart, l->d_partner,
                                                             l->d_entry, l->d_mirror, l->d_pos, dacc, dA, dB,
@@ -1296,10 +1592,11 @@ extern "C" int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
     if (stats) {
         stats->moves = nmol;
         stats->accepted = (int)hres[1];
-        stats->exact_resolutions = nmol;
+        stats->exact_resolutions = nexact;
         stats->retries = retries;
         stats->ms_pairs = ctx->timing ? epsr_ctx_last_kernel_ms(ctx, "mc_pair") : -1.0;
         stats->ms_resolve = ctx->timing ? epsr_ctx_last_kernel_ms(ctx, "mc_resolve") : -1.0;
     }
     return EPSR_OK;
 }
+

@@ -143,7 +143,7 @@ def test_molfit_bit_exact_with_correctly_rounded_libm(ctx, name, nsweep):
     st = D.ran1_state_array(idum)
     for _ in range(nsweep):
         D.molfit(ctx, box, tab, led, mc, st)
-        print(f"  sweep: accepted {mc.stats.accepted}/{mc.stats.moves}", flush=True)
+        print(f"  sweep: accepted {mc.stats.accepted}/{mc.stats.moves}, exact resolutions {mc.stats.exact_resolutions}", flush=True)
     torch.cuda.synchronize()
     co = box.get_coords()
     ob = ref["ob"]

@@ -139,18 +139,24 @@ def test_foqols_gpu_against_oracle(ctx, name):
     print(f"foqols {name}", flush=True)
     s, g = H.system(name), H.grid(name)
     for kw in ({}, {"fwhm": 0.02, "fwhmq": 0.01, "lorchqwidth": 0.1}):
-        ref, ldf = H.foqols(name, "glibc", **kw)
-        got, _ = D.foqols(ctx, g, s.rho, s.rmaxgr, **kw)
-        got = got.cpu().numpy()
-        a = ref.reshape(g.nq + 1, ldf)[1:, 1:]
-        b = got.reshape(g.nq + 1, ldf)[1:, 1:]
-        ulp = np.spacing(np.maximum(np.abs(a), np.float32(1e-30))).astype(np.float64)
-        err = np.abs(a.astype(np.float64) - b) / ulp
-        frac_exact = np.mean(a.view(np.int32) == b.view(np.int32))
-        rowscale = np.abs(a).max(axis=1, keepdims=True).astype(np.float64)
-        rel = (np.abs(a.astype(np.float64) - b) / np.maximum(rowscale, 1e-30)).max()
-        print(f"foqols {name} {kw}: max {err.max():.1f} ulp, max/rowscale {rel:.2e}, "
-              f"bit-identical fraction {frac_exact:.6f}")
-        if not kw:
-            assert frac_exact == 1.0
-        assert rel <= 1e-5
+        got, ldf = D.foqols(ctx, g, s.rho, s.rmaxgr, **kw)
+        b = got.cpu().numpy().reshape(g.nq + 1, ldf)[1:, 1:]
+        for flavour in ("cr", "glibc"):
+            ref, _ = H.foqols(name, flavour, **kw)
+            a = ref.reshape(g.nq + 1, ldf)[1:, 1:]
+            ulp = np.spacing(np.maximum(np.abs(a), np.float32(1e-30))).astype(np.float64)
+            err = np.abs(a.astype(np.float64) - b) / ulp
+            frac_exact = np.mean(a.view(np.int32) == b.view(np.int32))
+            rowscale = np.abs(a).max(axis=1, keepdims=True).astype(np.float64)
+            rel = (np.abs(a.astype(np.float64) - b) / np.maximum(rowscale, 1e-30)).max()
+            print(f"foqols {name} {kw} vs {flavour}: max {err.max():.1f} ulp, max/rowscale {rel:.2e}, "
+                  f"bit-identical fraction {frac_exact:.6f}")
+            if not kw:
+                assert frac_exact == 1.0
+            elif flavour == "cr":
+                # same correctly rounded functions: differences only from double sin/cos of
+                # two libms inside j1qr and pofr
+                assert rel <= 1e-6 and frac_exact >= 0.99
+            else:
+                # glibc sinf/cosf in pofr(x) = 3 (sin x - x cos x)/x^3: 1 ulp becomes ~1/x^2
+                assert rel <= 1e-4
