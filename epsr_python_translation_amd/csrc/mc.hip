@@ -1164,108 +1164,6 @@ __device__ __forceinline__ float f_up(double x)
     return (double)f < x ? nextafterf(f, INFINITY) : f;
 }
 
-#define RES_BLOCK 256
-#define RES_J 2
-#define RES_D 8
-
-// the chain: one workgroup, a software pipeline RES_D moves deep for the push rows
-__global__ void __launch_bounds__(RES_BLOCK)
-mc_resolve_fast(int nmol, const int *__restrict__ order, const int *__restrict__ pos,
-                const int *__restrict__ rowstart, const int *__restrict__ cnt, const int *__restrict__ partner,
-                const int *__restrict__ mirror, const float *__restrict__ A, const float *__restrict__ Bv,
-                const float *__restrict__ L, const int *__restrict__ koff, const int *__restrict__ kcnt,
-                const int *__restrict__ kpt, const float *__restrict__ kpv, const double *__restrict__ base_k,
-                const double *__restrict__ bound_k, const float *__restrict__ stream, const int *__restrict__ moff,
-                float temp, unsigned *__restrict__ acc_out, int *__restrict__ nexact_out)
-{
-    extern __shared__ unsigned sm[];
-    const int nw = (nmol + 31) >> 5;
-    unsigned *bits = sm;
-    float *accde = (float *)(bits + nw);
-    __shared__ int sdec;
-    const int tid = threadIdx.x;
-    for (int i = tid; i < nw; i += RES_BLOCK) bits[i] = 0u;
-    for (int i = tid; i < nmol; i += RES_BLOCK) accde[i] = 0.0f;
-    int nexact = 0;
-
-    int s_off[RES_D], s_c[RES_D];
-    double s_base[RES_D], s_bnd[RES_D];
-    float s_r[RES_D];
-    int s_pt[RES_D][RES_J];
-    float s_pv[RES_D][RES_J];
-    auto load = [&](int d, int k) {
-        int off = koff[k], c = kcnt[k];
-        s_off[d] = off;
-        s_c[d] = c;
-        s_base[d] = base_k[k];
-        s_bnd[d] = bound_k[k];
-        s_r[d] = stream[moff[k]];
-#pragma unroll
-        for (int j = 0; j < RES_J; ++j) {
-            int s = tid + j * RES_BLOCK;
-            s_pt[d][j] = s < c ? kpt[off + s] : -1;
-            s_pv[d][j] = s < c ? kpv[off + s] : 0.0f;
-        }
-    };
-#pragma unroll
-    for (int d = 0; d < RES_D; ++d)
-        if (d < nmol) load(d, d);
-    __syncthreads();
-    for (int k0 = 0; k0 < nmol; k0 += RES_D) {
-#pragma unroll
-        for (int d = 0; d < RES_D; ++d) {
-            const int k = k0 + d;
-            if (k >= nmol) break;
-            // interval of fdiff, then of ediff = fl(fl(fdiff + 0) * temp), then of etest
-            double v = s_base[d] + (double)accde[k];
-            double bnd = s_bnd[d];
-            float flo = f_down(v - bnd), fhi = f_up(v + bnd);
-            float elo = flo * temp, ehi = fhi * temp;
-            float tlo = etest_of(ehi), thi = etest_of(elo);
-            float r = s_r[d];
-            int dec;   // 1 accept, 0 reject, -1 undecided
-            if (r < tlo) dec = 1;
-            else if (r >= thi) dec = 0;
-            else dec = -1;
-            if (dec < 0) {
-                // exact: the reference's sequential float32 sum with the earlier decisions
-                if (tid == 0) {
-                    const int m = order[k];
-                    const int row = rowstart[m], c = cnt[m];
-                    float fdiff = 0.0f;
-                    for (int s = 1; s < c; ++s) {
-                        int i = row + s;
-                        int n = partner[i];
-                        int pn = pos[n];
-                        bool a = pn < k && ((bits[pn >> 5] >> (pn & 31)) & 1u);
-                        int mi = mirror[i];
-                        float en = a ? Bv[i] : A[i];
-                        float eo = (a && mi != rowstart[n]) ? A[mi] : L[i];
-                        fdiff = fdiff + en - eo;
-                    }
-                    float ediff = (fdiff + 0.0f) * temp;
-                    sdec = r < etest_of(ediff) ? 1 : 0;
-                }
-                __syncthreads();
-                dec = sdec;
-                nexact++;
-            }
-            if (dec) {
-                if (tid == 0) bits[k >> 5] |= 1u << (k & 31);
-#pragma unroll
-                for (int j = 0; j < RES_J; ++j) {
-                    int t = s_pt[d][j];
-                    if (t >= 0) atomicAdd(&accde[t], s_pv[d][j]);
-                }
-            }
-            __syncthreads();
-            if (k + RES_D < nmol) load(d, k + RES_D);
-        }
-    }
-    for (int i = tid; i < nw; i += RES_BLOCK) acc_out[i] = bits[i];
-    if (tid == 0) *nexact_out = nexact;
-}
-
 // exact fdiff of every accepted move (for fadd), one lane per chain position
 __global__ void mc_fdiff_accepted(int nmol, const int *__restrict__ order, const int *__restrict__ pos,
                                   const int *__restrict__ rowstart, const int *__restrict__ cnt,
@@ -1308,6 +1206,124 @@ __global__ void mc_fadd(int nmol, const unsigned *__restrict__ bits, const float
         }
     res[0] = fadd;
     res[1] = (float)nacc;
+}
+
+// ------------------------------------------------------------------ single-wave resolver
+// Same decisions as mc_resolve_fast, one wave64: LDS operations of one wave complete in
+// order, so pushes need no barrier.  The Metropolis test r < RN(exp(-ediff)) is decided
+// in log space: with nlr = -log(r) (exact to 1e-15 in double),
+//   accept for every ediff <= hi  if  hi <= 0  or  (hi < 20 and hi <= nlr - 2^-22)
+//   reject for every ediff >= lo  if  lo >= 20 or (lo > 0 and lo >= nlr + 1e-12)
+// (exp(-x) >= r (1 + 2^-23) gives RN(exp(-x)) > r; exp(-x) <= r gives RN(exp(-x)) <= r).
+// The ediff interval widens (v -/+ bound) * temp by 2^-22 relative to cover the float32
+// rounding of fdiff and of the product.
+#define RW_J 8
+#define RW_D 8
+
+__global__ void mc_nlogr(int nmol, const float *__restrict__ stream, const int *__restrict__ moff,
+                         double *__restrict__ nlr)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nmol) nlr[k] = -log((double)stream[moff[k]]);
+}
+
+__global__ void __launch_bounds__(64)
+mc_resolve_wave(int nmol, const int *__restrict__ order, const int *__restrict__ pos,
+                const int *__restrict__ rowstart, const int *__restrict__ cnt, const int *__restrict__ partner,
+                const int *__restrict__ mirror, const float *__restrict__ A, const float *__restrict__ Bv,
+                const float *__restrict__ L, const int *__restrict__ koff, const int *__restrict__ kcnt,
+                const int *__restrict__ kpt, const float *__restrict__ kpv, const double *__restrict__ base_k,
+                const double *__restrict__ bound_k, const double *__restrict__ nlr_k,
+                const float *__restrict__ stream, const int *__restrict__ moff, float temp,
+                unsigned *__restrict__ acc_out, int *__restrict__ nexact_out)
+{
+    extern __shared__ unsigned sm[];
+    const int nw = (nmol + 31) >> 5;
+    unsigned *bits = sm;
+    float *accde = (float *)(bits + nw);
+    float *term = accde + nmol;            // [2 * 64 * RW_J]
+    const int lane = threadIdx.x;
+    for (int i = lane; i < nw; i += 64) bits[i] = 0u;
+    for (int i = lane; i < nmol; i += 64) accde[i] = 0.0f;
+    int nexact = 0;
+    const double tempd = (double)temp;
+    const double rel = 2.384185791015625e-07;   // 2^-22
+
+    int s_off[RW_D], s_c[RW_D];
+    double s_base[RW_D], s_bnd[RW_D], s_nlr[RW_D];
+    int s_pt[RW_D][RW_J];
+    float s_pv[RW_D][RW_J];
+    auto load = [&](int d, int k) {
+        int off = koff[k], c = kcnt[k];
+        s_off[d] = off;
+        s_c[d] = c;
+        s_base[d] = base_k[k];
+        s_bnd[d] = bound_k[k];
+        s_nlr[d] = nlr_k[k];
+#pragma unroll
+        for (int j = 0; j < RW_J; ++j) {
+            int s = lane + j * 64;
+            s_pt[d][j] = s < c ? kpt[off + s] : -1;
+            s_pv[d][j] = s < c ? kpv[off + s] : 0.0f;
+        }
+    };
+#pragma unroll
+    for (int d = 0; d < RW_D; ++d)
+        if (d < nmol) load(d, d);
+    __syncthreads();
+    for (int k0 = 0; k0 < nmol; k0 += RW_D) {
+#pragma unroll
+        for (int d = 0; d < RW_D; ++d) {
+            const int k = k0 + d;
+            if (k >= nmol) break;
+            double v = s_base[d] + (double)accde[k];
+            double bnd = s_bnd[d];
+            double lo = (v - bnd) * tempd, hi = (v + bnd) * tempd;
+            lo = lo - fabs(lo) * rel - 1e-30;
+            hi = hi + fabs(hi) * rel + 1e-30;
+            double nlr = s_nlr[d];
+            int dec = -1;
+            if (hi <= 0.0 || (hi < 20.0 && hi <= nlr - rel)) dec = 1;
+            else if (lo >= 20.0 || (lo > 0.0 && lo >= nlr + 1e-12)) dec = 0;
+            if (dec < 0) {
+                // exact: gather the selected terms in parallel, sum them in order on lane 0
+                const int m = order[k];
+                const int row = rowstart[m], c = cnt[m];
+                for (int s = 1 + lane; s < c; s += 64) {
+                    int i = row + s;
+                    int n = partner[i];
+                    int pn = pos[n];
+                    bool a = pn < k && ((bits[pn >> 5] >> (pn & 31)) & 1u);
+                    int mi = mirror[i];
+                    term[2 * s] = a ? Bv[i] : A[i];
+                    term[2 * s + 1] = (a && mi != rowstart[n]) ? A[mi] : L[i];
+                }
+                __syncthreads();
+                int dd = 0;
+                if (lane == 0) {
+                    float fdiff = 0.0f;
+                    for (int s = 1; s < c; ++s) fdiff = fdiff + term[2 * s] - term[2 * s + 1];
+                    float ediff = (fdiff + 0.0f) * temp;
+                    dd = stream[moff[k]] < etest_of(ediff) ? 1 : 0;
+                }
+                dec = __shfl(dd, 0);
+                nexact++;
+                __syncthreads();
+            }
+            if (dec) {
+                if (lane == 0) bits[k >> 5] |= 1u << (k & 31);
+#pragma unroll
+                for (int j = 0; j < RW_J; ++j) {
+                    int t = s_pt[d][j];
+                    if (t >= 0) atomicAdd(&accde[t], s_pv[d][j]);
+                }
+            }
+            if (k + RW_D < nmol) load(d, k + RW_D);
+        }
+    }
+    __syncthreads();
+    for (int i = lane; i < nw; i += 64) acc_out[i] = bits[i];
+    if (lane == 0) *nexact_out = nexact;
 }
 
 static float adapt_ratio(float rejrate, int att, int rej)
@@ -1396,7 +1412,7 @@ extern "C" int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
         x.take<float>(nslots); x.take<float>(nslots); x.take<float>(nslots);
         x.take<unsigned>((nmol + 31) / 32); x.take<float>(nmol); x.take<float>(4); x.take<int>(2 * nmt + 2);
         x.take<float>(nslots); x.take<double>(nmol); x.take<double>(nmol); x.take<int>(nslots); x.take<float>(nslots);
-        x.take<int>(4);
+        x.take<int>(4); x.take<double>(nmol);
     };
     layout(q);
     S.base = (char *)ctx_scratch(ctx, q.off);
@@ -1419,6 +1435,7 @@ extern "C" int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
     int *dkpt = S.take<int>(nslots);
     float *dkpv = S.take<float>(nslots);
     int *dnex = S.take<int>(4);
+    double *dnlr = S.take<double>(nmol);
 
     PreparedTables pt;
     int rc = prepare_tables(ctx, box, tb, dT, dtmp, pt);
@@ -1526,18 +1543,20 @@ extern "C" int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
     timer_end(ctx, "mc_pair");
     int maxc = 1;
     for (int v : l->h_cnt) maxc = std::max(maxc, v);
-    const size_t lds_fast = sizeof(unsigned) * ((nmol + 31) / 32) + sizeof(float) * (size_t)nmol + 64;
-    const bool fast = maxc <= RES_J * RES_BLOCK && lds_fast <= 150 * 1024;
+    const size_t lds_fast = sizeof(unsigned) * ((nmol + 31) / 32) + sizeof(float) * (size_t)nmol +
+                            sizeof(float) * 2 * 64 * RW_J + 64;
+    const bool fast = maxc <= RW_J * 64 && lds_fast <= 150 * 1024;
     int nexact = nmol;
     if (fast) {
         mc_prepare<<<nmol, 64, 0, st>>>(nmol, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner, l->d_mirror, dA, dB, dL,
                                         ddelta, dbase, dbound);
         mc_push_lists<<<ceil_div(nslots, 256), 256, 0, st>>>(nslots, l->d_slotmol, l->d_rowstart, l->d_partner,
                                                              l->d_mirror, l->d_pos, l->d_koff, ddelta, dkpt, dkpv);
+        mc_nlogr<<<ceil_div(nmol, 256), 256, 0, st>>>(nmol, dstream, dmoff, dnlr);
         timer_begin(ctx, "mc_resolve");
-        mc_resolve_fast<<<1, RES_BLOCK, lds_fast, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt,
-                                                        l->d_partner, l->d_mirror, dA, dB, dL, l->d_koff, l->d_kcnt,
-                                                        dkpt, dkpv, dbase, dbound, dstream, dmoff, temp, dacc, dnex);
+        mc_resolve_wave<<<1, 64, lds_fast, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner,
+                                                 l->d_mirror, dA, dB, dL, l->d_koff, l->d_kcnt, dkpt, dkpv, dbase,
+                                                 dbound, dnlr, dstream, dmoff, temp, dacc, dnex);
         timer_end(ctx, "mc_resolve");
         mc_fdiff_accepted<<<ceil_div(nmol, 128), 128, 0, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt,
                                                                l->d_partner, l->d_mirror, dA, dB, dL, dacc, dfdiff);
@@ -1599,4 +1618,5 @@ extern "C" int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
     }
     return EPSR_OK;
 }
+
 
