@@ -41,7 +41,7 @@ struct epsr_ledger {
     int *d_rowstart = nullptr, *d_cnt = nullptr, *d_order = nullptr, *d_pos = nullptr;
     int *d_partner = nullptr, *d_entry = nullptr, *d_mirror = nullptr, *d_slotmol = nullptr;
     float *d_ergpair = nullptr;
-    int *d_koff = nullptr, *d_kcnt = nullptr;   // rows in chain order: offset and length
+    int maxpush = 0;   // most later partners a molecule's accepted move updates
     size_t cap_slots = 0, cap_entries = 0;
     bool ledger_ready = false;
     std::vector<int> h_rowstart, h_cnt;
@@ -65,8 +65,7 @@ extern "C" int epsr_ledger_create(epsr_ctx *ctx, const epsr_box *box, epsr_ledge
     l->nmol = box->nmol;
     int rc;
     if ((rc = dev_alloc(ctx, &l->d_order, box->nmol)) || (rc = dev_alloc(ctx, &l->d_pos, box->nmol)) ||
-        (rc = dev_alloc(ctx, &l->d_rowstart, box->nmol + 1)) || (rc = dev_alloc(ctx, &l->d_cnt, box->nmol)) ||
-        (rc = dev_alloc(ctx, &l->d_koff, box->nmol + 1)) || (rc = dev_alloc(ctx, &l->d_kcnt, box->nmol))) {
+        (rc = dev_alloc(ctx, &l->d_rowstart, box->nmol + 1)) || (rc = dev_alloc(ctx, &l->d_cnt, box->nmol))) {
         epsr_ledger_destroy(l);
         return rc;
     }
@@ -80,8 +79,6 @@ extern "C" void epsr_ledger_destroy(epsr_ledger *l)
     hipFree(l->d_rowstart); hipFree(l->d_cnt); hipFree(l->d_order); hipFree(l->d_pos);
     hipFree(l->d_partner); hipFree(l->d_entry); hipFree(l->d_mirror); hipFree(l->d_slotmol);
     hipFree(l->d_ergpair);
-    hipFree(l->d_koff);
-    hipFree(l->d_kcnt);
     delete l;
 }
 
@@ -491,6 +488,39 @@ __global__ void fmolec1_mirror(const int *__restrict__ sval, int nrec, int nself
     mirror[i] = r < nself ? -1 : sortedpos[nself + (((r - nself) >> 1) << 1) + (1 - ((r - nself) & 1))];
 }
 
+// The chain position of the later partner whose energy change an accepted move at chain
+// position k updates through slot i of its list, or -1.  The partner's own slot for the
+// pair (the mirror) counts unless it is that partner's slot 0, which en_change skips;
+// self entries have no mirror.
+__device__ __forceinline__ int push_target(int i, int k, const int *__restrict__ partner,
+                                           const int *__restrict__ mirror, const int *__restrict__ pos,
+                                           const int *__restrict__ rowstart)
+{
+    int mi = mirror[i];
+    if (mi < 0) return -1;
+    int m = partner[i];
+    int pm = pos[m];
+    return (pm > k && mi != rowstart[m]) ? pm : -1;
+}
+
+__global__ void count_pushes(int nslots, const int *__restrict__ slotmol, const int *__restrict__ partner,
+                             const int *__restrict__ mirror, const int *__restrict__ pos,
+                             const int *__restrict__ rowstart, int *__restrict__ npush)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nslots) return;
+    int k = pos[slotmol[i]];
+    if (push_target(i, k, partner, mirror, pos, rowstart) >= 0) atomicAdd(&npush[k], 1);
+}
+
+__global__ void max_int(const int *__restrict__ a, int n, int *__restrict__ out)
+{
+    int v = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v = max(v, a[i]);
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_down(v, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, v);
+}
+
 __global__ void flag_nonzero(const float *__restrict__ a, int n, int *__restrict__ f)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -683,6 +713,16 @@ extern "C" int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coord
     fmolec1_rows<<<ceil_div(nrec, 256), 256, 0, st>>>(skey, sval, nrec, nself, pim, pjm, pE, eintra, l->d_rowstart,
                                                       l->d_partner, l->d_entry, l->d_slotmol, spos, l->d_ergpair);
     fmolec1_mirror<<<ceil_div(nrec, 256), 256, 0, st>>>(sval, nrec, nself, spos, l->d_mirror);
+    // longest push list of the MC resolver
+    int *dpush = nullptr;
+    HIP_CHECK(ctx, hipMallocAsync((void **)&dpush, sizeof(int) * (nmol + 1), st));
+    HIP_CHECK(ctx, hipMemsetAsync(dpush, 0, sizeof(int) * (nmol + 1), st));
+    count_pushes<<<ceil_div(nrec, 256), 256, 0, st>>>(nrec, l->d_slotmol, l->d_partner, l->d_mirror, l->d_pos,
+                                                      l->d_rowstart, dpush);
+    max_int<<<std::max(1, std::min(64, ceil_div(nmol, 256))), 256, 0, st>>>(dpush, nmol, dpush + nmol);
+    int maxpush = 0;
+    HIP_CHECK(ctx, hipMemcpyAsync(&maxpush, dpush + nmol, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(ctx, hipFreeAsync(dpush, st));
     // FP64 totals
     double *dsum = nullptr;
     HIP_CHECK(ctx, hipMallocAsync((void **)&dsum, 2 * sizeof(double), st));
@@ -708,15 +748,7 @@ extern "C" int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coord
     // the neighbour cells, which is nnbrefmax, so the cap can never bind
     ARG_CHECK(ctx, maxcnt <= l->nnbrefmax, "epsr_fmolec1: a ledger list of %d exceeds nnbrefmax %d", maxcnt,
               l->nnbrefmax);
-    {
-        std::vector<int> koff(nmol + 1, 0), kcnt(nmol);
-        for (int k = 0; k < nmol; ++k) {
-            kcnt[k] = l->h_cnt[l->order[k]];
-            koff[k + 1] = koff[k] + kcnt[k];
-        }
-        HIP_CHECK(ctx, hipMemcpy(l->d_koff, koff.data(), sizeof(int) * (nmol + 1), hipMemcpyHostToDevice));
-        HIP_CHECK(ctx, hipMemcpy(l->d_kcnt, kcnt.data(), sizeof(int) * nmol, hipMemcpyHostToDevice));
-    }
+    l->maxpush = maxpush;
     l->nself = nself;
     l->npairent = npair;
     l->nslots = nrec;
@@ -1063,87 +1095,20 @@ __global__ void mc_commit_ledger(int nslots, const int *__restrict__ slotmol, co
 // when the decision is the same for every float32 result within the a-priori rounding
 // bound of the reference's sequential sum; otherwise the exact sequential float32 sum
 // is formed and used (same result as the reference in every case).
-//   bound_k = u * (1 + 1e-3) * sum_l |t_l| (N - l + 1)  (recursive summation, the
-//             largest variant of each term t_l, N terms)
-//           + u * D_k * (npb_k + 2)                       (float32 pushes of the deltas)
+//   bound_k = u * (1 + 1e-3) * W_k        (recursive summation: W_k bounds the sum of
+//                                           the moduli of all partial sums)
+//           + u * D_k * (npb_k + 2)       (float32 rounding and accumulation of deltas)
 // with u = 2^-24, D_k = sum |delta_s| and npb_k the number of earlier partners.
-
-// one wave per molecule: per-move base, bound and the chain position's push data
-__global__ void __launch_bounds__(64)
-mc_prepare(int nmol, const int *__restrict__ pos, const int *__restrict__ rowstart, const int *__restrict__ cnt,
-           const int *__restrict__ partner, const int *__restrict__ mirror, const float *__restrict__ A,
-           const float *__restrict__ Bv, const float *__restrict__ L, float *__restrict__ delta,
-           double *__restrict__ base_k, double *__restrict__ bound_k)
-{
-    const int m = blockIdx.x;
-    const int lane = threadIdx.x;
-    const int row = rowstart[m], c = cnt[m];
-    const int N = 2 * (c - 1);
-    const int pm = pos[m];
-    double b = 0.0, w = 0.0, dsum = 0.0;
-    int npb = 0;
-    for (int s = 1 + lane; s < c; s += 64) {
-        int i = row + s;
-        int n = partner[i];
-        bool before = pos[n] < pm;
-        float a = A[i], l = L[i];
-        double e0 = (double)a - (double)l;
-        float maxnew = fabsf(a), maxold = fabsf(l);
-        float dl = 0.0f;
-        if (before) {
-            int mi = mirror[i];
-            float lw = mi != rowstart[n] ? A[mi] : l;
-            float bv = Bv[i];
-            double e1 = (double)bv - (double)lw;
-            dl = (float)(e1 - e0);
-            maxnew = fmaxf(maxnew, fabsf(bv));
-            maxold = fmaxf(maxold, fabsf(lw));
-            dsum += fabs((double)dl);
-            npb++;
-        }
-        delta[i] = dl;
-        b += e0;
-        w += (double)maxnew * (double)(N - 2 * s + 2) + (double)maxold * (double)(N - 2 * s + 1);
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        b += __shfl_down(b, off);
-        w += __shfl_down(w, off);
-        dsum += __shfl_down(dsum, off);
-        npb += __shfl_down(npb, off);
-    }
-    if (lane == 0) {
-        const double u = 5.9604644775390625e-08;   // 2^-24
-        base_k[pm] = b;
-        bound_k[pm] = u * (1.001 * w + dsum * (double)(npb + 2)) + 1e-12 * fabs(b) + 1e-30;
-    }
-}
-
-// push lists in chain order: for slot i' of the molecule at chain position k, the
-// later partner's chain position (or -1) and the delta of the partner's slot
-__global__ void mc_push_lists(int nslots, const int *__restrict__ slotmol, const int *__restrict__ rowstart,
-                              const int *__restrict__ partner, const int *__restrict__ mirror,
-                              const int *__restrict__ pos, const int *__restrict__ koff, const float *__restrict__ delta,
-                              int *__restrict__ kpt, float *__restrict__ kpv)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nslots) return;
-    int n = slotmol[i];
-    int pn = pos[n];
-    int o = koff[pn] + (i - rowstart[n]);
-    int tgt = -1;
-    float v = 0.0f;
-    int mi = mirror[i];
-    if (mi >= 0) {
-        int m = partner[i];
-        int pm = pos[m];
-        if (pm > pn && mi != rowstart[m]) {
-            tgt = pm;
-            v = delta[mi];
-        }
-    }
-    kpt[o] = tgt;
-    kpv[o] = v;
-}
+//
+// The Metropolis test r < RN(exp(-ediff)), ediff = RN(fdiff * temp), is decided in log
+// space with nlr = -log(r) (double, relative error < 1e-15):
+//   accept for every ediff <= hi  if  hi <= 0  or  (hi < 20 and hi <= nlr - 2^-22)
+//   reject for every ediff >= lo  if  lo >= 20 or (lo > 0 and lo >= nlr + 1e-12)
+// (exp(-x) >= r (1 + 2^-22) gives RN(exp(-x)) > r; exp(-x) < r gives RN(exp(-x)) <= r),
+// where [lo, hi] = temp * [fdiff - bound, fdiff + bound] widened by 2^-22 relative for
+// the float32 rounding of fdiff and of the product.  Both conditions are monotone in
+// fdiff, so they become two float32 thresholds on the pushed sum accde_k:
+//   accept when accde_k <= fa_k, reject when accde_k >= fr_k, else sum exactly.
 
 __device__ __forceinline__ float etest_of(float ediff)
 {
@@ -1162,6 +1127,135 @@ __device__ __forceinline__ float f_up(double x)
 {
     float f = (float)x;
     return (double)f < x ? nextafterf(f, INFINITY) : f;
+}
+
+// thresholds on accde for a move with base sum b, rounding bound bnd and nlr = -log(r)
+__device__ void mc_thresholds(double b, double bnd, double tempd, double nlr, float &fa, float &fr)
+{
+    const double rel = 2.384185791015625e-07;   // 2^-22
+    const double tiny = 1e-30;
+    fa = -INFINITY;
+    fr = INFINITY;
+    if (!(tempd > 0.0) || !isfinite(tempd) || !isfinite(b) || !isfinite(bnd) || !isfinite(nlr)) return;
+    // accept iff hi' <= ta, hi' = h + |h| rel + tiny, h = (fdiff + bnd) temp
+    const double ta = nlr - rel < 20.0 ? fmax(0.0, nlr - rel) : 19.999999999;
+    const double ha = (ta - tiny) / (ta - tiny >= 0.0 ? 1.0 + rel : 1.0 - rel);
+    // reject iff lo' >= tr (> 0), lo' = l - |l| rel - tiny, l = (fdiff - bnd) temp
+    const double tr = fmin(20.0, nlr + 1e-12);
+    const double lr = (tr + tiny) / (1.0 - rel);
+    const double va = ha / tempd, vr = lr / tempd;
+    // cover the double roundings of these few operations
+    const double slack = 1e-14 * (fabs(va) + fabs(vr) + bnd + fabs(b)) + 1e-300;
+    fa = f_down(va - bnd - b - slack);
+    fr = f_up(vr + bnd - b + slack);
+}
+
+// one wave per molecule: the deltas of its slots and the decision thresholds of its
+// move, written to the head of the chain position's push row
+__global__ void __launch_bounds__(64)
+mc_prepare(int nmol, const int *__restrict__ pos, const int *__restrict__ rowstart, const int *__restrict__ cnt,
+           const int *__restrict__ partner, const int *__restrict__ mirror, const float *__restrict__ A,
+           const float *__restrict__ Bv, const float *__restrict__ L, const float *__restrict__ stream,
+           const int *__restrict__ moff, float temp, int W, float *__restrict__ delta, int2 *__restrict__ rows)
+{
+    // Each rounding of the sequential sum errs by at most u |partial sum|.  With
+    // P_s = |sum_{s'<=s} e0| + sum_{s'<=s, earlier partner} |delta| the partial sums obey
+    // |S_2s| <= P_s and |S_2s-1| <= P_s-1 + max|ergnew_s| whatever the earlier decisions,
+    // so W = sum_s (P_s-1 + max|ergnew_s| + P_s) bounds sum_j |S_j|.  The slots are
+    // scanned in order, 64 at a time.
+    const int m = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int row = rowstart[m], c = cnt[m];
+    const int pm = pos[m];
+    double carry_e = 0.0, carry_d = 0.0, pprev = 0.0, w = 0.0;
+    int npb = 0;
+    for (int s0 = 1; s0 < c; s0 += 64) {
+        const int s = s0 + lane;
+        double e0 = 0.0, ad = 0.0;
+        float maxnew = 0.0f;
+        if (s < c) {
+            int i = row + s;
+            int n = partner[i];
+            bool before = pos[n] < pm;
+            float a = A[i], l = L[i];
+            e0 = (double)a - (double)l;
+            maxnew = fabsf(a);
+            float dl = 0.0f;
+            if (before) {
+                int mi = mirror[i];
+                float lw = mi != rowstart[n] ? A[mi] : l;
+                float bv = Bv[i];
+                dl = (float)(((double)bv - (double)lw) - e0);
+                maxnew = fmaxf(maxnew, fabsf(bv));
+                ad = fabs((double)dl);
+                npb++;
+            }
+            delta[i] = dl;
+        }
+        double se = e0, sd = ad;
+        for (int off = 1; off < 64; off <<= 1) {
+            double te = __shfl_up(se, off), td = __shfl_up(sd, off);
+            if (lane >= off) {
+                se += te;
+                sd += td;
+            }
+        }
+        double pe = carry_e + se, pd = carry_d + sd;
+        double ps = fabs(pe) + pd;
+        double pm1 = __shfl_up(ps, 1);
+        if (lane == 0) pm1 = pprev;
+        if (s < c) w += pm1 + (double)maxnew + ps;
+        carry_e = __shfl(pe, 63);
+        carry_d = __shfl(pd, 63);
+        pprev = __shfl(ps, 63);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        w += __shfl_down(w, off);
+        npb += __shfl_down(npb, off);
+    }
+    if (lane == 0) {
+        const double u = 5.9604644775390625e-08;   // 2^-24
+        const double bnd = u * (1.001 * w + carry_d * (double)(npb + 2)) + 1e-12 * fabs(carry_e) + 1e-30;
+        const double nlr = -log((double)stream[moff[pm]]);
+        float fa, fr;
+        mc_thresholds(carry_e, bnd, (double)temp, nlr, fa, fr);
+        rows[(size_t)pm * W] = make_int2(__float_as_int(fa), __float_as_int(fr));
+    }
+}
+
+// one wave per chain position k: entries 1.. of row k list (later partner's chain
+// position, delta of the partner's slot) for every pair an accepted move at k updates;
+// the rest of the row holds (-1, 0).  The row width W exceeds the longest list.  Rows
+// nmol.. (grid beyond nmol) pad the chain to a whole number of resolver rings.
+__global__ void __launch_bounds__(64)
+mc_push_rows(int nmol, int W, const int *__restrict__ order, const int *__restrict__ pos,
+             const int *__restrict__ rowstart, const int *__restrict__ cnt, const int *__restrict__ partner,
+             const int *__restrict__ mirror, const float *__restrict__ delta, int2 *__restrict__ rows)
+{
+    const int k = blockIdx.x;
+    const int lane = threadIdx.x;
+    int2 *out = rows + (size_t)k * W;
+    if (k >= nmol) {
+        // padding row past the chain's end: rejected, no pushes
+        const int ninf = __float_as_int(-INFINITY);
+        for (int e = lane; e < W; e += 64) out[e] = e == 0 ? make_int2(ninf, ninf) : make_int2(-1, 0);
+        return;
+    }
+    const int n = order[k];
+    const int row = rowstart[n], c = cnt[n];
+    int fill = 1;
+    for (int s0 = 0; s0 < c; s0 += 64) {
+        const int s = s0 + lane;
+        int t = -1;
+        if (s < c) t = push_target(row + s, k, partner, mirror, pos, rowstart);
+        const unsigned long long b = __ballot(t >= 0);
+        if (t >= 0) {
+            int idx = fill + __popcll(b & ((1ull << lane) - 1ull));
+            out[idx] = make_int2(t, __float_as_int(delta[mirror[row + s]]));
+        }
+        fill += __popcll(b);
+    }
+    for (int e = fill + lane; e < W; e += 64) out[e] = make_int2(-1, 0);
 }
 
 // exact fdiff of every accepted move (for fadd), one lane per chain position
@@ -1209,100 +1303,89 @@ __global__ void mc_fadd(int nmol, const unsigned *__restrict__ bits, const float
 }
 
 // ------------------------------------------------------------------ single-wave resolver
-// Same decisions as mc_resolve_fast, one wave64: LDS operations of one wave complete in
-// order, so pushes need no barrier.  The Metropolis test r < RN(exp(-ediff)) is decided
-// in log space: with nlr = -log(r) (exact to 1e-15 in double),
-//   accept for every ediff <= hi  if  hi <= 0  or  (hi < 20 and hi <= nlr - 2^-22)
-//   reject for every ediff >= lo  if  lo >= 20 or (lo > 0 and lo >= nlr + 1e-12)
-// (exp(-x) >= r (1 + 2^-23) gives RN(exp(-x)) > r; exp(-x) <= r gives RN(exp(-x)) <= r).
-// The ediff interval widens (v -/+ bound) * temp by 2^-22 relative to cover the float32
-// rounding of fdiff and of the product.
-#define RW_J 8
-#define RW_D 8
+// One wave64 walks the chain.  LDS operations of one wave complete in order, so the
+// float32 pushes of an accepted move reach accde before any later read, with no
+// barrier.  The push rows stream through an LDS ring of two chunks of CH rows: while
+// chunk c is decided, chunk c+1 sits in the other half and chunk c+2 is in flight in
+// registers, so the chain never waits on global memory.  Lane 0 of entry 0 of a row
+// carries the move's thresholds.
+#define RW_MAXJ 8
 
-__global__ void mc_nlogr(int nmol, const float *__restrict__ stream, const int *__restrict__ moff,
-                         double *__restrict__ nlr)
-{
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < nmol) nlr[k] = -log((double)stream[moff[k]]);
-}
-
+template <int NJ, int CH>
 __global__ void __launch_bounds__(64)
-mc_resolve_wave(int nmol, const int *__restrict__ order, const int *__restrict__ pos,
+mc_resolve_rows(int nmol, const int *__restrict__ order, const int *__restrict__ pos,
                 const int *__restrict__ rowstart, const int *__restrict__ cnt, const int *__restrict__ partner,
                 const int *__restrict__ mirror, const float *__restrict__ A, const float *__restrict__ Bv,
-                const float *__restrict__ L, const int *__restrict__ koff, const int *__restrict__ kcnt,
-                const int *__restrict__ kpt, const float *__restrict__ kpv, const double *__restrict__ base_k,
-                const double *__restrict__ bound_k, const double *__restrict__ nlr_k,
-                const float *__restrict__ stream, const int *__restrict__ moff, float temp,
-                unsigned *__restrict__ acc_out, int *__restrict__ nexact_out)
+                const float *__restrict__ L, const int2 *__restrict__ rows, const float *__restrict__ stream,
+                const int *__restrict__ moff, float temp, unsigned *__restrict__ acc_out, int *__restrict__ nexact_out)
 {
-    extern __shared__ unsigned sm[];
-    const int nw = (nmol + 31) >> 5;
-    unsigned *bits = sm;
-    float *accde = (float *)(bits + nw);
-    float *term = accde + nmol;            // [2 * 64 * RW_J]
+    // rows nmol..npad-1 (npad a multiple of CH) are padding: rejected, no pushes
+    constexpr int W = NJ * 64;
+    extern __shared__ int2 smr[];
+    const int npad = (nmol + CH - 1) / CH * CH;
+    const int nchunk = npad / CH;
+    const int nw = (npad + 31) >> 5;
+    int2 *ring = smr;                                   // [2][CH * W]
+    float *accde = (float *)(ring + 2 * CH * W);        // [npad]
+    unsigned *bits = (unsigned *)(accde + npad);        // [nw]
+    float *term = (float *)(bits + nw);                 // [2 * longest list]
     const int lane = threadIdx.x;
     for (int i = lane; i < nw; i += 64) bits[i] = 0u;
-    for (int i = lane; i < nmol; i += 64) accde[i] = 0.0f;
+    for (int i = lane; i < npad; i += 64) accde[i] = 0.0f;
     int nexact = 0;
-    const double tempd = (double)temp;
-    const double rel = 2.384185791015625e-07;   // 2^-22
 
-    int s_off[RW_D], s_c[RW_D];
-    double s_base[RW_D], s_bnd[RW_D], s_nlr[RW_D];
-    int s_pt[RW_D][RW_J];
-    float s_pv[RW_D][RW_J];
-    auto load = [&](int d, int k) {
-        int off = koff[k], c = kcnt[k];
-        s_off[d] = off;
-        s_c[d] = c;
-        s_base[d] = base_k[k];
-        s_bnd[d] = bound_k[k];
-        s_nlr[d] = nlr_k[k];
+    int2 stage[CH][NJ];
+    auto fetch = [&](int c) {
+        const int2 *src = rows + (size_t)c * CH * W + lane;
 #pragma unroll
-        for (int j = 0; j < RW_J; ++j) {
-            int s = lane + j * 64;
-            s_pt[d][j] = s < c ? kpt[off + s] : -1;
-            s_pv[d][j] = s < c ? kpv[off + s] : 0.0f;
-        }
+        for (int r = 0; r < CH; ++r)
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) stage[r][j] = src[r * W + 64 * j];
     };
+    auto put = [&](int half) {
+        int2 *dst = ring + half * CH * W + lane;
 #pragma unroll
-    for (int d = 0; d < RW_D; ++d)
-        if (d < nmol) load(d, d);
+        for (int r = 0; r < CH; ++r)
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) dst[r * W + 64 * j] = stage[r][j];
+    };
+    fetch(0);
+    put(0);
+    if (nchunk > 1) fetch(1);
     __syncthreads();
-    for (int k0 = 0; k0 < nmol; k0 += RW_D) {
+    for (int c = 0; c < nchunk; ++c) {
+        if (c + 1 < nchunk) {
+            put((c + 1) & 1);
+            if (c + 2 < nchunk) fetch(c + 2);
+        }
+        const int2 *rb = ring + (c & 1) * CH * W + lane;
+        for (int r = 0; r < CH; ++r) {
+            const int k = c * CH + r;
+            int2 e[NJ];
 #pragma unroll
-        for (int d = 0; d < RW_D; ++d) {
-            const int k = k0 + d;
-            if (k >= nmol) break;
-            double v = s_base[d] + (double)accde[k];
-            double bnd = s_bnd[d];
-            double lo = (v - bnd) * tempd, hi = (v + bnd) * tempd;
-            lo = lo - fabs(lo) * rel - 1e-30;
-            hi = hi + fabs(hi) * rel + 1e-30;
-            double nlr = s_nlr[d];
-            int dec = -1;
-            if (hi <= 0.0 || (hi < 20.0 && hi <= nlr - rel)) dec = 1;
-            else if (lo >= 20.0 || (lo > 0.0 && lo >= nlr + 1e-12)) dec = 0;
+            for (int j = 0; j < NJ; ++j) e[j] = rb[r * W + 64 * j];
+            const float a = accde[k];
+            const float fa = __int_as_float(__builtin_amdgcn_readfirstlane(e[0].x));
+            const float fr = __int_as_float(__builtin_amdgcn_readfirstlane(e[0].y));
+            int dec = a <= fa ? 1 : (a >= fr ? 0 : -1);
             if (dec < 0) {
-                // exact: gather the selected terms in parallel, sum them in order on lane 0
+                // exact: gather the selected variant of every slot, lane 0 sums in order
                 const int m = order[k];
-                const int row = rowstart[m], c = cnt[m];
-                for (int s = 1 + lane; s < c; s += 64) {
+                const int row = rowstart[m], cm = cnt[m];
+                for (int s = 1 + lane; s < cm; s += 64) {
                     int i = row + s;
                     int n = partner[i];
                     int pn = pos[n];
-                    bool a = pn < k && ((bits[pn >> 5] >> (pn & 31)) & 1u);
+                    bool acc = pn < k && ((bits[pn >> 5] >> (pn & 31)) & 1u);
                     int mi = mirror[i];
-                    term[2 * s] = a ? Bv[i] : A[i];
-                    term[2 * s + 1] = (a && mi != rowstart[n]) ? A[mi] : L[i];
+                    term[2 * s] = acc ? Bv[i] : A[i];
+                    term[2 * s + 1] = (acc && mi != rowstart[n]) ? A[mi] : L[i];
                 }
                 __syncthreads();
                 int dd = 0;
                 if (lane == 0) {
                     float fdiff = 0.0f;
-                    for (int s = 1; s < c; ++s) fdiff = fdiff + term[2 * s] - term[2 * s + 1];
+                    for (int s = 1; s < cm; ++s) fdiff = fdiff + term[2 * s] - term[2 * s + 1];
                     float ediff = (fdiff + 0.0f) * temp;
                     dd = stream[moff[k]] < etest_of(ediff) ? 1 : 0;
                 }
@@ -1313,16 +1396,15 @@ mc_resolve_wave(int nmol, const int *__restrict__ order, const int *__restrict__
             if (dec) {
                 if (lane == 0) bits[k >> 5] |= 1u << (k & 31);
 #pragma unroll
-                for (int j = 0; j < RW_J; ++j) {
-                    int t = s_pt[d][j];
-                    if (t >= 0) atomicAdd(&accde[t], s_pv[d][j]);
+                for (int j = 0; j < NJ; ++j) {
+                    const int t = e[j].x;
+                    if (t >= 0 && (j > 0 || lane > 0)) atomicAdd(&accde[t], __int_as_float(e[j].y));
                 }
             }
-            if (k + RW_D < nmol) load(d, k + RW_D);
         }
     }
     __syncthreads();
-    for (int i = lane; i < nw; i += 64) acc_out[i] = bits[i];
+    for (int i = lane; i < (nmol + 31) >> 5; i += 64) acc_out[i] = bits[i];
     if (lane == 0) *nexact_out = nexact;
 }
 
@@ -1398,8 +1480,24 @@ extern "C" int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
     std::vector<float> hstream(total);
     for (int i = 0; i < total; ++i) hstream[i] = ran1_next(rng);
 
-    // device buffers
+    // device buffers; push rows of rw entries for the fast resolver
     const int nslots = l->nslots;
+    int maxc = 1;
+    for (int v : l->h_cnt) maxc = std::max(maxc, v);
+    int nj = 1;
+    while (nj * 64 < l->maxpush + 1) nj *= 2;
+    // the resolver's LDS: a ring of 2 chunks x (CH rows of rw entries), CH * rw = 4096
+    // entries (64 KB) or, when that does not fit, 2048; accde, acceptance bits, terms
+    const int npad = (nmol + 63) / 64 * 64;   // a whole number of chunks
+    auto lds_need = [&](int ring_entries) {
+        return sizeof(int2) * 2 * (size_t)ring_entries + sizeof(float) * (size_t)npad +
+               sizeof(unsigned) * (npad / 32) + sizeof(float) * 2 * (size_t)maxc + 64;
+    };
+    const size_t lds_cap = 160 * 1024;
+    const bool big_ring = lds_need(4096) <= lds_cap;
+    const size_t lds_fast = lds_need(big_ring ? 4096 : 2048);
+    const bool fast = nj <= RW_MAXJ && lds_fast <= lds_cap;
+    const int rw = fast ? nj * 64 : 0;
     Scratch S, q;
     auto layout = [&](Scratch &x) {
         x.take<float4>(natom); x.take<float4>(natom);
@@ -1411,8 +1509,7 @@ extern "C" int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
         x.take<float>(nmol); x.take<float>(nmol); x.take<float>(nmol);
         x.take<float>(nslots); x.take<float>(nslots); x.take<float>(nslots);
         x.take<unsigned>((nmol + 31) / 32); x.take<float>(nmol); x.take<float>(4); x.take<int>(2 * nmt + 2);
-        x.take<float>(nslots); x.take<double>(nmol); x.take<double>(nmol); x.take<int>(nslots); x.take<float>(nslots);
-        x.take<int>(4); x.take<double>(nmol);
+        x.take<float>(nslots); x.take<int>(4); x.take<int2>((size_t)npad * rw);
     };
     layout(q);
     S.base = (char *)ctx_scratch(ctx, q.off);
@@ -1431,11 +1528,8 @@ extern "C" int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
     float *dres = S.take<float>(4);
     int *drej = S.take<int>(2 * nmt + 2);
     float *ddelta = S.take<float>(nslots);
-    double *dbase = S.take<double>(nmol), *dbound = S.take<double>(nmol);
-    int *dkpt = S.take<int>(nslots);
-    float *dkpv = S.take<float>(nslots);
     int *dnex = S.take<int>(4);
-    double *dnlr = S.take<double>(nmol);
+    int2 *drows = S.take<int2>((size_t)npad * rw);
 
     PreparedTables pt;
     int rc = prepare_tables(ctx, box, tb, dT, dtmp, pt);
@@ -1541,22 +1635,29 @@ extern "C" int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
                                            dB, dL);
     }
     timer_end(ctx, "mc_pair");
-    int maxc = 1;
-    for (int v : l->h_cnt) maxc = std::max(maxc, v);
-    const size_t lds_fast = sizeof(unsigned) * ((nmol + 31) / 32) + sizeof(float) * (size_t)nmol +
-                            sizeof(float) * 2 * 64 * RW_J + 64;
-    const bool fast = maxc <= RW_J * 64 && lds_fast <= 150 * 1024;
     int nexact = nmol;
     if (fast) {
         mc_prepare<<<nmol, 64, 0, st>>>(nmol, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner, l->d_mirror, dA, dB, dL,
-                                        ddelta, dbase, dbound);
-        mc_push_lists<<<ceil_div(nslots, 256), 256, 0, st>>>(nslots, l->d_slotmol, l->d_rowstart, l->d_partner,
-                                                             l->d_mirror, l->d_pos, l->d_koff, ddelta, dkpt, dkpv);
-        mc_nlogr<<<ceil_div(nmol, 256), 256, 0, st>>>(nmol, dstream, dmoff, dnlr);
+                                        dstream, dmoff, temp, rw, ddelta, drows);
+        mc_push_rows<<<npad, 64, 0, st>>>(nmol, rw, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner,
+                                          l->d_mirror, ddelta, drows);
         timer_begin(ctx, "mc_resolve");
-        mc_resolve_wave<<<1, 64, lds_fast, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner,
-                                                 l->d_mirror, dA, dB, dL, l->d_koff, l->d_kcnt, dkpt, dkpv, dbase,
-                                                 dbound, dnlr, dstream, dmoff, temp, dacc, dnex);
+#define MC_RESOLVE(NJ, CH)                                                                                           \
+    mc_resolve_rows<NJ, CH><<<1, 64, lds_fast, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt,        \
+                                                     l->d_partner, l->d_mirror, dA, dB, dL, drows, dstream, dmoff, \
+                                                     temp, dacc, dnex)
+        if (big_ring) {
+            if (nj == 1) MC_RESOLVE(1, 64);
+            else if (nj == 2) MC_RESOLVE(2, 32);
+            else if (nj == 4) MC_RESOLVE(4, 16);
+            else MC_RESOLVE(8, 8);
+        } else {
+            if (nj == 1) MC_RESOLVE(1, 32);
+            else if (nj == 2) MC_RESOLVE(2, 16);
+            else if (nj == 4) MC_RESOLVE(4, 8);
+            else MC_RESOLVE(8, 4);
+        }
+#undef MC_RESOLVE
         timer_end(ctx, "mc_resolve");
         mc_fdiff_accepted<<<ceil_div(nmol, 128), 128, 0, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt,
                                                                l->d_partner, l->d_mirror, dA, dB, dL, dacc, dfdiff);

@@ -250,7 +250,7 @@ struct BasisTerm {
 
 // the serial per-term parameters of calc_pwrexpr_grad / calc_pwrexpq: positions by
 // dnext (:5313-5326), powers n, and the log normalisation with gamln = log((n+2)!)
-static void basis_terms(int nterms, float dmin, float dstep, float dmax, float sigmab, float rho, float rmin,
+static bool basis_terms(int nterms, float dmin, float dstep, float dmax, float sigmab, float rho, float rmin,
                         float fbroad, bool rspace, std::vector<BasisTerm> &out, int &ncfirst)
 {
     const float pi = 3.141592653f;
@@ -276,7 +276,8 @@ static void basis_terms(int nterms, float dmin, float dstep, float dmax, float s
             float gfac = 4.0f * pi * rho * sigma * sigma * sigma;
             float gfacln = (float)std::log((double)gfac);
             int nref = n + 3;
-            float sigln = (float)((nref < (int)gamln.size() ? gamln[nref] : 0.0) + (double)gfacln);
+            if (nref > ntmax + 1) return false;   // beyond gamln(1..ntmax+1)
+            float sigln = (float)(gamln[nref] + (double)gfacln);
             if (m == 1) sigln = sigln - rmin / sigma;
             bt.sigma = sigma;
             bt.sigln = sigln;
@@ -296,6 +297,7 @@ static void basis_terms(int nterms, float dmin, float dstep, float dmax, float s
         }
         d = dn;
     }
+    return true;
 }
 
 // calc_pwrexpr_grad element (:5266-5306); expr flat 1-based [(i-1)*nterms + m]
@@ -361,7 +363,7 @@ extern "C" int epsr_calc_pwrexpq(epsr_ctx *ctx, int lpt, int nterms, float dmin,
     HIP_CHECK(ctx, hipSetDevice(ctx->device));
     std::vector<BasisTerm> bt;
     int ncf = 0;
-    basis_terms(nterms, dmin, dstep, dmax, sigmab, rho, rmin, 1.0f, false, bt, ncf);
+    (void)basis_terms(nterms, dmin, dstep, dmax, sigmab, rho, rmin, 1.0f, false, bt, ncf);
     BasisTerm *dbt = (BasisTerm *)ctx_scratch(ctx, sizeof(BasisTerm) * (nterms + 1));
     if (!dbt) return EPSR_ERR_HIP;
     HIP_CHECK(ctx, hipMemcpyAsync(dbt, bt.data(), sizeof(BasisTerm) * (nterms + 1), hipMemcpyHostToDevice, ctx->stream));
@@ -451,7 +453,9 @@ extern "C" int epsr_generate_ep(epsr_ctx *ctx, int npt, int npt2, const float *x
     std::vector<BasisTerm> bt;
     int ncf = 0;
     // calc_pwrexpr_grad(npt2, nterms, pdmin, pdstep, pdmax, psigma2, rho, x, expgr, 0, ncfirst, 1, rbroad)
-    basis_terms(nterms, p->pdmin, p->pdstep, p->pdmax, p->psigma2, p->rho, 0.0f, 1.0f, true, bt, ncf);
+    ARG_CHECK(ctx, basis_terms(nterms, p->pdmin, p->pdstep, p->pdmax, p->psigma2, p->rho, 0.0f, 1.0f, true, bt, ncf),
+              "epsr_generate_ep: a basis function position needs ln((n+2)!) beyond the table of pdmax/psigma2+3 "
+              "terms (pdstep x ncoeffp exceeds pdmax)");
     int maxref = 0;
     for (int ip = 0; ip < npairs; ++ip) maxref = std::max(maxref, irefpcof[ip] + 1);
     std::vector<float> coef((size_t)npairs * (nterms + 1), 0.0f), scale(npairs);

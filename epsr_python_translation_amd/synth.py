@@ -305,6 +305,9 @@ def build_system(name, seed, L, molecules, type_names, *, cellst, rmaxgr, qstep,
     qcor = (charge[n1] * charge[n2] * F32(1390)).astype(F32)
     cnt = np.bincount(itype, minlength=ntype).astype(F32)
     cfrac = (cnt / F32(xa.shape[0])).astype(F32)
+    # Poisson-basis spacing: the basis positions must stay within pdmax = rmaxpt, as the
+    # reference sets pdstep from pdmax and ncoeffp
+    kw.setdefault("pdstep", float(F32(F32(kw.get("rmaxpt", 12.0)) / F32(nterms))))
     pcoefp = np.zeros((npairs, nterms + 1), dtype=F32)
     pcoefp[:, 1] = 1.0
     pcoefp[:, 2:] = rng.normal(0.0, 0.1, (npairs, nterms - 1)).astype(F32)

@@ -355,6 +355,10 @@ int ora_calc_pwrexpr_grad(int lpt, int nterms, float dmin, float dstep, float dm
         float sigma = rootd * fbroad * sigmab;
         int n = (sigma <= d && sigma > 0.0f && m > 1) ? ora_ifix(d / sigma - 1.0f) : 0;
         int nref = n + 3;
+        if (nref > ntmax + 1) {   /* gamln(1..ntmax+1): the reference raises IndexError */
+            free(gamln);
+            return -1;
+        }
         float gfac = 4.0f * pi * rho * sigma * sigma * sigma;
         float gfacln = ora_logf(gfac);
         float sigln = (float)(gamln[nref] + (double)gfacln);
@@ -399,7 +403,12 @@ float ora_generate_ep(int npt, int npt2, const float *x, const float *xstar, con
     const float summax = (float)exp(25.0);
     float *expgr = calloc((size_t)npt2 * nterms + 1, sizeof(float));
     float *pwtold = calloc((size_t)nterms + 1, sizeof(float));
-    ora_calc_pwrexpr_grad(npt2, nterms, pdmin, pdstep, pdmax, psigma2, rho, x, expgr, 0.0f, 1.0f, rbroad);
+    if (ora_calc_pwrexpr_grad(npt2, nterms, pdmin, pdstep, pdmax, psigma2, rho, x, expgr, 0.0f, 1.0f,
+                              rbroad) < 0) {
+        free(expgr);
+        free(pwtold);
+        return NAN;   /* a basis position needs ln((n+2)!) beyond the tabulated range */
+    }
     for (int ip = 0; ip < npairs; ++ip) {
         const float *pc = pcoefp + (size_t)irefpcof[ip] * ldp;
         for (int ic = 2; ic <= nterms; ++ic) pwtold[ic] = pc[ic];
