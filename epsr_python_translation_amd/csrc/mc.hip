@@ -101,7 +101,7 @@ static int h_ifix(float xf)
     return (int)std::trunc(x);
 }
 
-static int fold(int ix, int n)
+__host__ __device__ static inline int fold(int ix, int n)
 {
     while (ix < 0) ix += n;
     while (ix >= n) ix -= n;
@@ -196,23 +196,14 @@ extern "C" int epsr_cells_init(epsr_ctx *ctx, const epsr_box *box, epsr_ledger *
         }
         l->ncellneighb = (int)l->iaoff.size();
     }
-    // chain order of molfit (cells in order, molecules in cell order) and nnbrefmax
+    // chain order of molfit (cells in order, molecules in cell order)
     l->order.clear();
     l->order.reserve(nmol);
     for (int cc = 0; cc < l->ncell; ++cc)
         for (int i = 0; i < l->nmolsincell[cc]; ++i) l->order.push_back(l->molsincell[(size_t)cc * l->maxcellno + i]);
     l->pos.assign(nmol, 0);
     for (int k = 0; k < nmol; ++k) l->pos[l->order[k]] = k;
-    l->nnbrefmax = 0;
-    for (int cc = 0; cc < l->ncell; ++cc) {
-        int icc = cc / (l->na * l->nb), ib = (cc - icc * l->na * l->nb) / l->na, ia = cc - l->na * (ib + l->nb * icc);
-        int n = 0;
-        for (int k = 0; k < l->ncellneighb; ++k) {
-            int a = fold(ia + l->iaoff[k], l->na), b = fold(ib + l->iboff[k], l->nb), q = fold(icc + l->icoff[k], l->nc);
-            n += l->nmolsincell[a + l->na * (b + l->nb * q)];
-        }
-        l->nnbrefmax = std::max(l->nnbrefmax, n);
-    }
+    l->nnbrefmax = 0;   // get_nnbrefmax: formed on the GPU by epsr_fmolec1
     HIP_CHECK(ctx, hipMemcpy(l->d_order, l->order.data(), sizeof(int) * nmol, hipMemcpyHostToDevice));
     HIP_CHECK(ctx, hipMemcpy(l->d_pos, l->pos.data(), sizeof(int) * nmol, hipMemcpyHostToDevice));
     l->cells_ready = true;
@@ -561,6 +552,91 @@ struct Scratch {
     }
 };
 
+// neighbour cells of every cell (cellneighbourlist.py:199-219 order) and their counts
+__global__ void cell_neighbours(int ncell, int nbr, int na, int nb, int nc, const int *__restrict__ iaoff,
+                                const int *__restrict__ iboff, const int *__restrict__ icoff,
+                                const int *__restrict__ nmolsincell, int *__restrict__ nbcell,
+                                int *__restrict__ nbcnt)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)ncell * nbr) return;
+    int cc = (int)(t / nbr), kk = (int)(t - (long long)cc * nbr);
+    int icc = cc / (na * nb), ib = (cc - icc * na * nb) / na, ia = cc - na * (ib + nb * icc);
+    int cref = fold(ia + iaoff[kk], na) + na * (fold(ib + iboff[kk], nb) + nb * fold(icc + icoff[kk], nc));
+    nbcell[t] = cref;
+    nbcnt[t] = nmolsincell[cref];
+}
+
+// per cell: exclusive prefix of the neighbour-cell counts (nbpre, nbr + 1 per cell), the
+// length of the cell's neighbour molecule list (rowlen) and of its flattened copy
+// (flatlen, 0 for an empty cell); one block per cell
+__global__ void __launch_bounds__(256)
+cell_prefix(int nbr, const int *__restrict__ nbcnt, const int *__restrict__ nmolsincell, int *__restrict__ nbpre,
+            int *__restrict__ rowlen, int *__restrict__ flatlen)
+{
+    __shared__ int sw[4];
+    const int cc = blockIdx.x;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int *cnt = nbcnt + (size_t)cc * nbr;
+    int *pre = nbpre + (size_t)cc * (nbr + 1);
+    int carry = 0;
+    for (int k0 = 0; k0 < nbr; k0 += 256) {
+        const int k = k0 + threadIdx.x;
+        int v = k < nbr ? cnt[k] : 0, x = v;
+        for (int off = 1; off < 64; off <<= 1) {
+            int y = __shfl_up(x, off);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) sw[wv] = x;
+        __syncthreads();
+        int before = carry;
+        for (int w = 0; w < wv; ++w) before += sw[w];
+        if (k < nbr) pre[k] = before + x - v;
+        carry += sw[0] + sw[1] + sw[2] + sw[3];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        pre[nbr] = carry;
+        rowlen[cc] = carry;
+        flatlen[cc] = nmolsincell[cc] > 0 ? carry : 0;
+    }
+}
+
+// flattened neighbour molecule lists of the non-empty cells
+__global__ void cell_flatten(int ncell, int nbr, int maxcellno, const int *__restrict__ nbcell,
+                             const int *__restrict__ nbcnt, const int *__restrict__ nbpre,
+                             const int *__restrict__ cstart, const int *__restrict__ nmolsincell,
+                             const int *__restrict__ molsincell, int *__restrict__ nbflat)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)ncell * nbr) return;
+    int cc = (int)(t / nbr), kk = (int)(t - (long long)cc * nbr);
+    if (nmolsincell[cc] == 0) return;
+    int *dst = nbflat + cstart[cc] + nbpre[(size_t)cc * (nbr + 1) + kk];
+    const int *src = molsincell + (size_t)nbcell[t] * maxcellno;
+    for (int j = 0, n = nbcnt[t]; j < n; ++j) dst[j] = src[j];
+}
+
+// row length of every chain position (its cell's list), as int64 for the scan
+__global__ void chain_rowlen(int nmol, const int *__restrict__ order, const int *__restrict__ mcell,
+                             const int *__restrict__ rowlen, long long *__restrict__ out)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nmol) out[k] = rowlen[mcell[order[k]]];
+    if (k == nmol) out[k] = 0;
+}
+
+// stream-ordered release of temporaries on every return path
+struct StreamFree {
+    hipStream_t st;
+    std::vector<void *> p;
+    ~StreamFree()
+    {
+        for (void *q : p)
+            if (q) (void)hipFreeAsync(q, st);
+    }
+};
+
 static int hipcub_scan_int(epsr_ctx *ctx, const int *in, int *out, int n)
 {
     size_t tb = 0;
@@ -582,23 +658,62 @@ extern "C" int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coord
     HIP_CHECK(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     const int nmol = box->nmol;
-    // host: the neighbour list of every cell, flattened, and the row offsets of the
+    // neighbour lists on the GPU: per cell the neighbour cells and their prefix counts,
+    // the flattened molecule list of every non-empty cell and the row offsets of the
     // (chain position, neighbour position) items
-    std::vector<int> cstart(l->ncell + 1, 0), nbflat, tmp;
-    for (int cc = 0; cc < l->ncell; ++cc) {
-        cstart[cc] = (int)nbflat.size();
-        if (l->nmolsincell[cc] > 0) {
-            cell_nbref(l, cc, tmp);
-            nbflat.insert(nbflat.end(), tmp.begin(), tmp.end());
-        }
+    const int ncell = l->ncell, nbr = l->ncellneighb;
+    const long long ncn = (long long)ncell * nbr;
+    int *cw = nullptr;
+    const size_t cw_ints = 3 * (size_t)nbr + (size_t)ncell + (size_t)ncell * l->maxcellno + (size_t)nmol +
+                           2 * (size_t)ncn + (size_t)ncell * (nbr + 1) + 3 * ((size_t)ncell + 1) + 8;
+    long long *drow = nullptr, *dlen = nullptr;
+    StreamFree guard{st};
+    HIP_CHECK(ctx, hipMallocAsync((void **)&cw, sizeof(int) * cw_ints, st));
+    guard.p.push_back(cw);
+    HIP_CHECK(ctx, hipMallocAsync((void **)&drow, sizeof(long long) * (nmol + 1), st));
+    guard.p.push_back(drow);
+    HIP_CHECK(ctx, hipMallocAsync((void **)&dlen, sizeof(long long) * (nmol + 1), st));
+    guard.p.push_back(dlen);
+    int *doff = cw, *dnmc = doff + 3 * nbr, *dmic = dnmc + ncell, *dmcell = dmic + (size_t)ncell * l->maxcellno;
+    int *dnbcell = dmcell + nmol, *dnbcnt = dnbcell + ncn, *dnbpre = dnbcnt + ncn;
+    int *drowlen = dnbpre + (size_t)ncell * (nbr + 1), *dflatlen = drowlen + ncell + 1, *dcstart = dflatlen + ncell + 1;
+    int *dcw_tail = dcstart + ncell + 1;
+    {
+        std::vector<int> off(3 * (size_t)nbr);
+        std::copy(l->iaoff.begin(), l->iaoff.end(), off.begin());
+        std::copy(l->iboff.begin(), l->iboff.end(), off.begin() + nbr);
+        std::copy(l->icoff.begin(), l->icoff.end(), off.begin() + 2 * nbr);
+        HIP_CHECK(ctx, hipMemcpyAsync(doff, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice, st));
+        HIP_CHECK(ctx, hipMemcpyAsync(dnmc, l->nmolsincell.data(), sizeof(int) * ncell, hipMemcpyHostToDevice, st));
+        HIP_CHECK(ctx, hipMemcpyAsync(dmic, l->molsincell.data(), sizeof(int) * l->molsincell.size(),
+                                      hipMemcpyHostToDevice, st));
+        HIP_CHECK(ctx, hipMemcpyAsync(dmcell, l->molcellno.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, st));
+        HIP_CHECK(ctx, hipStreamSynchronize(st));   // the host vectors above are not pinned
     }
-    cstart[l->ncell] = (int)nbflat.size();
-    std::vector<long long> rowoff(nmol + 1, 0);
-    for (int k = 0; k < nmol; ++k) {
-        int cc = l->molcellno[l->order[k]];
-        rowoff[k + 1] = rowoff[k] + (cstart[cc + 1] - cstart[cc]);
+    cell_neighbours<<<(unsigned)ceil_div(ncn, 256), 256, 0, st>>>(ncell, nbr, l->na, l->nb, l->nc, doff, doff + nbr,
+                                                                  doff + 2 * nbr, dnmc, dnbcell, dnbcnt);
+    cell_prefix<<<ncell, 256, 0, st>>>(nbr, dnbcnt, dnmc, dnbpre, drowlen, dflatlen);
+    HIP_CHECK(ctx, hipMemsetAsync(dflatlen + ncell, 0, sizeof(int), st));
+    HIP_CHECK(ctx, hipMemsetAsync(dcw_tail, 0, sizeof(int), st));
+    int rc;
+    if ((rc = hipcub_scan_int(ctx, dflatlen, dcstart, ncell + 1))) return rc;
+    max_int<<<std::max(1, std::min(64, ceil_div(ncell, 256))), 256, 0, st>>>(drowlen, ncell, dcw_tail);
+    chain_rowlen<<<ceil_div(nmol + 1, 256), 256, 0, st>>>(nmol, l->d_order, dmcell, drowlen, dlen);
+    {
+        size_t tbytes = 0;
+        HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tbytes, dlen, drow, nmol + 1, st));
+        void *tmpb = nullptr;
+        HIP_CHECK(ctx, hipMallocAsync(&tmpb, tbytes + 16, st));
+        HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(tmpb, tbytes, dlen, drow, nmol + 1, st));
+        HIP_CHECK(ctx, hipFreeAsync(tmpb, st));
     }
-    const long long total = rowoff[nmol];
+    long long total = 0;
+    int nflat = 0, nnbrefmax = 0;
+    HIP_CHECK(ctx, hipMemcpyAsync(&total, drow + nmol, sizeof(long long), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(ctx, hipMemcpyAsync(&nflat, dcstart + ncell, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(ctx, hipMemcpyAsync(&nnbrefmax, dcw_tail, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(ctx, hipStreamSynchronize(st));
+    l->nnbrefmax = nnbrefmax;
     ARG_CHECK(ctx, total < (1LL << 31), "epsr_fmolec1: %lld candidate pairs exceed the 2^31 item limit", total);
 
     Scratch S;
@@ -607,8 +722,8 @@ extern "C" int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coord
         Scratch q;
         q.base = nullptr;
         q.take<float4>(box->natom); q.take<float4>((size_t)box->npairs * std::max(2, std::min(tb->ld, tb->npt2 + 1)));
-        q.take<int>(4); q.take<int>(l->ncell + 1); q.take<int>(nbflat.size()); q.take<long long>(nmol + 1);
-        q.take<int>(nmol); q.take<float>(total); q.take<int>(total); q.take<int>(total + 1);
+        q.take<int>(4); q.take<int>((size_t)nflat + 1);
+        q.take<float>(total); q.take<int>(total); q.take<int>(total + 1);
         q.take<int>(nmol); q.take<int>(nmol + 1);
         need = q.off;
     }
@@ -617,23 +732,17 @@ extern "C" int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coord
     float4 *dP = S.take<float4>(box->natom);
     float4 *dT = S.take<float4>((size_t)box->npairs * std::max(2, std::min(tb->ld, tb->npt2 + 1)));
     int *dtmp = S.take<int>(4);
-    int *dcstart = S.take<int>(l->ncell + 1);
-    int *dnb = S.take<int>(nbflat.size());
-    long long *drow = S.take<long long>(nmol + 1);
-    int *dmcell = S.take<int>(nmol);
+    int *dnb = S.take<int>((size_t)nflat + 1);
     float *dE = S.take<float>(total);
     int *dflag = S.take<int>(total);
     int *didx = S.take<int>(total + 1);
     int *dsflag = S.take<int>(nmol);
     int *dsidx = S.take<int>(nmol + 1);
-
-    HIP_CHECK(ctx, hipMemcpyAsync(dcstart, cstart.data(), sizeof(int) * (l->ncell + 1), hipMemcpyHostToDevice, st));
-    HIP_CHECK(ctx, hipMemcpyAsync(dnb, nbflat.data(), sizeof(int) * nbflat.size(), hipMemcpyHostToDevice, st));
-    HIP_CHECK(ctx, hipMemcpyAsync(drow, rowoff.data(), sizeof(long long) * (nmol + 1), hipMemcpyHostToDevice, st));
-    HIP_CHECK(ctx, hipMemcpyAsync(dmcell, l->molcellno.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, st));
+    cell_flatten<<<(unsigned)ceil_div(ncn, 256), 256, 0, st>>>(ncell, nbr, l->maxcellno, dnbcell, dnbcnt, dnbpre,
+                                                               dcstart, dnmc, dmic, dnb);
 
     PreparedTables pt;
-    int rc = prepare_tables(ctx, box, tb, dT, dtmp, pt);
+    rc = prepare_tables(ctx, box, tb, dT, dtmp, pt);
     if (rc) return rc;
     abs_positions<<<ceil_div(box->natom, 256), 256, 0, st>>>(c->xa, c->ya, c->za, c->xadd, c->yadd, c->zadd,
                                                              box->d_amol, box->d_itype, dP, box->natom);
@@ -657,9 +766,11 @@ extern "C" int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coord
     if ((rc = hipcub_scan_int(ctx, dflag, didx, (int)total))) return rc;
     flag_nonzero<<<ceil_div(nmol, 256), 256, 0, st>>>(eintra, nmol, dsflag);
     if ((rc = hipcub_scan_int(ctx, dsflag, dsidx, nmol))) return rc;
-    int hlast[2], hflag[2];
-    HIP_CHECK(ctx, hipMemcpyAsync(&hlast[0], didx + total - 1, sizeof(int), hipMemcpyDeviceToHost, st));
-    HIP_CHECK(ctx, hipMemcpyAsync(&hflag[0], dflag + total - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    int hlast[2] = {0, 0}, hflag[2] = {0, 0};
+    if (total > 0) {
+        HIP_CHECK(ctx, hipMemcpyAsync(&hlast[0], didx + total - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIP_CHECK(ctx, hipMemcpyAsync(&hflag[0], dflag + total - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    }
     HIP_CHECK(ctx, hipMemcpyAsync(&hlast[1], dsidx + nmol - 1, sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_CHECK(ctx, hipMemcpyAsync(&hflag[1], dsflag + nmol - 1, sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_CHECK(ctx, hipStreamSynchronize(st));
@@ -1041,10 +1152,11 @@ __global__ void mc_commit_mols(int nmol, const int *__restrict__ order, const in
                                float *__restrict__ zadd, int *__restrict__ nstepreject, int count_rot,
                                int *__restrict__ rej)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nmol) return;
+    const int kk = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = kk < nmol;
+    const int k = valid ? kk : nmol - 1;
     int m = order[k];
-    bool acc = (accb[k >> 5] >> (k & 31)) & 1u;
+    bool acc = valid && ((accb[k >> 5] >> (k & 31)) & 1u);
     if (acc) {
         for (int a = naf[m]; a <= nal[m]; ++a) {
             xa[a] = txa[a];
@@ -1055,11 +1167,22 @@ __global__ void mc_commit_mols(int nmol, const int *__restrict__ order, const in
         yadd[m] = tyadd[m];
         zadd[m] = tzadd[m];
         nstepreject[m] = 0;
-    } else {
+    } else if (valid) {
         nstepreject[m] += 1;
-        int mt = mtype[m];
-        if (count_rot) atomicAdd(&rej[2 * mt], 1);
-        if (toff[k] >= 0) atomicAdd(&rej[2 * mt + 1], 1);
+    }
+    // rejection counters per molecule type, one atomic per type present in the wave
+    const int mt = mtype[m];
+    bool rr = valid && !acc && count_rot, rt = valid && !acc && toff[k] >= 0;
+    unsigned long long live = __ballot(rr || rt);
+    while (live) {
+        const int t = __shfl(mt, __ffsll((long long)live) - 1);
+        const unsigned long long same = __ballot(mt == t);
+        const int nr = __popcll(__ballot(rr && mt == t)), nt = __popcll(__ballot(rt && mt == t));
+        if ((int)(threadIdx.x & 63) == __ffsll((long long)(same & live)) - 1) {
+            if (nr) atomicAdd(&rej[2 * t], nr);
+            if (nt) atomicAdd(&rej[2 * t + 1], nt);
+        }
+        live &= ~same;
     }
 }
 
@@ -1258,48 +1381,69 @@ mc_push_rows(int nmol, int W, const int *__restrict__ order, const int *__restri
     for (int e = fill + lane; e < W; e += 64) out[e] = make_int2(-1, 0);
 }
 
-// exact fdiff of every accepted move (for fadd), one lane per chain position
-__global__ void mc_fdiff_accepted(int nmol, const int *__restrict__ order, const int *__restrict__ pos,
-                                  const int *__restrict__ rowstart, const int *__restrict__ cnt,
-                                  const int *__restrict__ partner, const int *__restrict__ mirror,
-                                  const float *__restrict__ A, const float *__restrict__ Bv,
-                                  const float *__restrict__ L, const unsigned *__restrict__ bits,
-                                  float *__restrict__ fdiff_out)
+// exact fdiff of every accepted move (for fadd): one wave per chain position gathers
+// the selected variant of every slot into LDS, lane 0 sums them in the reference's order
+__global__ void __launch_bounds__(256)
+mc_fdiff_accepted(int nmol, const int *__restrict__ order, const int *__restrict__ pos,
+                  const int *__restrict__ rowstart, const int *__restrict__ cnt, const int *__restrict__ partner,
+                  const int *__restrict__ mirror, const float *__restrict__ A, const float *__restrict__ Bv,
+                  const float *__restrict__ L, const unsigned *__restrict__ bits, int stride,
+                  float *__restrict__ fdiff_out)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    extern __shared__ float2 fterm[];   // [4 waves][stride >= longest list]
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int k = blockIdx.x * 4 + wv;
     if (k >= nmol) return;
     if (!((bits[k >> 5] >> (k & 31)) & 1u)) return;
     const int m = order[k];
     const int row = rowstart[m], c = cnt[m];
-    float fdiff = 0.0f;
-    for (int s = 1; s < c; ++s) {
+    float2 *t = fterm + (size_t)wv * stride;
+    for (int s = 1 + lane; s < c; s += 64) {
         int i = row + s;
         int n = partner[i];
         int pn = pos[n];
         bool a = pn < k && ((bits[pn >> 5] >> (pn & 31)) & 1u);
         int mi = mirror[i];
-        float en = a ? Bv[i] : A[i];
-        float eo = (a && mi != rowstart[n]) ? A[mi] : L[i];
-        fdiff = fdiff + en - eo;
+        t[s] = make_float2(a ? Bv[i] : A[i], (a && mi != rowstart[n]) ? A[mi] : L[i]);
     }
-    fdiff_out[k] = fdiff;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    if (lane == 0) {
+        float fdiff = 0.0f;
+        for (int s = 1; s < c; ++s) {
+            float2 v = t[s];
+            fdiff = fdiff + v.x - v.y;
+        }
+        fdiff_out[k] = fdiff;
+    }
 }
 
 // fadd = fadd + fdiff over the accepted moves in chain order (molfit.py:405), and the
-// number of accepted moves
-__global__ void mc_fadd(int nmol, const unsigned *__restrict__ bits, const float *__restrict__ fdiff,
-                        float *__restrict__ res)
+// number of accepted moves: one wave, 64 chain positions at a time
+__global__ void __launch_bounds__(64)
+mc_fadd(int nmol, const unsigned *__restrict__ bits, const float *__restrict__ fdiff, float *__restrict__ res)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int lane = threadIdx.x;
     float fadd = 0.0f;
     int nacc = 0;
-    for (int k = 0; k < nmol; ++k)
-        if ((bits[k >> 5] >> (k & 31)) & 1u) {
-            fadd = fadd + fdiff[k];
-            nacc++;
+    for (int k0 = 0; k0 < nmol; k0 += 64) {
+        const int k = k0 + lane;
+        const bool a = k < nmol && ((bits[k >> 5] >> (k & 31)) & 1u);
+        const float v = a ? fdiff[k] : 0.0f;
+        unsigned long long msk = __ballot(a);
+        nacc += __popcll(msk);
+        const int vb = __float_as_int(v);
+        while (msk) {
+            const int i = __ffsll((long long)msk) - 1;
+            fadd = fadd + __int_as_float(__builtin_amdgcn_readlane(vb, i));
+            msk &= msk - 1ull;
         }
-    res[0] = fadd;
-    res[1] = (float)nacc;
+    }
+    if (lane == 0) {
+        res[0] = fadd;
+        res[1] = (float)nacc;
+    }
 }
 
 // ------------------------------------------------------------------ single-wave resolver
@@ -1496,7 +1640,7 @@ extern "C" int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
     const size_t lds_cap = 160 * 1024;
     const bool big_ring = lds_need(4096) <= lds_cap;
     const size_t lds_fast = lds_need(big_ring ? 4096 : 2048);
-    const bool fast = nj <= RW_MAXJ && lds_fast <= lds_cap;
+    const bool fast = nj <= RW_MAXJ && lds_fast <= lds_cap && sizeof(float2) * 4 * (size_t)maxc <= lds_cap;
     const int rw = fast ? nj * 64 : 0;
     Scratch S, q;
     auto layout = [&](Scratch &x) {
@@ -1659,8 +1803,9 @@ extern "C" int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
         }
 #undef MC_RESOLVE
         timer_end(ctx, "mc_resolve");
-        mc_fdiff_accepted<<<ceil_div(nmol, 128), 128, 0, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt,
-                                                               l->d_partner, l->d_mirror, dA, dB, dL, dacc, dfdiff);
+        mc_fdiff_accepted<<<ceil_div(nmol, 4), 256, sizeof(float2) * 4 * (size_t)maxc, st>>>(
+            nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner, l->d_mirror, dA, dB, dL, dacc, maxc,
+            dfdiff);
         mc_fadd<<<1, 64, 0, st>>>(nmol, dacc, dfdiff, dres);
         HIP_CHECK(ctx, hipMemcpyAsync(&nexact, dnex, sizeof(int), hipMemcpyDeviceToHost, st));
     } else {
