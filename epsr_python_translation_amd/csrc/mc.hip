@@ -41,7 +41,11 @@ struct epsr_ledger {
     int *d_rowstart = nullptr, *d_cnt = nullptr, *d_order = nullptr, *d_pos = nullptr;
     int *d_partner = nullptr, *d_entry = nullptr, *d_mirror = nullptr, *d_slotmol = nullptr;
     float *d_ergpair = nullptr;
-    int maxpush = 0;   // most later partners a molecule's accepted move updates
+    // pushes of the MC resolver: an accepted move at chain position k updates the energy
+    // change of its later partners outside its block of 64 positions through the row
+    // prow[k] .. prow[k+1] - 1, npush(k) entries padded to whole pieces of 128
+    int maxpush = 0, npushtot = 0, npad = 0;
+    int *d_prow = nullptr;   // [npad + 1]
     size_t cap_slots = 0, cap_entries = 0;
     bool ledger_ready = false;
     std::vector<int> h_rowstart, h_cnt;
@@ -65,7 +69,8 @@ extern "C" int epsr_ledger_create(epsr_ctx *ctx, const epsr_box *box, epsr_ledge
     l->nmol = box->nmol;
     int rc;
     if ((rc = dev_alloc(ctx, &l->d_order, box->nmol)) || (rc = dev_alloc(ctx, &l->d_pos, box->nmol)) ||
-        (rc = dev_alloc(ctx, &l->d_rowstart, box->nmol + 1)) || (rc = dev_alloc(ctx, &l->d_cnt, box->nmol))) {
+        (rc = dev_alloc(ctx, &l->d_rowstart, box->nmol + 1)) || (rc = dev_alloc(ctx, &l->d_cnt, box->nmol)) ||
+        (rc = dev_alloc(ctx, &l->d_prow, (box->nmol + 63) / 64 * 64 + 1))) {
         epsr_ledger_destroy(l);
         return rc;
     }
@@ -79,6 +84,7 @@ extern "C" void epsr_ledger_destroy(epsr_ledger *l)
     hipFree(l->d_rowstart); hipFree(l->d_cnt); hipFree(l->d_order); hipFree(l->d_pos);
     hipFree(l->d_partner); hipFree(l->d_entry); hipFree(l->d_mirror); hipFree(l->d_slotmol);
     hipFree(l->d_ergpair);
+    hipFree(l->d_prow);
     delete l;
 }
 
@@ -494,6 +500,10 @@ __device__ __forceinline__ int push_target(int i, int k, const int *__restrict__
     return (pm > k && mi != rowstart[m]) ? pm : -1;
 }
 
+// a push target outside the block of 64 chain positions of k (in-block pushes travel
+// through the resolver's registers)
+__device__ __forceinline__ bool far_push(int t, int k) { return t >= 0 && (t >> 6) != (k >> 6); }
+
 __global__ void count_pushes(int nslots, const int *__restrict__ slotmol, const int *__restrict__ partner,
                              const int *__restrict__ mirror, const int *__restrict__ pos,
                              const int *__restrict__ rowstart, int *__restrict__ npush)
@@ -501,7 +511,14 @@ __global__ void count_pushes(int nslots, const int *__restrict__ slotmol, const 
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nslots) return;
     int k = pos[slotmol[i]];
-    if (push_target(i, k, partner, mirror, pos, rowstart) >= 0) atomicAdd(&npush[k], 1);
+    if (far_push(push_target(i, k, partner, mirror, pos, rowstart), k)) atomicAdd(&npush[k], 1);
+}
+
+// row lengths: npush rounded up to whole pieces of 128 entries (1 KB)
+__global__ void piece_counts(const int *__restrict__ a, int n, int npad, int *__restrict__ out)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < npad + 1) out[t] = t < n ? (a[t] + 127) & ~127 : 0;
 }
 
 __global__ void max_int(const int *__restrict__ a, int n, int *__restrict__ out)
@@ -824,16 +841,20 @@ extern "C" int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coord
     fmolec1_rows<<<ceil_div(nrec, 256), 256, 0, st>>>(skey, sval, nrec, nself, pim, pjm, pE, eintra, l->d_rowstart,
                                                       l->d_partner, l->d_entry, l->d_slotmol, spos, l->d_ergpair);
     fmolec1_mirror<<<ceil_div(nrec, 256), 256, 0, st>>>(sval, nrec, nself, spos, l->d_mirror);
-    // longest push list of the MC resolver
+    // push rows of the MC resolver: counts, even-aligned row offsets, longest row
+    const int npad = (nmol + 63) / 64 * 64;
     int *dpush = nullptr;
-    HIP_CHECK(ctx, hipMallocAsync((void **)&dpush, sizeof(int) * (nmol + 1), st));
-    HIP_CHECK(ctx, hipMemsetAsync(dpush, 0, sizeof(int) * (nmol + 1), st));
+    HIP_CHECK(ctx, hipMallocAsync((void **)&dpush, sizeof(int) * (2 * (size_t)npad + 4), st));
+    guard.p.push_back(dpush);
+    HIP_CHECK(ctx, hipMemsetAsync(dpush, 0, sizeof(int) * (2 * (size_t)npad + 4), st));
     count_pushes<<<ceil_div(nrec, 256), 256, 0, st>>>(nrec, l->d_slotmol, l->d_partner, l->d_mirror, l->d_pos,
                                                       l->d_rowstart, dpush);
-    max_int<<<std::max(1, std::min(64, ceil_div(nmol, 256))), 256, 0, st>>>(dpush, nmol, dpush + nmol);
-    int maxpush = 0;
-    HIP_CHECK(ctx, hipMemcpyAsync(&maxpush, dpush + nmol, sizeof(int), hipMemcpyDeviceToHost, st));
-    HIP_CHECK(ctx, hipFreeAsync(dpush, st));
+    max_int<<<std::max(1, std::min(64, ceil_div(nmol, 256))), 256, 0, st>>>(dpush, nmol, dpush + 2 * npad + 2);
+    piece_counts<<<ceil_div(npad + 1, 256), 256, 0, st>>>(dpush, nmol, npad, dpush + npad + 1);
+    if ((rc = hipcub_scan_int(ctx, dpush + npad + 1, l->d_prow, npad + 1))) return rc;
+    int maxpush = 0, npushtot = 0;
+    HIP_CHECK(ctx, hipMemcpyAsync(&maxpush, dpush + 2 * npad + 2, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(ctx, hipMemcpyAsync(&npushtot, l->d_prow + npad, sizeof(int), hipMemcpyDeviceToHost, st));
     // FP64 totals
     double *dsum = nullptr;
     HIP_CHECK(ctx, hipMallocAsync((void **)&dsum, 2 * sizeof(double), st));
@@ -860,6 +881,8 @@ extern "C" int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coord
     ARG_CHECK(ctx, maxcnt <= l->nnbrefmax, "epsr_fmolec1: a ledger list of %d exceeds nnbrefmax %d", maxcnt,
               l->nnbrefmax);
     l->maxpush = maxpush;
+    l->npushtot = npushtot;
+    l->npad = npad;
     l->nself = nself;
     l->npairent = npair;
     l->nslots = nrec;
@@ -1273,13 +1296,16 @@ __device__ void mc_thresholds(double b, double bnd, double tempd, double nlr, fl
     fr = f_up(vr + bnd - b + slack);
 }
 
-// one wave per molecule: the deltas of its slots and the decision thresholds of its
-// move, written to the head of the chain position's push row
+// one wave per molecule: the deltas of its slots, the decision thresholds of its move
+// (thr, by chain position) and the deltas its earlier partners in the same block of 64
+// chain positions push to it: Mt[block][i][j] for partner at i, molecule at j (zero
+// elsewhere)
 __global__ void __launch_bounds__(64)
 mc_prepare(int nmol, const int *__restrict__ pos, const int *__restrict__ rowstart, const int *__restrict__ cnt,
            const int *__restrict__ partner, const int *__restrict__ mirror, const float *__restrict__ A,
            const float *__restrict__ Bv, const float *__restrict__ L, const float *__restrict__ stream,
-           const int *__restrict__ moff, float temp, int W, float *__restrict__ delta, int2 *__restrict__ rows)
+           const int *__restrict__ moff, float temp, float *__restrict__ delta, float2 *__restrict__ thr,
+           float *__restrict__ Mt)
 {
     // Each rounding of the sequential sum errs by at most u |partial sum|.  With
     // P_s = |sum_{s'<=s} e0| + sum_{s'<=s, earlier partner} |delta| the partial sums obey
@@ -1312,6 +1338,8 @@ mc_prepare(int nmol, const int *__restrict__ pos, const int *__restrict__ rowsta
                 maxnew = fmaxf(maxnew, fabsf(bv));
                 ad = fabs((double)dl);
                 npb++;
+                const int pn = pos[n];
+                if ((pn >> 6) == (pm >> 6)) Mt[(size_t)(pm >> 6) * 4096 + (pn & 63) * 64 + (pm & 63)] = dl;
             }
             delta[i] = dl;
         }
@@ -1342,43 +1370,40 @@ mc_prepare(int nmol, const int *__restrict__ pos, const int *__restrict__ rowsta
         const double nlr = -log((double)stream[moff[pm]]);
         float fa, fr;
         mc_thresholds(carry_e, bnd, (double)temp, nlr, fa, fr);
-        rows[(size_t)pm * W] = make_int2(__float_as_int(fa), __float_as_int(fr));
+        thr[pm] = make_float2(fa, fr);
     }
 }
 
-// one wave per chain position k: entries 1.. of row k list (later partner's chain
-// position, delta of the partner's slot) for every pair an accepted move at k updates;
-// the rest of the row holds (-1, 0).  The row width W exceeds the longest list.  Rows
-// nmol.. (grid beyond nmol) pad the chain to a whole number of resolver rings.
+// one wave per chain position k: its push row (later partner's chain position, delta of
+// the partner's slot) for the pairs an accepted move at k updates outside its block,
+// at prow[k]; chain positions nmol.. (grid up to the padded length) get thresholds that
+// reject, so the resolver can walk whole blocks
 __global__ void __launch_bounds__(64)
-mc_push_rows(int nmol, int W, const int *__restrict__ order, const int *__restrict__ pos,
+mc_push_rows(int nmol, const int *__restrict__ order, const int *__restrict__ pos,
              const int *__restrict__ rowstart, const int *__restrict__ cnt, const int *__restrict__ partner,
-             const int *__restrict__ mirror, const float *__restrict__ delta, int2 *__restrict__ rows)
+             const int *__restrict__ mirror, const float *__restrict__ delta, const int *__restrict__ prow,
+             int2 *__restrict__ pent, float2 *__restrict__ thr)
 {
     const int k = blockIdx.x;
     const int lane = threadIdx.x;
-    int2 *out = rows + (size_t)k * W;
     if (k >= nmol) {
-        // padding row past the chain's end: rejected, no pushes
-        const int ninf = __float_as_int(-INFINITY);
-        for (int e = lane; e < W; e += 64) out[e] = e == 0 ? make_int2(ninf, ninf) : make_int2(-1, 0);
+        if (lane == 0) thr[k] = make_float2(-INFINITY, -INFINITY);
         return;
     }
     const int n = order[k];
     const int row = rowstart[n], c = cnt[n];
-    int fill = 1;
+    int2 *out = pent + prow[k];
+    int fill = 0;
     for (int s0 = 0; s0 < c; s0 += 64) {
         const int s = s0 + lane;
         int t = -1;
         if (s < c) t = push_target(row + s, k, partner, mirror, pos, rowstart);
-        const unsigned long long b = __ballot(t >= 0);
-        if (t >= 0) {
-            int idx = fill + __popcll(b & ((1ull << lane) - 1ull));
-            out[idx] = make_int2(t, __float_as_int(delta[mirror[row + s]]));
-        }
+        const bool f = far_push(t, k);
+        const unsigned long long b = __ballot(f);
+        if (f) out[fill + __popcll(b & ((1ull << lane) - 1ull))] = make_int2(t, __float_as_int(delta[mirror[row + s]]));
         fill += __popcll(b);
     }
-    for (int e = fill + lane; e < W; e += 64) out[e] = make_int2(-1, 0);
+    for (int e = fill + lane, end = prow[k + 1] - prow[k]; e < end; e += 64) out[e] = make_int2(-1, 0);
 }
 
 // exact fdiff of every accepted move (for fadd): one wave per chain position gathers
@@ -1446,106 +1471,194 @@ mc_fadd(int nmol, const unsigned *__restrict__ bits, const float *__restrict__ f
     }
 }
 
-// ------------------------------------------------------------------ single-wave resolver
-// One wave64 walks the chain.  LDS operations of one wave complete in order, so the
-// float32 pushes of an accepted move reach accde before any later read, with no
-// barrier.  The push rows stream through an LDS ring of two chunks of CH rows: while
-// chunk c is decided, chunk c+1 sits in the other half and chunk c+2 is in flight in
-// registers, so the chain never waits on global memory.  Lane 0 of entry 0 of a row
-// carries the move's thresholds.
-#define RW_MAXJ 8
+// ------------------------------------------------------------------ block resolver
+// One wave64 walks the chain in blocks of 64 chain positions, lane j holding move kb+j:
+// its pushed sum acc (from LDS accde, which holds the pushes of earlier blocks), its two
+// thresholds and, in registers, the 64 deltas its earlier partners inside the block
+// push to it.  The 64 decisions of a block are then a chain of register operations:
+// decide move i from lane i's acc; if accepted, every lane adds its delta from i.
+// After the block, the rows of its accepted moves are copied into LDS (direct
+// global->LDS loads) and pushed into accde with LDS atomics.  LDS operations of one
+// wave complete in order, so the next block's reads see every push.
 
-template <int NJ, int CH>
-__global__ void __launch_bounds__(64)
-mc_resolve_rows(int nmol, const int *__restrict__ order, const int *__restrict__ pos,
-                const int *__restrict__ rowstart, const int *__restrict__ cnt, const int *__restrict__ partner,
-                const int *__restrict__ mirror, const float *__restrict__ A, const float *__restrict__ Bv,
-                const float *__restrict__ L, const int2 *__restrict__ rows, const float *__restrict__ stream,
-                const int *__restrict__ moff, float temp, unsigned *__restrict__ acc_out, int *__restrict__ nexact_out)
+// the reference's decision for move k from the exact sequential float32 sum: the wave
+// gathers the selected variant of every slot, lane 0 sums them in order (out of line:
+// rare, and it keeps the unrolled block walk small)
+__device__ __noinline__ int mc_exact_decide(int k, const int *__restrict__ order, const int *__restrict__ pos,
+                                            const int *__restrict__ rowstart, const int *__restrict__ cnt,
+                                            const int *__restrict__ partner, const int *__restrict__ mirror,
+                                            const float *__restrict__ A, const float *__restrict__ Bv,
+                                            const float *__restrict__ L, const float *__restrict__ stream,
+                                            const int *__restrict__ moff, float temp, const unsigned *bits,
+                                            float *term)
 {
-    // rows nmol..npad-1 (npad a multiple of CH) are padding: rejected, no pushes
-    constexpr int W = NJ * 64;
-    extern __shared__ int2 smr[];
-    const int npad = (nmol + CH - 1) / CH * CH;
-    const int nchunk = npad / CH;
-    const int nw = (npad + 31) >> 5;
-    int2 *ring = smr;                                   // [2][CH * W]
-    float *accde = (float *)(ring + 2 * CH * W);        // [npad]
-    unsigned *bits = (unsigned *)(accde + npad);        // [nw]
-    float *term = (float *)(bits + nw);                 // [2 * longest list]
     const int lane = threadIdx.x;
-    for (int i = lane; i < nw; i += 64) bits[i] = 0u;
-    for (int i = lane; i < npad; i += 64) accde[i] = 0.0f;
-    int nexact = 0;
-
-    int2 stage[CH][NJ];
-    auto fetch = [&](int c) {
-        const int2 *src = rows + (size_t)c * CH * W + lane;
-#pragma unroll
-        for (int r = 0; r < CH; ++r)
-#pragma unroll
-            for (int j = 0; j < NJ; ++j) stage[r][j] = src[r * W + 64 * j];
-    };
-    auto put = [&](int half) {
-        int2 *dst = ring + half * CH * W + lane;
-#pragma unroll
-        for (int r = 0; r < CH; ++r)
-#pragma unroll
-            for (int j = 0; j < NJ; ++j) dst[r * W + 64 * j] = stage[r][j];
-    };
-    fetch(0);
-    put(0);
-    if (nchunk > 1) fetch(1);
+    const int m = order[k];
+    const int row = rowstart[m], c = cnt[m];
+    for (int s = 1 + lane; s < c; s += 64) {
+        int i = row + s;
+        int n = partner[i];
+        int pn = pos[n];
+        bool acc = pn < k && ((bits[pn >> 5] >> (pn & 31)) & 1u);
+        int mi = mirror[i];
+        term[2 * s] = acc ? Bv[i] : A[i];
+        term[2 * s + 1] = (acc && mi != rowstart[n]) ? A[mi] : L[i];
+    }
     __syncthreads();
-    for (int c = 0; c < nchunk; ++c) {
-        if (c + 1 < nchunk) {
-            put((c + 1) & 1);
-            if (c + 2 < nchunk) fetch(c + 2);
-        }
-        const int2 *rb = ring + (c & 1) * CH * W + lane;
-        for (int r = 0; r < CH; ++r) {
-            const int k = c * CH + r;
-            int2 e[NJ];
+    int dd = 0;
+    if (lane == 0) {
+        float fdiff = 0.0f;
+        for (int s = 1; s < c; ++s) fdiff = fdiff + term[2 * s] - term[2 * s + 1];
+        float ediff = (fdiff + 0.0f) * temp;
+        dd = stream[moff[k]] < etest_of(ediff) ? 1 : 0;
+    }
+    dd = __shfl(dd, 0);
+    __syncthreads();
+    return dd;
+}
+
+struct BlockIn {
+    float M[64];   // delta pushed to this lane's move by in-block move i (0 if none)
+    float fa, fr;  // thresholds
+    int rp, rc;    // push row offset and length
+};
+
+__device__ __forceinline__ void mc_block_fetch(int b, const float2 *__restrict__ thr, const float *__restrict__ Mt,
+                                               const int *__restrict__ prow, BlockIn &x)
+{
+    const int lane = threadIdx.x;
+    const int k = b * 64 + lane;
+    const float2 t = thr[k];
+    x.fa = t.x;
+    x.fr = t.y;
+    x.rp = prow[k];
+    x.rc = prow[k + 1] - x.rp;
+    const float *src = Mt + (size_t)b * 4096 + lane;
 #pragma unroll
-            for (int j = 0; j < NJ; ++j) e[j] = rb[r * W + 64 * j];
-            const float a = accde[k];
-            const float fa = __int_as_float(__builtin_amdgcn_readfirstlane(e[0].x));
-            const float fr = __int_as_float(__builtin_amdgcn_readfirstlane(e[0].y));
-            int dec = a <= fa ? 1 : (a >= fr ? 0 : -1);
-            if (dec < 0) {
-                // exact: gather the selected variant of every slot, lane 0 sums in order
-                const int m = order[k];
-                const int row = rowstart[m], cm = cnt[m];
-                for (int s = 1 + lane; s < cm; s += 64) {
-                    int i = row + s;
-                    int n = partner[i];
-                    int pn = pos[n];
-                    bool acc = pn < k && ((bits[pn >> 5] >> (pn & 31)) & 1u);
-                    int mi = mirror[i];
-                    term[2 * s] = acc ? Bv[i] : A[i];
-                    term[2 * s + 1] = (acc && mi != rowstart[n]) ? A[mi] : L[i];
-                }
-                __syncthreads();
-                int dd = 0;
+    for (int i = 0; i < 64; ++i) x.M[i] = src[i * 64];
+}
+
+__global__ void __launch_bounds__(64)
+mc_resolve_blocks(int nmol, int npad, const int *__restrict__ order, const int *__restrict__ pos,
+                  const int *__restrict__ rowstart, const int *__restrict__ cnt, const int *__restrict__ partner,
+                  const int *__restrict__ mirror, const float *__restrict__ A, const float *__restrict__ Bv,
+                  const float *__restrict__ L, const float2 *__restrict__ thr, const float *__restrict__ Mt,
+                  const int *__restrict__ prow, const int2 *__restrict__ pent, const float *__restrict__ stream,
+                  const int *__restrict__ moff, float temp, int cap, unsigned *__restrict__ acc_out,
+                  int *__restrict__ nexact_out)
+{
+    extern __shared__ int4 smq[];
+    int2 *stg = (int2 *)smq;                       // [cap] staged push rows
+    float *accde = (float *)(stg + cap);           // [npad]
+    unsigned *bits = (unsigned *)(accde + npad);   // [npad / 32]
+    float *term = (float *)(bits + npad / 32);     // [2 * longest list]
+    const int lane = threadIdx.x;
+    for (int i = lane; i < npad / 32; i += 64) bits[i] = 0u;
+    for (int i = lane; i < npad; i += 64) accde[i] = 0.0f;
+    const int nblk = npad / 64;
+    int nexact = 0;
+    BlockIn x0, x1;
+    mc_block_fetch(0, thr, Mt, prow, x0);
+    __syncthreads();
+
+    // Stage (direct global->LDS copies) the push rows of the moves in cand, lowest first,
+    // while they fit in the cap entries of the staging area; lane i of soff receives the
+    // staging offset of move i.  Returns the staged moves.
+    auto stage = [&](const BlockIn &x, unsigned long long cand, int &soff) -> unsigned long long {
+        unsigned long long got = 0ull;
+        int so = 0;
+        while (cand) {
+            const int i = __ffsll((long long)cand) - 1;
+            const int c = __builtin_amdgcn_readlane(x.rc, i);
+            if (so + c > cap) break;
+            const int st = __builtin_amdgcn_readlane(x.rp, i);
+            for (int p = 0; p < c; p += 128)
+                __builtin_amdgcn_global_load_lds(
+                    (const void __attribute__((address_space(1))) *)(pent + st + p + 2 * lane),
+                    (void __attribute__((address_space(3))) *)(stg + so + p), 16, 0, 0);
+            soff = lane == i ? so : soff;
+            so += c;
+            got |= 1ull << i;
+            cand &= cand - 1ull;
+        }
+        return got;
+    };
+    // push the staged rows of the moves in msk into accde: the pieces of the next move
+    // are read before the atomics of the current one are issued
+    auto push = [&](const BlockIn &x, unsigned long long msk, int soff) {
+        auto rd = [&](int i, int4 (&e)[4]) {
+            const int c = __builtin_amdgcn_readlane(x.rc, i), o = __builtin_amdgcn_readlane(soff, i);
+            const int4 *src = (const int4 *)(stg + o) + lane;
+#pragma unroll
+            for (int p = 0; p < 4; ++p) e[p] = p * 128 < c ? src[p * 64] : make_int4(-1, 0, -1, 0);
+        };
+        auto apply = [&](const int4 (&e)[4]) {
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                if (e[p].x >= 0) atomicAdd(&accde[e[p].x], __int_as_float(e[p].y));
+                if (e[p].z >= 0) atomicAdd(&accde[e[p].z], __int_as_float(e[p].w));
+            }
+        };
+        if (!msk) return;
+        int4 cur[4], nxt[4];
+        rd(__ffsll((long long)msk) - 1, cur);
+        msk &= msk - 1ull;
+        while (msk) {
+            rd(__ffsll((long long)msk) - 1, nxt);
+            msk &= msk - 1ull;
+            apply(cur);
+#pragma unroll
+            for (int p = 0; p < 4; ++p) cur[p] = nxt[p];
+        }
+        apply(cur);
+    };
+
+    auto walk = [&](int b, const BlockIn &x, BlockIn &xn) {
+        // this block's inputs arrived during the previous block; with the counters at
+        // zero the next block's loads overlap the walk
+        __builtin_amdgcn_s_waitcnt(0);
+        if (b + 1 < nblk) mc_block_fetch(b + 1, thr, Mt, prow, xn);
+        int soff = 0;
+        const int kb = b * 64;
+        float acc = accde[kb + lane];
+        unsigned long long am = 0ull;
+#pragma unroll
+        for (int i = 0; i < 64; ++i) {
+            const unsigned long long ya = __ballot(acc <= x.fa);
+            const unsigned long long yr = __ballot(acc >= x.fr);
+            int dec = (int)((ya >> i) & 1ull);
+            if (!dec && !((yr >> i) & 1ull)) {
                 if (lane == 0) {
-                    float fdiff = 0.0f;
-                    for (int s = 1; s < cm; ++s) fdiff = fdiff + term[2 * s] - term[2 * s + 1];
-                    float ediff = (fdiff + 0.0f) * temp;
-                    dd = stream[moff[k]] < etest_of(ediff) ? 1 : 0;
+                    bits[kb >> 5] = (unsigned)am;
+                    bits[(kb >> 5) + 1] = (unsigned)(am >> 32);
                 }
-                dec = __shfl(dd, 0);
+                dec = __builtin_amdgcn_readfirstlane(mc_exact_decide(kb + i, order, pos, rowstart, cnt, partner,
+                                                                     mirror, A, Bv, L, stream, moff, temp, bits,
+                                                                     term));
                 nexact++;
-                __syncthreads();
             }
             if (dec) {
-                if (lane == 0) bits[k >> 5] |= 1u << (k & 31);
-#pragma unroll
-                for (int j = 0; j < NJ; ++j) {
-                    const int t = e[j].x;
-                    if (t >= 0 && (j > 0 || lane > 0)) atomicAdd(&accde[t], __int_as_float(e[j].y));
-                }
+                acc += x.M[i];
+                am |= 1ull << i;
             }
         }
+        if (lane == 0) {
+            bits[kb >> 5] = (unsigned)am;
+            bits[(kb >> 5) + 1] = (unsigned)(am >> 32);
+        }
+        // the rows of the accepted moves: staged and pushed in batches that fit
+        unsigned long long rest = am;
+        while (rest) {
+            __builtin_amdgcn_s_waitcnt(0);   // staging reads done before it is refilled
+            const unsigned long long got = stage(x, rest, soff);
+            __builtin_amdgcn_s_waitcnt(0);
+            push(x, got, soff);
+            rest &= ~got;
+        }
+    };
+    for (int b = 0; b < nblk; b += 2) {
+        walk(b, x0, x1);
+        if (b + 1 < nblk) walk(b + 1, x1, x0);
     }
     __syncthreads();
     for (int i = lane; i < (nmol + 31) >> 5; i += 64) acc_out[i] = bits[i];
@@ -1624,24 +1737,23 @@ extern "C" int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
     std::vector<float> hstream(total);
     for (int i = 0; i < total; ++i) hstream[i] = ran1_next(rng);
 
-    // device buffers; push rows of rw entries for the fast resolver
+    // device buffers.  The block resolver's LDS: staged push rows (cap entries, a multiple
+    // of 128 that holds the longest row), accde and acceptance bits over the padded chain,
+    // and the terms of an exact sum.
     const int nslots = l->nslots;
     int maxc = 1;
     for (int v : l->h_cnt) maxc = std::max(maxc, v);
-    int nj = 1;
-    while (nj * 64 < l->maxpush + 1) nj *= 2;
-    // the resolver's LDS: a ring of 2 chunks x (CH rows of rw entries), CH * rw = 4096
-    // entries (64 KB) or, when that does not fit, 2048; accde, acceptance bits, terms
-    const int npad = (nmol + 63) / 64 * 64;   // a whole number of chunks
-    auto lds_need = [&](int ring_entries) {
-        return sizeof(int2) * 2 * (size_t)ring_entries + sizeof(float) * (size_t)npad +
-               sizeof(unsigned) * (npad / 32) + sizeof(float) * 2 * (size_t)maxc + 64;
-    };
+    const int npad = l->npad;
     const size_t lds_cap = 160 * 1024;
-    const bool big_ring = lds_need(4096) <= lds_cap;
-    const size_t lds_fast = lds_need(big_ring ? 4096 : 2048);
-    const bool fast = nj <= RW_MAXJ && lds_fast <= lds_cap && sizeof(float2) * 4 * (size_t)maxc <= lds_cap;
-    const int rw = fast ? nj * 64 : 0;
+    const size_t lds_rest = sizeof(float) * (size_t)npad + sizeof(unsigned) * (npad / 32) +
+                            sizeof(float) * 2 * (size_t)maxc + 64;
+    const int cap_min = (l->maxpush + 127) / 128 * 128 + 128;
+    int cap = 0;
+    if (lds_rest < lds_cap) cap = (int)std::min<size_t>(8192, (lds_cap - lds_rest) / sizeof(int2) / 128 * 128);
+    // the block resolver reads at most 4 pieces (512 entries) of a push row
+    const bool fast = l->maxpush <= 512 && cap >= cap_min && sizeof(float2) * 4 * (size_t)maxc <= lds_cap;
+    const size_t lds_fast = sizeof(int2) * (size_t)cap + lds_rest;
+    const size_t nent = fast ? (size_t)l->npushtot + 256 : 0;
     Scratch S, q;
     auto layout = [&](Scratch &x) {
         x.take<float4>(natom); x.take<float4>(natom);
@@ -1653,7 +1765,8 @@ extern "C" int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
         x.take<float>(nmol); x.take<float>(nmol); x.take<float>(nmol);
         x.take<float>(nslots); x.take<float>(nslots); x.take<float>(nslots);
         x.take<unsigned>((nmol + 31) / 32); x.take<float>(nmol); x.take<float>(4); x.take<int>(2 * nmt + 2);
-        x.take<float>(nslots); x.take<int>(4); x.take<int2>((size_t)npad * rw);
+        x.take<float>(nslots); x.take<int>(4);
+        x.take<float2>(fast ? npad : 0); x.take<float>(fast ? (size_t)npad * 64 : 0); x.take<int2>(nent);
     };
     layout(q);
     S.base = (char *)ctx_scratch(ctx, q.off);
@@ -1673,7 +1786,9 @@ extern "C" int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
     int *drej = S.take<int>(2 * nmt + 2);
     float *ddelta = S.take<float>(nslots);
     int *dnex = S.take<int>(4);
-    int2 *drows = S.take<int2>((size_t)npad * rw);
+    float2 *dthr = S.take<float2>(fast ? npad : 0);
+    float *dMt = S.take<float>(fast ? (size_t)npad * 64 : 0);
+    int2 *dpent = S.take<int2>(nent);
 
     PreparedTables pt;
     int rc = prepare_tables(ctx, box, tb, dT, dtmp, pt);
@@ -1781,27 +1896,15 @@ extern "C" int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
     timer_end(ctx, "mc_pair");
     int nexact = nmol;
     if (fast) {
+        HIP_CHECK(ctx, hipMemsetAsync(dMt, 0, sizeof(float) * (size_t)npad * 64, st));
         mc_prepare<<<nmol, 64, 0, st>>>(nmol, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner, l->d_mirror, dA, dB, dL,
-                                        dstream, dmoff, temp, rw, ddelta, drows);
-        mc_push_rows<<<npad, 64, 0, st>>>(nmol, rw, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner,
-                                          l->d_mirror, ddelta, drows);
+                                        dstream, dmoff, temp, ddelta, dthr, dMt);
+        mc_push_rows<<<npad, 64, 0, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner,
+                                          l->d_mirror, ddelta, l->d_prow, dpent, dthr);
         timer_begin(ctx, "mc_resolve");
-#define MC_RESOLVE(NJ, CH)                                                                                           \
-    mc_resolve_rows<NJ, CH><<<1, 64, lds_fast, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt,        \
-                                                     l->d_partner, l->d_mirror, dA, dB, dL, drows, dstream, dmoff, \
-                                                     temp, dacc, dnex)
-        if (big_ring) {
-            if (nj == 1) MC_RESOLVE(1, 64);
-            else if (nj == 2) MC_RESOLVE(2, 32);
-            else if (nj == 4) MC_RESOLVE(4, 16);
-            else MC_RESOLVE(8, 8);
-        } else {
-            if (nj == 1) MC_RESOLVE(1, 32);
-            else if (nj == 2) MC_RESOLVE(2, 16);
-            else if (nj == 4) MC_RESOLVE(4, 8);
-            else MC_RESOLVE(8, 4);
-        }
-#undef MC_RESOLVE
+        mc_resolve_blocks<<<1, 64, lds_fast, st>>>(nmol, npad, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt,
+                                                   l->d_partner, l->d_mirror, dA, dB, dL, dthr, dMt, l->d_prow, dpent,
+                                                   dstream, dmoff, temp, cap, dacc, dnex);
         timer_end(ctx, "mc_resolve");
         mc_fdiff_accepted<<<ceil_div(nmol, 4), 256, sizeof(float2) * 4 * (size_t)maxc, st>>>(
             nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner, l->d_mirror, dA, dB, dL, dacc, maxc,
