@@ -127,7 +127,20 @@ SIGNATURES = {
                                     vp, C.c_float, ip]),
     "epsr_foqols": (C.c_int, [vp, C.c_int, vp, C.c_int, vp, vp, vp, C.c_float, C.c_float, C.c_float, C.c_float,
                               C.c_float, C.c_float, vp, C.c_int]),
+    "epsr_expand_all": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, C.c_float, vp]),
+    "epsr_smoop": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp]),
+    "epsr_fit_expansion": (C.c_int, [vp, C.c_int, vp, C.c_int, vp, C.c_longlong, vp, C.c_int, vp, vp, C.c_int, fp,
+                                     fp, fp, fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, i32p, C.c_int,
+                                     ip]),
+    "epsr_ptgnsq_diff": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, vp, C.c_int, vp, fp, fp, ip, ip, C.c_float,
+                                   vp]),
+    "epsr_ptgnsq_rfactor": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, fp]),
+    "epsr_ptgnsq_update": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, fp, ip, fp, fp, C.c_int, C.c_float, fp, fp,
+                                     C.c_int]),
 }
+
+FIT_MC = 0
+FIT_NORMAL = 1
 
 
 def _declare(L):

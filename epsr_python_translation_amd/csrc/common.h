@@ -69,6 +69,29 @@ int set_error(epsr_ctx *ctx, int code, const char *fmt, ...);
 void *ctx_scratch(epsr_ctx *ctx, size_t bytes);
 void *ctx_pinned(epsr_ctx *ctx, size_t bytes);
 
+// stream-ordered temporaries of one call, released (stream-ordered) on every return path
+struct StreamTemp {
+    epsr_ctx *ctx;
+    std::vector<void *> p;
+    explicit StreamTemp(epsr_ctx *c) : ctx(c) {}
+    StreamTemp(const StreamTemp &) = delete;
+    StreamTemp &operator=(const StreamTemp &) = delete;
+    void *alloc(size_t bytes)
+    {
+        void *q = nullptr;
+        if (hipMallocAsync(&q, bytes > 0 ? bytes : 1, ctx->stream) != hipSuccess) {
+            set_error(ctx, EPSR_ERR_HIP, "hipMallocAsync of %zu bytes failed", bytes);
+            return nullptr;
+        }
+        p.push_back(q);
+        return q;
+    }
+    ~StreamTemp()
+    {
+        for (void *q : p) (void)hipFreeAsync(q, ctx->stream);
+    }
+};
+
 // kernel timing with HIP events on the launch stream
 void timer_begin(epsr_ctx *ctx, const char *family);
 void timer_end(epsr_ctx *ctx, const char *family);

@@ -17,6 +17,9 @@ from . import _native
 F32 = np.float32
 
 
+HIP_STREAM_PER_THREAD = 2   # hipStreamPerThread
+
+
 class EpsrError(RuntimeError):
     pass
 
@@ -44,8 +47,12 @@ class Context:
             self.lib.epsr_ctx_enable_timing(self.h, 1)
 
     def set_stream(self, stream):
+        # torch names its default stream 0, and its work on it is ordered with HIP's
+        # per-thread default stream (handle 2), not with the legacy null stream that
+        # handle 0 means to the library
         self.stream = stream
-        self.check(self.lib.epsr_ctx_set_stream(self.h, C.c_void_p(stream.cuda_stream)))
+        handle = stream.cuda_stream if stream.cuda_stream != 0 else HIP_STREAM_PER_THREAD
+        self.check(self.lib.epsr_ctx_set_stream(self.h, C.c_void_p(handle)))
 
     def check(self, rc):
         if rc != 0:
@@ -333,3 +340,98 @@ def molfit(ctx: Context, box: DeviceBox, tables: DeviceTables, ledger: Ledger, m
                                   C.byref(mc.stats)))
     mc.fadds = F32(fad.value)
     return mc
+
+
+# ---------------------------------------------------------------- EP fit
+
+SUMMAX = F32(np.exp(25.0))   # expf(25) (epsr_standalone_rev3.py:4722)
+
+
+def expand_all(ctx: Context, nterms, npt, expr, coeff, summax=SUMMAX, out=None):
+    """expand(nterms, j, expr, coeff, summax) for j = 1..npt (dev tensors, 1-based)."""
+    if out is None:
+        out = ctx.zeros(npt + 1)
+    ctx.check(ctx.lib.epsr_expand_all(ctx.h, int(nterms), int(npt), _ptr(expr), _ptr(coeff), F32(summax),
+                                      _ptr(out)))
+    return out
+
+
+def smoop(ctx: Context, nqmin, nqmax, nsmoo, diff):
+    """auxiliary_routines.smoop on a device tensor, in place."""
+    ctx.check(ctx.lib.epsr_smoop(ctx.h, int(nqmin), int(nqmax), int(nsmoo), _ptr(diff)))
+    return diff
+
+
+class FitResult:
+    def __init__(self, stepr, acceptr, rejrat, chisqnew, ntries):
+        self.stepr, self.acceptr, self.rejrat, self.chisqnew, self.ntries = stepr, acceptr, rejrat, chisqnew, ntries
+
+
+def fit_expansion(ctx: Context, yin, expr, yout, wt, wtold, stepr, acceptr, rejrat, chisqnew, nmin, npt, ncfirst,
+                  nterms, nits, nfreq, ran1_state, mode=_native.FIT_MC, nchain=1, ldy=None, lde=0, ldo=None,
+                  ldw=None):
+    """fit_expansion for nchain chains (device tensors yin/expr/yout/wt/wtold, chain c at
+    c*ld).  stepr/acceptr/rejrat/chisqnew: scalars or arrays of nchain; ran1_state int32
+    [nchain*35], updated in place.  Returns FitResult with float32 arrays."""
+    ldy = ldy or npt + 1
+    ldo = ldo or npt + 1
+    ldw = ldw or nterms + 1
+    st = np.ascontiguousarray(np.broadcast_to(np.asarray(stepr, F32), (nchain,)), dtype=F32).copy()
+    ac = np.ascontiguousarray(np.broadcast_to(np.asarray(acceptr, F32), (nchain,)), dtype=F32).copy()
+    rj = np.ascontiguousarray(np.broadcast_to(np.asarray(rejrat, F32), (nchain,)), dtype=F32).copy()
+    cs = np.ascontiguousarray(np.broadcast_to(np.asarray(chisqnew, F32), (nchain,)), dtype=F32).copy()
+    nt = np.zeros(nchain, np.int32)
+    if ran1_state is None:
+        r1p = None
+    else:
+        assert ran1_state.dtype == np.int32 and ran1_state.size == 35 * nchain and ran1_state.flags["C_CONTIGUOUS"]
+        r1p = ran1_state.ctypes.data_as(C.POINTER(C.c_int32))
+    fp = C.POINTER(C.c_float)
+    ctx.check(ctx.lib.epsr_fit_expansion(
+        ctx.h, int(nchain), _ptr(yin), int(ldy), _ptr(expr), int(lde), _ptr(yout), int(ldo), _ptr(wt), _ptr(wtold),
+        int(ldw), st.ctypes.data_as(fp), ac.ctypes.data_as(fp), rj.ctypes.data_as(fp), cs.ctypes.data_as(fp),
+        int(nmin), int(npt), int(ncfirst), int(nterms), int(nits), int(nfreq), r1p, int(mode),
+        nt.ctypes.data_as(C.POINTER(C.c_int))))
+    return FitResult(st, ac, rj, cs, nt)
+
+
+def calc_pwrexpq(ctx: Context, lpt, nterms, dmin, dstep, dmax, sigmab, rho, q, expr=None, rmin=0.0):
+    """The Q-space Poisson basis; returns (expr tensor, ncfirst)."""
+    if expr is None:
+        expr = ctx.zeros(lpt * nterms + 1)
+    nc = C.c_int()
+    ctx.check(ctx.lib.epsr_calc_pwrexpq(ctx.h, int(lpt), int(nterms), F32(dmin), F32(dstep), F32(dmax), F32(sigmab),
+                                        F32(rho), _ptr(q), _ptr(expr), F32(rmin), C.byref(nc)))
+    return expr, nc.value
+
+
+def ptgnsq_diff(ctx: Context, nfile, nq, npairs, q, data, ldq, qwind, cwt, szeros, nqmin, nqmax, qminptemp,
+                qtemp=None):
+    if qtemp is None:
+        qtemp = ctx.zeros(nfile * ldq)
+    cwt_, cwtp = _fp(cwt)
+    sz_, szp = _fp(szeros)
+    mn_, mnp = _ip(nqmin)
+    mx_, mxp = _ip(nqmax)
+    ctx.check(ctx.lib.epsr_ptgnsq_diff(ctx.h, int(nfile), int(nq), int(npairs), _ptr(q), _ptr(data), int(ldq),
+                                       _ptr(qwind), cwtp, szp, mnp, mxp, F32(qminptemp), _ptr(qtemp)))
+    return qtemp
+
+
+def ptgnsq_rfactor(ctx: Context, i0, i1, fit, qtemp):
+    r = C.c_float()
+    ctx.check(ctx.lib.epsr_ptgnsq_rfactor(ctx.h, int(i0), int(i1), _ptr(fit), _ptr(qtemp), C.byref(r)))
+    return F32(r.value)
+
+
+def ptgnsq_update(ctx: Context, id_, nterms, npairs, pwtold, irefpcof, efacp, cwtpot, ldc, pfact, pcoefd, pcoefp,
+                  ldp):
+    """pcoefd/pcoefp: host float32 arrays updated in place."""
+    pw_, pwp = _fp(pwtold)
+    ir_, irp = _ip(irefpcof)
+    ef_, efp = _fp(efacp)
+    cw_, cwp = _fp(cwtpot)
+    assert pcoefd.dtype == F32 and pcoefp.dtype == F32 and pcoefd.flags["C_CONTIGUOUS"] and pcoefp.flags["C_CONTIGUOUS"]
+    fp = C.POINTER(C.c_float)
+    ctx.check(ctx.lib.epsr_ptgnsq_update(ctx.h, int(id_), int(nterms), int(npairs), pwp, irp, efp, cwp, int(ldc),
+                                         F32(pfact), pcoefd.ctypes.data_as(fp), pcoefp.ctypes.data_as(fp), int(ldp)))

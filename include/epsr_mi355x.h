@@ -238,6 +238,49 @@ int epsr_foqols(epsr_ctx *ctx, int nq, const float *q, int npt1, const float *x,
                 const float *ycoh, float rho, float fwhm, float fwhmq, float lorchqwidth,
                 float lorchqwidthq, float rmaxgr, float *foqols, int ldf);
 
+/* ---- expand: replaces epsr_standalone_rev3.expand(nterms, j, expr, coeff, summax)
+ * (:4924-4939) for every j = 1..npt at once; expr dev flat 1-based (row j at
+ * (j-1)*nterms), coeff dev [nterms+1], out dev [npt+1]. */
+int epsr_expand_all(epsr_ctx *ctx, int nterms, int npt, const float *expr, const float *coeff, float summax,
+                    float *out);
+
+/* ---- smoop: replaces auxiliary_routines.smoop(nqmin, nqmax, nsmoo, diff) (:744-764);
+ * diff dev, smoothed in place. */
+int epsr_smoop(epsr_ctx *ctx, int nqmin, int nqmax, int nsmoo, float *diff);
+
+/* ---- fit_expansion: replaces epsr_standalone_rev3.fit_expansion(yin, expr, yout, wt, wtold,
+ * stepr, acceptr, rejrat, chisqnew, nmin, npt, ncfirst, nterms, nits, nfreq) (:4806-4920)
+ * for nchain independent chains (datasets or replicas).  Chain c uses yin + c*ldy,
+ * expr + c*lde (lde = 0: one basis for all), yout + c*ldo, wt/wtold + c*ldw (all dev,
+ * 1-based), the host scalars stepr/acceptr/rejrat/chisqnew [nchain] (in/out, as the
+ * reference's return tuple) and the host ran1 state ran1_state + 35*c (in/out).
+ * EPSR_FIT_MC replays the reference's Metropolis fit exactly (correctly rounded expf);
+ * EPSR_FIT_NORMAL solves the FP64 normal equations for coefficients ncfirst..nterms
+ * (no ran1 use; acceptr = 0.05 and rejrat unchanged).  ntries (host, nullable) returns
+ * the tries run per chain. */
+#define EPSR_FIT_MC 0
+#define EPSR_FIT_NORMAL 1
+int epsr_fit_expansion(epsr_ctx *ctx, int nchain, const float *yin, int ldy, const float *expr, long long lde,
+                       float *yout, int ldo, float *wt, float *wtold, int ldw, float *stepr, float *acceptr,
+                       float *rejrat, float *chisqnew, int nmin, int npt, int ncfirst, int nterms, int nits,
+                       int nfreq, int32_t *ran1_state, int mode, int *ntries);
+
+/* ---- ptgnsq (epsr_standalone_rev3.py:3789-4021), neutron weights (cwt):
+ * epsr_ptgnsq_diff forms qtemp(i, id) for every dataset (:3827-3867): q dev [nq+1],
+ * data dev [nfile*ldq] (data[id*ldq+i]), qwind dev [npairs*ldq] (intermolecular
+ * partials, qwind[ic*ldq+i]), cwt host [nfile*npairs], szeros host [nfile], nqmin/nqmax
+ * host [nfile] (1-based), qtemp dev out [nfile*ldq]. */
+int epsr_ptgnsq_diff(epsr_ctx *ctx, int nfile, int nq, int npairs, const float *q, const float *data, int ldq,
+                     const float *qwind, const float *cwt, const float *szeros, const int *nqmin, const int *nqmax,
+                     float qminptemp, float *qtemp);
+/* rfactorep of one dataset over i0..i1 (:3977-3991); fit, qtemp dev; rfactor host out. */
+int epsr_ptgnsq_rfactor(epsr_ctx *ctx, int i0, int i1, const float *fit, const float *qtemp, float *rfactor);
+/* the coefficient update of dataset id (:3962-3975), host arrays: pcoefd[id*ldp+ic],
+ * pcoefp[irefp*ldp+ic], cwtpot[ip*ldc+id], pwtold [nterms+1]. */
+int epsr_ptgnsq_update(epsr_ctx *ctx, int id, int nterms, int npairs, const float *pwtold, const int *irefpcof,
+                       const float *efacp, const float *cwtpot, int ldc, float pfact, float *pcoefd, float *pcoefp,
+                       int ldp);
+
 #ifdef __cplusplus
 }
 #endif

@@ -208,6 +208,12 @@ void ora_fit_expansion(const float *yin, const float *expr, float *yout, float *
                        int npt, int ncfirst, int nterms, int nits, int nfreq, ora_ran1_t *rng,
                        int *ntries_out);
 
+/* ptgnsq pieces (neutron weights); layouts as the library's epsr_ptgnsq_diff */
+void  ora_ptgnsq_diff(int nfile, int nq, int npairs, const float *q, const float *data, int ldq,
+                      const float *qwind, const float *cwt, const float *szeros, const int *nqmin,
+                      const int *nqmax, float qminptemp, float *qtemp);
+float ora_ptgnsq_rfactor(int i0, int i1, const float *fit, const float *qt);
+
 #ifdef __cplusplus
 }
 #endif

@@ -167,6 +167,9 @@ def _declare(L):
         "ora_smoop": (None, [C.c_int, C.c_int, C.c_int, fp]),
         "ora_fit_expansion": (None, [fp, fp, fp, fp, fp, fp, fp, fp, fp, C.c_int, C.c_int, C.c_int,
                                      C.c_int, C.c_int, C.c_int, P(Ran1), ip]),
+        "ora_ptgnsq_diff": (None, [C.c_int, C.c_int, C.c_int, fp, fp, C.c_int, fp, fp, fp, ip, ip, C.c_float,
+                                   fp]),
+        "ora_ptgnsq_rfactor": (C.c_float, [C.c_int, C.c_int, fp, fp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
