@@ -116,6 +116,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--sweeps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-small-box", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first copy 256 MiB device-to-device (the known-size reference for PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding)",
@@ -128,6 +131,13 @@ def main():
     from epsr_python_translation_amd import pipeline as PL
     from epsr_python_translation_amd import synth
 
+    if args.calibrate_copy:
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     s = synth.config1()
     ctx = D.Context(0, timing=True)
     R = PL.Refinement(ctx, s, idum=-2024).build_tables()
@@ -146,6 +156,7 @@ def main():
     moves_per_step = args.sweeps * s.nmol
     atom_moves_per_step = args.sweeps * s.natom
     ktime = {}
+    mc_bytes = []
 
     def add(name, v):
         if v >= 0:
@@ -160,6 +171,7 @@ def main():
             if record:
                 add("mc_pair", ctx.kernel_ms("mc_pair"))
                 add("mc_resolve", ctx.kernel_ms("mc_resolve"))
+                mc_bytes.append(R.mc.stats.resolver_bytes)
         R.atogen(ngrsamples=s.nmol)
         if record:
             add("atogen_inter", ctx.kernel_ms("atogen_inter"))
@@ -174,7 +186,6 @@ def main():
     for _ in range(args.warmup):
         step(False)
     torch.cuda.synchronize()
-    acc0 = R.mc.stats.accepted
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step(True)
@@ -182,19 +193,39 @@ def main():
     elapsed = time.perf_counter() - t0
     ms_per_step = 1000.0 * elapsed / args.steps
     ktot = {k: float(np.sum(v)) / args.steps for k, v in ktime.items()}
-    # dominant kernel family per step
+    kavg = {k: float(np.mean(v)) for k, v in ktime.items()}
+
+    # algorithmic work per launch of each kernel family
+    npair_entries = R.ledger.nergpair - int(np.count_nonzero(s.eintra))
+    apm = float(np.mean(nat))                                     # atoms per molecule (3 for every water)
+    lookups_per_sweep = 3.0 * npair_entries * apm * apm          # A for both slots + Bv for the later one
+    work = {
+        "atogen_inter": ("mfma", OPS_PER_DISTANCE * inter_pairs, "TFLOP/s",
+                         f"{OPS_PER_DISTANCE} fp32 ops x {inter_pairs} distances"),
+        "mc_pair": ("mfma", OPS_PER_LOOKUP * lookups_per_sweep, "TFLOP/s",
+                    f"{OPS_PER_LOOKUP} fp32 ops x {lookups_per_sweep:.3g} atom-pair lookups"),
+        "gensq": ("mfma", 2.0 * g.nq * g.npt1 * 2 * s.npairs, "TFLOP/s", "2 nq npt1 2npairs flop"),
+        "stog": ("mfma", 2.0 * nr * g.nq * s.npairs, "TFLOP/s", "2 nr nq npairs flop"),
+        "mc_resolve": ("hbm", float(np.mean(mc_bytes)) if mc_bytes else 0.0, "GB/s",
+                       "thresholds + in-block deltas + row offsets + pushed rows + bits, bytes per sweep; "
+                       "a serial chain: latency-bound"),
+    }
+    kernels = {}
+    for k, (bound, units, unit, what) in work.items():
+        if k not in kavg:
+            continue
+        rate = units / (kavg[k] * 1e-3)
+        achieved = rate / 1e12 if unit == "TFLOP/s" else rate / 1e9
+        peak = FP32_PEAK_TFLOPS if unit == "TFLOP/s" else HBM_PEAK_GBS
+        kernels[k] = {"bound": bound, "achieved": achieved, "peak": peak, "unit": unit, "frac": achieved / peak,
+                      "ms": kavg[k], "work_per_launch": what}
     dom = max(ktot, key=ktot.get)
-    if dom == "atogen_inter":
-        k_ms = float(np.mean(ktime[dom]))
-        achieved = OPS_PER_DISTANCE * inter_pairs / (k_ms * 1e-3) / 1e12
-        work = f"{OPS_PER_DISTANCE} fp32 ops x {inter_pairs} distances per launch"
-    else:
-        k_ms = float(np.mean(ktime[dom]))
-        achieved = None
-        work = "serial Metropolis chain (latency bound)"
-    roof = {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": (achieved / FP32_PEAK_TFLOPS) if achieved else None, "traffic": None, "kernel_ms": k_ms,
-            "work": work, "per_step_kernel_ms": ktot}
+    kd = kernels.get(dom, {})
+    traffic = _traffic(dom)
+    roof = {"bound": kd.get("bound", "hbm"), "achieved": kd.get("achieved"), "peak": kd.get("peak"),
+            "unit": kd.get("unit"), "frac": kd.get("frac"), "traffic": traffic, "kernel": dom,
+            "kernel_ms": kavg.get(dom), "work_per_launch": kd.get("work_per_launch"), "per_step_kernel_ms": ktot,
+            "kernels": kernels}
     out = {
         "metric": "atom-moves/s",
         "value": atom_moves_per_step * args.steps / elapsed,
@@ -216,9 +247,49 @@ def main():
         "accepted_fraction": (R.mc.stats.accepted / max(1, R.mc.stats.moves)),
         "roofline": roof,
     }
+    if not args.no_small_box:
+        out["dmso_water_3200"] = small_box_rate(D, PL, synth, ctx)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(s)
     print(json.dumps(out), flush=True)
+
+
+HBM_PEAK_GBS = 8000.0
+FORTRAN_ATOM_MOVES_PER_S = 1.45e4   # derived lower bound for the 3200-atom test case (5 cycles in <= 1.1 s)
+
+
+def _traffic(kernel):
+    """HBM bytes per launch of the kernel on this workload, from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (calibrated against a copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_water20000.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t.get(kernel, {}).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def small_box_rate(D, PL, synth, ctx, sweeps=20):
+    """MC sweeps of the 3200-atom DMSO/water box (600 molecules): atom-moves/s of molfit
+    alone, beside the Fortran's derived rate on the same-size test case."""
+    import torch
+
+    s0 = synth.config0()
+    R0 = PL.Refinement(ctx, s0, idum=-77).build_tables()
+    R0.cells_and_ledger()
+    for _ in range(3):
+        R0.molfit()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(sweeps):
+        R0.molfit()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    rate = sweeps * s0.natom / dt
+    return {"workload": "200 DMSO + 400 water, 3200 atoms, L=31.7 A", "sweeps": sweeps,
+            "ms_per_sweep": 1000.0 * dt / sweeps, "mc_atom_moves_per_s": rate,
+            "x_fortran_derived": rate / FORTRAN_ATOM_MOVES_PER_S}
 
 
 if __name__ == "__main__":

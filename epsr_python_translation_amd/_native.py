@@ -91,7 +91,8 @@ class MCParams(C.Structure):
 
 class MCStats(C.Structure):
     _fields_ = [("moves", C.c_int), ("accepted", C.c_int), ("exact_resolutions", C.c_int), ("retries", C.c_int),
-                ("ms_pairs", C.c_double), ("ms_resolve", C.c_double)]
+                ("ms_pairs", C.c_double), ("ms_resolve", C.c_double), ("pushed_entries", C.c_longlong),
+                ("resolver_bytes", C.c_double)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/epsr_mi355x.h

@@ -101,4 +101,4 @@ def ledger() -> D.Ledger:
 
 
 def has_ledger() -> bool:
-    return _ledger is not None and getattr(_ledger, "nergpair", None) is not None
+    return _ledger is not None and _ledger.built

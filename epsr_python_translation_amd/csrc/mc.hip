@@ -1445,27 +1445,42 @@ mc_fdiff_accepted(int nmol, const int *__restrict__ order, const int *__restrict
 }
 
 // fadd = fadd + fdiff over the accepted moves in chain order (molfit.py:405), and the
-// number of accepted moves: one wave, 64 chain positions at a time
-__global__ void __launch_bounds__(64)
+// number of accepted moves.  One workgroup: each segment of the chain is compacted into
+// LDS (accepted values in chain order) by all threads, then thread 0 adds them in order.
+#define FADD_SEG 16384
+
+__global__ void __launch_bounds__(1024)
 mc_fadd(int nmol, const unsigned *__restrict__ bits, const float *__restrict__ fdiff, float *__restrict__ res)
 {
-    const int lane = threadIdx.x;
+    __shared__ float vals[FADD_SEG];
+    __shared__ int wsum[16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     float fadd = 0.0f;
     int nacc = 0;
-    for (int k0 = 0; k0 < nmol; k0 += 64) {
-        const int k = k0 + lane;
-        const bool a = k < nmol && ((bits[k >> 5] >> (k & 31)) & 1u);
-        const float v = a ? fdiff[k] : 0.0f;
-        unsigned long long msk = __ballot(a);
-        nacc += __popcll(msk);
-        const int vb = __float_as_int(v);
-        while (msk) {
-            const int i = __ffsll((long long)msk) - 1;
-            fadd = fadd + __int_as_float(__builtin_amdgcn_readlane(vb, i));
-            msk &= msk - 1ull;
+    for (int s0 = 0; s0 < nmol; s0 += FADD_SEG) {
+        const int s1 = min(nmol, s0 + FADD_SEG);
+        int fill = 0;
+        for (int k0 = s0; k0 < s1; k0 += 1024) {
+            const int k = k0 + (int)threadIdx.x;
+            const bool a = k < s1 && ((bits[k >> 5] >> (k & 31)) & 1u);
+            const unsigned long long b = __ballot(a);
+            if (lane == 0) wsum[w] = __popcll(b);
+            __syncthreads();
+            int off = fill, tot = 0;
+            for (int i = 0; i < 16; ++i) {
+                if (i < w) off += wsum[i];
+                tot += wsum[i];
+            }
+            if (a) vals[off + __popcll(b & ((1ull << lane) - 1ull))] = fdiff[k];
+            fill += tot;
+            __syncthreads();
         }
+        if (threadIdx.x == 0)
+            for (int i = 0; i < fill; ++i) fadd = fadd + vals[i];
+        nacc += fill;
+        __syncthreads();
     }
-    if (lane == 0) {
+    if (threadIdx.x == 0) {
         res[0] = fadd;
         res[1] = (float)nacc;
     }
@@ -1556,7 +1571,7 @@ mc_resolve_blocks(int nmol, int npad, const int *__restrict__ order, const int *
     for (int i = lane; i < npad / 32; i += 64) bits[i] = 0u;
     for (int i = lane; i < npad; i += 64) accde[i] = 0.0f;
     const int nblk = npad / 64;
-    int nexact = 0;
+    int nexact = 0, npushed = 0;   // exact sums formed; push-row entries staged
     BlockIn x0, x1;
     mc_block_fetch(0, thr, Mt, prow, x0);
     __syncthreads();
@@ -1572,6 +1587,7 @@ mc_resolve_blocks(int nmol, int npad, const int *__restrict__ order, const int *
             const int c = __builtin_amdgcn_readlane(x.rc, i);
             if (so + c > cap) break;
             const int st = __builtin_amdgcn_readlane(x.rp, i);
+            npushed += c;
             for (int p = 0; p < c; p += 128)
                 __builtin_amdgcn_global_load_lds(
                     (const void __attribute__((address_space(1))) *)(pent + st + p + 2 * lane),
@@ -1662,7 +1678,10 @@ mc_resolve_blocks(int nmol, int npad, const int *__restrict__ order, const int *
     }
     __syncthreads();
     for (int i = lane; i < (nmol + 31) >> 5; i += 64) acc_out[i] = bits[i];
-    if (lane == 0) *nexact_out = nexact;
+    if (lane == 0) {
+        nexact_out[0] = nexact;
+        nexact_out[1] = npushed;
+    }
 }
 
 static float adapt_ratio(float rejrate, int att, int rej)
@@ -1894,7 +1913,7 @@ extern "C" int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
                                            dB, dL);
     }
     timer_end(ctx, "mc_pair");
-    int nexact = nmol;
+    int nexact = nmol, npushed = 0;
     if (fast) {
         HIP_CHECK(ctx, hipMemsetAsync(dMt, 0, sizeof(float) * (size_t)npad * 64, st));
         mc_prepare<<<nmol, 64, 0, st>>>(nmol, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner, l->d_mirror, dA, dB, dL,
@@ -1909,8 +1928,12 @@ extern "C" int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
         mc_fdiff_accepted<<<ceil_div(nmol, 4), 256, sizeof(float2) * 4 * (size_t)maxc, st>>>(
             nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner, l->d_mirror, dA, dB, dL, dacc, maxc,
             dfdiff);
-        mc_fadd<<<1, 64, 0, st>>>(nmol, dacc, dfdiff, dres);
-        HIP_CHECK(ctx, hipMemcpyAsync(&nexact, dnex, sizeof(int), hipMemcpyDeviceToHost, st));
+        mc_fadd<<<1, 1024, 0, st>>>(nmol, dacc, dfdiff, dres);
+        int hnex[2];
+        HIP_CHECK(ctx, hipMemcpyAsync(hnex, dnex, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+        HIP_CHECK(ctx, hipStreamSynchronize(st));
+        nexact = hnex[0];
+        npushed = hnex[1];
     } else {
         size_t lds = sizeof(unsigned) * ((nmol + 31) / 32) + sizeof(float) * 2 * (size_t)maxc + 64;
         ARG_CHECK(ctx, lds <= 160 * 1024, "epsr_molfit: %d molecules x %d ledger slots exceed LDS", nmol, maxc);
@@ -1961,6 +1984,10 @@ extern "C" int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
         stats->moves = nmol;
         stats->accepted = (int)hres[1];
         stats->exact_resolutions = nexact;
+        stats->pushed_entries = npushed;
+        stats->resolver_bytes = fast ? (double)npad * (sizeof(float2) + 64 * sizeof(float) + sizeof(int)) +
+                                           (double)npushed * sizeof(int2) + (double)npad / 8.0
+                                     : 0.0;
         stats->retries = retries;
         stats->ms_pairs = ctx->timing ? epsr_ctx_last_kernel_ms(ctx, "mc_pair") : -1.0;
         stats->ms_resolve = ctx->timing ? epsr_ctx_last_kernel_ms(ctx, "mc_resolve") : -1.0;

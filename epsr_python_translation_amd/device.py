@@ -245,11 +245,13 @@ class Ledger:
         self.h = h
         self.nnbrefmax = 0
         self.nergpair = 0
+        self.built = False   # a ledger from fmolec1 on the current cells
 
     def cells_init(self, nmolcellin, rmax):
         c = self.box.coords()
         self.ctx.check(self.ctx.lib.epsr_cells_init(self.ctx.h, self.box.h, self.h, C.byref(c), int(nmolcellin),
                                                     F32(rmax)))
+        self.built = False
 
     def fmolec1(self, tables: DeviceTables, eintra):
         """Returns (ftest, ftestb) per molecule in FP64."""
@@ -264,6 +266,7 @@ class Ledger:
                                                  C.byref(f64), C.byref(fb64), None, None, C.byref(nn), C.byref(ne)))
         self.nnbrefmax = nn.value
         self.nergpair = ne.value
+        self.built = True
         return f64.value, fb64.value
 
     def export(self):

@@ -180,9 +180,15 @@ typedef struct {
     int ireset;
 } epsr_mc_params_t;
 
+/* per-cycle statistics: moves attempted and accepted, moves decided by forming the
+ * exact float32 sum, movem retries, push-row entries the resolver read, the resolver's
+ * algorithmic bytes (thresholds, in-block deltas, row offsets, pushed rows, acceptance
+ * bits) and the GPU time of the pair and resolver kernels (with timing enabled) */
 typedef struct {
     int moves, accepted, exact_resolutions, retries;
     double ms_pairs, ms_resolve;
+    long long pushed_entries;
+    double resolver_bytes;
 } epsr_mc_stats_t;
 
 int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *coords, const epsr_tables_t *tables,
