@@ -92,7 +92,25 @@ class MCParams(C.Structure):
 class MCStats(C.Structure):
     _fields_ = [("moves", C.c_int), ("accepted", C.c_int), ("exact_resolutions", C.c_int), ("retries", C.c_int),
                 ("ms_pairs", C.c_double), ("ms_resolve", C.c_double), ("pushed_entries", C.c_longlong),
-                ("resolver_bytes", C.c_double)]
+                ("resolver_bytes", C.c_double), ("stage1_rejected", C.c_int)]
+
+
+class DescParams(C.Structure):
+    _fields_ = [("ng", C.c_int), ("rc", C.c_float), ("r0", C.c_float), ("dmu", C.c_float), ("width", C.c_float),
+                ("xscale", C.c_float)]
+
+
+class MLP(C.Structure):
+    _fields_ = [("d", C.c_int), ("h", C.c_int), ("w", vp), ("wb", vp), ("m", vp), ("v", vp), ("step", C.c_int)]
+
+
+class DA(C.Structure):
+    _fields_ = [("mlp", C.POINTER(MLP)), ("desc", DescParams), ("seed", C.c_ulonglong), ("des", vp)]
+
+
+class MCRecord(C.Structure):
+    _fields_ = [("desc", C.POINTER(DescParams)), ("ldx", C.c_int), ("x", vp), ("target", vp), ("des", vp),
+                ("pass_", vp)]
 
 
 # name -> (restype, argtypes); every symbol declared in include/epsr_mi355x.h
@@ -120,6 +138,15 @@ SIGNATURES = {
     "epsr_en_mole_pair": (C.c_int, [vp, vp, C.POINTER(Coords), C.POINTER(Tables), C.c_int, vp, vp, vp]),
     "epsr_molfit": (C.c_int, [vp, vp, C.POINTER(Coords), C.POINTER(Tables), vp, C.POINTER(MCParams), fp, fp, ip, ip,
                               vp, i32p, fp, C.POINTER(MCStats)]),
+    "epsr_molfit_da": (C.c_int, [vp, vp, C.POINTER(Coords), C.POINTER(Tables), vp, C.POINTER(MCParams), fp, fp, ip,
+                                 ip, vp, i32p, fp, C.POINTER(DA), C.POINTER(MCRecord), C.POINTER(MCStats)]),
+    "epsr_descriptors": (C.c_int, [vp, vp, C.POINTER(Coords), C.POINTER(Coords), vp, C.POINTER(DescParams), C.c_int,
+                                   vp, vp]),
+    "epsr_mlp_nparams": (C.c_longlong, [C.c_int, C.c_int]),
+    "epsr_mlp_sync": (C.c_int, [vp, C.POINTER(MLP)]),
+    "epsr_mlp_forward": (C.c_int, [vp, C.POINTER(MLP), C.c_int, vp, C.c_int, vp]),
+    "epsr_mlp_grad": (C.c_int, [vp, C.POINTER(MLP), C.c_int, vp, C.c_int, vp, vp, fp]),
+    "epsr_mlp_adam": (C.c_int, [vp, C.POINTER(MLP), vp, C.c_float, C.c_float, C.c_float, C.c_float]),
     "epsr_truncation": (C.c_int, [vp, C.c_int, vp, C.c_float, C.c_float, vp, vp]),
     "epsr_generate_rp": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_int]),
     "epsr_generate_ep": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp, C.c_int, ip, C.c_int, fp, C.c_int, vp, vp, vp, vp,

@@ -48,6 +48,32 @@ struct epsr_box {
     int *d_npairref = nullptr, *d_amol = nullptr;
 };
 
+// ------------------------------------------------------------------ ledger
+
+struct epsr_ledger {
+    epsr_ctx *ctx = nullptr;
+    const epsr_box *box = nullptr;
+    int nmol = 0;
+    // cells (host)
+    int na = 0, nb = 0, nc = 0, ncell = 0, ncellneighb = 0, maxcellno = 0, nnbrefmax = 0;
+    std::vector<int> nmolsincell, molsincell, molcellno, iaoff, iboff, icoff;
+    std::vector<int> order, pos;
+    bool cells_ready = false;
+    // ledger (device)
+    int nself = 0, npairent = 0, nslots = 0, nentries = 0;
+    int *d_rowstart = nullptr, *d_cnt = nullptr, *d_order = nullptr, *d_pos = nullptr;
+    int *d_partner = nullptr, *d_entry = nullptr, *d_mirror = nullptr, *d_slotmol = nullptr;
+    float *d_ergpair = nullptr;
+    // pushes of the MC resolver: an accepted move at chain position k updates the energy
+    // change of its later partners outside its block of 64 positions through the row
+    // prow[k] .. prow[k+1] - 1, npush(k) entries padded to whole pieces of 128
+    int maxpush = 0, npushtot = 0, npad = 0;
+    int *d_prow = nullptr;   // [npad + 1]
+    size_t cap_slots = 0, cap_entries = 0;
+    bool ledger_ready = false;
+    std::vector<int> h_rowstart, h_cnt;
+};
+
 // ------------------------------------------------------------------ errors
 
 int set_error(epsr_ctx *ctx, int code, const char *fmt, ...);

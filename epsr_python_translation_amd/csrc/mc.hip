@@ -26,30 +26,7 @@
 #include <numeric>
 
 #include "common.h"
-
-struct epsr_ledger {
-    epsr_ctx *ctx = nullptr;
-    const epsr_box *box = nullptr;
-    int nmol = 0;
-    // cells (host)
-    int na = 0, nb = 0, nc = 0, ncell = 0, ncellneighb = 0, maxcellno = 0, nnbrefmax = 0;
-    std::vector<int> nmolsincell, molsincell, molcellno, iaoff, iboff, icoff;
-    std::vector<int> order, pos;
-    bool cells_ready = false;
-    // ledger (device)
-    int nself = 0, npairent = 0, nslots = 0, nentries = 0;
-    int *d_rowstart = nullptr, *d_cnt = nullptr, *d_order = nullptr, *d_pos = nullptr;
-    int *d_partner = nullptr, *d_entry = nullptr, *d_mirror = nullptr, *d_slotmol = nullptr;
-    float *d_ergpair = nullptr;
-    // pushes of the MC resolver: an accepted move at chain position k updates the energy
-    // change of its later partners outside its block of 64 positions through the row
-    // prow[k] .. prow[k+1] - 1, npush(k) entries padded to whole pieces of 128
-    int maxpush = 0, npushtot = 0, npad = 0;
-    int *d_prow = nullptr;   // [npad + 1]
-    size_t cap_slots = 0, cap_entries = 0;
-    bool ledger_ready = false;
-    std::vector<int> h_rowstart, h_cnt;
-};
+#include "surrogate.h"
 
 template <class T>
 static int dev_alloc(epsr_ctx *ctx, T **p, size_t n)
@@ -1091,7 +1068,8 @@ __global__ void mc_pairs(int nslots, const int *__restrict__ slotmol, const int 
                          const int *__restrict__ partner, const int *__restrict__ entry, const int *__restrict__ pos,
                          const float *__restrict__ ergpair, const float4 *__restrict__ Pold,
                          const float4 *__restrict__ Pnew, const int *__restrict__ naf, const int *__restrict__ nal,
-                         PairTab gt, int npairs, float *__restrict__ A, float *__restrict__ Bv, float *__restrict__ L)
+                         PairTab gt, int npairs, const float *__restrict__ shift, float *__restrict__ A,
+                         float *__restrict__ Bv, float *__restrict__ L)
 {
     extern __shared__ char smem[];
     PairTab t = LDS ? tables_to_lds(gt, npairs, smem) : gt;
@@ -1099,6 +1077,7 @@ __global__ void mc_pairs(int nslots, const int *__restrict__ slotmol, const int 
         int m = slotmol[i];
         L[i] = ergpair[entry[i]];
         if (i == rowstart[m]) continue;
+        if (shift && isnan(shift[pos[m]])) continue;   // rejected by the surrogate stage
         int n = partner[i];
         const float4 *P1 = Pnew + naf[m];
         int n1 = nal[m] - naf[m] + 1, n2 = nal[n] - naf[n] + 1;
@@ -1304,8 +1283,8 @@ __global__ void __launch_bounds__(64)
 mc_prepare(int nmol, const int *__restrict__ pos, const int *__restrict__ rowstart, const int *__restrict__ cnt,
            const int *__restrict__ partner, const int *__restrict__ mirror, const float *__restrict__ A,
            const float *__restrict__ Bv, const float *__restrict__ L, const float *__restrict__ stream,
-           const int *__restrict__ moff, float temp, float *__restrict__ delta, float2 *__restrict__ thr,
-           float *__restrict__ Mt)
+           const int *__restrict__ moff, float temp, const float *__restrict__ shift, float *__restrict__ delta,
+           float2 *__restrict__ thr, float *__restrict__ Mt)
 {
     // Each rounding of the sequential sum errs by at most u |partial sum|.  With
     // P_s = |sum_{s'<=s} e0| + sum_{s'<=s, earlier partner} |delta| the partial sums obey
@@ -1316,6 +1295,12 @@ mc_prepare(int nmol, const int *__restrict__ pos, const int *__restrict__ rowsta
     const int lane = threadIdx.x;
     const int row = rowstart[m], c = cnt[m];
     const int pm = pos[m];
+    const float sh = shift ? shift[pm] : 0.0f;
+    if (isnan(sh)) {
+        // rejected by the surrogate stage: never accepted, receives no pushes
+        if (lane == 0) thr[pm] = make_float2(-INFINITY, -INFINITY);
+        return;
+    }
     double carry_e = 0.0, carry_d = 0.0, pprev = 0.0, w = 0.0;
     int npb = 0;
     for (int s0 = 1; s0 < c; s0 += 64) {
@@ -1369,7 +1354,7 @@ mc_prepare(int nmol, const int *__restrict__ pos, const int *__restrict__ rowsta
         const double bnd = u * (1.001 * w + carry_d * (double)(npb + 2)) + 1e-12 * fabs(carry_e) + 1e-30;
         const double nlr = -log((double)stream[moff[pm]]);
         float fa, fr;
-        mc_thresholds(carry_e, bnd, (double)temp, nlr, fa, fr);
+        mc_thresholds(carry_e - (double)sh, bnd, (double)temp, nlr, fa, fr);
         thr[pm] = make_float2(fa, fr);
     }
 }
@@ -1382,7 +1367,7 @@ __global__ void __launch_bounds__(64)
 mc_push_rows(int nmol, const int *__restrict__ order, const int *__restrict__ pos,
              const int *__restrict__ rowstart, const int *__restrict__ cnt, const int *__restrict__ partner,
              const int *__restrict__ mirror, const float *__restrict__ delta, const int *__restrict__ prow,
-             int2 *__restrict__ pent, float2 *__restrict__ thr)
+             const float *__restrict__ shift, int2 *__restrict__ pent, float2 *__restrict__ thr)
 {
     const int k = blockIdx.x;
     const int lane = threadIdx.x;
@@ -1398,7 +1383,7 @@ mc_push_rows(int nmol, const int *__restrict__ order, const int *__restrict__ po
         const int s = s0 + lane;
         int t = -1;
         if (s < c) t = push_target(row + s, k, partner, mirror, pos, rowstart);
-        const bool f = far_push(t, k);
+        const bool f = far_push(t, k) && !(shift && isnan(shift[t]));
         const unsigned long long b = __ballot(f);
         if (f) out[fill + __popcll(b & ((1ull << lane) - 1ull))] = make_int2(t, __float_as_int(delta[mirror[row + s]]));
         fill += __popcll(b);
@@ -1504,8 +1489,8 @@ __device__ __noinline__ int mc_exact_decide(int k, const int *__restrict__ order
                                             const int *__restrict__ partner, const int *__restrict__ mirror,
                                             const float *__restrict__ A, const float *__restrict__ Bv,
                                             const float *__restrict__ L, const float *__restrict__ stream,
-                                            const int *__restrict__ moff, float temp, const unsigned *bits,
-                                            float *term)
+                                            const int *__restrict__ moff, float temp,
+                                            const float *__restrict__ shift, const unsigned *bits, float *term)
 {
     const int lane = threadIdx.x;
     const int m = order[k];
@@ -1524,7 +1509,8 @@ __device__ __noinline__ int mc_exact_decide(int k, const int *__restrict__ order
     if (lane == 0) {
         float fdiff = 0.0f;
         for (int s = 1; s < c; ++s) fdiff = fdiff + term[2 * s] - term[2 * s + 1];
-        float ediff = (fdiff + 0.0f) * temp;
+        // delayed acceptance tests dE - dEs (stage 2); plain molfit tests dE
+        float ediff = (shift ? fdiff - shift[k] : fdiff + 0.0f) * temp;
         dd = stream[moff[k]] < etest_of(ediff) ? 1 : 0;
     }
     dd = __shfl(dd, 0);
@@ -1559,8 +1545,8 @@ mc_resolve_blocks(int nmol, int npad, const int *__restrict__ order, const int *
                   const int *__restrict__ mirror, const float *__restrict__ A, const float *__restrict__ Bv,
                   const float *__restrict__ L, const float2 *__restrict__ thr, const float *__restrict__ Mt,
                   const int *__restrict__ prow, const int2 *__restrict__ pent, const float *__restrict__ stream,
-                  const int *__restrict__ moff, float temp, int cap, unsigned *__restrict__ acc_out,
-                  int *__restrict__ nexact_out)
+                  const int *__restrict__ moff, float temp, const float *__restrict__ shift, int cap,
+                  unsigned *__restrict__ acc_out, int *__restrict__ nexact_out)
 {
     extern __shared__ int4 smq[];
     int2 *stg = (int2 *)smq;                       // [cap] staged push rows
@@ -1649,8 +1635,8 @@ mc_resolve_blocks(int nmol, int npad, const int *__restrict__ order, const int *
                     bits[(kb >> 5) + 1] = (unsigned)(am >> 32);
                 }
                 dec = __builtin_amdgcn_readfirstlane(mc_exact_decide(kb + i, order, pos, rowstart, cnt, partner,
-                                                                     mirror, A, Bv, L, stream, moff, temp, bits,
-                                                                     term));
+                                                                     mirror, A, Bv, L, stream, moff, temp, shift,
+                                                                     bits, term));
                 nexact++;
             }
             if (dec) {
@@ -1684,6 +1670,46 @@ mc_resolve_blocks(int nmol, int npad, const int *__restrict__ order, const int *
     }
 }
 
+// Delayed acceptance, stage 1, by chain position k: dEs from the network output of
+// rows 2k (current) and 2k+1 (trial), or from des by molecule; the move passes with
+// probability min(1, exp(-dEs/simt)) tested in the float32 form of the Metropolis test,
+// against the uniform of its molecule.  shift[k] = dEs for a pass, NaN for a rejection.
+__global__ void da_stage1(int nmol, const int *__restrict__ order, const float *__restrict__ y,
+                          const float *__restrict__ des, float temp, unsigned long long seed,
+                          float *__restrict__ shift, float *__restrict__ rec_des, int *__restrict__ rec_pass,
+                          int *__restrict__ nrej)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nmol) return;
+    const int m = order[k];
+    const float d = des ? des[m] : y[2 * k + 1] - y[2 * k];
+    const float e1 = d * temp;
+    const bool pass = da_uniform(seed, m) < etest_of(e1);
+    shift[k] = pass ? d : __int_as_float(0x7fc00000);
+    if (rec_des) rec_des[k] = d;
+    if (rec_pass) rec_pass[k] = pass ? 1 : 0;
+    if (!pass) atomicAdd(nrej, 1);
+}
+
+// the exact energy change of the move at chain position k over its later partners
+// (still at their cycle-start positions): sum of A - L over those slots, FP64, one wave
+// per move; NaN where the surrogate stage rejected the move (its A was not formed)
+__global__ void __launch_bounds__(64)
+mc_record_target(int nmol, const int *__restrict__ order, const int *__restrict__ pos,
+                 const int *__restrict__ rowstart, const int *__restrict__ cnt, const int *__restrict__ partner,
+                 const float *__restrict__ A, const float *__restrict__ L, const float *__restrict__ shift,
+                 float *__restrict__ target)
+{
+    const int k = blockIdx.x, lane = threadIdx.x;
+    const int m = order[k];
+    const int row = rowstart[m], c = cnt[m];
+    double s = 0.0;
+    for (int j = 1 + lane; j < c; j += 64)
+        if (pos[partner[row + j]] > k) s += (double)A[row + j] - (double)L[row + j];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off);
+    if (lane == 0) target[k] = (shift && isnan(shift[k])) ? __int_as_float(0x7fc00000) : (float)s;
+}
+
 static float adapt_ratio(float rejrate, int att, int rej)
 {
     float rat;
@@ -1697,15 +1723,29 @@ static float adapt_ratio(float rejrate, int att, int rej)
     return rat;
 }
 
-extern "C" int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *c, const epsr_tables_t *tb,
-                           epsr_ledger *l, const epsr_mc_params_t *p, float *wholerot, float *wholetrans, int *nwhole,
-                           int *nmovem, int *nstepreject, int32_t *ran1_state, float *fadds, epsr_mc_stats_t *stats)
+static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *c, const epsr_tables_t *tb,
+                       epsr_ledger *l, const epsr_mc_params_t *p, float *wholerot, float *wholetrans, int *nwhole,
+                       int *nmovem, int *nstepreject, int32_t *ran1_state, float *fadds, const epsr_da_t *da,
+                       const epsr_mc_record_t *rec, epsr_mc_stats_t *stats)
 {
     if (!ctx) return EPSR_ERR_ARG;
     ARG_CHECK(ctx, box && c && tb && l && p && wholerot && wholetrans && nwhole && nmovem && nstepreject &&
                        ran1_state && fadds,
               "epsr_molfit: null argument");
     ARG_CHECK(ctx, l->ledger_ready, "epsr_molfit: no ledger (call epsr_fmolec1)");
+    // delayed acceptance and recording: descriptor rows of width ldx
+    const bool use_net = da && !da->des;
+    ARG_CHECK(ctx, !da || da->des || da->mlp, "epsr_molfit_da: neither a network nor des");
+    const bool want_x = use_net || (rec && rec->x);
+    const epsr_desc_params_t *dpar = da ? &da->desc : (rec ? rec->desc : nullptr);
+    int ldx = 0;
+    if (use_net) ldx = da->mlp->d;
+    if (rec && rec->x) {
+        ARG_CHECK(ctx, !use_net || rec->ldx == ldx, "epsr_molfit_da: rec->ldx=%d differs from the network's d=%d",
+                  rec ? rec->ldx : 0, ldx);
+        ldx = rec->ldx;
+    }
+    ARG_CHECK(ctx, !want_x || dpar, "epsr_molfit_da: recording descriptors needs rec->desc");
     HIP_CHECK(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     const int nmol = box->nmol, natom = box->natom, nmt = box->nmoltype;
@@ -1786,6 +1826,8 @@ extern "C" int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
         x.take<unsigned>((nmol + 31) / 32); x.take<float>(nmol); x.take<float>(4); x.take<int>(2 * nmt + 2);
         x.take<float>(nslots); x.take<int>(4);
         x.take<float2>(fast ? npad : 0); x.take<float>(fast ? (size_t)npad * 64 : 0); x.take<int2>(nent);
+        x.take<float>(da ? 2 * (size_t)nmol : 0); x.take<float>(da ? nmol : 0); x.take<int>(4);
+        x.take<uint16_t>(want_x && !(rec && rec->x) ? 2 * (size_t)nmol * ldx : 0);
     };
     layout(q);
     S.base = (char *)ctx_scratch(ctx, q.off);
@@ -1808,6 +1850,11 @@ extern "C" int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
     float2 *dthr = S.take<float2>(fast ? npad : 0);
     float *dMt = S.take<float>(fast ? (size_t)npad * 64 : 0);
     int2 *dpent = S.take<int2>(nent);
+    float *dy = S.take<float>(da ? 2 * (size_t)nmol : 0);
+    float *dshift = da ? S.take<float>(nmol) : (S.take<float>(0), nullptr);
+    int *dnrej = S.take<int>(4);
+    uint16_t *dX = S.take<uint16_t>(want_x && !(rec && rec->x) ? 2 * (size_t)nmol * ldx : 0);
+    ARG_CHECK(ctx, !da || fast, "epsr_molfit_da: delayed acceptance needs the block resolver (LDS too small)");
 
     PreparedTables pt;
     int rc = prepare_tables(ctx, box, tb, dT, dtmp, pt);
@@ -1900,30 +1947,46 @@ extern "C" int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
                                                         box->d_itype, Pold, natom);
     abs_positions<<<ceil_div(natom, 256), 256, 0, st>>>(txa, tya, tza, txadd, tyadd, tzadd, box->d_amol, box->d_itype,
                                                         Pnew, natom);
+    HIP_CHECK(ctx, hipMemsetAsync(dnrej, 0, sizeof(int), st));
+    if (want_x) {
+        uint16_t *X = (rec && rec->x) ? rec->x : dX;
+        rc = surr_descriptors_launch(ctx, box, l, Pold, Pnew, dpar, ldx, X, nullptr);
+        if (rc) return rc;
+        if (use_net) {
+            rc = surr_forward_launch(ctx, da->mlp, 2 * nmol, X, ldx, dy);
+            if (rc) return rc;
+        }
+    }
+    if (da)
+        da_stage1<<<ceil_div(nmol, 256), 256, 0, st>>>(nmol, l->d_order, dy, da->des, temp, da->seed, dshift,
+                                                      rec ? rec->des : nullptr, rec ? rec->pass : nullptr, dnrej);
     timer_begin(ctx, "mc_pair");
     if (pt.in_lds) {
         int g = persistent_blocks(pt.lds_bytes, nslots, 256);
         mc_pairs<true><<<g, 256, pt.lds_bytes, st>>>(nslots, l->d_slotmol, l->d_rowstart, l->d_partner, l->d_entry,
                                                      l->d_pos, l->d_ergpair, Pold, Pnew, box->d_naf, box->d_nal,
-                                                     pt.tab, box->npairs, dA, dB, dL);
+                                                     pt.tab, box->npairs, dshift, dA, dB, dL);
     } else {
         int g = persistent_blocks(0, nslots, 256);
         mc_pairs<false><<<g, 256, 0, st>>>(nslots, l->d_slotmol, l->d_rowstart, l->d_partner, l->d_entry, l->d_pos,
-                                           l->d_ergpair, Pold, Pnew, box->d_naf, box->d_nal, pt.tab, box->npairs, dA,
-                                           dB, dL);
+                                           l->d_ergpair, Pold, Pnew, box->d_naf, box->d_nal, pt.tab, box->npairs,
+                                           dshift, dA, dB, dL);
     }
     timer_end(ctx, "mc_pair");
+    if (rec && rec->target)
+        mc_record_target<<<nmol, 64, 0, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner, dA,
+                                              dL, dshift, rec->target);
     int nexact = nmol, npushed = 0;
     if (fast) {
         HIP_CHECK(ctx, hipMemsetAsync(dMt, 0, sizeof(float) * (size_t)npad * 64, st));
         mc_prepare<<<nmol, 64, 0, st>>>(nmol, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner, l->d_mirror, dA, dB, dL,
-                                        dstream, dmoff, temp, ddelta, dthr, dMt);
+                                        dstream, dmoff, temp, dshift, ddelta, dthr, dMt);
         mc_push_rows<<<npad, 64, 0, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner,
-                                          l->d_mirror, ddelta, l->d_prow, dpent, dthr);
+                                          l->d_mirror, ddelta, l->d_prow, dshift, dpent, dthr);
         timer_begin(ctx, "mc_resolve");
         mc_resolve_blocks<<<1, 64, lds_fast, st>>>(nmol, npad, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt,
                                                    l->d_partner, l->d_mirror, dA, dB, dL, dthr, dMt, l->d_prow, dpent,
-                                                   dstream, dmoff, temp, cap, dacc, dnex);
+                                                   dstream, dmoff, temp, dshift, cap, dacc, dnex);
         timer_end(ctx, "mc_resolve");
         mc_fdiff_accepted<<<ceil_div(nmol, 4), 256, sizeof(float2) * 4 * (size_t)maxc, st>>>(
             nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner, l->d_mirror, dA, dB, dL, dacc, maxc,
@@ -1951,7 +2014,9 @@ extern "C" int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
                                                         c->za, c->xadd, c->yadd, c->zadd, nstepreject,
                                                         p->stepr > 0.0f ? 1 : 0, drej);
     float hres[2];
+    int hnrej = 0;
     std::vector<int> rej(2 * nmt);
+    HIP_CHECK(ctx, hipMemcpyAsync(&hnrej, dnrej, sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_CHECK(ctx, hipMemcpyAsync(hres, dres, sizeof(float) * 2, hipMemcpyDeviceToHost, st));
     HIP_CHECK(ctx, hipMemcpyAsync(rej.data(), drej, sizeof(int) * 2 * nmt, hipMemcpyDeviceToHost, st));
     HIP_CHECK(ctx, hipGetLastError());
@@ -1991,8 +2056,26 @@ extern "C" int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
         stats->retries = retries;
         stats->ms_pairs = ctx->timing ? epsr_ctx_last_kernel_ms(ctx, "mc_pair") : -1.0;
         stats->ms_resolve = ctx->timing ? epsr_ctx_last_kernel_ms(ctx, "mc_resolve") : -1.0;
+        stats->stage1_rejected = hnrej;
     }
     return EPSR_OK;
+}
+
+extern "C" int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *c, const epsr_tables_t *tb,
+                           epsr_ledger *l, const epsr_mc_params_t *p, float *wholerot, float *wholetrans, int *nwhole,
+                           int *nmovem, int *nstepreject, int32_t *ran1_state, float *fadds, epsr_mc_stats_t *stats)
+{
+    return molfit_impl(ctx, box, c, tb, l, p, wholerot, wholetrans, nwhole, nmovem, nstepreject, ran1_state, fadds,
+                       nullptr, nullptr, stats);
+}
+
+extern "C" int epsr_molfit_da(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *c, const epsr_tables_t *tb,
+                              epsr_ledger *l, const epsr_mc_params_t *p, float *wholerot, float *wholetrans,
+                              int *nwhole, int *nmovem, int *nstepreject, int32_t *ran1_state, float *fadds,
+                              const epsr_da_t *da, const epsr_mc_record_t *rec, epsr_mc_stats_t *stats)
+{
+    return molfit_impl(ctx, box, c, tb, l, p, wholerot, wholetrans, nwhole, nmovem, nstepreject, ran1_state, fadds,
+                       da, rec, stats);
 }
 
 

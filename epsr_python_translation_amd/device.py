@@ -345,6 +345,37 @@ def molfit(ctx: Context, box: DeviceBox, tables: DeviceTables, ledger: Ledger, m
     return mc
 
 
+def molfit_da(ctx: Context, box: DeviceBox, tables: DeviceTables, ledger: Ledger, mc: MCState, ran1_state,
+              da=None, record=None):
+    """One MC cycle with delayed acceptance (da: a _native.DA, or None) and/or recording
+    (record: a _native.MCRecord, or None); with both None it is molfit."""
+    c = box.coords()
+    t = tables.struct()
+    fad = C.c_float(float(mc.fadds))
+    P = lambda a, ty=C.c_int: a.ctypes.data_as(C.POINTER(ty))
+    ctx.check(ctx.lib.epsr_molfit_da(ctx.h, box.h, C.byref(c), C.byref(t), ledger.h, C.byref(mc.params),
+                                     P(mc.wholerot, C.c_float), P(mc.wholetrans, C.c_float), P(mc.nwhole),
+                                     P(mc.nmovem), _ptr(mc.nstepreject), P(ran1_state, C.c_int32), C.byref(fad),
+                                     C.byref(da) if da is not None else None,
+                                     C.byref(record) if record is not None else None, C.byref(mc.stats)))
+    mc.fadds = F32(fad.value)
+    return mc
+
+
+def descriptors(ctx: Context, box: DeviceBox, ledger: Ledger, trial: dict, dp, ldx, with_f32=False):
+    """Descriptor rows 2k (current) and 2k+1 (trial) of every chain position; trial maps
+    xa..zadd to device tensors of the trial coordinates.  Returns (x bf16 bits as int16
+    [2 nmol, ldx], x fp32 or None)."""
+    n = 2 * box.nmol
+    x = ctx.zeros(n * ldx, torch.int16)
+    xf = ctx.zeros(n * ldx) if with_f32 else None
+    c = box.coords()
+    tc = _native.Coords(*(_ptr(trial[k]) for k in ("xa", "ya", "za", "xadd", "yadd", "zadd")))
+    ctx.check(ctx.lib.epsr_descriptors(ctx.h, box.h, C.byref(c), C.byref(tc), ledger.h, C.byref(dp), int(ldx),
+                                       _ptr(x), _ptr(xf)))
+    return x.view(n, ldx), (xf.view(n, ldx) if xf is not None else None)
+
+
 # ---------------------------------------------------------------- EP fit
 
 SUMMAX = F32(np.exp(25.0))   # expf(25) (epsr_standalone_rev3.py:4722)

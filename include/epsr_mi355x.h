@@ -189,11 +189,92 @@ typedef struct {
     double ms_pairs, ms_resolve;
     long long pushed_entries;
     double resolver_bytes;
+    int stage1_rejected;   /* delayed acceptance: moves the surrogate stage rejected */
 } epsr_mc_stats_t;
 
 int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *coords, const epsr_tables_t *tables,
                 epsr_ledger *l, const epsr_mc_params_t *p, float *wholerot, float *wholetrans, int *nwhole,
                 int *nmovem, int *nstepreject, int32_t *ran1_state, float *fadds, epsr_mc_stats_t *stats);
+
+/* ==== learned Delta-E surrogate and delayed-acceptance MC =====================
+ * No counterpart in the reference: these entry points add a two-stage Metropolis step
+ * beside molfit.  The surrogate is a site-energy network f over radial descriptors of a
+ * molecule: for the move of the molecule at chain position k, dEs = f(d_k(trial)) -
+ * f(d_k(current)), where d_k sums, over the molecule's ledger partners that come LATER in
+ * the chain (still at their positions at the start of the cycle when k moves), Gaussians
+ * of every atom-pair distance below rc into the channel of the pair's types.  Since dEs
+ * depends only on the current state and the trial and changes sign when the two are
+ * swapped, the two-stage rule
+ *   stage 1: accept with min(1, exp(-dEs/simt))      (uniform u1 = splitmix64(seed, molecule))
+ *   stage 2: accept with min(1, exp(-(dE - dEs)/simt))  (the ran1 number of plain molfit)
+ * keeps detailed balance with respect to the exact energy (Christen and Fox, 2005); dE is
+ * en_change's fdiff, formed exactly as in plain molfit. */
+
+/* descriptor definition: Gaussian g of channel c = npairref(type_a, type_b) adds
+ * exp(-((r - (r0 + g*dmu)) / width)^2) * 0.5*(cos(pi r / rc) + 1) for r < rc (Angstrom),
+ * summed in 2^-32 fixed point (order-independent, deterministic), times xscale */
+typedef struct {
+    int ng;             /* Gaussians per channel, 1..32 */
+    float rc, r0, dmu, width, xscale;
+} epsr_desc_params_t;
+
+/* descriptors of every chain position k for the current coordinates (row 2k) and the
+ * trial coordinates of its molecule (row 2k+1), over its later ledger partners at their
+ * current coordinates.  x dev bf16 [2*nmol*ldx], ldx >= npairs*ng, multiple of 32 (zero
+ * padded); xf dev fp32 [2*nmol*ldx] or NULL. */
+int epsr_descriptors(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *coords, const epsr_coords_t *trial,
+                     const epsr_ledger *l, const epsr_desc_params_t *dp, int ldx, uint16_t *x, float *xf);
+
+/* the network: y = w3 . silu(W2 silu(W1 x + b1) + b2) + b3, W1 [h][d], W2 [h][h];
+ * parameters in one fp32 vector w = [W1 | b1 | W2 | b2 | w3 | b3] (master copy), wb the
+ * bf16 copy of W1 then W2 the forward pass reads (epsr_mlp_sync refreshes it), m and v the
+ * Adam moments (NULL when not training).  d a multiple of 32, h = 128. */
+typedef struct {
+    int d, h;
+    float *w;
+    uint16_t *wb;
+    float *m, *v;
+    int step;
+} epsr_mlp_t;
+long long epsr_mlp_nparams(int d, int h);
+int epsr_mlp_sync(epsr_ctx *ctx, const epsr_mlp_t *mlp);
+/* y dev [nrows] = f(x row), x dev bf16 [nrows*ldx]; BF16 MFMA, FP32 accumulation */
+int epsr_mlp_forward(epsr_ctx *ctx, const epsr_mlp_t *mlp, int nrows, const uint16_t *x, int ldx, float *y);
+/* loss = mean_i (f(x[2i+1]) - f(x[2i]) - target[i])^2 over npair pairs and its gradient
+ * (dev [nparams], the layout of w) with respect to w through the bf16 forward pass; loss
+ * host out (nullable) */
+int epsr_mlp_grad(epsr_ctx *ctx, const epsr_mlp_t *mlp, int npair, const uint16_t *x, int ldx, const float *target,
+                  float *grad, float *loss);
+/* one Adam step on w (m, v, step updated) and the bf16 copy refreshed */
+int epsr_mlp_adam(epsr_ctx *ctx, epsr_mlp_t *mlp, const float *grad, float lr, float beta1, float beta2, float eps);
+
+/* delayed acceptance: the network, the descriptor definition, the stage-1 seed, and
+ * des (dev [nmol] by molecule, or NULL) that replaces the network's dEs when given */
+typedef struct {
+    const epsr_mlp_t *mlp;
+    epsr_desc_params_t desc;
+    unsigned long long seed;
+    const float *des;
+} epsr_da_t;
+
+/* what a cycle records by chain position (each pointer nullable): the descriptors (x,
+ * rows 2k and 2k+1, as epsr_descriptors), the exact energy change over the later
+ * partners (target, the quantity dEs models), dEs and the stage-1 outcome (pass) */
+typedef struct {
+    const epsr_desc_params_t *desc;   /* used when the cycle has no delayed acceptance */
+    int ldx;
+    uint16_t *x;
+    float *target;
+    float *des;
+    int *pass;
+} epsr_mc_record_t;
+
+/* molfit with delayed acceptance (da non-NULL) and/or recording (rec non-NULL); with
+ * both NULL it is epsr_molfit. */
+int epsr_molfit_da(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *coords, const epsr_tables_t *tables,
+                   epsr_ledger *l, const epsr_mc_params_t *p, float *wholerot, float *wholetrans, int *nwhole,
+                   int *nmovem, int *nstepreject, int32_t *ran1_state, float *fadds, const epsr_da_t *da,
+                   const epsr_mc_record_t *rec, epsr_mc_stats_t *stats);
 
 /* ---- truncation functions of initpt (src/epsr26/programs/epsr_standalone_rev3.py:3755-3767):
  * wfac = truncate(x, rminpt, rmaxpt), wfacc = truncatec(x, rmaxpt)

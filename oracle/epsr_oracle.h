@@ -160,6 +160,29 @@ int ora_molfit(const ora_box_t *b, float *xa, float *ya, float *za, const ora_ce
                const ora_tables_t *t, const ora_cells_t *cl, ora_ledger_t *l,
                ora_mcstate_t *mc, ora_ran1_t *rng, float *fadds, int *naccept_out);
 
+/* ---- delayed acceptance (no counterpart in the reference) ----
+ * ora_molfit with a two-stage Metropolis test: des[nmol] is the surrogate's dEs of each
+ * molecule's move; stage 1 passes when ora_da_uniform(seed, im) < etest(dEs/simt), stage 2
+ * tests ran1 < etest((fdiff - dEs)/simt); the ran1 number is drawn for every move.  des NULL
+ * is plain molfit.  target_out[nmol] (nullable, by molecule): the exact energy change of
+ * each move over its partners later in the chain, FP64 sum of en_change's terms;
+ * fdiff_out[nmol] (nullable, by molecule): en_change's fdiff of every move. */
+float ora_da_uniform(unsigned long long seed, int m);
+int ora_molfit_da(const ora_box_t *b, float *xa, float *ya, float *za, const ora_centres_t *c,
+                  const ora_tables_t *t, const ora_cells_t *cl, ora_ledger_t *l,
+                  ora_mcstate_t *mc, ora_ran1_t *rng, float *fadds, int *naccept_out,
+                  const float *des, unsigned long long seed, int *nrej_out, double *target_out,
+                  float *fdiff_out);
+
+/* ---- the surrogate's descriptors and network (ora_surrogate.c) ---- */
+void ora_descriptor(int na, const float *xm, const float *ym, const float *zm, const int *tm, int nb,
+                    const float *xn, const float *yn, const float *zn, const int *tn, const int *npairref,
+                    int ntype, int npairs, float dmax, int ng, float rc, float r0, float dmu, float width,
+                    float xscale, double *out);
+void ora_mlp_forward(int d, int h, const float *w, int bf16, int nrows, const float *x, int ldx, double *y);
+double ora_mlp_grad(int d, int h, const float *w, int bf16, int npair, const float *x, int ldx,
+                    const float *target, double *grad);
+
 /* ---- grids (src/epsr26/programs/epsr_standalone_rev3.py:2947-3071, 3319-3399, 5391-5419) ---- */
 typedef struct { int npt, npt1, npt2, nptmin, nptmax; } ora_grid_t;
 void  ora_inita_grid(float cellst, float rmaxgr, float rminpt, float rmaxpt, int mpz,

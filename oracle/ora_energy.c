@@ -397,9 +397,28 @@ static float adapt_ratio(float rejrate, int att, int rej)
 
 /* molfit.py:51-541: whole-molecule rotation + translation branch (nisum < intra,
  * no internal rotational groups, no tethered molecules). */
-int ora_molfit(const ora_box_t *b0, float *xa, float *ya, float *za, const ora_centres_t *c,
-               const ora_tables_t *t, const ora_cells_t *cl, ora_ledger_t *l,
-               ora_mcstate_t *mc, ora_ran1_t *rng, float *fadds, int *naccept_out)
+/* the splitmix64 uniform of delayed acceptance's stage 1 (the GPU's da_uniform) */
+float ora_da_uniform(unsigned long long seed, int m)
+{
+    unsigned long long z = seed + 0x9E3779B97F4A7C15ull * (unsigned long long)(m + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return ((float)(z >> 40) + 0.5f) * (1.0f / 16777216.0f);
+}
+
+static float etest_of(float ediff)
+{
+    if (ediff <= 0.0f) return 10.0f;
+    if (ediff < 20.0f) return ora_expf(-ediff);
+    return 0.0f;
+}
+
+static int molfit_impl(const ora_box_t *b0, float *xa, float *ya, float *za, const ora_centres_t *c,
+                       const ora_tables_t *t, const ora_cells_t *cl, ora_ledger_t *l,
+                       ora_mcstate_t *mc, ora_ran1_t *rng, float *fadds, int *naccept_out,
+                       const float *des, unsigned long long seed, int *nrej_out, double *target_out,
+                       float *fdiff_out)
 {
     ora_box_t bb = *b0;
     bb.xa = xa; bb.ya = ya; bb.za = za;
@@ -416,12 +435,16 @@ int ora_molfit(const ora_box_t *b0, float *xa, float *ya, float *za, const ora_c
     if (trstepmin < 1.0e-6f) trstepmin = 1.0e-6f;
     const float rotstepmax = 1.0f, rotstepmin = 1.0e-5f;
     float fadd = 0.0f;
-    int nacc = 0;
+    int nacc = 0, nrej1 = 0;
     int maxat = 0;
     for (int im = 0; im < b->nmol; ++im)
         if (b->nal[im] - b->naf[im] + 1 > maxat) maxat = b->nal[im] - b->naf[im] + 1;
     float *xr = malloc(sizeof(float) * (size_t)maxat * 3);
     float *ergnew = malloc(sizeof(float) * (size_t)(M > 0 ? M : 1));
+    /* chain position of every molecule (cells in order, molecules in a cell) */
+    int *cpos = malloc(sizeof(int) * (size_t)b->nmol);
+    for (int cc = 0, k = 0; cc < cl->ncell; ++cc)
+        for (int iref = 0; iref < cl->nmolsincell[cc]; ++iref) cpos[cl->molsincell[(size_t)cc * cl->maxcellno + iref]] = k++;
     for (int cc = 0; cc < cl->ncell; ++cc) {
         for (int iref = 0; iref < cl->nmolsincell[cc]; ++iref) {
             int im = cl->molsincell[(size_t)cc * cl->maxcellno + iref];
@@ -448,13 +471,43 @@ int ora_molfit(const ora_box_t *b0, float *xa, float *ya, float *za, const ora_c
                 mc->wholetrans[mt] = 0.0f;
             }
             int nnbref = 0;
-            float fdiff = ora_en_change(b, c, t, l, im, ergnew, &nnbref);
-            float ediff = (fdiff + 0.0f) * temp;
-            float etest;
-            if (ediff <= 0.0f) etest = 10.0f;
-            else if (ediff < 20.0f) etest = ora_expf(-ediff);
-            else etest = 0.0f;
-            if (ora_ran1(rng) < etest) {
+            int accept;
+            float fdiff = 0.0f;
+            if (target_out) {
+                /* exact energy change over the partners later in the chain */
+                int nn2 = 0;
+                (void)ora_en_change(b, c, t, l, im, ergnew, &nn2);
+                double s = 0.0;
+                for (int k = 1; k < nn2; ++k) {
+                    int jm = l->molergpair[(size_t)im * M + k];
+                    if (cpos[jm] > cpos[im])
+                        s += (double)ergnew[k] - (double)l->ergpair[l->ergpairref[(size_t)im * M + k]];
+                }
+                target_out[im] = s;
+            }
+            if (fdiff_out) {
+                int nn3 = 0;
+                fdiff_out[im] = ora_en_change(b, c, t, l, im, ergnew, &nn3);
+            }
+            if (!des) {
+                fdiff = ora_en_change(b, c, t, l, im, ergnew, &nnbref);
+                float ediff = (fdiff + 0.0f) * temp;
+                accept = ora_ran1(rng) < etest_of(ediff);
+            } else {
+                /* delayed acceptance: stage 1 on the surrogate's dEs, stage 2 on dE - dEs;
+                 * the ran1 number of the Metropolis test is drawn either way */
+                float d = des[im];
+                int pass = ora_da_uniform(seed, im) < etest_of(d * temp);
+                if (pass) {
+                    fdiff = ora_en_change(b, c, t, l, im, ergnew, &nnbref);
+                    accept = ora_ran1(rng) < etest_of((fdiff - d) * temp);
+                } else {
+                    (void)ora_ran1(rng);
+                    accept = 0;
+                    nrej1++;
+                }
+            }
+            if (accept) {
                 mc->nstepreject[im] = 0;
                 fadd = fadd + fdiff;
                 nacc++;
@@ -475,6 +528,7 @@ int ora_molfit(const ora_box_t *b0, float *xa, float *ya, float *za, const ora_c
     }
     free(xr);
     free(ergnew);
+    free(cpos);
     if (mc->ireset != 1) {
         for (int mt = 0; mt < b->nmoltype; ++mt) {
             float rat = adapt_ratio(mc->rejrate, mc->nwhole[2 * mt], mc->nwhole[2 * mt + 1]);
@@ -489,5 +543,23 @@ int ora_molfit(const ora_box_t *b0, float *xa, float *ya, float *za, const ora_c
     }
     *fadds = *fadds + fadd / (float)b->nmol;
     if (naccept_out) *naccept_out = nacc;
+    if (nrej_out) *nrej_out = nrej1;
     return 0;
+}
+
+int ora_molfit(const ora_box_t *b, float *xa, float *ya, float *za, const ora_centres_t *c,
+               const ora_tables_t *t, const ora_cells_t *cl, ora_ledger_t *l,
+               ora_mcstate_t *mc, ora_ran1_t *rng, float *fadds, int *naccept_out)
+{
+    return molfit_impl(b, xa, ya, za, c, t, cl, l, mc, rng, fadds, naccept_out, NULL, 0ull, NULL, NULL, NULL);
+}
+
+int ora_molfit_da(const ora_box_t *b, float *xa, float *ya, float *za, const ora_centres_t *c,
+                  const ora_tables_t *t, const ora_cells_t *cl, ora_ledger_t *l,
+                  ora_mcstate_t *mc, ora_ran1_t *rng, float *fadds, int *naccept_out,
+                  const float *des, unsigned long long seed, int *nrej_out, double *target_out,
+                  float *fdiff_out)
+{
+    return molfit_impl(b, xa, ya, za, c, t, cl, l, mc, rng, fadds, naccept_out, des, seed, nrej_out, target_out,
+                       fdiff_out);
 }

@@ -170,6 +170,14 @@ def _declare(L):
         "ora_ptgnsq_diff": (None, [C.c_int, C.c_int, C.c_int, fp, fp, C.c_int, fp, fp, fp, ip, ip, C.c_float,
                                    fp]),
         "ora_ptgnsq_rfactor": (C.c_float, [C.c_int, C.c_int, fp, fp]),
+        "ora_da_uniform": (C.c_float, [C.c_ulonglong, C.c_int]),
+        "ora_molfit_da": (C.c_int, [P(Box), fp, fp, fp, P(Centres), P(Tables), P(Cells), P(Ledger),
+                                    P(MCState), P(Ran1), fp, ip, fp, C.c_ulonglong, ip, dp, fp]),
+        "ora_descriptor": (None, [C.c_int, fp, fp, fp, ip, C.c_int, fp, fp, fp, ip, ip, C.c_int, C.c_int,
+                                  C.c_float, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
+                                  dp]),
+        "ora_mlp_forward": (None, [C.c_int, C.c_int, fp, C.c_int, C.c_int, fp, C.c_int, dp]),
+        "ora_mlp_grad": (C.c_double, [C.c_int, C.c_int, fp, C.c_int, C.c_int, fp, C.c_int, fp, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -1,0 +1,339 @@
+"""The learned surrogate and delayed-acceptance MC on the GPU against the oracle.
+
+* descriptors: the GPU rows (fp32) against the oracle's FP64 evaluation over the same
+  partners (rtol 2e-5: float32 expf/cosf and 2^-32 fixed point against FP64), and the bf16
+  rows are the round-to-nearest-even of the fp32 rows, bit for bit;
+* the network's forward pass and gradient against the oracle's FP64 pass with the same
+  bf16 roundings (weights, inputs, first-layer activations): y within 1e-3 (1 + max|y|),
+  each gradient block within 2e-3 of its largest element;
+* delayed acceptance: with dEs = 0 the cycle is plain molfit bit for bit; with a network's
+  dEs the chain (coordinates, ledger, counters, ran1 state) equals the oracle's two-stage
+  chain given the same dEs, bit for bit (correctly rounded libm on both sides);
+* the recorded later-partner energy change against the oracle's FP64 sum of en_change's
+  terms (rtol 1e-6);
+* what the trained network gains: accepted moves per exact energy evaluation against the
+  plain cycle, where every move needs one."""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import pyoracle as po
+from tests import helpers as H
+from tests.test_gpu_mc import _gpu_setup
+
+F32 = np.float32
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from epsr_python_translation_amd import device as D
+    return D.Context(0)
+
+
+def _abs_positions(box_np, s):
+    amol = np.repeat(np.arange(s.nmol), s.nal - s.naf + 1)
+    return ((box_np["xa"] + box_np["xadd"][amol]).astype(F32), (box_np["ya"] + box_np["yadd"][amol]).astype(F32),
+            (box_np["za"] + box_np["zadd"][amol]).astype(F32))
+
+
+def _bf16_rne(v):
+    u = v.astype(F32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return u.astype(np.uint16)
+
+
+def _chain(led, s):
+    ex = led.export()
+    order = ex["order"]
+    pos = np.empty_like(order)
+    pos[order] = np.arange(len(order))
+    return ex, order, pos
+
+
+def test_descriptors_against_oracle(ctx):
+    from epsr_python_translation_amd import device as D
+    from epsr_python_translation_amd import surrogate as SG
+    name = "c0"
+    print(f"descriptors {name}", flush=True)
+    s = H.system(name)
+    box, tab, led, _, _ = _gpu_setup(ctx, name)
+    rng = np.random.default_rng(1)
+    cur = box.get_coords()
+    trial = {k: torch.as_tensor(v).cuda() for k, v in cur.items()}
+    for k in ("xadd", "yadd", "zadd"):
+        trial[k] = torch.as_tensor((cur[k] + rng.uniform(-0.004, 0.004, s.nmol)).astype(F32)).cuda()
+    spec = SG.DescriptorSpec()
+    ldx = spec.width_for(s.npairs)
+    x, xf = D.descriptors(ctx, box, led, trial, spec.params(), ldx, with_f32=True)
+    torch.cuda.synchronize()
+    x = x.cpu().numpy().view(np.uint16)
+    xf = xf.cpu().numpy()
+    assert np.array_equal(x, _bf16_rne(xf))
+    ex, order, pos = _chain(led, s)
+    tnp = {k: v.cpu().numpy() for k, v in trial.items()}
+    Po, Pt = _abs_positions(cur, s), _abs_positions(tnp, s)
+    L = po.lib("cr")
+    npr = np.ascontiguousarray(s.npairref.ravel(), np.int32)
+    D_ = s.npairs * spec.ng
+    ks = sorted(set([0, 1, s.nmol - 1] + list(rng.choice(s.nmol, 30, replace=False))))
+    nonzero = 0
+    for k in ks:
+        m = int(order[k])
+        part = ex["molergpair"][m, 1:ex["nergpairm"][m]]
+        later = [int(n) for n in part if pos[n] > k]
+        atoms_m = np.arange(s.naf[m], s.nal[m] + 1)
+        atoms_n = np.concatenate([np.arange(s.naf[n], s.nal[n] + 1) for n in later]) if later else np.zeros(0, int)
+        for row, P in ((2 * k, Po), (2 * k + 1, Pt)):
+            out = np.zeros(D_)
+            L.ora_descriptor(len(atoms_m), po.Pf(P[0][atoms_m].copy()), po.Pf(P[1][atoms_m].copy()),
+                             po.Pf(P[2][atoms_m].copy()), po.Pi(s.itype[atoms_m].astype(np.int32).copy()),
+                             len(atoms_n), po.Pf(Po[0][atoms_n].copy()), po.Pf(Po[1][atoms_n].copy()),
+                             po.Pf(Po[2][atoms_n].copy()), po.Pi(s.itype[atoms_n].astype(np.int32).copy()),
+                             po.Pi(npr), s.ntype, s.npairs, F32(s.dmax), spec.ng, F32(spec.rc), F32(spec.r0),
+                             F32(spec.dmu), F32(spec.width), F32(spec.xscale), out.ctypes.data_as(po.dp))
+            np.testing.assert_allclose(xf[row, :D_], out, rtol=2e-5, atol=2e-5, err_msg=f"row {row}")
+            assert not xf[row, D_:].any()
+            nonzero += np.count_nonzero(out)
+    assert nonzero > 0
+    # the last chain position has no later partner: zero rows
+    assert not xf[2 * (s.nmol - 1)].any() and not xf[2 * (s.nmol - 1) + 1].any()
+
+
+def _bits_to_f32(x):
+    return (x.cpu().numpy().view(np.uint16).astype(np.uint32) << 16).view(F32)
+
+
+def test_network_forward_and_gradient_against_oracle(ctx):
+    from epsr_python_translation_amd import surrogate as SG
+    d, npair = 672, 48
+    rng = np.random.default_rng(2)
+    w = SG.Network.initial(d, seed=3)
+    w[-(SG.H + 1):-1] *= 8.0   # a larger output layer: gradients of every block well above rounding
+    net = SG.Network(ctx, d, w=w)
+    xv = np.abs(rng.normal(0, 1.0, (2 * npair, d))).astype(F32)
+    xb = _bf16_rne(xv)
+    x = torch.as_tensor(xb.view(np.int16)).cuda()
+    t = rng.normal(0, 1, npair).astype(F32)
+    tt = torch.as_tensor(t).cuda()
+    y = net.forward(x, d).cpu().numpy()
+    xr = ((xb.astype(np.uint32) << 16).view(F32)).ravel()
+    L = po.lib("cr")
+    yo = np.zeros(2 * npair)
+    L.ora_mlp_forward(d, SG.H, po.Pf(w), 1, 2 * npair, po.Pf(xr), d, yo.ctypes.data_as(po.dp))
+    print(f"forward: max |y| {np.abs(yo).max():.4g}, max err {np.abs(y - yo).max():.3g}", flush=True)
+    assert np.abs(y - yo).max() <= 1e-3 * (1.0 + np.abs(yo).max())
+    g, loss = net.grad(x, tt, d)
+    g = g.cpu().numpy()
+    go = np.zeros(len(w))
+    lo = L.ora_mlp_grad(d, SG.H, po.Pf(w), 1, npair, po.Pf(xr), d, po.Pf(t), go.ctypes.data_as(po.dp))
+    assert loss == pytest.approx(lo, rel=2e-3)
+    h = SG.H
+    blocks = {"W1": (0, h * d), "b1": (h * d, h), "W2": (h * d + h, h * h), "b2": (h * d + h + h * h, h),
+              "w3": (h * d + 2 * h + h * h, h)}
+    for nm, (o, sz) in blocks.items():
+        a, b = g[o:o + sz], go[o:o + sz]
+        err = np.abs(a - b).max()
+        print(f"grad {nm}: max |g| {np.abs(b).max():.4g}, max err {err:.3g}", flush=True)
+        assert err <= 2e-3 * np.abs(b).max(), nm
+    assert g[-1] == 0.0
+
+
+def test_adam_step_and_training_reduce_loss(ctx):
+    from epsr_python_translation_amd import surrogate as SG
+    d, npair = 64, 512
+    rng = np.random.default_rng(4)
+    net = SG.Network(ctx, d, seed=5)
+    w0 = net.parameters().copy()
+    g = torch.as_tensor(rng.normal(0, 1, net.n).astype(F32)).cuda()
+    net.adam(g, lr=1e-2)
+    gn = g.cpu().numpy().astype(np.float64)
+    m = 0.1 * gn
+    v = 0.001 * gn * gn
+    want = w0 - 1e-2 * (m / 0.1) / (np.sqrt(v / 0.001) + 1e-8)
+    np.testing.assert_allclose(net.parameters(), want, rtol=1e-5, atol=1e-6)
+    # a smooth target of the descriptor difference: the loss falls under training
+    net = SG.Network(ctx, d, seed=6)
+    xv = np.abs(rng.normal(0, 1, (2 * npair, d))).astype(F32)
+    t = (xv[1::2, :8].sum(1) - xv[0::2, :8].sum(1)).astype(F32)
+    x = torch.as_tensor(_bf16_rne(xv).view(np.int16)).cuda()
+    tt = torch.as_tensor(t).cuda()
+    hist = net.train(x.view(2 * npair, d), tt, d, epochs=60, batch=128, lr=3e-3)
+    print(f"training loss {hist[0]:.4g} -> {hist[-1]:.4g}", flush=True)
+    assert hist[-1] < 0.2 * hist[0]
+
+
+def _mc(ctx, s, wr, wt, ireset=1):
+    from epsr_python_translation_amd import device as D
+    return D.MCState(ctx, s.nmol, s.nmoltype, s.simt, wholerot=wr.copy(), wholetrans=wt.copy(), ireset=ireset)
+
+
+def test_zero_surrogate_is_plain_molfit(ctx):
+    from epsr_python_translation_amd import device as D
+    from epsr_python_translation_amd import surrogate as SG
+    name = "c0"
+    print(f"delayed acceptance, dEs = 0, {name}", flush=True)
+    s = H.system(name)
+    wr, wt = H.initial_steps(s)
+    outs = []
+    for da in (False, True):
+        box, tab, led, _, _ = _gpu_setup(ctx, name)
+        mc = _mc(ctx, s, wr, wt, ireset=0)
+        st = D.ran1_state_array(-31)
+        zero = ctx.zeros(s.nmol)
+        dam = SG.DelayedAcceptance(None, SG.DescriptorSpec(), seed=9, des=zero)
+        for _ in range(2):
+            if da:
+                dam.sweep(ctx, box, tab, led, mc, st)
+                assert mc.stats.stage1_rejected == 0
+            else:
+                D.molfit(ctx, box, tab, led, mc, st)
+        torch.cuda.synchronize()
+        outs.append((box.get_coords(), led.export()["ergpair"], st.copy(), mc.wholerot.copy(), mc.nwhole.copy(),
+                     F32(mc.fadds)))
+    a, b = outs
+    for k in a[0]:
+        assert np.array_equal(a[0][k].view(np.int32), b[0][k].view(np.int32)), k
+    assert np.array_equal(a[1].view(np.int32), b[1].view(np.int32))
+    np.testing.assert_array_equal(a[2], b[2])
+    assert np.array_equal(a[3].view(np.int32), b[3].view(np.int32))
+    np.testing.assert_array_equal(a[4], b[4])
+    assert a[5].view(np.int32) == b[5].view(np.int32)
+
+
+@pytest.mark.parametrize("name,nsweep", [("tiny_13_12_3", 2), ("c0", 2)])
+def test_delayed_acceptance_chain_against_oracle(ctx, name, nsweep):
+    from epsr_python_translation_amd import device as D
+    from epsr_python_translation_amd import surrogate as SG
+    print(f"delayed acceptance parity {name} sweeps={nsweep}", flush=True)
+    s = H.system(name)
+    spec = SG.DescriptorSpec()
+    ldx = spec.width_for(s.npairs)
+    w = SG.Network.initial(ldx, seed=11)
+    w[-(SG.H + 1):-1] *= 40.0   # dEs of order simt: stage 1 rejects a good share
+    net = SG.Network(ctx, ldx, w=w)
+    box, tab, led, _, _ = _gpu_setup(ctx, name)
+    wr, wt = H.initial_steps(s)
+    mc = _mc(ctx, s, wr, wt, ireset=0)
+    st = D.ran1_state_array(-4242)
+    dam = SG.DelayedAcceptance(net, spec, seed=1234)
+    rec = SG.Recorder(ctx, s.nmol, spec, ldx)
+    seeds, dess = [], []
+    _, order, _ = _chain(led, s)
+    nrej = 0
+    for _ in range(nsweep):
+        seeds.append(dam.seed)
+        dam.sweep(ctx, box, tab, led, mc, st, record=rec)
+        torch.cuda.synchronize()
+        d = np.empty(s.nmol, F32)
+        d[order] = rec.des.cpu().numpy()
+        dess.append(d)
+        nrej += mc.stats.stage1_rejected
+        print(f"  sweep: accepted {mc.stats.accepted}/{mc.stats.moves}, stage-1 rejected "
+              f"{mc.stats.stage1_rejected}, exact resolutions {mc.stats.exact_resolutions}", flush=True)
+    assert nrej > 0
+    # the oracle's two-stage chain with the same dEs and seeds
+    ref = H.oracle_fmolec1(name, "cr")
+    ob = ref["ob"]
+    wr2, wt2 = H.initial_steps(s)
+    nwh = np.zeros(2 * s.nmoltype, np.int32)
+    nmv = np.zeros(2 * s.nmoltype, np.int32)
+    nsr = np.zeros(s.nmol, np.int32)
+    omc = po.MCState(po.Pf(wr2), po.Pf(wt2), po.Pi(nwh), po.Pi(nmv), po.Pi(nsr), s.simt, F32(1), F32(1), F32(0.75), 0)
+    rng = po.Ran1.seeded(-4242)
+    L = po.lib("cr")
+    fadds, nacc, orej = C.c_float(0.0), C.c_int(), C.c_int()
+    for seed, d in zip(seeds, dess):
+        L.ora_molfit_da(C.byref(ob.box), po.Pf(ob.xa), po.Pf(ob.ya), po.Pf(ob.za), C.byref(ob.centres),
+                        C.byref(ref["ctab"]), C.byref(ref["cells"]), C.byref(ref["ledger"]), C.byref(omc),
+                        C.byref(rng), C.byref(fadds), C.byref(nacc), po.Pf(d), C.c_ulonglong(seed), C.byref(orej),
+                        None, None)
+    co = box.get_coords()
+    for k in ("xa", "ya", "za", "xadd", "yadd", "zadd"):
+        assert np.array_equal(co[k].view(np.int32), getattr(ob, k).view(np.int32)), k
+    np.testing.assert_array_equal(st, rng.as_array())
+    assert np.array_equal(mc.wholerot.view(np.int32), wr2.view(np.int32))
+    assert np.array_equal(mc.wholetrans.view(np.int32), wt2.view(np.int32))
+    np.testing.assert_array_equal(mc.nwhole, nwh)
+    np.testing.assert_array_equal(mc.nmovem, nmv)
+    np.testing.assert_array_equal(mc.nstepreject.cpu().numpy(), nsr)
+    assert F32(mc.fadds).view(np.int32) == F32(fadds.value).view(np.int32)
+    gd = led.export()
+    assert np.array_equal(gd["ergpair"].view(np.int32), H.ledger_dict(ref["ledger"], s.nmol)["ergpair"].view(np.int32))
+
+
+def test_recorded_target_against_oracle(ctx):
+    from epsr_python_translation_amd import device as D
+    from epsr_python_translation_amd import surrogate as SG
+    name = "c0"
+    s = H.system(name)
+    spec = SG.DescriptorSpec()
+    ldx = spec.width_for(s.npairs)
+    box, tab, led, _, _ = _gpu_setup(ctx, name)
+    wr, wt = H.initial_steps(s)
+    mc = _mc(ctx, s, wr, wt, ireset=0)
+    st = D.ran1_state_array(-77)
+    X, T = SG.collect(ctx, box, tab, led, mc, st, spec, 1, ldx)
+    torch.cuda.synchronize()
+    _, order, _ = _chain(led, s)
+    tg = np.empty(s.nmol, F32)
+    tg[order] = T.cpu().numpy()
+    ref = H.oracle_fmolec1(name, "cr")
+    ob = ref["ob"]
+    wr2, wt2 = H.initial_steps(s)
+    nwh = np.zeros(2 * s.nmoltype, np.int32)
+    nmv = np.zeros(2 * s.nmoltype, np.int32)
+    nsr = np.zeros(s.nmol, np.int32)
+    omc = po.MCState(po.Pf(wr2), po.Pf(wt2), po.Pi(nwh), po.Pi(nmv), po.Pi(nsr), s.simt, F32(1), F32(1), F32(0.75), 0)
+    rng = po.Ran1.seeded(-77)
+    target = np.zeros(s.nmol)
+    fadds, nacc, orej = C.c_float(0.0), C.c_int(), C.c_int()
+    po.lib("cr").ora_molfit_da(C.byref(ob.box), po.Pf(ob.xa), po.Pf(ob.ya), po.Pf(ob.za), C.byref(ob.centres),
+                               C.byref(ref["ctab"]), C.byref(ref["cells"]), C.byref(ref["ledger"]), C.byref(omc),
+                               C.byref(rng), C.byref(fadds), C.byref(nacc), None, C.c_ulonglong(0), C.byref(orej),
+                               target.ctypes.data_as(po.dp), None)
+    np.testing.assert_allclose(tg, target, rtol=1e-6, atol=1e-6)
+    np.testing.assert_array_equal(st, rng.as_array())
+
+
+def test_trained_network_gains_exact_evaluations(ctx):
+    """Train on plain-cycle data of the 3200-atom box, then compare accepted moves per
+    exact energy evaluation: plain molfit needs one evaluation per move; delayed
+    acceptance only for the moves stage 1 passes."""
+    from epsr_python_translation_amd import device as D
+    from epsr_python_translation_amd import surrogate as SG
+    name = "c0"
+    s = H.system(name)
+    spec = SG.DescriptorSpec()
+    ldx = spec.width_for(s.npairs)
+    box, tab, led, _, _ = _gpu_setup(ctx, name)
+    wr, wt = H.initial_steps(s)
+    mc = _mc(ctx, s, wr, wt, ireset=0)
+    st = D.ran1_state_array(-5)
+    for _ in range(5):
+        D.molfit(ctx, box, tab, led, mc, st)
+    mc.params.ireset = 1   # step sizes fixed from here on
+    X, T = SG.collect(ctx, box, tab, led, mc, st, spec, 200, ldx)
+    net = SG.Network(ctx, ldx, seed=1)
+    hist = net.train(X, T, ldx, epochs=3, batch=2000, lr=2e-3)
+    print(f"training loss {hist[0]:.4g} -> {hist[-1]:.4g}; target variance {float(T.var()):.4g}", flush=True)
+    acc_p = mov = 0
+    for _ in range(20):
+        D.molfit(ctx, box, tab, led, mc, st)
+        acc_p += mc.stats.accepted
+        mov += mc.stats.moves
+    dam = SG.DelayedAcceptance(net, spec, seed=77)
+    acc_d = ev_d = 0
+    for _ in range(20):
+        dam.sweep(ctx, box, tab, led, mc, st)
+        acc_d += mc.stats.accepted
+        ev_d += mc.stats.moves - mc.stats.stage1_rejected
+    plain = acc_p / mov
+    da = acc_d / max(1, ev_d)
+    print(f"accepted per exact evaluation: plain {plain:.3f}, delayed acceptance {da:.3f} "
+          f"(stage 1 passes {ev_d / mov:.3f} of moves, accepted fraction {acc_d / mov:.3f})", flush=True)
+    assert hist[-1] < 0.5 * hist[0]
+    assert da > 1.2 * plain
