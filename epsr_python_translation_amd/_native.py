@@ -145,7 +145,7 @@ SIGNATURES = {
     "epsr_mlp_nparams": (C.c_longlong, [C.c_int, C.c_int]),
     "epsr_mlp_sync": (C.c_int, [vp, C.POINTER(MLP)]),
     "epsr_mlp_forward": (C.c_int, [vp, C.POINTER(MLP), C.c_int, vp, C.c_int, vp]),
-    "epsr_mlp_grad": (C.c_int, [vp, C.POINTER(MLP), C.c_int, vp, C.c_int, vp, vp, fp]),
+    "epsr_mlp_grad": (C.c_int, [vp, C.POINTER(MLP), C.c_int, vp, C.c_int, vp, C.c_float, vp, fp]),
     "epsr_mlp_adam": (C.c_int, [vp, C.POINTER(MLP), vp, C.c_float, C.c_float, C.c_float, C.c_float]),
     "epsr_truncation": (C.c_int, [vp, C.c_int, vp, C.c_float, C.c_float, vp, vp]),
     "epsr_generate_rp": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.c_int]),

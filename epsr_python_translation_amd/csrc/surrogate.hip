@@ -349,14 +349,15 @@ __global__ void sum_parts(int n, int nz, const float *__restrict__ part, float *
 
 // output gradient of the pair loss: dy = +-2 e / npair, loss partials per block
 __global__ void __launch_bounds__(256)
-mlp_out_grad(int npair, const float *__restrict__ y, const float *__restrict__ target, float *__restrict__ dy,
-             double *__restrict__ lpart)
+mlp_out_grad(int npair, const float *__restrict__ y, const float *__restrict__ target, float tclip,
+             float *__restrict__ dy, double *__restrict__ lpart)
 {
     __shared__ double s[256];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double l = 0.0;
     if (i < npair) {
-        const float e = (y[2 * i + 1] - y[2 * i]) - target[i];
+        const float t = tclip > 0.0f ? fminf(fmaxf(target[i], -tclip), tclip) : target[i];
+        const float e = (y[2 * i + 1] - y[2 * i]) - t;
         const float g = 2.0f * e / (float)npair;
         dy[2 * i] = -g;
         dy[2 * i + 1] = g;
@@ -456,7 +457,7 @@ extern "C" int epsr_mlp_forward(epsr_ctx *ctx, const epsr_mlp_t *m, int nrows, c
 }
 
 extern "C" int epsr_mlp_grad(epsr_ctx *ctx, const epsr_mlp_t *m, int npair, const uint16_t *x, int ldx,
-                             const float *target, float *grad, float *loss)
+                             const float *target, float tclip, float *grad, float *loss)
 {
     if (!ctx) return EPSR_ERR_ARG;
     int rc = mlp_check(ctx, m);
@@ -486,7 +487,7 @@ extern "C" int epsr_mlp_grad(epsr_ctx *ctx, const epsr_mlp_t *m, int npair, cons
           *gb3 = gw3 + H;
     timer_begin(ctx, "mlp_train");
     mlp_forward_k<<<ceil_div(R, 64), 256, 0, st>>>(R, x, ldx, w, y, Z1, H1, Z2);
-    mlp_out_grad<<<lblocks, 256, 0, st>>>(npair, y, target, dy, lpart);
+    mlp_out_grad<<<lblocks, 256, 0, st>>>(npair, y, target, tclip, dy, lpart);
     sum_loss<<<1, 64, 0, st>>>(lblocks, npair, lpart, dloss);
     const size_t RH = (size_t)R * H;
     mlp_dz2<<<ceil_div((long)RH, 256), 256, 0, st>>>(R, dy, w.w3, Z2, dZ2);   // Z2 now holds H2

@@ -98,23 +98,26 @@ class Network:
                                                      D._ptr(y)))
         return y
 
-    def grad(self, x: torch.Tensor, target: torch.Tensor, ldx: int | None = None, out=None):
-        """(gradient, loss) of mean (f(x[2i+1]) - f(x[2i]) - target[i])^2"""
+    def grad(self, x: torch.Tensor, target: torch.Tensor, ldx: int | None = None, tclip: float = 0.0, out=None):
+        """(gradient, loss) of mean (f(x[2i+1]) - f(x[2i]) - t_i)^2, t_i the target clamped to
+        [-tclip, tclip] when tclip > 0"""
         ldx = int(ldx or x.shape[-1])
         npair = int(target.numel())
         g = self.grad_buf if out is None else out
         loss = C.c_float()
         self.ctx.check(self.ctx.lib.epsr_mlp_grad(self.ctx.h, C.byref(self.struct), npair, D._ptr(x), ldx,
-                                                  D._ptr(target), D._ptr(g), C.byref(loss)))
+                                                  D._ptr(target), F32(tclip), D._ptr(g), C.byref(loss)))
         return g, float(loss.value)
 
     def adam(self, grad, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
         self.ctx.check(self.ctx.lib.epsr_mlp_adam(self.ctx.h, C.byref(self.struct), D._ptr(grad), F32(lr),
                                                   F32(beta1), F32(beta2), F32(eps)))
 
-    def train(self, X, T, ldx, epochs=4, batch=4096, lr=1e-3, seed=0, log=None):
+    def train(self, X, T, ldx, epochs=4, batch=4096, lr=1e-3, tclip=0.0, seed=0, log=None):
         """Adam over contiguous batches of pairs (rows 2i, 2i+1 of X; T[i]), shuffled per
-        epoch; returns the mean loss of each epoch."""
+        epoch; returns the mean loss of each epoch.  Targets beyond +-tclip (moves into
+        overlaps, energies of 1e10 and more) are clamped: stage 1 only has to rank moves
+        whose acceptance is in doubt."""
         npair = int(T.numel())
         rng = np.random.default_rng(seed)
         starts = np.arange(0, npair - batch + 1, batch)
@@ -123,7 +126,7 @@ class Network:
             tot = 0.0
             for s0 in rng.permutation(starts):
                 s0 = int(s0)
-                g, loss = self.grad(X[2 * s0:2 * (s0 + batch)], T[s0:s0 + batch], ldx)
+                g, loss = self.grad(X[2 * s0:2 * (s0 + batch)], T[s0:s0 + batch], ldx, tclip=tclip)
                 self.adam(g, lr=lr)
                 tot += loss
             hist.append(tot / max(1, len(starts)))

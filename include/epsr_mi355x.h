@@ -240,11 +240,11 @@ long long epsr_mlp_nparams(int d, int h);
 int epsr_mlp_sync(epsr_ctx *ctx, const epsr_mlp_t *mlp);
 /* y dev [nrows] = f(x row), x dev bf16 [nrows*ldx]; BF16 MFMA, FP32 accumulation */
 int epsr_mlp_forward(epsr_ctx *ctx, const epsr_mlp_t *mlp, int nrows, const uint16_t *x, int ldx, float *y);
-/* loss = mean_i (f(x[2i+1]) - f(x[2i]) - target[i])^2 over npair pairs and its gradient
- * (dev [nparams], the layout of w) with respect to w through the bf16 forward pass; loss
- * host out (nullable) */
+/* loss = mean_i (f(x[2i+1]) - f(x[2i]) - t_i)^2 over npair pairs, t_i = target[i] clamped
+ * to [-tclip, tclip] when tclip > 0, and its gradient (dev [nparams], the layout of w) with
+ * respect to w through the bf16 forward pass; loss host out (nullable) */
 int epsr_mlp_grad(epsr_ctx *ctx, const epsr_mlp_t *mlp, int npair, const uint16_t *x, int ldx, const float *target,
-                  float *grad, float *loss);
+                  float tclip, float *grad, float *loss);
 /* one Adam step on w (m, v, step updated) and the bf16 copy refreshed */
 int epsr_mlp_adam(epsr_ctx *ctx, epsr_mlp_t *mlp, const float *grad, float lr, float beta1, float beta2, float eps);
 
