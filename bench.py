@@ -117,6 +117,8 @@ def main():
     ap.add_argument("--sweeps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-small-box", action="store_true")
+    ap.add_argument("--no-da", action="store_true", help="skip the surrogate / delayed-acceptance leg")
+    ap.add_argument("--da-samples", type=float, default=2e6, help="training pairs for the surrogate")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 256 MiB device-to-device (the known-size reference for PMC byte counters)")
     args = ap.parse_args()
@@ -249,6 +251,8 @@ def main():
     }
     if not args.no_small_box:
         out["dmso_water_3200"] = small_box_rate(D, PL, synth, ctx)
+    if not args.no_da:
+        out["surrogate_da"] = surrogate_leg(D, PL, synth, ctx, int(args.da_samples))
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(s)
     print(json.dumps(out), flush=True)
@@ -290,6 +294,117 @@ def small_box_rate(D, PL, synth, ctx, sweeps=20):
     return {"workload": "200 DMSO + 400 water, 3200 atoms, L=31.7 A", "sweeps": sweeps,
             "ms_per_sweep": 1000.0 * dt / sweeps, "mc_atom_moves_per_s": rate,
             "x_fortran_derived": rate / FORTRAN_ATOM_MOVES_PER_S}
+
+
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def surrogate_leg(D, PL, synth, ctx, nsamples, timed_sweeps=20, gr_sweeps=20):
+    """Learned Delta-E surrogate + delayed acceptance on 1600 DMSO + 3200 water (25,600
+    atoms): plain GPU MC records (descriptor, exact later-partner dE) pairs, the network
+    trains on them on the GPU, then accepted moves/s of plain and delayed-acceptance
+    cycles are timed with the step sizes frozen, and their intermolecular histograms are
+    compared over segments plain | DA | plain against the plain | plain spread."""
+    import torch
+
+    from epsr_python_translation_amd import surrogate as SG
+
+    s = synth.config3()
+    R = PL.Refinement(ctx, s, idum=-3303).build_tables()
+    R.cells_and_ledger()
+    tables = R.tables
+    for _ in range(10):
+        R.molfit(tables)
+    R.mc.params.ireset = 1   # step sizes fixed from here on
+    spec = SG.DescriptorSpec()
+    ldx = spec.width_for(s.npairs)
+    nsweep = max(1, int(np.ceil(nsamples / s.nmol)))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    X, T = SG.collect(ctx, R.box, tables, R.ledger, R.mc, R.ran1, spec, nsweep, ldx)
+    torch.cuda.synchronize()
+    t_collect = time.perf_counter() - t0
+    net = SG.Network(ctx, ldx, seed=1)
+    tclip = 25.0 * float(s.simt)
+    batch = 4096
+    epochs = 3
+    t0 = time.perf_counter()
+    hist = net.train(X, T, ldx, epochs=epochs, batch=batch, lr=2e-3, tclip=tclip)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    train_ms = ctx.kernel_ms("mlp_train")
+    tvar = float(T.clamp(-tclip, tclip).var())
+    del X, T
+
+    def run(da, n, timed=True):
+        acc = mov = rej = 0
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        fwd = []
+        for _ in range(n):
+            if da is None:
+                R.molfit(tables)
+            else:
+                da.sweep(ctx, R.box, tables, R.ledger, R.mc, R.ran1)
+                fwd.append(ctx.kernel_ms("mlp_forward"))
+            acc += R.mc.stats.accepted
+            mov += R.mc.stats.moves
+            rej += R.mc.stats.stage1_rejected
+        torch.cuda.synchronize()
+        return acc, mov, rej, time.perf_counter() - t, fwd
+
+    dam = SG.DelayedAcceptance(net, spec, seed=4242)
+    run(None, 2)
+    run(dam, 2)
+    ap_, mp, _, tp, _ = run(None, timed_sweeps)
+    ad, md, rd, td, fwd = run(dam, timed_sweeps)
+
+    def hist_segment(da):
+        acc = None
+        for _ in range(gr_sweeps):
+            if da is None:
+                R.molfit(tables)
+            else:
+                da.sweep(ctx, R.box, tables, R.ledger, R.mc, R.ran1)
+            R.atogen(ngrsamples=s.nmol)
+            w = R.wav.view(2, s.npairs, R.ldw)[1].double().clone()
+            acc = w if acc is None else acc + w
+        return acc.cpu().numpy()
+
+    hA = hist_segment(None)
+    hB = hist_segment(dam)
+    hC = hist_segment(None)
+    dist = lambda a, b: float(np.sum((a - b) ** 2) / np.sum(((a + b) / 2) ** 2))
+    plain_rate = ap_ / tp
+    da_rate = ad / td
+    flops_row = 2.0 * (ldx * 128 + 128 * 128 + 128)
+    fwd_ms = float(np.mean(fwd)) if fwd else None
+    fwd_tflops = (2 * s.nmol * flops_row / (fwd_ms * 1e-3) / 1e12) if fwd_ms else None
+    train_flops = 3.0 * 2 * batch * flops_row
+    return {
+        "workload": "1600 DMSO + 3200 water, 25,600 atoms, L=63.5 A; step sizes frozen after 10 adapting sweeps",
+        "descriptor": {"gaussians_per_type_pair": spec.ng, "rc_A": spec.rc, "width_A": spec.width,
+                       "channels": s.npairs, "d": ldx},
+        "network": "d -> 128 -> 128 -> 1, SiLU, BF16 weights/activations, FP32 accumulation",
+        "training": {"pairs": nsweep * s.nmol, "collect_s": t_collect, "epochs": epochs, "batch": batch,
+                     "train_s": t_train, "loss_first_epoch": hist[0], "loss_last_epoch": hist[-1],
+                     "clamped_target_variance": tvar, "target_clamp": tclip,
+                     "last_step_ms": train_ms,
+                     "last_step_tflops_vs_fp32_mfma": (train_flops / (train_ms * 1e-3) / 1e12) if train_ms > 0 else None},
+        "plain": {"sweeps": timed_sweeps, "ms_per_sweep": 1e3 * tp / timed_sweeps, "accepted_fraction": ap_ / mp,
+                  "accepted_moves_per_s": plain_rate},
+        "delayed_acceptance": {"sweeps": timed_sweeps, "ms_per_sweep": 1e3 * td / timed_sweeps,
+                               "accepted_fraction": ad / md, "stage1_pass_fraction": 1.0 - rd / md,
+                               "accepted_per_exact_evaluation": ad / max(1, md - rd),
+                               "accepted_moves_per_s": da_rate,
+                               "mlp_forward_ms": fwd_ms, "mlp_forward_tflops": fwd_tflops,
+                               "mlp_forward_frac_bf16_peak": (fwd_tflops / BF16_PEAK_TFLOPS) if fwd_tflops else None},
+        "da_over_plain_accepted_moves_per_s": da_rate / plain_rate,
+        "plain_accepted_per_exact_evaluation": ap_ / mp,
+        "histogram_distance": {"plain_vs_da": dist(hB, (hA + hC) / 2), "plain_vs_plain": dist(hA, hC),
+                               "definition": "sum (a-b)^2 / sum ((a+b)/2)^2 over the intermolecular pair histograms "
+                                             f"summed over {gr_sweeps} sweeps per segment"},
+    }
 
 
 if __name__ == "__main__":
