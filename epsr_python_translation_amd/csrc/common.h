@@ -64,11 +64,6 @@ struct epsr_ledger {
     int *d_rowstart = nullptr, *d_cnt = nullptr, *d_order = nullptr, *d_pos = nullptr;
     int *d_partner = nullptr, *d_entry = nullptr, *d_mirror = nullptr, *d_slotmol = nullptr;
     float *d_ergpair = nullptr;
-    // pushes of the MC resolver: an accepted move at chain position k updates the energy
-    // change of its later partners outside its block of 64 positions through the row
-    // prow[k] .. prow[k+1] - 1, npush(k) entries padded to whole pieces of 128
-    int maxpush = 0, npushtot = 0, npad = 0;
-    int *d_prow = nullptr;   // [npad + 1]
     size_t cap_slots = 0, cap_entries = 0;
     bool ledger_ready = false;
     std::vector<int> h_rowstart, h_cnt;

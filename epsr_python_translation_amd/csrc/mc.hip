@@ -13,9 +13,11 @@
 //   B  for every ledger slot (molecule m, partner n) the pair energies the chain can
 //      need: E(m_trial, n_current) and, when n moves earlier in the chain, also
 //      E(m_trial, n_trial) -- massively parallel, one lane per slot;
-//   C  a single workgroup walks the chain: for move k it selects, slot by slot, the
-//      variants implied by the earlier decisions, sums them in the reference's
-//      order (sequential float32) and applies the Metropolis test;
+//   C  the decisions: from rigorous bounds on the reference's float32 sums, every move
+//      whose outcome is already fixed by the decided part of its neighbourhood is
+//      decided at once, in passes over the undecided moves; a move whose earlier
+//      partners are all decided is decided exactly (its sequential float32 sum when the
+//      bounds leave it open), so the earliest undecided move always decides;
 //   D  the accepted trial positions and the final ledger values are written back.
 // Because every pair energy and every sum is the reference's float32 expression, the
 // result is the reference's (with a correctly rounded cosf/expf).
@@ -46,8 +48,7 @@ extern "C" int epsr_ledger_create(epsr_ctx *ctx, const epsr_box *box, epsr_ledge
     l->nmol = box->nmol;
     int rc;
     if ((rc = dev_alloc(ctx, &l->d_order, box->nmol)) || (rc = dev_alloc(ctx, &l->d_pos, box->nmol)) ||
-        (rc = dev_alloc(ctx, &l->d_rowstart, box->nmol + 1)) || (rc = dev_alloc(ctx, &l->d_cnt, box->nmol)) ||
-        (rc = dev_alloc(ctx, &l->d_prow, (box->nmol + 63) / 64 * 64 + 1))) {
+        (rc = dev_alloc(ctx, &l->d_rowstart, box->nmol + 1)) || (rc = dev_alloc(ctx, &l->d_cnt, box->nmol))) {
         epsr_ledger_destroy(l);
         return rc;
     }
@@ -61,7 +62,6 @@ extern "C" void epsr_ledger_destroy(epsr_ledger *l)
     hipFree(l->d_rowstart); hipFree(l->d_cnt); hipFree(l->d_order); hipFree(l->d_pos);
     hipFree(l->d_partner); hipFree(l->d_entry); hipFree(l->d_mirror); hipFree(l->d_slotmol);
     hipFree(l->d_ergpair);
-    hipFree(l->d_prow);
     delete l;
 }
 
@@ -462,42 +462,6 @@ __global__ void fmolec1_mirror(const int *__restrict__ sval, int nrec, int nself
     mirror[i] = r < nself ? -1 : sortedpos[nself + (((r - nself) >> 1) << 1) + (1 - ((r - nself) & 1))];
 }
 
-// The chain position of the later partner whose energy change an accepted move at chain
-// position k updates through slot i of its list, or -1.  The partner's own slot for the
-// pair (the mirror) counts unless it is that partner's slot 0, which en_change skips;
-// self entries have no mirror.
-__device__ __forceinline__ int push_target(int i, int k, const int *__restrict__ partner,
-                                           const int *__restrict__ mirror, const int *__restrict__ pos,
-                                           const int *__restrict__ rowstart)
-{
-    int mi = mirror[i];
-    if (mi < 0) return -1;
-    int m = partner[i];
-    int pm = pos[m];
-    return (pm > k && mi != rowstart[m]) ? pm : -1;
-}
-
-// a push target outside the block of 64 chain positions of k (in-block pushes travel
-// through the resolver's registers)
-__device__ __forceinline__ bool far_push(int t, int k) { return t >= 0 && (t >> 6) != (k >> 6); }
-
-__global__ void count_pushes(int nslots, const int *__restrict__ slotmol, const int *__restrict__ partner,
-                             const int *__restrict__ mirror, const int *__restrict__ pos,
-                             const int *__restrict__ rowstart, int *__restrict__ npush)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nslots) return;
-    int k = pos[slotmol[i]];
-    if (far_push(push_target(i, k, partner, mirror, pos, rowstart), k)) atomicAdd(&npush[k], 1);
-}
-
-// row lengths: npush rounded up to whole pieces of 128 entries (1 KB)
-__global__ void piece_counts(const int *__restrict__ a, int n, int npad, int *__restrict__ out)
-{
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < npad + 1) out[t] = t < n ? (a[t] + 127) & ~127 : 0;
-}
-
 __global__ void max_int(const int *__restrict__ a, int n, int *__restrict__ out)
 {
     int v = 0;
@@ -818,20 +782,6 @@ extern "C" int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coord
     fmolec1_rows<<<ceil_div(nrec, 256), 256, 0, st>>>(skey, sval, nrec, nself, pim, pjm, pE, eintra, l->d_rowstart,
                                                       l->d_partner, l->d_entry, l->d_slotmol, spos, l->d_ergpair);
     fmolec1_mirror<<<ceil_div(nrec, 256), 256, 0, st>>>(sval, nrec, nself, spos, l->d_mirror);
-    // push rows of the MC resolver: counts, even-aligned row offsets, longest row
-    const int npad = (nmol + 63) / 64 * 64;
-    int *dpush = nullptr;
-    HIP_CHECK(ctx, hipMallocAsync((void **)&dpush, sizeof(int) * (2 * (size_t)npad + 4), st));
-    guard.p.push_back(dpush);
-    HIP_CHECK(ctx, hipMemsetAsync(dpush, 0, sizeof(int) * (2 * (size_t)npad + 4), st));
-    count_pushes<<<ceil_div(nrec, 256), 256, 0, st>>>(nrec, l->d_slotmol, l->d_partner, l->d_mirror, l->d_pos,
-                                                      l->d_rowstart, dpush);
-    max_int<<<std::max(1, std::min(64, ceil_div(nmol, 256))), 256, 0, st>>>(dpush, nmol, dpush + 2 * npad + 2);
-    piece_counts<<<ceil_div(npad + 1, 256), 256, 0, st>>>(dpush, nmol, npad, dpush + npad + 1);
-    if ((rc = hipcub_scan_int(ctx, dpush + npad + 1, l->d_prow, npad + 1))) return rc;
-    int maxpush = 0, npushtot = 0;
-    HIP_CHECK(ctx, hipMemcpyAsync(&maxpush, dpush + 2 * npad + 2, sizeof(int), hipMemcpyDeviceToHost, st));
-    HIP_CHECK(ctx, hipMemcpyAsync(&npushtot, l->d_prow + npad, sizeof(int), hipMemcpyDeviceToHost, st));
     // FP64 totals
     double *dsum = nullptr;
     HIP_CHECK(ctx, hipMallocAsync((void **)&dsum, 2 * sizeof(double), st));
@@ -857,9 +807,6 @@ extern "C" int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coord
     // the neighbour cells, which is nnbrefmax, so the cap can never bind
     ARG_CHECK(ctx, maxcnt <= l->nnbrefmax, "epsr_fmolec1: a ledger list of %d exceeds nnbrefmax %d", maxcnt,
               l->nnbrefmax);
-    l->maxpush = maxpush;
-    l->npushtot = npushtot;
-    l->npad = npad;
     l->nself = nself;
     l->npairent = npair;
     l->nslots = nrec;
@@ -1275,16 +1222,15 @@ __device__ void mc_thresholds(double b, double bnd, double tempd, double nlr, fl
     fr = f_up(vr + bnd - b + slack);
 }
 
-// one wave per molecule: the deltas of its slots, the decision thresholds of its move
-// (thr, by chain position) and the deltas its earlier partners in the same block of 64
-// chain positions push to it: Mt[block][i][j] for partner at i, molecule at j (zero
-// elsewhere)
+// one wave per molecule: the deltas of its slots (delta, the change of its energy change
+// when that earlier partner is accepted) and the decision thresholds of its move (thr, by
+// chain position)
 __global__ void __launch_bounds__(64)
 mc_prepare(int nmol, const int *__restrict__ pos, const int *__restrict__ rowstart, const int *__restrict__ cnt,
            const int *__restrict__ partner, const int *__restrict__ mirror, const float *__restrict__ A,
            const float *__restrict__ Bv, const float *__restrict__ L, const float *__restrict__ stream,
            const int *__restrict__ moff, float temp, const float *__restrict__ shift, float *__restrict__ delta,
-           float2 *__restrict__ thr, float *__restrict__ Mt)
+           float2 *__restrict__ thr)
 {
     // Each rounding of the sequential sum errs by at most u |partial sum|.  With
     // P_s = |sum_{s'<=s} e0| + sum_{s'<=s, earlier partner} |delta| the partial sums obey
@@ -1323,8 +1269,6 @@ mc_prepare(int nmol, const int *__restrict__ pos, const int *__restrict__ rowsta
                 maxnew = fmaxf(maxnew, fabsf(bv));
                 ad = fabs((double)dl);
                 npb++;
-                const int pn = pos[n];
-                if ((pn >> 6) == (pm >> 6)) Mt[(size_t)(pm >> 6) * 4096 + (pn & 63) * 64 + (pm & 63)] = dl;
             }
             delta[i] = dl;
         }
@@ -1357,38 +1301,6 @@ mc_prepare(int nmol, const int *__restrict__ pos, const int *__restrict__ rowsta
         mc_thresholds(carry_e - (double)sh, bnd, (double)temp, nlr, fa, fr);
         thr[pm] = make_float2(fa, fr);
     }
-}
-
-// one wave per chain position k: its push row (later partner's chain position, delta of
-// the partner's slot) for the pairs an accepted move at k updates outside its block,
-// at prow[k]; chain positions nmol.. (grid up to the padded length) get thresholds that
-// reject, so the resolver can walk whole blocks
-__global__ void __launch_bounds__(64)
-mc_push_rows(int nmol, const int *__restrict__ order, const int *__restrict__ pos,
-             const int *__restrict__ rowstart, const int *__restrict__ cnt, const int *__restrict__ partner,
-             const int *__restrict__ mirror, const float *__restrict__ delta, const int *__restrict__ prow,
-             const float *__restrict__ shift, int2 *__restrict__ pent, float2 *__restrict__ thr)
-{
-    const int k = blockIdx.x;
-    const int lane = threadIdx.x;
-    if (k >= nmol) {
-        if (lane == 0) thr[k] = make_float2(-INFINITY, -INFINITY);
-        return;
-    }
-    const int n = order[k];
-    const int row = rowstart[n], c = cnt[n];
-    int2 *out = pent + prow[k];
-    int fill = 0;
-    for (int s0 = 0; s0 < c; s0 += 64) {
-        const int s = s0 + lane;
-        int t = -1;
-        if (s < c) t = push_target(row + s, k, partner, mirror, pos, rowstart);
-        const bool f = far_push(t, k) && !(shift && isnan(shift[t]));
-        const unsigned long long b = __ballot(f);
-        if (f) out[fill + __popcll(b & ((1ull << lane) - 1ull))] = make_int2(t, __float_as_int(delta[mirror[row + s]]));
-        fill += __popcll(b);
-    }
-    for (int e = fill + lane, end = prow[k + 1] - prow[k]; e < end; e += 64) out[e] = make_int2(-1, 0);
 }
 
 // exact fdiff of every accepted move (for fadd): one wave per chain position gathers
@@ -1471,203 +1383,168 @@ mc_fadd(int nmol, const unsigned *__restrict__ bits, const float *__restrict__ f
     }
 }
 
-// ------------------------------------------------------------------ block resolver
-// One wave64 walks the chain in blocks of 64 chain positions, lane j holding move kb+j:
-// its pushed sum acc (from LDS accde, which holds the pushes of earlier blocks), its two
-// thresholds and, in registers, the 64 deltas its earlier partners inside the block
-// push to it.  The 64 decisions of a block are then a chain of register operations:
-// decide move i from lane i's acc; if accepted, every lane adds its delta from i.
-// After the block, the rows of its accepted moves are copied into LDS (direct
-// global->LDS loads) and pushed into accde with LDS atomics.  LDS operations of one
-// wave complete in order, so the next block's reads see every push.
+// ------------------------------------------------------------------ parallel resolution
+// The decisions of a cycle, resolved by every CU at once.  A move's pushed sum is the sum
+// of delta over its earlier partners that were accepted; while some of them are
+// undecided it lies between sure + (negative undecided deltas) and sure + (positive
+// ones).  A move whose whole range falls on one side of its thresholds is decided without
+// waiting for those partners (every outcome gives the same decision); a move with no
+// undecided earlier partner is decided exactly as the chain would (thresholds, or the
+// reference's sequential float32 sum when they are ambiguous).  Passes repeat over the
+// undecided moves; the earliest undecided move in the chain always decides, so every
+// pass makes progress and the result is the chain's.  Sums are FP64: within 1e-16 of the
+// exact sum, far inside the float32 accumulation bound the thresholds allow for.
+// state[k]: 0 undecided, 1 accepted, 2 rejected.
 
-// the reference's decision for move k from the exact sequential float32 sum: the wave
-// gathers the selected variant of every slot, lane 0 sums them in order (out of line:
-// rare, and it keeps the unrolled block walk small)
-__device__ __noinline__ int mc_exact_decide(int k, const int *__restrict__ order, const int *__restrict__ pos,
-                                            const int *__restrict__ rowstart, const int *__restrict__ cnt,
-                                            const int *__restrict__ partner, const int *__restrict__ mirror,
-                                            const float *__restrict__ A, const float *__restrict__ Bv,
-                                            const float *__restrict__ L, const float *__restrict__ stream,
-                                            const int *__restrict__ moff, float temp,
-                                            const float *__restrict__ shift, const unsigned *bits, float *term)
+// one wave decides move k if it can; returns the decision (0: still undecided)
+__device__ int mc_pass_move(int k, const int *__restrict__ order, const int *__restrict__ pos,
+                            const int *__restrict__ rowstart, const int *__restrict__ cnt,
+                            const int *__restrict__ partner, const int *__restrict__ mirror,
+                            const float *__restrict__ A, const float *__restrict__ Bv, const float *__restrict__ L,
+                            const float *__restrict__ delta, const float2 *__restrict__ thr,
+                            const float *__restrict__ stream, const int *__restrict__ moff, float temp,
+                            const float *__restrict__ shift, volatile unsigned char *state, float2 *term,
+                            int *nexact)
 {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     const int m = order[k];
     const int row = rowstart[m], c = cnt[m];
+    double sure = 0.0, lo = 0.0, hi = 0.0;
+    int und = 0;
     for (int s = 1 + lane; s < c; s += 64) {
-        int i = row + s;
-        int n = partner[i];
-        int pn = pos[n];
-        bool acc = pn < k && ((bits[pn >> 5] >> (pn & 31)) & 1u);
-        int mi = mirror[i];
-        term[2 * s] = acc ? Bv[i] : A[i];
-        term[2 * s + 1] = (acc && mi != rowstart[n]) ? A[mi] : L[i];
+        const int i = row + s;
+        const int pn = pos[partner[i]];
+        if (pn < k) {
+            const int sp = state[pn];
+            const double d = (double)delta[i];
+            if (sp == 1) {
+                sure += d;
+            } else if (sp == 0) {
+                und = 1;
+                if (d < 0.0) lo += d;
+                else hi += d;
+            }
+        }
     }
-    __syncthreads();
+    for (int off = 32; off > 0; off >>= 1) {
+        sure += __shfl_xor(sure, off);
+        lo += __shfl_xor(lo, off);
+        hi += __shfl_xor(hi, off);
+        und |= __shfl_xor(und, off);
+    }
+    const float2 t = thr[k];
+    if (sure + hi <= (double)t.x) return 1;
+    if (sure + lo >= (double)t.y) return 2;
+    if (und) return 0;
+    // every earlier partner decided, thresholds ambiguous: the reference's sum
+    for (int s = 1 + lane; s < c; s += 64) {
+        const int i = row + s;
+        const int n = partner[i];
+        const int pn = pos[n];
+        const bool acc = pn < k && state[pn] == 1;
+        const int mi = mirror[i];
+        term[s] = make_float2(acc ? Bv[i] : A[i], (acc && mi != rowstart[n]) ? A[mi] : L[i]);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     int dd = 0;
     if (lane == 0) {
         float fdiff = 0.0f;
-        for (int s = 1; s < c; ++s) fdiff = fdiff + term[2 * s] - term[2 * s + 1];
-        // delayed acceptance tests dE - dEs (stage 2); plain molfit tests dE
-        float ediff = (shift ? fdiff - shift[k] : fdiff + 0.0f) * temp;
-        dd = stream[moff[k]] < etest_of(ediff) ? 1 : 0;
+        for (int s = 1; s < c; ++s) fdiff = fdiff + term[s].x - term[s].y;
+        const float ediff = (shift ? fdiff - shift[k] : fdiff + 0.0f) * temp;
+        dd = stream[moff[k]] < etest_of(ediff) ? 1 : 2;
+        atomicAdd(nexact, 1);
     }
-    dd = __shfl(dd, 0);
-    __syncthreads();
-    return dd;
+    __builtin_amdgcn_wave_barrier();
+    return __shfl(dd, 0);
 }
 
-struct BlockIn {
-    float M[64];   // delta pushed to this lane's move by in-block move i (0 if none)
-    float fa, fr;  // thresholds
-    int rp, rc;    // push row offset and length
-};
-
-__device__ __forceinline__ void mc_block_fetch(int b, const float2 *__restrict__ thr, const float *__restrict__ Mt,
-                                               const int *__restrict__ prow, BlockIn &x)
+// one pass over a work list (the whole chain when first): decided moves write state,
+// the others go to the next list
+__global__ void __launch_bounds__(256)
+mc_resolve_pass(int nmol, const int *__restrict__ order, const int *__restrict__ pos,
+                const int *__restrict__ rowstart, const int *__restrict__ cnt, const int *__restrict__ partner,
+                const int *__restrict__ mirror, const float *__restrict__ A, const float *__restrict__ Bv,
+                const float *__restrict__ L, const float *__restrict__ delta, const float2 *__restrict__ thr,
+                const float *__restrict__ stream, const int *__restrict__ moff, float temp,
+                const float *__restrict__ shift, int maxc, int first, const int *__restrict__ win,
+                const int *__restrict__ nin, int *__restrict__ wout, int *__restrict__ nout,
+                unsigned char *state, int *nexact)
 {
-    const int lane = threadIdx.x;
-    const int k = b * 64 + lane;
-    const float2 t = thr[k];
-    x.fa = t.x;
-    x.fr = t.y;
-    x.rp = prow[k];
-    x.rc = prow[k + 1] - x.rp;
-    const float *src = Mt + (size_t)b * 4096 + lane;
-#pragma unroll
-    for (int i = 0; i < 64; ++i) x.M[i] = src[i * 64];
-}
-
-__global__ void __launch_bounds__(64)
-mc_resolve_blocks(int nmol, int npad, const int *__restrict__ order, const int *__restrict__ pos,
-                  const int *__restrict__ rowstart, const int *__restrict__ cnt, const int *__restrict__ partner,
-                  const int *__restrict__ mirror, const float *__restrict__ A, const float *__restrict__ Bv,
-                  const float *__restrict__ L, const float2 *__restrict__ thr, const float *__restrict__ Mt,
-                  const int *__restrict__ prow, const int2 *__restrict__ pent, const float *__restrict__ stream,
-                  const int *__restrict__ moff, float temp, const float *__restrict__ shift, int cap,
-                  unsigned *__restrict__ acc_out, int *__restrict__ nexact_out)
-{
-    extern __shared__ int4 smq[];
-    int2 *stg = (int2 *)smq;                       // [cap] staged push rows
-    float *accde = (float *)(stg + cap);           // [npad]
-    unsigned *bits = (unsigned *)(accde + npad);   // [npad / 32]
-    float *term = (float *)(bits + npad / 32);     // [2 * longest list]
-    const int lane = threadIdx.x;
-    for (int i = lane; i < npad / 32; i += 64) bits[i] = 0u;
-    for (int i = lane; i < npad; i += 64) accde[i] = 0.0f;
-    const int nblk = npad / 64;
-    int nexact = 0, npushed = 0;   // exact sums formed; push-row entries staged
-    BlockIn x0, x1;
-    mc_block_fetch(0, thr, Mt, prow, x0);
-    __syncthreads();
-
-    // Stage (direct global->LDS copies) the push rows of the moves in cand, lowest first,
-    // while they fit in the cap entries of the staging area; lane i of soff receives the
-    // staging offset of move i.  Returns the staged moves.
-    auto stage = [&](const BlockIn &x, unsigned long long cand, int &soff) -> unsigned long long {
-        unsigned long long got = 0ull;
-        int so = 0;
-        while (cand) {
-            const int i = __ffsll((long long)cand) - 1;
-            const int c = __builtin_amdgcn_readlane(x.rc, i);
-            if (so + c > cap) break;
-            const int st = __builtin_amdgcn_readlane(x.rp, i);
-            npushed += c;
-            for (int p = 0; p < c; p += 128)
-                __builtin_amdgcn_global_load_lds(
-                    (const void __attribute__((address_space(1))) *)(pent + st + p + 2 * lane),
-                    (void __attribute__((address_space(3))) *)(stg + so + p), 16, 0, 0);
-            soff = lane == i ? so : soff;
-            so += c;
-            got |= 1ull << i;
-            cand &= cand - 1ull;
-        }
-        return got;
-    };
-    // push the staged rows of the moves in msk into accde: the pieces of the next move
-    // are read before the atomics of the current one are issued
-    auto push = [&](const BlockIn &x, unsigned long long msk, int soff) {
-        auto rd = [&](int i, int4 (&e)[4]) {
-            const int c = __builtin_amdgcn_readlane(x.rc, i), o = __builtin_amdgcn_readlane(soff, i);
-            const int4 *src = (const int4 *)(stg + o) + lane;
-#pragma unroll
-            for (int p = 0; p < 4; ++p) e[p] = p * 128 < c ? src[p * 64] : make_int4(-1, 0, -1, 0);
-        };
-        auto apply = [&](const int4 (&e)[4]) {
-#pragma unroll
-            for (int p = 0; p < 4; ++p) {
-                if (e[p].x >= 0) atomicAdd(&accde[e[p].x], __int_as_float(e[p].y));
-                if (e[p].z >= 0) atomicAdd(&accde[e[p].z], __int_as_float(e[p].w));
-            }
-        };
-        if (!msk) return;
-        int4 cur[4], nxt[4];
-        rd(__ffsll((long long)msk) - 1, cur);
-        msk &= msk - 1ull;
-        while (msk) {
-            rd(__ffsll((long long)msk) - 1, nxt);
-            msk &= msk - 1ull;
-            apply(cur);
-#pragma unroll
-            for (int p = 0; p < 4; ++p) cur[p] = nxt[p];
-        }
-        apply(cur);
-    };
-
-    auto walk = [&](int b, const BlockIn &x, BlockIn &xn) {
-        // this block's inputs arrived during the previous block; with the counters at
-        // zero the next block's loads overlap the walk
-        __builtin_amdgcn_s_waitcnt(0);
-        if (b + 1 < nblk) mc_block_fetch(b + 1, thr, Mt, prow, xn);
-        int soff = 0;
-        const int kb = b * 64;
-        float acc = accde[kb + lane];
-        unsigned long long am = 0ull;
-#pragma unroll
-        for (int i = 0; i < 64; ++i) {
-            const unsigned long long ya = __ballot(acc <= x.fa);
-            const unsigned long long yr = __ballot(acc >= x.fr);
-            int dec = (int)((ya >> i) & 1ull);
-            if (!dec && !((yr >> i) & 1ull)) {
-                if (lane == 0) {
-                    bits[kb >> 5] = (unsigned)am;
-                    bits[(kb >> 5) + 1] = (unsigned)(am >> 32);
-                }
-                dec = __builtin_amdgcn_readfirstlane(mc_exact_decide(kb + i, order, pos, rowstart, cnt, partner,
-                                                                     mirror, A, Bv, L, stream, moff, temp, shift,
-                                                                     bits, term));
-                nexact++;
-            }
-            if (dec) {
-                acc += x.M[i];
-                am |= 1ull << i;
-            }
-        }
+    extern __shared__ float2 pterm[];   // [4 waves][maxc]
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    float2 *term = pterm + (size_t)wv * maxc;
+    const int n = first ? nmol : *nin;
+    for (int w = blockIdx.x * 4 + wv; w < n; w += gridDim.x * 4) {
+        const int k = first ? w : win[w];
+        const int d = mc_pass_move(k, order, pos, rowstart, cnt, partner, mirror, A, Bv, L, delta, thr, stream,
+                                   moff, temp, shift, state, term, nexact);
         if (lane == 0) {
-            bits[kb >> 5] = (unsigned)am;
-            bits[(kb >> 5) + 1] = (unsigned)(am >> 32);
+            if (d) state[k] = (unsigned char)d;
+            else wout[atomicAdd(nout, 1)] = k;
         }
-        // the rows of the accepted moves: staged and pushed in batches that fit
-        unsigned long long rest = am;
-        while (rest) {
-            __builtin_amdgcn_s_waitcnt(0);   // staging reads done before it is refilled
-            const unsigned long long got = stage(x, rest, soff);
-            __builtin_amdgcn_s_waitcnt(0);
-            push(x, got, soff);
-            rest &= ~got;
-        }
-    };
-    for (int b = 0; b < nblk; b += 2) {
-        walk(b, x0, x1);
-        if (b + 1 < nblk) walk(b + 1, x1, x0);
+    }
+}
+
+// the remaining passes in one workgroup of 16 waves, with a barrier between passes,
+// until no move is undecided; lists ping-pong between wa and wb
+__global__ void __launch_bounds__(1024)
+mc_resolve_finish(const int *__restrict__ order, const int *__restrict__ pos, const int *__restrict__ rowstart,
+                  const int *__restrict__ cnt, const int *__restrict__ partner, const int *__restrict__ mirror,
+                  const float *__restrict__ A, const float *__restrict__ Bv, const float *__restrict__ L,
+                  const float *__restrict__ delta, const float2 *__restrict__ thr, const float *__restrict__ stream,
+                  const int *__restrict__ moff, float temp, const float *__restrict__ shift, int maxc,
+                  int *__restrict__ wa, const int *__restrict__ na, int *__restrict__ wb, unsigned char *state,
+                  int *nexact, int *npass)
+{
+    extern __shared__ float2 fterm2[];   // [16 waves][maxc]
+    __shared__ int ncur, nnext;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    float2 *term = fterm2 + (size_t)wv * maxc;
+    if (threadIdx.x == 0) {
+        ncur = *na;
+        nnext = 0;
     }
     __syncthreads();
-    for (int i = lane; i < (nmol + 31) >> 5; i += 64) acc_out[i] = bits[i];
-    if (lane == 0) {
-        nexact_out[0] = nexact;
-        nexact_out[1] = npushed;
+    int *cur = wa, *nxt = wb;
+    int passes = 0;
+    while (ncur > 0) {
+        for (int w = wv; w < ncur; w += 16) {
+            const int k = cur[w];
+            const int d = mc_pass_move(k, order, pos, rowstart, cnt, partner, mirror, A, Bv, L, delta, thr,
+                                       stream, moff, temp, shift, state, term, nexact);
+            if (lane == 0) {
+                if (d) state[k] = (unsigned char)d;
+                else nxt[atomicAdd(&nnext, 1)] = k;
+            }
+        }
+        __threadfence_block();
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            ncur = nnext;
+            nnext = 0;
+        }
+        passes++;
+        int *t = cur;
+        cur = nxt;
+        nxt = t;
+        __syncthreads();
     }
+    if (threadIdx.x == 0) *npass = passes;
+}
+
+// acceptance bits (by chain position) from the decisions
+__global__ void mc_state_bits(int nmol, const unsigned char *__restrict__ state, unsigned *__restrict__ bits)
+{
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= (nmol + 31) / 32) return;
+    unsigned b = 0u;
+    for (int j = 0; j < 32; ++j) {
+        const int k = w * 32 + j;
+        if (k < nmol && state[k] == 1) b |= 1u << j;
+    }
+    bits[w] = b;
 }
 
 // Delayed acceptance, stage 1, by chain position k: dEs from the network output of
@@ -1796,23 +1673,13 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     std::vector<float> hstream(total);
     for (int i = 0; i < total; ++i) hstream[i] = ran1_next(rng);
 
-    // device buffers.  The block resolver's LDS: staged push rows (cap entries, a multiple
-    // of 128 that holds the longest row), accde and acceptance bits over the padded chain,
-    // and the terms of an exact sum.
+    // device buffers.  The parallel resolution keeps the terms of an exact sum (the
+    // longest ledger list) in LDS for each of its waves.
     const int nslots = l->nslots;
     int maxc = 1;
     for (int v : l->h_cnt) maxc = std::max(maxc, v);
-    const int npad = l->npad;
-    const size_t lds_cap = 160 * 1024;
-    const size_t lds_rest = sizeof(float) * (size_t)npad + sizeof(unsigned) * (npad / 32) +
-                            sizeof(float) * 2 * (size_t)maxc + 64;
-    const int cap_min = (l->maxpush + 127) / 128 * 128 + 128;
-    int cap = 0;
-    if (lds_rest < lds_cap) cap = (int)std::min<size_t>(8192, (lds_cap - lds_rest) / sizeof(int2) / 128 * 128);
-    // the block resolver reads at most 4 pieces (512 entries) of a push row
-    const bool fast = l->maxpush <= 512 && cap >= cap_min && sizeof(float2) * 4 * (size_t)maxc <= lds_cap;
-    const size_t lds_fast = sizeof(int2) * (size_t)cap + lds_rest;
-    const size_t nent = fast ? (size_t)l->npushtot + 256 : 0;
+    const size_t lds_pass = sizeof(float2) * 4 * (size_t)maxc;
+    const bool fast = lds_pass <= 160 * 1024;
     Scratch S, q;
     auto layout = [&](Scratch &x) {
         x.take<float4>(natom); x.take<float4>(natom);
@@ -1825,9 +1692,10 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
         x.take<float>(nslots); x.take<float>(nslots); x.take<float>(nslots);
         x.take<unsigned>((nmol + 31) / 32); x.take<float>(nmol); x.take<float>(4); x.take<int>(2 * nmt + 2);
         x.take<float>(nslots); x.take<int>(4);
-        x.take<float2>(fast ? npad : 0); x.take<float>(fast ? (size_t)npad * 64 : 0); x.take<int2>(nent);
+        x.take<float2>(fast ? nmol : 0);
         x.take<float>(da ? 2 * (size_t)nmol : 0); x.take<float>(da ? nmol : 0); x.take<int>(4);
         x.take<uint16_t>(want_x && !(rec && rec->x) ? 2 * (size_t)nmol * ldx : 0);
+        x.take<unsigned char>(nmol); x.take<int>(nmol); x.take<int>(nmol); x.take<int>(8);
     };
     layout(q);
     S.base = (char *)ctx_scratch(ctx, q.off);
@@ -1847,14 +1715,15 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     int *drej = S.take<int>(2 * nmt + 2);
     float *ddelta = S.take<float>(nslots);
     int *dnex = S.take<int>(4);
-    float2 *dthr = S.take<float2>(fast ? npad : 0);
-    float *dMt = S.take<float>(fast ? (size_t)npad * 64 : 0);
-    int2 *dpent = S.take<int2>(nent);
+    float2 *dthr = S.take<float2>(fast ? nmol : 0);
     float *dy = S.take<float>(da ? 2 * (size_t)nmol : 0);
     float *dshift = da ? S.take<float>(nmol) : (S.take<float>(0), nullptr);
     int *dnrej = S.take<int>(4);
     uint16_t *dX = S.take<uint16_t>(want_x && !(rec && rec->x) ? 2 * (size_t)nmol * ldx : 0);
-    ARG_CHECK(ctx, !da || fast, "epsr_molfit_da: delayed acceptance needs the block resolver (LDS too small)");
+    unsigned char *dstate = S.take<unsigned char>(nmol);
+    int *dW0 = S.take<int>(nmol), *dW1 = S.take<int>(nmol);
+    int *dpc = S.take<int>(8);   // list counts of passes 0..2, exact sums, passes of the finisher
+    ARG_CHECK(ctx, !da || fast, "epsr_molfit_da: delayed acceptance needs the parallel resolution (LDS too small)");
 
     PreparedTables pt;
     int rc = prepare_tables(ctx, box, tb, dT, dtmp, pt);
@@ -1977,26 +1846,42 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
         mc_record_target<<<nmol, 64, 0, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner, dA,
                                               dL, dshift, rec->target);
     int nexact = nmol, npushed = 0;
+    int npasses = 0;
     if (fast) {
-        HIP_CHECK(ctx, hipMemsetAsync(dMt, 0, sizeof(float) * (size_t)npad * 64, st));
         mc_prepare<<<nmol, 64, 0, st>>>(nmol, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner, l->d_mirror, dA, dB, dL,
-                                        dstream, dmoff, temp, dshift, ddelta, dthr, dMt);
-        mc_push_rows<<<npad, 64, 0, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner,
-                                          l->d_mirror, ddelta, l->d_prow, dshift, dpent, dthr);
+                                        dstream, dmoff, temp, dshift, ddelta, dthr);
+        HIP_CHECK(ctx, hipMemsetAsync(dstate, 0, (size_t)nmol, st));
+        HIP_CHECK(ctx, hipMemsetAsync(dpc, 0, sizeof(int) * 8, st));
+        const int nwf = std::max(1, std::min(16, (int)((150 * 1024) / (sizeof(float2) * (size_t)maxc))));
+        const int gpass = std::min(ceil_div(nmol, 4), 2048);
         timer_begin(ctx, "mc_resolve");
-        mc_resolve_blocks<<<1, 64, lds_fast, st>>>(nmol, npad, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt,
-                                                   l->d_partner, l->d_mirror, dA, dB, dL, dthr, dMt, l->d_prow, dpent,
-                                                   dstream, dmoff, temp, dshift, cap, dacc, dnex);
+        mc_resolve_pass<<<gpass, 256, lds_pass, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt,
+                                                      l->d_partner, l->d_mirror, dA, dB, dL, ddelta, dthr, dstream,
+                                                      dmoff, temp, dshift, maxc, 1, nullptr, nullptr, dW0, dpc + 0,
+                                                      dstate, dpc + 3);
+        mc_resolve_pass<<<gpass, 256, lds_pass, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt,
+                                                      l->d_partner, l->d_mirror, dA, dB, dL, ddelta, dthr, dstream,
+                                                      dmoff, temp, dshift, maxc, 0, dW0, dpc + 0, dW1, dpc + 1,
+                                                      dstate, dpc + 3);
+        mc_resolve_pass<<<gpass, 256, lds_pass, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt,
+                                                      l->d_partner, l->d_mirror, dA, dB, dL, ddelta, dthr, dstream,
+                                                      dmoff, temp, dshift, maxc, 0, dW1, dpc + 1, dW0, dpc + 2,
+                                                      dstate, dpc + 3);
+        mc_resolve_finish<<<1, 64 * nwf, sizeof(float2) * nwf * (size_t)maxc, st>>>(
+            l->d_order, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner, l->d_mirror, dA, dB, dL, ddelta, dthr,
+            dstream, dmoff, temp, dshift, maxc, dW0, dpc + 2, dW1, dstate, dpc + 3, dpc + 4);
+        mc_state_bits<<<ceil_div((nmol + 31) / 32, 256), 256, 0, st>>>(nmol, dstate, dacc);
         timer_end(ctx, "mc_resolve");
         mc_fdiff_accepted<<<ceil_div(nmol, 4), 256, sizeof(float2) * 4 * (size_t)maxc, st>>>(
             nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner, l->d_mirror, dA, dB, dL, dacc, maxc,
             dfdiff);
         mc_fadd<<<1, 1024, 0, st>>>(nmol, dacc, dfdiff, dres);
-        int hnex[2];
-        HIP_CHECK(ctx, hipMemcpyAsync(hnex, dnex, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+        int hpc[8];
+        HIP_CHECK(ctx, hipMemcpyAsync(hpc, dpc, 8 * sizeof(int), hipMemcpyDeviceToHost, st));
         HIP_CHECK(ctx, hipStreamSynchronize(st));
-        nexact = hnex[0];
-        npushed = hnex[1];
+        nexact = hpc[3];
+        npushed = 0;
+        npasses = 3 + hpc[4];
     } else {
         size_t lds = sizeof(unsigned) * ((nmol + 31) / 32) + sizeof(float) * 2 * (size_t)maxc + 64;
         ARG_CHECK(ctx, lds <= 160 * 1024, "epsr_molfit: %d molecules x %d ledger slots exceed LDS", nmol, maxc);
@@ -2050,9 +1935,10 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
         stats->accepted = (int)hres[1];
         stats->exact_resolutions = nexact;
         stats->pushed_entries = npushed;
-        stats->resolver_bytes = fast ? (double)npad * (sizeof(float2) + 64 * sizeof(float) + sizeof(int)) +
-                                           (double)npushed * sizeof(int2) + (double)npad / 8.0
-                                     : 0.0;
+        stats->resolver_passes = npasses;
+        // the first pass reads every slot once (partner, its chain position, delta, its
+        // state) and each move's thresholds, and writes the decisions
+        stats->resolver_bytes = fast ? (double)nslots * 13.0 + (double)nmol * (sizeof(float2) + 1) : 0.0;
         stats->retries = retries;
         stats->ms_pairs = ctx->timing ? epsr_ctx_last_kernel_ms(ctx, "mc_pair") : -1.0;
         stats->ms_resolve = ctx->timing ? epsr_ctx_last_kernel_ms(ctx, "mc_resolve") : -1.0;

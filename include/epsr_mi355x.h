@@ -190,6 +190,7 @@ typedef struct {
     long long pushed_entries;
     double resolver_bytes;
     int stage1_rejected;   /* delayed acceptance: moves the surrogate stage rejected */
+    int resolver_passes;   /* passes of the parallel resolution */
 } epsr_mc_stats_t;
 
 int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *coords, const epsr_tables_t *tables,
