@@ -65,6 +65,20 @@ struct epsr_ledger {
     int *d_partner = nullptr, *d_entry = nullptr, *d_mirror = nullptr, *d_slotmol = nullptr;
     float *d_ergpair = nullptr;
     size_t cap_slots = 0, cap_entries = 0;
+    // the MC plan of the chain order: ran1 offsets of every move's rotation, translation
+    // and Metropolis values (for the stepr/stepm flags in plan_flags), attempts per type
+    bool plan_ready = false;
+    int plan_flags = 0, plan_total = 0;
+    std::vector<int> h_roff, h_toff, h_moff, att, type_nat;
+    int *d_roff = nullptr, *d_toff = nullptr, *d_moff = nullptr;
+    // pinned ran1 streams: the one a cycle uploads, and the next cycle's, drawn on the host
+    // while the GPU runs the cycle (valid when the next call starts from ahead_from)
+    float *pin[2] = {nullptr, nullptr};
+    size_t pin_cap = 0;
+    int pin_ahead = 0;
+    bool ahead_valid = false;
+    int ahead_total = 0;
+    int32_t ahead_from[35], ahead_to[35];
     bool ledger_ready = false;
     std::vector<int> h_rowstart, h_cnt;
 };
@@ -124,6 +138,7 @@ struct Ran1 {
 };
 // src/epsr26/library/auxiliary_routines.py:324-385 (Park-Miller + Bays-Durham shuffle)
 float ran1_next(Ran1 &s);
+void ran1_fill(Ran1 &s, float *out, long long n);   // n values of ran1_next, state in registers
 void ran1_load(Ran1 &s, const int32_t *a);
 void ran1_store(const Ran1 &s, int32_t *a);
 

@@ -431,7 +431,7 @@ extern "C" int epsr_fit_expansion(epsr_ctx *ctx, int nchain, const float *yin, i
                 if (have[c] - hch[c].cursor >= per_try || have[c] >= cap - 8) continue;
                 long long n = std::min(chunk, cap - 8 - have[c]);
                 snap[c].push_back({have[c], gen[c]});
-                for (long long i = 0; i < n; ++i) buf[(size_t)i] = ran1_next(gen[c]);
+                ran1_fill(gen[c], buf.data(), n);
                 HIP_CHECK(ctx, hipMemcpyAsync(dstream + (size_t)c * cap + have[c], buf.data(), sizeof(float) * n,
                                               hipMemcpyHostToDevice, st));
                 HIP_CHECK(ctx, hipStreamSynchronize(st));   // buf is reused

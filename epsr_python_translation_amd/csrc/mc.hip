@@ -62,6 +62,9 @@ extern "C" void epsr_ledger_destroy(epsr_ledger *l)
     hipFree(l->d_rowstart); hipFree(l->d_cnt); hipFree(l->d_order); hipFree(l->d_pos);
     hipFree(l->d_partner); hipFree(l->d_entry); hipFree(l->d_mirror); hipFree(l->d_slotmol);
     hipFree(l->d_ergpair);
+    hipFree(l->d_roff); hipFree(l->d_toff); hipFree(l->d_moff);
+    if (l->pin[0]) hipHostFree(l->pin[0]);
+    if (l->pin[1]) hipHostFree(l->pin[1]);
     delete l;
 }
 
@@ -353,28 +356,27 @@ __global__ void fmolec1_pairs(const long long *__restrict__ rowoff, int nmol, co
 {
     extern __shared__ char smem[];
     PairTab t = LDS ? tables_to_lds(gt, npairs, smem) : gt;
-    const long long total = rowoff[nmol];
-    for (long long item = (long long)blockIdx.x * blockDim.x + threadIdx.x; item < total;
-         item += (long long)gridDim.x * blockDim.x) {
-        // binary search of the chain position
-        int lo = 0, hi = nmol - 1;
-        while (lo < hi) {
-            int mid = (lo + hi + 1) >> 1;
-            if (rowoff[mid] <= item) lo = mid;
-            else hi = mid - 1;
+    // one wave per chain position (row k of the items), lanes over its neighbour positions
+    const int lane = threadIdx.x & 63;
+    const int nw = gridDim.x * (blockDim.x >> 6);
+    for (int k = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); k < nmol; k += nw) {
+        const long long r0 = rowoff[k];
+        const int len = (int)(rowoff[k + 1] - r0);
+        const int im = order[k];
+        const int *nb = nbflat + cstart[mcell[im]];
+        const float4 *P1 = P + naf[im];
+        const int n1 = nal[im] - naf[im] + 1;
+        for (int inn = lane; inn < len; inn += 64) {
+            const int jm = nb[inn];
+            float e = 0.0f;
+            int f = 0;
+            if (inn >= 1 && jm > im) {
+                e = mol_pair(P1, n1, P + naf[jm], nal[jm] - naf[jm] + 1, t);
+                f = e != 0.0f;
+            }
+            E[r0 + inn] = e;
+            flag[r0 + inn] = f;
         }
-        int k = lo;
-        int inn = (int)(item - rowoff[k]);
-        int im = order[k];
-        int jm = nbflat[cstart[mcell[im]] + inn];
-        float e = 0.0f;
-        int f = 0;
-        if (inn >= 1 && jm > im) {
-            e = mol_pair(P + naf[im], nal[im] - naf[im] + 1, P + naf[jm], nal[jm] - naf[jm] + 1, t);
-            f = e != 0.0f;
-        }
-        E[item] = e;
-        flag[item] = f;
     }
 }
 
@@ -384,22 +386,21 @@ __global__ void fmolec1_compact(const long long *__restrict__ rowoff, int nmol, 
                                 const int *__restrict__ flag, const int *__restrict__ idx, int *__restrict__ pim,
                                 int *__restrict__ pjm, float *__restrict__ pE)
 {
-    const long long total = rowoff[nmol];
-    for (long long item = (long long)blockIdx.x * blockDim.x + threadIdx.x; item < total;
-         item += (long long)gridDim.x * blockDim.x) {
-        if (!flag[item]) continue;
-        int lo = 0, hi = nmol - 1;
-        while (lo < hi) {
-            int mid = (lo + hi + 1) >> 1;
-            if (rowoff[mid] <= item) lo = mid;
-            else hi = mid - 1;
+    const int lane = threadIdx.x & 63;
+    const int nw = gridDim.x * (blockDim.x >> 6);
+    for (int k = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); k < nmol; k += nw) {
+        const long long r0 = rowoff[k];
+        const int len = (int)(rowoff[k + 1] - r0);
+        const int im = order[k];
+        const int *nb = nbflat + cstart[mcell[im]];
+        for (int inn = lane; inn < len; inn += 64) {
+            const long long item = r0 + inn;
+            if (!flag[item]) continue;
+            const int p = idx[item];
+            pim[p] = im;
+            pjm[p] = nb[inn];
+            pE[p] = E[item];
         }
-        int im = order[lo];
-        int inn = (int)(item - rowoff[lo]);
-        int p = idx[item];
-        pim[p] = im;
-        pjm[p] = nbflat[cstart[mcell[im]] + inn];
-        pE[p] = E[item];
     }
 }
 
@@ -491,9 +492,13 @@ __global__ void sum_f64(const float *__restrict__ a, int n, double *out)
 }
 
 // grid size for persistent loops
+// a grid that fills every CU: as many workgroups of `block` threads per CU as the LDS
+// (160 KB) and 32 waves allow, and no more than the items need
 static int persistent_blocks(size_t lds_bytes, long long items, int block)
 {
-    int per_cu = lds_bytes > 80 * 1024 ? 1 : (lds_bytes > 50 * 1024 ? 2 : 4);
+    int per_cu = 32 / std::max(1, block / 64);
+    if (lds_bytes > 0) per_cu = std::min<int>(per_cu, (int)((160 * 1024) / lds_bytes));
+    per_cu = std::max(1, per_cu);
     long long want = (items + block - 1) / block;
     return (int)std::max(1LL, std::min<long long>(want, 256LL * per_cu));
 }
@@ -705,7 +710,7 @@ extern "C" int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coord
     abs_positions<<<ceil_div(box->natom, 256), 256, 0, st>>>(c->xa, c->ya, c->za, c->xadd, c->yadd, c->zadd,
                                                              box->d_amol, box->d_itype, dP, box->natom);
     {
-        const int B = 256;
+        const int B = 512;
         timer_begin(ctx, "fmolec1_pair");
         if (pt.in_lds) {
             int g = persistent_blocks(pt.lds_bytes, total, B);
@@ -808,6 +813,8 @@ extern "C" int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coord
     ARG_CHECK(ctx, maxcnt <= l->nnbrefmax, "epsr_fmolec1: a ledger list of %d exceeds nnbrefmax %d", maxcnt,
               l->nnbrefmax);
     l->nself = nself;
+    l->plan_ready = false;
+    l->ahead_valid = false;
     l->npairent = npair;
     l->nslots = nrec;
     l->nentries = nent;
@@ -958,9 +965,9 @@ __global__ void mc_trial(int nmol, const int *__restrict__ order, const int *__r
                          const float *__restrict__ za, const float *__restrict__ xadd, const float *__restrict__ yadd,
                          const float *__restrict__ zadd, const float *__restrict__ stream, const int *__restrict__ roff,
                          const int *__restrict__ toff, const float *__restrict__ rstep, const float *__restrict__ tstep,
-                         float *__restrict__ txa, float *__restrict__ tya, float *__restrict__ tza,
-                         float *__restrict__ txadd, float *__restrict__ tyadd, float *__restrict__ tzadd,
-                         int *__restrict__ nochange)
+                         const int *__restrict__ mtype, const float *__restrict__ wtype, float *__restrict__ txa,
+                         float *__restrict__ tya, float *__restrict__ tza, float *__restrict__ txadd,
+                         float *__restrict__ tyadd, float *__restrict__ tzadd, int *__restrict__ nochange)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nmol) return;
@@ -973,7 +980,7 @@ __global__ void mc_trial(int nmol, const int *__restrict__ order, const int *__r
         tza[a] = za[a];
     }
     if (roff[k] >= 0 && a1 > a0) {
-        float st = rstep[k];
+        float st = rstep ? rstep[k] : wtype[2 * mtype[m]];
         for (int axis = 0; axis < 3; ++axis) {
             float phi = pi * st * (2.0f * stream[roff[k] + axis] - 1.0f);
             float cphi = cr_cos(phi);
@@ -996,7 +1003,7 @@ __global__ void mc_trial(int nmol, const int *__restrict__ order, const int *__r
     float x = xadd[m], y = yadd[m], z = zadd[m];
     float nx = x, ny = y, nz = z;
     if (toff[k] >= 0) {
-        float st = tstep[k];
+        float st = tstep ? tstep[k] : wtype[2 * mtype[m] + 1];
         nz = z + st * (2.0f * stream[toff[k]] - 1.0f);
         ny = y + st * (2.0f * stream[toff[k] + 1] - 1.0f);
         nx = x + st * (2.0f * stream[toff[k] + 2] - 1.0f);
@@ -1089,7 +1096,8 @@ mc_resolve(int nmol, const int *__restrict__ order, const int *__restrict__ pos,
     }
 }
 
-// D: write back positions, counters and the ledger
+// D: write back positions, counters and the ledger (nothing when redo is set: movem's
+// step doubling makes the host redo the cycle)
 __global__ void mc_commit_mols(int nmol, const int *__restrict__ order, const int *__restrict__ naf,
                                const int *__restrict__ nal, const int *__restrict__ mtype,
                                const unsigned *__restrict__ accb, const int *__restrict__ roff,
@@ -1099,8 +1107,9 @@ __global__ void mc_commit_mols(int nmol, const int *__restrict__ order, const in
                                const float *__restrict__ tzadd, float *__restrict__ xa, float *__restrict__ ya,
                                float *__restrict__ za, float *__restrict__ xadd, float *__restrict__ yadd,
                                float *__restrict__ zadd, int *__restrict__ nstepreject, int count_rot,
-                               int *__restrict__ rej)
+                               const int *__restrict__ redo, int *__restrict__ rej)
 {
+    if (*redo) return;
     const int kk = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = kk < nmol;
     const int k = valid ? kk : nmol - 1;
@@ -1140,10 +1149,10 @@ __global__ void mc_commit_ledger(int nslots, const int *__restrict__ slotmol, co
                                  const int *__restrict__ partner, const int *__restrict__ entry,
                                  const int *__restrict__ mirror, const int *__restrict__ pos,
                                  const unsigned *__restrict__ accb, const float *__restrict__ A,
-                                 const float *__restrict__ Bv, float *__restrict__ ergpair)
+                                 const float *__restrict__ Bv, const int *__restrict__ redo, float *__restrict__ ergpair)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nslots) return;
+    if (i >= nslots || *redo) return;
     int mi = mirror[i];
     if (mi < 0) return;
     int m = slotmol[i], n = partner[i];
@@ -1633,45 +1642,93 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     if (trstepmin < 1.0e-6f) trstepmin = 1.0e-6f;
     const float rotstepmax = 1.0f, rotstepmin = 1.0e-5f;
 
-    // host: per-move ran1 offsets and step sizes in chain order (molfit.py:367-403)
-    std::vector<int> roff(nmol), toff(nmol), moff(nmol);
-    std::vector<float> rstep(nmol), tstep(nmol);
-    std::vector<int> att(2 * nmt, 0);
+    // the plan of the chain (molfit.py:367-403): per move, the ran1 offsets of its
+    // rotation (-1 none, -3 a single atom with stepr > 0), translation and Metropolis
+    // values; it depends on the chain order and the stepr/stepm flags only
+    const int flags = (p->stepr > 0.0f ? 1 : 0) | (p->stepm > 0.0f ? 2 : 0);
+    if (!l->plan_ready || l->plan_flags != flags) {
+        l->h_roff.assign(nmol, -1);
+        l->h_toff.assign(nmol, -1);
+        l->h_moff.assign(nmol, 0);
+        l->att.assign(2 * nmt, 0);
+        l->type_nat.assign(nmt, 0);
+        int cur = 0;
+        for (int k = 0; k < nmol; ++k) {
+            const int m = l->order[k], mt = box->h_mtype[m];
+            const int nat = box->h_nal[m] - box->h_naf[m] + 1;
+            l->type_nat[mt] = nat;
+            if (p->stepr > 0.0f && nat > 1) {
+                l->att[2 * mt] += 1;
+                l->h_roff[k] = cur;
+                cur += 3;
+            } else if (p->stepr > 0.0f) {
+                l->h_roff[k] = -3;   // no rotation for a single atom; counted only if stepr > 0
+            }
+            if (p->stepm > 0.0f) {
+                l->att[2 * mt + 1] += 1;
+                l->h_toff[k] = cur;
+                cur += 3;
+            }
+            l->h_moff[k] = cur;
+            cur += 1;
+        }
+        l->plan_total = cur;
+        int rc0;
+        if ((rc0 = dev_alloc(ctx, &l->d_roff, nmol)) || (rc0 = dev_alloc(ctx, &l->d_toff, nmol)) ||
+            (rc0 = dev_alloc(ctx, &l->d_moff, nmol)))
+            return rc0;
+        HIP_CHECK(ctx, hipMemcpy(l->d_roff, l->h_roff.data(), sizeof(int) * nmol, hipMemcpyHostToDevice));
+        HIP_CHECK(ctx, hipMemcpy(l->d_toff, l->h_toff.data(), sizeof(int) * nmol, hipMemcpyHostToDevice));
+        HIP_CHECK(ctx, hipMemcpy(l->d_moff, l->h_moff.data(), sizeof(int) * nmol, hipMemcpyHostToDevice));
+        l->plan_flags = flags;
+        l->plan_ready = true;
+        l->ahead_valid = false;
+    }
+    // step sizes of every type for this cycle: the first move of a type fixes a zero
+    // step (molfit.py:375-392), and every move of the type then uses it
+    std::vector<int> att(l->att);
     std::vector<float> wr(wholerot, wholerot + nmt), wt(wholetrans, wholetrans + nmt);
-    int cursor = 0;
-    for (int k = 0; k < nmol; ++k) {
-        int m = l->order[k], mt = box->h_mtype[m];
-        int nat = box->h_nal[m] - box->h_naf[m] + 1;
-        roff[k] = -1;
-        toff[k] = -1;
-        if (p->stepr > 0.0f && nat > 1) {
+    std::vector<float> wtype(2 * nmt, 0.0f);
+    for (int mt = 0; mt < nmt; ++mt) {
+        if (l->type_nat[mt] == 0) continue;   // no molecule of this type
+        if (p->stepr > 0.0f && l->type_nat[mt] > 1) {
             if (wr[mt] <= 0.0f) wr[mt] = rotstepmin;
-            att[2 * mt] += 1;
-            roff[k] = cursor;
-            rstep[k] = wr[mt];
-            cursor += 3;
         } else {
             wr[mt] = 0.0f;
-            if (p->stepr > 0.0f) roff[k] = -3;   // no rotation for a single atom; counted only if stepr > 0
         }
         if (p->stepm > 0.0f) {
             if (wt[mt] <= 0.0f) wt[mt] = trstepmin;
-            att[2 * mt + 1] += 1;
-            toff[k] = cursor;
-            tstep[k] = wt[mt];
-            cursor += 3;
         } else {
             wt[mt] = 0.0f;
         }
-        moff[k] = cursor;
-        cursor += 1;
+        wtype[2 * mt] = wr[mt];
+        wtype[2 * mt + 1] = wt[mt];
     }
-    int total = cursor;
+    int total = l->plan_total;
+    // the ran1 stream of the cycle: drawn ahead by the previous call when it starts from
+    // this state, else drawn now; both in pinned memory
+    if (l->pin_cap < (size_t)total + 8) {
+        for (int i = 0; i < 2; ++i) {
+            if (l->pin[i]) HIP_CHECK(ctx, hipHostFree(l->pin[i]));
+            l->pin[i] = nullptr;
+        }
+        l->pin_cap = 0;
+        for (int i = 0; i < 2; ++i) HIP_CHECK(ctx, hipHostMalloc((void **)&l->pin[i], sizeof(float) * (total + 8)));
+        l->pin_cap = (size_t)total + 8;
+        l->ahead_valid = false;
+    }
     Ran1 rng;
     ran1_load(rng, ran1_state);
-    Ran1 rng0 = rng;
-    std::vector<float> hstream(total);
-    for (int i = 0; i < total; ++i) hstream[i] = ran1_next(rng);
+    const Ran1 rng0 = rng;
+    int use;
+    if (l->ahead_valid && l->ahead_total == total && std::memcmp(l->ahead_from, ran1_state, sizeof(l->ahead_from)) == 0) {
+        use = l->pin_ahead;
+        ran1_load(rng, l->ahead_to);
+    } else {
+        use = 1 - l->pin_ahead;
+        ran1_fill(rng, l->pin[use], total);
+    }
+    l->ahead_valid = false;
 
     // device buffers.  The parallel resolution keeps the terms of an exact sum (the
     // longest ledger list) in LDS for each of its waves.
@@ -1685,13 +1742,12 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
         x.take<float4>(natom); x.take<float4>(natom);
         x.take<float4>((size_t)box->npairs * std::max(2, std::min(tb->ld, tb->npt2 + 1)));
         x.take<int>(4);
-        x.take<float>(total + 8); x.take<int>(nmol); x.take<int>(nmol); x.take<int>(nmol);
-        x.take<float>(nmol); x.take<float>(nmol);
+        x.take<float>(total + 8); x.take<float>(2 * nmt);
         x.take<float>(natom); x.take<float>(natom); x.take<float>(natom);
         x.take<float>(nmol); x.take<float>(nmol); x.take<float>(nmol);
         x.take<float>(nslots); x.take<float>(nslots); x.take<float>(nslots);
         x.take<unsigned>((nmol + 31) / 32); x.take<float>(nmol); x.take<float>(4); x.take<int>(2 * nmt + 2);
-        x.take<float>(nslots); x.take<int>(4);
+        x.take<float>(nslots);
         x.take<float2>(fast ? nmol : 0);
         x.take<float>(da ? 2 * (size_t)nmol : 0); x.take<float>(da ? nmol : 0); x.take<int>(4);
         x.take<uint16_t>(want_x && !(rec && rec->x) ? 2 * (size_t)nmol * ldx : 0);
@@ -1702,10 +1758,9 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     if (!S.base) return EPSR_ERR_HIP;
     float4 *Pold = S.take<float4>(natom), *Pnew = S.take<float4>(natom);
     float4 *dT = S.take<float4>((size_t)box->npairs * std::max(2, std::min(tb->ld, tb->npt2 + 1)));
-    int *dtmp = S.take<int>(4);
+    int *dtmp = S.take<int>(4);   // [0] table scan, [1] no-change flag, [2], [3] flags of a redone cycle
     float *dstream = S.take<float>(total + 8);
-    int *droff = S.take<int>(nmol), *dtoff = S.take<int>(nmol), *dmoff = S.take<int>(nmol);
-    float *drstep = S.take<float>(nmol), *dtstep = S.take<float>(nmol);
+    float *dwtype = S.take<float>(2 * nmt);
     float *txa = S.take<float>(natom), *tya = S.take<float>(natom), *tza = S.take<float>(natom);
     float *txadd = S.take<float>(nmol), *tyadd = S.take<float>(nmol), *tzadd = S.take<float>(nmol);
     float *dA = S.take<float>(nslots), *dB = S.take<float>(nslots), *dL = S.take<float>(nslots);
@@ -1714,7 +1769,6 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     float *dres = S.take<float>(4);
     int *drej = S.take<int>(2 * nmt + 2);
     float *ddelta = S.take<float>(nslots);
-    int *dnex = S.take<int>(4);
     float2 *dthr = S.take<float2>(fast ? nmol : 0);
     float *dy = S.take<float>(da ? 2 * (size_t)nmol : 0);
     float *dshift = da ? S.take<float>(nmol) : (S.take<float>(0), nullptr);
@@ -1728,37 +1782,138 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     PreparedTables pt;
     int rc = prepare_tables(ctx, box, tb, dT, dtmp, pt);
     if (rc) return rc;
-    HIP_CHECK(ctx, hipMemcpyAsync(dstream, hstream.data(), sizeof(float) * total, hipMemcpyHostToDevice, st));
-    HIP_CHECK(ctx, hipMemcpyAsync(droff, roff.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, st));
-    HIP_CHECK(ctx, hipMemcpyAsync(dtoff, toff.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, st));
-    HIP_CHECK(ctx, hipMemcpyAsync(dmoff, moff.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, st));
-    HIP_CHECK(ctx, hipMemcpyAsync(drstep, rstep.data(), sizeof(float) * nmol, hipMemcpyHostToDevice, st));
-    HIP_CHECK(ctx, hipMemcpyAsync(dtstep, tstep.data(), sizeof(float) * nmol, hipMemcpyHostToDevice, st));
-    HIP_CHECK(ctx, hipMemsetAsync(dtmp + 1, 0, sizeof(int), st));
+    HIP_CHECK(ctx, hipMemcpyAsync(dstream, l->pin[use], sizeof(float) * total, hipMemcpyHostToDevice, st));
+    HIP_CHECK(ctx, hipMemcpyAsync(dwtype, wtype.data(), sizeof(float) * 2 * nmt, hipMemcpyHostToDevice, st));
+    HIP_CHECK(ctx, hipMemsetAsync(dtmp + 1, 0, 3 * sizeof(int), st));
     mc_trial<<<ceil_div(nmol, 128), 128, 0, st>>>(nmol, l->d_order, box->d_naf, box->d_nal, c->xa, c->ya, c->za, c->xadd,
-                                                  c->yadd, c->zadd, dstream, droff, dtoff, drstep, dtstep, txa, tya,
-                                                  tza, txadd, tyadd, tzadd, dtmp + 1);
-    int nochange = 0;
-    HIP_CHECK(ctx, hipMemcpyAsync(&nochange, dtmp + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+                                                  c->yadd, c->zadd, dstream, l->d_roff, l->d_toff, nullptr, nullptr,
+                                                  box->d_mtype, dwtype, txa, tya, tza, txadd, tyadd, tzadd, dtmp + 1);
+
+    float hres[2];
+    int hnrej = 0, hflag = 0;
+    int hpc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    std::vector<int> rej(2 * nmt);
+    int nexact = nmol, npasses = 0;
+    // B..D of the cycle from trial positions in txa..tzadd and the stream at stream; the
+    // commits are skipped when *redo is set
+    auto run = [&](const float *stream, const int *moff, const int *roff, const int *toff, const int *redo) -> int {
+        int rc1;
+        abs_positions<<<ceil_div(natom, 256), 256, 0, st>>>(c->xa, c->ya, c->za, c->xadd, c->yadd, c->zadd,
+                                                            box->d_amol, box->d_itype, Pold, natom);
+        abs_positions<<<ceil_div(natom, 256), 256, 0, st>>>(txa, tya, tza, txadd, tyadd, tzadd, box->d_amol,
+                                                            box->d_itype, Pnew, natom);
+        HIP_CHECK(ctx, hipMemsetAsync(dnrej, 0, sizeof(int), st));
+        if (want_x) {
+            uint16_t *X = (rec && rec->x) ? rec->x : dX;
+            if ((rc1 = surr_descriptors_launch(ctx, box, l, Pold, Pnew, dpar, ldx, X, nullptr))) return rc1;
+            if (use_net && (rc1 = surr_forward_launch(ctx, da->mlp, 2 * nmol, X, ldx, dy))) return rc1;
+        }
+        if (da)
+            da_stage1<<<ceil_div(nmol, 256), 256, 0, st>>>(nmol, l->d_order, dy, da->des, temp, da->seed, dshift,
+                                                          rec ? rec->des : nullptr, rec ? rec->pass : nullptr, dnrej);
+        timer_begin(ctx, "mc_pair");
+        if (pt.in_lds) {
+            int g = persistent_blocks(pt.lds_bytes, nslots, 512);
+            mc_pairs<true><<<g, 512, pt.lds_bytes, st>>>(nslots, l->d_slotmol, l->d_rowstart, l->d_partner,
+                                                         l->d_entry, l->d_pos, l->d_ergpair, Pold, Pnew, box->d_naf,
+                                                         box->d_nal, pt.tab, box->npairs, dshift, dA, dB, dL);
+        } else {
+            int g = persistent_blocks(0, nslots, 512);
+            mc_pairs<false><<<g, 512, 0, st>>>(nslots, l->d_slotmol, l->d_rowstart, l->d_partner, l->d_entry,
+                                               l->d_pos, l->d_ergpair, Pold, Pnew, box->d_naf, box->d_nal, pt.tab,
+                                               box->npairs, dshift, dA, dB, dL);
+        }
+        timer_end(ctx, "mc_pair");
+        if (rec && rec->target)
+            mc_record_target<<<nmol, 64, 0, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt,
+                                                  l->d_partner, dA, dL, dshift, rec->target);
+        if (fast) {
+            mc_prepare<<<nmol, 64, 0, st>>>(nmol, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner, l->d_mirror, dA,
+                                            dB, dL, stream, moff, temp, dshift, ddelta, dthr);
+            HIP_CHECK(ctx, hipMemsetAsync(dstate, 0, (size_t)nmol, st));
+            HIP_CHECK(ctx, hipMemsetAsync(dpc, 0, sizeof(int) * 8, st));
+            const int nwf = std::max(1, std::min(16, (int)((150 * 1024) / (sizeof(float2) * (size_t)maxc))));
+            const int gpass = std::min(ceil_div(nmol, 4), 2048);
+            timer_begin(ctx, "mc_resolve");
+            mc_resolve_pass<<<gpass, 256, lds_pass, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt,
+                                                          l->d_partner, l->d_mirror, dA, dB, dL, ddelta, dthr,
+                                                          stream, moff, temp, dshift, maxc, 1, nullptr, nullptr, dW0,
+                                                          dpc + 0, dstate, dpc + 3);
+            mc_resolve_pass<<<gpass, 256, lds_pass, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt,
+                                                          l->d_partner, l->d_mirror, dA, dB, dL, ddelta, dthr,
+                                                          stream, moff, temp, dshift, maxc, 0, dW0, dpc + 0, dW1,
+                                                          dpc + 1, dstate, dpc + 3);
+            mc_resolve_pass<<<gpass, 256, lds_pass, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt,
+                                                          l->d_partner, l->d_mirror, dA, dB, dL, ddelta, dthr,
+                                                          stream, moff, temp, dshift, maxc, 0, dW1, dpc + 1, dW0,
+                                                          dpc + 2, dstate, dpc + 3);
+            mc_resolve_finish<<<1, 64 * nwf, sizeof(float2) * nwf * (size_t)maxc, st>>>(
+                l->d_order, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner, l->d_mirror, dA, dB, dL, ddelta, dthr,
+                stream, moff, temp, dshift, maxc, dW0, dpc + 2, dW1, dstate, dpc + 3, dpc + 4);
+            mc_state_bits<<<ceil_div((nmol + 31) / 32, 256), 256, 0, st>>>(nmol, dstate, dacc);
+            timer_end(ctx, "mc_resolve");
+            mc_fdiff_accepted<<<ceil_div(nmol, 4), 256, sizeof(float2) * 4 * (size_t)maxc, st>>>(
+                nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner, l->d_mirror, dA, dB, dL, dacc,
+                maxc, dfdiff);
+            mc_fadd<<<1, 1024, 0, st>>>(nmol, dacc, dfdiff, dres);
+            HIP_CHECK(ctx, hipMemcpyAsync(hpc, dpc, 8 * sizeof(int), hipMemcpyDeviceToHost, st));
+        } else {
+            size_t lds = sizeof(unsigned) * ((nmol + 31) / 32) + sizeof(float) * 2 * (size_t)maxc + 64;
+            ARG_CHECK(ctx, lds <= 160 * 1024, "epsr_molfit: %d molecules x %d ledger slots exceed LDS", nmol, maxc);
+            timer_begin(ctx, "mc_resolve");
+            mc_resolve<<<1, 256, lds, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner,
+                                            l->d_mirror, dA, dB, dL, stream, moff, temp, maxc, dacc, dfdiff, dres);
+            timer_end(ctx, "mc_resolve");
+        }
+        HIP_CHECK(ctx, hipMemsetAsync(drej, 0, sizeof(int) * 2 * nmt, st));
+        mc_commit_ledger<<<ceil_div(nslots, 256), 256, 0, st>>>(nslots, l->d_slotmol, l->d_rowstart, l->d_partner,
+                                                                l->d_entry, l->d_mirror, l->d_pos, dacc, dA, dB, redo,
+                                                                l->d_ergpair);
+        mc_commit_mols<<<ceil_div(nmol, 128), 128, 0, st>>>(nmol, l->d_order, box->d_naf, box->d_nal, box->d_mtype,
+                                                            dacc, roff, toff, txa, tya, tza, txadd, tyadd, tzadd, c->xa,
+                                                            c->ya, c->za, c->xadd, c->yadd, c->zadd, nstepreject,
+                                                            p->stepr > 0.0f ? 1 : 0, redo, drej);
+        HIP_CHECK(ctx, hipMemcpyAsync(&hnrej, dnrej, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIP_CHECK(ctx, hipMemcpyAsync(hres, dres, sizeof(float) * 2, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(ctx, hipMemcpyAsync(rej.data(), drej, sizeof(int) * 2 * nmt, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(ctx, hipGetLastError());
+        return EPSR_OK;
+    };
+    if ((rc = run(dstream, l->d_moff, l->d_roff, l->d_toff, dtmp + 1))) return rc;
+    HIP_CHECK(ctx, hipMemcpyAsync(&hflag, dtmp + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    // while the GPU runs the cycle: the next cycle's stream, from this one's end state
+    const Ran1 rend = rng;
+    {
+        const int nxt = 1 - use;
+        Ran1 r2 = rend;
+        ran1_fill(r2, l->pin[nxt], total);
+        ran1_store(rend, l->ahead_from);
+        ran1_store(r2, l->ahead_to);
+        l->ahead_total = total;
+        l->pin_ahead = nxt;
+    }
     HIP_CHECK(ctx, hipStreamSynchronize(st));
+    l->ahead_valid = hflag == 0;
     int retries = 0;
-    if (nochange) {
+    if (hflag) {
         // movem doubles its step until the centre moves (molfit.py:747-758); the doubled
-        // step persists for the molecule type.  Resolve the chain on the host from the
-        // current centres, then redo the trial moves.
+        // step persists for the molecule type.  Nothing was committed: resolve the chain on
+        // the host from the current centres, then run the cycle again with those offsets.
         std::vector<float> cx(nmol), cy(nmol), cz(nmol);
         HIP_CHECK(ctx, hipMemcpy(cx.data(), c->xadd, sizeof(float) * nmol, hipMemcpyDeviceToHost));
         HIP_CHECK(ctx, hipMemcpy(cy.data(), c->yadd, sizeof(float) * nmol, hipMemcpyDeviceToHost));
         HIP_CHECK(ctx, hipMemcpy(cz.data(), c->zadd, sizeof(float) * nmol, hipMemcpyDeviceToHost));
         rng = rng0;
-        hstream.clear();
+        std::vector<float> hstream;
         auto draw = [&](int idx) {
             while ((int)hstream.size() <= idx) hstream.push_back(ran1_next(rng));
             return hstream[idx];
         };
+        std::vector<int> roff(nmol), toff(nmol), moff(nmol);
+        std::vector<float> rstep(nmol), tstep(nmol);
         wr.assign(wholerot, wholerot + nmt);
         wt.assign(wholetrans, wholetrans + nmt);
-        cursor = 0;
+        int cursor = 0;
         for (int k = 0; k < nmol; ++k) {
             int m = l->order[k], mt = box->h_mtype[m];
             int nat = box->h_nal[m] - box->h_naf[m] + 1;
@@ -1773,19 +1928,19 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
             }
             if (p->stepm > 0.0f) {
                 if (wt[mt] <= 0.0f) wt[mt] = trstepmin;
-                float s = wt[mt];
+                float sz = wt[mt];
                 while (true) {
-                    float nz = cz[m] + s * (2.0f * draw(cursor) - 1.0f);
-                    float ny = cy[m] + s * (2.0f * draw(cursor + 1) - 1.0f);
-                    float nx = cx[m] + s * (2.0f * draw(cursor + 2) - 1.0f);
+                    float nz = cz[m] + sz * (2.0f * draw(cursor) - 1.0f);
+                    float ny = cy[m] + sz * (2.0f * draw(cursor + 1) - 1.0f);
+                    float nx = cx[m] + sz * (2.0f * draw(cursor + 2) - 1.0f);
                     if (!(nx == cx[m] && ny == cy[m] && nz == cz[m])) break;
-                    s = s * 2.0f;
+                    sz = sz * 2.0f;
                     cursor += 3;
                     retries++;
                 }
                 toff[k] = cursor;
-                tstep[k] = s;
-                wt[mt] = s;
+                tstep[k] = sz;
+                wt[mt] = sz;
                 cursor += 3;
             } else {
                 wt[mt] = 0.0f;
@@ -1797,8 +1952,12 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
         }
         total = cursor;
         hstream.resize(total);
-        void *dstream2 = nullptr;
-        HIP_CHECK(ctx, hipMallocAsync(&dstream2, sizeof(float) * (total + 8), st));
+        StreamTemp T(ctx);
+        float *dstream2 = (float *)T.alloc(sizeof(float) * (total + 8));
+        int *droff = (int *)T.alloc(sizeof(int) * nmol), *dtoff = (int *)T.alloc(sizeof(int) * nmol);
+        int *dmoff = (int *)T.alloc(sizeof(int) * nmol);
+        float *drstep = (float *)T.alloc(sizeof(float) * nmol), *dtstep = (float *)T.alloc(sizeof(float) * nmol);
+        if (!dstream2 || !droff || !dtoff || !dmoff || !drstep || !dtstep) return EPSR_ERR_HIP;
         HIP_CHECK(ctx, hipMemcpyAsync(dstream2, hstream.data(), sizeof(float) * total, hipMemcpyHostToDevice, st));
         HIP_CHECK(ctx, hipMemcpyAsync(droff, roff.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, st));
         HIP_CHECK(ctx, hipMemcpyAsync(dtoff, toff.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, st));
@@ -1806,107 +1965,16 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
         HIP_CHECK(ctx, hipMemcpyAsync(drstep, rstep.data(), sizeof(float) * nmol, hipMemcpyHostToDevice, st));
         HIP_CHECK(ctx, hipMemcpyAsync(dtstep, tstep.data(), sizeof(float) * nmol, hipMemcpyHostToDevice, st));
         mc_trial<<<ceil_div(nmol, 128), 128, 0, st>>>(nmol, l->d_order, box->d_naf, box->d_nal, c->xa, c->ya, c->za,
-                                                      c->xadd, c->yadd, c->zadd, (const float *)dstream2, droff, dtoff,
-                                                      drstep, dtstep, txa, tya, tza, txadd, tyadd, tzadd, dtmp + 2);
+                                                      c->xadd, c->yadd, c->zadd, dstream2, droff, dtoff, drstep,
+                                                      dtstep, box->d_mtype, dwtype, txa, tya, tza, txadd, tyadd, tzadd,
+                                                      dtmp + 2);
+        if ((rc = run(dstream2, dmoff, droff, dtoff, dtmp + 3))) return rc;
         HIP_CHECK(ctx, hipStreamSynchronize(st));
-        dstream = (float *)dstream2;   // used by mc_resolve below; freed at the end
     }
-    // trial and current absolute positions
-    abs_positions<<<ceil_div(natom, 256), 256, 0, st>>>(c->xa, c->ya, c->za, c->xadd, c->yadd, c->zadd, box->d_amol,
-                                                        box->d_itype, Pold, natom);
-    abs_positions<<<ceil_div(natom, 256), 256, 0, st>>>(txa, tya, tza, txadd, tyadd, tzadd, box->d_amol, box->d_itype,
-                                                        Pnew, natom);
-    HIP_CHECK(ctx, hipMemsetAsync(dnrej, 0, sizeof(int), st));
-    if (want_x) {
-        uint16_t *X = (rec && rec->x) ? rec->x : dX;
-        rc = surr_descriptors_launch(ctx, box, l, Pold, Pnew, dpar, ldx, X, nullptr);
-        if (rc) return rc;
-        if (use_net) {
-            rc = surr_forward_launch(ctx, da->mlp, 2 * nmol, X, ldx, dy);
-            if (rc) return rc;
-        }
-    }
-    if (da)
-        da_stage1<<<ceil_div(nmol, 256), 256, 0, st>>>(nmol, l->d_order, dy, da->des, temp, da->seed, dshift,
-                                                      rec ? rec->des : nullptr, rec ? rec->pass : nullptr, dnrej);
-    timer_begin(ctx, "mc_pair");
-    if (pt.in_lds) {
-        int g = persistent_blocks(pt.lds_bytes, nslots, 256);
-        mc_pairs<true><<<g, 256, pt.lds_bytes, st>>>(nslots, l->d_slotmol, l->d_rowstart, l->d_partner, l->d_entry,
-                                                     l->d_pos, l->d_ergpair, Pold, Pnew, box->d_naf, box->d_nal,
-                                                     pt.tab, box->npairs, dshift, dA, dB, dL);
-    } else {
-        int g = persistent_blocks(0, nslots, 256);
-        mc_pairs<false><<<g, 256, 0, st>>>(nslots, l->d_slotmol, l->d_rowstart, l->d_partner, l->d_entry, l->d_pos,
-                                           l->d_ergpair, Pold, Pnew, box->d_naf, box->d_nal, pt.tab, box->npairs,
-                                           dshift, dA, dB, dL);
-    }
-    timer_end(ctx, "mc_pair");
-    if (rec && rec->target)
-        mc_record_target<<<nmol, 64, 0, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner, dA,
-                                              dL, dshift, rec->target);
-    int nexact = nmol, npushed = 0;
-    int npasses = 0;
     if (fast) {
-        mc_prepare<<<nmol, 64, 0, st>>>(nmol, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner, l->d_mirror, dA, dB, dL,
-                                        dstream, dmoff, temp, dshift, ddelta, dthr);
-        HIP_CHECK(ctx, hipMemsetAsync(dstate, 0, (size_t)nmol, st));
-        HIP_CHECK(ctx, hipMemsetAsync(dpc, 0, sizeof(int) * 8, st));
-        const int nwf = std::max(1, std::min(16, (int)((150 * 1024) / (sizeof(float2) * (size_t)maxc))));
-        const int gpass = std::min(ceil_div(nmol, 4), 2048);
-        timer_begin(ctx, "mc_resolve");
-        mc_resolve_pass<<<gpass, 256, lds_pass, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt,
-                                                      l->d_partner, l->d_mirror, dA, dB, dL, ddelta, dthr, dstream,
-                                                      dmoff, temp, dshift, maxc, 1, nullptr, nullptr, dW0, dpc + 0,
-                                                      dstate, dpc + 3);
-        mc_resolve_pass<<<gpass, 256, lds_pass, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt,
-                                                      l->d_partner, l->d_mirror, dA, dB, dL, ddelta, dthr, dstream,
-                                                      dmoff, temp, dshift, maxc, 0, dW0, dpc + 0, dW1, dpc + 1,
-                                                      dstate, dpc + 3);
-        mc_resolve_pass<<<gpass, 256, lds_pass, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt,
-                                                      l->d_partner, l->d_mirror, dA, dB, dL, ddelta, dthr, dstream,
-                                                      dmoff, temp, dshift, maxc, 0, dW1, dpc + 1, dW0, dpc + 2,
-                                                      dstate, dpc + 3);
-        mc_resolve_finish<<<1, 64 * nwf, sizeof(float2) * nwf * (size_t)maxc, st>>>(
-            l->d_order, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner, l->d_mirror, dA, dB, dL, ddelta, dthr,
-            dstream, dmoff, temp, dshift, maxc, dW0, dpc + 2, dW1, dstate, dpc + 3, dpc + 4);
-        mc_state_bits<<<ceil_div((nmol + 31) / 32, 256), 256, 0, st>>>(nmol, dstate, dacc);
-        timer_end(ctx, "mc_resolve");
-        mc_fdiff_accepted<<<ceil_div(nmol, 4), 256, sizeof(float2) * 4 * (size_t)maxc, st>>>(
-            nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner, l->d_mirror, dA, dB, dL, dacc, maxc,
-            dfdiff);
-        mc_fadd<<<1, 1024, 0, st>>>(nmol, dacc, dfdiff, dres);
-        int hpc[8];
-        HIP_CHECK(ctx, hipMemcpyAsync(hpc, dpc, 8 * sizeof(int), hipMemcpyDeviceToHost, st));
-        HIP_CHECK(ctx, hipStreamSynchronize(st));
         nexact = hpc[3];
-        npushed = 0;
         npasses = 3 + hpc[4];
-    } else {
-        size_t lds = sizeof(unsigned) * ((nmol + 31) / 32) + sizeof(float) * 2 * (size_t)maxc + 64;
-        ARG_CHECK(ctx, lds <= 160 * 1024, "epsr_molfit: %d molecules x %d ledger slots exceed LDS", nmol, maxc);
-        timer_begin(ctx, "mc_resolve");
-        mc_resolve<<<1, 256, lds, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner,
-                                        l->d_mirror, dA, dB, dL, dstream, dmoff, temp, maxc, dacc, dfdiff, dres);
-        timer_end(ctx, "mc_resolve");
     }
-    HIP_CHECK(ctx, hipMemsetAsync(drej, 0, sizeof(int) * 2 * nmt, st));
-    mc_commit_ledger<<<ceil_div(nslots, 256), 256, 0, st>>>(nslots, l->d_slotmol, l->d_rowstart, l->d_partner,
-                                                            l->d_entry, l->d_mirror, l->d_pos, dacc, dA, dB,
-                                                            l->d_ergpair);
-    mc_commit_mols<<<ceil_div(nmol, 128), 128, 0, st>>>(nmol, l->d_order, box->d_naf, box->d_nal, box->d_mtype, dacc,
-                                                        droff, dtoff, txa, tya, tza, txadd, tyadd, tzadd, c->xa, c->ya,
-                                                        c->za, c->xadd, c->yadd, c->zadd, nstepreject,
-                                                        p->stepr > 0.0f ? 1 : 0, drej);
-    float hres[2];
-    int hnrej = 0;
-    std::vector<int> rej(2 * nmt);
-    HIP_CHECK(ctx, hipMemcpyAsync(&hnrej, dnrej, sizeof(int), hipMemcpyDeviceToHost, st));
-    HIP_CHECK(ctx, hipMemcpyAsync(hres, dres, sizeof(float) * 2, hipMemcpyDeviceToHost, st));
-    HIP_CHECK(ctx, hipMemcpyAsync(rej.data(), drej, sizeof(int) * 2 * nmt, hipMemcpyDeviceToHost, st));
-    HIP_CHECK(ctx, hipGetLastError());
-    HIP_CHECK(ctx, hipStreamSynchronize(st));
-    if (nochange) HIP_CHECK(ctx, hipFree(dstream));
     // counters and the step-size adaptation (molfit.py:465-536)
     for (int mt = 0; mt < nmt; ++mt) {
         nwhole[2 * mt] = att[2 * mt];
@@ -1934,7 +2002,7 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
         stats->moves = nmol;
         stats->accepted = (int)hres[1];
         stats->exact_resolutions = nexact;
-        stats->pushed_entries = npushed;
+        stats->pushed_entries = 0;
         stats->resolver_passes = npasses;
         // the first pass reads every slot once (partner, its chain position, delta, its
         // state) and each move's thresholds, and writes the decisions

@@ -258,6 +258,42 @@ float ran1_next(Ran1 &s)
     return r;
 }
 
+// n values of the same stream with the state in registers: IA*idum mod (2^31 - 1) formed
+// with a 64-bit product folded at bit 31 (exact, as Schrage's factorisation is), and the
+// shuffle index iy/NDIV as iy >> 26 (NDIV = 2^26, iy >= 0)
+void ran1_fill(Ran1 &s, float *out, long long n)
+{
+    const float AM = 4.656612873077393e-10f, RNMX = 0.9999998807907104f;
+    long long i = 0;
+    while (i < n) {
+        // the (re)seeding call, taken whenever idum <= 0 or iy == 0 as ran1 does
+        if (s.idum <= 0 || s.iy == 0) {
+            out[i++] = ran1_next(s);
+            continue;
+        }
+        const long long first = i;
+        uint32_t idum = (uint32_t)s.idum;
+        int32_t iy = s.iy;
+        int32_t iv[32];
+        for (int j = 0; j < 32; ++j) iv[j] = s.iv[j];
+        for (; i < n && iy != 0 && idum != 0u; ++i) {
+            const uint64_t x = (uint64_t)idum * 16807u;
+            uint32_t r = (uint32_t)(x & 0x7fffffffu) + (uint32_t)(x >> 31);
+            if (r >= 0x7fffffffu) r -= 0x7fffffffu;
+            idum = r;
+            const int j = iy >> 26;
+            iy = iv[j];
+            iv[j] = (int32_t)idum;
+            const float f = AM * (float)iy;
+            out[i] = f > RNMX ? RNMX : f;
+        }
+        s.idum = (int32_t)idum;
+        s.iy = iy;
+        for (int j = 0; j < 32; ++j) s.iv[j] = iv[j];
+        s.mcall = (int32_t)((uint32_t)s.mcall + (uint32_t)(i - first));
+    }
+}
+
 void ran1_load(Ran1 &s, const int32_t *a)
 {
     s.mcall = a[0]; s.idum = a[1]; s.iy = a[2];
@@ -276,7 +312,7 @@ extern "C" int epsr_ran1_stream(int32_t *state, int n, float *out)
     if (!state || (n > 0 && !out)) return EPSR_ERR_ARG;
     Ran1 s;
     ran1_load(s, state);
-    for (int i = 0; i < n; ++i) out[i] = ran1_next(s);
+    ran1_fill(s, out, n);
     ran1_store(s, state);
     return EPSR_OK;
 }
