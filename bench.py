@@ -118,6 +118,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-small-box", action="store_true")
     ap.add_argument("--no-da", action="store_true", help="skip the surrogate / delayed-acceptance leg")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-box ensemble leg")
     ap.add_argument("--da-samples", type=float, default=2e6, help="training pairs for the surrogate")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 256 MiB device-to-device (the known-size reference for PMC byte counters)")
@@ -253,6 +254,8 @@ def main():
         out["dmso_water_3200"] = small_box_rate(D, PL, synth, ctx)
     if not args.no_da:
         out["surrogate_da"] = surrogate_leg(D, PL, synth, ctx, int(args.da_samples))
+    if not args.no_ensemble:
+        out["ensemble_64"] = ensemble_leg(synth)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(s)
     print(json.dumps(out), flush=True)
@@ -297,6 +300,48 @@ def small_box_rate(D, PL, synth, ctx, sweeps=20):
 
 
 BF16_PEAK_TFLOPS = 2500.0
+
+
+def ensemble_leg(synth, sweeps=5, steps=2, workers=8):
+    """64 independent DMSO/water boxes (300-6000 molecules, log-uniform; DMSO fraction
+    U(0, 0.5); 0.1 atoms/A^3): per step and box the ledger (FP64 energy totals), 5 MC
+    sweeps, atogen (1000 origins), gensq and stog.  Boxes run on `workers` streams with
+    autotuned launch shapes and the transform stage replayed from HIP graphs; the same
+    work on one stream is timed beside it."""
+    import torch
+
+    from epsr_python_translation_amd import ensemble as E
+
+    systems = synth.config2_boxes()
+    atoms = sum(s.natom for s in systems)
+    res = {"workload": "64 boxes, 300-6000 molecules (log-uniform), DMSO fraction U(0,0.5), 0.1 atoms/A^3",
+           "boxes": len(systems), "atoms": atoms, "sweeps_per_step": sweeps}
+    for nw in (workers, 1):
+        t0 = time.perf_counter()
+        ens = E.Ensemble(systems, nworkers=nw, idum0=-7000, tune=True)
+        setup = time.perf_counter() - t0
+        ens.step(sweeps=sweeps)   # warm-up; captures the transform graphs
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        acc = mov = 0
+        for _ in range(steps):
+            st = ens.step(sweeps=sweeps)
+            acc += st["accepted"]
+            mov += st["moves"]
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        res[f"workers_{nw}"] = {"setup_s": setup, "ms_per_step": 1e3 * dt,
+                                "atom_moves_per_s": sweeps * atoms / dt, "accepted_fraction": acc / max(1, mov)}
+        if nw == workers:
+            shapes = {}
+            for v in ens.shapes.values():
+                k = f"atogen_wg={v.get('atogen_workgroups')},pair_block={v.get('pair_block')}"
+                shapes[k] = shapes.get(k, 0) + 1
+            res["tuned_shapes"] = shapes
+            res["energy_totals_fp64_first_boxes"] = [float(x) for x in ens.totals[:4, 0]]
+        del ens
+    res["speedup_streams"] = res[f"workers_1"]["ms_per_step"] / res[f"workers_{workers}"]["ms_per_step"]
+    return res
 
 
 def surrogate_leg(D, PL, synth, ctx, nsamples, timed_sweeps=20, gr_sweeps=20):

@@ -95,6 +95,10 @@ class MCStats(C.Structure):
                 ("resolver_bytes", C.c_double), ("stage1_rejected", C.c_int), ("resolver_passes", C.c_int)]
 
 
+LAUNCH_ATOGEN_WORKGROUPS = 1
+LAUNCH_PAIR_BLOCK = 2
+
+
 class DescParams(C.Structure):
     _fields_ = [("ng", C.c_int), ("rc", C.c_float), ("r0", C.c_float), ("dmu", C.c_float), ("width", C.c_float),
                 ("xscale", C.c_float)]
@@ -124,6 +128,11 @@ SIGNATURES = {
     "epsr_ctx_enable_timing": (C.c_int, [vp, C.c_int]),
     "epsr_box_create": (C.c_int, [vp, C.POINTER(BoxDesc), C.POINTER(vp)]),
     "epsr_box_destroy": (None, [vp]),
+    "epsr_box_set_launch": (C.c_int, [vp, C.c_int, C.c_int]),
+    "epsr_graph_begin": (C.c_int, [vp]),
+    "epsr_graph_end": (C.c_int, [vp, C.POINTER(vp)]),
+    "epsr_graph_launch": (C.c_int, [vp, vp]),
+    "epsr_graph_destroy": (None, [vp]),
     "epsr_ran1_stream": (C.c_int, [i32p, C.c_int, fp]),
     "epsr_atogen": (C.c_int, [vp, vp, C.POINTER(Coords), C.c_int, C.c_int, vp, vp, vp, ip, i32p, vp, vp, vp, vp]),
     "epsr_gensq": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp, C.c_int, vp, vp, C.c_int, C.c_int]),

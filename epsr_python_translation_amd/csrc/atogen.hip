@@ -384,8 +384,8 @@ extern "C" int epsr_atogen(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
             return set_error(ctx, EPSR_ERR_CAPACITY,
                              "epsr_atogen: histogram of %d pairs x %d bins does not fit in LDS", npairs, npt1);
         int gx = ceil_div(norigin, ATOGEN_BLOCK);
-        // partner slices so that the launch has a few workgroups per CU
-        int gy = std::max(1, std::min(64, ceil_div(1024, gx)));
+        // partner slices so that the launch has the box's target of workgroups (a few per CU)
+        int gy = std::max(1, std::min(256, ceil_div(box->atogen_workgroups, gx)));
         gy = std::min(gy, std::max(1, natom / ATOGEN_TILE));
         HIP_CHECK(ctx, hipMemsetAsync(dMX, 0, sizeof(unsigned), st));
         abs_max_coord<<<64, 256, 0, st>>>(dP, natom, dMX);

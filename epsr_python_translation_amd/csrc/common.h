@@ -32,6 +32,8 @@ struct epsr_ctx {
     // growable pinned host staging buffer
     void *pinned = nullptr;
     size_t pinned_bytes = 0;
+    // between epsr_graph_begin and epsr_graph_end
+    bool capturing = false;
 };
 
 struct epsr_box {
@@ -46,6 +48,9 @@ struct epsr_box {
     // device copies
     int *d_itype = nullptr, *d_naf = nullptr, *d_nal = nullptr, *d_mtype = nullptr;
     int *d_npairref = nullptr, *d_amol = nullptr;
+    // launch shapes (epsr_box_set_launch): workgroups atogen's pair kernel aims for, and
+    // the workgroup size of the pair-energy kernels
+    int atogen_workgroups = 1024, pair_block = 512;
 };
 
 // ------------------------------------------------------------------ ledger
@@ -99,6 +104,12 @@ int set_error(epsr_ctx *ctx, int code, const char *fmt, ...);
     do {                                                                                 \
         if (!(cond)) return set_error((ctx), EPSR_ERR_ARG, __VA_ARGS__);                  \
     } while (0)
+
+// true while the context's stream is being captured into a graph: no host reads, no
+// synchronisation, no allocation may happen then
+bool ctx_capturing(epsr_ctx *ctx);
+// synchronise the context's stream unless it is being captured
+hipError_t ctx_sync(epsr_ctx *ctx);
 
 // scratch management (device); returns nullptr and sets the error on failure
 void *ctx_scratch(epsr_ctx *ctx, size_t bytes);

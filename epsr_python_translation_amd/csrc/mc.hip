@@ -710,7 +710,7 @@ extern "C" int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coord
     abs_positions<<<ceil_div(box->natom, 256), 256, 0, st>>>(c->xa, c->ya, c->za, c->xadd, c->yadd, c->zadd,
                                                              box->d_amol, box->d_itype, dP, box->natom);
     {
-        const int B = 512;
+        const int B = box->pair_block;
         timer_begin(ctx, "fmolec1_pair");
         if (pt.in_lds) {
             int g = persistent_blocks(pt.lds_bytes, total, B);
@@ -1813,13 +1813,13 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
                                                           rec ? rec->des : nullptr, rec ? rec->pass : nullptr, dnrej);
         timer_begin(ctx, "mc_pair");
         if (pt.in_lds) {
-            int g = persistent_blocks(pt.lds_bytes, nslots, 512);
-            mc_pairs<true><<<g, 512, pt.lds_bytes, st>>>(nslots, l->d_slotmol, l->d_rowstart, l->d_partner,
+            int g = persistent_blocks(pt.lds_bytes, nslots, box->pair_block);
+            mc_pairs<true><<<g, box->pair_block, pt.lds_bytes, st>>>(nslots, l->d_slotmol, l->d_rowstart, l->d_partner,
                                                          l->d_entry, l->d_pos, l->d_ergpair, Pold, Pnew, box->d_naf,
                                                          box->d_nal, pt.tab, box->npairs, dshift, dA, dB, dL);
         } else {
-            int g = persistent_blocks(0, nslots, 512);
-            mc_pairs<false><<<g, 512, 0, st>>>(nslots, l->d_slotmol, l->d_rowstart, l->d_partner, l->d_entry,
+            int g = persistent_blocks(0, nslots, box->pair_block);
+            mc_pairs<false><<<g, box->pair_block, 0, st>>>(nslots, l->d_slotmol, l->d_rowstart, l->d_partner, l->d_entry,
                                                l->d_pos, l->d_ergpair, Pold, Pnew, box->d_naf, box->d_nal, pt.tab,
                                                box->npairs, dshift, dA, dB, dL);
         }

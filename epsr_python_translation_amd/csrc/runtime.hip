@@ -69,9 +69,13 @@ extern "C" int epsr_ctx_enable_timing(epsr_ctx *ctx, int on)
     return EPSR_OK;
 }
 
+bool ctx_capturing(epsr_ctx *ctx) { return ctx->capturing; }
+
+hipError_t ctx_sync(epsr_ctx *ctx) { return ctx_capturing(ctx) ? hipSuccess : hipStreamSynchronize(ctx->stream); }
+
 void timer_begin(epsr_ctx *ctx, const char *family)
 {
-    if (!ctx->timing) return;
+    if (!ctx->timing || ctx_capturing(ctx)) return;
     KernelTimer &t = ctx->timers[family];
     if (!t.start) {
         hipEventCreate(&t.start);
@@ -82,7 +86,7 @@ void timer_begin(epsr_ctx *ctx, const char *family)
 
 void timer_end(epsr_ctx *ctx, const char *family)
 {
-    if (!ctx->timing) return;
+    if (!ctx->timing || ctx_capturing(ctx)) return;
     KernelTimer &t = ctx->timers[family];
     hipEventRecord(t.stop, ctx->stream);
     t.pending = true;
@@ -108,6 +112,11 @@ extern "C" double epsr_ctx_last_kernel_ms(const epsr_ctx *cctx, const char *fami
 void *ctx_scratch(epsr_ctx *ctx, size_t bytes)
 {
     if (bytes <= ctx->scratch_bytes) return ctx->scratch;
+    if (ctx_capturing(ctx)) {
+        set_error(ctx, EPSR_ERR_ARG, "scratch of %zu bytes needed while capturing: run the work once before capture",
+                  bytes);
+        return nullptr;
+    }
     if (ctx->scratch) {
         hipStreamSynchronize(ctx->stream);
         hipFree(ctx->scratch);
@@ -222,6 +231,78 @@ extern "C" void epsr_box_destroy(epsr_box *b)
     hipFree(b->d_itype); hipFree(b->d_naf); hipFree(b->d_nal); hipFree(b->d_mtype);
     hipFree(b->d_npairref); hipFree(b->d_amol);
     delete b;
+}
+
+extern "C" int epsr_box_set_launch(epsr_box *b, int knob, int value)
+{
+    if (!b) return EPSR_ERR_ARG;
+    epsr_ctx *ctx = b->ctx;
+    switch (knob) {
+    case EPSR_LAUNCH_ATOGEN_WORKGROUPS:
+        ARG_CHECK(ctx, value >= 1 && value <= (1 << 20), "epsr_box_set_launch: atogen workgroups %d", value);
+        b->atogen_workgroups = value;
+        return EPSR_OK;
+    case EPSR_LAUNCH_PAIR_BLOCK:
+        ARG_CHECK(ctx, value == 256 || value == 512 || value == 1024, "epsr_box_set_launch: pair block %d", value);
+        b->pair_block = value;
+        return EPSR_OK;
+    default:
+        return set_error(ctx, EPSR_ERR_ARG, "epsr_box_set_launch: unknown knob %d", knob);
+    }
+}
+
+// ------------------------------------------------------------------ graphs
+struct epsr_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" int epsr_graph_begin(epsr_ctx *ctx)
+{
+    if (!ctx) return EPSR_ERR_ARG;
+    HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    ARG_CHECK(ctx, ctx->stream != nullptr && ctx->stream != hipStreamPerThread,
+              "epsr_graph_begin: capture needs a stream of its own (epsr_ctx_set_stream)");
+    HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    ctx->capturing = true;
+    return EPSR_OK;
+}
+
+extern "C" int epsr_graph_end(epsr_ctx *ctx, epsr_graph **out)
+{
+    if (!ctx) return EPSR_ERR_ARG;
+    ARG_CHECK(ctx, out, "epsr_graph_end: null argument");
+    epsr_graph *g = new epsr_graph();
+    ctx->capturing = false;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &g->graph);
+    if (e != hipSuccess) {
+        delete g;
+        return set_error(ctx, EPSR_ERR_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+    }
+    e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        hipGraphDestroy(g->graph);
+        delete g;
+        return set_error(ctx, EPSR_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+    }
+    *out = g;
+    return EPSR_OK;
+}
+
+extern "C" int epsr_graph_launch(epsr_ctx *ctx, const epsr_graph *g)
+{
+    if (!ctx) return EPSR_ERR_ARG;
+    ARG_CHECK(ctx, g && g->exec, "epsr_graph_launch: null graph");
+    HIP_CHECK(ctx, hipGraphLaunch(g->exec, ctx->stream));
+    return EPSR_OK;
+}
+
+extern "C" void epsr_graph_destroy(epsr_graph *g)
+{
+    if (!g) return;
+    if (g->exec) hipGraphExecDestroy(g->exec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    delete g;
 }
 
 // ------------------------------------------------------------------ ran1

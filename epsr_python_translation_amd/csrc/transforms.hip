@@ -143,7 +143,7 @@ extern "C" int epsr_gensq(epsr_ctx *ctx, int nq, int npairs, int npt1, int ldw, 
     }
     timer_end(ctx, "gensq");
     HIP_CHECK(ctx, hipGetLastError());
-    HIP_CHECK(ctx, hipStreamSynchronize(st));
+    HIP_CHECK(ctx, ctx_sync(ctx));
     return EPSR_OK;
 }
 
@@ -159,8 +159,16 @@ __device__ __forceinline__ float dev_lorch(float x, float pi)
     return 0.0f;
 }
 
+// sfac = 0.5*qstep/pi/pi/rho with qstep = q(4) - q(3) (:4076-4077), on the device
+__device__ __forceinline__ float stog_sfac(const float *__restrict__ q, float rho)
+{
+    const float pi = 3.14159265358979323846f;
+    const float qstep = q[4] - q[3];
+    return 0.5f * qstep / pi / pi / rho;
+}
+
 // ato_manipulation_rev4.py:4066-4111, one thread per (r point j, column in)
-__global__ void stog_exact(float sfac, const float *__restrict__ q, const float *__restrict__ sofq, int ldq,
+__global__ void stog_exact(float rho, const float *__restrict__ q, const float *__restrict__ sofq, int ldq,
                            const float *__restrict__ r, float *__restrict__ gofr, int ldg,
                            const int *__restrict__ nqmin, const int *__restrict__ nqmax, int nr, int ncols,
                            const float *__restrict__ xwt)
@@ -170,6 +178,7 @@ __global__ void stog_exact(float sfac, const float *__restrict__ q, const float 
     if (t >= nr * ncols) return;
     int in = t / nr;
     int j = t - in * nr + 1;
+    const float sfac = stog_sfac(q, rho);
     float rr = r[j];
     float qmax = pi / q[nqmax[in]];
     const float *sq = sofq + (size_t)in * ldq;
@@ -203,7 +212,7 @@ __global__ void stog_prepare_b(const float *__restrict__ q, const float *__restr
 }
 
 // stog, mode 1: A(j, i) = q_i sin(q_i r_j) (q_i^2 at r = 0) generated in registers.
-__global__ void __launch_bounds__(64) stog_mfma(float sfac, const float *__restrict__ q, int nqpad,
+__global__ void __launch_bounds__(64) stog_mfma(float rho, const float *__restrict__ q, int nqpad,
                                                 const float *__restrict__ B, const float *__restrict__ r,
                                                 float *__restrict__ gofr, int ldg, int nr, int ncols,
                                                 const float *__restrict__ xwt)
@@ -214,6 +223,7 @@ __global__ void __launch_bounds__(64) stog_mfma(float sfac, const float *__restr
     const int jrow = m0 + rl + 1;   // r index (1-based)
     const float rr = jrow <= nr ? r[jrow] : 0.0f;
     const int ncol = n0 + rl;
+    const float sfac = stog_sfac(q, rho);
     floatx4 acc = {0.f, 0.f, 0.f, 0.f};
     for (int k = 0; k < nqpad; k += 4) {
         int i = k + kq;
@@ -242,17 +252,10 @@ extern "C" int epsr_stog(epsr_ctx *ctx, float rho, const float *q, const float *
     ARG_CHECK(ctx, mode == 0 || mode == 1, "epsr_stog: mode must be 0 or 1");
     HIP_CHECK(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
-    // qstep = q(4) - q(3); sfac = 0.5*qstep/pi/pi/rho (:4076-4077)
-    float qh[5];
-    HIP_CHECK(ctx, hipMemcpyAsync(qh, q, sizeof(float) * 5, hipMemcpyDeviceToHost, st));
-    HIP_CHECK(ctx, hipStreamSynchronize(st));
-    const float pi = 3.14159265358979323846f;
-    float qstep = qh[4] - qh[3];
-    float sfac = 0.5f * qstep / pi / pi / rho;
     timer_begin(ctx, "stog");
     if (mode == 0) {
         int n = nr * ncols;
-        stog_exact<<<ceil_div(n, 64), 64, 0, st>>>(sfac, q, sofq, ldq, r, gofr, ldg, nqmin, nqmax, nr, ncols, xwt);
+        stog_exact<<<ceil_div(n, 64), 64, 0, st>>>(rho, q, sofq, ldq, r, gofr, ldg, nqmin, nqmax, nr, ncols, xwt);
     } else {
         // the Q range used is 1..ldq-1 (nqmax <= ldq-1 is the caller's contract)
         int nqpad = ldq;
@@ -260,11 +263,11 @@ extern "C" int epsr_stog(epsr_ctx *ctx, float rho, const float *q, const float *
         if (!B) return EPSR_ERR_HIP;
         stog_prepare_b<<<ceil_div(nqpad * ncols, 256), 256, 0, st>>>(q, sofq, ldq, nqmin, nqmax, nqpad, ncols, B);
         dim3 grid(ceil_div(nr, 16), ceil_div(ncols, 16));
-        stog_mfma<<<grid, 64, 0, st>>>(sfac, q, nqpad, B, r, gofr, ldg, nr, ncols, xwt);
+        stog_mfma<<<grid, 64, 0, st>>>(rho, q, nqpad, B, r, gofr, ldg, nr, ncols, xwt);
     }
     timer_end(ctx, "stog");
     HIP_CHECK(ctx, hipGetLastError());
-    HIP_CHECK(ctx, hipStreamSynchronize(st));
+    HIP_CHECK(ctx, ctx_sync(ctx));
     return EPSR_OK;
 }
 

@@ -154,6 +154,10 @@ class DeviceBox:
     def get_coords(self):
         return {k: getattr(self, k).cpu().numpy() for k in ("xa", "ya", "za", "xadd", "yadd", "zadd")}
 
+    def set_launch(self, knob, value):
+        """launch shape of this box's kernels (_native.LAUNCH_*); results do not depend on it"""
+        self.ctx.check(self.ctx.lib.epsr_box_set_launch(self.h, int(knob), int(value)))
+
     def close(self):
         if getattr(self, "h", None):
             self.ctx.lib.epsr_box_destroy(self.h)
@@ -301,6 +305,50 @@ class Ledger:
             self.close()
         except Exception:
             pass
+
+
+class Graph:
+    """Work issued on a context's stream inside ``with Graph.capture(ctx) as g:`` is
+    recorded into a HIP graph; ``g.launch()`` replays it on the same stream."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.h = None
+
+    @classmethod
+    def capture(cls, ctx: Context):
+        return _Capture(cls(ctx))
+
+    def launch(self):
+        self.ctx.check(self.ctx.lib.epsr_graph_launch(self.ctx.h, self.h))
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.epsr_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _Capture:
+    def __init__(self, g: Graph):
+        self.g = g
+
+    def __enter__(self):
+        self.g.ctx.check(self.g.ctx.lib.epsr_graph_begin(self.g.ctx.h))
+        return self.g
+
+    def __exit__(self, et, ev, tb):
+        h = C.c_void_p()
+        rc = self.g.ctx.lib.epsr_graph_end(self.g.ctx.h, C.byref(h))
+        if et is None:
+            self.g.ctx.check(rc)
+            self.g.h = h
+        return False
 
 
 def en_mole_pair(ctx: Context, box: DeviceBox, tables: DeviceTables, im, jm):

@@ -116,10 +116,28 @@ class Refinement:
                                 ngrsamples, self.ran1, wav=self.wav, potex=self.potex, potexmax=self.potexmax)
         return ngr
 
-    def gensq(self, mode=1):
-        D.gensq(self.ctx, self.g.nq, self.s.npairs, self.g.npt1, self.ldw, self.wav, self.ycoh, self.foqols,
+    def gensq(self, mode=1, ctx=None):
+        D.gensq(ctx or self.ctx, self.g.nq, self.s.npairs, self.g.npt1, self.ldw, self.wav, self.ycoh, self.foqols,
                 self.ldf, self.sofq, self.ldq, mode=mode)
         return self.sofq
+
+    def stog(self, mode=1, ctx=None):
+        """g(r) of the intermolecular partials on the r grid 1..npt1 (stog, Q points 2..nq, no
+        weights), into self.gofr [npairs][npt1 + 1].  ctx: another context on the same
+        device (a graph's), else the box's own."""
+        s, g = self.s, self.g
+        ctx = ctx or self.ctx
+        if getattr(self, "gofr", None) is None:
+            self.nr = g.npt1
+            self.r = ctx.tensor(g.x[:self.nr + 1])
+            self.ldg = self.nr + 1
+            self.gofr = ctx.zeros(s.npairs * self.ldg)
+            self.nqmin = ctx.tensor(np.full(s.npairs, 2, np.int32))
+            self.nqmax = ctx.tensor(np.full(s.npairs, g.nq, np.int32))
+            self.xwt = ctx.tensor(np.zeros(s.npairs, F32))
+        D.stog(ctx, s.rho, self.q, self.sofq[s.npairs * self.ldq:], self.ldq, self.r, self.gofr, self.ldg,
+               self.nqmin, self.nqmax, self.nr, s.npairs, self.xwt, mode=mode)
+        return self.gofr
 
 
 class EPParams(C.Structure):

@@ -70,6 +70,26 @@ typedef struct {
 } epsr_box_desc_t;
 
 int epsr_box_create(epsr_ctx *ctx, const epsr_box_desc_t *desc, epsr_box **out);
+
+/* Launch shapes of a box's kernels (no counterpart in the reference: the reference has
+ * no launches), set by the autotuner per box size: EPSR_LAUNCH_ATOGEN_WORKGROUPS, the
+ * workgroups atogen's pair kernel aims for (partner slices per origin block; default
+ * 1024), and EPSR_LAUNCH_PAIR_BLOCK, the workgroup size of mc_pairs and fmolec1's pair
+ * kernel (256, 512 or 1024; default 512).  Results do not depend on them. */
+#define EPSR_LAUNCH_ATOGEN_WORKGROUPS 1
+#define EPSR_LAUNCH_PAIR_BLOCK 2
+int epsr_box_set_launch(epsr_box *box, int knob, int value);
+
+/* HIP graphs of a context's stream (no counterpart in the reference): work issued on
+ * the stream between epsr_graph_begin and epsr_graph_end is captured, not run, and
+ * epsr_graph_launch replays it.  Entry points that read results back to the host (MC
+ * cycles, atogen's origin selection, fmolec1) cannot be captured; gensq and stog can,
+ * after one eager call has sized the context's scratch. */
+typedef struct epsr_graph epsr_graph;
+int epsr_graph_begin(epsr_ctx *ctx);
+int epsr_graph_end(epsr_ctx *ctx, epsr_graph **out);
+int epsr_graph_launch(epsr_ctx *ctx, const epsr_graph *graph);
+void epsr_graph_destroy(epsr_graph *graph);
 void epsr_box_destroy(epsr_box *box);
 
 /* Mutable coordinates, dev arrays owned by the caller: atom offsets from the
