@@ -80,12 +80,28 @@ struct epsr_ledger {
     // while the GPU runs the cycle (valid when the next call starts from ahead_from)
     float *pin[2] = {nullptr, nullptr};
     size_t pin_cap = 0;
+    // pinned block of a cycle's small transfers (results down, step sizes up)
+    int *hio = nullptr;
+    size_t hio_cap = 0;
     int pin_ahead = 0;
     bool ahead_valid = false;
     int ahead_total = 0;
     int32_t ahead_from[35], ahead_to[35];
     bool ledger_ready = false;
-    std::vector<int> h_rowstart, h_cnt;
+    int maxcnt = 0;   // the longest list
+    // neighbour offsets were formed for this lattice size and rmax
+    int nbr_na = 0;
+    float nbr_rmax = -1.0f;
+    // device copy of the cell arrays of cells_init, its pinned staging buffer and a small
+    // pinned block for fmolec1's scalars
+    int *d_cells = nullptr;
+    size_t cap_cells = 0;
+    void *hstage = nullptr;
+    size_t hstage_cap = 0;
+    int *hsc = nullptr;
+    // fmolec1's device work buffers (grow-only, kept between calls)
+    void *wbuf[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    size_t wcap[5] = {0, 0, 0, 0, 0};
 };
 
 // ------------------------------------------------------------------ errors

@@ -65,6 +65,11 @@ extern "C" void epsr_ledger_destroy(epsr_ledger *l)
     hipFree(l->d_roff); hipFree(l->d_toff); hipFree(l->d_moff);
     if (l->pin[0]) hipHostFree(l->pin[0]);
     if (l->pin[1]) hipHostFree(l->pin[1]);
+    if (l->hio) hipHostFree(l->hio);
+    if (l->hstage) hipHostFree(l->hstage);
+    if (l->hsc) hipHostFree(l->hsc);
+    hipFree(l->d_cells);
+    for (void *w : l->wbuf) hipFree(w);
     delete l;
 }
 
@@ -102,6 +107,21 @@ static int h_cellcoord(float x, int n)
     return fold(ix, n);
 }
 
+// grow-only pinned staging buffer of the ledger; a stream synchronisation first, since
+// copies from the old buffer may still be in flight
+static int ledger_stage(epsr_ctx *ctx, epsr_ledger *l, size_t bytes)
+{
+    if (bytes <= l->hstage_cap) return EPSR_OK;
+    HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (l->hstage) HIP_CHECK(ctx, hipHostFree(l->hstage));
+    l->hstage = nullptr;
+    l->hstage_cap = 0;
+    const size_t want = bytes + bytes / 8 + 4096;
+    HIP_CHECK(ctx, hipHostMalloc(&l->hstage, want));
+    l->hstage_cap = want;
+    return EPSR_OK;
+}
+
 extern "C" int epsr_cells_init(epsr_ctx *ctx, const epsr_box *box, epsr_ledger *l, const epsr_coords_t *c,
                                int nmolcellin, float rmax)
 {
@@ -109,11 +129,15 @@ extern "C" int epsr_cells_init(epsr_ctx *ctx, const epsr_box *box, epsr_ledger *
     ARG_CHECK(ctx, box && l && c && nmolcellin > 0, "epsr_cells_init: bad argument");
     HIP_CHECK(ctx, hipSetDevice(ctx->device));
     const int nmol = box->nmol;
-    std::vector<float> xadd(nmol), yadd(nmol), zadd(nmol);
     hipStream_t st = ctx->stream;
-    HIP_CHECK(ctx, hipMemcpyAsync(xadd.data(), c->xadd, sizeof(float) * nmol, hipMemcpyDeviceToHost, st));
-    HIP_CHECK(ctx, hipMemcpyAsync(yadd.data(), c->yadd, sizeof(float) * nmol, hipMemcpyDeviceToHost, st));
-    HIP_CHECK(ctx, hipMemcpyAsync(zadd.data(), c->zadd, sizeof(float) * nmol, hipMemcpyDeviceToHost, st));
+    int rc;
+    // the centres come down through the pinned staging buffer (the previous call's uploads
+    // from it are ordered before these copies on the stream)
+    if ((rc = ledger_stage(ctx, l, sizeof(float) * 3 * (size_t)nmol))) return rc;
+    const float *xadd = (const float *)l->hstage, *yadd = xadd + nmol, *zadd = yadd + nmol;
+    HIP_CHECK(ctx, hipMemcpyAsync((void *)xadd, c->xadd, sizeof(float) * nmol, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(ctx, hipMemcpyAsync((void *)yadd, c->yadd, sizeof(float) * nmol, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(ctx, hipMemcpyAsync((void *)zadd, c->zadd, sizeof(float) * nmol, hipMemcpyDeviceToHost, st));
     HIP_CHECK(ctx, hipStreamSynchronize(st));
     // initialise_cellneighbourlist (:23-96)
     int nmolcell = std::min(nmolcellin, nmol);
@@ -142,8 +166,8 @@ extern "C" int epsr_cells_init(epsr_ctx *ctx, const epsr_box *box, epsr_ledger *
         int cc = l->molcellno[im];
         l->molsincell[(size_t)cc * l->maxcellno + l->nmolsincell[cc]++] = im;
     }
-    // makecellneighbourlist (:102-194)
-    {
+    // makecellneighbourlist (:102-194); the offsets depend on the lattice and rmax only
+    if (l->nbr_na != l->na || l->nbr_rmax != rmax) {
         float cella = 1.0f / (float)l->na, cellb = 1.0f / (float)l->nb, cellc = 1.0f / (float)l->nc;
         float cella2 = 0.5f * cella, cellb2 = 0.5f * cellb, cellc2 = 0.5f * cellc;
         float xc[8], yc[8], zc[8];
@@ -181,6 +205,8 @@ extern "C" int epsr_cells_init(epsr_ctx *ctx, const epsr_box *box, epsr_ledger *
             }
         }
         l->ncellneighb = (int)l->iaoff.size();
+        l->nbr_na = l->na;
+        l->nbr_rmax = rmax;
     }
     // chain order of molfit (cells in order, molecules in cell order)
     l->order.clear();
@@ -190,8 +216,29 @@ extern "C" int epsr_cells_init(epsr_ctx *ctx, const epsr_box *box, epsr_ledger *
     l->pos.assign(nmol, 0);
     for (int k = 0; k < nmol; ++k) l->pos[l->order[k]] = k;
     l->nnbrefmax = 0;   // get_nnbrefmax: formed on the GPU by epsr_fmolec1
-    HIP_CHECK(ctx, hipMemcpy(l->d_order, l->order.data(), sizeof(int) * nmol, hipMemcpyHostToDevice));
-    HIP_CHECK(ctx, hipMemcpy(l->d_pos, l->pos.data(), sizeof(int) * nmol, hipMemcpyHostToDevice));
+    // upload, asynchronously from the pinned staging buffer: the cell arrays for
+    // epsr_fmolec1 ([offsets a, b, c][molecules per cell][members of each cell, maxcellno
+    // each][cell of each molecule]), the chain order and the chain position of each molecule
+    const int nbr = l->ncellneighb;
+    const size_t ncw = 3 * (size_t)nbr + (size_t)l->ncell + (size_t)l->ncell * l->maxcellno + (size_t)nmol;
+    if ((rc = ledger_stage(ctx, l, sizeof(int) * (ncw + 2 * (size_t)nmol)))) return rc;
+    int *hu = (int *)l->hstage;
+    std::copy(l->iaoff.begin(), l->iaoff.end(), hu);
+    std::copy(l->iboff.begin(), l->iboff.end(), hu + nbr);
+    std::copy(l->icoff.begin(), l->icoff.end(), hu + 2 * nbr);
+    std::copy(l->nmolsincell.begin(), l->nmolsincell.end(), hu + 3 * nbr);
+    std::copy(l->molsincell.begin(), l->molsincell.end(), hu + 3 * nbr + l->ncell);
+    std::copy(l->molcellno.begin(), l->molcellno.end(), hu + ncw - nmol);
+    std::copy(l->order.begin(), l->order.end(), hu + ncw);
+    std::copy(l->pos.begin(), l->pos.end(), hu + ncw + nmol);
+    if (l->cap_cells < ncw) {
+        const size_t cap = ncw + ncw / 8 + 64;
+        if ((rc = dev_alloc(ctx, &l->d_cells, cap))) return rc;
+        l->cap_cells = cap;
+    }
+    HIP_CHECK(ctx, hipMemcpyAsync(l->d_cells, hu, sizeof(int) * ncw, hipMemcpyHostToDevice, st));
+    HIP_CHECK(ctx, hipMemcpyAsync(l->d_order, hu + ncw, sizeof(int) * nmol, hipMemcpyHostToDevice, st));
+    HIP_CHECK(ctx, hipMemcpyAsync(l->d_pos, hu + ncw + nmol, sizeof(int) * nmol, hipMemcpyHostToDevice, st));
     l->cells_ready = true;
     l->ledger_ready = false;
     return EPSR_OK;
@@ -298,19 +345,25 @@ struct PreparedTables {
     bool in_lds;
 };
 
-static int prepare_tables(epsr_ctx *ctx, const epsr_box *box, const epsr_tables_t *tb, float4 *dT, int *dscratch_int,
-                          PreparedTables &pt)
+// the last bin (up to npt2) where any of the four tables is non-zero, into *dlast
+static int tables_extent(epsr_ctx *ctx, const epsr_box *box, const epsr_tables_t *tb, int *dlast)
 {
     hipStream_t st = ctx->stream;
     const int np = box->npairs, ld = tb->ld;
-    int lim = std::min(tb->npt2, ld - 1);
-    HIP_CHECK(ctx, hipMemsetAsync(dscratch_int, 0, sizeof(int), st));
+    const int lim = std::min(tb->npt2, ld - 1);
+    HIP_CHECK(ctx, hipMemsetAsync(dlast, 0, sizeof(int), st));
     last_nonzero<<<ceil_div((long)np * ld, 256), 256, 0, st>>>(tb->xavs, tb->xavs1, tb->ecoul, tb->ecoul1, ld, np, lim,
-                                                               dscratch_int);
-    int last = 0;
-    HIP_CHECK(ctx, hipMemcpyAsync(&last, dscratch_int, sizeof(int), hipMemcpyDeviceToHost, st));
-    HIP_CHECK(ctx, hipStreamSynchronize(st));
-    int nte = std::max(2, last + 1);
+                                                               dlast);
+    return EPSR_OK;
+}
+
+// pack the tables up to bin `last` into dT
+static int tables_pack(epsr_ctx *ctx, const epsr_box *box, const epsr_tables_t *tb, int last, float4 *dT,
+                       PreparedTables &pt)
+{
+    hipStream_t st = ctx->stream;
+    const int np = box->npairs, ld = tb->ld;
+    const int nte = std::max(2, last + 1);
     pack_tables<<<ceil_div((long)np * nte, 256), 256, 0, st>>>(tb->xavs, tb->xavs1, tb->ecoul, tb->ecoul1, ld, np, nte,
                                                                dT);
     pt.tab.T = dT;
@@ -323,6 +376,17 @@ static int prepare_tables(epsr_ctx *ctx, const epsr_box *box, const epsr_tables_
     pt.lds_bytes = sizeof(float4) * (size_t)np * nte + sizeof(float) * nte + sizeof(int) * box->ntype * box->ntype + 64;
     pt.in_lds = pt.lds_bytes <= 150 * 1024;
     return EPSR_OK;
+}
+
+static int prepare_tables(epsr_ctx *ctx, const epsr_box *box, const epsr_tables_t *tb, float4 *dT, int *dscratch_int,
+                          PreparedTables &pt)
+{
+    int rc = tables_extent(ctx, box, tb, dscratch_int);
+    if (rc) return rc;
+    int last = 0;
+    HIP_CHECK(ctx, hipMemcpyAsync(&last, dscratch_int, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return tables_pack(ctx, box, tb, last, dT, pt);
 }
 
 // copy the tables into LDS; returns a PairTab pointing at LDS
@@ -346,17 +410,18 @@ __device__ PairTab tables_to_lds(const PairTab &g, int npairs, char *smem)
 // ------------------------------------------------------------------ fmolec1 kernels
 // F1: for every chain position k (molecule im) and every position inn >= 1 of its cell's
 // neighbour list with jm > im: E = en_mole_pair(im, jm) (ato_manipulation_rev4.py:2454-2462).
-// Items are (k, inn) flattened with rowoff (int64 prefix over k); persistent blocks.
+// One wave per chain row; the non-zero energies of a row and their partners are packed to
+// the front of the row's item range (rowoff, int64 prefix over k) in neighbour order, and
+// their number goes to rowcnt[k].
 template <bool LDS>
 __global__ void fmolec1_pairs(const long long *__restrict__ rowoff, int nmol, const int *__restrict__ order,
                               const int *__restrict__ mcell, const int *__restrict__ cstart,
                               const int *__restrict__ nbflat, const float4 *__restrict__ P,
                               const int *__restrict__ naf, const int *__restrict__ nal, PairTab gt, int npairs,
-                              float *__restrict__ E, int *__restrict__ flag)
+                              float *__restrict__ rowE, int *__restrict__ rowJ, int *__restrict__ rowcnt)
 {
     extern __shared__ char smem[];
     PairTab t = LDS ? tables_to_lds(gt, npairs, smem) : gt;
-    // one wave per chain position (row k of the items), lanes over its neighbour positions
     const int lane = threadIdx.x & 63;
     const int nw = gridDim.x * (blockDim.x >> 6);
     for (int k = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); k < nmol; k += nw) {
@@ -366,76 +431,103 @@ __global__ void fmolec1_pairs(const long long *__restrict__ rowoff, int nmol, co
         const int *nb = nbflat + cstart[mcell[im]];
         const float4 *P1 = P + naf[im];
         const int n1 = nal[im] - naf[im] + 1;
-        for (int inn = lane; inn < len; inn += 64) {
-            const int jm = nb[inn];
+        int base = 0;
+        for (int i0 = 0; i0 < len; i0 += 64) {   // wave-uniform trip count
+            const int inn = i0 + lane;
             float e = 0.0f;
-            int f = 0;
-            if (inn >= 1 && jm > im) {
-                e = mol_pair(P1, n1, P + naf[jm], nal[jm] - naf[jm] + 1, t);
-                f = e != 0.0f;
+            int jm = 0;
+            if (inn < len) {
+                jm = nb[inn];
+                if (inn >= 1 && jm > im) e = mol_pair(P1, n1, P + naf[jm], nal[jm] - naf[jm] + 1, t);
             }
-            E[r0 + inn] = e;
-            flag[r0 + inn] = f;
+            const bool keep = e != 0.0f;
+            const unsigned long long mask = __ballot(keep);
+            if (keep) {
+                const int at = base + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                                                     __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+                rowE[r0 + at] = e;
+                rowJ[r0 + at] = jm;
+            }
+            base += __popcll(mask);
         }
+        if (lane == 0) rowcnt[k] = base;
     }
 }
 
-__global__ void fmolec1_compact(const long long *__restrict__ rowoff, int nmol, const int *__restrict__ order,
-                                const int *__restrict__ mcell, const int *__restrict__ cstart,
-                                const int *__restrict__ nbflat, const float *__restrict__ E,
-                                const int *__restrict__ flag, const int *__restrict__ idx, int *__restrict__ pim,
-                                int *__restrict__ pjm, float *__restrict__ pE)
+// cat[m] = (eintra[m] != 0) for the self entries, cat[2 nmol] = 0 (rows in between come
+// from fmolec1_pairs); its exclusive scan numbers the ledger entries: self entries in
+// molecule order, then the stored pairs in (chain position, neighbour position) order
+__global__ void fmolec1_self_flags(const float *__restrict__ eintra, int nmol, int *__restrict__ cat)
 {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nmol) cat[t] = eintra[t] != 0.0f;
+    if (t == nmol) cat[2 * nmol] = 0;
+}
+
+// records of the per-molecule lists and the stored energies.  ent is the scan above:
+// entry e < nself = ent[nmol] is a self entry, e = nself + p the pair p.  Record r < nself
+// is the self entry r (key: its molecule); pair p gives records nself + 2p (key: the
+// row's molecule) and nself + 2p + 1 (key: the partner); val = r.
+__global__ void fmolec1_entries(int nmol, const int *__restrict__ order, const long long *__restrict__ rowoff,
+                                const float *__restrict__ rowE, const int *__restrict__ rowJ,
+                                const int *__restrict__ ent, const float *__restrict__ eintra, int *__restrict__ key,
+                                int *__restrict__ val, float *__restrict__ ergpair)
+{
+    const int nself = ent[nmol];
+    for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < nmol; m += gridDim.x * blockDim.x) {
+        const float e = eintra[m];
+        if (e != 0.0f) {
+            const int r = ent[m];
+            key[r] = m;
+            val[r] = r;
+            ergpair[r] = e;
+        }
+    }
     const int lane = threadIdx.x & 63;
     const int nw = gridDim.x * (blockDim.x >> 6);
     for (int k = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); k < nmol; k += nw) {
-        const long long r0 = rowoff[k];
-        const int len = (int)(rowoff[k + 1] - r0);
+        const int e0 = ent[nmol + k], c = ent[nmol + k + 1] - e0;
         const int im = order[k];
-        const int *nb = nbflat + cstart[mcell[im]];
-        for (int inn = lane; inn < len; inn += 64) {
-            const long long item = r0 + inn;
-            if (!flag[item]) continue;
-            const int p = idx[item];
-            pim[p] = im;
-            pjm[p] = nb[inn];
-            pE[p] = E[item];
+        const long long r0 = rowoff[k];
+        for (int j = lane; j < c; j += 64) {
+            const int e = e0 + j, r = nself + 2 * (e - nself);
+            key[r] = im;
+            val[r] = r;
+            key[r + 1] = rowJ[r0 + j];
+            val[r + 1] = r + 1;
+            ergpair[e] = rowE[r0 + j];
         }
     }
 }
 
-// records for the per-molecule lists: self record r < nself (key = molecule), pair p
-// gives records nself + 2p (key = first molecule) and nself + 2p + 1 (key = second)
-__global__ void fmolec1_records(int nmol, const float *__restrict__ eintra, const int *__restrict__ sidx, int nself,
-                                int npair, const int *__restrict__ pim, const int *__restrict__ pjm,
-                                int *__restrict__ key, int *__restrict__ val)
+// row starts from the records sorted by molecule: rowstart[m] = first record of m (the
+// next molecule's for an empty list), rowstart[nmol] = nrec
+__global__ void fmolec1_rowstart(const int *__restrict__ skey, int nrec, int nmol, int *__restrict__ rowstart)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < nmol && eintra[t] != 0.0f) {
-        int r = sidx[t];
-        key[r] = t;
-        val[r] = r;
-    }
-    if (t < npair) {
-        int r = nself + 2 * t;
-        key[r] = pim[t];
-        val[r] = r;
-        key[r + 1] = pjm[t];
-        val[r + 1] = r + 1;
-    }
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > nrec) return;
+    const int lo = i == 0 ? 0 : skey[i - 1] + 1;
+    const int hi = i == nrec ? nmol : skey[i];
+    for (int m = lo; m <= hi; ++m) rowstart[m] = i;
 }
 
-__global__ void count_keys(const int *__restrict__ key, int n, int *__restrict__ cnt)
+// list lengths and the longest list
+__global__ void fmolec1_counts(const int *__restrict__ rowstart, int nmol, int *__restrict__ cnt,
+                               int *__restrict__ maxcnt)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) atomicAdd(&cnt[key[t]], 1);
+    int v = 0;
+    if (t < nmol) {
+        v = rowstart[t + 1] - rowstart[t];
+        cnt[t] = v;
+    }
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_down(v, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(maxcnt, v);
 }
 
 __global__ void fmolec1_rows(const int *__restrict__ skey, const int *__restrict__ sval, int nrec, int nself,
-                             const int *__restrict__ pim, const int *__restrict__ pjm, const float *__restrict__ pE,
-                             const float *__restrict__ eintra, const int *__restrict__ rowstart,
-                             int *__restrict__ partner, int *__restrict__ entry, int *__restrict__ slotmol,
-                             int *__restrict__ sortedpos, float *__restrict__ ergpair)
+                             const int *__restrict__ key, int *__restrict__ partner, int *__restrict__ entry,
+                             int *__restrict__ slotmol, int *__restrict__ sortedpos)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nrec) return;
@@ -445,12 +537,10 @@ __global__ void fmolec1_rows(const int *__restrict__ skey, const int *__restrict
     if (r < nself) {
         partner[i] = m;
         entry[i] = r;
-        ergpair[r] = eintra[m];
     } else {
-        int p = (r - nself) >> 1, side = (r - nself) & 1;
-        partner[i] = side ? pim[p] : pjm[p];
-        entry[i] = nself + p;
-        if (!side) ergpair[nself + p] = pE[p];
+        const int q = r - nself;
+        partner[i] = key[nself + (q ^ 1)];   // the other record of the pair holds the partner
+        entry[i] = nself + (q >> 1);
     }
 }
 
@@ -460,7 +550,7 @@ __global__ void fmolec1_mirror(const int *__restrict__ sval, int nrec, int nself
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nrec) return;
     int r = sval[i];
-    mirror[i] = r < nself ? -1 : sortedpos[nself + (((r - nself) >> 1) << 1) + (1 - ((r - nself) & 1))];
+    mirror[i] = r < nself ? -1 : sortedpos[nself + ((r - nself) ^ 1)];
 }
 
 __global__ void max_int(const int *__restrict__ a, int n, int *__restrict__ out)
@@ -471,24 +561,37 @@ __global__ void max_int(const int *__restrict__ a, int n, int *__restrict__ out)
     if ((threadIdx.x & 63) == 0) atomicMax(out, v);
 }
 
-__global__ void flag_nonzero(const float *__restrict__ a, int n, int *__restrict__ f)
-{
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < n) f[t] = a[t] != 0.0f;
-}
-
-__global__ void sum_f64(const float *__restrict__ a, int n, double *out)
+// FP64 sums of two float arrays in a fixed order: per-block partials (blockIdx.y picks
+// the array), then one block per array adds the partials
+__global__ void __launch_bounds__(256)
+sum_f64_parts(const float *__restrict__ a, int na, const float *__restrict__ b, int nb, double *__restrict__ part)
 {
     __shared__ double s[256];
+    const float *x = blockIdx.y ? b : a;
+    const int n = blockIdx.y ? nb : na;
     double v = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v += (double)a[i];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) v += (double)x[i];
     s[threadIdx.x] = v;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicAdd(out, s[0]);
+    if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = s[0];
+}
+
+__global__ void __launch_bounds__(256) sum_f64_final(const double *__restrict__ part, int nparts, double *__restrict__ out)
+{
+    __shared__ double s[256];
+    double v = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += 256) v += part[blockIdx.x * nparts + i];
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = s[0];
 }
 
 // grid size for persistent loops
@@ -589,26 +692,55 @@ __global__ void chain_rowlen(int nmol, const int *__restrict__ order, const int 
     if (k == nmol) out[k] = 0;
 }
 
-// stream-ordered release of temporaries on every return path
-struct StreamFree {
-    hipStream_t st;
-    std::vector<void *> p;
-    ~StreamFree()
-    {
-        for (void *q : p)
-            if (q) (void)hipFreeAsync(q, st);
+// grow-only device work buffer `i` of the ledger (kept between calls)
+static int ledger_work(epsr_ctx *ctx, epsr_ledger *l, int i, size_t bytes, void **out)
+{
+    if (bytes > l->wcap[i]) {
+        if (l->wbuf[i]) {
+            HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+            HIP_CHECK(ctx, hipFree(l->wbuf[i]));
+        }
+        l->wbuf[i] = nullptr;
+        l->wcap[i] = 0;
+        const size_t want = bytes + bytes / 8 + 4096;
+        HIP_CHECK(ctx, hipMalloc(&l->wbuf[i], want));
+        l->wcap[i] = want;
     }
-};
+    *out = l->wbuf[i];
+    return EPSR_OK;
+}
 
-static int hipcub_scan_int(epsr_ctx *ctx, const int *in, int *out, int n)
+enum { LW_CELL = 0, LW_ROW = 1, LW_TMP = 2, LW_REC = 3, LW_SUM = 4 };
+
+// exclusive sums with the ledger's temporary storage
+template <class T>
+static int ledger_scan(epsr_ctx *ctx, epsr_ledger *l, const T *in, T *out, int n)
 {
     size_t tb = 0;
     HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, n, ctx->stream));
     void *tmp = nullptr;
-    HIP_CHECK(ctx, hipMallocAsync(&tmp, tb + 16, ctx->stream));
+    int rc = ledger_work(ctx, l, LW_TMP, tb + 16, &tmp);
+    if (rc) return rc;
     HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, tb, in, out, n, ctx->stream));
-    HIP_CHECK(ctx, hipFreeAsync(tmp, ctx->stream));
     return EPSR_OK;
+}
+
+// the host-side scalars of the list set-up: candidate items (clamped to INT_MAX), flat
+// list length, longest list, last non-zero table bin
+__global__ void fmolec1_report(const long long *__restrict__ total, const int *__restrict__ nflat,
+                               const int *__restrict__ tail, int *__restrict__ rep)
+{
+    const long long t = *total;
+    rep[0] = t >= 0x7fffffffLL ? 0x7fffffff : (int)t;
+    rep[1] = *nflat;
+    rep[2] = tail[0];
+    rep[3] = tail[1];
+}
+
+__global__ void copy_two(const int *__restrict__ a, const int *__restrict__ b, int *__restrict__ out)
+{
+    out[0] = *a;
+    out[1] = *b;
 }
 
 extern "C" int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *c, const epsr_tables_t *tb,
@@ -621,197 +753,152 @@ extern "C" int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coord
     HIP_CHECK(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     const int nmol = box->nmol;
-    // neighbour lists on the GPU: per cell the neighbour cells and their prefix counts,
-    // the flattened molecule list of every non-empty cell and the row offsets of the
-    // (chain position, neighbour position) items
+    int rc;
+    if (!l->hsc) HIP_CHECK(ctx, hipHostMalloc((void **)&l->hsc, 64));
+    int *hsc = l->hsc;
+    // neighbour lists on the GPU from the cell arrays cells_init uploaded: per cell the
+    // neighbour cells and their prefix counts, the flattened molecule list of every
+    // non-empty cell and the row offsets of the (chain position, neighbour position) items
     const int ncell = l->ncell, nbr = l->ncellneighb;
     const long long ncn = (long long)ncell * nbr;
+    const size_t cw_ints = 2 * (size_t)ncn + (size_t)ncell * (nbr + 1) + 3 * ((size_t)ncell + 1) + 16;
     int *cw = nullptr;
-    const size_t cw_ints = 3 * (size_t)nbr + (size_t)ncell + (size_t)ncell * l->maxcellno + (size_t)nmol +
-                           2 * (size_t)ncn + (size_t)ncell * (nbr + 1) + 3 * ((size_t)ncell + 1) + 8;
-    long long *drow = nullptr, *dlen = nullptr;
-    StreamFree guard{st};
-    HIP_CHECK(ctx, hipMallocAsync((void **)&cw, sizeof(int) * cw_ints, st));
-    guard.p.push_back(cw);
-    HIP_CHECK(ctx, hipMallocAsync((void **)&drow, sizeof(long long) * (nmol + 1), st));
-    guard.p.push_back(drow);
-    HIP_CHECK(ctx, hipMallocAsync((void **)&dlen, sizeof(long long) * (nmol + 1), st));
-    guard.p.push_back(dlen);
-    int *doff = cw, *dnmc = doff + 3 * nbr, *dmic = dnmc + ncell, *dmcell = dmic + (size_t)ncell * l->maxcellno;
-    int *dnbcell = dmcell + nmol, *dnbcnt = dnbcell + ncn, *dnbpre = dnbcnt + ncn;
+    long long *drow = nullptr;
+    if ((rc = ledger_work(ctx, l, LW_CELL, sizeof(int) * cw_ints, (void **)&cw))) return rc;
+    if ((rc = ledger_work(ctx, l, LW_ROW, sizeof(long long) * 2 * ((size_t)nmol + 1), (void **)&drow))) return rc;
+    long long *dlen = drow + nmol + 1;
+    const int *doff = l->d_cells, *dnmc = doff + 3 * nbr, *dmic = dnmc + ncell;
+    const int *dmcell = dmic + (size_t)ncell * l->maxcellno;
+    int *dnbcell = cw, *dnbcnt = dnbcell + ncn, *dnbpre = dnbcnt + ncn;
     int *drowlen = dnbpre + (size_t)ncell * (nbr + 1), *dflatlen = drowlen + ncell + 1, *dcstart = dflatlen + ncell + 1;
-    int *dcw_tail = dcstart + ncell + 1;
-    {
-        std::vector<int> off(3 * (size_t)nbr);
-        std::copy(l->iaoff.begin(), l->iaoff.end(), off.begin());
-        std::copy(l->iboff.begin(), l->iboff.end(), off.begin() + nbr);
-        std::copy(l->icoff.begin(), l->icoff.end(), off.begin() + 2 * nbr);
-        HIP_CHECK(ctx, hipMemcpyAsync(doff, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice, st));
-        HIP_CHECK(ctx, hipMemcpyAsync(dnmc, l->nmolsincell.data(), sizeof(int) * ncell, hipMemcpyHostToDevice, st));
-        HIP_CHECK(ctx, hipMemcpyAsync(dmic, l->molsincell.data(), sizeof(int) * l->molsincell.size(),
-                                      hipMemcpyHostToDevice, st));
-        HIP_CHECK(ctx, hipMemcpyAsync(dmcell, l->molcellno.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, st));
-        HIP_CHECK(ctx, hipStreamSynchronize(st));   // the host vectors above are not pinned
-    }
+    int *dtail = dcstart + ncell + 1;   // [0] longest list, [1] last table bin, [4..7] report
     cell_neighbours<<<(unsigned)ceil_div(ncn, 256), 256, 0, st>>>(ncell, nbr, l->na, l->nb, l->nc, doff, doff + nbr,
                                                                   doff + 2 * nbr, dnmc, dnbcell, dnbcnt);
     cell_prefix<<<ncell, 256, 0, st>>>(nbr, dnbcnt, dnmc, dnbpre, drowlen, dflatlen);
     HIP_CHECK(ctx, hipMemsetAsync(dflatlen + ncell, 0, sizeof(int), st));
-    HIP_CHECK(ctx, hipMemsetAsync(dcw_tail, 0, sizeof(int), st));
-    int rc;
-    if ((rc = hipcub_scan_int(ctx, dflatlen, dcstart, ncell + 1))) return rc;
-    max_int<<<std::max(1, std::min(64, ceil_div(ncell, 256))), 256, 0, st>>>(drowlen, ncell, dcw_tail);
+    HIP_CHECK(ctx, hipMemsetAsync(dtail, 0, sizeof(int), st));
+    if ((rc = ledger_scan<int>(ctx, l, dflatlen, dcstart, ncell + 1))) return rc;
+    max_int<<<std::max(1, std::min(64, ceil_div(ncell, 256))), 256, 0, st>>>(drowlen, ncell, dtail);
     chain_rowlen<<<ceil_div(nmol + 1, 256), 256, 0, st>>>(nmol, l->d_order, dmcell, drowlen, dlen);
-    {
-        size_t tbytes = 0;
-        HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tbytes, dlen, drow, nmol + 1, st));
-        void *tmpb = nullptr;
-        HIP_CHECK(ctx, hipMallocAsync(&tmpb, tbytes + 16, st));
-        HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(tmpb, tbytes, dlen, drow, nmol + 1, st));
-        HIP_CHECK(ctx, hipFreeAsync(tmpb, st));
-    }
-    long long total = 0;
-    int nflat = 0, nnbrefmax = 0;
-    HIP_CHECK(ctx, hipMemcpyAsync(&total, drow + nmol, sizeof(long long), hipMemcpyDeviceToHost, st));
-    HIP_CHECK(ctx, hipMemcpyAsync(&nflat, dcstart + ncell, sizeof(int), hipMemcpyDeviceToHost, st));
-    HIP_CHECK(ctx, hipMemcpyAsync(&nnbrefmax, dcw_tail, sizeof(int), hipMemcpyDeviceToHost, st));
+    if ((rc = ledger_scan<long long>(ctx, l, dlen, drow, nmol + 1))) return rc;
+    if ((rc = tables_extent(ctx, box, tb, dtail + 1))) return rc;
+    fmolec1_report<<<1, 1, 0, st>>>(drow + nmol, dcstart + ncell, dtail, dtail + 4);
+    HIP_CHECK(ctx, hipMemcpyAsync(hsc, dtail + 4, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_CHECK(ctx, hipStreamSynchronize(st));
-    l->nnbrefmax = nnbrefmax;
-    ARG_CHECK(ctx, total < (1LL << 31), "epsr_fmolec1: %lld candidate pairs exceed the 2^31 item limit", total);
+    const long long total = hsc[0];
+    const int nflat = hsc[1], last = hsc[3];
+    l->nnbrefmax = hsc[2];
+    ARG_CHECK(ctx, total < 0x7fffffffLL, "epsr_fmolec1: candidate pairs exceed the 2^31 item limit");
 
     Scratch S;
-    size_t need = 0;
+    auto layout = [&](Scratch &x) {
+        x.take<float4>(box->natom);
+        x.take<float4>((size_t)box->npairs * std::max(2, std::min(tb->ld, tb->npt2 + 1)));
+        x.take<int>((size_t)nflat + 1);
+        x.take<float>(total); x.take<int>(total);
+        x.take<int>(2 * (size_t)nmol + 1); x.take<int>(2 * (size_t)nmol + 1);
+    };
     {
         Scratch q;
-        q.base = nullptr;
-        q.take<float4>(box->natom); q.take<float4>((size_t)box->npairs * std::max(2, std::min(tb->ld, tb->npt2 + 1)));
-        q.take<int>(4); q.take<int>((size_t)nflat + 1);
-        q.take<float>(total); q.take<int>(total); q.take<int>(total + 1);
-        q.take<int>(nmol); q.take<int>(nmol + 1);
-        need = q.off;
+        layout(q);
+        S.base = (char *)ctx_scratch(ctx, q.off);
+        if (!S.base) return EPSR_ERR_HIP;
     }
-    S.base = (char *)ctx_scratch(ctx, need);
-    if (!S.base) return EPSR_ERR_HIP;
     float4 *dP = S.take<float4>(box->natom);
     float4 *dT = S.take<float4>((size_t)box->npairs * std::max(2, std::min(tb->ld, tb->npt2 + 1)));
-    int *dtmp = S.take<int>(4);
     int *dnb = S.take<int>((size_t)nflat + 1);
-    float *dE = S.take<float>(total);
-    int *dflag = S.take<int>(total);
-    int *didx = S.take<int>(total + 1);
-    int *dsflag = S.take<int>(nmol);
-    int *dsidx = S.take<int>(nmol + 1);
+    float *rowE = S.take<float>(total);
+    int *rowJ = S.take<int>(total);
+    int *cat = S.take<int>(2 * (size_t)nmol + 1);
+    int *ent = S.take<int>(2 * (size_t)nmol + 1);
     cell_flatten<<<(unsigned)ceil_div(ncn, 256), 256, 0, st>>>(ncell, nbr, l->maxcellno, dnbcell, dnbcnt, dnbpre,
                                                                dcstart, dnmc, dmic, dnb);
-
     PreparedTables pt;
-    rc = prepare_tables(ctx, box, tb, dT, dtmp, pt);
-    if (rc) return rc;
+    if ((rc = tables_pack(ctx, box, tb, last, dT, pt))) return rc;
     abs_positions<<<ceil_div(box->natom, 256), 256, 0, st>>>(c->xa, c->ya, c->za, c->xadd, c->yadd, c->zadd,
                                                              box->d_amol, box->d_itype, dP, box->natom);
     {
         const int B = box->pair_block;
         timer_begin(ctx, "fmolec1_pair");
         if (pt.in_lds) {
-            int g = persistent_blocks(pt.lds_bytes, total, B);
+            int g = persistent_blocks(pt.lds_bytes, (long long)nmol * 64, B);
             fmolec1_pairs<true><<<g, B, pt.lds_bytes, st>>>(drow, nmol, l->d_order, dmcell, dcstart, dnb, dP,
-                                                            box->d_naf, box->d_nal, pt.tab, box->npairs, dE, dflag);
+                                                            box->d_naf, box->d_nal, pt.tab, box->npairs, rowE, rowJ,
+                                                            cat + nmol);
         } else {
-            int g = persistent_blocks(0, total, B);
+            int g = persistent_blocks(0, (long long)nmol * 64, B);
             fmolec1_pairs<false><<<g, B, 0, st>>>(drow, nmol, l->d_order, dmcell, dcstart, dnb, dP, box->d_naf,
-                                                  box->d_nal, pt.tab, box->npairs, dE, dflag);
+                                                  box->d_nal, pt.tab, box->npairs, rowE, rowJ, cat + nmol);
         }
         timer_end(ctx, "fmolec1_pair");
     }
-    // global entry numbers: self entries first (molecule order, eintra != 0), then the
-    // stored pairs in (chain position, neighbour position) order
-    HIP_CHECK(ctx, hipMemsetAsync(didx + total, 0, sizeof(int), st));
-    if ((rc = hipcub_scan_int(ctx, dflag, didx, (int)total))) return rc;
-    flag_nonzero<<<ceil_div(nmol, 256), 256, 0, st>>>(eintra, nmol, dsflag);
-    if ((rc = hipcub_scan_int(ctx, dsflag, dsidx, nmol))) return rc;
-    int hlast[2] = {0, 0}, hflag[2] = {0, 0};
-    if (total > 0) {
-        HIP_CHECK(ctx, hipMemcpyAsync(&hlast[0], didx + total - 1, sizeof(int), hipMemcpyDeviceToHost, st));
-        HIP_CHECK(ctx, hipMemcpyAsync(&hflag[0], dflag + total - 1, sizeof(int), hipMemcpyDeviceToHost, st));
-    }
-    HIP_CHECK(ctx, hipMemcpyAsync(&hlast[1], dsidx + nmol - 1, sizeof(int), hipMemcpyDeviceToHost, st));
-    HIP_CHECK(ctx, hipMemcpyAsync(&hflag[1], dsflag + nmol - 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    fmolec1_self_flags<<<ceil_div(nmol + 1, 256), 256, 0, st>>>(eintra, nmol, cat);
+    if ((rc = ledger_scan<int>(ctx, l, cat, ent, 2 * nmol + 1))) return rc;
+    copy_two<<<1, 1, 0, st>>>(ent + nmol, ent + 2 * nmol, dtail + 8);
+    HIP_CHECK(ctx, hipMemcpyAsync(hsc, dtail + 8, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_CHECK(ctx, hipStreamSynchronize(st));
-    const int npair = hlast[0] + hflag[0];
-    const int nself = hlast[1] + hflag[1];
+    const int nself = hsc[0];
+    const int nent = hsc[1];
+    const int npair = nent - nself;
     const int nrec = nself + 2 * npair;
-    const int nent = nself + npair;
 
-    // pair arrays and the sort buffers (separate allocations: they outlive the scratch)
-    int *pim = nullptr, *pjm = nullptr, *key = nullptr, *val = nullptr, *skey = nullptr, *sval = nullptr,
-        *spos = nullptr;
-    float *pE = nullptr;
-    HIP_CHECK(ctx, hipMallocAsync((void **)&pim, sizeof(int) * (npair + 1), st));
-    HIP_CHECK(ctx, hipMallocAsync((void **)&pjm, sizeof(int) * (npair + 1), st));
-    HIP_CHECK(ctx, hipMallocAsync((void **)&pE, sizeof(float) * (npair + 1), st));
-    HIP_CHECK(ctx, hipMallocAsync((void **)&key, sizeof(int) * (nrec + 1), st));
-    HIP_CHECK(ctx, hipMallocAsync((void **)&val, sizeof(int) * (nrec + 1), st));
-    HIP_CHECK(ctx, hipMallocAsync((void **)&skey, sizeof(int) * (nrec + 1), st));
-    HIP_CHECK(ctx, hipMallocAsync((void **)&sval, sizeof(int) * (nrec + 1), st));
-    HIP_CHECK(ctx, hipMallocAsync((void **)&spos, sizeof(int) * (nrec + 1), st));
-    {
-        int g = persistent_blocks(0, total, 256);
-        fmolec1_compact<<<g, 256, 0, st>>>(drow, nmol, l->d_order, dmcell, dcstart, dnb, dE, dflag, didx, pim, pjm, pE);
+    // the per-molecule lists: records sorted by molecule (stable, so a list keeps the self
+    // entry first and its pairs in entry order)
+    if (l->cap_slots < (size_t)nrec + 1) {
+        const size_t cap = (size_t)nrec + nrec / 8 + 64;
+        if ((rc = dev_alloc(ctx, &l->d_partner, cap)) || (rc = dev_alloc(ctx, &l->d_entry, cap)) ||
+            (rc = dev_alloc(ctx, &l->d_mirror, cap)) || (rc = dev_alloc(ctx, &l->d_slotmol, cap)))
+            return rc;
+        l->cap_slots = cap;
     }
-    fmolec1_records<<<ceil_div(std::max(nmol, npair), 256), 256, 0, st>>>(nmol, eintra, dsidx, nself, npair, pim, pjm,
-                                                                           key, val);
+    if (l->cap_entries < (size_t)nent + 1) {
+        const size_t cap = (size_t)nent + nent / 8 + 64;
+        if ((rc = dev_alloc(ctx, &l->d_ergpair, cap))) return rc;
+        l->cap_entries = cap;
+    }
+    int *key = nullptr;
+    const size_t nr1 = (size_t)nrec + 1;
+    if ((rc = ledger_work(ctx, l, LW_REC, sizeof(int) * 5 * nr1, (void **)&key))) return rc;
+    int *val = key + nr1, *skey = val + nr1, *sval = skey + nr1, *spos = sval + nr1;
     {
+        int g = persistent_blocks(0, (long long)nmol * 64, 256);
+        fmolec1_entries<<<g, 256, 0, st>>>(nmol, l->d_order, drow, rowE, rowJ, ent, eintra, key, val, l->d_ergpair);
+    }
+    if (nrec > 0) {
         int bits = 1;
         while ((1 << bits) < nmol) ++bits;
         size_t tbytes = 0;
         HIP_CHECK(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, key, skey, val, sval, nrec, 0, bits, st));
         void *tmpb = nullptr;
-        HIP_CHECK(ctx, hipMallocAsync(&tmpb, tbytes + 16, st));
+        if ((rc = ledger_work(ctx, l, LW_TMP, tbytes + 16, &tmpb))) return rc;
         HIP_CHECK(ctx, hipcub::DeviceRadixSort::SortPairs(tmpb, tbytes, key, skey, val, sval, nrec, 0, bits, st));
-        HIP_CHECK(ctx, hipFreeAsync(tmpb, st));
     }
-    // rows
-    if (l->cap_slots < (size_t)nrec + 1) {
-        if ((rc = dev_alloc(ctx, &l->d_partner, nrec + 1)) || (rc = dev_alloc(ctx, &l->d_entry, nrec + 1)) ||
-            (rc = dev_alloc(ctx, &l->d_mirror, nrec + 1)) || (rc = dev_alloc(ctx, &l->d_slotmol, nrec + 1)))
-            return rc;
-        l->cap_slots = (size_t)nrec + 1;
+    const int gsum = 256;
+    double *dpart = nullptr;
+    if ((rc = ledger_work(ctx, l, LW_SUM, sizeof(double) * (2 * gsum + 4), (void **)&dpart))) return rc;
+    double *dout = dpart + 2 * gsum;   // [0] pairs, [1] self energies, then the longest list (int)
+    int *dmaxcnt = (int *)(dout + 2);
+    HIP_CHECK(ctx, hipMemsetAsync(dmaxcnt, 0, sizeof(int), st));
+    fmolec1_rowstart<<<ceil_div(nrec + 1, 256), 256, 0, st>>>(skey, nrec, nmol, l->d_rowstart);
+    fmolec1_counts<<<ceil_div(nmol, 256), 256, 0, st>>>(l->d_rowstart, nmol, l->d_cnt, dmaxcnt);
+    if (nrec > 0) {
+        fmolec1_rows<<<ceil_div(nrec, 256), 256, 0, st>>>(skey, sval, nrec, nself, key, l->d_partner, l->d_entry,
+                                                          l->d_slotmol, spos);
+        fmolec1_mirror<<<ceil_div(nrec, 256), 256, 0, st>>>(sval, nrec, nself, spos, l->d_mirror);
     }
-    if (l->cap_entries < (size_t)nent + 1) {
-        if ((rc = dev_alloc(ctx, &l->d_ergpair, nent + 1))) return rc;
-        l->cap_entries = (size_t)nent + 1;
-    }
-    HIP_CHECK(ctx, hipMemsetAsync(l->d_cnt, 0, sizeof(int) * nmol, st));
-    count_keys<<<ceil_div(nrec, 256), 256, 0, st>>>(skey, nrec, l->d_cnt);
-    if ((rc = hipcub_scan_int(ctx, l->d_cnt, l->d_rowstart, nmol))) return rc;
-    fmolec1_rows<<<ceil_div(nrec, 256), 256, 0, st>>>(skey, sval, nrec, nself, pim, pjm, pE, eintra, l->d_rowstart,
-                                                      l->d_partner, l->d_entry, l->d_slotmol, spos, l->d_ergpair);
-    fmolec1_mirror<<<ceil_div(nrec, 256), 256, 0, st>>>(sval, nrec, nself, spos, l->d_mirror);
-    // FP64 totals
-    double *dsum = nullptr;
-    HIP_CHECK(ctx, hipMallocAsync((void **)&dsum, 2 * sizeof(double), st));
-    HIP_CHECK(ctx, hipMemsetAsync(dsum, 0, 2 * sizeof(double), st));
-    sum_f64<<<64, 256, 0, st>>>(pE, npair, dsum);
-    sum_f64<<<16, 256, 0, st>>>(eintra, nmol, dsum + 1);
-    double hs[2];
-    HIP_CHECK(ctx, hipMemcpyAsync(hs, dsum, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
-    l->h_cnt.resize(nmol);
-    l->h_rowstart.resize(nmol + 1);
-    HIP_CHECK(ctx, hipMemcpyAsync(l->h_cnt.data(), l->d_cnt, sizeof(int) * nmol, hipMemcpyDeviceToHost, st));
-    HIP_CHECK(ctx, hipMemcpyAsync(l->h_rowstart.data(), l->d_rowstart, sizeof(int) * nmol, hipMemcpyDeviceToHost, st));
-    HIP_CHECK(ctx, hipFreeAsync(dsum, st));
-    HIP_CHECK(ctx, hipFreeAsync(pim, st)); HIP_CHECK(ctx, hipFreeAsync(pjm, st)); HIP_CHECK(ctx, hipFreeAsync(pE, st));
-    HIP_CHECK(ctx, hipFreeAsync(key, st)); HIP_CHECK(ctx, hipFreeAsync(val, st));
-    HIP_CHECK(ctx, hipFreeAsync(skey, st)); HIP_CHECK(ctx, hipFreeAsync(sval, st)); HIP_CHECK(ctx, hipFreeAsync(spos, st));
+    // FP64 totals of the stored pair energies and the molecules' own energies
+    sum_f64_parts<<<dim3(gsum, 2), 256, 0, st>>>(l->d_ergpair + nself, npair, eintra, nmol, dpart);
+    sum_f64_final<<<2, 256, 0, st>>>(dpart, gsum, dout);
+    double *hs = (double *)hsc;
+    HIP_CHECK(ctx, hipMemcpyAsync(hs, dout, 2 * sizeof(double) + sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_CHECK(ctx, hipGetLastError());
     HIP_CHECK(ctx, hipStreamSynchronize(st));
-    l->h_rowstart[nmol] = nrec;
-    int maxcnt = 0;
-    for (int v : l->h_cnt) maxcnt = std::max(maxcnt, v);
+    const int maxcnt = *(const int *)(hs + 2);
     // set_ergpair caps every list at nnbrefmax; the cells bound a list by the molecules of
     // the neighbour cells, which is nnbrefmax, so the cap can never bind
     ARG_CHECK(ctx, maxcnt <= l->nnbrefmax, "epsr_fmolec1: a ledger list of %d exceeds nnbrefmax %d", maxcnt,
               l->nnbrefmax);
+    l->maxcnt = maxcnt;
     l->nself = nself;
     l->plan_ready = false;
     l->ahead_valid = false;
@@ -835,17 +922,22 @@ extern "C" int epsr_ledger_export(epsr_ctx *ctx, const epsr_ledger *l, int ldl, 
     ARG_CHECK(ctx, l && l->ledger_ready, "epsr_ledger_export: no ledger built");
     HIP_CHECK(ctx, hipSetDevice(ctx->device));
     const int nmol = l->nmol;
-    std::vector<int> partner(l->nslots), entry(l->nslots);
-    HIP_CHECK(ctx, hipMemcpy(partner.data(), l->d_partner, sizeof(int) * l->nslots, hipMemcpyDeviceToHost));
-    HIP_CHECK(ctx, hipMemcpy(entry.data(), l->d_entry, sizeof(int) * l->nslots, hipMemcpyDeviceToHost));
-    if (ergpair) HIP_CHECK(ctx, hipMemcpy(ergpair, l->d_ergpair, sizeof(float) * l->nentries, hipMemcpyDeviceToHost));
+    hipStream_t st = ctx->stream;
+    std::vector<int> partner(l->nslots), entry(l->nslots), rowstart(nmol + 1), cnt(nmol);
+    HIP_CHECK(ctx, hipMemcpyAsync(partner.data(), l->d_partner, sizeof(int) * l->nslots, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(ctx, hipMemcpyAsync(entry.data(), l->d_entry, sizeof(int) * l->nslots, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(ctx, hipMemcpyAsync(rowstart.data(), l->d_rowstart, sizeof(int) * (nmol + 1), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(ctx, hipMemcpyAsync(cnt.data(), l->d_cnt, sizeof(int) * nmol, hipMemcpyDeviceToHost, st));
+    if (ergpair)
+        HIP_CHECK(ctx, hipMemcpyAsync(ergpair, l->d_ergpair, sizeof(float) * l->nentries, hipMemcpyDeviceToHost, st));
+    HIP_CHECK(ctx, hipStreamSynchronize(st));
     for (int m = 0; m < nmol; ++m) {
-        int n = l->h_cnt[m];
+        int n = cnt[m];
         ARG_CHECK(ctx, n <= ldl, "epsr_ledger_export: ldl %d < list length %d", ldl, n);
         if (nergpairm) nergpairm[m] = n;
         for (int s = 0; s < n; ++s) {
-            if (molergpair) molergpair[(size_t)m * ldl + s] = partner[l->h_rowstart[m] + s];
-            if (ergpairref) ergpairref[(size_t)m * ldl + s] = entry[l->h_rowstart[m] + s];
+            if (molergpair) molergpair[(size_t)m * ldl + s] = partner[rowstart[m] + s];
+            if (ergpairref) ergpairref[(size_t)m * ldl + s] = entry[rowstart[m] + s];
         }
     }
     if (order) std::copy(l->order.begin(), l->order.end(), order);
@@ -1674,21 +1766,38 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
         }
         l->plan_total = cur;
         int rc0;
-        if ((rc0 = dev_alloc(ctx, &l->d_roff, nmol)) || (rc0 = dev_alloc(ctx, &l->d_toff, nmol)) ||
-            (rc0 = dev_alloc(ctx, &l->d_moff, nmol)))
+        if (!l->d_roff && ((rc0 = dev_alloc(ctx, &l->d_roff, nmol)) || (rc0 = dev_alloc(ctx, &l->d_toff, nmol)) ||
+                           (rc0 = dev_alloc(ctx, &l->d_moff, nmol))))
             return rc0;
-        HIP_CHECK(ctx, hipMemcpy(l->d_roff, l->h_roff.data(), sizeof(int) * nmol, hipMemcpyHostToDevice));
-        HIP_CHECK(ctx, hipMemcpy(l->d_toff, l->h_toff.data(), sizeof(int) * nmol, hipMemcpyHostToDevice));
-        HIP_CHECK(ctx, hipMemcpy(l->d_moff, l->h_moff.data(), sizeof(int) * nmol, hipMemcpyHostToDevice));
+        // stream-ordered uploads: the host vectors stay unchanged until the next plan,
+        // which follows a synchronisation
+        HIP_CHECK(ctx, hipMemcpyAsync(l->d_roff, l->h_roff.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, st));
+        HIP_CHECK(ctx, hipMemcpyAsync(l->d_toff, l->h_toff.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, st));
+        HIP_CHECK(ctx, hipMemcpyAsync(l->d_moff, l->h_moff.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, st));
         l->plan_flags = flags;
         l->plan_ready = true;
         l->ahead_valid = false;
     }
     // step sizes of every type for this cycle: the first move of a type fixes a zero
     // step (molfit.py:375-392), and every move of the type then uses it
+    // pinned host block of the cycle's small transfers: the device result block
+    // (16 + 2 nmt ints), the no-change flag, the step sizes per type (2 nmt floats).  The
+    // previous call ended with a synchronisation, so nothing reads it now.
+    const size_t hio_need = sizeof(int) * (20 + 4 * (size_t)nmt);
+    if (l->hio_cap < hio_need) {
+        if (l->hio) HIP_CHECK(ctx, hipHostFree(l->hio));
+        l->hio = nullptr;
+        l->hio_cap = 0;
+        HIP_CHECK(ctx, hipHostMalloc((void **)&l->hio, hio_need));
+        l->hio_cap = hio_need;
+    }
+    int *hblk = l->hio;
+    std::fill(hblk, hblk + 20 + 4 * nmt, 0);
+    int *hpc = hblk, *rej = hblk + 16, *hflag = hblk + 16 + 2 * nmt;
+    const float *hres = (const float *)(hblk + 12);
+    float *wtype = (float *)(hblk + 20 + 2 * nmt);
     std::vector<int> att(l->att);
     std::vector<float> wr(wholerot, wholerot + nmt), wt(wholetrans, wholetrans + nmt);
-    std::vector<float> wtype(2 * nmt, 0.0f);
     for (int mt = 0; mt < nmt; ++mt) {
         if (l->type_nat[mt] == 0) continue;   // no molecule of this type
         if (p->stepr > 0.0f && l->type_nat[mt] > 1) {
@@ -1733,8 +1842,7 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     // device buffers.  The parallel resolution keeps the terms of an exact sum (the
     // longest ledger list) in LDS for each of its waves.
     const int nslots = l->nslots;
-    int maxc = 1;
-    for (int v : l->h_cnt) maxc = std::max(maxc, v);
+    const int maxc = std::max(1, l->maxcnt);
     const size_t lds_pass = sizeof(float2) * 4 * (size_t)maxc;
     const bool fast = lds_pass <= 160 * 1024;
     Scratch S, q;
@@ -1746,12 +1854,12 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
         x.take<float>(natom); x.take<float>(natom); x.take<float>(natom);
         x.take<float>(nmol); x.take<float>(nmol); x.take<float>(nmol);
         x.take<float>(nslots); x.take<float>(nslots); x.take<float>(nslots);
-        x.take<unsigned>((nmol + 31) / 32); x.take<float>(nmol); x.take<float>(4); x.take<int>(2 * nmt + 2);
+        x.take<unsigned>((nmol + 31) / 32); x.take<float>(nmol);
         x.take<float>(nslots);
         x.take<float2>(fast ? nmol : 0);
-        x.take<float>(da ? 2 * (size_t)nmol : 0); x.take<float>(da ? nmol : 0); x.take<int>(4);
+        x.take<float>(da ? 2 * (size_t)nmol : 0); x.take<float>(da ? nmol : 0);
         x.take<uint16_t>(want_x && !(rec && rec->x) ? 2 * (size_t)nmol * ldx : 0);
-        x.take<unsigned char>(nmol); x.take<int>(nmol); x.take<int>(nmol); x.take<int>(8);
+        x.take<unsigned char>(nmol); x.take<int>(nmol); x.take<int>(nmol); x.take<int>(16 + 2 * nmt);
     };
     layout(q);
     S.base = (char *)ctx_scratch(ctx, q.off);
@@ -1766,33 +1874,31 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     float *dA = S.take<float>(nslots), *dB = S.take<float>(nslots), *dL = S.take<float>(nslots);
     unsigned *dacc = S.take<unsigned>((nmol + 31) / 32);
     float *dfdiff = S.take<float>(nmol);
-    float *dres = S.take<float>(4);
-    int *drej = S.take<int>(2 * nmt + 2);
     float *ddelta = S.take<float>(nslots);
     float2 *dthr = S.take<float2>(fast ? nmol : 0);
     float *dy = S.take<float>(da ? 2 * (size_t)nmol : 0);
     float *dshift = da ? S.take<float>(nmol) : (S.take<float>(0), nullptr);
-    int *dnrej = S.take<int>(4);
     uint16_t *dX = S.take<uint16_t>(want_x && !(rec && rec->x) ? 2 * (size_t)nmol * ldx : 0);
     unsigned char *dstate = S.take<unsigned char>(nmol);
     int *dW0 = S.take<int>(nmol), *dW1 = S.take<int>(nmol);
-    int *dpc = S.take<int>(8);   // list counts of passes 0..2, exact sums, passes of the finisher
+    // the cycle's results, copied to the host in one transfer: [0..7] list counts of
+    // passes 0..2, exact sums, passes of the finisher; [8] stage-1 rejections; [12..15]
+    // the float results of mc_fadd; [16..] rejections per type
+    int *dblk = S.take<int>(16 + 2 * nmt);
+    int *dpc = dblk, *dnrej = dblk + 8, *drej = dblk + 16;
+    float *dres = (float *)(dblk + 12);
     ARG_CHECK(ctx, !da || fast, "epsr_molfit_da: delayed acceptance needs the parallel resolution (LDS too small)");
 
     PreparedTables pt;
     int rc = prepare_tables(ctx, box, tb, dT, dtmp, pt);
     if (rc) return rc;
     HIP_CHECK(ctx, hipMemcpyAsync(dstream, l->pin[use], sizeof(float) * total, hipMemcpyHostToDevice, st));
-    HIP_CHECK(ctx, hipMemcpyAsync(dwtype, wtype.data(), sizeof(float) * 2 * nmt, hipMemcpyHostToDevice, st));
+    HIP_CHECK(ctx, hipMemcpyAsync(dwtype, wtype, sizeof(float) * 2 * nmt, hipMemcpyHostToDevice, st));
     HIP_CHECK(ctx, hipMemsetAsync(dtmp + 1, 0, 3 * sizeof(int), st));
     mc_trial<<<ceil_div(nmol, 128), 128, 0, st>>>(nmol, l->d_order, box->d_naf, box->d_nal, c->xa, c->ya, c->za, c->xadd,
                                                   c->yadd, c->zadd, dstream, l->d_roff, l->d_toff, nullptr, nullptr,
                                                   box->d_mtype, dwtype, txa, tya, tza, txadd, tyadd, tzadd, dtmp + 1);
 
-    float hres[2];
-    int hnrej = 0, hflag = 0;
-    int hpc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    std::vector<int> rej(2 * nmt);
     int nexact = nmol, npasses = 0;
     // B..D of the cycle from trial positions in txa..tzadd and the stream at stream; the
     // commits are skipped when *redo is set
@@ -1856,7 +1962,6 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
                 nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner, l->d_mirror, dA, dB, dL, dacc,
                 maxc, dfdiff);
             mc_fadd<<<1, 1024, 0, st>>>(nmol, dacc, dfdiff, dres);
-            HIP_CHECK(ctx, hipMemcpyAsync(hpc, dpc, 8 * sizeof(int), hipMemcpyDeviceToHost, st));
         } else {
             size_t lds = sizeof(unsigned) * ((nmol + 31) / 32) + sizeof(float) * 2 * (size_t)maxc + 64;
             ARG_CHECK(ctx, lds <= 160 * 1024, "epsr_molfit: %d molecules x %d ledger slots exceed LDS", nmol, maxc);
@@ -1873,14 +1978,12 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
                                                             dacc, roff, toff, txa, tya, tza, txadd, tyadd, tzadd, c->xa,
                                                             c->ya, c->za, c->xadd, c->yadd, c->zadd, nstepreject,
                                                             p->stepr > 0.0f ? 1 : 0, redo, drej);
-        HIP_CHECK(ctx, hipMemcpyAsync(&hnrej, dnrej, sizeof(int), hipMemcpyDeviceToHost, st));
-        HIP_CHECK(ctx, hipMemcpyAsync(hres, dres, sizeof(float) * 2, hipMemcpyDeviceToHost, st));
-        HIP_CHECK(ctx, hipMemcpyAsync(rej.data(), drej, sizeof(int) * 2 * nmt, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(ctx, hipMemcpyAsync(hblk, dblk, sizeof(int) * (16 + 2 * (size_t)nmt), hipMemcpyDeviceToHost, st));
         HIP_CHECK(ctx, hipGetLastError());
         return EPSR_OK;
     };
     if ((rc = run(dstream, l->d_moff, l->d_roff, l->d_toff, dtmp + 1))) return rc;
-    HIP_CHECK(ctx, hipMemcpyAsync(&hflag, dtmp + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(ctx, hipMemcpyAsync(hflag, dtmp + 1, sizeof(int), hipMemcpyDeviceToHost, st));
     // while the GPU runs the cycle: the next cycle's stream, from this one's end state
     const Ran1 rend = rng;
     {
@@ -1893,16 +1996,17 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
         l->pin_ahead = nxt;
     }
     HIP_CHECK(ctx, hipStreamSynchronize(st));
-    l->ahead_valid = hflag == 0;
+    l->ahead_valid = *hflag == 0;
     int retries = 0;
-    if (hflag) {
+    if (*hflag) {
         // movem doubles its step until the centre moves (molfit.py:747-758); the doubled
         // step persists for the molecule type.  Nothing was committed: resolve the chain on
         // the host from the current centres, then run the cycle again with those offsets.
         std::vector<float> cx(nmol), cy(nmol), cz(nmol);
-        HIP_CHECK(ctx, hipMemcpy(cx.data(), c->xadd, sizeof(float) * nmol, hipMemcpyDeviceToHost));
-        HIP_CHECK(ctx, hipMemcpy(cy.data(), c->yadd, sizeof(float) * nmol, hipMemcpyDeviceToHost));
-        HIP_CHECK(ctx, hipMemcpy(cz.data(), c->zadd, sizeof(float) * nmol, hipMemcpyDeviceToHost));
+        HIP_CHECK(ctx, hipMemcpyAsync(cx.data(), c->xadd, sizeof(float) * nmol, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(ctx, hipMemcpyAsync(cy.data(), c->yadd, sizeof(float) * nmol, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(ctx, hipMemcpyAsync(cz.data(), c->zadd, sizeof(float) * nmol, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(ctx, hipStreamSynchronize(st));
         rng = rng0;
         std::vector<float> hstream;
         auto draw = [&](int idx) {
@@ -2010,7 +2114,7 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
         stats->retries = retries;
         stats->ms_pairs = ctx->timing ? epsr_ctx_last_kernel_ms(ctx, "mc_pair") : -1.0;
         stats->ms_resolve = ctx->timing ? epsr_ctx_last_kernel_ms(ctx, "mc_resolve") : -1.0;
-        stats->stage1_rejected = hnrej;
+        stats->stage1_rejected = hblk[8];
     }
     return EPSR_OK;
 }

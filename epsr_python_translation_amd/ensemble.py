@@ -6,8 +6,8 @@ side on the GPU while every box keeps the exact per-box semantics of
 :class:`pipeline.Refinement` (the MC chain, atogen and the transforms give the same bits
 as when the box runs alone).  Launch shapes come from the autotuner, once per box size
 class.  The transform stage of a worker's boxes (gensq and stog, no host reads) is
-captured into one HIP graph on its first step and replayed afterwards: one launch per
-worker instead of four per box.  The MC cycles and atogen read results back to the host
+captured into one HIP graph when the ensemble is built and replayed by every step: one
+launch per worker instead of four per box.  The MC cycles and atogen read results back to the host
 (acceptance counters, origin selection with ran1), so they run as eager calls.
 """
 
@@ -78,6 +78,17 @@ class Ensemble:
             torch.cuda.synchronize()
         # MC cycles the tuner ran on each box (its size class's representative)
         self.tune_sweeps = [getattr(R, "tune_sweeps", 0) for R in self.boxes]
+        # the transform graphs: an eager pass sizes every graph context's scratch, then the
+        # workers' passes are captured one at a time, with no other work in flight (a call
+        # that synchronises the device from another thread would invalidate a capture)
+        self._parallel(self._transforms)
+        for wk in self.workers:
+            wk.gctx.check(wk.gctx.lib.epsr_ctx_synchronize(wk.gctx.h))
+            with torch.cuda.stream(wk.stream):
+                with D.Graph.capture(wk.gctx) as g:
+                    self._transforms(wk)
+            wk.graph = g
+        torch.cuda.synchronize()
 
     def _ledger(self, R):
         f64, fb64 = R.cells_and_ledger()
@@ -120,12 +131,6 @@ class Ensemble:
                     acc += R.mc.stats.accepted
                     mov += R.mc.stats.moves
                 R.atogen(ngrsamples=ngrsamples)
-            if wk.graph is None:
-                self._transforms(wk)              # sizes the graph context's scratch
-                wk.gctx.check(wk.gctx.lib.epsr_ctx_synchronize(wk.gctx.h))
-                with D.Graph.capture(wk.gctx) as g:
-                    self._transforms(wk)
-                wk.graph = g
             wk.graph.launch()
             stats[id(wk)] = (acc, mov)
         t = time.perf_counter()
