@@ -108,11 +108,9 @@ __global__ void abs_max_coord(const float4 *__restrict__ P, int n, unsigned *__r
     if (threadIdx.x == 0) atomicMax(out, __float_as_uint(s[0]));   // non-negative floats order as uints
 }
 
-__device__ __forceinline__ float mic_small(float x)
+void launch_abs_max_coord(const float4 *P, int n, unsigned *out, hipStream_t st)
 {
-    float a = x >= 0.5f ? 1.0f : 0.0f;
-    float b = x <= -0.5f ? 1.0f : 0.0f;
-    return x - (a - b);
+    abs_max_coord<<<64, 256, 0, st>>>(P, n, out);
 }
 
 template <bool POTEX>
@@ -140,7 +138,7 @@ atogen_inter_fast(const float4 *__restrict__ P, const int *__restrict__ amol, co
         srm[i] = rminex_abs[i];
     }
     for (int i = tid; i < ntype * ntype; i += ATOGEN_BLOCK) snp[i] = npairref_g[i];
-    const bool small = __uint_as_float(*maxcoord) <= 0.74f;
+    const bool small = __uint_as_float(*maxcoord) <= SMALL_COORD_LIMIT;
 
     const int oi = blockIdx.x * ATOGEN_BLOCK + tid;
     const bool active = oi < norigin;

@@ -34,6 +34,10 @@ struct epsr_ctx {
     size_t pinned_bytes = 0;
     // between epsr_graph_begin and epsr_graph_end
     bool capturing = false;
+    // a second stream (default priority) for work that can overlap the main stream's, and
+    // the events that fork it from and join it back to the main stream
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 };
 
 struct epsr_box {
@@ -50,7 +54,7 @@ struct epsr_box {
     int *d_npairref = nullptr, *d_amol = nullptr;
     // launch shapes (epsr_box_set_launch): workgroups atogen's pair kernel aims for, and
     // the workgroup size of the pair-energy kernels
-    int atogen_workgroups = 1024, pair_block = 512;
+    int atogen_workgroups = 1024, pair_block = 1024;
 };
 
 // ------------------------------------------------------------------ ledger
@@ -68,6 +72,10 @@ struct epsr_ledger {
     int nself = 0, npairent = 0, nslots = 0, nentries = 0;
     int *d_rowstart = nullptr, *d_cnt = nullptr, *d_order = nullptr, *d_pos = nullptr;
     int *d_partner = nullptr, *d_entry = nullptr, *d_mirror = nullptr, *d_slotmol = nullptr;
+    // per slot: partner's chain position, mirror unless the partner's first slot (else -1),
+    // and the packed record of the MC pair kernel
+    int *d_pn = nullptr, *d_mw = nullptr;
+    int4 *d_srec = nullptr;
     float *d_ergpair = nullptr;
     size_t cap_slots = 0, cap_entries = 0;
     // the MC plan of the chain order: ran1 offsets of every move's rotation, translation
@@ -89,6 +97,18 @@ struct epsr_ledger {
     int32_t ahead_from[35], ahead_to[35];
     bool ledger_ready = false;
     int maxcnt = 0;   // the longest list
+    // table extents measured for the arrays in tab_ptr (xstar, xavs, xavs1, ecoul, ecoul1)
+    bool tab_ok = false;
+    const void *tab_ptr[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    int tab_ld = 0, tab_npt2 = 0, tab_last = 0, tab_lastnear = 0;
+    // their packing (valid when tab_packed)
+    float4 *d_tab = nullptr;
+    size_t cap_tab = 0;
+    bool tab_packed = false;
+    // slots (not the first of a list) whose partner comes earlier in the chain order
+    int *d_bvslot = nullptr;
+    int nbv = 0;
+    size_t cap_bv = 0;
     // neighbour offsets were formed for this lattice size and rmax
     int nbr_na = 0;
     float nbr_rmax = -1.0f;
@@ -126,6 +146,8 @@ int set_error(epsr_ctx *ctx, int code, const char *fmt, ...);
 bool ctx_capturing(epsr_ctx *ctx);
 // synchronise the context's stream unless it is being captured
 hipError_t ctx_sync(epsr_ctx *ctx);
+// the context's side stream and its fork/join events, created on first use
+int ctx_side(epsr_ctx *ctx, hipStream_t *out);
 
 // scratch management (device); returns nullptr and sets the error on failure
 void *ctx_scratch(epsr_ctx *ctx, size_t bytes);
@@ -197,6 +219,22 @@ __device__ __forceinline__ float dev_anint(float x)
 
 // minimagec, cubic branch (src/epsr26/library/minimagec.py:48-62)
 __device__ __forceinline__ float dev_mic(float x) { return x - dev_anint(x); }
+
+// dev_mic for |x| < 1.5, where NINT(x) is -1, 0 or 1: the same integer is subtracted, so
+// the result is the same float
+__device__ __forceinline__ float mic_small(float x)
+{
+    float a = x >= 0.5f ? 1.0f : 0.0f;
+    float b = x <= -0.5f ? 1.0f : 0.0f;
+    return x - (a - b);
+}
+
+// coordinates of every atom within [-0.74, 0.74] (so differences lie in (-1.5, 1.5)),
+// from the maximum |coordinate| that launch_abs_max_coord leaves as float bits
+#define SMALL_COORD_LIMIT 0.74f
+
+// *out = max(*out, max |x|, |y|, |z| of P[0..n)) as float bits; *out starts at 0
+void launch_abs_max_coord(const float4 *P, int n, unsigned *out, hipStream_t st);
 
 // IFIX (fmath.py:252-260) for arguments known to lie in [0, 2^31)
 __device__ __forceinline__ int dev_ifix_pos(float x) { return (int)truncf(x); }

@@ -69,6 +69,11 @@ extern "C" void epsr_ledger_destroy(epsr_ledger *l)
     if (l->hstage) hipHostFree(l->hstage);
     if (l->hsc) hipHostFree(l->hsc);
     hipFree(l->d_cells);
+    hipFree(l->d_bvslot);
+    hipFree(l->d_pn);
+    hipFree(l->d_mw);
+    hipFree(l->d_srec);
+    hipFree(l->d_tab);
     for (void *w : l->wbuf) hipFree(w);
     delete l;
 }
@@ -258,40 +263,87 @@ static void cell_nbref(const epsr_ledger *l, int cc, std::vector<int> &out)
 }
 
 // ------------------------------------------------------------------ pair energy
-// Packed tables: T[ic*nte + iz] = (xavs, xavs1, ecoul, ecoul1)(iz, ic) for iz < nte;
-// beyond nte every table entry is +0, where the reference's lookup also gives +0.
+// Packed tables: bins [0, nnear) as T[ic*nnear + iz] = (xavs, xavs1, ecoul, ecoul1)(iz, ic);
+// bins [nnear, nte), where xavs and xavs1 are +0 in every column, as
+// F[ic*(nte - nnear) + iz - nnear] = (ecoul, ecoul1)(iz, ic); beyond nte every table entry
+// is +0, where the reference's lookup also gives +0.  The split keeps the tables of a
+// long Coulomb tail small enough for two workgroups' copies per CU in LDS.
 struct PairTab {
     const float4 *T;
+    const float2 *F;
     const float *xs;
     const int *npr;
-    int nte, npt2, ntype;
-    float stepcl;
+    int nte, nnear, npt2, ntype;
+    float stepcl, inv_stepcl;   // inv_stepcl = 1/stepcl rounded to float
 };
 
-// en_atom_inter_scale (molfit.py:1250-1279) with scalefac = 1
+// IEEE float square root.  For 2^-96 <= x < inf: the hardware estimate s (within 1 ulp)
+// or a neighbour, chosen by the signs of the exact residuals x - s_dn*s and x - s_up*s
+// (the same selection the compiler's correctly rounded sqrtf makes, without its scaling
+// of tiny arguments and its special-value fix-up, which that range does not need);
+// other arguments take sqrtf itself.
+__device__ __forceinline__ float sqrt_rn(float x)
+{
+    if (!(x >= 0x1p-96f && x < INFINITY)) return sqrtf(x);
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sdn = __int_as_float(__float_as_int(s) - 1), sup = __int_as_float(__float_as_int(s) + 1);
+    const float rdn = fmaf(-sdn, s, x), rup = fmaf(-sup, s, x);
+    float r = rdn <= 0.0f ? sdn : s;
+    return rup > 0.0f ? sup : r;
+}
+
+// en_atom_inter_scale (molfit.py:1250-1279) with scalefac = 1.  SMALL: every coordinate
+// lies within +-SMALL_COORD_LIMIT (mic_small gives dev_mic's result).  The bin is
+// trunc(dtest / stepcl) + 2 with the correctly rounded quotient Q: q = dtest * inv_stepcl
+// lies within 1.2e-7 q of dtest / stepcl (two roundings of 2^-24) and Q within 6e-8 Q of
+// it, so when q's fraction keeps clear of 0 and 1 by 1.2e-6 q, trunc(q) = trunc(Q);
+// otherwise (about 2.4e-6 q of the pairs) the division is made.
+template <bool SMALL>
 __device__ __forceinline__ float atom_pair(float x1, float y1, float z1, float x2, float y2, float z2, int ic,
                                            const PairTab &t)
 {
-    float xd = dev_mic(x2 - x1), yd = dev_mic(y2 - y1), zd = dev_mic(z2 - z1);
+    float xd, yd, zd;
+    if (SMALL) {
+        xd = mic_small(x2 - x1);
+        yd = mic_small(y2 - y1);
+        zd = mic_small(z2 - z1);
+    } else {
+        xd = dev_mic(x2 - x1);
+        yd = dev_mic(y2 - y1);
+        zd = dev_mic(z2 - z1);
+    }
     float dsq = xd * xd + yd * yd + zd * zd;
-    float dtest = sqrtf(dsq);
-    float rat = dtest / t.stepcl;
+    float dtest = sqrt_rn(dsq);
+    const float q = dtest * t.inv_stepcl;
+    const float tq = truncf(q), fq = q - tq, mq = 1.2e-6f * q;
+    int iz;
+    if (fq > mq && fq < 1.0f - mq && q < 4194304.0f) {
+        iz = (int)tq + 2;
+    } else {
+        const float rat = dtest / t.stepcl;
+        if (!(rat < 2.0e9f)) return 0.0f;
+        iz = (int)truncf(rat) + 2;
+    }
     float sume = 0.0f;
-    if (rat < 2.0e9f) {
-        int iz = (int)truncf(rat) + 2;
-        if (iz <= t.npt2 && iz < t.nte) {
-            float4 v = t.T[ic * t.nte + iz];
-            float dtdif = dtest - t.xs[iz];
-            sume = v.x + v.y * dtdif;
-            sume = sume + v.z + v.w * dtdif;
+    if (iz <= t.npt2 && iz < t.nte) {
+        float4 v;
+        if (iz < t.nnear) {
+            v = t.T[ic * t.nnear + iz];
+        } else {
+            const float2 f = t.F[ic * (t.nte - t.nnear) + (iz - t.nnear)];
+            v = make_float4(0.0f, 0.0f, f.x, f.y);   // the stored +0 of xavs, xavs1
         }
+        float dtdif = dtest - t.xs[iz];
+        sume = v.x + v.y * dtdif;
+        sume = sume + v.z + v.w * dtdif;
     }
     return sume;
 }
 
 // en_mole_pair (molfit.py:1527-1578): sequential sum, molecule 1 outer, molecule 2 inner
-__device__ __forceinline__ float mol_pair(const float4 *__restrict__ P1, int na1, const float4 *__restrict__ P2,
-                                          int nb2, const PairTab &t)
+template <bool SMALL>
+__device__ __forceinline__ float mol_pair_t(const float4 *__restrict__ P1, int na1, const float4 *__restrict__ P2,
+                                            int nb2, const PairTab &t)
 {
     float sume = 0.0f;
     for (int ia = 0; ia < na1; ++ia) {
@@ -299,23 +351,73 @@ __device__ __forceinline__ float mol_pair(const float4 *__restrict__ P1, int na1
         int it = __float_as_int(p.w) * t.ntype;
         for (int ja = 0; ja < nb2; ++ja) {
             float4 q = P2[ja];
-            sume = sume + atom_pair(p.x, p.y, p.z, q.x, q.y, q.z, t.npr[it + __float_as_int(q.w)], t);
+            sume = sume + atom_pair<SMALL>(p.x, p.y, p.z, q.x, q.y, q.z, t.npr[it + __float_as_int(q.w)], t);
         }
     }
     return sume;
 }
 
-__global__ void pack_tables(const float *__restrict__ xavs, const float *__restrict__ xavs1,
-                            const float *__restrict__ ecoul, const float *__restrict__ ecoul1, int ld, int npairs,
-                            int nte, float4 *__restrict__ T)
+// small: every coordinate of both molecules lies within +-SMALL_COORD_LIMIT
+__device__ __forceinline__ float mol_pair(const float4 *__restrict__ P1, int na1, const float4 *__restrict__ P2,
+                                          int nb2, const PairTab &t, bool small = false)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= npairs * nte) return;
-    int ic = t / nte, iz = t - ic * nte;
-    size_t o = (size_t)ic * ld + iz;
-    T[t] = make_float4(xavs[o], xavs1[o], ecoul[o], ecoul1[o]);
+    return small ? mol_pair_t<true>(P1, na1, P2, nb2, t) : mol_pair_t<false>(P1, na1, P2, nb2, t);
 }
 
+// the flag above from the float bits launch_abs_max_coord leaves (null: unknown)
+__device__ __forceinline__ bool coords_small(const unsigned *maxcoord)
+{
+    return maxcoord && __uint_as_float(*maxcoord) <= SMALL_COORD_LIMIT;
+}
+
+// mol_pair_t<true> for molecules of at most three atoms: the six positions are loaded
+// before any pair is evaluated; the sum runs in the same order
+__device__ __forceinline__ float mol_pair3(const float4 *__restrict__ P1, int n1, const float4 *__restrict__ P2,
+                                           int n2, const PairTab &t)
+{
+    float4 p[3], q[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        p[j] = j < n1 ? P1[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        q[j] = j < n2 ? P2[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    float sume = 0.0f;
+#pragma unroll
+    for (int ia = 0; ia < 3; ++ia) {
+        if (ia >= n1) break;
+        const int it = __float_as_int(p[ia].w) * t.ntype;
+#pragma unroll
+        for (int ja = 0; ja < 3; ++ja) {
+            if (ja >= n2) break;
+            sume = sume + atom_pair<true>(p[ia].x, p[ia].y, p[ia].z, q[ja].x, q[ja].y, q[ja].z,
+                                          t.npr[it + __float_as_int(q[ja].w)], t);
+        }
+    }
+    return sume;
+}
+
+// T (near bins) then F (far bins) in one buffer
+__global__ void pack_tables(const float *__restrict__ xavs, const float *__restrict__ xavs1,
+                            const float *__restrict__ ecoul, const float *__restrict__ ecoul1, int ld, int npairs,
+                            int nte, int nnear, float4 *__restrict__ T)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nfar = nte - nnear;
+    if (t < npairs * nnear) {
+        int ic = t / nnear, iz = t - ic * nnear;
+        size_t o = (size_t)ic * ld + iz;
+        T[t] = make_float4(xavs[o], xavs1[o], ecoul[o], ecoul1[o]);
+        return;
+    }
+    t -= npairs * nnear;
+    if (t >= npairs * nfar) return;
+    int ic = t / nfar, iz = nnear + (t - ic * nfar);
+    size_t o = (size_t)ic * ld + iz;
+    ((float2 *)(T + (size_t)npairs * nnear))[t] = make_float2(ecoul[o], ecoul1[o]);
+}
+
+// out[0]: the last bin (up to lim) where any of the four tables has a non-zero bit
+// pattern; out[1]: the same for xavs and xavs1
 __global__ void last_nonzero(const float *__restrict__ a, const float *__restrict__ b, const float *__restrict__ c,
                              const float *__restrict__ d, int ld, int npairs, int lim, int *out)
 {
@@ -323,7 +425,9 @@ __global__ void last_nonzero(const float *__restrict__ a, const float *__restric
     if (t >= npairs * ld) return;
     int iz = t % ld;
     if (iz > lim) return;
-    if (__float_as_int(a[t]) | __float_as_int(b[t]) | __float_as_int(c[t]) | __float_as_int(d[t])) atomicMax(out, iz);
+    const int ab = __float_as_int(a[t]) | __float_as_int(b[t]);
+    if (ab) atomicMax(out + 1, iz);
+    if (ab | __float_as_int(c[t]) | __float_as_int(d[t])) atomicMax(out, iz);
 }
 
 // absolute atom positions (x = xa + xadd, one fp32 add) with the type in .w
@@ -338,6 +442,39 @@ __global__ void abs_positions(const float *__restrict__ xa, const float *__restr
     P[a] = make_float4(xa[a] + xadd[m], ya[a] + yadd[m], za[a] + zadd[m], __int_as_float(itype[a]));
 }
 
+// abs_positions of the current (Pold) and trial (Pnew) coordinates in one launch, and the
+// largest |coordinate| of both into *maxcoord (float bits; starts at 0)
+__global__ void __launch_bounds__(256)
+abs_positions2(int natom, const float *__restrict__ xa, const float *__restrict__ ya, const float *__restrict__ za,
+               const float *__restrict__ xadd, const float *__restrict__ yadd, const float *__restrict__ zadd,
+               const float *__restrict__ txa, const float *__restrict__ tya, const float *__restrict__ tza,
+               const float *__restrict__ txadd, const float *__restrict__ tyadd, const float *__restrict__ tzadd,
+               const int *__restrict__ amol, const int *__restrict__ itype, float4 *__restrict__ Pold,
+               float4 *__restrict__ Pnew, unsigned *__restrict__ maxcoord)
+{
+    __shared__ float sm[4];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    float mx = 0.0f;
+    if (t < 2 * natom) {
+        const bool trial = t >= natom;
+        const int a = trial ? t - natom : t;
+        const int m = amol[a];
+        const float4 p = trial ? make_float4(txa[a] + txadd[m], tya[a] + tyadd[m], tza[a] + tzadd[m],
+                                             __int_as_float(itype[a]))
+                               : make_float4(xa[a] + xadd[m], ya[a] + yadd[m], za[a] + zadd[m],
+                                             __int_as_float(itype[a]));
+        (trial ? Pnew : Pold)[a] = p;
+        mx = fmaxf(fabsf(p.x), fmaxf(fabsf(p.y), fabsf(p.z)));
+    }
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_down(mx, off));
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        mx = fmaxf(fmaxf(sm[0], sm[1]), fmaxf(sm[2], sm[3]));
+        atomicMax(maxcoord, __float_as_uint(mx));   // non-negative floats order as their bits
+    }
+}
+
 // Tables prepared for a call: packed float4 rows up to the last non-zero bin.
 struct PreparedTables {
     PairTab tab;
@@ -345,63 +482,163 @@ struct PreparedTables {
     bool in_lds;
 };
 
-// the last bin (up to npt2) where any of the four tables is non-zero, into *dlast
+// the last bin (up to npt2) where any of the four tables is non-zero, into dlast[0], and
+// where xavs or xavs1 is, into dlast[1]
 static int tables_extent(epsr_ctx *ctx, const epsr_box *box, const epsr_tables_t *tb, int *dlast)
 {
     hipStream_t st = ctx->stream;
     const int np = box->npairs, ld = tb->ld;
     const int lim = std::min(tb->npt2, ld - 1);
-    HIP_CHECK(ctx, hipMemsetAsync(dlast, 0, sizeof(int), st));
+    HIP_CHECK(ctx, hipMemsetAsync(dlast, 0, 2 * sizeof(int), st));
     last_nonzero<<<ceil_div((long)np * ld, 256), 256, 0, st>>>(tb->xavs, tb->xavs1, tb->ecoul, tb->ecoul1, ld, np, lim,
                                                                dlast);
     return EPSR_OK;
 }
 
-// pack the tables up to bin `last` into dT
-static int tables_pack(epsr_ctx *ctx, const epsr_box *box, const epsr_tables_t *tb, int last, float4 *dT,
-                       PreparedTables &pt)
+// the description of tables packed in dT up to bin `last`, four-wide up to bin `lastnear`
+static void tables_describe(const epsr_box *box, const epsr_tables_t *tb, int last, int lastnear, float4 *dT,
+                            PreparedTables &pt)
 {
-    hipStream_t st = ctx->stream;
-    const int np = box->npairs, ld = tb->ld;
+    const int np = box->npairs;
     const int nte = std::max(2, last + 1);
-    pack_tables<<<ceil_div((long)np * nte, 256), 256, 0, st>>>(tb->xavs, tb->xavs1, tb->ecoul, tb->ecoul1, ld, np, nte,
-                                                               dT);
+    const int nnear = std::min(nte, std::max(2, lastnear + 1));
     pt.tab.T = dT;
+    pt.tab.F = (const float2 *)(dT + (size_t)np * nnear);
+    pt.tab.nnear = nnear;
     pt.tab.xs = tb->xstar;
     pt.tab.npr = box->d_npairref;
     pt.tab.nte = nte;
     pt.tab.npt2 = tb->npt2;
     pt.tab.ntype = box->ntype;
     pt.tab.stepcl = box->cellst / box->dmax;
-    pt.lds_bytes = sizeof(float4) * (size_t)np * nte + sizeof(float) * nte + sizeof(int) * box->ntype * box->ntype + 64;
+    pt.tab.inv_stepcl = 1.0f / pt.tab.stepcl;
+    pt.lds_bytes = sizeof(float4) * (size_t)np * nnear + sizeof(float2) * (size_t)np * (nte - nnear) +
+                   sizeof(float) * nte + sizeof(int) * box->ntype * box->ntype + 64;
     pt.in_lds = pt.lds_bytes <= 150 * 1024;
+}
+
+// pack the tables up to bin `last` into dT, four-wide up to bin `lastnear`
+static int tables_pack(epsr_ctx *ctx, const epsr_box *box, const epsr_tables_t *tb, int last, int lastnear, float4 *dT,
+                       PreparedTables &pt)
+{
+    tables_describe(box, tb, last, lastnear, dT, pt);
+    const int np = box->npairs;
+    const long items = (long)np * pt.tab.nte;
+    pack_tables<<<ceil_div(items, 256), 256, 0, ctx->stream>>>(tb->xavs, tb->xavs1, tb->ecoul, tb->ecoul1, tb->ld, np,
+                                                              pt.tab.nte, pt.tab.nnear, dT);
     return EPSR_OK;
 }
 
+// dscratch_int: two ints of device memory; the extents go to extents_out if given
 static int prepare_tables(epsr_ctx *ctx, const epsr_box *box, const epsr_tables_t *tb, float4 *dT, int *dscratch_int,
-                          PreparedTables &pt)
+                          PreparedTables &pt, int *extents_out = nullptr)
 {
     int rc = tables_extent(ctx, box, tb, dscratch_int);
     if (rc) return rc;
-    int last = 0;
-    HIP_CHECK(ctx, hipMemcpyAsync(&last, dscratch_int, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    int last[2] = {0, 0};
+    HIP_CHECK(ctx, hipMemcpyAsync(last, dscratch_int, 2 * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    return tables_pack(ctx, box, tb, last, dT, pt);
+    if (extents_out) {
+        extents_out[0] = last[0];
+        extents_out[1] = last[1];
+    }
+    return tables_pack(ctx, box, tb, last[0], last[1], dT, pt);
+}
+
+// the table extents measured for the arrays of tb (by fmolec1 or a cycle), kept in the
+// ledger so that MC cycles on the same arrays need not wait for them
+static void tab_remember(epsr_ledger *l, const epsr_tables_t *tb, int last, int lastnear)
+{
+    l->tab_ptr[0] = tb->xstar;
+    l->tab_ptr[1] = tb->xavs;
+    l->tab_ptr[2] = tb->xavs1;
+    l->tab_ptr[3] = tb->ecoul;
+    l->tab_ptr[4] = tb->ecoul1;
+    l->tab_ld = tb->ld;
+    l->tab_npt2 = tb->npt2;
+    l->tab_last = last;
+    l->tab_lastnear = lastnear;
+    l->tab_ok = true;
+    l->tab_packed = false;
+}
+
+// the ledger's buffer for packed tables, large enough for tb
+static int ledger_tables(epsr_ctx *ctx, const epsr_box *box, const epsr_tables_t *tb, epsr_ledger *l)
+{
+    const size_t need = (size_t)box->npairs * std::max(2, std::min(tb->ld, tb->npt2 + 1));
+    l->tab_packed = false;
+    if (l->cap_tab < need) {
+        HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        HIP_CHECK(ctx, hipFree(l->d_tab));
+        l->d_tab = nullptr;
+        l->cap_tab = 0;
+        HIP_CHECK(ctx, hipMalloc((void **)&l->d_tab, sizeof(float4) * need));
+        l->cap_tab = need;
+    }
+    return EPSR_OK;
+}
+
+static bool tab_known(const epsr_ledger *l, const epsr_tables_t *tb)
+{
+    return l->tab_ok && l->tab_ptr[0] == tb->xstar && l->tab_ptr[1] == tb->xavs && l->tab_ptr[2] == tb->xavs1 &&
+           l->tab_ptr[3] == tb->ecoul && l->tab_ptr[4] == tb->ecoul1 && l->tab_ld == tb->ld &&
+           l->tab_npt2 == tb->npt2;
+}
+
+// bit 30 of the cycle's no-change count (a count of moves, far below 2^30) marks table
+// extents measured on the device beyond the ones the cycle packed
+#define TABLES_STALE (1 << 30)
+
+// a packing of the tables to verify against the arrays it came from
+struct TabCheck {
+    const float *a, *b, *c, *d;   // xavs, xavs1, ecoul, ecoul1, [np][ld]
+    const float4 *T;              // the packing (null: nothing to verify)
+    int ld, np, lim, nte, nnear;
+};
+
+// any bit of the arrays (bins up to lim) that the packing does not reproduce sets
+// TABLES_STALE in *flag: a changed value, or a non-zero value beyond the packed extents
+__device__ void tables_verify(const TabCheck &tc, int *flag)
+{
+    if (!tc.T) return;
+    const float2 *F = (const float2 *)(tc.T + (size_t)tc.np * tc.nnear);
+    const int nfar = tc.nte - tc.nnear;
+    bool bad = false;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < tc.np * tc.ld; t += gridDim.x * blockDim.x) {
+        const int ic = t / tc.ld, iz = t - ic * tc.ld;
+        if (iz > tc.lim) continue;
+        const int a = __float_as_int(tc.a[t]), b = __float_as_int(tc.b[t]);
+        const int c = __float_as_int(tc.c[t]), d = __float_as_int(tc.d[t]);
+        if (iz < tc.nnear) {
+            const float4 v = tc.T[ic * tc.nnear + iz];
+            bad |= a != __float_as_int(v.x) || b != __float_as_int(v.y) || c != __float_as_int(v.z) ||
+                   d != __float_as_int(v.w);
+        } else if (iz < tc.nte) {
+            const float2 f = F[ic * nfar + (iz - tc.nnear)];
+            bad |= (a | b) != 0 || c != __float_as_int(f.x) || d != __float_as_int(f.y);
+        } else {
+            bad |= (a | b | c | d) != 0;
+        }
+    }
+    if (bad) atomicOr(flag, TABLES_STALE);
 }
 
 // copy the tables into LDS; returns a PairTab pointing at LDS
 __device__ PairTab tables_to_lds(const PairTab &g, int npairs, char *smem)
 {
     float4 *T = (float4 *)smem;
-    int nT = npairs * g.nte;
-    float *xs = (float *)(T + nT);
+    const int nT = npairs * g.nnear, nF = npairs * (g.nte - g.nnear);
+    float2 *F = (float2 *)(T + nT);
+    float *xs = (float *)(F + nF);
     int *npr = (int *)(xs + g.nte);
     for (int i = threadIdx.x; i < nT; i += blockDim.x) T[i] = g.T[i];
+    for (int i = threadIdx.x; i < nF; i += blockDim.x) F[i] = g.F[i];
     for (int i = threadIdx.x; i < g.nte; i += blockDim.x) xs[i] = g.xs[i];
     for (int i = threadIdx.x; i < g.ntype * g.ntype; i += blockDim.x) npr[i] = g.npr[i];
     __syncthreads();
     PairTab t = g;
     t.T = T;
+    t.F = F;
     t.xs = xs;
     t.npr = npr;
     return t;
@@ -418,10 +655,12 @@ __global__ void fmolec1_pairs(const long long *__restrict__ rowoff, int nmol, co
                               const int *__restrict__ mcell, const int *__restrict__ cstart,
                               const int *__restrict__ nbflat, const float4 *__restrict__ P,
                               const int *__restrict__ naf, const int *__restrict__ nal, PairTab gt, int npairs,
-                              float *__restrict__ rowE, int *__restrict__ rowJ, int *__restrict__ rowcnt)
+                              const unsigned *__restrict__ maxcoord, float *__restrict__ rowE, int *__restrict__ rowJ,
+                              int *__restrict__ rowcnt)
 {
     extern __shared__ char smem[];
     PairTab t = LDS ? tables_to_lds(gt, npairs, smem) : gt;
+    const bool small = coords_small(maxcoord);
     const int lane = threadIdx.x & 63;
     const int nw = gridDim.x * (blockDim.x >> 6);
     for (int k = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); k < nmol; k += nw) {
@@ -438,7 +677,7 @@ __global__ void fmolec1_pairs(const long long *__restrict__ rowoff, int nmol, co
             int jm = 0;
             if (inn < len) {
                 jm = nb[inn];
-                if (inn >= 1 && jm > im) e = mol_pair(P1, n1, P + naf[jm], nal[jm] - naf[jm] + 1, t);
+                if (inn >= 1 && jm > im) e = mol_pair(P1, n1, P + naf[jm], nal[jm] - naf[jm] + 1, t, small);
             }
             const bool keep = e != 0.0f;
             const unsigned long long mask = __ballot(keep);
@@ -551,6 +790,46 @@ __global__ void fmolec1_mirror(const int *__restrict__ sval, int nrec, int nself
     if (i >= nrec) return;
     int r = sval[i];
     mirror[i] = r < nself ? -1 : sortedpos[nself + ((r - nself) ^ 1)];
+}
+
+// slots (past the first of their list) whose partner comes earlier in the chain: the MC
+// cycle evaluates their pair at both new positions
+__global__ void fmolec1_bv_flags(int nrec, const int *__restrict__ slotmol, const int *__restrict__ partner,
+                                 const int *__restrict__ rowstart, const int *__restrict__ pos, int *__restrict__ flag)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > nrec) return;
+    if (i == nrec) {
+        flag[i] = 0;
+        return;
+    }
+    const int m = slotmol[i];
+    flag[i] = i != rowstart[m] && pos[partner[i]] < pos[m];
+}
+
+__global__ void fmolec1_bv_list(int nrec, const int *__restrict__ flag, const int *__restrict__ idx,
+                                int *__restrict__ bvslot, int *__restrict__ nbv)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nrec && flag[i]) bvslot[idx[i]] = i;
+    if (i == nrec) *nbv = idx[nrec];
+}
+
+// per slot, what the MC cycle's kernels would otherwise gather: the partner's chain
+// position (pn), the mirror slot unless it is the partner's first slot (mw, else -1), and
+// srec = (molecule | first-slot bit 31, partner, first atom | atom count << 24 of both)
+__global__ void fmolec1_slot_aux(int nrec, const int *__restrict__ slotmol, const int *__restrict__ partner,
+                                 const int *__restrict__ mirror, const int *__restrict__ rowstart,
+                                 const int *__restrict__ pos, const int *__restrict__ naf, const int *__restrict__ nal,
+                                 int *__restrict__ pn, int *__restrict__ mw, int4 *__restrict__ srec)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nrec) return;
+    const int m = slotmol[i], n = partner[i], mi = mirror[i];
+    pn[i] = pos[n];
+    mw[i] = (mi >= 0 && mi != rowstart[n]) ? mi : -1;
+    srec[i] = make_int4(m | (i == rowstart[m] ? (int)0x80000000u : 0), n, naf[m] | ((nal[m] - naf[m] + 1) << 24),
+                        naf[n] | ((nal[n] - naf[n] + 1) << 24));
 }
 
 __global__ void max_int(const int *__restrict__ a, int n, int *__restrict__ out)
@@ -726,7 +1005,7 @@ static int ledger_scan(epsr_ctx *ctx, epsr_ledger *l, const T *in, T *out, int n
 }
 
 // the host-side scalars of the list set-up: candidate items (clamped to INT_MAX), flat
-// list length, longest list, last non-zero table bin
+// list length, longest list, last non-zero table bins (all four tables, xavs/xavs1)
 __global__ void fmolec1_report(const long long *__restrict__ total, const int *__restrict__ nflat,
                                const int *__restrict__ tail, int *__restrict__ rep)
 {
@@ -735,6 +1014,7 @@ __global__ void fmolec1_report(const long long *__restrict__ total, const int *_
     rep[1] = *nflat;
     rep[2] = tail[0];
     rep[3] = tail[1];
+    rep[4] = tail[2];
 }
 
 __global__ void copy_two(const int *__restrict__ a, const int *__restrict__ b, int *__restrict__ out)
@@ -754,6 +1034,10 @@ extern "C" int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coord
     hipStream_t st = ctx->stream;
     const int nmol = box->nmol;
     int rc;
+    // the slot records pack a first atom in 24 bits and an atom count in 8
+    ARG_CHECK(ctx, box->natom < (1 << 24) && box->max_atoms_per_mol < 256,
+              "epsr_fmolec1: %d atoms (limit 2^24) or %d atoms in a molecule (limit 255)", box->natom,
+              box->max_atoms_per_mol);
     if (!l->hsc) HIP_CHECK(ctx, hipHostMalloc((void **)&l->hsc, 64));
     int *hsc = l->hsc;
     // neighbour lists on the GPU from the cell arrays cells_init uploaded: per cell the
@@ -771,7 +1055,8 @@ extern "C" int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coord
     const int *dmcell = dmic + (size_t)ncell * l->maxcellno;
     int *dnbcell = cw, *dnbcnt = dnbcell + ncn, *dnbpre = dnbcnt + ncn;
     int *drowlen = dnbpre + (size_t)ncell * (nbr + 1), *dflatlen = drowlen + ncell + 1, *dcstart = dflatlen + ncell + 1;
-    int *dtail = dcstart + ncell + 1;   // [0] longest list, [1] last table bin, [4..7] report
+    // [0] longest list, [1..2] table extents, [3] largest |coordinate|, [4..8] report, [10..11] entry counts
+    int *dtail = dcstart + ncell + 1;
     cell_neighbours<<<(unsigned)ceil_div(ncn, 256), 256, 0, st>>>(ncell, nbr, l->na, l->nb, l->nc, doff, doff + nbr,
                                                                   doff + 2 * nbr, dnmc, dnbcell, dnbcnt);
     cell_prefix<<<ncell, 256, 0, st>>>(nbr, dnbcnt, dnmc, dnbpre, drowlen, dflatlen);
@@ -783,17 +1068,17 @@ extern "C" int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coord
     if ((rc = ledger_scan<long long>(ctx, l, dlen, drow, nmol + 1))) return rc;
     if ((rc = tables_extent(ctx, box, tb, dtail + 1))) return rc;
     fmolec1_report<<<1, 1, 0, st>>>(drow + nmol, dcstart + ncell, dtail, dtail + 4);
-    HIP_CHECK(ctx, hipMemcpyAsync(hsc, dtail + 4, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(ctx, hipMemcpyAsync(hsc, dtail + 4, 5 * sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_CHECK(ctx, hipStreamSynchronize(st));
     const long long total = hsc[0];
-    const int nflat = hsc[1], last = hsc[3];
+    const int nflat = hsc[1], last = hsc[3], lastnear = hsc[4];
     l->nnbrefmax = hsc[2];
+    tab_remember(l, tb, last, lastnear);
     ARG_CHECK(ctx, total < 0x7fffffffLL, "epsr_fmolec1: candidate pairs exceed the 2^31 item limit");
 
     Scratch S;
     auto layout = [&](Scratch &x) {
         x.take<float4>(box->natom);
-        x.take<float4>((size_t)box->npairs * std::max(2, std::min(tb->ld, tb->npt2 + 1)));
         x.take<int>((size_t)nflat + 1);
         x.take<float>(total); x.take<int>(total);
         x.take<int>(2 * (size_t)nmol + 1); x.take<int>(2 * (size_t)nmol + 1);
@@ -805,7 +1090,6 @@ extern "C" int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coord
         if (!S.base) return EPSR_ERR_HIP;
     }
     float4 *dP = S.take<float4>(box->natom);
-    float4 *dT = S.take<float4>((size_t)box->npairs * std::max(2, std::min(tb->ld, tb->npt2 + 1)));
     int *dnb = S.take<int>((size_t)nflat + 1);
     float *rowE = S.take<float>(total);
     int *rowJ = S.take<int>(total);
@@ -814,28 +1098,33 @@ extern "C" int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coord
     cell_flatten<<<(unsigned)ceil_div(ncn, 256), 256, 0, st>>>(ncell, nbr, l->maxcellno, dnbcell, dnbcnt, dnbpre,
                                                                dcstart, dnmc, dmic, dnb);
     PreparedTables pt;
-    if ((rc = tables_pack(ctx, box, tb, last, dT, pt))) return rc;
+    if ((rc = ledger_tables(ctx, box, tb, l))) return rc;
+    if ((rc = tables_pack(ctx, box, tb, last, lastnear, l->d_tab, pt))) return rc;
+    l->tab_packed = true;
     abs_positions<<<ceil_div(box->natom, 256), 256, 0, st>>>(c->xa, c->ya, c->za, c->xadd, c->yadd, c->zadd,
                                                              box->d_amol, box->d_itype, dP, box->natom);
+    HIP_CHECK(ctx, hipMemsetAsync(dtail + 3, 0, sizeof(int), st));
+    launch_abs_max_coord(dP, box->natom, (unsigned *)(dtail + 3), st);
     {
         const int B = box->pair_block;
         timer_begin(ctx, "fmolec1_pair");
         if (pt.in_lds) {
             int g = persistent_blocks(pt.lds_bytes, (long long)nmol * 64, B);
             fmolec1_pairs<true><<<g, B, pt.lds_bytes, st>>>(drow, nmol, l->d_order, dmcell, dcstart, dnb, dP,
-                                                            box->d_naf, box->d_nal, pt.tab, box->npairs, rowE, rowJ,
-                                                            cat + nmol);
+                                                            box->d_naf, box->d_nal, pt.tab, box->npairs,
+                                                            (const unsigned *)(dtail + 3), rowE, rowJ, cat + nmol);
         } else {
             int g = persistent_blocks(0, (long long)nmol * 64, B);
             fmolec1_pairs<false><<<g, B, 0, st>>>(drow, nmol, l->d_order, dmcell, dcstart, dnb, dP, box->d_naf,
-                                                  box->d_nal, pt.tab, box->npairs, rowE, rowJ, cat + nmol);
+                                                  box->d_nal, pt.tab, box->npairs, (const unsigned *)(dtail + 3), rowE,
+                                                  rowJ, cat + nmol);
         }
         timer_end(ctx, "fmolec1_pair");
     }
     fmolec1_self_flags<<<ceil_div(nmol + 1, 256), 256, 0, st>>>(eintra, nmol, cat);
     if ((rc = ledger_scan<int>(ctx, l, cat, ent, 2 * nmol + 1))) return rc;
-    copy_two<<<1, 1, 0, st>>>(ent + nmol, ent + 2 * nmol, dtail + 8);
-    HIP_CHECK(ctx, hipMemcpyAsync(hsc, dtail + 8, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    copy_two<<<1, 1, 0, st>>>(ent + nmol, ent + 2 * nmol, dtail + 10);
+    HIP_CHECK(ctx, hipMemcpyAsync(hsc, dtail + 10, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_CHECK(ctx, hipStreamSynchronize(st));
     const int nself = hsc[0];
     const int nent = hsc[1];
@@ -847,7 +1136,9 @@ extern "C" int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coord
     if (l->cap_slots < (size_t)nrec + 1) {
         const size_t cap = (size_t)nrec + nrec / 8 + 64;
         if ((rc = dev_alloc(ctx, &l->d_partner, cap)) || (rc = dev_alloc(ctx, &l->d_entry, cap)) ||
-            (rc = dev_alloc(ctx, &l->d_mirror, cap)) || (rc = dev_alloc(ctx, &l->d_slotmol, cap)))
+            (rc = dev_alloc(ctx, &l->d_mirror, cap)) || (rc = dev_alloc(ctx, &l->d_slotmol, cap)) ||
+            (rc = dev_alloc(ctx, &l->d_pn, cap)) || (rc = dev_alloc(ctx, &l->d_mw, cap)) ||
+            (rc = dev_alloc(ctx, &l->d_srec, cap)))
             return rc;
         l->cap_slots = cap;
     }
@@ -876,24 +1167,38 @@ extern "C" int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coord
     const int gsum = 256;
     double *dpart = nullptr;
     if ((rc = ledger_work(ctx, l, LW_SUM, sizeof(double) * (2 * gsum + 4), (void **)&dpart))) return rc;
-    double *dout = dpart + 2 * gsum;   // [0] pairs, [1] self energies, then the longest list (int)
-    int *dmaxcnt = (int *)(dout + 2);
-    HIP_CHECK(ctx, hipMemsetAsync(dmaxcnt, 0, sizeof(int), st));
+    double *dout = dpart + 2 * gsum;   // [0] pairs, [1] self energies, then (int) the longest list, nbv
+    int *dmaxcnt = (int *)(dout + 2), *dnbv = dmaxcnt + 1;
+    HIP_CHECK(ctx, hipMemsetAsync(dmaxcnt, 0, 2 * sizeof(int), st));
     fmolec1_rowstart<<<ceil_div(nrec + 1, 256), 256, 0, st>>>(skey, nrec, nmol, l->d_rowstart);
     fmolec1_counts<<<ceil_div(nmol, 256), 256, 0, st>>>(l->d_rowstart, nmol, l->d_cnt, dmaxcnt);
     if (nrec > 0) {
         fmolec1_rows<<<ceil_div(nrec, 256), 256, 0, st>>>(skey, sval, nrec, nself, key, l->d_partner, l->d_entry,
                                                           l->d_slotmol, spos);
         fmolec1_mirror<<<ceil_div(nrec, 256), 256, 0, st>>>(sval, nrec, nself, spos, l->d_mirror);
+        if (l->cap_bv < (size_t)nrec) {
+            const size_t cap = (size_t)nrec + nrec / 8 + 64;
+            if ((rc = dev_alloc(ctx, &l->d_bvslot, cap))) return rc;
+            l->cap_bv = cap;
+        }
+        // key and val are free again: the flags and their scan
+        fmolec1_bv_flags<<<ceil_div(nrec + 1, 256), 256, 0, st>>>(nrec, l->d_slotmol, l->d_partner, l->d_rowstart,
+                                                                  l->d_pos, key);
+        if ((rc = ledger_scan<int>(ctx, l, key, val, nrec + 1))) return rc;
+        fmolec1_bv_list<<<ceil_div(nrec + 1, 256), 256, 0, st>>>(nrec, key, val, l->d_bvslot, dnbv);
+        fmolec1_slot_aux<<<ceil_div(nrec, 256), 256, 0, st>>>(nrec, l->d_slotmol, l->d_partner, l->d_mirror,
+                                                              l->d_rowstart, l->d_pos, box->d_naf, box->d_nal,
+                                                              l->d_pn, l->d_mw, l->d_srec);
     }
     // FP64 totals of the stored pair energies and the molecules' own energies
     sum_f64_parts<<<dim3(gsum, 2), 256, 0, st>>>(l->d_ergpair + nself, npair, eintra, nmol, dpart);
     sum_f64_final<<<2, 256, 0, st>>>(dpart, gsum, dout);
     double *hs = (double *)hsc;
-    HIP_CHECK(ctx, hipMemcpyAsync(hs, dout, 2 * sizeof(double) + sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(ctx, hipMemcpyAsync(hs, dout, 2 * sizeof(double) + 2 * sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_CHECK(ctx, hipGetLastError());
     HIP_CHECK(ctx, hipStreamSynchronize(st));
-    const int maxcnt = *(const int *)(hs + 2);
+    const int maxcnt = ((const int *)(hs + 2))[0];
+    l->nbv = ((const int *)(hs + 2))[1];
     // set_ergpair caps every list at nnbrefmax; the cells bound a list by the molecules of
     // the neighbour cells, which is nnbrefmax, so the cap can never bind
     ARG_CHECK(ctx, maxcnt <= l->nnbrefmax, "epsr_fmolec1: a ledger list of %d exceeds nnbrefmax %d", maxcnt,
@@ -1059,8 +1364,13 @@ __global__ void mc_trial(int nmol, const int *__restrict__ order, const int *__r
                          const int *__restrict__ toff, const float *__restrict__ rstep, const float *__restrict__ tstep,
                          const int *__restrict__ mtype, const float *__restrict__ wtype, float *__restrict__ txa,
                          float *__restrict__ tya, float *__restrict__ tza, float *__restrict__ txadd,
-                         float *__restrict__ tyadd, float *__restrict__ tzadd, int *__restrict__ nochange)
+                         float *__restrict__ tyadd, float *__restrict__ tzadd, int *__restrict__ nochange,
+                         TabCheck tc, int *__restrict__ zero, int nzero)
 {
+    // the cycle's set-up rides along: verify the packed tables, clear the counters in zero
+    tables_verify(tc, nochange);
+    if (blockIdx.x == 0)
+        for (int i = threadIdx.x; i < nzero; i += blockDim.x) zero[i] = 0;
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nmol) return;
     const float pi = 3.14159265358979323846f;
@@ -1110,25 +1420,33 @@ __global__ void mc_trial(int nmol, const int *__restrict__ order, const int *__r
 // A = E(m_trial, n_current), Bv = E(m_trial, n_trial) when n moves before m, and the
 // ledger value at the start of the cycle.
 template <bool LDS>
-__global__ void mc_pairs(int nslots, const int *__restrict__ slotmol, const int *__restrict__ rowstart,
-                         const int *__restrict__ partner, const int *__restrict__ entry, const int *__restrict__ pos,
-                         const float *__restrict__ ergpair, const float4 *__restrict__ Pold,
-                         const float4 *__restrict__ Pnew, const int *__restrict__ naf, const int *__restrict__ nal,
-                         PairTab gt, int npairs, const float *__restrict__ shift, float *__restrict__ A,
-                         float *__restrict__ Bv, float *__restrict__ L)
+__global__ void mc_pairs(int nslots, int nbv, const int *__restrict__ bvslot, const int4 *__restrict__ srec,
+                         const int *__restrict__ entry, const int *__restrict__ pos, const float *__restrict__ ergpair,
+                         const float4 *__restrict__ Pold, const float4 *__restrict__ Pnew, PairTab gt, int npairs,
+                         const unsigned *__restrict__ maxcoord, const float *__restrict__ shift,
+                         float *__restrict__ A, float *__restrict__ Bv, float *__restrict__ L)
 {
     extern __shared__ char smem[];
     PairTab t = LDS ? tables_to_lds(gt, npairs, smem) : gt;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nslots; i += gridDim.x * blockDim.x) {
-        int m = slotmol[i];
-        L[i] = ergpair[entry[i]];
-        if (i == rowstart[m]) continue;
-        if (shift && isnan(shift[pos[m]])) continue;   // rejected by the surrogate stage
-        int n = partner[i];
-        const float4 *P1 = Pnew + naf[m];
-        int n1 = nal[m] - naf[m] + 1, n2 = nal[n] - naf[n] + 1;
-        A[i] = mol_pair(P1, n1, Pold + naf[n], n2, t);
-        if (pos[n] < pos[m]) Bv[i] = mol_pair(P1, n1, Pnew + naf[n], n2, t);
+    const bool small = coords_small(maxcoord);
+    // items [0, nslots): A (and L) of every slot; items [nslots, nslots + nbv): Bv of the
+    // slots whose partner moves earlier in the chain (bvslot, from the ledger), so every
+    // lane of a wave evaluates one molecule pair.  srec[i] holds the slot's molecule (bit
+    // 31: first slot of its list), partner and both molecules' first atom and atom count.
+    for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < nslots + nbv; it += gridDim.x * blockDim.x) {
+        const bool bv = it >= nslots;
+        const int i = bv ? bvslot[it - nslots] : it;
+        const int4 r = srec[i];
+        if (!bv) {
+            L[i] = ergpair[entry[i]];
+            if (r.x < 0) continue;
+        }
+        if (shift && isnan(shift[pos[r.x & 0x7fffffff]])) continue;   // rejected by the surrogate stage
+        const float4 *P1 = Pnew + (r.z & 0xffffff), *P2 = (bv ? Pnew : Pold) + (r.w & 0xffffff);
+        const int n1 = (int)((unsigned)r.z >> 24), n2 = (int)((unsigned)r.w >> 24);
+        const float e = (small && n1 <= 3 && n2 <= 3) ? mol_pair3(P1, n1, P2, n2, t) : mol_pair(P1, n1, P2, n2, t, small);
+        if (bv) Bv[i] = e;
+        else A[i] = e;
     }
 }
 
@@ -1326,22 +1644,28 @@ __device__ void mc_thresholds(double b, double bnd, double tempd, double nlr, fl
 // one wave per molecule: the deltas of its slots (delta, the change of its energy change
 // when that earlier partner is accepted) and the decision thresholds of its move (thr, by
 // chain position)
-__global__ void __launch_bounds__(64)
+#define PREP_CH 4
+__global__ void __launch_bounds__(256)
 mc_prepare(int nmol, const int *__restrict__ pos, const int *__restrict__ rowstart, const int *__restrict__ cnt,
-           const int *__restrict__ partner, const int *__restrict__ mirror, const float *__restrict__ A,
+           const int *__restrict__ pnv, const int *__restrict__ mw, const float *__restrict__ A,
            const float *__restrict__ Bv, const float *__restrict__ L, const float *__restrict__ stream,
            const int *__restrict__ moff, float temp, const float *__restrict__ shift, float *__restrict__ delta,
-           float2 *__restrict__ thr)
+           float2 *__restrict__ thr, unsigned char *__restrict__ state, int *__restrict__ counters)
 {
     // Each rounding of the sequential sum errs by at most u |partial sum|.  With
     // P_s = |sum_{s'<=s} e0| + sum_{s'<=s, earlier partner} |delta| the partial sums obey
     // |S_2s| <= P_s and |S_2s-1| <= P_s-1 + max|ergnew_s| whatever the earlier decisions,
-    // so W = sum_s (P_s-1 + max|ergnew_s| + P_s) bounds sum_j |S_j|.  The slots are
-    // scanned in order, 64 at a time.
-    const int m = blockIdx.x;
-    const int lane = threadIdx.x;
+    // so W = sum_s (P_s-1 + max|ergnew_s| + P_s) bounds sum_j |S_j|.  One wave per
+    // molecule; the slots are scanned in order, 64 at a time, with the loads of PREP_CH
+    // chunks issued together.
+    const int m = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    // the resolver's counters and every move's state start at zero
+    if (blockIdx.x == 0 && threadIdx.x < 8) counters[threadIdx.x] = 0;
+    if (m >= nmol) return;
     const int row = rowstart[m], c = cnt[m];
     const int pm = pos[m];
+    if (lane == 0) state[pm] = 0;
     const float sh = shift ? shift[pm] : 0.0f;
     if (isnan(sh)) {
         // rejected by the surrogate stage: never accepted, receives no pushes
@@ -1350,45 +1674,55 @@ mc_prepare(int nmol, const int *__restrict__ pos, const int *__restrict__ rowsta
     }
     double carry_e = 0.0, carry_d = 0.0, pprev = 0.0, w = 0.0;
     int npb = 0;
-    for (int s0 = 1; s0 < c; s0 += 64) {
-        const int s = s0 + lane;
-        double e0 = 0.0, ad = 0.0;
-        float maxnew = 0.0f;
-        if (s < c) {
-            int i = row + s;
-            int n = partner[i];
-            bool before = pos[n] < pm;
-            float a = A[i], l = L[i];
-            e0 = (double)a - (double)l;
-            maxnew = fabsf(a);
-            float dl = 0.0f;
-            if (before) {
-                int mi = mirror[i];
-                float lw = mi != rowstart[n] ? A[mi] : l;
-                float bv = Bv[i];
-                dl = (float)(((double)bv - (double)lw) - e0);
-                maxnew = fmaxf(maxnew, fabsf(bv));
-                ad = fabs((double)dl);
-                npb++;
-            }
-            delta[i] = dl;
-        }
-        double se = e0, sd = ad;
-        for (int off = 1; off < 64; off <<= 1) {
-            double te = __shfl_up(se, off), td = __shfl_up(sd, off);
-            if (lane >= off) {
-                se += te;
-                sd += td;
+    for (int g0 = 1; g0 < c; g0 += 64 * PREP_CH) {
+        double e0[PREP_CH], ad[PREP_CH];
+        float maxnew[PREP_CH];
+#pragma unroll
+        for (int j = 0; j < PREP_CH; ++j) {
+            const int s = g0 + 64 * j + lane;
+            e0[j] = 0.0;
+            ad[j] = 0.0;
+            maxnew[j] = 0.0f;
+            if (s < c) {
+                int i = row + s;
+                bool before = pnv[i] < pm;
+                float a = A[i], l = L[i];
+                e0[j] = (double)a - (double)l;
+                maxnew[j] = fabsf(a);
+                float dl = 0.0f;
+                if (before) {
+                    int mi = mw[i];
+                    float lw = mi >= 0 ? A[mi] : l;
+                    float bv = Bv[i];
+                    dl = (float)(((double)bv - (double)lw) - e0[j]);
+                    maxnew[j] = fmaxf(maxnew[j], fabsf(bv));
+                    ad[j] = fabs((double)dl);
+                    npb++;
+                }
+                delta[i] = dl;
             }
         }
-        double pe = carry_e + se, pd = carry_d + sd;
-        double ps = fabs(pe) + pd;
-        double pm1 = __shfl_up(ps, 1);
-        if (lane == 0) pm1 = pprev;
-        if (s < c) w += pm1 + (double)maxnew + ps;
-        carry_e = __shfl(pe, 63);
-        carry_d = __shfl(pd, 63);
-        pprev = __shfl(ps, 63);
+#pragma unroll
+        for (int j = 0; j < PREP_CH; ++j) {
+            if (g0 + 64 * j >= c) break;   // wave-uniform
+            const int s = g0 + 64 * j + lane;
+            double se = e0[j], sd = ad[j];
+            for (int off = 1; off < 64; off <<= 1) {
+                double te = __shfl_up(se, off), td = __shfl_up(sd, off);
+                if (lane >= off) {
+                    se += te;
+                    sd += td;
+                }
+            }
+            double pe = carry_e + se, pd = carry_d + sd;
+            double ps = fabs(pe) + pd;
+            double pm1 = __shfl_up(ps, 1);
+            if (lane == 0) pm1 = pprev;
+            if (s < c) w += pm1 + (double)maxnew[j] + ps;
+            carry_e = __shfl(pe, 63);
+            carry_d = __shfl(pd, 63);
+            pprev = __shfl(ps, 63);
+        }
     }
     for (int off = 32; off > 0; off >>= 1) {
         w += __shfl_down(w, off);
@@ -1407,9 +1741,9 @@ mc_prepare(int nmol, const int *__restrict__ pos, const int *__restrict__ rowsta
 // exact fdiff of every accepted move (for fadd): one wave per chain position gathers
 // the selected variant of every slot into LDS, lane 0 sums them in the reference's order
 __global__ void __launch_bounds__(256)
-mc_fdiff_accepted(int nmol, const int *__restrict__ order, const int *__restrict__ pos,
-                  const int *__restrict__ rowstart, const int *__restrict__ cnt, const int *__restrict__ partner,
-                  const int *__restrict__ mirror, const float *__restrict__ A, const float *__restrict__ Bv,
+mc_fdiff_accepted(int nmol, const int *__restrict__ order, const int *__restrict__ rowstart,
+                  const int *__restrict__ cnt, const int *__restrict__ pnv, const int *__restrict__ mw,
+                  const float *__restrict__ A, const float *__restrict__ Bv,
                   const float *__restrict__ L, const unsigned *__restrict__ bits, int stride,
                   float *__restrict__ fdiff_out)
 {
@@ -1423,11 +1757,10 @@ mc_fdiff_accepted(int nmol, const int *__restrict__ order, const int *__restrict
     float2 *t = fterm + (size_t)wv * stride;
     for (int s = 1 + lane; s < c; s += 64) {
         int i = row + s;
-        int n = partner[i];
-        int pn = pos[n];
+        int pn = pnv[i];
         bool a = pn < k && ((bits[pn >> 5] >> (pn & 31)) & 1u);
-        int mi = mirror[i];
-        t[s] = make_float2(a ? Bv[i] : A[i], (a && mi != rowstart[n]) ? A[mi] : L[i]);
+        int mi = mw[i];
+        t[s] = make_float2(a ? Bv[i] : A[i], (a && mi >= 0) ? A[mi] : L[i]);
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
@@ -1444,24 +1777,33 @@ mc_fdiff_accepted(int nmol, const int *__restrict__ order, const int *__restrict
 
 // fadd = fadd + fdiff over the accepted moves in chain order (molfit.py:405), and the
 // number of accepted moves.  One workgroup: each segment of the chain is compacted into
-// LDS (accepted values in chain order) by all threads, then thread 0 adds them in order.
+// LDS (accepted values in chain order) by all threads, their loads issued together, then
+// thread 0 adds them in order, 64 per batch of LDS reads.
 #define FADD_SEG 16384
 
 __global__ void __launch_bounds__(1024)
 mc_fadd(int nmol, const unsigned *__restrict__ bits, const float *__restrict__ fdiff, float *__restrict__ res)
 {
-    __shared__ float vals[FADD_SEG];
+    __shared__ float4 vals4[FADD_SEG / 4];
     __shared__ int wsum[16];
+    float *vals = (float *)vals4;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     float fadd = 0.0f;
     int nacc = 0;
     for (int s0 = 0; s0 < nmol; s0 += FADD_SEG) {
         const int s1 = min(nmol, s0 + FADD_SEG);
+        bool a[FADD_SEG / 1024];
+        float v[FADD_SEG / 1024];
+#pragma unroll
+        for (int j = 0; j < FADD_SEG / 1024; ++j) {
+            const int k = s0 + j * 1024 + (int)threadIdx.x;
+            a[j] = k < s1 && ((bits[k >> 5] >> (k & 31)) & 1u);
+            v[j] = a[j] ? fdiff[k] : 0.0f;
+        }
         int fill = 0;
-        for (int k0 = s0; k0 < s1; k0 += 1024) {
-            const int k = k0 + (int)threadIdx.x;
-            const bool a = k < s1 && ((bits[k >> 5] >> (k & 31)) & 1u);
-            const unsigned long long b = __ballot(a);
+#pragma unroll
+        for (int j = 0; j < FADD_SEG / 1024; ++j) {
+            const unsigned long long b = __ballot(a[j]);
             if (lane == 0) wsum[w] = __popcll(b);
             __syncthreads();
             int off = fill, tot = 0;
@@ -1469,12 +1811,26 @@ mc_fadd(int nmol, const unsigned *__restrict__ bits, const float *__restrict__ f
                 if (i < w) off += wsum[i];
                 tot += wsum[i];
             }
-            if (a) vals[off + __popcll(b & ((1ull << lane) - 1ull))] = fdiff[k];
+            if (a[j]) vals[off + __popcll(b & ((1ull << lane) - 1ull))] = v[j];
             fill += tot;
             __syncthreads();
         }
-        if (threadIdx.x == 0)
-            for (int i = 0; i < fill; ++i) fadd = fadd + vals[i];
+        if (threadIdx.x == 0) {
+            int i = 0;
+            for (; i + 64 <= fill; i += 64) {
+                float4 q[16];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) q[j] = vals4[(i >> 2) + j];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    fadd = fadd + q[j].x;
+                    fadd = fadd + q[j].y;
+                    fadd = fadd + q[j].z;
+                    fadd = fadd + q[j].w;
+                }
+            }
+            for (; i < fill; ++i) fadd = fadd + vals[i];
+        }
         nacc += fill;
         __syncthreads();
     }
@@ -1498,9 +1854,10 @@ mc_fadd(int nmol, const unsigned *__restrict__ bits, const float *__restrict__ f
 // state[k]: 0 undecided, 1 accepted, 2 rejected.
 
 // one wave decides move k if it can; returns the decision (0: still undecided)
-__device__ int mc_pass_move(int k, const int *__restrict__ order, const int *__restrict__ pos,
-                            const int *__restrict__ rowstart, const int *__restrict__ cnt,
-                            const int *__restrict__ partner, const int *__restrict__ mirror,
+// pnv[i]: chain position of slot i's partner; mw[i]: the mirror slot, -1 when it is the
+// partner's first slot (a slot that no move rewrites)
+__device__ int mc_pass_move(int k, const int *__restrict__ order, const int *__restrict__ rowstart,
+                            const int *__restrict__ cnt, const int *__restrict__ pnv, const int *__restrict__ mw,
                             const float *__restrict__ A, const float *__restrict__ Bv, const float *__restrict__ L,
                             const float *__restrict__ delta, const float2 *__restrict__ thr,
                             const float *__restrict__ stream, const int *__restrict__ moff, float temp,
@@ -1514,7 +1871,7 @@ __device__ int mc_pass_move(int k, const int *__restrict__ order, const int *__r
     int und = 0;
     for (int s = 1 + lane; s < c; s += 64) {
         const int i = row + s;
-        const int pn = pos[partner[i]];
+        const int pn = pnv[i];
         if (pn < k) {
             const int sp = state[pn];
             const double d = (double)delta[i];
@@ -1540,11 +1897,10 @@ __device__ int mc_pass_move(int k, const int *__restrict__ order, const int *__r
     // every earlier partner decided, thresholds ambiguous: the reference's sum
     for (int s = 1 + lane; s < c; s += 64) {
         const int i = row + s;
-        const int n = partner[i];
-        const int pn = pos[n];
+        const int pn = pnv[i];
         const bool acc = pn < k && state[pn] == 1;
-        const int mi = mirror[i];
-        term[s] = make_float2(acc ? Bv[i] : A[i], (acc && mi != rowstart[n]) ? A[mi] : L[i]);
+        const int mi = mw[i];
+        term[s] = make_float2(acc ? Bv[i] : A[i], (acc && mi >= 0) ? A[mi] : L[i]);
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -1564,9 +1920,9 @@ __device__ int mc_pass_move(int k, const int *__restrict__ order, const int *__r
 // one pass over a work list (the whole chain when first): decided moves write state,
 // the others go to the next list
 __global__ void __launch_bounds__(256)
-mc_resolve_pass(int nmol, const int *__restrict__ order, const int *__restrict__ pos,
-                const int *__restrict__ rowstart, const int *__restrict__ cnt, const int *__restrict__ partner,
-                const int *__restrict__ mirror, const float *__restrict__ A, const float *__restrict__ Bv,
+mc_resolve_pass(int nmol, const int *__restrict__ order, const int *__restrict__ rowstart,
+                const int *__restrict__ cnt, const int *__restrict__ pnv, const int *__restrict__ mw,
+                const float *__restrict__ A, const float *__restrict__ Bv,
                 const float *__restrict__ L, const float *__restrict__ delta, const float2 *__restrict__ thr,
                 const float *__restrict__ stream, const int *__restrict__ moff, float temp,
                 const float *__restrict__ shift, int maxc, int first, const int *__restrict__ win,
@@ -1579,8 +1935,8 @@ mc_resolve_pass(int nmol, const int *__restrict__ order, const int *__restrict__
     const int n = first ? nmol : *nin;
     for (int w = blockIdx.x * 4 + wv; w < n; w += gridDim.x * 4) {
         const int k = first ? w : win[w];
-        const int d = mc_pass_move(k, order, pos, rowstart, cnt, partner, mirror, A, Bv, L, delta, thr, stream,
-                                   moff, temp, shift, state, term, nexact);
+        const int d = mc_pass_move(k, order, rowstart, cnt, pnv, mw, A, Bv, L, delta, thr, stream, moff, temp,
+                                   shift, state, term, nexact);
         if (lane == 0) {
             if (d) state[k] = (unsigned char)d;
             else wout[atomicAdd(nout, 1)] = k;
@@ -1591,8 +1947,8 @@ mc_resolve_pass(int nmol, const int *__restrict__ order, const int *__restrict__
 // the remaining passes in one workgroup of 16 waves, with a barrier between passes,
 // until no move is undecided; lists ping-pong between wa and wb
 __global__ void __launch_bounds__(1024)
-mc_resolve_finish(const int *__restrict__ order, const int *__restrict__ pos, const int *__restrict__ rowstart,
-                  const int *__restrict__ cnt, const int *__restrict__ partner, const int *__restrict__ mirror,
+mc_resolve_finish(const int *__restrict__ order, const int *__restrict__ rowstart, const int *__restrict__ cnt,
+                  const int *__restrict__ pnv, const int *__restrict__ mw,
                   const float *__restrict__ A, const float *__restrict__ Bv, const float *__restrict__ L,
                   const float *__restrict__ delta, const float2 *__restrict__ thr, const float *__restrict__ stream,
                   const int *__restrict__ moff, float temp, const float *__restrict__ shift, int maxc,
@@ -1613,8 +1969,8 @@ mc_resolve_finish(const int *__restrict__ order, const int *__restrict__ pos, co
     while (ncur > 0) {
         for (int w = wv; w < ncur; w += 16) {
             const int k = cur[w];
-            const int d = mc_pass_move(k, order, pos, rowstart, cnt, partner, mirror, A, Bv, L, delta, thr,
-                                       stream, moff, temp, shift, state, term, nexact);
+            const int d = mc_pass_move(k, order, rowstart, cnt, pnv, mw, A, Bv, L, delta, thr, stream, moff, temp,
+                                       shift, state, term, nexact);
             if (lane == 0) {
                 if (d) state[k] = (unsigned char)d;
                 else nxt[atomicAdd(&nnext, 1)] = k;
@@ -1793,7 +2149,7 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     }
     int *hblk = l->hio;
     std::fill(hblk, hblk + 20 + 4 * nmt, 0);
-    int *hpc = hblk, *rej = hblk + 16, *hflag = hblk + 16 + 2 * nmt;
+    int *hpc = hblk, *rej = hblk + 16, *hflag = hblk + 9;
     const float *hres = (const float *)(hblk + 12);
     float *wtype = (float *)(hblk + 20 + 2 * nmt);
     std::vector<int> att(l->att);
@@ -1816,14 +2172,15 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     int total = l->plan_total;
     // the ran1 stream of the cycle: drawn ahead by the previous call when it starts from
     // this state, else drawn now; both in pinned memory
-    if (l->pin_cap < (size_t)total + 8) {
+    if (l->pin_cap < (size_t)total + 2 * nmt + 8) {
         for (int i = 0; i < 2; ++i) {
             if (l->pin[i]) HIP_CHECK(ctx, hipHostFree(l->pin[i]));
             l->pin[i] = nullptr;
         }
         l->pin_cap = 0;
-        for (int i = 0; i < 2; ++i) HIP_CHECK(ctx, hipHostMalloc((void **)&l->pin[i], sizeof(float) * (total + 8)));
-        l->pin_cap = (size_t)total + 8;
+        for (int i = 0; i < 2; ++i)
+            HIP_CHECK(ctx, hipHostMalloc((void **)&l->pin[i], sizeof(float) * (total + 2 * nmt + 8)));
+        l->pin_cap = (size_t)total + 2 * nmt + 8;
         l->ahead_valid = false;
     }
     Ran1 rng;
@@ -1848,9 +2205,8 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     Scratch S, q;
     auto layout = [&](Scratch &x) {
         x.take<float4>(natom); x.take<float4>(natom);
-        x.take<float4>((size_t)box->npairs * std::max(2, std::min(tb->ld, tb->npt2 + 1)));
-        x.take<int>(4);
-        x.take<float>(total + 8); x.take<float>(2 * nmt);
+        x.take<int>(8);
+        x.take<float>(total + 2 * nmt + 8);
         x.take<float>(natom); x.take<float>(natom); x.take<float>(natom);
         x.take<float>(nmol); x.take<float>(nmol); x.take<float>(nmol);
         x.take<float>(nslots); x.take<float>(nslots); x.take<float>(nslots);
@@ -1865,10 +2221,11 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     S.base = (char *)ctx_scratch(ctx, q.off);
     if (!S.base) return EPSR_ERR_HIP;
     float4 *Pold = S.take<float4>(natom), *Pnew = S.take<float4>(natom);
-    float4 *dT = S.take<float4>((size_t)box->npairs * std::max(2, std::min(tb->ld, tb->npt2 + 1)));
-    int *dtmp = S.take<int>(4);   // [0] table scan, [1] no-change flag, [2], [3] flags of a redone cycle
-    float *dstream = S.take<float>(total + 8);
-    float *dwtype = S.take<float>(2 * nmt);
+    // [2], [3] flags of a redone cycle, [4..5] table extents
+    int *dtmp = S.take<int>(8);
+    // the ran1 stream of the cycle, then the step sizes of every type (one upload)
+    float *dstream = S.take<float>(total + 2 * nmt + 8);
+    float *dwtype = dstream + total;
     float *txa = S.take<float>(natom), *tya = S.take<float>(natom), *tza = S.take<float>(natom);
     float *txadd = S.take<float>(nmol), *tyadd = S.take<float>(nmol), *tzadd = S.take<float>(nmol);
     float *dA = S.take<float>(nslots), *dB = S.take<float>(nslots), *dL = S.take<float>(nslots);
@@ -1881,34 +2238,52 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     uint16_t *dX = S.take<uint16_t>(want_x && !(rec && rec->x) ? 2 * (size_t)nmol * ldx : 0);
     unsigned char *dstate = S.take<unsigned char>(nmol);
     int *dW0 = S.take<int>(nmol), *dW1 = S.take<int>(nmol);
-    // the cycle's results, copied to the host in one transfer: [0..7] list counts of
-    // passes 0..2, exact sums, passes of the finisher; [8] stage-1 rejections; [12..15]
-    // the float results of mc_fadd; [16..] rejections per type
+    // the cycle's counters and results, copied to the host in one transfer: [0..7] list
+    // counts of passes 0..2, exact sums, passes of the finisher; [8] stage-1 rejections;
+    // [9] moves that did not change (| TABLES_STALE); [10] largest |coordinate| of the old
+    // and trial positions; [12..15] the float results of mc_fadd; [16..] rejections per type
     int *dblk = S.take<int>(16 + 2 * nmt);
-    int *dpc = dblk, *dnrej = dblk + 8, *drej = dblk + 16;
+    int *dpc = dblk, *dnrej = dblk + 8, *dnochange = dblk + 9, *drej = dblk + 16;
+    unsigned *dmaxc = (unsigned *)(dblk + 10);
     float *dres = (float *)(dblk + 12);
     ARG_CHECK(ctx, !da || fast, "epsr_molfit_da: delayed acceptance needs the parallel resolution (LDS too small)");
 
+    // the packed tables: the ledger keeps the packing of the arrays it last measured, and
+    // mc_trial verifies it against them (a cycle that finds them changed in place commits
+    // nothing and runs again with a fresh packing); else they are measured and packed now
     PreparedTables pt;
-    int rc = prepare_tables(ctx, box, tb, dT, dtmp, pt);
-    if (rc) return rc;
-    HIP_CHECK(ctx, hipMemcpyAsync(dstream, l->pin[use], sizeof(float) * total, hipMemcpyHostToDevice, st));
-    HIP_CHECK(ctx, hipMemcpyAsync(dwtype, wtype, sizeof(float) * 2 * nmt, hipMemcpyHostToDevice, st));
-    HIP_CHECK(ctx, hipMemsetAsync(dtmp + 1, 0, 3 * sizeof(int), st));
+    int rc;
+    TabCheck tc{tb->xavs, tb->xavs1, tb->ecoul, tb->ecoul1, nullptr, tb->ld, box->npairs,
+                std::min(tb->npt2, tb->ld - 1), 0, 0};
+    if (tab_known(l, tb) && l->tab_packed) {
+        tables_describe(box, tb, l->tab_last, l->tab_lastnear, l->d_tab, pt);
+        tc.T = l->d_tab;
+        tc.nte = pt.tab.nte;
+        tc.nnear = pt.tab.nnear;
+    } else {
+        if ((rc = ledger_tables(ctx, box, tb, l))) return rc;
+        int ext[2];
+        if ((rc = prepare_tables(ctx, box, tb, l->d_tab, dtmp + 4, pt, ext))) return rc;
+        tab_remember(l, tb, ext[0], ext[1]);
+        l->tab_packed = true;
+    }
+    std::memcpy(l->pin[use] + total, wtype, sizeof(float) * 2 * nmt);
+    HIP_CHECK(ctx, hipMemcpyAsync(dstream, l->pin[use], sizeof(float) * (total + 2 * nmt), hipMemcpyHostToDevice, st));
+    HIP_CHECK(ctx, hipMemsetAsync(dblk + 8, 0, 4 * sizeof(int), st));
     mc_trial<<<ceil_div(nmol, 128), 128, 0, st>>>(nmol, l->d_order, box->d_naf, box->d_nal, c->xa, c->ya, c->za, c->xadd,
                                                   c->yadd, c->zadd, dstream, l->d_roff, l->d_toff, nullptr, nullptr,
-                                                  box->d_mtype, dwtype, txa, tya, tza, txadd, tyadd, tzadd, dtmp + 1);
+                                                  box->d_mtype, dwtype, txa, tya, tza, txadd, tyadd, tzadd, dnochange,
+                                                  tc, drej, 2 * nmt);
 
     int nexact = nmol, npasses = 0;
     // B..D of the cycle from trial positions in txa..tzadd and the stream at stream; the
     // commits are skipped when *redo is set
     auto run = [&](const float *stream, const int *moff, const int *roff, const int *toff, const int *redo) -> int {
         int rc1;
-        abs_positions<<<ceil_div(natom, 256), 256, 0, st>>>(c->xa, c->ya, c->za, c->xadd, c->yadd, c->zadd,
-                                                            box->d_amol, box->d_itype, Pold, natom);
-        abs_positions<<<ceil_div(natom, 256), 256, 0, st>>>(txa, tya, tza, txadd, tyadd, tzadd, box->d_amol,
-                                                            box->d_itype, Pnew, natom);
-        HIP_CHECK(ctx, hipMemsetAsync(dnrej, 0, sizeof(int), st));
+        bool joined = false;
+        abs_positions2<<<ceil_div(2 * (long)natom, 256), 256, 0, st>>>(
+            natom, c->xa, c->ya, c->za, c->xadd, c->yadd, c->zadd, txa, tya, tza, txadd, tyadd, tzadd, box->d_amol,
+            box->d_itype, Pold, Pnew, dmaxc);
         if (want_x) {
             uint16_t *X = (rec && rec->x) ? rec->x : dX;
             if ((rc1 = surr_descriptors_launch(ctx, box, l, Pold, Pnew, dpar, ldx, X, nullptr))) return rc1;
@@ -1919,49 +2294,54 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
                                                           rec ? rec->des : nullptr, rec ? rec->pass : nullptr, dnrej);
         timer_begin(ctx, "mc_pair");
         if (pt.in_lds) {
-            int g = persistent_blocks(pt.lds_bytes, nslots, box->pair_block);
-            mc_pairs<true><<<g, box->pair_block, pt.lds_bytes, st>>>(nslots, l->d_slotmol, l->d_rowstart, l->d_partner,
-                                                         l->d_entry, l->d_pos, l->d_ergpair, Pold, Pnew, box->d_naf,
-                                                         box->d_nal, pt.tab, box->npairs, dshift, dA, dB, dL);
+            int g = persistent_blocks(pt.lds_bytes, (long long)nslots + l->nbv, box->pair_block);
+            mc_pairs<true><<<g, box->pair_block, pt.lds_bytes, st>>>(
+                nslots, l->nbv, l->d_bvslot, l->d_srec, l->d_entry, l->d_pos, l->d_ergpair, Pold, Pnew, pt.tab,
+                box->npairs, dmaxc, dshift, dA, dB, dL);
         } else {
-            int g = persistent_blocks(0, nslots, box->pair_block);
-            mc_pairs<false><<<g, box->pair_block, 0, st>>>(nslots, l->d_slotmol, l->d_rowstart, l->d_partner, l->d_entry,
-                                               l->d_pos, l->d_ergpair, Pold, Pnew, box->d_naf, box->d_nal, pt.tab,
-                                               box->npairs, dshift, dA, dB, dL);
+            int g = persistent_blocks(0, (long long)nslots + l->nbv, box->pair_block);
+            mc_pairs<false><<<g, box->pair_block, 0, st>>>(
+                nslots, l->nbv, l->d_bvslot, l->d_srec, l->d_entry, l->d_pos, l->d_ergpair, Pold, Pnew, pt.tab,
+                box->npairs, dmaxc, dshift, dA, dB, dL);
         }
         timer_end(ctx, "mc_pair");
         if (rec && rec->target)
             mc_record_target<<<nmol, 64, 0, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt,
                                                   l->d_partner, dA, dL, dshift, rec->target);
         if (fast) {
-            mc_prepare<<<nmol, 64, 0, st>>>(nmol, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner, l->d_mirror, dA,
-                                            dB, dL, stream, moff, temp, dshift, ddelta, dthr);
-            HIP_CHECK(ctx, hipMemsetAsync(dstate, 0, (size_t)nmol, st));
-            HIP_CHECK(ctx, hipMemsetAsync(dpc, 0, sizeof(int) * 8, st));
+            mc_prepare<<<ceil_div(nmol, 4), 256, 0, st>>>(nmol, l->d_pos, l->d_rowstart, l->d_cnt, l->d_pn, l->d_mw,
+                                                          dA, dB, dL, stream, moff, temp, dshift, ddelta, dthr, dstate,
+                                                          dpc);
             const int nwf = std::max(1, std::min(16, (int)((150 * 1024) / (sizeof(float2) * (size_t)maxc))));
             const int gpass = std::min(ceil_div(nmol, 4), 2048);
             timer_begin(ctx, "mc_resolve");
-            mc_resolve_pass<<<gpass, 256, lds_pass, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt,
-                                                          l->d_partner, l->d_mirror, dA, dB, dL, ddelta, dthr,
+            mc_resolve_pass<<<gpass, 256, lds_pass, st>>>(nmol, l->d_order, l->d_rowstart, l->d_cnt, l->d_pn,
+                                                          l->d_mw, dA, dB, dL, ddelta, dthr,
                                                           stream, moff, temp, dshift, maxc, 1, nullptr, nullptr, dW0,
                                                           dpc + 0, dstate, dpc + 3);
-            mc_resolve_pass<<<gpass, 256, lds_pass, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt,
-                                                          l->d_partner, l->d_mirror, dA, dB, dL, ddelta, dthr,
+            mc_resolve_pass<<<gpass, 256, lds_pass, st>>>(nmol, l->d_order, l->d_rowstart, l->d_cnt, l->d_pn,
+                                                          l->d_mw, dA, dB, dL, ddelta, dthr,
                                                           stream, moff, temp, dshift, maxc, 0, dW0, dpc + 0, dW1,
                                                           dpc + 1, dstate, dpc + 3);
-            mc_resolve_pass<<<gpass, 256, lds_pass, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt,
-                                                          l->d_partner, l->d_mirror, dA, dB, dL, ddelta, dthr,
+            mc_resolve_pass<<<gpass, 256, lds_pass, st>>>(nmol, l->d_order, l->d_rowstart, l->d_cnt, l->d_pn,
+                                                          l->d_mw, dA, dB, dL, ddelta, dthr,
                                                           stream, moff, temp, dshift, maxc, 0, dW1, dpc + 1, dW0,
                                                           dpc + 2, dstate, dpc + 3);
             mc_resolve_finish<<<1, 64 * nwf, sizeof(float2) * nwf * (size_t)maxc, st>>>(
-                l->d_order, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner, l->d_mirror, dA, dB, dL, ddelta, dthr,
-                stream, moff, temp, dshift, maxc, dW0, dpc + 2, dW1, dstate, dpc + 3, dpc + 4);
+                l->d_order, l->d_rowstart, l->d_cnt, l->d_pn, l->d_mw, dA, dB, dL, ddelta, dthr, stream, moff, temp,
+                dshift, maxc, dW0, dpc + 2, dW1, dstate, dpc + 3, dpc + 4);
             mc_state_bits<<<ceil_div((nmol + 31) / 32, 256), 256, 0, st>>>(nmol, dstate, dacc);
             timer_end(ctx, "mc_resolve");
-            mc_fdiff_accepted<<<ceil_div(nmol, 4), 256, sizeof(float2) * 4 * (size_t)maxc, st>>>(
-                nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner, l->d_mirror, dA, dB, dL, dacc,
-                maxc, dfdiff);
-            mc_fadd<<<1, 1024, 0, st>>>(nmol, dacc, dfdiff, dres);
+            // fadd only reports: its kernels run on the side stream beside the commits
+            hipStream_t side = nullptr;
+            if ((rc1 = ctx_side(ctx, &side))) return rc1;
+            HIP_CHECK(ctx, hipEventRecord(ctx->ev_fork, st));
+            HIP_CHECK(ctx, hipStreamWaitEvent(side, ctx->ev_fork, 0));
+            mc_fdiff_accepted<<<ceil_div(nmol, 4), 256, sizeof(float2) * 4 * (size_t)maxc, side>>>(
+                nmol, l->d_order, l->d_rowstart, l->d_cnt, l->d_pn, l->d_mw, dA, dB, dL, dacc, maxc, dfdiff);
+            mc_fadd<<<1, 1024, 0, side>>>(nmol, dacc, dfdiff, dres);
+            HIP_CHECK(ctx, hipEventRecord(ctx->ev_join, side));
+            joined = true;
         } else {
             size_t lds = sizeof(unsigned) * ((nmol + 31) / 32) + sizeof(float) * 2 * (size_t)maxc + 64;
             ARG_CHECK(ctx, lds <= 160 * 1024, "epsr_molfit: %d molecules x %d ledger slots exceed LDS", nmol, maxc);
@@ -1970,7 +2350,6 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
                                             l->d_mirror, dA, dB, dL, stream, moff, temp, maxc, dacc, dfdiff, dres);
             timer_end(ctx, "mc_resolve");
         }
-        HIP_CHECK(ctx, hipMemsetAsync(drej, 0, sizeof(int) * 2 * nmt, st));
         mc_commit_ledger<<<ceil_div(nslots, 256), 256, 0, st>>>(nslots, l->d_slotmol, l->d_rowstart, l->d_partner,
                                                                 l->d_entry, l->d_mirror, l->d_pos, dacc, dA, dB, redo,
                                                                 l->d_ergpair);
@@ -1978,12 +2357,12 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
                                                             dacc, roff, toff, txa, tya, tza, txadd, tyadd, tzadd, c->xa,
                                                             c->ya, c->za, c->xadd, c->yadd, c->zadd, nstepreject,
                                                             p->stepr > 0.0f ? 1 : 0, redo, drej);
+        if (joined) HIP_CHECK(ctx, hipStreamWaitEvent(st, ctx->ev_join, 0));
         HIP_CHECK(ctx, hipMemcpyAsync(hblk, dblk, sizeof(int) * (16 + 2 * (size_t)nmt), hipMemcpyDeviceToHost, st));
         HIP_CHECK(ctx, hipGetLastError());
         return EPSR_OK;
     };
-    if ((rc = run(dstream, l->d_moff, l->d_roff, l->d_toff, dtmp + 1))) return rc;
-    HIP_CHECK(ctx, hipMemcpyAsync(hflag, dtmp + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    if ((rc = run(dstream, l->d_moff, l->d_roff, l->d_toff, dnochange))) return rc;
     // while the GPU runs the cycle: the next cycle's stream, from this one's end state
     const Ran1 rend = rng;
     {
@@ -1997,6 +2376,13 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     }
     HIP_CHECK(ctx, hipStreamSynchronize(st));
     l->ahead_valid = *hflag == 0;
+    if (*hflag & TABLES_STALE) {
+        // the tables changed in place since their extents were measured; nothing was
+        // committed and no state was updated: measure them and run the cycle again
+        l->tab_ok = false;
+        return molfit_impl(ctx, box, c, tb, l, p, wholerot, wholetrans, nwhole, nmovem, nstepreject, ran1_state, fadds,
+                           da, rec, stats);
+    }
     int retries = 0;
     if (*hflag) {
         // movem doubles its step until the centre moves (molfit.py:747-758); the doubled
@@ -2068,10 +2454,15 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
         HIP_CHECK(ctx, hipMemcpyAsync(dmoff, moff.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, st));
         HIP_CHECK(ctx, hipMemcpyAsync(drstep, rstep.data(), sizeof(float) * nmol, hipMemcpyHostToDevice, st));
         HIP_CHECK(ctx, hipMemcpyAsync(dtstep, tstep.data(), sizeof(float) * nmol, hipMemcpyHostToDevice, st));
+        // counters of the redone cycle start at zero; its trial moves all change
+        HIP_CHECK(ctx, hipMemsetAsync(dblk + 8, 0, 4 * sizeof(int), st));
+        HIP_CHECK(ctx, hipMemsetAsync(dtmp + 2, 0, 2 * sizeof(int), st));
+        TabCheck none = tc;
+        none.T = nullptr;
         mc_trial<<<ceil_div(nmol, 128), 128, 0, st>>>(nmol, l->d_order, box->d_naf, box->d_nal, c->xa, c->ya, c->za,
                                                       c->xadd, c->yadd, c->zadd, dstream2, droff, dtoff, drstep,
                                                       dtstep, box->d_mtype, dwtype, txa, tya, tza, txadd, tyadd, tzadd,
-                                                      dtmp + 2);
+                                                      dtmp + 2, none, drej, 2 * nmt);
         if ((rc = run(dstream2, dmoff, droff, dtoff, dtmp + 3))) return rc;
         HIP_CHECK(ctx, hipStreamSynchronize(st));
     }

@@ -40,10 +40,27 @@ extern "C" void epsr_ctx_destroy(epsr_ctx *ctx)
         if (kv.second.start) hipEventDestroy(kv.second.start);
         if (kv.second.stop) hipEventDestroy(kv.second.stop);
     }
+    if (ctx->side) {
+        hipStreamSynchronize(ctx->side);
+        hipStreamDestroy(ctx->side);
+    }
+    if (ctx->ev_fork) hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) hipEventDestroy(ctx->ev_join);
     if (ctx->scratch) hipFree(ctx->scratch);
     if (ctx->pinned) hipHostFree(ctx->pinned);
     if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
     delete ctx;
+}
+
+int ctx_side(epsr_ctx *ctx, hipStream_t *out)
+{
+    if (!ctx->side) {
+        HIP_CHECK(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+        HIP_CHECK(ctx, hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming));
+        HIP_CHECK(ctx, hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming));
+    }
+    *out = ctx->side;
+    return EPSR_OK;
 }
 
 extern "C" const char *epsr_last_error(const epsr_ctx *ctx) { return ctx ? ctx->err.c_str() : "no context"; }

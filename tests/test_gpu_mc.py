@@ -178,3 +178,53 @@ def test_molfit_against_glibc_libm_chain(ctx):
     print(f"molfit vs glibc chain: {same * 100:.1f}% of centres identical after one sweep")
     np.testing.assert_array_equal(st, ref["ran1"])      # RNG consumption does not depend on libm
     assert same > 0.5
+
+
+@pytest.mark.parametrize("change", ["grow", "scale"])
+def test_molfit_after_tables_changed_in_place(ctx, change):
+    """MC cycles on tables whose arrays were changed in place after they were measured and
+    packed (the empirical potential grown past its last non-zero bin, or scaled within its
+    range) give the bits of cycles on a copy of the changed tables, packed afresh."""
+    from epsr_python_translation_amd import device as D
+    name = "c0"
+    s = H.system(name)
+    t = H.oracle_tables(name)
+    ld, npt2 = t["ld"], t["npt2"]
+    xavs = np.asarray(t["xavs"], F32).reshape(-1, ld)
+    last = int(np.nonzero(np.any(xavs[:, :npt2 + 1] != 0, axis=0))[0].max())
+    grown = xavs.copy()
+    if change == "grow":
+        hi = min(npt2, last + 300)
+        assert hi > last + 1
+        grown[:, last + 1:hi] = F32(2e-3)
+        print(f"tables changed in place: xavs grown from bin {last} to {hi - 1} of {npt2}", flush=True)
+    else:
+        grown = (grown * F32(1.25)).astype(F32)
+        print(f"tables changed in place: xavs scaled by 1.25 up to bin {last}", flush=True)
+    outs = []
+    for inplace in (True, False):
+        box, tab, led, _, _ = _gpu_setup(ctx, name)
+        wr, wt = H.initial_steps(s)
+        mc = D.MCState(ctx, s.nmol, s.nmoltype, s.simt, wholerot=wr, wholetrans=wt)
+        st = D.ran1_state_array(-31)
+        D.molfit(ctx, box, tab, led, mc, st)
+        torch.cuda.synchronize()
+        if inplace:
+            tab.xavs.copy_(torch.as_tensor(grown.ravel()))
+        else:
+            tab = D.DeviceTables(ctx, ld, npt2, t["xstar"], grown.ravel(), t["xavs1"], t["ecoul"], t["ecoul1"])
+        torch.cuda.synchronize()
+        for _ in range(2):
+            D.molfit(ctx, box, tab, led, mc, st)
+        torch.cuda.synchronize()
+        co = box.get_coords()
+        outs.append(dict(co=co, st=st.copy(), erg=led.export()["ergpair"].copy(), wr=mc.wholerot.copy(),
+                         wt=mc.wholetrans.copy(), fadds=F32(mc.fadds)))
+    a, b = outs
+    for k in ("xa", "ya", "za", "xadd", "yadd", "zadd"):
+        assert np.array_equal(a["co"][k].view(np.int32), b["co"][k].view(np.int32)), k
+    np.testing.assert_array_equal(a["st"], b["st"])
+    assert np.array_equal(a["erg"].view(np.int32), b["erg"].view(np.int32))
+    assert np.array_equal(a["wr"].view(np.int32), b["wr"].view(np.int32))
+    assert np.array_equal(a["wt"].view(np.int32), b["wt"].view(np.int32))
+    assert a["fadds"].view(np.int32) == b["fadds"].view(np.int32)
