@@ -38,6 +38,8 @@ struct epsr_ctx {
     // the events that fork it from and join it back to the main stream
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // marks the end of a call's work for ctx_wait
+    hipEvent_t ev_done = nullptr;
 };
 
 struct epsr_box {
@@ -95,6 +97,12 @@ struct epsr_ledger {
     bool ahead_valid = false;
     int ahead_total = 0;
     int32_t ahead_from[35], ahead_to[35];
+    // device copies of the two streams; the next cycle's is uploaded on the side stream
+    // while the current cycle runs (ahead_uploaded), behind the event ev_up
+    float *d_stream[2] = {nullptr, nullptr};
+    size_t dstream_cap = 0;
+    bool ahead_uploaded = false;
+    hipEvent_t ev_up = nullptr;
     bool ledger_ready = false;
     int maxcnt = 0;   // the longest list
     // table extents measured for the arrays in tab_ptr (xstar, xavs, xavs1, ecoul, ecoul1)
@@ -148,6 +156,9 @@ bool ctx_capturing(epsr_ctx *ctx);
 hipError_t ctx_sync(epsr_ctx *ctx);
 // the context's side stream and its fork/join events, created on first use
 int ctx_side(epsr_ctx *ctx, hipStream_t *out);
+// wait for the work issued so far on the context's stream by polling an event (a short
+// wait wakes sooner than a stream synchronisation)
+int ctx_wait(epsr_ctx *ctx);
 
 // scratch management (device); returns nullptr and sets the error on failure
 void *ctx_scratch(epsr_ctx *ctx, size_t bytes);

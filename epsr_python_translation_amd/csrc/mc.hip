@@ -65,7 +65,13 @@ extern "C" void epsr_ledger_destroy(epsr_ledger *l)
     hipFree(l->d_roff); hipFree(l->d_toff); hipFree(l->d_moff);
     if (l->pin[0]) hipHostFree(l->pin[0]);
     if (l->pin[1]) hipHostFree(l->pin[1]);
+    if (l->ev_up) {
+        hipEventSynchronize(l->ev_up);
+        hipEventDestroy(l->ev_up);
+    }
     if (l->hio) hipHostFree(l->hio);
+    hipFree(l->d_stream[0]);
+    hipFree(l->d_stream[1]);
     if (l->hstage) hipHostFree(l->hstage);
     if (l->hsc) hipHostFree(l->hsc);
     hipFree(l->d_cells);
@@ -1357,12 +1363,30 @@ __device__ __forceinline__ float cr_exp(float x) { return (float)exp((double)x);
 // A: trial moves.  rotatm (molfit.py:671-711) about z, y, x with correctly rounded
 // cosf; movem (:732-759) with the translation triplet at toff (the host has resolved
 // any step doubling).  nochange is reported so the host can resolve it.
+// the step sizes (rotation, translation) of up to 16 molecule types, passed by value
+#define STEP_TYPES 16
+struct StepArgs {
+    float w[2 * STEP_TYPES];
+};
+
+// w[j] of a by-value StepArgs without dynamic indexing of the argument (which would
+// copy it to private memory)
+__device__ __forceinline__ float step_of(const StepArgs &sa, int j)
+{
+    float v = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 2 * STEP_TYPES; ++t)
+        if (t == j) v = sa.w[t];
+    return v;
+}
+
 __global__ void mc_trial(int nmol, const int *__restrict__ order, const int *__restrict__ naf,
                          const int *__restrict__ nal, const float *__restrict__ xa, const float *__restrict__ ya,
                          const float *__restrict__ za, const float *__restrict__ xadd, const float *__restrict__ yadd,
                          const float *__restrict__ zadd, const float *__restrict__ stream, const int *__restrict__ roff,
                          const int *__restrict__ toff, const float *__restrict__ rstep, const float *__restrict__ tstep,
-                         const int *__restrict__ mtype, const float *__restrict__ wtype, float *__restrict__ txa,
+                         const int *__restrict__ mtype, const float *__restrict__ wtype, StepArgs sa,
+                         float *__restrict__ txa,
                          float *__restrict__ tya, float *__restrict__ tza, float *__restrict__ txadd,
                          float *__restrict__ tyadd, float *__restrict__ tzadd, int *__restrict__ nochange,
                          TabCheck tc, int *__restrict__ zero, int nzero)
@@ -1382,7 +1406,7 @@ __global__ void mc_trial(int nmol, const int *__restrict__ order, const int *__r
         tza[a] = za[a];
     }
     if (roff[k] >= 0 && a1 > a0) {
-        float st = rstep ? rstep[k] : wtype[2 * mtype[m]];
+        float st = rstep ? rstep[k] : (wtype ? wtype[2 * mtype[m]] : step_of(sa, 2 * mtype[m]));
         for (int axis = 0; axis < 3; ++axis) {
             float phi = pi * st * (2.0f * stream[roff[k] + axis] - 1.0f);
             float cphi = cr_cos(phi);
@@ -1405,7 +1429,7 @@ __global__ void mc_trial(int nmol, const int *__restrict__ order, const int *__r
     float x = xadd[m], y = yadd[m], z = zadd[m];
     float nx = x, ny = y, nz = z;
     if (toff[k] >= 0) {
-        float st = tstep ? tstep[k] : wtype[2 * mtype[m] + 1];
+        float st = tstep ? tstep[k] : (wtype ? wtype[2 * mtype[m] + 1] : step_of(sa, 2 * mtype[m] + 1));
         nz = z + st * (2.0f * stream[toff[k]] - 1.0f);
         ny = y + st * (2.0f * stream[toff[k] + 1] - 1.0f);
         nx = x + st * (2.0f * stream[toff[k] + 2] - 1.0f);
@@ -1555,25 +1579,32 @@ __global__ void mc_commit_mols(int nmol, const int *__restrict__ order, const in
 }
 
 // each pair entry is finalised by its slot in the row of the molecule that moves later
-__global__ void mc_commit_ledger(int nslots, const int *__restrict__ slotmol, const int *__restrict__ rowstart,
-                                 const int *__restrict__ partner, const int *__restrict__ entry,
-                                 const int *__restrict__ mirror, const int *__restrict__ pos,
-                                 const unsigned *__restrict__ accb, const float *__restrict__ A,
-                                 const float *__restrict__ Bv, const int *__restrict__ redo, float *__restrict__ ergpair)
+// ledger write-back (molfit.py:417-447): the entry of a pair takes the energy of the
+// later of its two molecules' accepted moves.  Items: the slots whose partner comes
+// earlier (bvslot; they hold the later molecule's side of every pair but the first slots),
+// then the first slot of every molecule, which may also be a pair slot.
+__global__ void mc_commit_ledger(int nbv, int nmol, const int *__restrict__ bvslot, const int *__restrict__ rowstart,
+                                 const int4 *__restrict__ srec, const int *__restrict__ pos,
+                                 const int *__restrict__ pnv, const int *__restrict__ mw,
+                                 const int *__restrict__ entry, const unsigned *__restrict__ accb,
+                                 const float *__restrict__ A, const float *__restrict__ Bv, const int *__restrict__ redo,
+                                 float *__restrict__ ergpair)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nslots || *redo) return;
-    int mi = mirror[i];
-    if (mi < 0) return;
-    int m = slotmol[i], n = partner[i];
-    int pm = pos[m], pn = pos[n];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nbv + nmol || *redo) return;
+    const bool first = t >= nbv;
+    const int i = first ? rowstart[t - nbv] : bvslot[t];
+    if (first && i == rowstart[t - nbv + 1]) return;   // an empty list
+    const int4 r = srec[i];
+    const int m = r.x & 0x7fffffff;
+    if (r.y == m) return;   // the molecule's own entry
+    const int pm = pos[m], pn = pnv[i];
     if (pm < pn) return;
-    bool am = (accb[pm >> 5] >> (pm & 31)) & 1u;
-    bool an = (accb[pn >> 5] >> (pn & 31)) & 1u;
-    bool wm = am && i != rowstart[m];
-    bool wn = an && mi != rowstart[n];
-    if (wm) ergpair[entry[i]] = an ? Bv[i] : A[i];
-    else if (wn) ergpair[entry[i]] = A[mi];
+    const bool am = (accb[pm >> 5] >> (pm & 31)) & 1u;
+    const bool an = (accb[pn >> 5] >> (pn & 31)) & 1u;
+    const int mi = mw[i];
+    if (am && !first) ergpair[entry[i]] = an ? Bv[i] : A[i];
+    else if (an && mi >= 0) ergpair[entry[i]] = A[mi];
 }
 
 // ------------------------------------------------------------------ fast resolution
@@ -2173,6 +2204,7 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     // the ran1 stream of the cycle: drawn ahead by the previous call when it starts from
     // this state, else drawn now; both in pinned memory
     if (l->pin_cap < (size_t)total + 2 * nmt + 8) {
+        if (l->ev_up) HIP_CHECK(ctx, hipEventSynchronize(l->ev_up));
         for (int i = 0; i < 2; ++i) {
             if (l->pin[i]) HIP_CHECK(ctx, hipHostFree(l->pin[i]));
             l->pin[i] = nullptr;
@@ -2187,14 +2219,30 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     ran1_load(rng, ran1_state);
     const Ran1 rng0 = rng;
     int use;
+    bool uploaded = false;   // already in l->d_stream[use], behind the event ev_up
     if (l->ahead_valid && l->ahead_total == total && std::memcmp(l->ahead_from, ran1_state, sizeof(l->ahead_from)) == 0) {
         use = l->pin_ahead;
         ran1_load(rng, l->ahead_to);
+        uploaded = l->ahead_uploaded;
     } else {
         use = 1 - l->pin_ahead;
+        if (l->ev_up) HIP_CHECK(ctx, hipEventSynchronize(l->ev_up));   // no upload still reads the pinned buffers
         ran1_fill(rng, l->pin[use], total);
     }
     l->ahead_valid = false;
+    l->ahead_uploaded = false;
+    // the device copies of the two streams
+    if (l->dstream_cap < (size_t)total + 8) {
+        HIP_CHECK(ctx, hipStreamSynchronize(st));
+        if (l->ev_up) HIP_CHECK(ctx, hipEventSynchronize(l->ev_up));
+        const size_t cap = (size_t)total + total / 8 + 64;
+        for (int i = 0; i < 2; ++i) {
+            int rc0;
+            if ((rc0 = dev_alloc(ctx, &l->d_stream[i], cap))) return rc0;
+        }
+        l->dstream_cap = cap;
+        uploaded = false;
+    }
 
     // device buffers.  The parallel resolution keeps the terms of an exact sum (the
     // longest ledger list) in LDS for each of its waves.
@@ -2206,7 +2254,7 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     auto layout = [&](Scratch &x) {
         x.take<float4>(natom); x.take<float4>(natom);
         x.take<int>(8);
-        x.take<float>(total + 2 * nmt + 8);
+        x.take<float>(2 * nmt);
         x.take<float>(natom); x.take<float>(natom); x.take<float>(natom);
         x.take<float>(nmol); x.take<float>(nmol); x.take<float>(nmol);
         x.take<float>(nslots); x.take<float>(nslots); x.take<float>(nslots);
@@ -2223,9 +2271,10 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     float4 *Pold = S.take<float4>(natom), *Pnew = S.take<float4>(natom);
     // [2], [3] flags of a redone cycle, [4..5] table extents
     int *dtmp = S.take<int>(8);
-    // the ran1 stream of the cycle, then the step sizes of every type (one upload)
-    float *dstream = S.take<float>(total + 2 * nmt + 8);
-    float *dwtype = dstream + total;
+    // the ran1 stream of the cycle (in the ledger) and, for more than STEP_TYPES molecule
+    // types, the step sizes of every type (else they go by value to mc_trial)
+    float *dstream = l->d_stream[use];
+    float *dwtype = S.take<float>(2 * nmt);
     float *txa = S.take<float>(natom), *tya = S.take<float>(natom), *tza = S.take<float>(natom);
     float *txadd = S.take<float>(nmol), *tyadd = S.take<float>(nmol), *tzadd = S.take<float>(nmol);
     float *dA = S.take<float>(nslots), *dB = S.take<float>(nslots), *dL = S.take<float>(nslots);
@@ -2267,13 +2316,24 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
         tab_remember(l, tb, ext[0], ext[1]);
         l->tab_packed = true;
     }
-    std::memcpy(l->pin[use] + total, wtype, sizeof(float) * 2 * nmt);
-    HIP_CHECK(ctx, hipMemcpyAsync(dstream, l->pin[use], sizeof(float) * (total + 2 * nmt), hipMemcpyHostToDevice, st));
+    const bool steps_by_value = nmt <= STEP_TYPES;
+    StepArgs sa{};
+    if (steps_by_value)
+        for (int j = 0; j < 2 * nmt; ++j) sa.w[j] = wtype[j];
+    if (uploaded) {
+        HIP_CHECK(ctx, hipStreamWaitEvent(st, l->ev_up, 0));
+    } else {
+        HIP_CHECK(ctx, hipMemcpyAsync(dstream, l->pin[use], sizeof(float) * total, hipMemcpyHostToDevice, st));
+    }
+    if (!steps_by_value) {
+        std::memcpy(l->pin[use] + total, wtype, sizeof(float) * 2 * nmt);
+        HIP_CHECK(ctx, hipMemcpyAsync(dwtype, l->pin[use] + total, sizeof(float) * 2 * nmt, hipMemcpyHostToDevice, st));
+    }
     HIP_CHECK(ctx, hipMemsetAsync(dblk + 8, 0, 4 * sizeof(int), st));
     mc_trial<<<ceil_div(nmol, 128), 128, 0, st>>>(nmol, l->d_order, box->d_naf, box->d_nal, c->xa, c->ya, c->za, c->xadd,
                                                   c->yadd, c->zadd, dstream, l->d_roff, l->d_toff, nullptr, nullptr,
-                                                  box->d_mtype, dwtype, txa, tya, tza, txadd, tyadd, tzadd, dnochange,
-                                                  tc, drej, 2 * nmt);
+                                                  box->d_mtype, steps_by_value ? nullptr : dwtype, sa, txa, tya, tza,
+                                                  txadd, tyadd, tzadd, dnochange, tc, drej, 2 * nmt);
 
     int nexact = nmol, npasses = 0;
     // B..D of the cycle from trial positions in txa..tzadd and the stream at stream; the
@@ -2350,9 +2410,9 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
                                             l->d_mirror, dA, dB, dL, stream, moff, temp, maxc, dacc, dfdiff, dres);
             timer_end(ctx, "mc_resolve");
         }
-        mc_commit_ledger<<<ceil_div(nslots, 256), 256, 0, st>>>(nslots, l->d_slotmol, l->d_rowstart, l->d_partner,
-                                                                l->d_entry, l->d_mirror, l->d_pos, dacc, dA, dB, redo,
-                                                                l->d_ergpair);
+        mc_commit_ledger<<<ceil_div((long)l->nbv + nmol, 256), 256, 0, st>>>(
+            l->nbv, nmol, l->d_bvslot, l->d_rowstart, l->d_srec, l->d_pos, l->d_pn, l->d_mw, l->d_entry, dacc, dA, dB,
+            redo, l->d_ergpair);
         mc_commit_mols<<<ceil_div(nmol, 128), 128, 0, st>>>(nmol, l->d_order, box->d_naf, box->d_nal, box->d_mtype,
                                                             dacc, roff, toff, txa, tya, tza, txadd, tyadd, tzadd, c->xa,
                                                             c->ya, c->za, c->xadd, c->yadd, c->zadd, nstepreject,
@@ -2368,14 +2428,22 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     {
         const int nxt = 1 - use;
         Ran1 r2 = rend;
+        if (l->ev_up) HIP_CHECK(ctx, hipEventSynchronize(l->ev_up));
         ran1_fill(r2, l->pin[nxt], total);
         ran1_store(rend, l->ahead_from);
         ran1_store(r2, l->ahead_to);
         l->ahead_total = total;
         l->pin_ahead = nxt;
+        // and its upload, beside this cycle's kernels (the buffer is not this cycle's)
+        hipStream_t side = nullptr;
+        if ((rc = ctx_side(ctx, &side))) return rc;
+        if (!l->ev_up) HIP_CHECK(ctx, hipEventCreateWithFlags(&l->ev_up, hipEventDisableTiming));
+        HIP_CHECK(ctx, hipMemcpyAsync(l->d_stream[nxt], l->pin[nxt], sizeof(float) * total, hipMemcpyHostToDevice, side));
+        HIP_CHECK(ctx, hipEventRecord(l->ev_up, side));
     }
-    HIP_CHECK(ctx, hipStreamSynchronize(st));
+    if ((rc = ctx_wait(ctx))) return rc;
     l->ahead_valid = *hflag == 0;
+    l->ahead_uploaded = l->ahead_valid;
     if (*hflag & TABLES_STALE) {
         // the tables changed in place since their extents were measured; nothing was
         // committed and no state was updated: measure them and run the cycle again
@@ -2461,8 +2529,8 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
         none.T = nullptr;
         mc_trial<<<ceil_div(nmol, 128), 128, 0, st>>>(nmol, l->d_order, box->d_naf, box->d_nal, c->xa, c->ya, c->za,
                                                       c->xadd, c->yadd, c->zadd, dstream2, droff, dtoff, drstep,
-                                                      dtstep, box->d_mtype, dwtype, txa, tya, tza, txadd, tyadd, tzadd,
-                                                      dtmp + 2, none, drej, 2 * nmt);
+                                                      dtstep, box->d_mtype, steps_by_value ? nullptr : dwtype, sa, txa,
+                                                      tya, tza, txadd, tyadd, tzadd, dtmp + 2, none, drej, 2 * nmt);
         if ((rc = run(dstream2, dmoff, droff, dtoff, dtmp + 3))) return rc;
         HIP_CHECK(ctx, hipStreamSynchronize(st));
     }

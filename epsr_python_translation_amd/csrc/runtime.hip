@@ -45,11 +45,24 @@ extern "C" void epsr_ctx_destroy(epsr_ctx *ctx)
         hipStreamDestroy(ctx->side);
     }
     if (ctx->ev_fork) hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_done) hipEventDestroy(ctx->ev_done);
     if (ctx->ev_join) hipEventDestroy(ctx->ev_join);
     if (ctx->scratch) hipFree(ctx->scratch);
     if (ctx->pinned) hipHostFree(ctx->pinned);
     if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
     delete ctx;
+}
+
+int ctx_wait(epsr_ctx *ctx)
+{
+    if (!ctx->ev_done) HIP_CHECK(ctx, hipEventCreateWithFlags(&ctx->ev_done, hipEventDisableTiming));
+    HIP_CHECK(ctx, hipEventRecord(ctx->ev_done, ctx->stream));
+    for (;;) {
+        const hipError_t e = hipEventQuery(ctx->ev_done);
+        if (e == hipSuccess) return EPSR_OK;
+        if (e != hipErrorNotReady)
+            return set_error(ctx, EPSR_ERR_HIP, "hipEventQuery failed: %s", hipGetErrorString(e));
+    }
 }
 
 int ctx_side(epsr_ctx *ctx, hipStream_t *out)

@@ -379,16 +379,27 @@ class MCState:
         self.stats = _native.MCStats()
 
 
+def _mc_args(mc: MCState, ran1_state):
+    """ctypes pointers to the host arrays of mc and ran1_state, built once per set of arrays
+    (building them costs several microseconds per array and call)"""
+    objs = (mc.wholerot, mc.wholetrans, mc.nwhole, mc.nmovem, mc.nstepreject, ran1_state)
+    cached = getattr(mc, "_args", None)
+    if cached is None or any(a is not b for a, b in zip(cached[0], objs)):
+        P = lambda a, ty=C.c_int: a.ctypes.data_as(C.POINTER(ty))
+        args = (P(mc.wholerot, C.c_float), P(mc.wholetrans, C.c_float), P(mc.nwhole), P(mc.nmovem),
+                _ptr(mc.nstepreject), P(ran1_state, C.c_int32))
+        cached = (objs, args)   # holds the arrays, so their ids and buffers stay valid
+        mc._args = cached
+    return cached[1]
+
+
 def molfit(ctx: Context, box: DeviceBox, tables: DeviceTables, ledger: Ledger, mc: MCState, ran1_state):
     """One MC cycle on the GPU; updates coordinates, ledger, mc and ran1_state in place."""
     c = box.coords()
     t = tables.struct()
     fad = C.c_float(float(mc.fadds))
-    P = lambda a, ty=C.c_int: a.ctypes.data_as(C.POINTER(ty))
     ctx.check(ctx.lib.epsr_molfit(ctx.h, box.h, C.byref(c), C.byref(t), ledger.h, C.byref(mc.params),
-                                  P(mc.wholerot, C.c_float), P(mc.wholetrans, C.c_float), P(mc.nwhole),
-                                  P(mc.nmovem), _ptr(mc.nstepreject), P(ran1_state, C.c_int32), C.byref(fad),
-                                  C.byref(mc.stats)))
+                                  *_mc_args(mc, ran1_state), C.byref(fad), C.byref(mc.stats)))
     mc.fadds = F32(fad.value)
     return mc
 
@@ -400,10 +411,8 @@ def molfit_da(ctx: Context, box: DeviceBox, tables: DeviceTables, ledger: Ledger
     c = box.coords()
     t = tables.struct()
     fad = C.c_float(float(mc.fadds))
-    P = lambda a, ty=C.c_int: a.ctypes.data_as(C.POINTER(ty))
     ctx.check(ctx.lib.epsr_molfit_da(ctx.h, box.h, C.byref(c), C.byref(t), ledger.h, C.byref(mc.params),
-                                     P(mc.wholerot, C.c_float), P(mc.wholetrans, C.c_float), P(mc.nwhole),
-                                     P(mc.nmovem), _ptr(mc.nstepreject), P(ran1_state, C.c_int32), C.byref(fad),
+                                     *_mc_args(mc, ran1_state), C.byref(fad),
                                      C.byref(da) if da is not None else None,
                                      C.byref(record) if record is not None else None, C.byref(mc.stats)))
     mc.fadds = F32(fad.value)
