@@ -7,6 +7,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/epsr_mi355x.h"
@@ -88,7 +89,7 @@ struct epsr_ledger {
     int *d_roff = nullptr, *d_toff = nullptr, *d_moff = nullptr;
     // pinned ran1 streams: the one a cycle uploads, and the next cycle's, drawn on the host
     // while the GPU runs the cycle (valid when the next call starts from ahead_from)
-    float *pin[2] = {nullptr, nullptr};
+    float *pin[3] = {nullptr, nullptr, nullptr};
     size_t pin_cap = 0;
     // pinned block of a cycle's small transfers (results down, step sizes up)
     int *hio = nullptr;
@@ -99,10 +100,15 @@ struct epsr_ledger {
     int32_t ahead_from[35], ahead_to[35];
     // device copies of the two streams; the next cycle's is uploaded on the side stream
     // while the current cycle runs (ahead_uploaded), behind the event ev_up
-    float *d_stream[2] = {nullptr, nullptr};
+    float *d_stream[3] = {nullptr, nullptr, nullptr};
     size_t dstream_cap = 0;
     bool ahead_uploaded = false;
     hipEvent_t ev_up = nullptr;
+    // a host thread drawing the stream of the cycle after next into pin[gen_buf], from
+    // the state gen_from (gen_total values; 0: none), ending in gen_to
+    std::thread gen;
+    int gen_buf = -1, gen_total = 0;
+    int32_t gen_from[35], gen_to[35];
     bool ledger_ready = false;
     int maxcnt = 0;   // the longest list
     // table extents measured for the arrays in tab_ptr (xstar, xavs, xavs1, ecoul, ecoul1)
@@ -131,6 +137,12 @@ struct epsr_ledger {
     void *wbuf[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     size_t wcap[5] = {0, 0, 0, 0, 0};
 };
+
+// wait for the ledger's stream-drawing thread, if one runs
+inline void gen_join(epsr_ledger *l)
+{
+    if (l->gen.joinable()) l->gen.join();
+}
 
 // ------------------------------------------------------------------ errors
 

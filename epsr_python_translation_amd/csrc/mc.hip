@@ -63,15 +63,15 @@ extern "C" void epsr_ledger_destroy(epsr_ledger *l)
     hipFree(l->d_partner); hipFree(l->d_entry); hipFree(l->d_mirror); hipFree(l->d_slotmol);
     hipFree(l->d_ergpair);
     hipFree(l->d_roff); hipFree(l->d_toff); hipFree(l->d_moff);
-    if (l->pin[0]) hipHostFree(l->pin[0]);
-    if (l->pin[1]) hipHostFree(l->pin[1]);
+    gen_join(l);
+    for (float *q : l->pin)
+        if (q) hipHostFree(q);
     if (l->ev_up) {
         hipEventSynchronize(l->ev_up);
         hipEventDestroy(l->ev_up);
     }
     if (l->hio) hipHostFree(l->hio);
-    hipFree(l->d_stream[0]);
-    hipFree(l->d_stream[1]);
+    for (float *q : l->d_stream) hipFree(q);
     if (l->hstage) hipHostFree(l->hstage);
     if (l->hsc) hipHostFree(l->hsc);
     hipFree(l->d_cells);
@@ -1806,71 +1806,6 @@ mc_fdiff_accepted(int nmol, const int *__restrict__ order, const int *__restrict
     }
 }
 
-// fadd = fadd + fdiff over the accepted moves in chain order (molfit.py:405), and the
-// number of accepted moves.  One workgroup: each segment of the chain is compacted into
-// LDS (accepted values in chain order) by all threads, their loads issued together, then
-// thread 0 adds them in order, 64 per batch of LDS reads.
-#define FADD_SEG 16384
-
-__global__ void __launch_bounds__(1024)
-mc_fadd(int nmol, const unsigned *__restrict__ bits, const float *__restrict__ fdiff, float *__restrict__ res)
-{
-    __shared__ float4 vals4[FADD_SEG / 4];
-    __shared__ int wsum[16];
-    float *vals = (float *)vals4;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    float fadd = 0.0f;
-    int nacc = 0;
-    for (int s0 = 0; s0 < nmol; s0 += FADD_SEG) {
-        const int s1 = min(nmol, s0 + FADD_SEG);
-        bool a[FADD_SEG / 1024];
-        float v[FADD_SEG / 1024];
-#pragma unroll
-        for (int j = 0; j < FADD_SEG / 1024; ++j) {
-            const int k = s0 + j * 1024 + (int)threadIdx.x;
-            a[j] = k < s1 && ((bits[k >> 5] >> (k & 31)) & 1u);
-            v[j] = a[j] ? fdiff[k] : 0.0f;
-        }
-        int fill = 0;
-#pragma unroll
-        for (int j = 0; j < FADD_SEG / 1024; ++j) {
-            const unsigned long long b = __ballot(a[j]);
-            if (lane == 0) wsum[w] = __popcll(b);
-            __syncthreads();
-            int off = fill, tot = 0;
-            for (int i = 0; i < 16; ++i) {
-                if (i < w) off += wsum[i];
-                tot += wsum[i];
-            }
-            if (a[j]) vals[off + __popcll(b & ((1ull << lane) - 1ull))] = v[j];
-            fill += tot;
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) {
-            int i = 0;
-            for (; i + 64 <= fill; i += 64) {
-                float4 q[16];
-#pragma unroll
-                for (int j = 0; j < 16; ++j) q[j] = vals4[(i >> 2) + j];
-#pragma unroll
-                for (int j = 0; j < 16; ++j) {
-                    fadd = fadd + q[j].x;
-                    fadd = fadd + q[j].y;
-                    fadd = fadd + q[j].z;
-                    fadd = fadd + q[j].w;
-                }
-            }
-            for (; i < fill; ++i) fadd = fadd + vals[i];
-        }
-        nacc += fill;
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        res[0] = fadd;
-        res[1] = (float)nacc;
-    }
-}
-
 // ------------------------------------------------------------------ parallel resolution
 // The decisions of a cycle, resolved by every CU at once.  A move's pushed sum is the sum
 // of delta over its earlier partners that were accepted; while some of them are
@@ -2088,6 +2023,21 @@ static float adapt_ratio(float rejrate, int att, int rej)
     return rat;
 }
 
+// the stream of the cycle after next, drawn by a host thread while this call runs
+static void gen_start(epsr_ledger *l, int buf, const Ran1 &from, int total)
+{
+    l->gen_buf = buf;
+    l->gen_total = total;
+    ran1_store(from, l->gen_from);
+    float *out = l->pin[buf];
+    int32_t *to = l->gen_to;
+    l->gen = std::thread([from, out, total, to]() {
+        Ran1 r = from;
+        ran1_fill(r, out, total);
+        ran1_store(r, to);
+    });
+}
+
 static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *c, const epsr_tables_t *tb,
                        epsr_ledger *l, const epsr_mc_params_t *p, float *wholerot, float *wholetrans, int *nwhole,
                        int *nmovem, int *nstepreject, int32_t *ran1_state, float *fadds, const epsr_da_t *da,
@@ -2170,7 +2120,8 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     // pinned host block of the cycle's small transfers: the device result block
     // (16 + 2 nmt ints), the no-change flag, the step sizes per type (2 nmt floats).  The
     // previous call ended with a synchronisation, so nothing reads it now.
-    const size_t hio_need = sizeof(int) * (20 + 4 * (size_t)nmt);
+    const int nwords = (nmol + 31) / 32;
+    const size_t hio_need = sizeof(int) * (20 + 4 * (size_t)nmt) + sizeof(float) * (size_t)nmol + sizeof(int) * nwords;
     if (l->hio_cap < hio_need) {
         if (l->hio) HIP_CHECK(ctx, hipHostFree(l->hio));
         l->hio = nullptr;
@@ -2183,6 +2134,9 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     int *hpc = hblk, *rej = hblk + 16, *hflag = hblk + 9;
     const float *hres = (const float *)(hblk + 12);
     float *wtype = (float *)(hblk + 20 + 2 * nmt);
+    // the exact fdiff of every move and the acceptance bits, for fadd on the host
+    float *hfd = (float *)(hblk + 20 + 4 * nmt);
+    const unsigned *hbits = (const unsigned *)(hfd + nmol);
     std::vector<int> att(l->att);
     std::vector<float> wr(wholerot, wholerot + nmt), wt(wholetrans, wholetrans + nmt);
     for (int mt = 0; mt < nmt; ++mt) {
@@ -2204,13 +2158,14 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     // the ran1 stream of the cycle: drawn ahead by the previous call when it starts from
     // this state, else drawn now; both in pinned memory
     if (l->pin_cap < (size_t)total + 2 * nmt + 8) {
+        gen_join(l);
         if (l->ev_up) HIP_CHECK(ctx, hipEventSynchronize(l->ev_up));
-        for (int i = 0; i < 2; ++i) {
+        for (int i = 0; i < 3; ++i) {
             if (l->pin[i]) HIP_CHECK(ctx, hipHostFree(l->pin[i]));
             l->pin[i] = nullptr;
         }
         l->pin_cap = 0;
-        for (int i = 0; i < 2; ++i)
+        for (int i = 0; i < 3; ++i)
             HIP_CHECK(ctx, hipHostMalloc((void **)&l->pin[i], sizeof(float) * (total + 2 * nmt + 8)));
         l->pin_cap = (size_t)total + 2 * nmt + 8;
         l->ahead_valid = false;
@@ -2225,7 +2180,9 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
         ran1_load(rng, l->ahead_to);
         uploaded = l->ahead_uploaded;
     } else {
-        use = 1 - l->pin_ahead;
+        gen_join(l);   // a stream drawn for another state: unused
+        l->gen_total = 0;
+        use = 0;
         if (l->ev_up) HIP_CHECK(ctx, hipEventSynchronize(l->ev_up));   // no upload still reads the pinned buffers
         ran1_fill(rng, l->pin[use], total);
     }
@@ -2236,7 +2193,7 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
         HIP_CHECK(ctx, hipStreamSynchronize(st));
         if (l->ev_up) HIP_CHECK(ctx, hipEventSynchronize(l->ev_up));
         const size_t cap = (size_t)total + total / 8 + 64;
-        for (int i = 0; i < 2; ++i) {
+        for (int i = 0; i < 3; ++i) {
             int rc0;
             if ((rc0 = dev_alloc(ctx, &l->d_stream[i], cap))) return rc0;
         }
@@ -2290,7 +2247,8 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     // the cycle's counters and results, copied to the host in one transfer: [0..7] list
     // counts of passes 0..2, exact sums, passes of the finisher; [8] stage-1 rejections;
     // [9] moves that did not change (| TABLES_STALE); [10] largest |coordinate| of the old
-    // and trial positions; [12..15] the float results of mc_fadd; [16..] rejections per type
+    // and trial positions; [12..15] fadd and the acceptances of the one-workgroup resolution;
+    // [16..] rejections per type
     int *dblk = S.take<int>(16 + 2 * nmt);
     int *dpc = dblk, *dnrej = dblk + 8, *dnochange = dblk + 9, *drej = dblk + 16;
     unsigned *dmaxc = (unsigned *)(dblk + 10);
@@ -2392,14 +2350,17 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
                 dshift, maxc, dW0, dpc + 2, dW1, dstate, dpc + 3, dpc + 4);
             mc_state_bits<<<ceil_div((nmol + 31) / 32, 256), 256, 0, st>>>(nmol, dstate, dacc);
             timer_end(ctx, "mc_resolve");
-            // fadd only reports: its kernels run on the side stream beside the commits
+            // fadd only reports: the exact fdiff of the accepted moves is formed on the side
+            // stream beside the commits, and summed on the host (a sequential float sum is
+            // several times faster on a CPU core than on one GPU lane)
             hipStream_t side = nullptr;
             if ((rc1 = ctx_side(ctx, &side))) return rc1;
             HIP_CHECK(ctx, hipEventRecord(ctx->ev_fork, st));
             HIP_CHECK(ctx, hipStreamWaitEvent(side, ctx->ev_fork, 0));
             mc_fdiff_accepted<<<ceil_div(nmol, 4), 256, sizeof(float2) * 4 * (size_t)maxc, side>>>(
                 nmol, l->d_order, l->d_rowstart, l->d_cnt, l->d_pn, l->d_mw, dA, dB, dL, dacc, maxc, dfdiff);
-            mc_fadd<<<1, 1024, 0, side>>>(nmol, dacc, dfdiff, dres);
+            HIP_CHECK(ctx, hipMemcpyAsync(hfd, dfdiff, sizeof(float) * nmol, hipMemcpyDeviceToHost, side));
+            HIP_CHECK(ctx, hipMemcpyAsync((void *)hbits, dacc, sizeof(unsigned) * nwords, hipMemcpyDeviceToHost, side));
             HIP_CHECK(ctx, hipEventRecord(ctx->ev_join, side));
             joined = true;
         } else {
@@ -2424,12 +2385,25 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     };
     if ((rc = run(dstream, l->d_moff, l->d_roff, l->d_toff, dnochange))) return rc;
     // while the GPU runs the cycle: the next cycle's stream, from this one's end state
+    // (drawn by the thread the previous call started, if it started from this state), and
+    // a thread for the one after
     const Ran1 rend = rng;
     {
-        const int nxt = 1 - use;
-        Ran1 r2 = rend;
-        if (l->ev_up) HIP_CHECK(ctx, hipEventSynchronize(l->ev_up));
-        ran1_fill(r2, l->pin[nxt], total);
+        int32_t rend_a[35];
+        ran1_store(rend, rend_a);
+        gen_join(l);
+        int nxt;
+        Ran1 r2;
+        if (l->gen_total == total && l->gen_buf != use && std::memcmp(l->gen_from, rend_a, sizeof(rend_a)) == 0) {
+            nxt = l->gen_buf;
+            ran1_load(r2, l->gen_to);
+        } else {
+            nxt = (use + 1) % 3;
+            r2 = rend;
+            if (l->ev_up) HIP_CHECK(ctx, hipEventSynchronize(l->ev_up));
+            ran1_fill(r2, l->pin[nxt], total);
+        }
+        l->gen_total = 0;
         ran1_store(rend, l->ahead_from);
         ran1_store(r2, l->ahead_to);
         l->ahead_total = total;
@@ -2440,6 +2414,8 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
         if (!l->ev_up) HIP_CHECK(ctx, hipEventCreateWithFlags(&l->ev_up, hipEventDisableTiming));
         HIP_CHECK(ctx, hipMemcpyAsync(l->d_stream[nxt], l->pin[nxt], sizeof(float) * total, hipMemcpyHostToDevice, side));
         HIP_CHECK(ctx, hipEventRecord(l->ev_up, side));
+        // the third buffer: its last upload finished before this cycle's kernels started
+        gen_start(l, 3 - use - nxt, r2, total);
     }
     if ((rc = ctx_wait(ctx))) return rc;
     l->ahead_valid = *hflag == 0;
@@ -2559,11 +2535,23 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     }
     std::copy(wr.begin(), wr.end(), wholerot);
     std::copy(wt.begin(), wt.end(), wholetrans);
-    *fadds = *fadds + hres[0] / (float)nmol;
+    // fadd = fadd + fdiff over the accepted moves in chain order (molfit.py:405)
+    float fadd = hres[0];
+    int nacc = (int)hres[1];
+    if (fast) {
+        fadd = 0.0f;
+        nacc = 0;
+        for (int w = 0; w < nwords; ++w)
+            for (unsigned b = hbits[w]; b; b &= b - 1u) {   // the accepted moves of the word, in order
+                fadd = fadd + hfd[32 * w + __builtin_ctz(b)];
+                ++nacc;
+            }
+    }
+    *fadds = *fadds + fadd / (float)nmol;
     ran1_store(rng, ran1_state);
     if (stats) {
         stats->moves = nmol;
-        stats->accepted = (int)hres[1];
+        stats->accepted = nacc;
         stats->exact_resolutions = nexact;
         stats->pushed_entries = 0;
         stats->resolver_passes = npasses;

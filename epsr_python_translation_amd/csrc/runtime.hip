@@ -1,6 +1,8 @@
 // Context, box topology upload, scratch memory, kernel timing and the host ran1.
 #include <cstdarg>
 
+#include <algorithm>
+
 #include "common.h"
 
 int set_error(epsr_ctx *ctx, int code, const char *fmt, ...)
@@ -369,39 +371,64 @@ float ran1_next(Ran1 &s)
     return r;
 }
 
-// n values of the same stream with the state in registers: IA*idum mod (2^31 - 1) formed
-// with a 64-bit product folded at bit 31 (exact, as Schrage's factorisation is), and the
-// shuffle index iy/NDIV as iy >> 26 (NDIV = 2^26, iy >= 0)
+// a * x mod (2^31 - 1) for 0 < a, x < 2^31 - 1: the 62-bit product folded at bit 31
+// (2^31 = 1 mod 2^31 - 1); the sum is below 2 (2^31 - 1), and never a multiple of it
+static inline uint32_t mulmod_m31(uint32_t x, uint32_t a)
+{
+    const uint64_t p = (uint64_t)x * a;
+    uint32_t r = (uint32_t)(p & 0x7fffffffu) + (uint32_t)(p >> 31);
+    if (r >= 0x7fffffffu) r -= 0x7fffffffu;
+    return r;
+}
+
+// n values of the same stream.  IA*idum mod (2^31 - 1) is exact as Schrage's
+// factorisation is; the shuffle index iy/NDIV is iy >> 26 (NDIV = 2^26, iy >= 0).  In
+// blocks: the generator values first, eight chains apart (each steps by IA^8, so the
+// multiplications are independent), then the shuffle (a chain of table reads), then the
+// conversion to float.
 void ran1_fill(Ran1 &s, float *out, long long n)
 {
     const float AM = 4.656612873077393e-10f, RNMX = 0.9999998807907104f;
+    const uint32_t IA = 16807u;
+    uint32_t ia8 = 1u;
+    for (int k = 0; k < 8; ++k) ia8 = mulmod_m31(ia8, IA);
+    constexpr int BLOCK = 4096;
+    uint32_t v[BLOCK];
     long long i = 0;
     while (i < n) {
-        // the (re)seeding call, taken whenever idum <= 0 or iy == 0 as ran1 does
+        // the (re)seeding call, taken whenever idum <= 0 or iy == 0 as ran1 does; in a
+        // seeded state neither is ever reached again (idum stays in [1, 2^31 - 2], iy
+        // takes earlier idum values)
         if (s.idum <= 0 || s.iy == 0) {
             out[i++] = ran1_next(s);
             continue;
         }
-        const long long first = i;
-        uint32_t idum = (uint32_t)s.idum;
+        const int nb = (int)std::min<long long>(BLOCK, n - i);
+        uint32_t x = (uint32_t)s.idum, c[8];
+        for (int k = 0; k < 8 && k < nb; ++k) v[k] = c[k] = x = mulmod_m31(x, IA);
+        int k8 = 8;
+        for (; k8 + 8 <= nb; k8 += 8)   // the eight chains in registers
+            for (int q = 0; q < 8; ++q) v[k8 + q] = c[q] = mulmod_m31(c[q], ia8);
+        for (; k8 < nb; ++k8) v[k8] = mulmod_m31(v[k8 - 8], ia8);
+        const int32_t last = (int32_t)v[nb - 1];
         int32_t iy = s.iy;
         int32_t iv[32];
         for (int j = 0; j < 32; ++j) iv[j] = s.iv[j];
-        for (; i < n && iy != 0 && idum != 0u; ++i) {
-            const uint64_t x = (uint64_t)idum * 16807u;
-            uint32_t r = (uint32_t)(x & 0x7fffffffu) + (uint32_t)(x >> 31);
-            if (r >= 0x7fffffffu) r -= 0x7fffffffu;
-            idum = r;
+        for (int k = 0; k < nb; ++k) {
             const int j = iy >> 26;
             iy = iv[j];
-            iv[j] = (int32_t)idum;
-            const float f = AM * (float)iy;
-            out[i] = f > RNMX ? RNMX : f;
+            iv[j] = (int32_t)v[k];
+            v[k] = (uint32_t)iy;
         }
-        s.idum = (int32_t)idum;
+        for (int k = 0; k < nb; ++k) {
+            const float f = AM * (float)(int32_t)v[k];
+            out[i + k] = f > RNMX ? RNMX : f;
+        }
+        s.idum = last;
         s.iy = iy;
         for (int j = 0; j < 32; ++j) s.iv[j] = iv[j];
-        s.mcall = (int32_t)((uint32_t)s.mcall + (uint32_t)(i - first));
+        s.mcall = (int32_t)((uint32_t)s.mcall + (uint32_t)nb);
+        i += nb;
     }
 }
 

@@ -39,3 +39,24 @@ def test_host_ran1_of_the_library_matches_gfortran_golden():
                                   out.ctypes.data_as(C.POINTER(C.c_float))) == 0
         assert out.view(np.int32).tolist() == vals, head
         assert state.tolist() == [int(v) for v in st.split()], head
+
+
+def test_ran1_long_streams_match_the_oracle():
+    """The product's blocked host ran1 (generator values eight chains apart, then the
+    shuffle) against the oracle's one-value-at-a-time restatement over long streams, in
+    pieces of uneven length so that blocks start and end anywhere."""
+    from oracle import pyoracle as po
+    L = _native.lib()
+    O = po.lib()
+    for seed in (-1, -2718, -2147483647, 12345):
+        state = np.zeros(35, dtype=np.int32)
+        state[1] = seed
+        s = po.Ran1.seeded(seed)
+        for n in (1, 4095, 4096, 4097, 100000, 37, 140001):
+            out = np.empty(n, dtype=np.float32)
+            ref = np.empty(n, dtype=np.float32)
+            assert L.epsr_ran1_stream(state.ctypes.data_as(C.POINTER(C.c_int32)), n,
+                                      out.ctypes.data_as(C.POINTER(C.c_float))) == 0
+            O.ora_ran1_stream(C.byref(s), n, po.Pf(ref))
+            assert np.array_equal(out.view(np.int32), ref.view(np.int32)), (seed, n)
+            assert state.tolist() == s.as_array().tolist(), (seed, n)
