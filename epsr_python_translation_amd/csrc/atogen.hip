@@ -90,7 +90,8 @@ __global__ void atogen_intra(const float *__restrict__ xa, const float *__restri
 //    of them) take the exact path: correctly rounded sqrtf and division.
 //  * SMALL: every coordinate lies in [-0.74, 0.74] (checked on the device before the
 //    launch), so differences lie in (-1.5, 1.5) and NINT(x) is -1, 0 or 1.
-//  * POTEX: close-contact counting only when some rminex is non-zero.
+//  * potex (read on the device): close-contact counting only when some rminex is non-zero.
+//  * oatoms null: the origins are atoms 0..norigin-1 (every molecule but the last).
 __global__ void abs_max_coord(const float4 *__restrict__ P, int n, unsigned *__restrict__ out)
 {
     __shared__ float s[256];
@@ -108,58 +109,33 @@ __global__ void abs_max_coord(const float4 *__restrict__ P, int n, unsigned *__r
     if (threadIdx.x == 0) atomicMax(out, __float_as_uint(s[0]));   // non-negative floats order as uints
 }
 
+// |rminex(irefpcof(ic))| per pair, and out[1] = 1 when one is non-zero
+__global__ void atogen_rminex(int npairs, const int *__restrict__ irefpcof, const float *__restrict__ rminex,
+                              float *__restrict__ rabs, unsigned *__restrict__ out)
+{
+    const int ic = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ic >= npairs) return;
+    const float v = 1.0f * fabsf(rminex[irefpcof[ic]]);
+    rabs[ic] = v;
+    if (v > 0.0f) atomicOr(out + 1, 1u);
+}
+
 void launch_abs_max_coord(const float4 *P, int n, unsigned *out, hipStream_t st)
 {
     abs_max_coord<<<64, 256, 0, st>>>(P, n, out);
 }
 
-template <bool POTEX>
-__global__ void __launch_bounds__(ATOGEN_BLOCK)
-atogen_inter_fast(const float4 *__restrict__ P, const int *__restrict__ amol, const int *__restrict__ nal,
-                  const int *__restrict__ oatoms, int norigin, int full, int natom,
-                  const int *__restrict__ npairref_g, int ntype, int npairs, int npt1, int ldw, float stepcl,
-                  float inv_stepcl, float dmax, const float *__restrict__ rminex_abs,
-                  const unsigned *__restrict__ maxcoord, uint32_t *__restrict__ counts,
-                  uint32_t *__restrict__ potex_counts)
+// the tiles of partner atoms p0..p1e-1 against this thread's origin atom (p1), into the
+// block's histograms; every thread of the block takes part in the tile loads
+template <bool SMALL, bool POTEX>
+__device__ __forceinline__ void atogen_tiles(const float4 *__restrict__ P, const int *__restrict__ amol, int p0,
+                                             int p1e, bool active, int lo, int mymol, float4 p1, int rowoff, int ldh,
+                                             int npt1, float stepcl, float inv_stepcl, float dmax, const int *snp,
+                                             const float *srm, float4 *tileP, int *tileM, uint32_t *hist,
+                                             uint32_t *spot)
 {
-    extern __shared__ uint32_t smem[];
-    const int ldh = npt1 + 1;
-    uint32_t *hist = smem;
-    uint32_t *spot = hist + (size_t)npairs * ldh;
-    int *snp = (int *)(spot + npairs);
-    float *srm = (float *)(snp + ntype * ntype);
-    float4 *tileP = (float4 *)(((uintptr_t)(srm + npairs) + 15) & ~(uintptr_t)15);
-    int *tileM = (int *)(tileP + ATOGEN_TILE);
-
     const int tid = threadIdx.x;
-    for (int i = tid; i < npairs * ldh; i += ATOGEN_BLOCK) hist[i] = 0u;
-    for (int i = tid; i < npairs; i += ATOGEN_BLOCK) {
-        spot[i] = 0u;
-        srm[i] = rminex_abs[i];
-    }
-    for (int i = tid; i < ntype * ntype; i += ATOGEN_BLOCK) snp[i] = npairref_g[i];
-    const bool small = __uint_as_float(*maxcoord) <= SMALL_COORD_LIMIT;
-
-    const int oi = blockIdx.x * ATOGEN_BLOCK + tid;
-    const bool active = oi < norigin;
-    float4 p1 = make_float4(0.f, 0.f, 0.f, 0.f);
-    int mymol = -1, lo = natom, rowoff = 0;
-    if (active) {
-        int ia = oatoms[oi];
-        p1 = P[ia];
-        mymol = amol[ia];
-        lo = full ? nal[mymol] + 1 : 0;
-        rowoff = __float_as_int(p1.w) * ntype;
-    }
-    const int first = blockIdx.x * ATOGEN_BLOCK;
-    const int blo = full ? nal[amol[oatoms[first]]] + 1 : 0;
-    const int span = natom - blo;
-    const int per = (span + gridDim.y - 1) / gridDim.y;
-    const int p0 = blo + blockIdx.y * per;
-    const int p1e = min(natom, p0 + per);
     const float amb = 1.2e-6f;    // 10 ulp of q; the approximation is within 3 ulp
-    __syncthreads();
-
     for (int t0 = p0; t0 < p1e; t0 += ATOGEN_TILE) {
         int n = min(ATOGEN_TILE, p1e - t0);
         __syncthreads();
@@ -175,7 +151,7 @@ atogen_inter_fast(const float4 *__restrict__ P, const int *__restrict__ amol, co
             float4 p2 = tileP[k];
             if (tileM[k] == mymol) continue;
             float xx = p1.x - p2.x, yy = p1.y - p2.y, zz = p1.z - p2.z;
-            if (small) {
+            if (SMALL) {
                 xx = mic_small(xx);
                 yy = mic_small(yy);
                 zz = mic_small(zz);
@@ -202,6 +178,71 @@ atogen_inter_fast(const float4 *__restrict__ P, const int *__restrict__ amol, co
             if (POTEX && dtest * dmax < srm[icref]) atomicAdd(&spot[icref], 1u);
         }
     }
+}
+
+__global__ void __launch_bounds__(ATOGEN_BLOCK)
+atogen_inter_fast(const float4 *__restrict__ P, const int *__restrict__ amol, const int *__restrict__ nal,
+                  const int *__restrict__ oatoms, int norigin, int full, int natom,
+                  const int *__restrict__ npairref_g, int ntype, int npairs, int npt1, int ldw, float stepcl,
+                  float inv_stepcl, float dmax, const float *__restrict__ rminex_abs,
+                  const unsigned *__restrict__ maxcoord, uint32_t *__restrict__ counts,
+                  uint32_t *__restrict__ potex_counts)
+{
+    // maxcoord[1]: some |rminex| is non-zero
+    const bool potex = maxcoord[1] != 0u;
+    extern __shared__ uint32_t smem[];
+    const int ldh = npt1 + 1;
+    uint32_t *hist = smem;
+    uint32_t *spot = hist + (size_t)npairs * ldh;
+    int *snp = (int *)(spot + npairs);
+    float *srm = (float *)(snp + ntype * ntype);
+    float4 *tileP = (float4 *)(((uintptr_t)(srm + npairs) + 15) & ~(uintptr_t)15);
+    int *tileM = (int *)(tileP + ATOGEN_TILE);
+
+    const int tid = threadIdx.x;
+    for (int i = tid; i < npairs * ldh; i += ATOGEN_BLOCK) hist[i] = 0u;
+    for (int i = tid; i < npairs; i += ATOGEN_BLOCK) {
+        spot[i] = 0u;
+        srm[i] = rminex_abs[i];
+    }
+    for (int i = tid; i < ntype * ntype; i += ATOGEN_BLOCK) snp[i] = npairref_g[i];
+    const bool small = __uint_as_float(*maxcoord) <= SMALL_COORD_LIMIT;
+
+    const int oi = blockIdx.x * ATOGEN_BLOCK + tid;
+    const bool active = oi < norigin;
+    float4 p1 = make_float4(0.f, 0.f, 0.f, 0.f);
+    int mymol = -1, lo = natom, rowoff = 0;
+    if (active) {
+        int ia = oatoms ? oatoms[oi] : oi;
+        p1 = P[ia];
+        mymol = amol[ia];
+        lo = full ? nal[mymol] + 1 : 0;
+        rowoff = __float_as_int(p1.w) * ntype;
+    }
+    const int first = blockIdx.x * ATOGEN_BLOCK;
+    const int blo = full ? nal[amol[oatoms ? oatoms[first] : first]] + 1 : 0;
+    const int span = natom - blo;
+    const int per = (span + gridDim.y - 1) / gridDim.y;
+    const int p0 = blo + blockIdx.y * per;
+    const int p1e = min(natom, p0 + per);
+    __syncthreads();
+
+    // the pair loop, specialised on the two launch-wide flags
+    if (small) {
+        if (potex)
+            atogen_tiles<true, true>(P, amol, p0, p1e, active, lo, mymol, p1, rowoff, ldh, npt1, stepcl, inv_stepcl,
+                                     dmax, snp, srm, tileP, tileM, hist, spot);
+        else
+            atogen_tiles<true, false>(P, amol, p0, p1e, active, lo, mymol, p1, rowoff, ldh, npt1, stepcl, inv_stepcl,
+                                      dmax, snp, srm, tileP, tileM, hist, spot);
+    } else {
+        if (potex)
+            atogen_tiles<false, true>(P, amol, p0, p1e, active, lo, mymol, p1, rowoff, ldh, npt1, stepcl, inv_stepcl,
+                                      dmax, snp, srm, tileP, tileM, hist, spot);
+        else
+            atogen_tiles<false, false>(P, amol, p0, p1e, active, lo, mymol, p1, rowoff, ldh, npt1, stepcl,
+                                       inv_stepcl, dmax, snp, srm, tileP, tileM, hist, spot);
+    }
     __syncthreads();
     for (int i = tid; i < npairs * ldh; i += ATOGEN_BLOCK) {
         uint32_t v = hist[i];
@@ -210,7 +251,7 @@ atogen_inter_fast(const float4 *__restrict__ P, const int *__restrict__ amol, co
             atomicAdd(&counts[((size_t)npairs + ic) * ldw + iz], v);
         }
     }
-    if (POTEX)
+    if (potex)
         for (int i = tid; i < npairs; i += ATOGEN_BLOCK)
             if (spot[i]) atomicAdd(&potex_counts[i], spot[i]);
 }
@@ -283,16 +324,32 @@ extern "C" int epsr_atogen(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
     Ran1 rng;
     ran1_load(rng, ran1_state);
     std::vector<int> oatoms;
-    oatoms.reserve(full ? natom : (size_t)ngr * box->max_atoms_per_mol * 2);
-    for (int im = 0; im <= im2; ++im) {
-        int mt = box->h_mtype[im];
-        float r = ran1_next(rng);
-        if (r < probaccept || molsampled[mt] < 1 || molfraction[mt] < probaccept) {
-            molsampled[mt] += 1;
+    int norigin = 0;
+    // one ran1 value per candidate molecule, drawn as a block
+    std::vector<float> rv((size_t)im2 + 1);
+    ran1_fill(rng, rv.data(), (long long)im2 + 1);
+    if (full) {
+        // every candidate is taken (r < 1 <= probaccept): the origins are the atoms of
+        // molecules 0..im2, a contiguous range the kernel enumerates itself
+        for (int im = 0; im <= im2; ++im) {
+            molsampled[box->h_mtype[im]] += 1;
             for (int t = 0; t < ntype; ++t)
                 csampled[t] = csampled[t] + (float)box->h_type_count[(size_t)im * ntype + t];
-            for (int a = box->h_naf[im]; a <= box->h_nal[im]; ++a) oatoms.push_back(a);
         }
+        norigin = im2 >= 0 ? box->h_nal[im2] + 1 : 0;
+    } else {
+        oatoms.reserve((size_t)ngr * box->max_atoms_per_mol * 2);
+        for (int im = 0; im <= im2; ++im) {
+            int mt = box->h_mtype[im];
+            float r = rv[im];
+            if (r < probaccept || molsampled[mt] < 1 || molfraction[mt] < probaccept) {
+                molsampled[mt] += 1;
+                for (int t = 0; t < ntype; ++t)
+                    csampled[t] = csampled[t] + (float)box->h_type_count[(size_t)im * ntype + t];
+                for (int a = box->h_naf[im]; a <= box->h_nal[im]; ++a) oatoms.push_back(a);
+            }
+        }
+        norigin = (int)oatoms.size();
     }
     ran1_store(rng, ran1_state);
     *ngrsamples = ngr;
@@ -313,17 +370,16 @@ extern "C" int epsr_atogen(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
     }
 
     // ---- device scratch layout
-    const int norigin = (int)oatoms.size();
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
     size_t oP = take(sizeof(float4) * natom);
-    size_t oO = take(sizeof(int) * (norigin > 0 ? norigin : 1));
+    size_t oO = take(sizeof(int) * (oatoms.size() > 0 ? oatoms.size() : 1));
     size_t oC = counts_out ? 0 : take(sizeof(uint32_t) * 2 * (size_t)npairs * ldw);
     size_t oPC = take(sizeof(uint32_t) * npairs);
     size_t oN = take(sizeof(float) * 2 * npairs);
     size_t oF = take(sizeof(float) * npairs);
     size_t oR = take(sizeof(float) * npairs);
-    size_t oMX = take(sizeof(unsigned));
+    size_t oMX = take(2 * sizeof(unsigned));
     char *scr = (char *)ctx_scratch(ctx, off);
     if (!scr) return EPSR_ERR_HIP;
     float4 *dP = (float4 *)(scr + oP);
@@ -333,33 +389,24 @@ extern "C" int epsr_atogen(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
     float *dN = (float *)(scr + oN), *dF = (float *)(scr + oF), *dR = (float *)(scr + oR);
     unsigned *dMX = (unsigned *)(scr + oMX);
 
-    // staging of the small host arrays through pinned memory, one copy
-    size_t hb = sizeof(int) * (size_t)norigin + sizeof(float) * 4 * npairs;
+    // staging of the small host arrays through pinned memory
+    const int nlist = (int)oatoms.size();
+    size_t hb = sizeof(int) * (size_t)nlist + sizeof(float) * 4 * npairs;
     char *pin = (char *)ctx_pinned(ctx, hb);
     if (!pin) return EPSR_ERR_HIP;
     HIP_CHECK(ctx, hipStreamSynchronize(st));   // the pinned buffer may still feed a previous copy
-    memcpy(pin, oatoms.data(), sizeof(int) * (size_t)norigin);
-    float *pf = (float *)(pin + sizeof(int) * (size_t)norigin);
+    memcpy(pin, oatoms.data(), sizeof(int) * (size_t)nlist);
+    float *pf = (float *)(pin + sizeof(int) * (size_t)nlist);
     memcpy(pf, hnorm.data(), sizeof(float) * 2 * npairs);
     memcpy(pf + 2 * npairs, hcfj.data(), sizeof(float) * npairs);
-    if (norigin) HIP_CHECK(ctx, hipMemcpyAsync(dO, pin, sizeof(int) * (size_t)norigin, hipMemcpyHostToDevice, st));
+    if (nlist) HIP_CHECK(ctx, hipMemcpyAsync(dO, pin, sizeof(int) * (size_t)nlist, hipMemcpyHostToDevice, st));
     HIP_CHECK(ctx, hipMemcpyAsync(dN, pf, sizeof(float) * 2 * npairs, hipMemcpyHostToDevice, st));
     HIP_CHECK(ctx, hipMemcpyAsync(dF, pf + 2 * npairs, sizeof(float) * npairs, hipMemcpyHostToDevice, st));
 
-    // |rminex(irefpcof(ic))| per pair, gathered on the host side of the stream
-    {
-        std::vector<int> hir(npairs);
-        HIP_CHECK(ctx, hipMemcpyAsync(hir.data(), irefpcof, sizeof(int) * npairs, hipMemcpyDeviceToHost, st));
-        HIP_CHECK(ctx, hipStreamSynchronize(st));
-        int maxref = 0;
-        for (int v : hir) maxref = std::max(maxref, v + 1);
-        std::vector<float> hrm(maxref);
-        HIP_CHECK(ctx, hipMemcpyAsync(hrm.data(), rminex, sizeof(float) * maxref, hipMemcpyDeviceToHost, st));
-        HIP_CHECK(ctx, hipStreamSynchronize(st));
-        float *pr = pf + 3 * npairs;
-        for (int ic = 0; ic < npairs; ++ic) pr[ic] = 1.0f * fabsf(hrm[hir[ic]]);
-        HIP_CHECK(ctx, hipMemcpyAsync(dR, pr, sizeof(float) * npairs, hipMemcpyHostToDevice, st));
-    }
+    // |rminex(irefpcof(ic))| per pair, gathered on the device; dMX[1] says whether any is
+    // non-zero (the pair kernel then counts close contacts)
+    HIP_CHECK(ctx, hipMemsetAsync(dMX, 0, 2 * sizeof(unsigned), st));
+    atogen_rminex<<<ceil_div(npairs, 64), 64, 0, st>>>(npairs, irefpcof, rminex, dR, dMX);
 
     HIP_CHECK(ctx, hipMemsetAsync(dC, 0, sizeof(uint32_t) * 2 * (size_t)npairs * ldw, st));
     HIP_CHECK(ctx, hipMemsetAsync(dPC, 0, sizeof(uint32_t) * npairs, st));
@@ -385,26 +432,18 @@ extern "C" int epsr_atogen(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
         // partner slices so that the launch has the box's target of workgroups (a few per CU)
         int gy = std::max(1, std::min(256, ceil_div(box->atogen_workgroups, gx)));
         gy = std::min(gy, std::max(1, natom / ATOGEN_TILE));
-        HIP_CHECK(ctx, hipMemsetAsync(dMX, 0, sizeof(unsigned), st));
         abs_max_coord<<<64, 256, 0, st>>>(dP, natom, dMX);
-        bool potex_on = false;
-        for (int ic = 0; ic < npairs; ++ic) potex_on = potex_on || pf[3 * npairs + ic] > 0.0f;
         const float inv_stepcl = 1.0f / stepcl;
         timer_begin(ctx, "atogen_inter");
-        if (potex_on)
-            hipLaunchKernelGGL(atogen_inter_fast<true>, dim3(gx, gy), dim3(ATOGEN_BLOCK), lds, st, dP, box->d_amol,
-                               box->d_nal, dO, norigin, full ? 1 : 0, natom, box->d_npairref, ntype, npairs, npt1, ldw,
-                               stepcl, inv_stepcl, box->dmax, dR, dMX, dC, dPC);
-        else
-            hipLaunchKernelGGL(atogen_inter_fast<false>, dim3(gx, gy), dim3(ATOGEN_BLOCK), lds, st, dP, box->d_amol,
-                               box->d_nal, dO, norigin, full ? 1 : 0, natom, box->d_npairref, ntype, npairs, npt1, ldw,
-                               stepcl, inv_stepcl, box->dmax, dR, dMX, dC, dPC);
+        hipLaunchKernelGGL(atogen_inter_fast, dim3(gx, gy), dim3(ATOGEN_BLOCK), lds, st, dP, box->d_amol, box->d_nal,
+                           nlist ? dO : nullptr, norigin, full ? 1 : 0, natom, box->d_npairref, ntype, npairs, npt1,
+                           ldw, stepcl, inv_stepcl, box->dmax, dR, dMX, dC, dPC);
         timer_end(ctx, "atogen_inter");
     }
     int total = 2 * npairs * npt1;
     atogen_normalise<<<ceil_div(std::max(total, npairs), 256), 256, 0, st>>>(dC, dPC, dN, dF, ycoh, npairs, npt1, ldw,
                                                                               addwav, wav, potex, potexmax);
+    // no synchronisation: the outputs are device arrays, ordered on the stream
     HIP_CHECK(ctx, hipGetLastError());
-    HIP_CHECK(ctx, hipStreamSynchronize(st));
     return EPSR_OK;
 }

@@ -85,30 +85,27 @@ struct epsr_ledger {
     // and Metropolis values (for the stepr/stepm flags in plan_flags), attempts per type
     bool plan_ready = false;
     int plan_flags = 0, plan_total = 0;
-    std::vector<int> h_roff, h_toff, h_moff, att, type_nat;
+    std::vector<int> att, type_nat;
     int *d_roff = nullptr, *d_toff = nullptr, *d_moff = nullptr;
-    // pinned ran1 streams: the one a cycle uploads, and the next cycle's, drawn on the host
-    // while the GPU runs the cycle (valid when the next call starts from ahead_from)
+    // pinned ran1 windows: pin[b] holds win_len[b] values of the stream (0: none) and
+    // win_ck[b] the state at every 256th of them, from the window's start; a cycle reads its
+    // stream in the window where its ran1 state lies.  pin_ahead: the window the previous
+    // call uploaded to d_stream[pin_ahead] (complete when ahead_uploaded, behind ev_up)
     float *pin[3] = {nullptr, nullptr, nullptr};
     size_t pin_cap = 0;
-    // pinned block of a cycle's small transfers (results down, step sizes up)
-    int *hio = nullptr;
-    size_t hio_cap = 0;
-    int pin_ahead = 0;
-    bool ahead_valid = false;
-    int ahead_total = 0;
-    int32_t ahead_from[35], ahead_to[35];
-    // device copies of the two streams; the next cycle's is uploaded on the side stream
-    // while the current cycle runs (ahead_uploaded), behind the event ev_up
+    int win_len[3] = {0, 0, 0};
+    std::vector<int32_t> win_ck[3];
+    int pin_ahead = -1;
     float *d_stream[3] = {nullptr, nullptr, nullptr};
     size_t dstream_cap = 0;
     bool ahead_uploaded = false;
     hipEvent_t ev_up = nullptr;
-    // a host thread drawing the stream of the cycle after next into pin[gen_buf], from
-    // the state gen_from (gen_total values; 0: none), ending in gen_to
+    // a host thread drawing window gen_buf (gen_len values) for the cycle after next
     std::thread gen;
-    int gen_buf = -1, gen_total = 0;
-    int32_t gen_from[35], gen_to[35];
+    int gen_buf = -1, gen_len = 0;
+    // pinned block of a cycle's small transfers (results down, step sizes up)
+    int *hio = nullptr;
+    size_t hio_cap = 0;
     bool ledger_ready = false;
     int maxcnt = 0;   // the longest list
     // table extents measured for the arrays in tab_ptr (xstar, xavs, xavs1, ecoul, ecoul1)

@@ -1212,7 +1212,6 @@ extern "C" int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coord
     l->maxcnt = maxcnt;
     l->nself = nself;
     l->plan_ready = false;
-    l->ahead_valid = false;
     l->npairent = npair;
     l->nslots = nrec;
     l->nentries = nent;
@@ -2023,19 +2022,89 @@ static float adapt_ratio(float rejrate, int att, int rej)
     return rat;
 }
 
-// the stream of the cycle after next, drawn by a host thread while this call runs
-static void gen_start(epsr_ledger *l, int buf, const Ran1 &from, int total)
+// ---- ran1 windows.  A window holds len values of the stream from a state, with the
+// state at every 256th value; a cycle whose ran1 state lies in a window reads its stream
+// there, at that offset (other consumers, such as atogen's origin draws, may have moved
+// the state on since the window was drawn).
+static void fill_window(Ran1 from, float *out, int len, std::vector<int32_t> &ck)
+{
+    ck.assign(35 * ((size_t)len / 256 + 1), 0);
+    for (int o = 0;; o += 256) {
+        ran1_store(from, &ck[35 * ((size_t)o / 256)]);
+        if (o >= len) break;
+        const int n = std::min(256, len - o);
+        ran1_fill(from, out + o, n);
+        if (n < 256) break;
+    }
+}
+
+// the state at offset o of window b
+static void window_state(const epsr_ledger *l, int b, int o, Ran1 &out)
+{
+    ran1_load(out, &l->win_ck[b][35 * ((size_t)o / 256)]);
+    for (int k = 0; k < o % 256; ++k) (void)ran1_next(out);
+}
+
+// the offset of state in window b with room for need values after it, or -1
+static int window_offset(const epsr_ledger *l, int b, const int32_t *state, int need)
+{
+    if (b < 0 || l->win_len[b] < need) return -1;
+    const uint32_t o = (uint32_t)state[0] - (uint32_t)l->win_ck[b][0];   // calls made since its start
+    if (o > (uint32_t)(l->win_len[b] - need)) return -1;
+    Ran1 r;
+    window_state(l, b, (int)o, r);
+    int32_t a[35];
+    ran1_store(r, a);
+    return std::memcmp(a, state, sizeof(a)) == 0 ? (int)o : -1;
+}
+
+// a host thread draws window buf (len values from `from`); the window is usable once
+// gen_finish has joined it
+static void gen_start(epsr_ledger *l, int buf, const Ran1 &from, int len)
 {
     l->gen_buf = buf;
-    l->gen_total = total;
-    ran1_store(from, l->gen_from);
+    l->gen_len = len;
+    l->win_len[buf] = 0;
     float *out = l->pin[buf];
-    int32_t *to = l->gen_to;
-    l->gen = std::thread([from, out, total, to]() {
-        Ran1 r = from;
-        ran1_fill(r, out, total);
-        ran1_store(r, to);
-    });
+    std::vector<int32_t> *ck = &l->win_ck[buf];
+    l->gen = std::thread([from, out, len, ck]() { fill_window(from, out, len, *ck); });
+}
+
+static void gen_finish(epsr_ledger *l)
+{
+    gen_join(l);
+    if (l->gen_buf >= 0) l->win_len[l->gen_buf] = l->gen_len;
+    l->gen_buf = -1;
+}
+
+// the plan of the chain (molfit.py:367-403) on the device.  flags: 1 stepr > 0, 2 stepm > 0.
+// Values per move: 3 for a rotation (molecules of more than one atom), 3 for a translation,
+// 1 for the Metropolis test
+__global__ void mc_plan_counts(int nmol, const int *__restrict__ order, const int *__restrict__ naf,
+                               const int *__restrict__ nal, int flags, int *__restrict__ cnt)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nmol) return;
+    const int m = order[k];
+    const bool rot = (flags & 1) && nal[m] > naf[m];
+    cnt[k] = (rot ? 3 : 0) + ((flags & 2) ? 3 : 0) + 1;
+}
+
+// offsets from the exclusive sums in base (overwritten by moff): roff -1 without
+// rotation, -3 for a single atom when stepr > 0; toff -1 without translation
+__global__ void mc_plan_offsets(int nmol, const int *__restrict__ order, const int *__restrict__ naf,
+                                const int *__restrict__ nal, int flags, int *__restrict__ roff,
+                                int *__restrict__ toff, int *__restrict__ base_moff)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nmol) return;
+    const int m = order[k];
+    const bool rot = (flags & 1) && nal[m] > naf[m];
+    const int b = base_moff[k];
+    roff[k] = rot ? b : ((flags & 1) ? -3 : -1);
+    const int t = b + (rot ? 3 : 0);
+    toff[k] = (flags & 2) ? t : -1;
+    base_moff[k] = t + ((flags & 2) ? 3 : 0);
 }
 
 static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *c, const epsr_tables_t *tb,
@@ -2076,44 +2145,37 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     // values; it depends on the chain order and the stepr/stepm flags only
     const int flags = (p->stepr > 0.0f ? 1 : 0) | (p->stepm > 0.0f ? 2 : 0);
     if (!l->plan_ready || l->plan_flags != flags) {
-        l->h_roff.assign(nmol, -1);
-        l->h_toff.assign(nmol, -1);
-        l->h_moff.assign(nmol, 0);
+        // attempts per type and the stream length do not depend on the chain order
         l->att.assign(2 * nmt, 0);
         l->type_nat.assign(nmt, 0);
         int cur = 0;
-        for (int k = 0; k < nmol; ++k) {
-            const int m = l->order[k], mt = box->h_mtype[m];
+        for (int m = 0; m < nmol; ++m) {
+            const int mt = box->h_mtype[m];
             const int nat = box->h_nal[m] - box->h_naf[m] + 1;
             l->type_nat[mt] = nat;
             if (p->stepr > 0.0f && nat > 1) {
                 l->att[2 * mt] += 1;
-                l->h_roff[k] = cur;
                 cur += 3;
-            } else if (p->stepr > 0.0f) {
-                l->h_roff[k] = -3;   // no rotation for a single atom; counted only if stepr > 0
             }
             if (p->stepm > 0.0f) {
                 l->att[2 * mt + 1] += 1;
-                l->h_toff[k] = cur;
                 cur += 3;
             }
-            l->h_moff[k] = cur;
             cur += 1;
         }
         l->plan_total = cur;
+        // the offsets do: values per move, their exclusive sum over the chain, the offsets
         int rc0;
         if (!l->d_roff && ((rc0 = dev_alloc(ctx, &l->d_roff, nmol)) || (rc0 = dev_alloc(ctx, &l->d_toff, nmol)) ||
                            (rc0 = dev_alloc(ctx, &l->d_moff, nmol))))
             return rc0;
-        // stream-ordered uploads: the host vectors stay unchanged until the next plan,
-        // which follows a synchronisation
-        HIP_CHECK(ctx, hipMemcpyAsync(l->d_roff, l->h_roff.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, st));
-        HIP_CHECK(ctx, hipMemcpyAsync(l->d_toff, l->h_toff.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, st));
-        HIP_CHECK(ctx, hipMemcpyAsync(l->d_moff, l->h_moff.data(), sizeof(int) * nmol, hipMemcpyHostToDevice, st));
+        mc_plan_counts<<<ceil_div(nmol, 256), 256, 0, st>>>(nmol, l->d_order, box->d_naf, box->d_nal, flags,
+                                                            l->d_toff);
+        if ((rc0 = ledger_scan<int>(ctx, l, l->d_toff, l->d_moff, nmol))) return rc0;
+        mc_plan_offsets<<<ceil_div(nmol, 256), 256, 0, st>>>(nmol, l->d_order, box->d_naf, box->d_nal, flags,
+                                                             l->d_roff, l->d_toff, l->d_moff);
         l->plan_flags = flags;
         l->plan_ready = true;
-        l->ahead_valid = false;
     }
     // step sizes of every type for this cycle: the first move of a type fixes a zero
     // step (molfit.py:375-392), and every move of the type then uses it
@@ -2157,42 +2219,54 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     int total = l->plan_total;
     // the ran1 stream of the cycle: drawn ahead by the previous call when it starts from
     // this state, else drawn now; both in pinned memory
-    if (l->pin_cap < (size_t)total + 2 * nmt + 8) {
+    // windows of total + extra values: room for atogen's origin draws (one per molecule)
+    // between two cycles
+    const int wlen = total + nmol + 256;
+    if (l->pin_cap < (size_t)wlen + 2 * nmt + 8) {
         gen_join(l);
+        l->gen_buf = -1;
         if (l->ev_up) HIP_CHECK(ctx, hipEventSynchronize(l->ev_up));
         for (int i = 0; i < 3; ++i) {
             if (l->pin[i]) HIP_CHECK(ctx, hipHostFree(l->pin[i]));
             l->pin[i] = nullptr;
+            l->win_len[i] = 0;
         }
         l->pin_cap = 0;
+        l->pin_ahead = -1;
         for (int i = 0; i < 3; ++i)
-            HIP_CHECK(ctx, hipHostMalloc((void **)&l->pin[i], sizeof(float) * (total + 2 * nmt + 8)));
-        l->pin_cap = (size_t)total + 2 * nmt + 8;
-        l->ahead_valid = false;
+            HIP_CHECK(ctx, hipHostMalloc((void **)&l->pin[i], sizeof(float) * ((size_t)wlen + 2 * nmt + 8)));
+        l->pin_cap = (size_t)wlen + 2 * nmt + 8;
     }
     Ran1 rng;
     ran1_load(rng, ran1_state);
     const Ran1 rng0 = rng;
-    int use;
-    bool uploaded = false;   // already in l->d_stream[use], behind the event ev_up
-    if (l->ahead_valid && l->ahead_total == total && std::memcmp(l->ahead_from, ran1_state, sizeof(l->ahead_from)) == 0) {
-        use = l->pin_ahead;
-        ran1_load(rng, l->ahead_to);
-        uploaded = l->ahead_uploaded;
-    } else {
-        gen_join(l);   // a stream drawn for another state: unused
-        l->gen_total = 0;
-        use = 0;
-        if (l->ev_up) HIP_CHECK(ctx, hipEventSynchronize(l->ev_up));   // no upload still reads the pinned buffers
-        ran1_fill(rng, l->pin[use], total);
+    // the cycle's stream: in the window the previous call uploaded when ran1_state lies in
+    // it, else in a window drawn now
+    int use = -1, off = 0;
+    bool uploaded = false;   // the window is in l->d_stream[use], behind the event ev_up
+    {
+        const int o = window_offset(l, l->pin_ahead, ran1_state, total);
+        if (o >= 0) {
+            use = l->pin_ahead;
+            off = o;
+            uploaded = l->ahead_uploaded;
+        }
     }
-    l->ahead_valid = false;
+    if (use < 0) {
+        gen_finish(l);
+        use = l->pin_ahead == 0 ? 1 : 0;
+        if (l->ev_up) HIP_CHECK(ctx, hipEventSynchronize(l->ev_up));   // no upload still reads the pinned buffers
+        fill_window(rng, l->pin[use], wlen, l->win_ck[use]);
+        l->win_len[use] = wlen;
+    }
+    l->pin_ahead = -1;
     l->ahead_uploaded = false;
-    // the device copies of the two streams
-    if (l->dstream_cap < (size_t)total + 8) {
+    window_state(l, use, off + total, rng);   // the state after the cycle's stream
+    // the device copies of the windows
+    if (l->dstream_cap < (size_t)wlen + 8) {
         HIP_CHECK(ctx, hipStreamSynchronize(st));
         if (l->ev_up) HIP_CHECK(ctx, hipEventSynchronize(l->ev_up));
-        const size_t cap = (size_t)total + total / 8 + 64;
+        const size_t cap = (size_t)wlen + wlen / 8 + 64;
         for (int i = 0; i < 3; ++i) {
             int rc0;
             if ((rc0 = dev_alloc(ctx, &l->d_stream[i], cap))) return rc0;
@@ -2230,7 +2304,7 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     int *dtmp = S.take<int>(8);
     // the ran1 stream of the cycle (in the ledger) and, for more than STEP_TYPES molecule
     // types, the step sizes of every type (else they go by value to mc_trial)
-    float *dstream = l->d_stream[use];
+    float *dstream = l->d_stream[use] + off;
     float *dwtype = S.take<float>(2 * nmt);
     float *txa = S.take<float>(natom), *tya = S.take<float>(natom), *tza = S.take<float>(natom);
     float *txadd = S.take<float>(nmol), *tyadd = S.take<float>(nmol), *tzadd = S.take<float>(nmol);
@@ -2281,11 +2355,11 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     if (uploaded) {
         HIP_CHECK(ctx, hipStreamWaitEvent(st, l->ev_up, 0));
     } else {
-        HIP_CHECK(ctx, hipMemcpyAsync(dstream, l->pin[use], sizeof(float) * total, hipMemcpyHostToDevice, st));
+        HIP_CHECK(ctx, hipMemcpyAsync(dstream, l->pin[use] + off, sizeof(float) * total, hipMemcpyHostToDevice, st));
     }
     if (!steps_by_value) {
-        std::memcpy(l->pin[use] + total, wtype, sizeof(float) * 2 * nmt);
-        HIP_CHECK(ctx, hipMemcpyAsync(dwtype, l->pin[use] + total, sizeof(float) * 2 * nmt, hipMemcpyHostToDevice, st));
+        std::memcpy(l->pin[use] + wlen, wtype, sizeof(float) * 2 * nmt);
+        HIP_CHECK(ctx, hipMemcpyAsync(dwtype, l->pin[use] + wlen, sizeof(float) * 2 * nmt, hipMemcpyHostToDevice, st));
     }
     HIP_CHECK(ctx, hipMemsetAsync(dblk + 8, 0, 4 * sizeof(int), st));
     mc_trial<<<ceil_div(nmol, 128), 128, 0, st>>>(nmol, l->d_order, box->d_naf, box->d_nal, c->xa, c->ya, c->za, c->xadd,
@@ -2385,41 +2459,43 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     };
     if ((rc = run(dstream, l->d_moff, l->d_roff, l->d_toff, dnochange))) return rc;
     // while the GPU runs the cycle: the next cycle's stream, from this one's end state
-    // (drawn by the thread the previous call started, if it started from this state), and
-    // a thread for the one after
+    // (the window the thread of the previous call drew, when this cycle's end state lies
+    // in it), and a thread for the window after, from the next cycle's expected end
     const Ran1 rend = rng;
     {
         int32_t rend_a[35];
         ran1_store(rend, rend_a);
-        gen_join(l);
-        int nxt;
-        Ran1 r2;
-        if (l->gen_total == total && l->gen_buf != use && std::memcmp(l->gen_from, rend_a, sizeof(rend_a)) == 0) {
-            nxt = l->gen_buf;
-            ran1_load(r2, l->gen_to);
-        } else {
-            nxt = (use + 1) % 3;
-            r2 = rend;
-            if (l->ev_up) HIP_CHECK(ctx, hipEventSynchronize(l->ev_up));
-            ran1_fill(r2, l->pin[nxt], total);
+        const int gb = l->gen_buf;
+        gen_finish(l);
+        int nxt = -1, o2 = 0;
+        if (gb >= 0 && gb != use) {
+            const int o = window_offset(l, gb, rend_a, total);
+            if (o >= 0) {
+                nxt = gb;
+                o2 = o;
+            }
         }
-        l->gen_total = 0;
-        ran1_store(rend, l->ahead_from);
-        ran1_store(r2, l->ahead_to);
-        l->ahead_total = total;
-        l->pin_ahead = nxt;
-        // and its upload, beside this cycle's kernels (the buffer is not this cycle's)
+        if (nxt < 0) {
+            nxt = (use + 1) % 3;
+            if (l->ev_up) HIP_CHECK(ctx, hipEventSynchronize(l->ev_up));
+            fill_window(rend, l->pin[nxt], wlen, l->win_ck[nxt]);
+            l->win_len[nxt] = wlen;
+        }
+        // its upload, beside this cycle's kernels (the buffer is not this cycle's)
         hipStream_t side = nullptr;
         if ((rc = ctx_side(ctx, &side))) return rc;
         if (!l->ev_up) HIP_CHECK(ctx, hipEventCreateWithFlags(&l->ev_up, hipEventDisableTiming));
-        HIP_CHECK(ctx, hipMemcpyAsync(l->d_stream[nxt], l->pin[nxt], sizeof(float) * total, hipMemcpyHostToDevice, side));
+        HIP_CHECK(ctx, hipMemcpyAsync(l->d_stream[nxt], l->pin[nxt], sizeof(float) * l->win_len[nxt],
+                                      hipMemcpyHostToDevice, side));
         HIP_CHECK(ctx, hipEventRecord(l->ev_up, side));
-        // the third buffer: its last upload finished before this cycle's kernels started
-        gen_start(l, 3 - use - nxt, r2, total);
+        l->pin_ahead = nxt;
+        // the third buffer (its last upload finished before this cycle's kernels started)
+        Ran1 r3;
+        window_state(l, nxt, o2 + total, r3);
+        gen_start(l, 3 - use - nxt, r3, wlen);
     }
     if ((rc = ctx_wait(ctx))) return rc;
-    l->ahead_valid = *hflag == 0;
-    l->ahead_uploaded = l->ahead_valid;
+    l->ahead_uploaded = true;
     if (*hflag & TABLES_STALE) {
         // the tables changed in place since their extents were measured; nothing was
         // committed and no state was updated: measure them and run the cycle again
