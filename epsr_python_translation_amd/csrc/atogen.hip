@@ -88,46 +88,23 @@ __global__ void atogen_intra(const float *__restrict__ xa, const float *__restri
 //    correctly rounded quotient q; NINT(q) can only differ from NINT of the approximation
 //    when q lies within that distance of a half-integer, and those pairs (about 1e-6 * q
 //    of them) take the exact path: correctly rounded sqrtf and division.
-//  * SMALL: every coordinate lies in [-0.74, 0.74] (checked on the device before the
-//    launch), so differences lie in (-1.5, 1.5) and NINT(x) is -1, 0 or 1.
+//  * the minimum-image components enter only the squared distance (mic_sq).
 //  * potex (read on the device): close-contact counting only when some rminex is non-zero.
 //  * oatoms null: the origins are atoms 0..norigin-1 (every molecule but the last).
-__global__ void abs_max_coord(const float4 *__restrict__ P, int n, unsigned *__restrict__ out)
-{
-    __shared__ float s[256];
-    float m = 0.0f;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        float4 p = P[i];
-        m = fmaxf(m, fmaxf(fabsf(p.x), fmaxf(fabsf(p.y), fabsf(p.z))));
-    }
-    s[threadIdx.x] = m;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) s[threadIdx.x] = fmaxf(s[threadIdx.x], s[threadIdx.x + w]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) atomicMax(out, __float_as_uint(s[0]));   // non-negative floats order as uints
-}
-
-// |rminex(irefpcof(ic))| per pair, and out[1] = 1 when one is non-zero
+// |rminex(irefpcof(ic))| per pair, and *flag = 1 when one is non-zero
 __global__ void atogen_rminex(int npairs, const int *__restrict__ irefpcof, const float *__restrict__ rminex,
-                              float *__restrict__ rabs, unsigned *__restrict__ out)
+                              float *__restrict__ rabs, unsigned *__restrict__ flag)
 {
     const int ic = blockIdx.x * blockDim.x + threadIdx.x;
     if (ic >= npairs) return;
     const float v = 1.0f * fabsf(rminex[irefpcof[ic]]);
     rabs[ic] = v;
-    if (v > 0.0f) atomicOr(out + 1, 1u);
-}
-
-void launch_abs_max_coord(const float4 *P, int n, unsigned *out, hipStream_t st)
-{
-    abs_max_coord<<<64, 256, 0, st>>>(P, n, out);
+    if (v > 0.0f) atomicOr(flag, 1u);
 }
 
 // the tiles of partner atoms p0..p1e-1 against this thread's origin atom (p1), into the
 // block's histograms; every thread of the block takes part in the tile loads
-template <bool SMALL, bool POTEX>
+template <bool POTEX>
 __device__ __forceinline__ void atogen_tiles(const float4 *__restrict__ P, const int *__restrict__ amol, int p0,
                                              int p1e, bool active, int lo, int mymol, float4 p1, int rowoff, int ldh,
                                              int npt1, float stepcl, float inv_stepcl, float dmax, const int *snp,
@@ -150,24 +127,16 @@ __device__ __forceinline__ void atogen_tiles(const float4 *__restrict__ P, const
         for (int k = kstart; k < n; ++k) {
             float4 p2 = tileP[k];
             if (tileM[k] == mymol) continue;
-            float xx = p1.x - p2.x, yy = p1.y - p2.y, zz = p1.z - p2.z;
-            if (SMALL) {
-                xx = mic_small(xx);
-                yy = mic_small(yy);
-                zz = mic_small(zz);
-            } else {
-                xx = dev_mic(xx);
-                yy = dev_mic(yy);
-                zz = dev_mic(zz);
-            }
+            const float xx = mic_sq(p1.x - p2.x), yy = mic_sq(p1.y - p2.y), zz = mic_sq(p1.z - p2.z);
             float dsq = xx * xx + yy * yy + zz * zz;
-            float qa = __builtin_amdgcn_sqrtf(dsq) * inv_stepcl;
-            float tq = truncf(qa);
-            float fr = qa - tq;
+            const float qa = __builtin_amdgcn_sqrtf(dsq) * inv_stepcl;
+            const float rq = rintf(qa);
             int iz1;
             float dtest = 0.0f;
-            if (fabsf(fr - 0.5f) > amb * qa + 1.0e-30f) {
-                iz1 = (int)tq + (fr > 0.5f ? 1 : 0) + 2;
+            // away from half-integers (by more than amb q) rint(q) is NINT of the exact
+            // quotient; the bound is conservative, so fmaf may form it
+            if (fabsf(qa - rq) < fmaf(-amb, qa, 0.5f)) {
+                iz1 = (int)rq + 2;
                 if (POTEX) dtest = sqrtf(dsq);
             } else {
                 dtest = sqrtf(dsq);
@@ -185,11 +154,10 @@ atogen_inter_fast(const float4 *__restrict__ P, const int *__restrict__ amol, co
                   const int *__restrict__ oatoms, int norigin, int full, int natom,
                   const int *__restrict__ npairref_g, int ntype, int npairs, int npt1, int ldw, float stepcl,
                   float inv_stepcl, float dmax, const float *__restrict__ rminex_abs,
-                  const unsigned *__restrict__ maxcoord, uint32_t *__restrict__ counts,
+                  const unsigned *__restrict__ potex_flag, uint32_t *__restrict__ counts,
                   uint32_t *__restrict__ potex_counts)
 {
-    // maxcoord[1]: some |rminex| is non-zero
-    const bool potex = maxcoord[1] != 0u;
+    const bool potex = *potex_flag != 0u;   // some |rminex| is non-zero
     extern __shared__ uint32_t smem[];
     const int ldh = npt1 + 1;
     uint32_t *hist = smem;
@@ -206,7 +174,6 @@ atogen_inter_fast(const float4 *__restrict__ P, const int *__restrict__ amol, co
         srm[i] = rminex_abs[i];
     }
     for (int i = tid; i < ntype * ntype; i += ATOGEN_BLOCK) snp[i] = npairref_g[i];
-    const bool small = __uint_as_float(*maxcoord) <= SMALL_COORD_LIMIT;
 
     const int oi = blockIdx.x * ATOGEN_BLOCK + tid;
     const bool active = oi < norigin;
@@ -227,22 +194,13 @@ atogen_inter_fast(const float4 *__restrict__ P, const int *__restrict__ amol, co
     const int p1e = min(natom, p0 + per);
     __syncthreads();
 
-    // the pair loop, specialised on the two launch-wide flags
-    if (small) {
-        if (potex)
-            atogen_tiles<true, true>(P, amol, p0, p1e, active, lo, mymol, p1, rowoff, ldh, npt1, stepcl, inv_stepcl,
-                                     dmax, snp, srm, tileP, tileM, hist, spot);
-        else
-            atogen_tiles<true, false>(P, amol, p0, p1e, active, lo, mymol, p1, rowoff, ldh, npt1, stepcl, inv_stepcl,
-                                      dmax, snp, srm, tileP, tileM, hist, spot);
-    } else {
-        if (potex)
-            atogen_tiles<false, true>(P, amol, p0, p1e, active, lo, mymol, p1, rowoff, ldh, npt1, stepcl, inv_stepcl,
-                                      dmax, snp, srm, tileP, tileM, hist, spot);
-        else
-            atogen_tiles<false, false>(P, amol, p0, p1e, active, lo, mymol, p1, rowoff, ldh, npt1, stepcl,
-                                       inv_stepcl, dmax, snp, srm, tileP, tileM, hist, spot);
-    }
+    // the pair loop, specialised on the launch-wide flag
+    if (potex)
+        atogen_tiles<true>(P, amol, p0, p1e, active, lo, mymol, p1, rowoff, ldh, npt1, stepcl, inv_stepcl, dmax, snp,
+                           srm, tileP, tileM, hist, spot);
+    else
+        atogen_tiles<false>(P, amol, p0, p1e, active, lo, mymol, p1, rowoff, ldh, npt1, stepcl, inv_stepcl, dmax,
+                            snp, srm, tileP, tileM, hist, spot);
     __syncthreads();
     for (int i = tid; i < npairs * ldh; i += ATOGEN_BLOCK) {
         uint32_t v = hist[i];
@@ -379,7 +337,7 @@ extern "C" int epsr_atogen(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
     size_t oN = take(sizeof(float) * 2 * npairs);
     size_t oF = take(sizeof(float) * npairs);
     size_t oR = take(sizeof(float) * npairs);
-    size_t oMX = take(2 * sizeof(unsigned));
+    size_t oMX = take(sizeof(unsigned));
     char *scr = (char *)ctx_scratch(ctx, off);
     if (!scr) return EPSR_ERR_HIP;
     float4 *dP = (float4 *)(scr + oP);
@@ -403,9 +361,9 @@ extern "C" int epsr_atogen(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
     HIP_CHECK(ctx, hipMemcpyAsync(dN, pf, sizeof(float) * 2 * npairs, hipMemcpyHostToDevice, st));
     HIP_CHECK(ctx, hipMemcpyAsync(dF, pf + 2 * npairs, sizeof(float) * npairs, hipMemcpyHostToDevice, st));
 
-    // |rminex(irefpcof(ic))| per pair, gathered on the device; dMX[1] says whether any is
+    // |rminex(irefpcof(ic))| per pair, gathered on the device; dMX says whether any is
     // non-zero (the pair kernel then counts close contacts)
-    HIP_CHECK(ctx, hipMemsetAsync(dMX, 0, 2 * sizeof(unsigned), st));
+    HIP_CHECK(ctx, hipMemsetAsync(dMX, 0, sizeof(unsigned), st));
     atogen_rminex<<<ceil_div(npairs, 64), 64, 0, st>>>(npairs, irefpcof, rminex, dR, dMX);
 
     HIP_CHECK(ctx, hipMemsetAsync(dC, 0, sizeof(uint32_t) * 2 * (size_t)npairs * ldw, st));
@@ -432,7 +390,6 @@ extern "C" int epsr_atogen(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
         // partner slices so that the launch has the box's target of workgroups (a few per CU)
         int gy = std::max(1, std::min(256, ceil_div(box->atogen_workgroups, gx)));
         gy = std::min(gy, std::max(1, natom / ATOGEN_TILE));
-        abs_max_coord<<<64, 256, 0, st>>>(dP, natom, dMX);
         const float inv_stepcl = 1.0f / stepcl;
         timer_begin(ctx, "atogen_inter");
         hipLaunchKernelGGL(atogen_inter_fast, dim3(gx, gy), dim3(ATOGEN_BLOCK), lds, st, dP, box->d_amol, box->d_nal,

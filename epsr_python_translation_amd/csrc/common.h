@@ -240,21 +240,10 @@ __device__ __forceinline__ float dev_anint(float x)
 // minimagec, cubic branch (src/epsr26/library/minimagec.py:48-62)
 __device__ __forceinline__ float dev_mic(float x) { return x - dev_anint(x); }
 
-// dev_mic for |x| < 1.5, where NINT(x) is -1, 0 or 1: the same integer is subtracted, so
-// the result is the same float
-__device__ __forceinline__ float mic_small(float x)
-{
-    float a = x >= 0.5f ? 1.0f : 0.0f;
-    float b = x <= -0.5f ? 1.0f : 0.0f;
-    return x - (a - b);
-}
-
-// coordinates of every atom within [-0.74, 0.74] (so differences lie in (-1.5, 1.5)),
-// from the maximum |coordinate| that launch_abs_max_coord leaves as float bits
-#define SMALL_COORD_LIMIT 0.74f
-
-// *out = max(*out, max |x|, |y|, |z| of P[0..n)) as float bits; *out starts at 0
-void launch_abs_max_coord(const float4 *P, int n, unsigned *out, hipStream_t st);
+// dev_mic(x) where only its square is used: x - rint(x).  NINT (half away from zero) and
+// rint (half to even) differ only at half-integers, where the two differences are +0.5
+// and -0.5 (both exact), with the same square; elsewhere the two are the same float.
+__device__ __forceinline__ float mic_sq(float x) { return x - rintf(x); }
 
 // IFIX (fmath.py:252-260) for arguments known to lie in [0, 2^31)
 __device__ __forceinline__ int dev_ifix_pos(float x) { return (int)truncf(x); }

@@ -298,26 +298,16 @@ __device__ __forceinline__ float sqrt_rn(float x)
     return rup > 0.0f ? sup : r;
 }
 
-// en_atom_inter_scale (molfit.py:1250-1279) with scalefac = 1.  SMALL: every coordinate
-// lies within +-SMALL_COORD_LIMIT (mic_small gives dev_mic's result).  The bin is
+// en_atom_inter_scale (molfit.py:1250-1279) with scalefac = 1; the minimum-image
+// components enter only the squared distance (mic_sq).  The bin is
 // trunc(dtest / stepcl) + 2 with the correctly rounded quotient Q: q = dtest * inv_stepcl
 // lies within 1.2e-7 q of dtest / stepcl (two roundings of 2^-24) and Q within 6e-8 Q of
 // it, so when q's fraction keeps clear of 0 and 1 by 1.2e-6 q, trunc(q) = trunc(Q);
 // otherwise (about 2.4e-6 q of the pairs) the division is made.
-template <bool SMALL>
 __device__ __forceinline__ float atom_pair(float x1, float y1, float z1, float x2, float y2, float z2, int ic,
                                            const PairTab &t)
 {
-    float xd, yd, zd;
-    if (SMALL) {
-        xd = mic_small(x2 - x1);
-        yd = mic_small(y2 - y1);
-        zd = mic_small(z2 - z1);
-    } else {
-        xd = dev_mic(x2 - x1);
-        yd = dev_mic(y2 - y1);
-        zd = dev_mic(z2 - z1);
-    }
+    const float xd = mic_sq(x2 - x1), yd = mic_sq(y2 - y1), zd = mic_sq(z2 - z1);
     float dsq = xd * xd + yd * yd + zd * zd;
     float dtest = sqrt_rn(dsq);
     const float q = dtest * t.inv_stepcl;
@@ -347,9 +337,8 @@ __device__ __forceinline__ float atom_pair(float x1, float y1, float z1, float x
 }
 
 // en_mole_pair (molfit.py:1527-1578): sequential sum, molecule 1 outer, molecule 2 inner
-template <bool SMALL>
-__device__ __forceinline__ float mol_pair_t(const float4 *__restrict__ P1, int na1, const float4 *__restrict__ P2,
-                                            int nb2, const PairTab &t)
+__device__ __forceinline__ float mol_pair(const float4 *__restrict__ P1, int na1, const float4 *__restrict__ P2,
+                                          int nb2, const PairTab &t)
 {
     float sume = 0.0f;
     for (int ia = 0; ia < na1; ++ia) {
@@ -357,26 +346,13 @@ __device__ __forceinline__ float mol_pair_t(const float4 *__restrict__ P1, int n
         int it = __float_as_int(p.w) * t.ntype;
         for (int ja = 0; ja < nb2; ++ja) {
             float4 q = P2[ja];
-            sume = sume + atom_pair<SMALL>(p.x, p.y, p.z, q.x, q.y, q.z, t.npr[it + __float_as_int(q.w)], t);
+            sume = sume + atom_pair(p.x, p.y, p.z, q.x, q.y, q.z, t.npr[it + __float_as_int(q.w)], t);
         }
     }
     return sume;
 }
 
-// small: every coordinate of both molecules lies within +-SMALL_COORD_LIMIT
-__device__ __forceinline__ float mol_pair(const float4 *__restrict__ P1, int na1, const float4 *__restrict__ P2,
-                                          int nb2, const PairTab &t, bool small = false)
-{
-    return small ? mol_pair_t<true>(P1, na1, P2, nb2, t) : mol_pair_t<false>(P1, na1, P2, nb2, t);
-}
-
-// the flag above from the float bits launch_abs_max_coord leaves (null: unknown)
-__device__ __forceinline__ bool coords_small(const unsigned *maxcoord)
-{
-    return maxcoord && __uint_as_float(*maxcoord) <= SMALL_COORD_LIMIT;
-}
-
-// mol_pair_t<true> for molecules of at most three atoms: the six positions are loaded
+// mol_pair for molecules of at most three atoms: the six positions are loaded
 // before any pair is evaluated; the sum runs in the same order
 __device__ __forceinline__ float mol_pair3(const float4 *__restrict__ P1, int n1, const float4 *__restrict__ P2,
                                            int n2, const PairTab &t)
@@ -395,11 +371,18 @@ __device__ __forceinline__ float mol_pair3(const float4 *__restrict__ P1, int n1
 #pragma unroll
         for (int ja = 0; ja < 3; ++ja) {
             if (ja >= n2) break;
-            sume = sume + atom_pair<true>(p[ia].x, p[ia].y, p[ia].z, q[ja].x, q[ja].y, q[ja].z,
-                                          t.npr[it + __float_as_int(q[ja].w)], t);
+            sume = sume + atom_pair(p[ia].x, p[ia].y, p[ia].z, q[ja].x, q[ja].y, q[ja].z,
+                                    t.npr[it + __float_as_int(q[ja].w)], t);
         }
     }
     return sume;
+}
+
+// the pair energy of two molecules, through mol_pair3 when neither has more than three atoms
+__device__ __forceinline__ float mol_pair_any(const float4 *__restrict__ P1, int na1, const float4 *__restrict__ P2,
+                                              int nb2, const PairTab &t)
+{
+    return (na1 <= 3 && nb2 <= 3) ? mol_pair3(P1, na1, P2, nb2, t) : mol_pair(P1, na1, P2, nb2, t);
 }
 
 // T (near bins) then F (far bins) in one buffer
@@ -448,19 +431,16 @@ __global__ void abs_positions(const float *__restrict__ xa, const float *__restr
     P[a] = make_float4(xa[a] + xadd[m], ya[a] + yadd[m], za[a] + zadd[m], __int_as_float(itype[a]));
 }
 
-// abs_positions of the current (Pold) and trial (Pnew) coordinates in one launch, and the
-// largest |coordinate| of both into *maxcoord (float bits; starts at 0)
+// abs_positions of the current (Pold) and trial (Pnew) coordinates in one launch
 __global__ void __launch_bounds__(256)
 abs_positions2(int natom, const float *__restrict__ xa, const float *__restrict__ ya, const float *__restrict__ za,
                const float *__restrict__ xadd, const float *__restrict__ yadd, const float *__restrict__ zadd,
                const float *__restrict__ txa, const float *__restrict__ tya, const float *__restrict__ tza,
                const float *__restrict__ txadd, const float *__restrict__ tyadd, const float *__restrict__ tzadd,
                const int *__restrict__ amol, const int *__restrict__ itype, float4 *__restrict__ Pold,
-               float4 *__restrict__ Pnew, unsigned *__restrict__ maxcoord)
+               float4 *__restrict__ Pnew)
 {
-    __shared__ float sm[4];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    float mx = 0.0f;
     if (t < 2 * natom) {
         const bool trial = t >= natom;
         const int a = trial ? t - natom : t;
@@ -470,14 +450,6 @@ abs_positions2(int natom, const float *__restrict__ xa, const float *__restrict_
                                : make_float4(xa[a] + xadd[m], ya[a] + yadd[m], za[a] + zadd[m],
                                              __int_as_float(itype[a]));
         (trial ? Pnew : Pold)[a] = p;
-        mx = fmaxf(fabsf(p.x), fmaxf(fabsf(p.y), fabsf(p.z)));
-    }
-    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_down(mx, off));
-    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = mx;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        mx = fmaxf(fmaxf(sm[0], sm[1]), fmaxf(sm[2], sm[3]));
-        atomicMax(maxcoord, __float_as_uint(mx));   // non-negative floats order as their bits
     }
 }
 
@@ -661,12 +633,10 @@ __global__ void fmolec1_pairs(const long long *__restrict__ rowoff, int nmol, co
                               const int *__restrict__ mcell, const int *__restrict__ cstart,
                               const int *__restrict__ nbflat, const float4 *__restrict__ P,
                               const int *__restrict__ naf, const int *__restrict__ nal, PairTab gt, int npairs,
-                              const unsigned *__restrict__ maxcoord, float *__restrict__ rowE, int *__restrict__ rowJ,
-                              int *__restrict__ rowcnt)
+                              float *__restrict__ rowE, int *__restrict__ rowJ, int *__restrict__ rowcnt)
 {
     extern __shared__ char smem[];
     PairTab t = LDS ? tables_to_lds(gt, npairs, smem) : gt;
-    const bool small = coords_small(maxcoord);
     const int lane = threadIdx.x & 63;
     const int nw = gridDim.x * (blockDim.x >> 6);
     for (int k = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); k < nmol; k += nw) {
@@ -683,7 +653,7 @@ __global__ void fmolec1_pairs(const long long *__restrict__ rowoff, int nmol, co
             int jm = 0;
             if (inn < len) {
                 jm = nb[inn];
-                if (inn >= 1 && jm > im) e = mol_pair(P1, n1, P + naf[jm], nal[jm] - naf[jm] + 1, t, small);
+                if (inn >= 1 && jm > im) e = mol_pair_any(P1, n1, P + naf[jm], nal[jm] - naf[jm] + 1, t);
             }
             const bool keep = e != 0.0f;
             const unsigned long long mask = __ballot(keep);
@@ -1061,7 +1031,7 @@ extern "C" int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coord
     const int *dmcell = dmic + (size_t)ncell * l->maxcellno;
     int *dnbcell = cw, *dnbcnt = dnbcell + ncn, *dnbpre = dnbcnt + ncn;
     int *drowlen = dnbpre + (size_t)ncell * (nbr + 1), *dflatlen = drowlen + ncell + 1, *dcstart = dflatlen + ncell + 1;
-    // [0] longest list, [1..2] table extents, [3] largest |coordinate|, [4..8] report, [10..11] entry counts
+    // [0] longest list, [1..2] table extents, [4..8] report, [10..11] entry counts
     int *dtail = dcstart + ncell + 1;
     cell_neighbours<<<(unsigned)ceil_div(ncn, 256), 256, 0, st>>>(ncell, nbr, l->na, l->nb, l->nc, doff, doff + nbr,
                                                                   doff + 2 * nbr, dnmc, dnbcell, dnbcnt);
@@ -1109,21 +1079,18 @@ extern "C" int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coord
     l->tab_packed = true;
     abs_positions<<<ceil_div(box->natom, 256), 256, 0, st>>>(c->xa, c->ya, c->za, c->xadd, c->yadd, c->zadd,
                                                              box->d_amol, box->d_itype, dP, box->natom);
-    HIP_CHECK(ctx, hipMemsetAsync(dtail + 3, 0, sizeof(int), st));
-    launch_abs_max_coord(dP, box->natom, (unsigned *)(dtail + 3), st);
     {
         const int B = box->pair_block;
         timer_begin(ctx, "fmolec1_pair");
         if (pt.in_lds) {
             int g = persistent_blocks(pt.lds_bytes, (long long)nmol * 64, B);
             fmolec1_pairs<true><<<g, B, pt.lds_bytes, st>>>(drow, nmol, l->d_order, dmcell, dcstart, dnb, dP,
-                                                            box->d_naf, box->d_nal, pt.tab, box->npairs,
-                                                            (const unsigned *)(dtail + 3), rowE, rowJ, cat + nmol);
+                                                            box->d_naf, box->d_nal, pt.tab, box->npairs, rowE, rowJ,
+                                                            cat + nmol);
         } else {
             int g = persistent_blocks(0, (long long)nmol * 64, B);
             fmolec1_pairs<false><<<g, B, 0, st>>>(drow, nmol, l->d_order, dmcell, dcstart, dnb, dP, box->d_naf,
-                                                  box->d_nal, pt.tab, box->npairs, (const unsigned *)(dtail + 3), rowE,
-                                                  rowJ, cat + nmol);
+                                                  box->d_nal, pt.tab, box->npairs, rowE, rowJ, cat + nmol);
         }
         timer_end(ctx, "fmolec1_pair");
     }
@@ -1446,12 +1413,11 @@ template <bool LDS>
 __global__ void mc_pairs(int nslots, int nbv, const int *__restrict__ bvslot, const int4 *__restrict__ srec,
                          const int *__restrict__ entry, const int *__restrict__ pos, const float *__restrict__ ergpair,
                          const float4 *__restrict__ Pold, const float4 *__restrict__ Pnew, PairTab gt, int npairs,
-                         const unsigned *__restrict__ maxcoord, const float *__restrict__ shift,
-                         float *__restrict__ A, float *__restrict__ Bv, float *__restrict__ L)
+                         const float *__restrict__ shift, float *__restrict__ A, float *__restrict__ Bv,
+                         float *__restrict__ L)
 {
     extern __shared__ char smem[];
     PairTab t = LDS ? tables_to_lds(gt, npairs, smem) : gt;
-    const bool small = coords_small(maxcoord);
     // items [0, nslots): A (and L) of every slot; items [nslots, nslots + nbv): Bv of the
     // slots whose partner moves earlier in the chain (bvslot, from the ledger), so every
     // lane of a wave evaluates one molecule pair.  srec[i] holds the slot's molecule (bit
@@ -1467,7 +1433,7 @@ __global__ void mc_pairs(int nslots, int nbv, const int *__restrict__ bvslot, co
         if (shift && isnan(shift[pos[r.x & 0x7fffffff]])) continue;   // rejected by the surrogate stage
         const float4 *P1 = Pnew + (r.z & 0xffffff), *P2 = (bv ? Pnew : Pold) + (r.w & 0xffffff);
         const int n1 = (int)((unsigned)r.z >> 24), n2 = (int)((unsigned)r.w >> 24);
-        const float e = (small && n1 <= 3 && n2 <= 3) ? mol_pair3(P1, n1, P2, n2, t) : mol_pair(P1, n1, P2, n2, t, small);
+        const float e = mol_pair_any(P1, n1, P2, n2, t);
         if (bv) Bv[i] = e;
         else A[i] = e;
     }
@@ -2320,12 +2286,11 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     int *dW0 = S.take<int>(nmol), *dW1 = S.take<int>(nmol);
     // the cycle's counters and results, copied to the host in one transfer: [0..7] list
     // counts of passes 0..2, exact sums, passes of the finisher; [8] stage-1 rejections;
-    // [9] moves that did not change (| TABLES_STALE); [10] largest |coordinate| of the old
-    // and trial positions; [12..15] fadd and the acceptances of the one-workgroup resolution;
+    // [9] moves that did not change (| TABLES_STALE); [12..15] fadd and the acceptances of
+    // the one-workgroup resolution;
     // [16..] rejections per type
     int *dblk = S.take<int>(16 + 2 * nmt);
     int *dpc = dblk, *dnrej = dblk + 8, *dnochange = dblk + 9, *drej = dblk + 16;
-    unsigned *dmaxc = (unsigned *)(dblk + 10);
     float *dres = (float *)(dblk + 12);
     ARG_CHECK(ctx, !da || fast, "epsr_molfit_da: delayed acceptance needs the parallel resolution (LDS too small)");
 
@@ -2375,7 +2340,7 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
         bool joined = false;
         abs_positions2<<<ceil_div(2 * (long)natom, 256), 256, 0, st>>>(
             natom, c->xa, c->ya, c->za, c->xadd, c->yadd, c->zadd, txa, tya, tza, txadd, tyadd, tzadd, box->d_amol,
-            box->d_itype, Pold, Pnew, dmaxc);
+            box->d_itype, Pold, Pnew);
         if (want_x) {
             uint16_t *X = (rec && rec->x) ? rec->x : dX;
             if ((rc1 = surr_descriptors_launch(ctx, box, l, Pold, Pnew, dpar, ldx, X, nullptr))) return rc1;
@@ -2389,12 +2354,12 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
             int g = persistent_blocks(pt.lds_bytes, (long long)nslots + l->nbv, box->pair_block);
             mc_pairs<true><<<g, box->pair_block, pt.lds_bytes, st>>>(
                 nslots, l->nbv, l->d_bvslot, l->d_srec, l->d_entry, l->d_pos, l->d_ergpair, Pold, Pnew, pt.tab,
-                box->npairs, dmaxc, dshift, dA, dB, dL);
+                box->npairs, dshift, dA, dB, dL);
         } else {
             int g = persistent_blocks(0, (long long)nslots + l->nbv, box->pair_block);
             mc_pairs<false><<<g, box->pair_block, 0, st>>>(
                 nslots, l->nbv, l->d_bvslot, l->d_srec, l->d_entry, l->d_pos, l->d_ergpair, Pold, Pnew, pt.tab,
-                box->npairs, dmaxc, dshift, dA, dB, dL);
+                box->npairs, dshift, dA, dB, dL);
         }
         timer_end(ctx, "mc_pair");
         if (rec && rec->target)

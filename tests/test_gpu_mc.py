@@ -228,3 +228,51 @@ def test_molfit_after_tables_changed_in_place(ctx, change):
     assert np.array_equal(a["wr"].view(np.int32), b["wr"].view(np.int32))
     assert np.array_equal(a["wt"].view(np.int32), b["wt"].view(np.int32))
     assert a["fadds"].view(np.int32) == b["fadds"].view(np.int32)
+
+
+@pytest.mark.parametrize("gap", [1, 599, 5000])
+def test_molfit_after_other_ran1_draws(ctx, gap):
+    """Cycles separated by `gap` draws of another ran1 consumer (atogen draws one value per
+    molecule) read their streams where the caller's state lies: within the window the
+    previous cycle uploaded (gap up to nmol + 256), else in a window drawn afresh; the
+    chain equals the oracle's with the same draws between cycles."""
+    from epsr_python_translation_amd import _native
+    from epsr_python_translation_amd import device as D
+    name = "c0"
+    s = H.system(name)
+    idum = -4242
+    print(f"molfit with {gap} other draws between cycles", flush=True)
+    ref = H.oracle_fmolec1(name, "cr")
+    ob = ref["ob"]
+    wr, wt = H.initial_steps(s)
+    nwh = np.zeros(2 * s.nmoltype, np.int32)
+    nmv = np.zeros(2 * s.nmoltype, np.int32)
+    nsr = np.zeros(s.nmol, np.int32)
+    omc = po.MCState(po.Pf(wr), po.Pf(wt), po.Pi(nwh), po.Pi(nmv), po.Pi(nsr), s.simt, F32(1), F32(1), F32(0.75), 0)
+    rng = po.Ran1.seeded(idum)
+    fadds = C.c_float(0.0)
+    nacc = C.c_int()
+    junk = np.empty(gap, np.float32)
+    L = po.lib("cr")
+    box, tab, led, _, _ = _gpu_setup(ctx, name)
+    gwr, gwt = H.initial_steps(s)
+    mc = D.MCState(ctx, s.nmol, s.nmoltype, s.simt, wholerot=gwr, wholetrans=gwt)
+    st = D.ran1_state_array(idum)
+    lib = _native.lib()
+    for cycle in range(3):
+        if cycle:
+            L.ora_ran1_stream(C.byref(rng), gap, po.Pf(junk))
+            assert lib.epsr_ran1_stream(st.ctypes.data_as(C.POINTER(C.c_int32)), gap,
+                                        junk.ctypes.data_as(C.POINTER(C.c_float))) == 0
+        L.ora_molfit(C.byref(ob.box), po.Pf(ob.xa), po.Pf(ob.ya), po.Pf(ob.za), C.byref(ob.centres),
+                     C.byref(ref["ctab"]), C.byref(ref["cells"]), C.byref(ref["ledger"]), C.byref(omc), C.byref(rng),
+                     C.byref(fadds), C.byref(nacc))
+        D.molfit(ctx, box, tab, led, mc, st)
+    torch.cuda.synchronize()
+    co = box.get_coords()
+    for k in ("xa", "ya", "za", "xadd", "yadd", "zadd"):
+        assert np.array_equal(co[k].view(np.int32), getattr(ob, k).view(np.int32)), k
+    np.testing.assert_array_equal(st, rng.as_array())
+    assert np.array_equal(led.export()["ergpair"].view(np.int32),
+                          H.ledger_dict(ref["ledger"], s.nmol)["ergpair"].view(np.int32))
+    assert F32(mc.fadds).view(np.int32) == F32(fadds.value).view(np.int32)
