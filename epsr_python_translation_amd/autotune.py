@@ -14,7 +14,7 @@ import torch
 
 from . import _native
 
-ATOGEN_WORKGROUPS = (256, 512, 1024, 2048, 4096)
+ATOGEN_WORKGROUPS = (1024, 2048, 4096, 8192, 16384)
 PAIR_BLOCKS = (256, 512, 1024)
 
 _CACHE: dict = {}
