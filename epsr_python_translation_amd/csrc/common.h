@@ -66,10 +66,10 @@ struct epsr_ledger {
     epsr_ctx *ctx = nullptr;
     const epsr_box *box = nullptr;
     int nmol = 0;
-    // cells (host)
-    int na = 0, nb = 0, nc = 0, ncell = 0, ncellneighb = 0, maxcellno = 0, nnbrefmax = 0;
-    std::vector<int> nmolsincell, molsincell, molcellno, iaoff, iboff, icoff;
-    std::vector<int> order, pos;
+    // cells: the lattice and neighbour offsets on the host; the cell of every molecule,
+    // the chain order (d_order) and chain positions (d_pos) on the device
+    int na = 0, nb = 0, nc = 0, ncell = 0, ncellneighb = 0, nnbrefmax = 0;
+    std::vector<int> iaoff, iboff, icoff;
     bool cells_ready = false;
     // ledger (device)
     int nself = 0, npairent = 0, nslots = 0, nentries = 0;
@@ -123,10 +123,14 @@ struct epsr_ledger {
     // neighbour offsets were formed for this lattice size and rmax
     int nbr_na = 0;
     float nbr_rmax = -1.0f;
-    // device copy of the cell arrays of cells_init, its pinned staging buffer and a small
-    // pinned block for fmolec1's scalars
+    // the cell arrays of cells_init ([offsets a, b, c][molecules per cell][start of each
+    // cell in the chain order, ncell + 2][cell of each molecule]), the pinned staging
+    // buffer of the offsets' upload (ev_stage: after it) and a small pinned block for
+    // fmolec1's scalars
     int *d_cells = nullptr;
     size_t cap_cells = 0;
+    bool off_uploaded = false;
+    hipEvent_t ev_stage = nullptr;
     void *hstage = nullptr;
     size_t hstage_cap = 0;
     int *hsc = nullptr;

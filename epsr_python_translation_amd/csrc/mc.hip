@@ -80,6 +80,7 @@ extern "C" void epsr_ledger_destroy(epsr_ledger *l)
     hipFree(l->d_mw);
     hipFree(l->d_srec);
     hipFree(l->d_tab);
+    if (l->ev_stage) hipEventDestroy(l->ev_stage);
     for (void *w : l->wbuf) hipFree(w);
     delete l;
 }
@@ -110,14 +111,6 @@ __host__ __device__ static inline int fold(int ix, int n)
     return ix;
 }
 
-// cellneighbourlist.py:309-330, cubic lattice in box fractions
-static int h_cellcoord(float x, int n)
-{
-    float x1 = x - (float)h_nint(x);
-    int ix = h_ifix((x1 + 0.5f) * (float)n);
-    return fold(ix, n);
-}
-
 // grow-only pinned staging buffer of the ledger; a stream synchronisation first, since
 // copies from the old buffer may still be in flight
 static int ledger_stage(epsr_ctx *ctx, epsr_ledger *l, size_t bytes)
@@ -133,6 +126,93 @@ static int ledger_stage(epsr_ctx *ctx, epsr_ledger *l, size_t bytes)
     return EPSR_OK;
 }
 
+// grow-only device work buffer `i` of the ledger (kept between calls)
+static int ledger_work(epsr_ctx *ctx, epsr_ledger *l, int i, size_t bytes, void **out)
+{
+    if (bytes > l->wcap[i]) {
+        if (l->wbuf[i]) {
+            HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+            HIP_CHECK(ctx, hipFree(l->wbuf[i]));
+        }
+        l->wbuf[i] = nullptr;
+        l->wcap[i] = 0;
+        const size_t want = bytes + bytes / 8 + 4096;
+        HIP_CHECK(ctx, hipMalloc(&l->wbuf[i], want));
+        l->wcap[i] = want;
+    }
+    *out = l->wbuf[i];
+    return EPSR_OK;
+}
+
+enum { LW_CELL = 0, LW_ROW = 1, LW_TMP = 2, LW_REC = 3, LW_SUM = 4 };
+
+// exclusive sums with the ledger's temporary storage
+template <class T>
+static int ledger_scan(epsr_ctx *ctx, epsr_ledger *l, const T *in, T *out, int n)
+{
+    size_t tb = 0;
+    HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, n, ctx->stream));
+    void *tmp = nullptr;
+    int rc = ledger_work(ctx, l, LW_TMP, tb + 16, &tmp);
+    if (rc) return rc;
+    HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, tb, in, out, n, ctx->stream));
+    return EPSR_OK;
+}
+
+// device half of the cell functions: the cell of every molecule (atomcellno,
+// cellneighbourlist.py:317-330 with the cubic lattice: fc = z, fb = y - fc*0,
+// fa = x - fb*0 - fc*0 in box fractions; each coordinate x - NINT(x), then
+// IFIX((x1 + 0.5) n) folded into [0, n)), and the identity as the sort's values
+__device__ static int d_cellcoord(float x, int n)
+{
+    // NINT from the double value with the host's 64-bit wrap (h_nint), IFIX (h_ifix)
+    const double xd = (double)x;
+    long long ti = 0;
+    if (xd >= -9.2233720368547758e18 && xd < 9.2233720368547758e18) {
+        const double t = trunc(xd), f = xd - t;
+        ti = (long long)t;
+        if (f >= 0.5) ti += 1;
+        else if (f <= -0.5) ti -= 1;
+    }
+    const float x1 = x - (float)(int)(unsigned)(unsigned long long)ti;
+    const double v = (double)((x1 + 0.5f) * (float)n);
+    const int ix = (v > -2147483649.0 && v < 2147483648.0) ? (int)trunc(v) : (int)0x80000000u;
+    return fold(ix, n);
+}
+
+__global__ void cell_assign(int nmol, const float *__restrict__ xadd, const float *__restrict__ yadd,
+                            const float *__restrict__ zadd, int na, int nb, int nc, int *__restrict__ mcell,
+                            int *__restrict__ ident)
+{
+    const int im = blockIdx.x * blockDim.x + threadIdx.x;
+    if (im >= nmol) return;
+    const float fc = zadd[im] / 1.0f;
+    const float fb = (yadd[im] - fc * 0.0f) / 1.0f;
+    const float fa = (xadd[im] - fb * 0.0f - fc * 0.0f) / 1.0f;
+    mcell[im] = d_cellcoord(fa, na) + na * (d_cellcoord(fb, nb) + nb * d_cellcoord(fc, nc));
+    ident[im] = im;
+}
+
+// from the molecules sorted by cell (stable, so each cell keeps increasing molecule
+// numbers, as molsincell lists them): each cell's start in the chain (cstart, ncell + 1)
+// and count (nmc), and every molecule's chain position
+__global__ void cell_bounds(const int *__restrict__ skey, int nmol, int ncell, int *__restrict__ cstart)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > nmol) return;
+    const int lo = i == 0 ? 0 : skey[i - 1] + 1;
+    const int hi = i == nmol ? ncell : skey[i];
+    for (int cc = lo; cc <= hi; ++cc) cstart[cc] = i;
+}
+
+__global__ void cell_counts(const int *__restrict__ cstart, int ncell, const int *__restrict__ order, int nmol,
+                            int *__restrict__ nmc, int *__restrict__ pos)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ncell) nmc[t] = cstart[t + 1] - cstart[t];
+    if (t < nmol) pos[order[t]] = t;
+}
+
 extern "C" int epsr_cells_init(epsr_ctx *ctx, const epsr_box *box, epsr_ledger *l, const epsr_coords_t *c,
                                int nmolcellin, float rmax)
 {
@@ -142,41 +222,13 @@ extern "C" int epsr_cells_init(epsr_ctx *ctx, const epsr_box *box, epsr_ledger *
     const int nmol = box->nmol;
     hipStream_t st = ctx->stream;
     int rc;
-    // the centres come down through the pinned staging buffer (the previous call's uploads
-    // from it are ordered before these copies on the stream)
-    if ((rc = ledger_stage(ctx, l, sizeof(float) * 3 * (size_t)nmol))) return rc;
-    const float *xadd = (const float *)l->hstage, *yadd = xadd + nmol, *zadd = yadd + nmol;
-    HIP_CHECK(ctx, hipMemcpyAsync((void *)xadd, c->xadd, sizeof(float) * nmol, hipMemcpyDeviceToHost, st));
-    HIP_CHECK(ctx, hipMemcpyAsync((void *)yadd, c->yadd, sizeof(float) * nmol, hipMemcpyDeviceToHost, st));
-    HIP_CHECK(ctx, hipMemcpyAsync((void *)zadd, c->zadd, sizeof(float) * nmol, hipMemcpyDeviceToHost, st));
-    HIP_CHECK(ctx, hipStreamSynchronize(st));
-    // initialise_cellneighbourlist (:23-96)
+    // initialise_cellneighbourlist (:23-96): the lattice depends on nmol alone
     int nmolcell = std::min(nmolcellin, nmol);
     int ncelldiv = h_nint(powf((float)nmol / (float)nmolcell, 1.0f / 3.0f));
     if (ncelldiv < 1) ncelldiv = 1;
     float cellsize = 2.0f * 1.0f / (float)ncelldiv;
     l->na = l->nb = l->nc = 2 * h_ifix(1.0f / cellsize) + 1;
     l->ncell = l->na * l->nb * l->nc;
-    l->nmolsincell.assign(l->ncell, 0);
-    l->molcellno.assign(nmol, 0);
-    l->maxcellno = 0;
-    for (int im = 0; im < nmol; ++im) {
-        // atomcellno (:317-330): fc = z/clatz, fb = (y - fc*claty)/blaty, fa = ...
-        float fc = zadd[im] / 1.0f;
-        float fb = (yadd[im] - fc * 0.0f) / 1.0f;
-        float fa = (xadd[im] - fb * 0.0f - fc * 0.0f) / 1.0f;
-        int ia = h_cellcoord(fa, l->na), ib = h_cellcoord(fb, l->nb), ic = h_cellcoord(fc, l->nc);
-        int cc = ia + l->na * (ib + l->nb * ic);
-        l->molcellno[im] = cc;
-        l->nmolsincell[cc] += 1;
-        l->maxcellno = std::max(l->maxcellno, l->nmolsincell[cc]);
-    }
-    l->molsincell.assign((size_t)l->ncell * std::max(1, l->maxcellno), 0);
-    std::fill(l->nmolsincell.begin(), l->nmolsincell.end(), 0);
-    for (int im = 0; im < nmol; ++im) {
-        int cc = l->molcellno[im];
-        l->molsincell[(size_t)cc * l->maxcellno + l->nmolsincell[cc]++] = im;
-    }
     // makecellneighbourlist (:102-194); the offsets depend on the lattice and rmax only
     if (l->nbr_na != l->na || l->nbr_rmax != rmax) {
         float cella = 1.0f / (float)l->na, cellb = 1.0f / (float)l->nb, cellc = 1.0f / (float)l->nc;
@@ -218,54 +270,56 @@ extern "C" int epsr_cells_init(epsr_ctx *ctx, const epsr_box *box, epsr_ledger *
         l->ncellneighb = (int)l->iaoff.size();
         l->nbr_na = l->na;
         l->nbr_rmax = rmax;
+        l->off_uploaded = false;
     }
-    // chain order of molfit (cells in order, molecules in cell order)
-    l->order.clear();
-    l->order.reserve(nmol);
-    for (int cc = 0; cc < l->ncell; ++cc)
-        for (int i = 0; i < l->nmolsincell[cc]; ++i) l->order.push_back(l->molsincell[(size_t)cc * l->maxcellno + i]);
-    l->pos.assign(nmol, 0);
-    for (int k = 0; k < nmol; ++k) l->pos[l->order[k]] = k;
-    l->nnbrefmax = 0;   // get_nnbrefmax: formed on the GPU by epsr_fmolec1
-    // upload, asynchronously from the pinned staging buffer: the cell arrays for
-    // epsr_fmolec1 ([offsets a, b, c][molecules per cell][members of each cell, maxcellno
-    // each][cell of each molecule]), the chain order and the chain position of each molecule
-    const int nbr = l->ncellneighb;
-    const size_t ncw = 3 * (size_t)nbr + (size_t)l->ncell + (size_t)l->ncell * l->maxcellno + (size_t)nmol;
-    if ((rc = ledger_stage(ctx, l, sizeof(int) * (ncw + 2 * (size_t)nmol)))) return rc;
-    int *hu = (int *)l->hstage;
-    std::copy(l->iaoff.begin(), l->iaoff.end(), hu);
-    std::copy(l->iboff.begin(), l->iboff.end(), hu + nbr);
-    std::copy(l->icoff.begin(), l->icoff.end(), hu + 2 * nbr);
-    std::copy(l->nmolsincell.begin(), l->nmolsincell.end(), hu + 3 * nbr);
-    std::copy(l->molsincell.begin(), l->molsincell.end(), hu + 3 * nbr + l->ncell);
-    std::copy(l->molcellno.begin(), l->molcellno.end(), hu + ncw - nmol);
-    std::copy(l->order.begin(), l->order.end(), hu + ncw);
-    std::copy(l->pos.begin(), l->pos.end(), hu + ncw + nmol);
+    // the cells of the molecules, the chain order (cells in order, molecules in cell
+    // order) and the chain positions, on the device; the neighbour offsets go up when
+    // they change (through the pinned staging buffer, whose previous upload this
+    // stream has ordered before)
+    const int nbr = l->ncellneighb, ncell = l->ncell;
+    const size_t ncw = 3 * (size_t)nbr + 2 * (size_t)ncell + 2 + (size_t)nmol;
     if (l->cap_cells < ncw) {
         const size_t cap = ncw + ncw / 8 + 64;
         if ((rc = dev_alloc(ctx, &l->d_cells, cap))) return rc;
         l->cap_cells = cap;
+        l->off_uploaded = false;
     }
-    HIP_CHECK(ctx, hipMemcpyAsync(l->d_cells, hu, sizeof(int) * ncw, hipMemcpyHostToDevice, st));
-    HIP_CHECK(ctx, hipMemcpyAsync(l->d_order, hu + ncw, sizeof(int) * nmol, hipMemcpyHostToDevice, st));
-    HIP_CHECK(ctx, hipMemcpyAsync(l->d_pos, hu + ncw + nmol, sizeof(int) * nmol, hipMemcpyHostToDevice, st));
+    if (!l->off_uploaded) {
+        if ((rc = ledger_stage(ctx, l, sizeof(int) * 3 * (size_t)nbr))) return rc;
+        if (l->ev_stage) HIP_CHECK(ctx, hipEventSynchronize(l->ev_stage));
+        int *hu = (int *)l->hstage;
+        std::copy(l->iaoff.begin(), l->iaoff.end(), hu);
+        std::copy(l->iboff.begin(), l->iboff.end(), hu + nbr);
+        std::copy(l->icoff.begin(), l->icoff.end(), hu + 2 * nbr);
+        HIP_CHECK(ctx, hipMemcpyAsync(l->d_cells, hu, sizeof(int) * 3 * nbr, hipMemcpyHostToDevice, st));
+        if (!l->ev_stage) HIP_CHECK(ctx, hipEventCreateWithFlags(&l->ev_stage, hipEventDisableTiming));
+        HIP_CHECK(ctx, hipEventRecord(l->ev_stage, st));
+        l->off_uploaded = true;
+    }
+    int *dnmc = l->d_cells + 3 * nbr, *dcst = dnmc + ncell, *dmcell = dcst + ncell + 2;
+    int *kv = nullptr;
+    if ((rc = ledger_work(ctx, l, LW_REC, sizeof(int) * 2 * ((size_t)nmol + 1), (void **)&kv))) return rc;
+    int *ident = kv, *skey = kv + nmol + 1;
+    cell_assign<<<ceil_div(nmol, 256), 256, 0, st>>>(nmol, c->xadd, c->yadd, c->zadd, l->na, l->nb, l->nc, dmcell,
+                                                     ident);
+    {
+        int bits = 1;
+        while ((1 << bits) < ncell) ++bits;
+        size_t tbytes = 0;
+        HIP_CHECK(ctx, hipcub::DeviceRadixSort::SortPairs(nullptr, tbytes, dmcell, skey, ident, l->d_order, nmol, 0,
+                                                         bits, st));
+        void *tmpb = nullptr;
+        if ((rc = ledger_work(ctx, l, LW_TMP, tbytes + 16, &tmpb))) return rc;
+        HIP_CHECK(ctx, hipcub::DeviceRadixSort::SortPairs(tmpb, tbytes, dmcell, skey, ident, l->d_order, nmol, 0, bits,
+                                                         st));
+    }
+    cell_bounds<<<ceil_div(nmol + 1, 256), 256, 0, st>>>(skey, nmol, ncell, dcst);
+    cell_counts<<<ceil_div(std::max(ncell, nmol), 256), 256, 0, st>>>(dcst, ncell, l->d_order, nmol, dnmc, l->d_pos);
+    HIP_CHECK(ctx, hipGetLastError());
+    l->nnbrefmax = 0;   // get_nnbrefmax: formed on the GPU by epsr_fmolec1
     l->cells_ready = true;
     l->ledger_ready = false;
     return EPSR_OK;
-}
-
-// the molecules of the neighbour cells of cell cc, in the order of
-// make_neighbour_list_from_cells (cellneighbourlist.py:199-219)
-static void cell_nbref(const epsr_ledger *l, int cc, std::vector<int> &out)
-{
-    out.clear();
-    int icc = cc / (l->na * l->nb), ib = (cc - icc * l->na * l->nb) / l->na, ia = cc - l->na * (ib + l->nb * icc);
-    for (int k = 0; k < l->ncellneighb; ++k) {
-        int a = fold(ia + l->iaoff[k], l->na), b = fold(ib + l->iboff[k], l->nb), q = fold(icc + l->icoff[k], l->nc);
-        int cref = a + l->na * (b + l->nb * q);
-        for (int i = 0; i < l->nmolsincell[cref]; ++i) out.push_back(l->molsincell[(size_t)cref * l->maxcellno + i]);
-    }
 }
 
 // ------------------------------------------------------------------ pair energy
@@ -924,17 +978,18 @@ cell_prefix(int nbr, const int *__restrict__ nbcnt, const int *__restrict__ nmol
 }
 
 // flattened neighbour molecule lists of the non-empty cells
-__global__ void cell_flatten(int ncell, int nbr, int maxcellno, const int *__restrict__ nbcell,
-                             const int *__restrict__ nbcnt, const int *__restrict__ nbpre,
-                             const int *__restrict__ cstart, const int *__restrict__ nmolsincell,
-                             const int *__restrict__ molsincell, int *__restrict__ nbflat)
+// (the molecules of a cell are the chain order's run from its start, cellstart)
+__global__ void cell_flatten(int ncell, int nbr, const int *__restrict__ nbcell, const int *__restrict__ nbcnt,
+                             const int *__restrict__ nbpre, const int *__restrict__ cstart,
+                             const int *__restrict__ nmolsincell, const int *__restrict__ order,
+                             const int *__restrict__ cellstart, int *__restrict__ nbflat)
 {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)ncell * nbr) return;
     int cc = (int)(t / nbr), kk = (int)(t - (long long)cc * nbr);
     if (nmolsincell[cc] == 0) return;
     int *dst = nbflat + cstart[cc] + nbpre[(size_t)cc * (nbr + 1) + kk];
-    const int *src = molsincell + (size_t)nbcell[t] * maxcellno;
+    const int *src = order + cellstart[nbcell[t]];
     for (int j = 0, n = nbcnt[t]; j < n; ++j) dst[j] = src[j];
 }
 
@@ -945,39 +1000,6 @@ __global__ void chain_rowlen(int nmol, const int *__restrict__ order, const int 
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < nmol) out[k] = rowlen[mcell[order[k]]];
     if (k == nmol) out[k] = 0;
-}
-
-// grow-only device work buffer `i` of the ledger (kept between calls)
-static int ledger_work(epsr_ctx *ctx, epsr_ledger *l, int i, size_t bytes, void **out)
-{
-    if (bytes > l->wcap[i]) {
-        if (l->wbuf[i]) {
-            HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-            HIP_CHECK(ctx, hipFree(l->wbuf[i]));
-        }
-        l->wbuf[i] = nullptr;
-        l->wcap[i] = 0;
-        const size_t want = bytes + bytes / 8 + 4096;
-        HIP_CHECK(ctx, hipMalloc(&l->wbuf[i], want));
-        l->wcap[i] = want;
-    }
-    *out = l->wbuf[i];
-    return EPSR_OK;
-}
-
-enum { LW_CELL = 0, LW_ROW = 1, LW_TMP = 2, LW_REC = 3, LW_SUM = 4 };
-
-// exclusive sums with the ledger's temporary storage
-template <class T>
-static int ledger_scan(epsr_ctx *ctx, epsr_ledger *l, const T *in, T *out, int n)
-{
-    size_t tb = 0;
-    HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, n, ctx->stream));
-    void *tmp = nullptr;
-    int rc = ledger_work(ctx, l, LW_TMP, tb + 16, &tmp);
-    if (rc) return rc;
-    HIP_CHECK(ctx, hipcub::DeviceScan::ExclusiveSum(tmp, tb, in, out, n, ctx->stream));
-    return EPSR_OK;
 }
 
 // the host-side scalars of the list set-up: candidate items (clamped to INT_MAX), flat
@@ -1027,8 +1049,7 @@ extern "C" int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coord
     if ((rc = ledger_work(ctx, l, LW_CELL, sizeof(int) * cw_ints, (void **)&cw))) return rc;
     if ((rc = ledger_work(ctx, l, LW_ROW, sizeof(long long) * 2 * ((size_t)nmol + 1), (void **)&drow))) return rc;
     long long *dlen = drow + nmol + 1;
-    const int *doff = l->d_cells, *dnmc = doff + 3 * nbr, *dmic = dnmc + ncell;
-    const int *dmcell = dmic + (size_t)ncell * l->maxcellno;
+    const int *doff = l->d_cells, *dnmc = doff + 3 * nbr, *dcst = dnmc + ncell, *dmcell = dcst + ncell + 2;
     int *dnbcell = cw, *dnbcnt = dnbcell + ncn, *dnbpre = dnbcnt + ncn;
     int *drowlen = dnbpre + (size_t)ncell * (nbr + 1), *dflatlen = drowlen + ncell + 1, *dcstart = dflatlen + ncell + 1;
     // [0] longest list, [1..2] table extents, [4..8] report, [10..11] entry counts
@@ -1071,8 +1092,8 @@ extern "C" int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coord
     int *rowJ = S.take<int>(total);
     int *cat = S.take<int>(2 * (size_t)nmol + 1);
     int *ent = S.take<int>(2 * (size_t)nmol + 1);
-    cell_flatten<<<(unsigned)ceil_div(ncn, 256), 256, 0, st>>>(ncell, nbr, l->maxcellno, dnbcell, dnbcnt, dnbpre,
-                                                               dcstart, dnmc, dmic, dnb);
+    cell_flatten<<<(unsigned)ceil_div(ncn, 256), 256, 0, st>>>(ncell, nbr, dnbcell, dnbcnt, dnbpre, dcstart, dnmc,
+                                                               l->d_order, dcst, dnb);
     PreparedTables pt;
     if ((rc = ledger_tables(ctx, box, tb, l))) return rc;
     if ((rc = tables_pack(ctx, box, tb, last, lastnear, l->d_tab, pt))) return rc;
@@ -1217,7 +1238,10 @@ extern "C" int epsr_ledger_export(epsr_ctx *ctx, const epsr_ledger *l, int ldl, 
             if (ergpairref) ergpairref[(size_t)m * ldl + s] = entry[rowstart[m] + s];
         }
     }
-    if (order) std::copy(l->order.begin(), l->order.end(), order);
+    if (order) {
+        HIP_CHECK(ctx, hipMemcpyAsync(order, l->d_order, sizeof(int) * nmol, hipMemcpyDeviceToHost, st));
+        HIP_CHECK(ctx, hipStreamSynchronize(st));
+    }
     return EPSR_OK;
 }
 
@@ -2474,6 +2498,8 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
         // step persists for the molecule type.  Nothing was committed: resolve the chain on
         // the host from the current centres, then run the cycle again with those offsets.
         std::vector<float> cx(nmol), cy(nmol), cz(nmol);
+        std::vector<int> horder(nmol);
+        HIP_CHECK(ctx, hipMemcpyAsync(horder.data(), l->d_order, sizeof(int) * nmol, hipMemcpyDeviceToHost, st));
         HIP_CHECK(ctx, hipMemcpyAsync(cx.data(), c->xadd, sizeof(float) * nmol, hipMemcpyDeviceToHost, st));
         HIP_CHECK(ctx, hipMemcpyAsync(cy.data(), c->yadd, sizeof(float) * nmol, hipMemcpyDeviceToHost, st));
         HIP_CHECK(ctx, hipMemcpyAsync(cz.data(), c->zadd, sizeof(float) * nmol, hipMemcpyDeviceToHost, st));
@@ -2490,7 +2516,7 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
         wt.assign(wholetrans, wholetrans + nmt);
         int cursor = 0;
         for (int k = 0; k < nmol; ++k) {
-            int m = l->order[k], mt = box->h_mtype[m];
+            int m = horder[k], mt = box->h_mtype[m];
             int nat = box->h_nal[m] - box->h_naf[m] + 1;
             if (p->stepr > 0.0f && nat > 1) {
                 if (wr[mt] <= 0.0f) wr[mt] = rotstepmin;
