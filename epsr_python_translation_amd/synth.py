@@ -202,6 +202,10 @@ class System:
     power: float = 12.0
     rejrate: float = 0.75
     seed: int = 0
+    # per type pair, from a .pcof file: the EP's minimum separation and repulsive-core
+    # coefficient (read_pcof); None means zero for every pair
+    rminex: np.ndarray = None
+    pcoefex: np.ndarray = None
 
     @property
     def nmol(self):

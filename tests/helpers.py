@@ -4,17 +4,31 @@ from __future__ import annotations
 
 import ctypes as C
 import functools
+import os
 
 import numpy as np
 
-from epsr_python_translation_amd import synth, tables
+from epsr_python_translation_amd import formats, synth, tables
 from oracle import pyoracle as po
 
 F32 = np.float32
 
 
+EPSR_TEST = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "epsr_test")
+
+
+@functools.lru_cache(maxsize=None)
+def epsr_test_files():
+    """The reference's EPSR test box: (.ato, .pcof, .EPSR.inp) parsed."""
+    return (formats.read_ato(os.path.join(EPSR_TEST, "200DMSO400Water.ato.gz")),
+            formats.read_pcof(os.path.join(EPSR_TEST, "200DMSO400Water.pcof.gz")),
+            formats.read_keywords(os.path.join(EPSR_TEST, "200DMSO400Water.EPSR.inp.gz")))
+
+
 @functools.lru_cache(maxsize=None)
 def system(name: str):
+    if name == "epsr_test":
+        return formats.system_from_files(*epsr_test_files(), name="epsr_test")
     if name == "c0":
         return synth.config0()
     if name == "c1":

@@ -38,6 +38,8 @@ def _gpu_atogen(ctx, name, ngrsamples, idum, rminex=None):
     ("tiny_14_40_0", 7, -99),         # sampled origins (addwav 1), under-represented types forced
     ("c0", 0, -12345),                # 200 DMSO + 400 water, all 600 molecules
     ("c0", 150, -31),                 # 3200 atoms, 150 sampled origins
+    ("epsr_test", 0, -7),             # the reference's EPSR test box read from its .ato
+    ("epsr_test", 200, -3),
 ])
 def test_atogen_bit_exact(ctx, name, ngr, idum):
     print(f"atogen parity {name} ngr={ngr}", flush=True)

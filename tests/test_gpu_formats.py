@@ -1,0 +1,71 @@
+"""The reference's EPSR test box, read from its own .ato/.pcof/.EPSR.inp, through the GPU
+path: one MC sweep restarted from the ran1 state saved in the .ato against the oracle
+from the same state, bit for bit, then written back as an .ato and read again."""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+from epsr_python_translation_amd import formats as FM
+from oracle import pyoracle as po
+from tests import helpers as H
+
+F32 = np.float32
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from epsr_python_translation_amd import device as D
+    return D.Context(0)
+
+
+def test_epsr_test_box_restarts_from_its_ato(ctx, tmp_path):
+    from epsr_python_translation_amd import device as D
+    from epsr_python_translation_amd import tables as T
+    print("epsr_test restart: one sweep from the .ato's ran1 state", flush=True)
+    a, _, _ = H.epsr_test_files()
+    name = "epsr_test"
+    s, g = H.system(name), H.grid(name)
+    # oracle: ledger, then one molfit sweep from the saved state
+    ref = H.oracle_fmolec1(name, "cr")
+    ob = ref["ob"]
+    wr, wt = H.initial_steps(s)
+    nwh = np.zeros(2 * s.nmoltype, np.int32)
+    nmv = np.zeros(2 * s.nmoltype, np.int32)
+    nsr = np.zeros(s.nmol, np.int32)
+    mcs = po.MCState(po.Pf(wr), po.Pf(wt), po.Pi(nwh), po.Pi(nmv), po.Pi(nsr), s.simt, F32(1), F32(1), F32(0.75), 0)
+    rng = po.Ran1.from_array(a.ran1)
+    fadds, nacc = C.c_float(0.0), C.c_int()
+    po.lib("cr").ora_molfit(C.byref(ob.box), po.Pf(ob.xa), po.Pf(ob.ya), po.Pf(ob.za), C.byref(ob.centres),
+                            C.byref(ref["ctab"]), C.byref(ref["cells"]), C.byref(ref["ledger"]), C.byref(mcs),
+                            C.byref(rng), C.byref(fadds), C.byref(nacc))
+    # GPU: the same box and tables, the same saved state
+    t = H.oracle_tables(name)
+    box = D.DeviceBox.from_system(ctx, s)
+    tab = D.DeviceTables(ctx, t["ld"], t["npt2"], t["xstar"], t["xavs"], t["xavs1"], t["ecoul"], t["ecoul1"])
+    led = D.Ledger(ctx, box)
+    led.cells_init(s.nmolcell, T.sepmax(s, g))
+    led.fmolec1(tab, s.eintra)
+    wr2, wt2 = H.initial_steps(s)
+    mc = D.MCState(ctx, s.nmol, s.nmoltype, s.simt, wholerot=wr2, wholetrans=wt2)
+    st = np.ascontiguousarray(a.ran1.copy())
+    D.molfit(ctx, box, tab, led, mc, st)
+    torch.cuda.synchronize()
+    print(f"  accepted {mc.stats.accepted}/{mc.stats.moves}", flush=True)
+    co = box.get_coords()
+    for k in ("xa", "ya", "za", "xadd", "yadd", "zadd"):
+        assert np.array_equal(co[k].view(np.int32), getattr(ob, k).view(np.int32)), k
+    np.testing.assert_array_equal(st, rng.as_array())
+    assert F32(mc.fadds).view(np.int32) == F32(fadds.value).view(np.int32)
+    # write the result as an .ato and read it back: the ran1 state exactly, the
+    # coordinates to the file's precision
+    b = FM.update_ato(a, co["xadd"], co["yadd"], co["zadd"], co["xa"], co["ya"], co["za"], ran1=st)
+    path = tmp_path / "after.ato"
+    FM.write_ato(path, b)
+    c = FM.read_ato(path)
+    np.testing.assert_array_equal(c.ran1, st)
+    s2 = FM.system_from_files(c, *H.epsr_test_files()[1:])
+    assert np.max(np.abs(s2.xadd - co["xadd"])) < 1e-7 and np.max(np.abs(s2.xa - co["xa"])) < 1e-8

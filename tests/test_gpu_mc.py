@@ -41,7 +41,7 @@ def _gpu_setup(ctx, name, coords=None):
     return box, tab, led, f64, fb64
 
 
-@pytest.mark.parametrize("name", ["tiny_12_6_2", "c0", "c1"])
+@pytest.mark.parametrize("name", ["tiny_12_6_2", "c0", "c1", "epsr_test"])
 def test_fmolec1_ledger_bit_exact(ctx, name):
     print(f"fmolec1 parity {name}", flush=True)
     s = H.system(name)
@@ -131,7 +131,7 @@ def _run_oracle_molfit(name, flavour, nsweep, idum):
                 ran1=rng.as_array(), fadds=F32(fadds.value))
 
 
-@pytest.mark.parametrize("name,nsweep", [("tiny_13_12_3", 3), ("c0", 2), ("c1", 1)])
+@pytest.mark.parametrize("name,nsweep", [("tiny_13_12_3", 3), ("c0", 2), ("c1", 1), ("epsr_test", 2)])
 def test_molfit_bit_exact_with_correctly_rounded_libm(ctx, name, nsweep):
     from epsr_python_translation_amd import device as D
     print(f"molfit parity {name} sweeps={nsweep}", flush=True)
