@@ -295,6 +295,27 @@ class Ledger:
                                                    _ptr(ergnew), C.byref(fd)))
         return F32(fd.value)
 
+    def press(self, tables: DeviceTables, npt1: int, simt, volume, slots: bool = False):
+        """press (ato_manipulation_rev4.py:3901-4063) at the box's current coordinates over
+        this ledger's partner lists: (energy per molecule, pressure in kbar, virial sum) and,
+        with slots=True, each ledger slot's energy and virial as [nmol, nnbrefmax] arrays in
+        the reference's layout."""
+        c = self.box.coords()
+        t = tables.struct()
+        e, p, w = C.c_float(), C.c_float(), C.c_double()
+        ldl = max(1, self.nnbrefmax)
+        se = sw = None
+        if slots:
+            se = np.zeros((self.box.nmol, ldl), F32)
+            sw = np.zeros((self.box.nmol, ldl), F32)
+        fp_ = C.POINTER(C.c_float)
+        self.ctx.check(self.ctx.lib.epsr_press(
+            self.ctx.h, self.box.h, C.byref(c), C.byref(t), self.h, int(npt1), F32(simt), F32(volume), C.byref(e),
+            C.byref(p), C.byref(w), ldl, se.ctypes.data_as(fp_) if slots else None,
+            sw.ctypes.data_as(fp_) if slots else None))
+        out = (F32(e.value), F32(p.value), float(w.value))
+        return out + (se, sw) if slots else out
+
     def close(self):
         if getattr(self, "h", None):
             self.ctx.lib.epsr_ledger_destroy(self.h)

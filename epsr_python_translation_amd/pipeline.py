@@ -107,6 +107,15 @@ class Refinement:
         self.ledger.cells_init(self.s.nmolcell, T.sepmax(self.s, self.g))
         return self.ledger.fmolec1(self.tables, self.eintra)
 
+    def press(self, energf=None, pressf=None, wsum=None):
+        """press (ato_manipulation_rev4.py:3901-4063) over the current ledger: (energy per
+        molecule, pressure in kbar, energabs, energthreeb, prthreebout)."""
+        from . import press as P
+        s = self.s
+        vol = F32(s.dmax) * F32(s.dmax) * F32(s.dmax)
+        return P.press(self.ledger, self.tables, self.g.npt1, s.simt, vol, pcoefp=self.pcoefp, energf=energf,
+                       pressf=pressf, wsum=wsum, nmol=s.nmol)
+
     def molfit(self, tables=None):
         D.molfit(self.ctx, self.box, tables or self.tables, self.ledger, self.mc, self.ran1)
         return self.mc

@@ -186,6 +186,21 @@ int epsr_en_change(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *coor
 int epsr_en_mole_pair(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *coords, const epsr_tables_t *tables,
                       int npair, const int *im, const int *jm, float *energy);
 
+/* ---- press: replaces ato_manipulation_rev4.press(energy, pressc, energf, pressf, energabs,
+ * energthreeb, prthreebout) (src/epsr26/library/ato_manipulation_rev4.py:3901-4063; Fortran
+ * ato_manipulation_rev4.f:press) without three-body terms or coordination constraints:
+ * the whole-box intermolecular energy and virial pressure over the ledger's partner lists
+ * (make_neighbour_list_from_mols, ergpairreference.py:165-188) at the current coordinates.
+ * Outputs (host): energy = the reference's float32 sum / nmol (kJ/mol), pressc (kbar) and
+ * presst (the double virial sum).  npt1: the last bin used (2 <= npt1 < ld); simt =
+ * 0.008314 T; volume in A^3.  slote/slotw (host, optional, [nmol*ldl], ldl >= nnbrefmax):
+ * each ledger slot's energy and virial in the reference's layout, 0 where the partner is
+ * not later in the numbering.  abs_energy_ep and the per-dataset energf/pressf
+ * normalisation are host arithmetic (press.py). */
+int epsr_press(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *coords, const epsr_tables_t *tables,
+               const epsr_ledger *l, int npt1, float simt, float volume, float *energy, float *pressc,
+               double *presst, int ldl, float *slote, float *slotw);
+
 /* ---- molfit: replaces molfit.molfit(fadds, faddbs, nisum, intra, rotfreq, ireset)
  * (src/epsr26/library/molfit.py:51-541) for the whole-molecule branch (nisum < intra,
  * no internal rotational groups, no tethered molecules): one MC cycle over the cells,

@@ -141,6 +141,14 @@ int ora_fmolec1(const ora_box_t *b, const ora_centres_t *c, const ora_tables_t *
                 const ora_cells_t *cl, const float *eintra, ora_ledger_t *l,
                 float *ftest, float *ftestb, double *ftest_f64);
 
+/* ---- press (src/epsr26/library/ato_manipulation_rev4.py:3901-4063) ----
+ * Whole-box intermolecular energy (kJ/mol per molecule) and virial pressure (kbar)
+ * over the ledger's partner lists; slote/slotw [nmol*nnbrefmax] (may be NULL)
+ * receive each ledger slot's energy and virial (0 where the partner is not later). */
+void ora_press(const ora_box_t *b, const ora_centres_t *c, const ora_tables_t *t, const ora_ledger_t *l,
+               int npt1, float simt, float volume, float *energy, float *pressc, double *presst,
+               float *slote, float *slotw);
+
 /* ---- en_change (src/epsr26/library/molfit.py:805-919) ----
  * Uses the ledger list of im (make_neighbour_list_from_mols) and the current
  * coordinates; ergnew[nnbref] out (slot 0 = ergold[0]). Returns fdiff. */

@@ -164,3 +164,26 @@ def oracle_fmolec1(name, flavour="glibc", ob=None, cells=None):
 def initial_steps(s):
     """Initial whole-molecule rotation (fraction of pi) and translation (box fraction) steps."""
     return (np.full(s.nmoltype, 0.05, F32), np.full(s.nmoltype, F32(0.1) / F32(s.dmax), F32))
+
+
+def box_volume(s):
+    """clustercom.volume of a cubic box: alatx * blaty * clatz in float32 (Angstrom^3)."""
+    return F32(s.dmax) * F32(s.dmax) * F32(s.dmax)
+
+
+def oracle_press(name, flavour="glibc", ref=None, tables=None, slots=True):
+    """ora_press over the oracle's own ledger of `name` (or of ref = oracle_fmolec1(...))."""
+    s, g = system(name), grid(name)
+    ref = ref or oracle_fmolec1(name, flavour)
+    tt = ctables(tables) if tables is not None else ref["ctab"]
+    led = ref["ledger"]
+    M = max(1, led.nnbrefmax)
+    se = np.zeros(s.nmol * M, F32)
+    sw = np.zeros(s.nmol * M, F32)
+    e, p = C.c_float(), C.c_float()
+    w = C.c_double()
+    po.lib(flavour).ora_press(C.byref(ref["ob"].box), C.byref(ref["ob"].centres), C.byref(tt), C.byref(led), g.npt1,
+                              F32(s.simt), box_volume(s), C.byref(e), C.byref(p), C.byref(w),
+                              po.Pf(se) if slots else None, po.Pf(sw) if slots else None)
+    return dict(energy=F32(e.value), pressc=F32(p.value), presst=w.value, slote=se.reshape(s.nmol, M),
+                slotw=sw.reshape(s.nmol, M), ref=ref)
