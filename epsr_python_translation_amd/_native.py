@@ -145,6 +145,8 @@ SIGNATURES = {
     "epsr_ledger_export": (C.c_int, [vp, vp, C.c_int, ip, ip, ip, fp, ip]),
     "epsr_en_change": (C.c_int, [vp, vp, C.POINTER(Coords), C.POINTER(Tables), vp, C.c_int, vp, fp]),
     "epsr_en_mole_pair": (C.c_int, [vp, vp, C.POINTER(Coords), C.POINTER(Tables), C.c_int, vp, vp, vp]),
+    "epsr_sumsq": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, C.c_double, C.c_double, vp, vp]),
+    "epsr_sumgr": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, vp, C.c_double, C.c_double, vp, vp, vp]),
     "epsr_press": (C.c_int, [vp, vp, C.POINTER(Coords), C.POINTER(Tables), vp, C.c_int, C.c_float, C.c_float, fp, fp,
                              C.POINTER(C.c_double), C.c_int, fp, fp]),
     "epsr_molfit": (C.c_int, [vp, vp, C.POINTER(Coords), C.POINTER(Tables), vp, C.POINTER(MCParams), fp, fp, ip, ip,

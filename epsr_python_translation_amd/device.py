@@ -328,6 +328,19 @@ class Ledger:
             pass
 
 
+def sumsq(ctx: Context, npairs: int, nq: int, ldq: int, qwind, fc1: float, fc2: float, soq, sq2):
+    """sumsq (epsr_standalone_rev3.py:3642-3685): soq/sq2 [2*npairs*ldq] updated in place."""
+    ctx.check(ctx.lib.epsr_sumsq(ctx.h, int(npairs), int(nq), int(ldq), _ptr(qwind), float(fc1), float(fc2),
+                                 _ptr(soq), _ptr(sq2)))
+
+
+def sumgr(ctx: Context, npairs: int, nz: int, ldw: int, wav, ycoh, fc1: float, fc2: float, gor, gr2, entsig):
+    """sumgr (epsr_standalone_rev3.py:3578-3637): gor/gr2 [2*npairs*ldw] updated in place,
+    entsig [npairs*ldw] the zero limits."""
+    ctx.check(ctx.lib.epsr_sumgr(ctx.h, int(npairs), int(nz), int(ldw), _ptr(wav), _ptr(ycoh), float(fc1),
+                                 float(fc2), _ptr(gor), _ptr(gr2), _ptr(entsig)))
+
+
 class Graph:
     """Work issued on a context's stream inside ``with Graph.capture(ctx) as g:`` is
     recorded into a HIP graph; ``g.launch()`` replays it on the same stream."""

@@ -23,6 +23,18 @@ from . import tables as T
 F32 = np.float32
 
 
+def sum_factors(nsum: int, nsums: int = 0, nesum: int = 0):
+    """The accumulation factors of common /sumvalues/ that runepsr17 sets before sumsq and
+    sumgr (epsr_standalone_rev3.py:1613-1618): (fc1, fc2) as doubles."""
+    if nsums >= 0:
+        fc1 = float(F32(1.0) / F32(nsum))
+        fc2 = float(F32(nsum - 1)) * fc1
+    else:
+        fc1 = float(F32(1.0) / F32(nesum + 1))
+        fc2 = float(F32(nesum)) * fc1
+    return fc1, fc2
+
+
 class Refinement:
     def __init__(self, ctx: D.Context, system, idum: int = -1, timing: bool = False):
         self.ctx = ctx
@@ -106,6 +118,20 @@ class Refinement:
         """initialise_cellneighbourlist + fmolec1 (:1259-1281).  Returns (ftest, ftestb)."""
         self.ledger.cells_init(self.s.nmolcell, T.sepmax(self.s, self.g))
         return self.ledger.fmolec1(self.tables, self.eintra)
+
+    def accumulate(self, nsum: int, nsums: int = 0, nesum: int = 0):
+        """sumsq + sumgr (epsr_standalone_rev3.py:1632-1642) on the current sofq and wav: the
+        running averages soq/sq2 and gor/gr2 and the zero limits entsig."""
+        s, g, ctx = self.s, self.g, self.ctx
+        if getattr(self, "soq", None) is None:
+            self.soq = ctx.zeros(2 * s.npairs * self.ldq)
+            self.sq2 = ctx.zeros(2 * s.npairs * self.ldq)
+            self.gor = ctx.zeros(2 * s.npairs * self.ldw)
+            self.gr2 = ctx.zeros(2 * s.npairs * self.ldw)
+            self.entsig = ctx.zeros(s.npairs * self.ldw)
+        fc1, fc2 = sum_factors(nsum, nsums, nesum)
+        D.sumsq(ctx, s.npairs, g.nq, self.ldq, self.sofq, fc1, fc2, self.soq, self.sq2)
+        D.sumgr(ctx, s.npairs, g.npt1, self.ldw, self.wav, self.ycoh, fc1, fc2, self.gor, self.gr2, self.entsig)
 
     def press(self, energf=None, pressf=None, wsum=None):
         """press (ato_manipulation_rev4.py:3901-4063) over the current ledger: (energy per

@@ -186,6 +186,23 @@ int epsr_en_change(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *coor
 int epsr_en_mole_pair(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *coords, const epsr_tables_t *tables,
                       int npair, const int *im, const int *jm, float *energy);
 
+/* ---- sumsq: replaces epsr_standalone_rev3.sumsq(nq, qwind)
+ * (src/epsr26/programs/epsr_standalone_rev3.py:3642-3685; Fortran epsr_standalone_rev3.f:sumsq):
+ * the running average and error bar of every intra (k = 0) and inter (k = 1) partial S(Q),
+ * Q points 1..nq.  qwind dev [2*npairs*ldq] (gensq's sofq, [(k*npairs + ic)*ldq + j]);
+ * soq/sq2 dev, same layout (epsr16.soqintra/soqinter and sq2intra/sq2inter), updated in
+ * place; fc1, fc2: common /sumvalues/ (epsr_sum_factors in pipeline.py). */
+int epsr_sumsq(epsr_ctx *ctx, int npairs, int nq, int ldq, const float *qwind, double fc1, double fc2,
+               float *soq, float *sq2);
+
+/* ---- sumgr: replaces epsr_standalone_rev3.sumgr(nz, wav) (:3578-3637): the same for the
+ * partial g(r), r points 2..nz, with add1 = wav / ycoh (0 where ycoh <= 0); wav dev
+ * [2*npairs*ldw], ycoh dev [ldw]; gor/gr2 dev [2*npairs*ldw] (gorintra/gorinter,
+ * gr2intra/gr2inter); entsig dev [npairs*ldw] receives each pair's zero limit
+ * (epsr16.entsig). */
+int epsr_sumgr(epsr_ctx *ctx, int npairs, int nz, int ldw, const float *wav, const float *ycoh, double fc1,
+               double fc2, float *gor, float *gr2, float *entsig);
+
 /* ---- press: replaces ato_manipulation_rev4.press(energy, pressc, energf, pressf, energabs,
  * energthreeb, prthreebout) (src/epsr26/library/ato_manipulation_rev4.py:3901-4063; Fortran
  * ato_manipulation_rev4.f:press) without three-body terms or coordination constraints:

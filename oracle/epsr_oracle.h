@@ -149,6 +149,13 @@ void ora_press(const ora_box_t *b, const ora_centres_t *c, const ora_tables_t *t
                int npt1, float simt, float volume, float *energy, float *pressc, double *presst,
                float *slote, float *slotw);
 
+/* ---- sumsq / sumgr (src/epsr26/programs/epsr_standalone_rev3.py:3642-3685, 3578-3637) ----
+ * running averages and error bars, intra (k = 0) and inter (k = 1) rows
+ * [(k*npairs + ic)*ld + j]; fc1, fc2 the accumulation factors of common /sumvalues/. */
+void ora_sumsq(int npairs, int nq, int ldq, const float *qwind, double fc1, double fc2, float *soq, float *sq2);
+void ora_sumgr(int npairs, int nz, int ldw, const float *wav, const float *ycoh, double fc1, double fc2,
+               float *gor, float *gr2, float *entsig);
+
 /* ---- en_change (src/epsr26/library/molfit.py:805-919) ----
  * Uses the ledger list of im (make_neighbour_list_from_mols) and the current
  * coordinates; ergnew[nnbref] out (slot 0 = ergold[0]). Returns fdiff. */
