@@ -250,6 +250,7 @@ def main():
         "accepted_fraction": (R.mc.stats.accepted / max(1, R.mc.stats.moves)),
         "roofline": roof,
     }
+    out["iteration_extras"] = iteration_extras(R, s)
     if not args.no_small_box:
         out["dmso_water_3200"] = small_box_rate(D, PL, synth, ctx)
     if not args.no_da:
@@ -275,6 +276,35 @@ def _traffic(kernel):
         return t.get(kernel, {}).get("bytes_per_launch")
     except (OSError, ValueError):
         return None
+
+
+def iteration_extras(R, s, reps=5):
+    """The once-per-iteration steps around the EP refinement on the same water20000 box,
+    after the timed steps (not part of `value`): press over the ledger (GPU pair sums, host
+    running sums in the reference's order) and sumsq + sumgr (running averages).  Wall
+    clock per call, synchronised, median of `reps`."""
+    import torch
+    torch.cuda.synchronize()
+    tp = []
+    for _ in range(reps + 1):
+        t0 = time.perf_counter()
+        e, p, *_ = R.press()
+        tp.append(time.perf_counter() - t0)
+    ta = []
+    R.gensq()
+    for n in range(1, reps + 2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        R.accumulate(n)
+        torch.cuda.synchronize()
+        ta.append(time.perf_counter() - t0)
+    npairs, g = s.npairs, R.g
+    elems = 2 * npairs * g.nq + 2 * npairs * (g.npt1 - 1)
+    ms_a = float(np.median(ta[1:])) * 1e3
+    return {"press": {"ms": float(np.median(tp[1:])) * 1e3, "energy_kj_mol": float(e), "pressure_kbar": float(p),
+                      "ledger_slots": int(R.ledger.nergpair)},
+            "sumsq_sumgr": {"ms": ms_a, "elements": elems, "bytes": 20 * elems,
+                            "GB_per_s_wall": 20 * elems / (ms_a * 1e-3) / 1e9}}
 
 
 def small_box_rate(D, PL, synth, ctx, sweeps=20):

@@ -147,6 +147,8 @@ SIGNATURES = {
     "epsr_en_mole_pair": (C.c_int, [vp, vp, C.POINTER(Coords), C.POINTER(Tables), C.c_int, vp, vp, vp]),
     "epsr_sumsq": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, C.c_double, C.c_double, vp, vp]),
     "epsr_sumgr": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, vp, C.c_double, C.c_double, vp, vp, vp]),
+    "epsr_invert_cwt": (C.c_int, [vp, vp, C.c_int, vp, C.c_int, vp, vp, C.c_int, C.c_float, vp, C.c_int, C.c_int,
+                                  vp, ip]),
     "epsr_press": (C.c_int, [vp, vp, C.POINTER(Coords), C.POINTER(Tables), vp, C.c_int, C.c_float, C.c_float, fp, fp,
                              C.POINTER(C.c_double), C.c_int, fp, fp]),
     "epsr_molfit": (C.c_int, [vp, vp, C.POINTER(Coords), C.POINTER(Tables), vp, C.POINTER(MCParams), fp, fp, ip, ip,

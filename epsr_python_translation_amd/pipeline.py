@@ -35,6 +35,29 @@ def sum_factors(nsum: int, nsums: int = 0, nesum: int = 0):
     return fc1, fc2
 
 
+def invert_cwt(ctx, cwt, cwtt, feedback, efacm, ran1_state):
+    """invert_cwt (epsr_standalone_rev3.py:5423-5672) through the library: cwt [nfile,
+    npairs] and cwtt [npairs, npairs + nfile] are float64 arrays updated in place (cwt
+    scaled by feedback * efacm, cwtt the transpose); ran1_state int32[35] in place.
+    Returns (carray, carrayold, itimes)."""
+    cwt = np.asarray(cwt)
+    cwtt = np.asarray(cwtt)
+    assert cwt.dtype == np.float64 and cwtt.dtype == np.float64 and cwt.flags["C_CONTIGUOUS"] and \
+        cwtt.flags["C_CONTIGUOUS"]
+    assert ran1_state.dtype == np.int32 and ran1_state.size == 35
+    nfile, npairs = cwt.shape
+    nnew = npairs + nfile
+    assert cwtt.shape == (npairs, nnew)
+    ef = np.ascontiguousarray(efacm, dtype=F32)
+    carray = np.zeros(nnew)
+    carrayold = np.zeros((nnew, nnew))
+    it = C.c_int()
+    ctx.check(ctx.lib.epsr_invert_cwt(ctx.h, cwt.ctypes.data, npairs, cwtt.ctypes.data, nnew, carray.ctypes.data,
+                                      carrayold.ctypes.data, nnew, F32(feedback), ef.ctypes.data, nfile, npairs,
+                                      ran1_state.ctypes.data, C.byref(it)))
+    return carray, carrayold, it.value
+
+
 class Refinement:
     def __init__(self, ctx: D.Context, system, idum: int = -1, timing: bool = False):
         self.ctx = ctx

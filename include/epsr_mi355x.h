@@ -203,6 +203,18 @@ int epsr_sumsq(epsr_ctx *ctx, int npairs, int nq, int ldq, const float *qwind, d
 int epsr_sumgr(epsr_ctx *ctx, int npairs, int nz, int ldw, const float *wav, const float *ycoh, double fc1,
                double fc2, float *gor, float *gr2, float *entsig);
 
+/* ---- invert_cwt: replaces epsr_standalone_rev3.invert_cwt(cwt, cwtt, carray, carrayold,
+ * feedback, efacm, nfile, npairs, mfile, mpairs) (src/epsr26/programs/epsr_standalone_rev3.py:
+ * 5423-5672; Fortran epsr_standalone_rev3.f:invert_cwt): the Monte Carlo transpose of the
+ * weights matrix, host arrays (real*8) in and out, 0-based and row-major: cwt [nfile rows,
+ * ldc] (file i, pair j; scaled in place by feedback * efacm(i) as the reference does),
+ * cwtt [npairs rows, ldt >= npairs + nfile] (pair j, column i), carray [npairs + nfile],
+ * carrayold [(npairs + nfile) rows, ldo] (both optional); ran1_state in/out.  Host work
+ * (a serial walk over at most 51 x 51 elements); itimes_out: step-size rounds taken. */
+int epsr_invert_cwt(epsr_ctx *ctx, double *cwt, int ldc, double *cwtt, int ldt, double *carray, double *carrayold,
+                    int ldo, float feedback, const float *efacm, int nfile, int npairs, int32_t *ran1_state,
+                    int *itimes_out);
+
 /* ---- press: replaces ato_manipulation_rev4.press(energy, pressc, energf, pressf, energabs,
  * energthreeb, prthreebout) (src/epsr26/library/ato_manipulation_rev4.py:3901-4063; Fortran
  * ato_manipulation_rev4.f:press) without three-body terms or coordination constraints:

@@ -156,6 +156,12 @@ void ora_sumsq(int npairs, int nq, int ldq, const float *qwind, double fc1, doub
 void ora_sumgr(int npairs, int nz, int ldw, const float *wav, const float *ycoh, double fc1, double fc2,
                float *gor, float *gr2, float *entsig);
 
+/* ---- invert_cwt (src/epsr26/programs/epsr_standalone_rev3.py:5423-5672) ----
+ * cwt [nfile*npairs], cwtt [npairs*(npairs+nfile)], carray [npairs+nfile],
+ * carrayold [(npairs+nfile)^2], 0-based; returns the number of step-size rounds. */
+int ora_invert_cwt(double *cwt, double *cwtt, double *carray, double *carrayold, float feedback, const float *efacm,
+                   int nfile, int npairs, ora_ran1_t *rng);
+
 /* ---- en_change (src/epsr26/library/molfit.py:805-919) ----
  * Uses the ledger list of im (make_neighbour_list_from_mols) and the current
  * coordinates; ergnew[nnbref] out (slot 0 = ergold[0]). Returns fdiff. */

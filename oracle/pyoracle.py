@@ -140,6 +140,7 @@ def _declare(L):
         "ora_fmolec1": (C.c_int, [P(Box), P(Centres), P(Tables), P(Cells), fp, P(Ledger), fp, fp, dp]),
         "ora_sumsq": (None, [C.c_int, C.c_int, C.c_int, fp, C.c_double, C.c_double, fp, fp]),
         "ora_sumgr": (None, [C.c_int, C.c_int, C.c_int, fp, fp, C.c_double, C.c_double, fp, fp, fp]),
+        "ora_invert_cwt": (C.c_int, [dp, dp, dp, dp, C.c_float, fp, C.c_int, C.c_int, P(Ran1)]),
         "ora_press": (None, [P(Box), P(Centres), P(Tables), P(Ledger), C.c_int, C.c_float, C.c_float, fp, fp, dp,
                              fp, fp]),
         "ora_en_change": (C.c_float, [P(Box), P(Centres), P(Tables), P(Ledger), C.c_int, fp, ip]),
