@@ -862,6 +862,38 @@ def write_pcof(path, p: PcofFile):
         f.write(format_pcof(p))
 
 
+# ------------------------------------------------------------------ .wts (neutron weights)
+
+
+@dataclasses.dataclass
+class WtsFile:
+    """A neutron weights file (the neutron branch of get_wts, epsr_standalone_rev3.py:
+    4308-4318): the density factor, then per type pair its labels and the intramolecular
+    and intermolecular weights (the rest of each value record is not read)."""
+
+    rhofac: np.float32
+    pairs: list                # [(a1, a2, w_intra, w_inter)] in file order, labels character*3
+
+
+def read_wts(path) -> WtsFile:
+    L = _Lines(_read_text(path))
+    if any(k in str(path) for k in ("X-ray.wts", "x-ray.wts", "XWTS.wts")):
+        raise NotImplementedError("read_wts: X-ray weights files are not handled (neutron weights only)")
+    rhofac = f32(L.items(1)[0])
+    pairs = []
+    while not L.at_end():
+        lab = L.line()
+        if not lab.strip():
+            continue
+        a1, a2 = fa(lab[1:4], 3), fa(lab[5:8], 3)          # (1x,a3,1x,a3)
+        try:
+            w = L.items(2)
+        except EOFError:
+            break
+        pairs.append((a1, a2, f32(w[0]), f32(w[1])))
+    return WtsFile(rhofac=rhofac, pairs=pairs)
+
+
 # ------------------------------------------------------------------ the box
 
 
@@ -926,8 +958,8 @@ def system_from_files(ato, pcof=None, inp=None, name=None):
     ecor = np.sqrt(eps[n1] * eps[n2]).astype(F32)
     dcor = (F32(0.5) * (sig[n1] + sig[n2])).astype(F32)
     qcor = (charge[n1] * charge[n2] * F32(1390)).astype(F32)
-    _, _, _, cfrac = ato_composition(a)
-    kw = {}
+    _, _, cfraccg, cfrac = ato_composition(a)
+    kw = {"cfraccg": cfraccg}
     npairs = ip
     if pc is not None:
         ncoeffp = pc.ncoeffp

@@ -58,6 +58,67 @@ def invert_cwt(ctx, cwt, cwtt, feedback, efacm, ran1_state):
     return carray, carrayold, it.value
 
 
+def get_wts(ctx, s, wts, feedback, efacm, ran1_state, cwtt=None, recompute=True):
+    """The neutron branch of get_wts (epsr_standalone_rev3.py:4180-4479) for box `s` and the
+    parsed weights files `wts` (formats.read_wts, one per dataset): per dataset and type
+    pair the weights ciwt (intra) and cwt (inter), wsum = sum |cwt| and xwt = sum cwt, and,
+    when `recompute` (ireset = 1 or feedback/efacm changed), the inverted matrix through
+    invert_cwt from cwtt = [0 | I].  Returns a dict with ciwt, cwt [nfile, npairs] (float32),
+    wsum, xwt [nfile], cwtt [npairs, npairs + nfile] and cwtpot [npairs, nfile] (float64)."""
+    nfile, npairs = len(wts), s.npairs
+    labels = [fa3(t) for t in s.type_names]
+    weight = np.zeros((2, nfile, npairs), F32)
+    for i, w in enumerate(wts):
+        for a1, a2, w1, w2 in w.pairs:                 # wtype (:4540-4558)
+            n1 = max([k + 1 for k, lab in enumerate(labels) if lab == a1], default=0)
+            n2 = max([k + 1 for k, lab in enumerate(labels) if lab == a2], default=0)
+            if n1 > 0 and n2 > 0:
+                ic = int(s.npairref[n1 - 1, n2 - 1])
+                weight[0, i, ic], weight[1, i, ic] = w1, w2
+    cfraccg = s.cfraccg if s.cfraccg is not None else s.cfrac
+    ciwt = np.zeros((nfile, npairs), F32)
+    cwt = np.zeros((nfile, npairs), F32)
+    wsum = np.zeros(nfile, F32)
+    xwt = np.zeros(nfile, F32)
+    for i in range(nfile):
+        for ic in range(npairs):
+            it, jt = int(s.n1extr[ic]), int(s.n2extr[ic])
+            cprod = F32(cfraccg[it]) * F32(cfraccg[jt])
+            if it != jt:
+                cprod = F32(2.0) * cprod
+            ciwt[i, ic] = cprod * weight[0, i, ic]
+            cwt[i, ic] = cprod * weight[1, i, ic]
+            wsum[i] = F32(float(wsum[i]) + float(abs(cwt[i, ic])))
+            xwt[i] = F32(float(xwt[i]) + float(cwt[i, ic]))
+    out = dict(ciwt=ciwt, cwt=cwt, wsum=wsum, xwt=xwt)
+    nnew = npairs + nfile
+    if recompute or cwtt is None:
+        cwtt = np.zeros((npairs, nnew))
+        for ip in range(npairs):
+            cwtt[ip, nfile + ip] = 1.0
+        cwtpot = np.zeros((nfile, npairs))
+        for i in range(nfile):
+            for ic in range(npairs):
+                it, jt = int(s.n1extr[ic]), int(s.n2extr[ic])
+                if cfraccg[it] > 0.0 and cfraccg[jt] > 0.0:
+                    rat = F32(s.cfrac[it]) * F32(s.cfrac[jt]) / (F32(cfraccg[it]) * F32(cfraccg[jt]))
+                else:
+                    rat = F32(0.0)
+                cwtpot[i, ic] = float(rat) * float(cwt[i, ic])
+        invert_cwt(ctx, cwtpot, cwtt, feedback, efacm, ran1_state)
+    out["cwtt"] = cwtt
+    pot = cwtt[:, :nfile].copy()
+    for ic in range(npairs):
+        if s.n1extr[ic] != s.n2extr[ic]:
+            pot[ic] = 0.5 * pot[ic]
+    out["cwtpot"] = pot
+    return out
+
+
+def fa3(label):
+    return (label + "   ")[:3]
+
+
 class Refinement:
     def __init__(self, ctx: D.Context, system, idum: int = -1, timing: bool = False):
         self.ctx = ctx

@@ -206,6 +206,8 @@ class System:
     # coefficient (read_pcof); None means zero for every pair
     rminex: np.ndarray = None
     pcoefex: np.ndarray = None
+    # per type, the coarse-grained atomic fractions (molecom.cfraccg); None means cfrac
+    cfraccg: np.ndarray = None
 
     @property
     def nmol(self):

@@ -5,6 +5,8 @@ Sources (data files of the reference, not code):
       (3200 atoms, cubic, L = 33.1213 A) with its saved ran1 state and its 116-term EP
   EPSR-test/DMSO.ato, test_problems/200DMSO400Water/EPSR-test-ex46/WaterMolecule.ato
       -- single-molecule files
+  test_problems/200DMSO400Water/EPSR-test-ex8/DMSODWater{D,Null,H}.NWTStot.wts
+      -- the neutron weights of the test box's three datasets
 Run from the repository root where /root/reference exists:
   python tests/golden/epsr_test/make_fixtures.py
 """
@@ -15,7 +17,10 @@ import shutil
 REF = "/root/reference"
 HERE = os.path.dirname(os.path.abspath(__file__))
 FILES = ["EPSR-test/200DMSO400Water.ato", "EPSR-test/200DMSO400Water.pcof", "EPSR-test/200DMSO400Water.EPSR.inp",
-         "EPSR-test/DMSO.ato", "test_problems/200DMSO400Water/EPSR-test-ex46/WaterMolecule.ato"]
+         "EPSR-test/DMSO.ato", "test_problems/200DMSO400Water/EPSR-test-ex46/WaterMolecule.ato",
+         "test_problems/200DMSO400Water/EPSR-test-ex8/DMSODWaterD.NWTStot.wts",
+         "test_problems/200DMSO400Water/EPSR-test-ex8/DMSODWaterNull.NWTStot.wts",
+         "test_problems/200DMSO400Water/EPSR-test-ex8/DMSODWaterH.NWTStot.wts"]
 
 for rel in FILES:
     dst = os.path.join(HERE, os.path.basename(rel) + ".gz")

@@ -124,3 +124,13 @@ def test_fortran_field_formats():
     assert FM.lw_real(F32(-9.93487984e-02)) == "  -9.93487984E-02" and FM.lw_int(21) == "          21"
     assert FM.fe(float("nan"), 12, 5) == "         NaN"
     assert FM.words("feedback    0.9               Confidence factor", 12) == ["feedback", "0.9"]
+
+
+def test_wts_files_read_every_pair():
+    for n in ("DMSODWaterD", "DMSODWaterNull", "DMSODWaterH"):
+        w = FM.read_wts(os.path.join(D, n + ".NWTStot.wts.gz"))
+        assert w.rhofac == 1.0 and len(w.pairs) == 21
+        labs = {(a, b) for a, b, _, _ in w.pairs}
+        assert ("S  ", "S  ") in labs and ("H1 ", "H1 ") in labs
+    w = FM.read_wts(os.path.join(D, "DMSODWaterH.NWTStot.wts.gz"))
+    assert w.pairs[-1] == ("H1 ", "H1 ", F32(0.139801204), F32(0.139801204))
