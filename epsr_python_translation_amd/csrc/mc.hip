@@ -682,6 +682,23 @@ __device__ PairTab tables_to_lds(const PairTab &g, int npairs, char *smem)
 // One wave per chain row; the non-zero energies of a row and their partners are packed to
 // the front of the row's item range (rowoff, int64 prefix over k) in neighbour order, and
 // their number goes to rowcnt[k].
+// position of the r-th set bit (0-based) of m, for r < popcount(m)
+__device__ __forceinline__ int nth_set_bit(unsigned long long m, int r)
+{
+    int pos = 0, c = __popc((unsigned)m);
+    if (r >= c) { r -= c; m >>= 32; pos += 32; }
+    c = __popc((unsigned)m & 0xffffu);
+    if (r >= c) { r -= c; m >>= 16; pos += 16; }
+    c = __popc((unsigned)m & 0xffu);
+    if (r >= c) { r -= c; m >>= 8; pos += 8; }
+    c = __popc((unsigned)m & 0xfu);
+    if (r >= c) { r -= c; m >>= 4; pos += 4; }
+    c = __popc((unsigned)m & 0x3u);
+    if (r >= c) { r -= c; m >>= 2; pos += 2; }
+    if (r >= (int)(m & 1u)) pos += 1;
+    return pos;
+}
+
 template <bool LDS>
 __global__ void fmolec1_pairs(const long long *__restrict__ rowoff, int nmol, const int *__restrict__ order,
                               const int *__restrict__ mcell, const int *__restrict__ cstart,
@@ -700,25 +717,51 @@ __global__ void fmolec1_pairs(const long long *__restrict__ rowoff, int nmol, co
         const int *nb = nbflat + cstart[mcell[im]];
         const float4 *P1 = P + naf[im];
         const int n1 = nal[im] - naf[im] + 1;
-        int base = 0;
-        for (int i0 = 0; i0 < len; i0 += 64) {   // wave-uniform trip count
-            const int inn = i0 + lane;
+        // Only candidates later in the numbering are evaluated (half of them).  They are
+        // gathered, in candidate order, into a queue of one partner per lane and evaluated
+        // 64 at a time, so no lane idles on an earlier partner; the stored energies keep the
+        // candidate order.
+        int base = 0, qj = 0, qn = 0;   // qj: this lane's queued partner, valid for lane < qn
+        auto flush = [&](int nvalid) {
             float e = 0.0f;
-            int jm = 0;
-            if (inn < len) {
-                jm = nb[inn];
-                if (inn >= 1 && jm > im) e = mol_pair_any(P1, n1, P + naf[jm], nal[jm] - naf[jm] + 1, t);
-            }
+            if (lane < nvalid) e = mol_pair_any(P1, n1, P + naf[qj], nal[qj] - naf[qj] + 1, t);
             const bool keep = e != 0.0f;
             const unsigned long long mask = __ballot(keep);
             if (keep) {
                 const int at = base + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
                                                                      __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
                 rowE[r0 + at] = e;
-                rowJ[r0 + at] = jm;
+                rowJ[r0 + at] = qj;
             }
             base += __popcll(mask);
+        };
+        for (int i0 = 0; i0 < len; i0 += 64) {   // wave-uniform trip count
+            const int inn = i0 + lane;
+            int jm = 0;
+            bool work = false;
+            if (inn < len) {
+                jm = nb[inn];
+                work = inn >= 1 && jm > im;
+            }
+            const unsigned long long wm = __ballot(work);
+            const int cnt = __popcll(wm);
+            // queue lanes qn .. qn + cnt - 1 (those below 64) take the working candidates in order
+            const int r = lane - qn;
+            const bool take = r >= 0 && r < cnt;
+            const int v = __shfl(jm, take ? nth_set_bit(wm, r) : lane);
+            if (take) qj = v;
+            if (qn + cnt >= 64) {
+                flush(64);
+                const int r2 = lane + 64 - qn;   // the rest: ranks 64 - qn .. cnt - 1
+                const bool rest = r2 < cnt;
+                const int v2 = __shfl(jm, rest ? nth_set_bit(wm, r2) : lane);
+                qj = rest ? v2 : 0;
+                qn = qn + cnt - 64;
+            } else {
+                qn += cnt;
+            }
         }
+        if (qn > 0) flush(qn);
         if (lane == 0) rowcnt[k] = base;
     }
 }
