@@ -119,6 +119,7 @@ def main():
     ap.add_argument("--no-small-box", action="store_true")
     ap.add_argument("--no-da", action="store_true", help="skip the surrogate / delayed-acceptance leg")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-box ensemble leg")
+    ap.add_argument("--no-extras", action="store_true", help="skip press and the running averages after the step")
     ap.add_argument("--da-samples", type=float, default=2e6, help="training pairs for the surrogate")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 256 MiB device-to-device (the known-size reference for PMC byte counters)")
@@ -250,7 +251,8 @@ def main():
         "accepted_fraction": (R.mc.stats.accepted / max(1, R.mc.stats.moves)),
         "roofline": roof,
     }
-    out["iteration_extras"] = iteration_extras(R, s)
+    if not args.no_extras:
+        out["iteration_extras"] = iteration_extras(R, s)
     if not args.no_small_box:
         out["dmso_water_3200"] = small_box_rate(D, PL, synth, ctx)
     if not args.no_da:
