@@ -120,6 +120,7 @@ def main():
     ap.add_argument("--no-da", action="store_true", help="skip the surrogate / delayed-acceptance leg")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-box ensemble leg")
     ap.add_argument("--no-extras", action="store_true", help="skip press and the running averages after the step")
+    ap.add_argument("--no-replicas", action="store_true", help="skip the 16-replica SiO2 refinement leg")
     ap.add_argument("--da-samples", type=float, default=2e6, help="training pairs for the surrogate")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy 256 MiB device-to-device (the known-size reference for PMC byte counters)")
@@ -259,6 +260,8 @@ def main():
         out["surrogate_da"] = surrogate_leg(D, PL, synth, ctx, int(args.da_samples))
     if not args.no_ensemble:
         out["ensemble_64"] = ensemble_leg(synth)
+    if not args.no_replicas:
+        out["replicas_16"] = replicas_leg(ctx)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(s)
     print(json.dumps(out), flush=True)
@@ -374,6 +377,37 @@ def ensemble_leg(synth, sweeps=5, steps=2, workers=8):
         del ens
     res["speedup_streams"] = res[f"workers_1"]["ms_per_step"] / res[f"workers_{workers}"]["ms_per_step"]
     return res
+
+
+def replicas_leg(ctx, nrep=16, workers=8, iters=2, sweeps=1):
+    """16 replicas of the SiO2 refinement (8000 Si + 16,000 O each, three weighted S(Q)
+    targets from another configuration, 41-term EP per pair): per iteration and replica the
+    ledger, `sweeps` MC cycles, atogen, gensq, the running averages and ptgnsq (FP64
+    normal-equation fit, EP update, generate_ep).  One warm-up iteration, then `iters`
+    timed; the reference's schedule is 50 iterations."""
+    import torch
+
+    from epsr_python_translation_amd import _native
+    from epsr_python_translation_amd import replicas as RP
+    t0 = time.perf_counter()
+    wts, data, fr = RP.synthetic_targets(ctx)
+    ens = RP.ReplicaRefinement(nrep=nrep, nworkers=workers)
+    ens.set_targets(wts, data)
+    setup = time.perf_counter() - t0
+    rf = [float(np.mean([r[0] for r in ens.iterate(sweeps=sweeps, fit_mode=_native.FIT_NORMAL)]))]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        res = ens.iterate(sweeps=sweeps, fit_mode=_native.FIT_NORMAL)
+        rf.append(float(np.mean([r[0] for r in res])))
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / iters
+    atoms = sum(s.natom for s in ens.systems)
+    del ens
+    return {"workload": f"{nrep} replicas x SiO2 24,000 atoms, 3 datasets (isotope fractions "
+                        f"{', '.join(f'{x:.2f}' for x in fr)}), {sweeps} MC sweep(s) per iteration",
+            "setup_s": setup, "ms_per_iteration": 1e3 * dt, "atom_moves_per_s": sweeps * atoms / dt,
+            "projected_50_iterations_s": 50 * dt, "mean_rfactor_by_iteration": rf}
 
 
 def surrogate_leg(D, PL, synth, ctx, nsamples, timed_sweeps=20, gr_sweeps=20):

@@ -63,7 +63,7 @@ def get_wts(ctx, s, wts, feedback, efacm, ran1_state, cwtt=None, recompute=True)
     parsed weights files `wts` (formats.read_wts, one per dataset): per dataset and type
     pair the weights ciwt (intra) and cwt (inter), wsum = sum |cwt| and xwt = sum cwt, and,
     when `recompute` (ireset = 1 or feedback/efacm changed), the inverted matrix through
-    invert_cwt from cwtt = [0 | I].  Returns a dict with ciwt, cwt [nfile, npairs] (float32),
+    invert_cwt from cwtt = [0 | I] (else from the given cwtt; with neither, cwtpot is None).  Returns a dict with ciwt, cwt [nfile, npairs] (float32),
     wsum, xwt [nfile], cwtt [npairs, npairs + nfile] and cwtpot [npairs, nfile] (float64)."""
     nfile, npairs = len(wts), s.npairs
     labels = [fa3(t) for t in s.type_names]
@@ -92,7 +92,7 @@ def get_wts(ctx, s, wts, feedback, efacm, ran1_state, cwtt=None, recompute=True)
             xwt[i] = F32(float(xwt[i]) + float(cwt[i, ic]))
     out = dict(ciwt=ciwt, cwt=cwt, wsum=wsum, xwt=xwt)
     nnew = npairs + nfile
-    if recompute or cwtt is None:
+    if recompute:
         cwtt = np.zeros((npairs, nnew))
         for ip in range(npairs):
             cwtt[ip, nfile + ip] = 1.0
@@ -107,6 +107,9 @@ def get_wts(ctx, s, wts, feedback, efacm, ran1_state, cwtt=None, recompute=True)
                 cwtpot[i, ic] = float(rat) * float(cwt[i, ic])
         invert_cwt(ctx, cwtpot, cwtt, feedback, efacm, ran1_state)
     out["cwtt"] = cwtt
+    if cwtt is None:           # no inversion asked for and none given
+        out["cwtpot"] = None
+        return out
     pot = cwtt[:, :nfile].copy()
     for ic in range(npairs):
         if s.n1extr[ic] != s.n2extr[ic]:
@@ -216,6 +219,59 @@ class Refinement:
         fc1, fc2 = sum_factors(nsum, nsums, nesum)
         D.sumsq(ctx, s.npairs, g.nq, self.ldq, self.sofq, fc1, fc2, self.soq, self.sq2)
         D.sumgr(ctx, s.npairs, g.npt1, self.ldw, self.wav, self.ycoh, fc1, fc2, self.gor, self.gr2, self.entsig)
+
+    def ptgnsq(self, data, cwt, cwtpot, nqmin=None, szeros=None, potfac=1.0, fit_mode=None, npits=1000,
+               paccept=5.0e-4, pstepd=0.002):
+        """One refinement of the empirical potential from the running intermolecular S(Q)
+        (accumulate first), as ptgnsq does it (epsr_standalone_rev3.py:3789-4021, neutron
+        weights): per dataset the difference data - sum_ic cwt soqinter, a Poisson-basis fit
+        of it (fit_expansion), the update of pcoefd/pcoefp, then generate_ep.
+        data: float32 [nfile, nq + 1] (Q points 1..nq); cwt float32 [nfile, npairs];
+        cwtpot float64/32 [npairs, nfile] (get_wts).  Returns (r-factors, chi^2) per dataset."""
+        ctx, s, g = self.ctx, self.s, self.g
+        nfile, npairs, nq, ldq = data.shape[0], s.npairs, g.nq, self.ldq
+        fit_mode = _native.FIT_MC if fit_mode is None else fit_mode
+        nterms = s.nterms
+        if getattr(self, "pcoefd", None) is None:
+            self.pcoefd = np.zeros((nfile + 1, nterms + 1), F32)
+            self.pstepd = np.full(nfile + 1, F32(pstepd), F32)
+        nqmin = np.full(nfile, 2, np.int32) if nqmin is None else np.asarray(nqmin, np.int32)
+        nqmax = np.full(nfile, nq, np.int32)
+        szeros = np.zeros(nfile, F32) if szeros is None else np.asarray(szeros, F32)
+        dd = ctx.tensor(np.ascontiguousarray(np.asarray(data, F32)[:, :ldq]).reshape(-1))
+        qwind = self.soq[npairs * ldq:2 * npairs * ldq]
+        cw = np.ascontiguousarray(np.asarray(cwt, F32)).reshape(-1)
+        qt = D.ptgnsq_diff(ctx, nfile, nq, npairs, self.q, dd, ldq, qwind, cw, szeros, nqmin, nqmax, F32(0.0))
+        pcp = np.zeros((npairs + 1, nterms + 1), F32)
+        pcp[1:] = self.pcoefp
+        irefp = np.arange(1, npairs + 1, dtype=np.int32)
+        efacp = np.ones(npairs + 1, F32)
+        cwp = np.zeros((npairs, nfile + 1), F32)
+        cwp[:, 1:] = np.asarray(cwtpot, F32)
+        rfac = np.zeros(nfile, F32)
+        chisq = np.zeros(nfile, F32)
+        for i in range(nfile):
+            idn = i + 1
+            nqm = int(nqmax[i])
+            expsq, _ = D.calc_pwrexpq(ctx, nqm, nterms, s.pdmin, s.pdstep, s.rmaxpt, s.psigma2, s.rho, self.q)
+            dt = -qt[i * ldq:i * ldq + nqm + 1]
+            pw = np.zeros(nterms + 1, F32)
+            pw[1:] = self.pcoefd[idn, 1:]
+            dts, dw, dwo = ctx.zeros(nqm + 1), ctx.tensor(pw), ctx.tensor(pw)
+            r = D.fit_expansion(ctx, dt.contiguous(), expsq, dts, dw, dwo, self.pstepd[idn], F32(paccept), 0.0, 0.0,
+                                1, nqm, 1, nterms, npits, 10, self.ran1 if fit_mode == _native.FIT_MC else None,
+                                mode=fit_mode)
+            self.pstepd[idn] = r.stepr[0]
+            chisq[i] = r.chisqnew[0]
+            D.ptgnsq_update(ctx, idn, nterms, npairs, dwo.cpu().numpy(), irefp, efacp, cwp.reshape(-1), nfile + 1,
+                            F32(potfac), self.pcoefd, pcp, nterms + 1)
+            i0 = int(nqmin[i]) if szeros[i] == 0.0 else 1
+            fit_full = ctx.zeros(ldq)
+            fit_full[:nqm + 1] = dts
+            rfac[i] = D.ptgnsq_rfactor(ctx, i0, nq, fit_full, qt[i * ldq:(i + 1) * ldq].contiguous())
+        self.pcoefp = np.ascontiguousarray(pcp[1:])
+        self.generate_ep()
+        return rfac, chisq
 
     def press(self, energf=None, pressf=None, wsum=None):
         """press (ato_manipulation_rev4.py:3901-4063) over the current ledger: (energy per
