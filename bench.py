@@ -11,7 +11,7 @@ The headline metric is atom-moves/s over the whole step; pair-distances/s over t
 step is reported beside it.  A second, smaller workload (200 DMSO + 400 water, 3200
 atoms, one sweep per step) gives the MC rate of the reference's test case.
 
-Prints one JSON line.  --gpus N > 1 is rejected: this build runs on one GPU.
+Prints one JSON line.  --gpus N > 1 is rejected: the library runs on one GPU.
 """
 
 from __future__ import annotations
@@ -126,7 +126,7 @@ def main():
                     help="first copy 256 MiB device-to-device (the known-size reference for PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding)",
+        print("bench.py runs on one GPU only: the library has no multi-GPU path",
               file=sys.stderr)
         sys.exit(2)
 
@@ -200,7 +200,7 @@ def main():
     ktot = {k: float(np.sum(v)) / args.steps for k, v in ktime.items()}
     kavg = {k: float(np.mean(v)) for k, v in ktime.items()}
 
-    # algorithmic work per launch of each kernel family (DESIGN.md section 4)
+    # algorithmic work per launch of each kernel family
     npair_entries = R.ledger.nergpair - int(np.count_nonzero(s.eintra))
     apm = float(np.mean(nat))                                     # atoms per molecule (3 for every water)
     lookups_per_sweep = 3.0 * npair_entries * apm * apm          # A for both slots + Bv for the later one

@@ -1,6 +1,6 @@
 """Fill the COMMON mirror from a synthetic box (epsr_python_translation_amd.synth), the
 way the reference's readers fill it from .ato/.EPSR.inp files (those formats are not
-part of this build): 1-based atoms, molecules, types and pairs, column-major tables."""
+part of this package): 1-based atoms, molecules, types and pairs, column-major tables."""
 
 from __future__ import annotations
 

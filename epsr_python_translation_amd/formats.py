@@ -909,7 +909,7 @@ def system_from_files(ato, pcof=None, inp=None, name=None):
     rmaxgr <= 0 or beyond half the box becomes half the box (epsr_standalone_rev3.py:2938-2941).
 
     The GPU path is cubic and rigid: a non-cubic box or Q-type atoms are rejected.  The
-    intramolecular energy of each molecule (en_mole_intra) is outside this build and is set
+    intramolecular energy of each molecule (en_mole_intra) is outside this package and is set
     to the same fixed placeholder the synthetic boxes use (it shifts the ledger total only).
     """
     from . import synth

@@ -20,7 +20,7 @@ def molfit(fadds, faddbs, nisum, intra, rotfreq, ireset):
     ledger of the last fmolec1.  Returns (fadds, faddbs, nisum)."""
     global last_stats
     if nisum >= intra:
-        raise NotImplementedError("molfit: the intramolecular branch (nisum >= intra) is not part of this build")
+        raise NotImplementedError("molfit: the intramolecular branch (nisum >= intra) is not part of this package")
     if not gpu.has_ledger():
         raise RuntimeError("molfit: no pair-energy ledger (call fmolec1 first)")
     c = gpu.ctx()

@@ -5,7 +5,7 @@ values and its host bookkeeping.
 The pair sums run in `epsr_press` (csrc/press.hip) over the ledger's partner lists; this
 module adds what press does on the host side of those sums: abs_energy_ep (:4128-4157)
 and the per-dataset energf/pressf normalisation (:4056-4061).  Three-body potentials and
-coordination constraints are outside this build (energthreeb and prthreebout are 0).
+coordination constraints are outside this package (energthreeb and prthreebout are 0).
 """
 
 from __future__ import annotations

@@ -168,7 +168,7 @@ def ptgnsq(nsum, nq, q, wav, data, qwind, rminpt, rmaxpt, szeros, overallfac, gf
     c = gpu.ctx()
     nfile, npairs = epsr16.nfile, molecom.npairs
     if np.any(weights.xfile[1:nfile + 1] > 0):
-        raise NotImplementedError("ptgnsq: X-ray weights are not part of this build")
+        raise NotImplementedError("ptgnsq: X-ray weights are not part of this package")
     ldq = int(nq) + 1
     qminptemp = epsr16.qminm if epsr16.qminp <= 0.0 else epsr16.qminp
     dq = c.tensor(np.asarray(q[:ldq], F32))

@@ -3,7 +3,7 @@
 
 Every replica is a full `pipeline.Refinement` with its own seed and ran1 stream; one
 iteration of a replica is what the reference's runepsr17 loop does between potential
-refinements, restricted to this build's path: the ledger, `sweeps` MC cycles, atogen,
+refinements, restricted to this package's path: the ledger, `sweeps` MC cycles, atogen,
 gensq, the running averages (sumsq/sumgr) and ptgnsq (difference from the data, the
 Poisson-basis fit, the EP update and generate_ep).  Replicas run on the ensemble's worker
 streams and threads, so one replica's serial host steps overlap the others' kernels; a

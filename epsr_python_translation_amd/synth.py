@@ -277,7 +277,7 @@ def build_system(name, seed, L, molecules, type_names, *, cellst, rmaxgr, qstep,
             naf.append(atom)
             nal.append(atom + len(names) - 1)
             mtype.append(mt)
-            # intramolecular energy (en_mole_intra is outside this build): multi-atom
+            # intramolecular energy (en_mole_intra is outside this package): multi-atom
             # molecules carry a fixed non-zero value so their self ledger slot is used,
             # single atoms carry 0 as EPSR computes for them
             eintra.append(0.001 * len(names) if len(names) > 1 else 0.0)
