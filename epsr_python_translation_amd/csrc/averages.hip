@@ -60,19 +60,38 @@ __global__ void running_average(int rows, int jlo, int npts, int ld, const float
     accumulate(add1, fc1, fc2, avg + o, err + o);
 }
 
-// sumgr's zero limit of pair ic: entsig(j) = sum over j' = 2..j of ycoh (g_inter - 1)
-__global__ void zero_limit(int npairs, int nz, int ld, const float *__restrict__ ycoh,
-                           const float *__restrict__ gor, float *__restrict__ entsig)
+// sumgr's zero limit of pair ic: entsig(j) = sum over j' = 2..j of ycoh (g_inter - 1).
+// One workgroup per pair: the float32 terms are formed in parallel into LDS, one lane
+// then runs the double running sum in order over LDS, and the results go out in parallel.
+constexpr int ZL_CHUNK = 4096;
+__global__ void __launch_bounds__(256)
+zero_limit(int npairs, int nz, int ld, const float *__restrict__ ycoh, const float *__restrict__ gor,
+           float *__restrict__ entsig)
 {
-    const int ic = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ic >= npairs) return;
+    __shared__ float term[ZL_CHUNK];
+    __shared__ double carry;
+    const int ic = blockIdx.x;
     const float *g = gor + ((size_t)npairs + ic) * ld;
     float *e = entsig + (size_t)ic * ld;
-    double z = 0.0;
-    e[1] = 0.0f;
-    for (int j = 2; j <= nz; ++j) {
-        z = z + (double)(ycoh[j] * (g[j] - 1.0f));
-        e[j] = (float)z;
+    if (threadIdx.x == 0) {
+        e[1] = 0.0f;
+        carry = 0.0;
+    }
+    for (int j0 = 2; j0 <= nz; j0 += ZL_CHUNK) {
+        const int n = min(ZL_CHUNK, nz - j0 + 1);
+        for (int k = threadIdx.x; k < n; k += blockDim.x) term[k] = ycoh[j0 + k] * (g[j0 + k] - 1.0f);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double z = carry;
+            for (int k = 0; k < n; ++k) {
+                z = z + (double)term[k];
+                term[k] = (float)z;
+            }
+            carry = z;
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < n; k += blockDim.x) e[j0 + k] = term[k];
+        __syncthreads();
     }
 }
 
@@ -101,7 +120,7 @@ extern "C" int epsr_sumgr(epsr_ctx *ctx, int npairs, int nz, int ldw, const floa
     const long n = 2L * npairs * (nz - 1);
     running_average<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(2 * npairs, 2, nz - 1, ldw, wav, ycoh, fc1,
                                                                           fc2, gor, gr2);
-    zero_limit<<<(npairs + 63) / 64, 64, 0, ctx->stream>>>(npairs, nz, ldw, ycoh, gor, entsig);
+    zero_limit<<<npairs, 256, 0, ctx->stream>>>(npairs, nz, ldw, ycoh, gor, entsig);
     HIP_CHECK(ctx, hipGetLastError());
     return EPSR_OK;
 }
