@@ -33,3 +33,27 @@ def test_replica_in_ensemble_equals_replica_alone():
     assert np.array_equal(alone.boxes[0].pcoefp.view(np.int32), got.view(np.int32))
     assert np.array_equal(res1[1][0][0].view(np.int32), res[1][1][0].view(np.int32))
     assert np.all(np.isfinite(res[1][1][0])) and np.all(np.isfinite(res[1][1][1]))
+
+
+def test_replica_iteration_with_the_monte_carlo_fit():
+    """The same iteration with fit_expansion's exact Monte Carlo replay: it draws from the
+    replica's ran1 stream, and two runs from the same state agree bit for bit."""
+    from epsr_python_translation_amd import _native
+    from epsr_python_translation_amd import device as D
+    from epsr_python_translation_amd import replicas as RP
+    print("replicas: Monte Carlo fit, 1 replica x 1 iteration, twice", flush=True)
+    ctx = D.Context(0)
+    wts, data, _ = RP.synthetic_targets(ctx)
+    out = []
+    for _ in range(2):
+        ens = RP.ReplicaRefinement(nrep=1, nworkers=1, seed0=42, idum0=-4002)
+        ens.set_targets(wts, data)
+        st0 = ens.boxes[0].ran1.copy()
+        rf, chi = ens.iterate(sweeps=1, fit_mode=_native.FIT_MC)[0]
+        torch.cuda.synchronize()
+        assert not np.array_equal(ens.boxes[0].ran1, st0)
+        assert np.all(np.isfinite(rf)) and np.all(np.isfinite(chi))
+        out.append((ens.boxes[0].pcoefp.copy(), ens.boxes[0].ran1.copy(), chi))
+        del ens
+    assert np.array_equal(out[0][0].view(np.int32), out[1][0].view(np.int32))
+    np.testing.assert_array_equal(out[0][1], out[1][1])
