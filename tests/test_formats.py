@@ -134,3 +134,17 @@ def test_wts_files_read_every_pair():
         assert ("S  ", "S  ") in labs and ("H1 ", "H1 ") in labs
     w = FM.read_wts(os.path.join(D, "DMSODWaterH.NWTStot.wts.gz"))
     assert w.pairs[-1] == ("H1 ", "H1 ", F32(0.139801204), F32(0.139801204))
+
+
+def test_pcof_with_new_coefficients_reads_back_exactly(tmp_path):
+    """After a refinement changes the coefficients, what write_pcof writes reads back to the
+    same float32 values (nine significant digits carry every float32)."""
+    p = FM.read_pcof(os.path.join(D, "200DMSO400Water.pcof.gz"))
+    rng = np.random.default_rng(8)
+    for pr in p.pairs:
+        pr.pcoefp = (rng.standard_normal(pr.pcoefp.shape) * 10.0 ** rng.uniform(-8, 3, pr.pcoefp.shape)).astype(F32)
+    path = tmp_path / "new.pcof"
+    FM.write_pcof(path, p)
+    q = FM.read_pcof(path)
+    for a, b in zip(p.pairs, q.pairs):
+        assert np.array_equal(a.pcoefp.view(np.int32), b.pcoefp.view(np.int32))
