@@ -1942,8 +1942,10 @@ mc_resolve_pass(int nmol, const int *__restrict__ order, const int *__restrict__
     }
 }
 
-// the remaining passes in one workgroup of 16 waves, with a barrier between passes,
-// until no move is undecided; lists ping-pong between wa and wb
+// the remaining passes in one workgroup (up to 16 waves, as many as the per-wave term
+// rows of maxc slots fit in LDS), with a barrier between passes, until no move is
+// undecided; lists ping-pong between wa and wb.  Then the acceptance bits by chain
+// position from the decisions.
 __global__ void __launch_bounds__(1024)
 mc_resolve_finish(const int *__restrict__ order, const int *__restrict__ rowstart, const int *__restrict__ cnt,
                   const int *__restrict__ pnv, const int *__restrict__ mw,
@@ -1951,11 +1953,11 @@ mc_resolve_finish(const int *__restrict__ order, const int *__restrict__ rowstar
                   const float *__restrict__ delta, const float2 *__restrict__ thr, const float *__restrict__ stream,
                   const int *__restrict__ moff, float temp, const float *__restrict__ shift, int maxc,
                   int *__restrict__ wa, const int *__restrict__ na, int *__restrict__ wb, unsigned char *state,
-                  int *nexact, int *npass)
+                  int *nexact, int *npass, int nmol, unsigned *__restrict__ bits)
 {
-    extern __shared__ float2 fterm2[];   // [16 waves][maxc]
+    extern __shared__ float2 fterm2[];   // [waves][maxc]
     __shared__ int ncur, nnext;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = (int)(blockDim.x >> 6);
     float2 *term = fterm2 + (size_t)wv * maxc;
     if (threadIdx.x == 0) {
         ncur = *na;
@@ -1965,7 +1967,7 @@ mc_resolve_finish(const int *__restrict__ order, const int *__restrict__ rowstar
     int *cur = wa, *nxt = wb;
     int passes = 0;
     while (ncur > 0) {
-        for (int w = wv; w < ncur; w += 16) {
+        for (int w = wv; w < ncur; w += nwv) {
             const int k = cur[w];
             const int d = mc_pass_move(k, order, rowstart, cnt, pnv, mw, A, Bv, L, delta, thr, stream, moff, temp,
                                        shift, state, term, nexact);
@@ -1987,20 +1989,16 @@ mc_resolve_finish(const int *__restrict__ order, const int *__restrict__ rowstar
         __syncthreads();
     }
     if (threadIdx.x == 0) *npass = passes;
+    for (int w = threadIdx.x; w < (nmol + 31) / 32; w += blockDim.x) {
+        unsigned b = 0u;
+        for (int j = 0; j < 32; ++j) {
+            const int k = w * 32 + j;
+            if (k < nmol && state[k] == 1) b |= 1u << j;
+        }
+        bits[w] = b;
+    }
 }
 
-// acceptance bits (by chain position) from the decisions
-__global__ void mc_state_bits(int nmol, const unsigned char *__restrict__ state, unsigned *__restrict__ bits)
-{
-    const int w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= (nmol + 31) / 32) return;
-    unsigned b = 0u;
-    for (int j = 0; j < 32; ++j) {
-        const int k = w * 32 + j;
-        if (k < nmol && state[k] == 1) b |= 1u << j;
-    }
-    bits[w] = b;
-}
 
 // Delayed acceptance, stage 1, by chain position k: dEs from the network output of
 // rows 2k (current) and 2k+1 (trial), or from des by molecule; the move passes with
@@ -2453,8 +2451,7 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
                                                           dpc + 2, dstate, dpc + 3);
             mc_resolve_finish<<<1, 64 * nwf, sizeof(float2) * nwf * (size_t)maxc, st>>>(
                 l->d_order, l->d_rowstart, l->d_cnt, l->d_pn, l->d_mw, dA, dB, dL, ddelta, dthr, stream, moff, temp,
-                dshift, maxc, dW0, dpc + 2, dW1, dstate, dpc + 3, dpc + 4);
-            mc_state_bits<<<ceil_div((nmol + 31) / 32, 256), 256, 0, st>>>(nmol, dstate, dacc);
+                dshift, maxc, dW0, dpc + 2, dW1, dstate, dpc + 3, dpc + 4, nmol, dacc);
             timer_end(ctx, "mc_resolve");
             // fadd only reports: the exact fdiff of the accepted moves is formed on the side
             // stream beside the commits, and summed on the host (a sequential float sum is
