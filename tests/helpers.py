@@ -29,6 +29,9 @@ def epsr_test_files():
 def system(name: str):
     if name == "epsr_test":
         return formats.system_from_files(*epsr_test_files(), name="epsr_test")
+    if name == "dense_1400":
+        # 1400 waters in a 30 A box: every molecule in every ledger row (rows of 1400 slots)
+        return synth.tiny(seed=5, nwater=1400, L=30.0)
     if name == "c0":
         return synth.config0()
     if name == "c1":

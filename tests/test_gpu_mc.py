@@ -131,7 +131,7 @@ def _run_oracle_molfit(name, flavour, nsweep, idum):
                 ran1=rng.as_array(), fadds=F32(fadds.value))
 
 
-@pytest.mark.parametrize("name,nsweep", [("tiny_13_12_3", 3), ("c0", 2), ("c1", 1), ("epsr_test", 2)])
+@pytest.mark.parametrize("name,nsweep", [("tiny_13_12_3", 3), ("c0", 2), ("c1", 1), ("epsr_test", 2), ("dense_1400", 1)])
 def test_molfit_bit_exact_with_correctly_rounded_libm(ctx, name, nsweep):
     from epsr_python_translation_amd import device as D
     print(f"molfit parity {name} sweeps={nsweep}", flush=True)
