@@ -72,16 +72,20 @@ __device__ __forceinline__ floatx4 mfma_tile_16x16(const float *__restrict__ A, 
 }
 
 // gensq, mode 1.  B(j, col): intramolecular wav for col < npairs, (wav2 - ycoh) *
-// natomsphere for the intermolecular columns; rows j = 2..npt1.  grid.x: 16-row
-// tiles of Q points, grid.y: 16-column tiles; one wave per workgroup.
+// natomsphere for the intermolecular columns; rows j = 2..npt1.  grid.x: 16-row tiles of Q
+// points, grid.y: 16-column tiles, grid.z: GENSQ_KSPLIT slices of the r rows (a handful of
+// columns gives too few tiles to fill the GPU otherwise); one wave per workgroup writes its
+// slice's partial tile, and gensq_reduce adds the slices in a fixed order.
+constexpr int GENSQ_KSPLIT = 16;
 __global__ void __launch_bounds__(64) gensq_mfma(int nq, int npairs, int npt1, int ldw, const float *__restrict__ wav,
                                                  const float *__restrict__ ycoh, const float *__restrict__ foqols,
                                                  int ldf, const float *__restrict__ natomsphere,
-                                                 const float *__restrict__ colsum, float *__restrict__ sofq, int ldq)
+                                                 float *__restrict__ part)
 {
-    const int ncol = 2 * npairs;
+    const int ncol = 2 * npairs, ncolp = (int)gridDim.y * 16;
     const int m0 = blockIdx.x * 16;   // Q rows, 0-based: Q index = m + 1
     const int n0 = blockIdx.y * 16;
+    const int kz = blockIdx.z;
     auto bfun = [&](int j, int col) -> float {
         if (col >= ncol) return 0.0f;
         if (col < npairs) return wav[(size_t)col * ldw + j];
@@ -89,16 +93,29 @@ __global__ void __launch_bounds__(64) gensq_mfma(int nq, int npairs, int npt1, i
         float nas = natomsphere ? natomsphere[ic] : 1.0f;
         return (wav[((size_t)npairs + ic) * ldw + j] - ycoh[j]) * nas;
     };
-    // A rows start at foqols + ldf (Q index 1); columns j are direct indices
-    floatx4 acc = mfma_tile_16x16(foqols + ldf, ldf, nq, m0, n0, 2, npt1 + 1, bfun);
+    // A rows start at foqols + ldf (Q index 1); columns j are direct indices 2..npt1
+    const int K0 = 2, K1 = npt1 + 1;
+    const int span = (((K1 - K0) + (int)gridDim.z - 1) / (int)gridDim.z + 3) & ~3;
+    const int kb = K0 + kz * span, ke = min(K1, kb + span);
+    floatx4 acc = mfma_tile_16x16(foqols + ldf, ldf, nq, m0, n0, kb, ke, bfun);
     const int lane = threadIdx.x & 63;
     const int col = n0 + (lane & 15);
-    if (col >= ncol) return;
-    bool zero = col >= npairs && colsum[col - npairs] <= 0.0f;
     for (int v = 0; v < 4; ++v) {
         int m = m0 + 4 * (lane >> 4) + v;
-        if (m < nq) sofq[(size_t)col * ldq + m + 1] = zero ? 0.0f : acc[v];
+        if (m < nq) part[((size_t)kz * ncolp + col) * nq + m] = acc[v];
     }
+}
+
+__global__ void gensq_reduce(int nq, int npairs, int ncolp, int ksplit, const float *__restrict__ part,
+                             const float *__restrict__ colsum, float *__restrict__ sofq, int ldq)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2L * npairs * nq) return;
+    const int col = (int)(t / nq), m = (int)(t - (long)col * nq);
+    float sum = 0.0f;
+    for (int kz = 0; kz < ksplit; ++kz) sum = sum + part[((size_t)kz * ncolp + col) * nq + m];
+    const bool zero = col >= npairs && colsum[col - npairs] <= 0.0f;
+    sofq[(size_t)col * ldq + m + 1] = zero ? 0.0f : sum;
 }
 
 // sum3 of gensq (:4785-4793) is only tested for <= 0.  The intermolecular histogram is
@@ -135,11 +152,16 @@ extern "C" int epsr_gensq(epsr_ctx *ctx, int nq, int npairs, int npt1, int ldw, 
         int n = nq * 2 * npairs;
         gensq_exact<<<ceil_div(n, 64), 64, 0, st>>>(nq, npairs, npt1, ldw, wav, ycoh, foqols, ldf, natomsphere, sofq, ldq);
     } else {
-        float *cs = (float *)ctx_scratch(ctx, sizeof(float) * npairs);
-        if (!cs) return EPSR_ERR_HIP;
+        const int ncolp = 16 * ceil_div(2 * npairs, 16);
+        const size_t csb = (sizeof(float) * npairs + 255) & ~(size_t)255;
+        char *sc = (char *)ctx_scratch(ctx, csb + sizeof(float) * (size_t)GENSQ_KSPLIT * ncolp * nq);
+        if (!sc) return EPSR_ERR_HIP;
+        float *cs = (float *)sc, *part = (float *)(sc + csb);
         gensq_colsum<<<npairs, 256, 0, st>>>(npairs, npt1, ldw, wav, cs);
-        dim3 grid(ceil_div(nq, 16), ceil_div(2 * npairs, 16));
-        gensq_mfma<<<grid, 64, 0, st>>>(nq, npairs, npt1, ldw, wav, ycoh, foqols, ldf, natomsphere, cs, sofq, ldq);
+        dim3 grid(ceil_div(nq, 16), ncolp / 16, GENSQ_KSPLIT);
+        gensq_mfma<<<grid, 64, 0, st>>>(nq, npairs, npt1, ldw, wav, ycoh, foqols, ldf, natomsphere, part);
+        gensq_reduce<<<ceil_div(2L * npairs * nq, 256), 256, 0, st>>>(nq, npairs, ncolp, GENSQ_KSPLIT, part, cs,
+                                                                      sofq, ldq);
     }
     timer_end(ctx, "gensq");
     HIP_CHECK(ctx, hipGetLastError());
