@@ -234,34 +234,51 @@ __global__ void stog_prepare_b(const float *__restrict__ q, const float *__restr
 }
 
 // stog, mode 1: A(j, i) = q_i sin(q_i r_j) (q_i^2 at r = 0) generated in registers.
-__global__ void __launch_bounds__(64) stog_mfma(float rho, const float *__restrict__ q, int nqpad,
-                                                const float *__restrict__ B, const float *__restrict__ r,
-                                                float *__restrict__ gofr, int ldg, int nr, int ncols,
-                                                const float *__restrict__ xwt)
+// grid.z slices of the Q points (STOG_KSPLIT): each wave writes its slice's partial
+// tile; stog_reduce adds the slices in a fixed order and applies rho, the 1/r and xwt
+constexpr int STOG_KSPLIT = 8;
+__global__ void __launch_bounds__(64) stog_mfma(const float *__restrict__ q, int nqpad, const float *__restrict__ B,
+                                                const float *__restrict__ r, int nr, int ncols,
+                                                float *__restrict__ part)
 {
     const int lane = threadIdx.x & 63;
-    const int m0 = blockIdx.x * 16, n0 = blockIdx.y * 16;
+    const int m0 = blockIdx.x * 16, n0 = blockIdx.y * 16, kz = blockIdx.z;
+    const int ncolp = (int)gridDim.y * 16;
     const int rl = lane & 15, kq = lane >> 4;
     const int jrow = m0 + rl + 1;   // r index (1-based)
     const float rr = jrow <= nr ? r[jrow] : 0.0f;
     const int ncol = n0 + rl;
-    const float sfac = stog_sfac(q, rho);
+    const int span = ((nqpad + (int)gridDim.z - 1) / (int)gridDim.z + 3) & ~3;
+    const int kb = kz * span, ke = min(nqpad, kb + span);
     floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int k = 0; k < nqpad; k += 4) {
+    for (int k = kb; k < ke; k += 4) {
         int i = k + kq;
-        float qq = i < nqpad ? q[i] : 0.0f;
+        float qq = i < ke ? q[i] : 0.0f;
         float a = 0.0f;
-        if (jrow <= nr && i < nqpad && i >= 1) a = rr > 0.0f ? qq * sinf(qq * rr) : qq * qq;
-        float bv = (ncol < ncols && i < nqpad) ? B[(size_t)ncol * nqpad + i] : 0.0f;
+        if (jrow <= nr && i < ke && i >= 1) a = rr > 0.0f ? qq * sinf(qq * rr) : qq * qq;
+        float bv = (ncol < ncols && i < ke) ? B[(size_t)ncol * nqpad + i] : 0.0f;
         acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv, acc, 0, 0, 0);
     }
-    if (ncol >= ncols) return;
+    const int col = n0 + (lane & 15);
     for (int v = 0; v < 4; ++v) {
-        int j = m0 + 4 * kq + v + 1;
-        if (j > nr) continue;
-        float rj = r[j];
-        gofr[(size_t)ncol * ldg + j] = rj > 0.0f ? xwt[ncol] + sfac * acc[v] / rj : xwt[ncol] + sfac * acc[v];
+        int j = m0 + 4 * kq + v;   // 0-based row
+        if (j < nr) part[((size_t)kz * ncolp + col) * nr + j] = acc[v];
     }
+}
+
+__global__ void stog_reduce(float rho, const float *__restrict__ q, int nr, int ncols, int ncolp, int ksplit,
+                            const float *__restrict__ part, const float *__restrict__ r, float *__restrict__ gofr,
+                            int ldg, const float *__restrict__ xwt)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)ncols * nr) return;
+    const int col = (int)(t / nr), j0 = (int)(t - (long)col * nr);
+    float sum = 0.0f;
+    for (int kz = 0; kz < ksplit; ++kz) sum = sum + part[((size_t)kz * ncolp + col) * nr + j0];
+    const float sfac = stog_sfac(q, rho);
+    const int j = j0 + 1;
+    const float rj = r[j];
+    gofr[(size_t)col * ldg + j] = rj > 0.0f ? xwt[col] + sfac * sum / rj : xwt[col] + sfac * sum;
 }
 
 extern "C" int epsr_stog(epsr_ctx *ctx, float rho, const float *q, const float *sofq, int ldq, const float *r,
@@ -281,11 +298,16 @@ extern "C" int epsr_stog(epsr_ctx *ctx, float rho, const float *q, const float *
     } else {
         // the Q range used is 1..ldq-1 (nqmax <= ldq-1 is the caller's contract)
         int nqpad = ldq;
-        float *B = (float *)ctx_scratch(ctx, sizeof(float) * (size_t)nqpad * ncols);
-        if (!B) return EPSR_ERR_HIP;
+        const int ncolp = 16 * ceil_div(ncols, 16);
+        const size_t bb = (sizeof(float) * (size_t)nqpad * ncols + 255) & ~(size_t)255;
+        char *sc = (char *)ctx_scratch(ctx, bb + sizeof(float) * (size_t)STOG_KSPLIT * ncolp * nr);
+        if (!sc) return EPSR_ERR_HIP;
+        float *B = (float *)sc, *part = (float *)(sc + bb);
         stog_prepare_b<<<ceil_div(nqpad * ncols, 256), 256, 0, st>>>(q, sofq, ldq, nqmin, nqmax, nqpad, ncols, B);
-        dim3 grid(ceil_div(nr, 16), ceil_div(ncols, 16));
-        stog_mfma<<<grid, 64, 0, st>>>(rho, q, nqpad, B, r, gofr, ldg, nr, ncols, xwt);
+        dim3 grid(ceil_div(nr, 16), ncolp / 16, STOG_KSPLIT);
+        stog_mfma<<<grid, 64, 0, st>>>(q, nqpad, B, r, nr, ncols, part);
+        stog_reduce<<<ceil_div((long)ncols * nr, 256), 256, 0, st>>>(rho, q, nr, ncols, ncolp, STOG_KSPLIT, part, r,
+                                                                     gofr, ldg, xwt);
     }
     timer_end(ctx, "stog");
     HIP_CHECK(ctx, hipGetLastError());
