@@ -1,0 +1,298 @@
+"""Write tests/golden/ref_tinybox.npz: input/output vectors of the reference's own routines.
+
+Run by hand in the build container only (never on a GPU box, never by a test):
+
+    python tests/golden/make_ref_vectors.py [--out tests/golden/ref_tinybox.npz]
+
+It imports the reference's Python translation from /root/reference/src and runs its `epsr`
+program (``epsr26.run``) on a reduced copy of the reference's EPSR-test case: the test box's
+own DMSO and water molecules (bonds, angle constraints, rotational groups and all), 4 DMSO +
+20 water re-centred at seeded random positions in a 16 A cubic box, with the case's .pcof, .wts
+and data files unchanged and these .EPSR.inp changes: iinit 1 (fresh accumulators, so the first
+MC cycle is an intramolecular shake), ntimes 6 (shake, four whole-molecule cycles, one cycle
+with internal rotations at rotfreq 5), ngrsamples 10 (atogen's random-origin branch), nmolcell 4
+(several cells), nq 200.  While it runs, the routines on the hot path are wrapped in their
+modules (the reference's own calls go through module attributes, src/epsr26/trace.py:1-12)
+and each call's inputs and outputs are recorded:
+
+  cells         cellneighbourlist.initialise_cellneighbourlist (library/cellneighbourlist.py:23)
+  fmolec1.<k>   ato_manipulation_rev4.fmolec1 (library/ato_manipulation_rev4.py:2374)
+  molfit.<k>    molfit.molfit (library/molfit.py:51)
+  atogen.<k>    ato_manipulation_rev4.atogen (library/ato_manipulation_rev4.py:3640)
+  fit.<k>       epsr_standalone_rev3.fit_expansion (programs/epsr_standalone_rev3.py:4806)
+  genep.<k>     epsr_standalone_rev3.generate_ep (programs/epsr_standalone_rev3.py:5876)
+
+plus the box topology and the potential tables the MC cycles use.  Arrays keep the
+reference's 1-based layout with the unused element 0, sliced to the used extent.  The
+transcendentals come from this container's glibc (recorded as `glibc`), the library the
+oracle's glibc flavour calls.
+"""
+
+from __future__ import annotations
+
+import argparse
+import dataclasses
+import os
+import platform
+import shutil
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+REF = "/root/reference"
+F32 = np.float32
+
+
+def make_case(work):
+    """The reduced EPSR-test case in `work` (written with this package's byte-exact
+    .ato/.EPSR.inp writers)."""
+    sys.path.insert(0, REPO)
+    from epsr_python_translation_amd import formats as F
+
+    shutil.copytree(os.path.join(REF, "EPSR-test"), work, dirs_exist_ok=True)
+    for f in os.listdir(work):
+        os.chmod(os.path.join(work, f), 0o644)
+    ato = os.path.join(work, "200DMSO400Water.ato")
+    a = F.read_ato(ato)
+    L = 16.0
+    mols = [m for m in a.molecules if m.nato == 10][:4] + [m for m in a.molecules if m.nato == 3][:20]
+    rng = np.random.default_rng(7)
+    cen = []
+    for _ in mols:
+        while True:
+            c = rng.uniform(-L / 2, L / 2, 3)
+            if all(np.linalg.norm((c - d) - L * np.round((c - d) / L)) > 3.2 for d in cen):
+                break
+        cen.append(c)
+    mols = [dataclasses.replace(m, centre=np.array(c, F32)) for m, c in zip(mols, cen)]
+    b = dataclasses.replace(a, molecules=mols, nmol=len(mols), dmax=F32(L),
+                            lattice=(F32(L), F32(L), F32(L)) + tuple(a.lattice[3:]))
+    F.write_ato(ato, b)
+    inp = os.path.join(work, "200DMSO400Water.EPSR.inp")
+    kf = F.read_keywords(inp)
+    for k, v in (("iinit", "1"), ("ntimes", "6"), ("ngrsamples", "10"), ("nmolcell", "4"), ("nq", "200")):
+        kf.set(k, v)
+    F.write_keywords(inp, kf)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=os.path.join(HERE, "ref_tinybox.npz"))
+    args = ap.parse_args()
+    work = tempfile.mkdtemp(prefix="epsr_ref_")
+    make_case(work)
+    sys.path.insert(0, os.path.join(REF, "src"))
+    from epsr26 import run
+    from epsr26.common import cellneighbourlist as cnl
+    from epsr26.common import clustercom as cc
+    from epsr26.common import epsr16 as e16
+    from epsr26.common import epsrinput_par as ei
+    from epsr26.common import ergpairreference as erd
+    from epsr26.common import molecom as mc
+    from epsr26.common import molecom_mod as mm
+    from epsr26.common import potential_par as pp
+    from epsr26.common import ran1com as rc
+    from epsr26.library import ato_manipulation_rev4 as am
+    from epsr26.library import cellneighbourlist as cnl_lib
+    from epsr26.library import molfit as mf
+    from epsr26.programs import epsr_standalone_rev3 as es
+
+    rec = {}
+    count = {}
+
+    def put(prefix, d):
+        for k, v in d.items():
+            rec[f"{prefix}.{k}"] = np.array(v).copy()
+
+    def nxt(name):
+        k = count.get(name, 0)
+        count[name] = k + 1
+        return f"{name}.{k}"
+
+    def ran1():
+        return np.array([rc.mcall, rc.idum, rc.iy] + [int(rc.iv[i]) for i in range(1, 33)], np.int32)
+
+    def coords():
+        na, nm = mc.natom, cc.nmol
+        return dict(xa=mc.xa[1:na + 1], ya=mc.ya[1:na + 1], za=mc.za[1:na + 1],
+                    xadd=cc.xadd[1:nm + 1], yadd=cc.yadd[1:nm + 1], zadd=cc.zadd[1:nm + 1])
+
+    def ledger():
+        nm = cc.nmol
+        return dict(nnbrefmax=erd.nnbrefmax, nergpair=erd.nergpair, nergpairm=erd.nergpairm[1:nm + 1],
+                    molergpair=erd.molergpair[:, 1:nm + 1], ergpairref=erd.ergpairref[:, 1:nm + 1],
+                    ergpair=erd.ergpair[1:erd.nergpair + 1])
+
+    def mcstate():
+        nm, nt = cc.nmol, cc.nmoltype
+        return dict(wholerot=cc.wholerot[1:nt + 1], wholetrans=cc.wholetrans[1:nt + 1],
+                    internrot=cc.internrot[1:nt + 1], interntrans=cc.interntrans[1:nt + 1],
+                    nstepreject=cc.nstepreject[1:nm + 1], nwhole=cc.nwhole[:, 1:nt + 1],
+                    nmovem=cc.nmovem[:, 1:nt + 1], ninter=cc.ninter[:, 1:nt + 1], nmovei=cc.nmovei[:, 1:nt + 1],
+                    ran1=ran1())
+
+    def tables():
+        n2, np_ = e16.npt2, mc.npairs
+        d = {k: getattr(pp, k)[:n2 + 1, 1:np_ + 1].T for k in ("xavs", "xavs1", "ecoul", "ecoul1", "xav", "xav1")}
+        d.update(xstar=e16.xstar[:n2 + 1], x=e16.x[:n2 + 1], npt2=n2, npt1=e16.npt1, npt=e16.npt)
+        return d
+
+    def topology():
+        na, nm, nt, np_ = mc.natom, cc.nmol, mc.ntype, mc.npairs
+        ngr = len(mm.rotgrp) - 1
+        ndih = max([int(mc.ndihedralang[i]) for i in range(1, ngr + 1)] + [0])
+        d = dict(natom=na, nmol=nm, ntype=nt, npairs=np_, nmoltype=cc.nmoltype,
+                 itype=mc.itype[1:na + 1], npairref=mc.npairref[1:nt + 1, 1:nt + 1],
+                 n1extr=mc.n1extr[1:np_ + 1], n2extr=mc.n2extr[1:np_ + 1], cfrac=mc.cfrac[1:nt + 1],
+                 natomr=mc.natomr, sizefactor=mc.sizefactor, dmaxmol=mc.dmaxmol, qtype=mc.qtype[1:na + 1],
+                 imageatom=mc.imageatom[1:na + 1], nbond=mc.nbond[1:na + 1], iabond=mc.iabond[1:na + 1],
+                 tbond=mc.tbond[1:na + 1], amassr=mc.amassr[1:np_ + 1], mref=mc.mref[1:na + 1],
+                 tempb=mc.tempb, tempangwt=mc.tempangwt, tempdihwt=mc.tempdihwt, tempangreq=mc.tempangreq,
+                 tempdihreq=mc.tempdihreq, simt=mc.simt, refine_intra=mc.refine_intra,
+                 refine_intra_fac=mc.refine_intra_fac, stepr=mc.stepr, stepm=mc.stepm, stepri=mc.stepri,
+                 stepmi=mc.stepmi, tolout=mc.tolout, labrotdih=mc.labrotdih[:, :ndih + 1, :ngr + 1],
+                 ndihedralang=mc.ndihedralang[:ngr + 1], dihedralang=mc.dihedralang[:ndih + 1, :ngr + 1],
+                 naf=cc.naf[1:nm + 1], nal=cc.nal[1:nm + 1], nato=cc.nato[1:nm + 1], mtype=cc.mtype[1:nm + 1],
+                 ngrot=cc.ngrot[1:nm + 1], nrf=cc.nrf[1:nm + 1], nrl=cc.nrl[1:nm + 1], dmax=cc.dmax,
+                 tethered=cc.tethered[1:nm + 1], labaxi=mm.labaxi, rotf=mm.rotf, rotl=mm.rotl, labrot=mm.labrot,
+                 rotgrp=np.array([str(s) for s in mm.rotgrp], dtype="U3"), coarsegraining=mm.coarsegraining,
+                 cellst=e16.cellst, rejrate=e16.rejrate, rho=e16.rho)
+        return d
+
+    def wrap(mod, name, before, after):
+        orig = getattr(mod, name)
+
+        def w(*a):
+            key = before(a)
+            out = orig(*a)
+            after(key, a, out)
+            return out
+        setattr(mod, name, w)
+
+    # ---- cells
+    def cells_b(a):
+        key = nxt("cells")
+        put(key, dict(nmolcellin=a[0], nmol=a[4], rmax=a[5]))
+        return key
+
+    def cells_a(key, a, out):
+        nc = cnl.ncell
+        put(key, dict(ncell=nc, ncelldiva=cnl.ncelldiva, ncelldivb=cnl.ncelldivb, ncelldivc=cnl.ncelldivc,
+                      maxcellno=cnl.maxcellno, ncellneighb=cnl.ncellneighb, nmolsincell=cnl.nmolsincell[1:nc + 1],
+                      molsincell=cnl.molsincell[:, 1:nc + 1], molcellno=cnl.molcellno[1:cc.nmol + 1]))
+    wrap(cnl_lib, "initialise_cellneighbourlist", cells_b, cells_a)
+
+    # ---- fmolec1
+    def fm_b(a):
+        key = nxt("fmolec1")
+        if "topo.natom" not in rec:
+            put("topo", topology())
+        put(key, coords())
+        put(key + ".tables", tables())
+        return key
+
+    def fm_a(key, a, out):
+        put(key, ledger())
+        put(key, dict(ftest=out[2], ftestb=out[3]))
+    wrap(am, "fmolec1", fm_b, fm_a)
+
+    # ---- molfit
+    def mf_b(a):
+        key = nxt("molfit")
+        put(key + ".in", coords())
+        put(key + ".in", mcstate())
+        put(key + ".in", ledger())
+        put(key, dict(fadds=a[0], faddbs=a[1], nisum=a[2], intra=a[3], rotfreq=a[4], ireset=a[5],
+                      stepr=mc.stepr, stepm=mc.stepm, stepri=mc.stepri, stepmi=mc.stepmi, simt=mc.simt,
+                      rejrate=e16.rejrate))
+        return key
+
+    def mf_a(key, a, out):
+        put(key + ".out", coords())
+        put(key + ".out", mcstate())
+        put(key + ".out", ledger())
+        put(key + ".out", dict(fadds=out[0], faddbs=out[1], nisum=out[2]))
+    wrap(mf, "molfit", mf_b, mf_a)
+
+    # ---- atogen
+    def at_b(a):
+        key = nxt("atogen")
+        put(key + ".in", coords())
+        put(key + ".in", dict(ngrsamples=a[1], ran1=ran1(), ycoh=e16.ycoh[:e16.npt1 + 1],
+                              irefpcof=pp.irefpcof[1:mc.npairs + 1], rminex=pp.rminex[1:mc.npairs + 1]))
+        return key
+
+    def at_a(key, a, out):
+        wav = a[0]
+        put(key + ".out", dict(ngrsamples=out, ran1=ran1(), wav=wav[1:3, 1:e16.npt1 + 1, 1:mc.npairs + 1],
+                               potex=pp.potex[1:mc.npairs + 1], potexmax=pp.potexmax[1:mc.npairs + 1]))
+    wrap(am, "atogen", at_b, at_a)
+
+    # ---- fit_expansion(yin, expr, yout, wt, wtold, stepr, acceptr, rejrat, chisqnew, nmin, npt,
+    #                    ncfirst, nterms, nits, nfreq)
+    def fit_b(a):
+        key = nxt("fit")
+        npt, nterms = int(a[10]), int(a[12])
+        put(key + ".in", dict(yin=a[0][:npt + 1], expr=a[1][:npt * nterms + 1], yout=a[2][:npt + 1],
+                              wt=a[3][:nterms + 1], wtold=a[4][:nterms + 1], stepr=a[5], acceptr=a[6],
+                              rejrat=a[7], chisqnew=a[8], nmin=a[9], npt=npt, ncfirst=a[11], nterms=nterms,
+                              nits=a[13], nfreq=a[14], ran1=ran1()))
+        return key
+
+    def fit_a(key, a, out):
+        npt, nterms = int(a[10]), int(a[12])
+        put(key + ".out", dict(yout=a[2][:npt + 1], wt=a[3][:nterms + 1], wtold=a[4][:nterms + 1],
+                               stepr=out[0], acceptr=out[1], rejrat=out[2], chisqnew=out[3], ran1=ran1()))
+    wrap(es, "fit_expansion", fit_b, fit_a)
+
+    # ---- generate_ep
+    def ep_b(a):
+        key = nxt("genep")
+        n = pp.ncoeffp
+        put(key + ".in", dict(changepotfac=a[0], pcoefp=pp.pcoefp[:n + 1, 1:mc.npairs + 1].T,
+                              irefpcof=pp.irefpcof[1:mc.npairs + 1], ncoeffp=n, pdmin=pp.pdmin, pdstep=pp.pdstep,
+                              pdmax=pp.pdmax, psigma2=pp.psigma2, rho=e16.rho, absolute_energy=pp.absolute_energy,
+                              erequnit=pp.erequnit, ereqstep=pp.ereqstep, npt=e16.npt, npt2=e16.npt2,
+                              x=e16.x[:e16.npt + 1], xstar=e16.xstar[:e16.npt + 1],
+                              wfac=pp.wfac[:e16.npt + 1], ecoul=pp.ecoul[:e16.npt + 1, 1:mc.npairs + 1].T,
+                              dmax=cc.dmax, rbroad=ei.rbroad, gaussian=pp.gaussian,
+                              npgauss=pp.npgauss[1:mc.npairs + 1]))
+        return key
+
+    def ep_a(key, a, out):
+        n = pp.ncoeffp
+        d = {k: getattr(pp, k)[:e16.npt + 1, 1:mc.npairs + 1].T for k in ("xavs", "xavs1", "xavtot", "xavtot1")}
+        d["pcoefp"] = pp.pcoefp[:n + 1, 1:mc.npairs + 1].T
+        put(key + ".out", d)
+    wrap(es, "generate_ep", ep_b, ep_a)
+
+    cwd = os.getcwd()
+    os.chdir(work)
+    try:
+        status = run.main(["epsr", "./", "epsr", "200DMSO400Water.EPSR.inp"])
+    finally:
+        os.chdir(cwd)
+    if status != 0:
+        raise SystemExit(f"reference epsr run ended with status {status}")
+    # identical arrays are stored once; a later copy becomes the string "=<first key>"
+    seen, out = {}, {}
+    for k, v in rec.items():
+        sig = (v.dtype.str, v.shape, v.tobytes())
+        if v.size > 16 and sig in seen:
+            out[k] = np.array("=" + seen[sig])
+        else:
+            seen.setdefault(sig, k)
+            out[k] = v
+    rec = out
+    rec["meta.glibc"] = np.array("-".join(platform.libc_ver()))
+    rec["meta.counts"] = np.array(sorted(count.items()), dtype=object).astype(str)
+    np.savez_compressed(args.out, **rec)
+    shutil.rmtree(work)
+    print("wrote", args.out, {k: v for k, v in count.items()})
+
+
+if __name__ == "__main__":
+    main()

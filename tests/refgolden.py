@@ -1,0 +1,143 @@
+"""Reader of tests/golden/ref_tinybox.npz (written by tests/golden/make_ref_vectors.py from the
+reference's own routines) and builders of the oracle's state from it.
+
+The fixture keeps the reference's 1-based layout (element 0 unused, molecule and atom
+numbers 1-based); the builders convert to the oracle's 0-based conventions.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import functools
+import os
+
+import numpy as np
+
+from oracle import pyoracle as po
+
+F32 = np.float32
+PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_tinybox.npz")
+
+
+@functools.lru_cache(maxsize=None)
+def load():
+    raw = np.load(PATH, allow_pickle=False)
+    d = {k: raw[k] for k in raw.files}
+    for k, v in list(d.items()):
+        if v.dtype.kind == "U" and v.shape == () and str(v).startswith("="):
+            d[k] = d[str(v)[1:]]
+    return d
+
+
+def group(prefix):
+    d = load()
+    n = len(prefix) + 1
+    return {k[n:]: v for k, v in d.items() if k.startswith(prefix + ".")}
+
+
+def count(name):
+    d = load()
+    return len({k.split(".")[1] for k in d if k.startswith(name + ".")})
+
+
+class State:
+    """Oracle-side copies of the recorded box: topology (0-based), coordinates, tables."""
+
+    def __init__(self, coords, tables):
+        t = group("topo")
+        self.t = t
+        self.nmol, self.natom = int(t["nmol"]), int(t["natom"])
+        self.ntype, self.npairs, self.nmoltype = int(t["ntype"]), int(t["npairs"]), int(t["nmoltype"])
+        c = np.ascontiguousarray
+        self.xa, self.ya, self.za = (c(coords[k], F32).copy() for k in ("xa", "ya", "za"))
+        self.xadd, self.yadd, self.zadd = (c(coords[k], F32).copy() for k in ("xadd", "yadd", "zadd"))
+        self.itype = c(t["itype"] - 1, np.int32)
+        self.npairref = c((t["npairref"] - 1).reshape(-1), np.int32)
+        self.n1extr = c(t["n1extr"] - 1, np.int32)
+        self.n2extr = c(t["n2extr"] - 1, np.int32)
+        self.naf = c(t["naf"] - 1, np.int32)
+        self.nal = c(t["nal"] - 1, np.int32)
+        self.mtype = c(t["mtype"] - 1, np.int32)
+        self.cfrac = c(t["cfrac"], F32)
+        self.box = po.Box(self.nmol, self.natom, self.ntype, self.npairs, self.nmoltype, po.Pf(self.xa),
+                          po.Pf(self.ya), po.Pf(self.za), po.Pi(self.itype), po.Pi(self.npairref),
+                          po.Pi(self.n1extr), po.Pi(self.n2extr), po.Pi(self.naf), po.Pi(self.nal),
+                          po.Pi(self.mtype), po.Pf(self.cfrac), F32(t["natomr"]), F32(t["sizefactor"]),
+                          F32(t["dmax"]), F32(t["cellst"]))
+        self.centres = po.Centres(po.Pf(self.xadd), po.Pf(self.yadd), po.Pf(self.zadd))
+        self.tab = {k: c(tables[k], F32).reshape(-1).copy() for k in ("xavs", "xavs1", "ecoul", "ecoul1")}
+        self.xstar = c(tables["xstar"], F32)
+        self.ld = int(tables["xavs"].shape[1])
+        self.npt2 = int(tables["npt2"])
+        self.tables = po.Tables(self.ld, self.npt2, po.Pf(self.xstar), po.Pf(self.tab["xavs"]),
+                                po.Pf(self.tab["xavs1"]), po.Pf(self.tab["ecoul"]), po.Pf(self.tab["ecoul1"]))
+
+
+def ledger_from_record(cap, nmol):
+    """An oracle ledger holding a recorded ledger (0-based partners and entries)."""
+    M = int(cap["nnbrefmax"])
+    ne = int(cap["nergpair"])
+    keep = {}
+    keep["nergpairm"] = np.ascontiguousarray(cap["nergpairm"], np.int32)
+    mol = np.zeros((nmol, M), np.int32)
+    ref = np.zeros((nmol, M), np.int32)
+    for im in range(nmol):
+        n = int(cap["nergpairm"][im])
+        mol[im, :n] = cap["molergpair"][1:n + 1, im] - 1
+        ref[im, :n] = cap["ergpairref"][1:n + 1, im] - 1
+    keep["molergpair"] = np.ascontiguousarray(mol.reshape(-1))
+    keep["ergpairref"] = np.ascontiguousarray(ref.reshape(-1))
+    keep["ergpair"] = np.ascontiguousarray(np.concatenate([cap["ergpair"].astype(F32), np.zeros(1, F32)]))
+    led = po.Ledger()
+    led.nmol, led.nnbrefmax, led.nergpair, led.nergpairmax = nmol, M, ne, ne + 1
+    led.ergpair = po.Pf(keep["ergpair"])
+    led.molergpair = po.Pi(keep["molergpair"])
+    led.ergpairref = po.Pi(keep["ergpairref"])
+    led.nergpairm = po.Pi(keep["nergpairm"])
+    led._keep = keep
+    return led
+
+
+def ledger_arrays(led, nmol):
+    """(nergpairm, partners per row, entry values per row) of an oracle ledger, 1-based
+    partners so rows compare with the record directly."""
+    M = led.nnbrefmax
+    nm = np.ctypeslib.as_array(led.nergpairm, shape=(nmol,)).copy()
+    mol = np.ctypeslib.as_array(led.molergpair, shape=(nmol * M,)).reshape(nmol, M)
+    ref = np.ctypeslib.as_array(led.ergpairref, shape=(nmol * M,)).reshape(nmol, M)
+    erg = np.ctypeslib.as_array(led.ergpair, shape=(max(led.nergpair, 1),))
+    rows = [(mol[im, :nm[im]] + 1).copy() for im in range(nmol)]
+    refs = [(ref[im, :nm[im]] + 1).copy() for im in range(nmol)]
+    vals = [erg[ref[im, :nm[im]]].copy() for im in range(nmol)]
+    return nm, rows, refs, vals, erg[:led.nergpair].copy()
+
+
+def record_rows(cap, nmol):
+    nm = cap["nergpairm"].astype(np.int32)
+    rows = [cap["molergpair"][1:nm[im] + 1, im].astype(np.int32) for im in range(nmol)]
+    refs = [cap["ergpairref"][1:nm[im] + 1, im].astype(np.int32) for im in range(nmol)]
+    erg = cap["ergpair"].astype(F32)
+    vals = [erg[r - 1] for r in refs]
+    return nm, rows, refs, vals, erg
+
+
+def self_energies(cap, nmol):
+    """Each molecule's intramolecular energy as the reference stored it (slot 1 of its row
+    when that slot is the molecule itself; 0 when nothing was stored)."""
+    out = np.zeros(nmol, F32)
+    for im in range(nmol):
+        if int(cap["nergpairm"][im]) > 0 and int(cap["molergpair"][1, im]) == im + 1:
+            out[im] = cap["ergpair"][int(cap["ergpairref"][1, im]) - 1]
+    return out
+
+
+def ran1(a):
+    return po.Ran1.from_array(np.asarray(a, np.int32))
+
+
+def f32_bits(a):
+    return np.ascontiguousarray(a, F32).view(np.int32)
+
+
+def as_ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
