@@ -195,6 +195,53 @@ int ora_molfit_da(const ora_box_t *b, float *xa, float *ya, float *za, const ora
                   const float *des, unsigned long long seed, int *nrej_out, double *target_out,
                   float *fdiff_out);
 
+/* ---- intramolecular energies and moves (ora_intra.c; molfit.py:545-655, 922-1524;
+ *      ato_manipulation_rev4.py:442-475) ----
+ * Atoms, molecules and rotational groups 0-based.  Bonds: atom ia has nbond[ia] entries
+ * iabond/tbond[ia*ldb + j] (partner atom, length in molecule box units; a negative length
+ * is an angle constraint).  imageatom[ia] is -1 when ia has none.  Rotational groups of
+ * molecule im: nrf[im]..nrl[im] (empty when nrl < nrf); group g rotates labrot[rotf[g]..rotl[g]]
+ * about the axis labaxi[2g] -> labaxi[2g+1]; isdih[g] marks a DIH group with ndihedralang[g]
+ * dihedrals (atom pairs labrotdih[(g*ldd + kd)*2 + 0/1], angles dihedralang[g*ldd + kd],
+ * radians).  xav/xav1 [npairs*ld]: the reference potential the intramolecular non-bonded
+ * pairs use (potential_par.xav/xav1).  tempang/tempdih are set by ora_molfit_full from the
+ * weights in ora_mcfull_t. */
+typedef struct {
+    const int *qtype, *imageatom, *mref, *nbond, *iabond;
+    const float *tbond;
+    int ldb;
+    const float *amassr, *amassa;
+    const int *ngrot, *nrf, *nrl, *labaxi, *rotf, *rotl, *labrot, *isdih, *ndihedralang, *labrotdih;
+    const float *dihedralang;
+    int ldd;
+    const float *xav, *xav1;
+    float dmaxmol, tempb, simt, tempang, tempdih, refine_intra_fac;
+    int refine_intra;
+} ora_intra_t;
+
+float ora_en_atom_intra(const ora_box_t *b, const ora_tables_t *t, const ora_intra_t *in, int ia, int ja,
+                        float *sumevib, float *sumeang);
+float ora_en_dihedral(const float *co1, const float *co2, const float *cd, float dihedralang);
+float ora_en_mole_dihedral(const ora_box_t *b, const ora_intra_t *in, int im);
+float ora_en_mole_intra(const ora_box_t *b, const ora_tables_t *t, const ora_intra_t *in, int im, float *sumevib,
+                        float *sumeang, float *sumedih);
+
+/* the complete molfit cycle (molfit.py:51-541): the shake branch when *nisum >= intra,
+ * else whole-molecule moves over the cells with internal rotations every rotfreq cycles;
+ * all four step sizes adapted.  Counters [2*nmoltype]: [2*mt] attempts, [2*mt+1]
+ * rejections.  Returns the number of Metropolis tests. */
+typedef struct {
+    float *wholerot, *wholetrans, *internrot, *interntrans;   /* [nmoltype] */
+    int *nwhole, *nmovem, *ninter, *nmovei;                    /* [2*nmoltype] */
+    int *nstepreject;                                          /* [nmol] */
+    float simt, stepr, stepm, stepri, stepmi, rejrate, tempangwt, tempdihwt;
+    int ireset;
+} ora_mcfull_t;
+int ora_molfit_full(const ora_box_t *b, float *xa, float *ya, float *za, const ora_centres_t *c,
+                    const ora_tables_t *t, ora_intra_t *in, const ora_cells_t *cl, ora_ledger_t *l,
+                    ora_mcfull_t *mc, ora_ran1_t *rng, float *fadds, float *faddbs, int *nisum, int intra,
+                    int rotfreq);
+
 /* ---- the surrogate's descriptors and network (ora_surrogate.c) ---- */
 void ora_descriptor(int na, const float *xm, const float *ym, const float *zm, const int *tm, int nb,
                     const float *xn, const float *yn, const float *zn, const int *tn, const int *npairref,

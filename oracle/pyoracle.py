@@ -102,6 +102,22 @@ class MCState(C.Structure):
                 ("stepm", C.c_float), ("rejrate", C.c_float), ("ireset", C.c_int)]
 
 
+class Intra(C.Structure):
+    _fields_ = [("qtype", ip), ("imageatom", ip), ("mref", ip), ("nbond", ip), ("iabond", ip), ("tbond", fp),
+                ("ldb", C.c_int), ("amassr", fp), ("amassa", fp), ("ngrot", ip), ("nrf", ip), ("nrl", ip),
+                ("labaxi", ip), ("rotf", ip), ("rotl", ip), ("labrot", ip), ("isdih", ip), ("ndihedralang", ip),
+                ("labrotdih", ip), ("dihedralang", fp), ("ldd", C.c_int), ("xav", fp), ("xav1", fp),
+                ("dmaxmol", C.c_float), ("tempb", C.c_float), ("simt", C.c_float), ("tempang", C.c_float),
+                ("tempdih", C.c_float), ("refine_intra_fac", C.c_float), ("refine_intra", C.c_int)]
+
+
+class MCFull(C.Structure):
+    _fields_ = [("wholerot", fp), ("wholetrans", fp), ("internrot", fp), ("interntrans", fp), ("nwhole", ip),
+                ("nmovem", ip), ("ninter", ip), ("nmovei", ip), ("nstepreject", ip), ("simt", C.c_float),
+                ("stepr", C.c_float), ("stepm", C.c_float), ("stepri", C.c_float), ("stepmi", C.c_float),
+                ("rejrate", C.c_float), ("tempangwt", C.c_float), ("tempdihwt", C.c_float), ("ireset", C.c_int)]
+
+
 class Grid(C.Structure):
     _fields_ = [("npt", C.c_int), ("npt1", C.c_int), ("npt2", C.c_int), ("nptmin", C.c_int),
                 ("nptmax", C.c_int)]
@@ -178,6 +194,12 @@ def _declare(L):
         "ora_da_uniform": (C.c_float, [C.c_ulonglong, C.c_int]),
         "ora_molfit_da": (C.c_int, [P(Box), fp, fp, fp, P(Centres), P(Tables), P(Cells), P(Ledger),
                                     P(MCState), P(Ran1), fp, ip, fp, C.c_ulonglong, ip, dp, fp]),
+        "ora_en_atom_intra": (C.c_float, [P(Box), P(Tables), P(Intra), C.c_int, C.c_int, fp, fp]),
+        "ora_en_dihedral": (C.c_float, [fp, fp, fp, C.c_float]),
+        "ora_en_mole_dihedral": (C.c_float, [P(Box), P(Intra), C.c_int]),
+        "ora_en_mole_intra": (C.c_float, [P(Box), P(Tables), P(Intra), C.c_int, fp, fp, fp]),
+        "ora_molfit_full": (C.c_int, [P(Box), fp, fp, fp, P(Centres), P(Tables), P(Intra), P(Cells), P(Ledger),
+                                      P(MCFull), P(Ran1), fp, fp, ip, C.c_int, C.c_int]),
         "ora_descriptor": (None, [C.c_int, fp, fp, fp, ip, C.c_int, fp, fp, fp, ip, ip, C.c_int, C.c_int,
                                   C.c_float, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
                                   dp]),

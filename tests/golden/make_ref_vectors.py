@@ -149,7 +149,8 @@ def main():
                  n1extr=mc.n1extr[1:np_ + 1], n2extr=mc.n2extr[1:np_ + 1], cfrac=mc.cfrac[1:nt + 1],
                  natomr=mc.natomr, sizefactor=mc.sizefactor, dmaxmol=mc.dmaxmol, qtype=mc.qtype[1:na + 1],
                  imageatom=mc.imageatom[1:na + 1], nbond=mc.nbond[1:na + 1], iabond=mc.iabond[1:na + 1],
-                 tbond=mc.tbond[1:na + 1], amassr=mc.amassr[1:np_ + 1], mref=mc.mref[1:na + 1],
+                 tbond=mc.tbond[1:na + 1], amassr=mc.amassr[1:np_ + 1], amassa=mc.amassa[1:nt + 1],
+                 mref=mc.mref[1:na + 1],
                  tempb=mc.tempb, tempangwt=mc.tempangwt, tempdihwt=mc.tempdihwt, tempangreq=mc.tempangreq,
                  tempdihreq=mc.tempdihreq, simt=mc.simt, refine_intra=mc.refine_intra,
                  refine_intra_fac=mc.refine_intra_fac, stepr=mc.stepr, stepm=mc.stepm, stepri=mc.stepri,
@@ -207,7 +208,7 @@ def main():
         put(key + ".in", ledger())
         put(key, dict(fadds=a[0], faddbs=a[1], nisum=a[2], intra=a[3], rotfreq=a[4], ireset=a[5],
                       stepr=mc.stepr, stepm=mc.stepm, stepri=mc.stepri, stepmi=mc.stepmi, simt=mc.simt,
-                      rejrate=e16.rejrate))
+                      rejrate=e16.rejrate, tempangwt=mc.tempangwt, tempdihwt=mc.tempdihwt, tempb=mc.tempb))
         return key
 
     def mf_a(key, a, out):

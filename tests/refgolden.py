@@ -141,3 +141,54 @@ def f32_bits(a):
 
 def as_ptr(a):
     return a.ctypes.data_as(C.c_void_p)
+
+
+def intra_from_record(st, tables, tempang=None, tempdih=None):
+    """The oracle's intramolecular description (0-based) of the recorded box; tables supply
+    the reference potential xav/xav1 on the same grid as the tables of `st`."""
+    t = st.t
+    c = np.ascontiguousarray
+    keep = {}
+    na = st.natom
+    keep["qtype"] = c(t["qtype"], np.int32)
+    keep["imageatom"] = c(t["imageatom"] - 1, np.int32)
+    keep["mref"] = c(t["mref"] - 1, np.int32)
+    keep["nbond"] = c(t["nbond"], np.int32)
+    ldb = int(t["iabond"].shape[1]) - 1
+    keep["iabond"] = c(t["iabond"][:, 1:] - 1, np.int32).reshape(-1)
+    keep["tbond"] = c(t["tbond"][:, 1:], F32).reshape(-1)
+    keep["amassr"] = c(t["amassr"], F32)
+    keep["amassa"] = c(t["amassa"], F32)
+    keep["ngrot"] = c(t["ngrot"], np.int32)
+    keep["nrf"] = c(t["nrf"] - 1, np.int32)
+    keep["nrl"] = c(t["nrl"] - 1, np.int32)
+    ng = int(t["labaxi"].shape[0]) - 1
+    keep["labaxi"] = c(t["labaxi"][1:, 1:3] - 1, np.int32).reshape(-1)
+    keep["rotf"] = c(t["rotf"][1:] - 1, np.int32)
+    keep["rotl"] = c(t["rotl"][1:] - 1, np.int32)
+    keep["labrot"] = c(t["labrot"][1:] - 1, np.int32)
+    keep["isdih"] = c([1 if str(g) == "DIH" else 0 for g in t["rotgrp"][1:]], np.int32)
+    keep["ndihedralang"] = c(t["ndihedralang"][1:ng + 1], np.int32)
+    ldd = max(int(t["labrotdih"].shape[1]) - 1, 1)
+    lrd = np.zeros((ng, ldd, 2), np.int32)
+    dang = np.zeros((ng, ldd), F32)
+    if t["labrotdih"].shape[1] > 1:
+        lrd[:, :, 0] = t["labrotdih"][1, 1:, 1:].T - 1
+        lrd[:, :, 1] = t["labrotdih"][2, 1:, 1:].T - 1
+        dang[:, :] = t["dihedralang"][1:, 1:].T
+    keep["labrotdih"] = c(lrd.reshape(-1))
+    keep["dihedralang"] = c(dang.reshape(-1))
+    keep["xav"] = c(tables["xav"], F32).reshape(-1).copy()
+    keep["xav1"] = c(tables["xav1"], F32).reshape(-1).copy()
+    assert keep["qtype"].shape[0] == na
+    it = po.Intra(po.Pi(keep["qtype"]), po.Pi(keep["imageatom"]), po.Pi(keep["mref"]), po.Pi(keep["nbond"]),
+                  po.Pi(keep["iabond"]), po.Pf(keep["tbond"]), ldb, po.Pf(keep["amassr"]), po.Pf(keep["amassa"]),
+                  po.Pi(keep["ngrot"]), po.Pi(keep["nrf"]), po.Pi(keep["nrl"]), po.Pi(keep["labaxi"]),
+                  po.Pi(keep["rotf"]), po.Pi(keep["rotl"]), po.Pi(keep["labrot"]), po.Pi(keep["isdih"]),
+                  po.Pi(keep["ndihedralang"]), po.Pi(keep["labrotdih"]), po.Pf(keep["dihedralang"]), ldd,
+                  po.Pf(keep["xav"]), po.Pf(keep["xav1"]), F32(t["dmaxmol"]), F32(t["tempb"]), F32(t["simt"]),
+                  F32(t["tempangreq"] if tempang is None else tempang),
+                  F32(F32(t["tempdihreq"]) * F32(t["tempb"]) if tempdih is None else tempdih),
+                  F32(t["refine_intra_fac"]), int(bool(t["refine_intra"])))
+    it._keep = keep
+    return it

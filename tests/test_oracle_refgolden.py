@@ -215,3 +215,70 @@ def test_generate_ep_matches_the_reference(L, k):
         got = out[key].reshape(npairs, ld)[:, 1:npt + 1]
         want = cout[key][:, 1:npt + 1]
         assert np.array_equal(RG.f32_bits(got), RG.f32_bits(want)), key
+
+
+@pytest.mark.parametrize("k", [0, 1])
+def test_en_mole_intra_gives_the_references_self_energies(L, k):
+    # fmolec1 stores en_mole_intra of every molecule as its self entry (weights
+    # tempang = tempangreq, tempdih = tempdihreq * tempb, ato_manipulation_rev4.py:2416-2419)
+    cap = RG.group(f"fmolec1.{k}")
+    tab = RG.group(f"fmolec1.{k}.tables")
+    st = RG.State(cap, tab)
+    it = RG.intra_from_record(st, tab)
+    want = RG.self_energies(cap, st.nmol)
+    got = np.array([L.ora_en_mole_intra(C.byref(st.box), C.byref(st.tables), C.byref(it), im, None, None, None)
+                    for im in range(st.nmol)], F32)
+    assert np.array_equal(RG.f32_bits(got), RG.f32_bits(want))
+    assert np.all(want != 0.0)
+    fb = F32(0.0)
+    for e in got:
+        fb = F32(fb + e)
+    assert F32(fb / F32(st.nmol)) == F32(cap["ftestb"])
+
+
+def _run_full(L, k):
+    cap = RG.group(f"molfit.{k}")
+    cin = RG.group(f"molfit.{k}.in")
+    tab = RG.group("fmolec1.0.tables")
+    st = RG.State(cin, tab)
+    it = RG.intra_from_record(st, tab)
+    cl = _cells(L, RG.group("cells.0"), RG.State(RG.group("fmolec1.0"), tab))
+    led = RG.ledger_from_record(cin, st.nmol)
+    nt = st.nmoltype
+    arr = {key: np.ascontiguousarray(cin[key], F32).copy() for key in ("wholerot", "wholetrans", "internrot",
+                                                                     "interntrans")}
+    cnt = {key: np.zeros(2 * nt, np.int32) for key in ("nwhole", "nmovem", "ninter", "nmovei")}
+    nsr = np.ascontiguousarray(cin["nstepreject"], np.int32).copy()
+    mc = po.MCFull(po.Pf(arr["wholerot"]), po.Pf(arr["wholetrans"]), po.Pf(arr["internrot"]),
+                   po.Pf(arr["interntrans"]), po.Pi(cnt["nwhole"]), po.Pi(cnt["nmovem"]), po.Pi(cnt["ninter"]),
+                   po.Pi(cnt["nmovei"]), po.Pi(nsr), F32(cap["simt"]), F32(cap["stepr"]), F32(cap["stepm"]),
+                   F32(cap["stepri"]), F32(cap["stepmi"]), F32(cap["rejrate"]), F32(cap["tempangwt"]),
+                   F32(cap["tempdihwt"]), int(cap["ireset"]))
+    rng = RG.ran1(cin["ran1"])
+    fadds, faddbs = C.c_float(F32(cap["fadds"])), C.c_float(F32(cap["faddbs"]))
+    nisum = C.c_int(int(cap["nisum"]))
+    L.ora_molfit_full(C.byref(st.box), po.Pf(st.xa), po.Pf(st.ya), po.Pf(st.za), C.byref(st.centres),
+                      C.byref(st.tables), C.byref(it), C.byref(cl), C.byref(led), C.byref(mc), C.byref(rng),
+                      C.byref(fadds), C.byref(faddbs), C.byref(nisum), int(cap["intra"]), int(cap["rotfreq"]))
+    return st, led, arr, cnt, nsr, rng, fadds, faddbs, nisum
+
+
+@pytest.mark.parametrize("k", range(6))
+def test_molfit_full_cycle_matches_the_reference(L, k):
+    # every recorded cycle: the shake (nisum >= intra), whole-molecule cycles and the cycle
+    # with internal rotations (nisum a multiple of rotfreq)
+    st, led, arr, cnt, nsr, rng, fadds, faddbs, nisum = _run_full(L, k)
+    cout = RG.group(f"molfit.{k}.out")
+    nt = st.nmoltype
+    assert np.array_equal(rng.as_array(), cout["ran1"]), "ran1 state after the cycle"
+    for key in ("xa", "ya", "za", "xadd", "yadd", "zadd"):
+        assert np.array_equal(RG.f32_bits(getattr(st, key)), RG.f32_bits(cout[key])), key
+    for key, a in arr.items():
+        assert np.array_equal(RG.f32_bits(a), RG.f32_bits(cout[key])), key
+    for key, a in cnt.items():
+        assert np.array_equal(a.reshape(nt, 2).T, cout[key][1:3]), key
+    assert np.array_equal(nsr, cout["nstepreject"])
+    erg = np.ctypeslib.as_array(led.ergpair, shape=(led.nergpair,))
+    assert np.array_equal(RG.f32_bits(erg), RG.f32_bits(cout["ergpair"])), "ledger after the cycle"
+    assert F32(fadds.value) == F32(cout["fadds"]) and F32(faddbs.value) == F32(cout["faddbs"])
+    assert nisum.value == int(cout["nisum"])
