@@ -28,6 +28,7 @@
 #include <numeric>
 
 #include "common.h"
+#include "pairs.h"
 #include "surrogate.h"
 
 template <class T>
@@ -323,122 +324,6 @@ extern "C" int epsr_cells_init(epsr_ctx *ctx, const epsr_box *box, epsr_ledger *
 }
 
 // ------------------------------------------------------------------ pair energy
-// Packed tables: bins [0, nnear) as T[ic*nnear + iz] = (xavs, xavs1, ecoul, ecoul1)(iz, ic);
-// bins [nnear, nte), where xavs and xavs1 are +0 in every column, as
-// F[ic*(nte - nnear) + iz - nnear] = (ecoul, ecoul1)(iz, ic); beyond nte every table entry
-// is +0, where the reference's lookup also gives +0.  The split keeps the tables of a
-// long Coulomb tail small enough for two workgroups' copies per CU in LDS.
-struct PairTab {
-    const float4 *T;
-    const float2 *F;
-    const float *xs;
-    const int *npr;
-    int nte, nnear, npt2, ntype;
-    float stepcl, inv_stepcl;   // inv_stepcl = 1/stepcl rounded to float
-};
-
-// IEEE float square root.  For 2^-96 <= x < inf: the hardware estimate s (within 1 ulp)
-// or a neighbour, chosen by the signs of the exact residuals x - s_dn*s and x - s_up*s
-// (the same selection the compiler's correctly rounded sqrtf makes, without its scaling
-// of tiny arguments and its special-value fix-up, which that range does not need);
-// other arguments take sqrtf itself.
-__device__ __forceinline__ float sqrt_rn(float x)
-{
-    if (!(x >= 0x1p-96f && x < INFINITY)) return sqrtf(x);
-    const float s = __builtin_amdgcn_sqrtf(x);
-    const float sdn = __int_as_float(__float_as_int(s) - 1), sup = __int_as_float(__float_as_int(s) + 1);
-    const float rdn = fmaf(-sdn, s, x), rup = fmaf(-sup, s, x);
-    float r = rdn <= 0.0f ? sdn : s;
-    return rup > 0.0f ? sup : r;
-}
-
-// en_atom_inter_scale (molfit.py:1250-1279) with scalefac = 1; the minimum-image
-// components enter only the squared distance (mic_sq).  The bin is
-// trunc(dtest / stepcl) + 2 with the correctly rounded quotient Q: q = dtest * inv_stepcl
-// lies within 1.2e-7 q of dtest / stepcl (two roundings of 2^-24) and Q within 6e-8 Q of
-// it, so when q's fraction keeps clear of 0 and 1 by 1.2e-6 q, trunc(q) = trunc(Q);
-// otherwise (about 2.4e-6 q of the pairs) the division is made.
-__device__ __forceinline__ float atom_pair(float x1, float y1, float z1, float x2, float y2, float z2, int ic,
-                                           const PairTab &t)
-{
-    const float xd = mic_sq(x2 - x1), yd = mic_sq(y2 - y1), zd = mic_sq(z2 - z1);
-    float dsq = xd * xd + yd * yd + zd * zd;
-    float dtest = sqrt_rn(dsq);
-    const float q = dtest * t.inv_stepcl;
-    const float tq = truncf(q), fq = q - tq, mq = 1.2e-6f * q;
-    int iz;
-    if (fq > mq && fq < 1.0f - mq && q < 4194304.0f) {
-        iz = (int)tq + 2;
-    } else {
-        const float rat = dtest / t.stepcl;
-        if (!(rat < 2.0e9f)) return 0.0f;
-        iz = (int)truncf(rat) + 2;
-    }
-    float sume = 0.0f;
-    if (iz <= t.npt2 && iz < t.nte) {
-        float4 v;
-        if (iz < t.nnear) {
-            v = t.T[ic * t.nnear + iz];
-        } else {
-            const float2 f = t.F[ic * (t.nte - t.nnear) + (iz - t.nnear)];
-            v = make_float4(0.0f, 0.0f, f.x, f.y);   // the stored +0 of xavs, xavs1
-        }
-        float dtdif = dtest - t.xs[iz];
-        sume = v.x + v.y * dtdif;
-        sume = sume + v.z + v.w * dtdif;
-    }
-    return sume;
-}
-
-// en_mole_pair (molfit.py:1527-1578): sequential sum, molecule 1 outer, molecule 2 inner
-__device__ __forceinline__ float mol_pair(const float4 *__restrict__ P1, int na1, const float4 *__restrict__ P2,
-                                          int nb2, const PairTab &t)
-{
-    float sume = 0.0f;
-    for (int ia = 0; ia < na1; ++ia) {
-        float4 p = P1[ia];
-        int it = __float_as_int(p.w) * t.ntype;
-        for (int ja = 0; ja < nb2; ++ja) {
-            float4 q = P2[ja];
-            sume = sume + atom_pair(p.x, p.y, p.z, q.x, q.y, q.z, t.npr[it + __float_as_int(q.w)], t);
-        }
-    }
-    return sume;
-}
-
-// mol_pair for molecules of at most three atoms: the six positions are loaded
-// before any pair is evaluated; the sum runs in the same order
-__device__ __forceinline__ float mol_pair3(const float4 *__restrict__ P1, int n1, const float4 *__restrict__ P2,
-                                           int n2, const PairTab &t)
-{
-    float4 p[3], q[3];
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-        p[j] = j < n1 ? P1[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        q[j] = j < n2 ? P2[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    }
-    float sume = 0.0f;
-#pragma unroll
-    for (int ia = 0; ia < 3; ++ia) {
-        if (ia >= n1) break;
-        const int it = __float_as_int(p[ia].w) * t.ntype;
-#pragma unroll
-        for (int ja = 0; ja < 3; ++ja) {
-            if (ja >= n2) break;
-            sume = sume + atom_pair(p[ia].x, p[ia].y, p[ia].z, q[ja].x, q[ja].y, q[ja].z,
-                                    t.npr[it + __float_as_int(q[ja].w)], t);
-        }
-    }
-    return sume;
-}
-
-// the pair energy of two molecules, through mol_pair3 when neither has more than three atoms
-__device__ __forceinline__ float mol_pair_any(const float4 *__restrict__ P1, int na1, const float4 *__restrict__ P2,
-                                              int nb2, const PairTab &t)
-{
-    return (na1 <= 3 && nb2 <= 3) ? mol_pair3(P1, na1, P2, nb2, t) : mol_pair(P1, na1, P2, nb2, t);
-}
-
 // T (near bins) then F (far bins) in one buffer
 __global__ void pack_tables(const float *__restrict__ xavs, const float *__restrict__ xavs1,
                             const float *__restrict__ ecoul, const float *__restrict__ ecoul1, int ld, int npairs,
@@ -507,12 +392,6 @@ abs_positions2(int natom, const float *__restrict__ xa, const float *__restrict_
     }
 }
 
-// Tables prepared for a call: packed float4 rows up to the last non-zero bin.
-struct PreparedTables {
-    PairTab tab;
-    size_t lds_bytes;
-    bool in_lds;
-};
 
 // the last bin (up to npt2) where any of the four tables is non-zero, into dlast[0], and
 // where xavs or xavs1 is, into dlast[1]
@@ -575,6 +454,12 @@ static int prepare_tables(epsr_ctx *ctx, const epsr_box *box, const epsr_tables_
         extents_out[1] = last[1];
     }
     return tables_pack(ctx, box, tb, last[0], last[1], dT, pt);
+}
+
+int mc_prepare_tables(epsr_ctx *ctx, const epsr_box *box, const epsr_tables_t *tb, float4 *dT, int *dscratch_int,
+                      PreparedTables &pt)
+{
+    return prepare_tables(ctx, box, tb, dT, dscratch_int, pt);
 }
 
 // the table extents measured for the arrays of tb (by fmolec1 or a cycle), kept in the
