@@ -117,6 +117,25 @@ class MCRecord(C.Structure):
                 ("pass_", vp)]
 
 
+class IntraDesc(C.Structure):
+    _fields_ = [("ldb", C.c_int), ("qtype", ip), ("imageatom", ip), ("nbond", ip), ("iabond", ip), ("tbond", fp),
+                ("amassr", fp), ("amassa", fp), ("ngroup", C.c_int), ("nlabrot", C.c_int), ("ldd", C.c_int),
+                ("nrf", ip), ("nrl", ip), ("labaxi", ip), ("rotf", ip), ("rotl", ip), ("labrot", ip), ("isdih", ip),
+                ("ndihedralang", ip), ("labrotdih", ip), ("dihedralang", fp), ("dmaxmol", C.c_float),
+                ("tempb", C.c_float), ("refine_intra_fac", C.c_float), ("refine_intra", C.c_int)]
+
+
+class MCFullParams(C.Structure):
+    _fields_ = [("simt", C.c_float), ("stepr", C.c_float), ("stepm", C.c_float), ("stepri", C.c_float),
+                ("stepmi", C.c_float), ("rejrate", C.c_float), ("tempangwt", C.c_float), ("tempdihwt", C.c_float),
+                ("ireset", C.c_int), ("intra", C.c_int), ("rotfreq", C.c_int)]
+
+
+class MCSteps(C.Structure):
+    _fields_ = [("wholerot", fp), ("wholetrans", fp), ("internrot", fp), ("interntrans", fp), ("nwhole", ip),
+                ("nmovem", ip), ("ninter", ip), ("nmovei", ip)]
+
+
 # name -> (restype, argtypes); every symbol declared in include/epsr_mi355x.h
 SIGNATURES = {
     "epsr_ctx_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
@@ -155,6 +174,11 @@ SIGNATURES = {
                               vp, i32p, fp, C.POINTER(MCStats)]),
     "epsr_molfit_da": (C.c_int, [vp, vp, C.POINTER(Coords), C.POINTER(Tables), vp, C.POINTER(MCParams), fp, fp, ip,
                                  ip, vp, i32p, fp, C.POINTER(DA), C.POINTER(MCRecord), C.POINTER(MCStats)]),
+    "epsr_box_set_intra": (C.c_int, [vp, C.POINTER(IntraDesc)]),
+    "epsr_en_mole_intra": (C.c_int, [vp, vp, C.POINTER(Coords), C.POINTER(Tables), vp, vp, C.c_float, C.c_float,
+                                     C.c_float, vp]),
+    "epsr_molfit_full": (C.c_int, [vp, vp, C.POINTER(Coords), C.POINTER(Tables), vp, vp, vp, C.POINTER(MCFullParams),
+                                   C.POINTER(MCSteps), vp, i32p, fp, fp, ip, C.POINTER(MCStats)]),
     "epsr_descriptors": (C.c_int, [vp, vp, C.POINTER(Coords), C.POINTER(Coords), vp, C.POINTER(DescParams), C.c_int,
                                    vp, vp]),
     "epsr_mlp_nparams": (C.c_longlong, [C.c_int, C.c_int]),

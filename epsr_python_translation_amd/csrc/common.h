@@ -43,8 +43,22 @@ struct epsr_ctx {
     hipEvent_t ev_done = nullptr;
 };
 
+// intramolecular topology of a box (epsr_box_set_intra), device copies
+struct IntraTopo {
+    int ldb = 0, ngroup = 0, nlabrot = 0, ldd = 1, refine_intra = 0, max_groups = 0;
+    float dmaxmol = 1.f, tempb = 0.f, refine_intra_fac = 0.f;
+    std::vector<int> h_ngrot;   // groups per molecule
+    int *d_qtype = nullptr, *d_imageatom = nullptr, *d_nbond = nullptr, *d_iabond = nullptr;
+    float *d_tbond = nullptr, *d_amassr = nullptr, *d_amassa = nullptr;
+    int *d_nrf = nullptr, *d_nrl = nullptr, *d_labaxi = nullptr, *d_rotf = nullptr, *d_rotl = nullptr;
+    int *d_labrot = nullptr, *d_isdih = nullptr, *d_ndih = nullptr, *d_labrotdih = nullptr;
+    float *d_dihedralang = nullptr;
+};
+void intra_free(IntraTopo *t);
+
 struct epsr_box {
     epsr_ctx *ctx = nullptr;
+    IntraTopo *intra = nullptr;
     int nmol = 0, natom = 0, ntype = 0, npairs = 0, nmoltype = 0;
     float natomr = 0.f, sizefactor = 1.f, dmax = 1.f, cellst = 0.f;
     // host copies

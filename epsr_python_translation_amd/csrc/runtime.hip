@@ -262,6 +262,7 @@ extern "C" void epsr_box_destroy(epsr_box *b)
     if (!b) return;
     hipFree(b->d_itype); hipFree(b->d_naf); hipFree(b->d_nal); hipFree(b->d_mtype);
     hipFree(b->d_npairref); hipFree(b->d_amol);
+    intra_free(b->intra);
     delete b;
 }
 

@@ -154,6 +154,13 @@ class DeviceBox:
     def get_coords(self):
         return {k: getattr(self, k).cpu().numpy() for k in ("xa", "ya", "za", "xadd", "yadd", "zadd")}
 
+    def set_intra(self, topo):
+        """upload the intramolecular topology (intra.IntraTopology) the intramolecular
+        energies and moves read"""
+        d, keep = topo.desc()
+        self.ctx.check(self.ctx.lib.epsr_box_set_intra(self.h, C.byref(d)))
+        self.intra = topo
+
     def set_launch(self, knob, value):
         """launch shape of this box's kernels (_native.LAUNCH_*); results do not depend on it"""
         self.ctx.check(self.ctx.lib.epsr_box_set_launch(self.h, int(knob), int(value)))
@@ -435,6 +442,65 @@ def molfit(ctx: Context, box: DeviceBox, tables: DeviceTables, ledger: Ledger, m
     ctx.check(ctx.lib.epsr_molfit(ctx.h, box.h, C.byref(c), C.byref(t), ledger.h, C.byref(mc.params),
                                   *_mc_args(mc, ran1_state), C.byref(fad), C.byref(mc.stats)))
     mc.fadds = F32(fad.value)
+    return mc
+
+
+def en_mole_intra(ctx: Context, box: DeviceBox, tables: DeviceTables, simt, tempang, tempdih, xav=None, xav1=None,
+                  out=None):
+    """en_mole_intra of every molecule at the current coordinates: float32 tensor [nmol]."""
+    out = ctx.zeros(box.nmol) if out is None else out
+    c = box.coords()
+    t = tables.struct()
+    ctx.check(ctx.lib.epsr_en_mole_intra(ctx.h, box.h, C.byref(c), C.byref(t), _ptr(xav) if xav is not None else None,
+                                         _ptr(xav1) if xav1 is not None else None, F32(simt), F32(tempang),
+                                         F32(tempdih), _ptr(out)))
+    return out
+
+
+class MCFullState:
+    """Everything molfit keeps between cycles (clustercom step sizes and counters,
+    nstepreject, fadds/faddbs, nisum) plus the run parameters (molecom.stepr/stepm/stepri/
+    stepmi, the intramolecular weights, rejrate, ireset, intra, rotfreq)."""
+
+    def __init__(self, ctx: Context, nmol, nmoltype, simt, *, stepr=1.0, stepm=1.0, stepri=0.0, stepmi=0.0,
+                 rejrate=0.75, tempangwt=1.0, tempdihwt=0.0, ireset=0, intra=100, rotfreq=5, nisum=None,
+                 wholerot=None, wholetrans=None, internrot=None, interntrans=None, nstepreject=None):
+        self.params = _native.MCFullParams(F32(simt), F32(stepr), F32(stepm), F32(stepri), F32(stepmi), F32(rejrate),
+                                           F32(tempangwt), F32(tempdihwt), int(ireset), int(intra), int(rotfreq))
+        z = lambda a: np.ascontiguousarray(np.zeros(nmoltype, F32) if a is None else a, dtype=F32).copy()
+        self.wholerot, self.wholetrans = z(wholerot), z(wholetrans)
+        self.internrot, self.interntrans = z(internrot), z(interntrans)
+        self.nwhole, self.nmovem = np.zeros(2 * nmoltype, np.int32), np.zeros(2 * nmoltype, np.int32)
+        self.ninter, self.nmovei = np.zeros(2 * nmoltype, np.int32), np.zeros(2 * nmoltype, np.int32)
+        self.nstepreject = ctx.zeros(nmol, torch.int32)
+        if nstepreject is not None:
+            self.nstepreject.copy_(torch.from_numpy(np.ascontiguousarray(nstepreject, dtype=np.int32)))
+        self.fadds = F32(0.0)
+        self.faddbs = F32(0.0)
+        self.nisum = int(intra if nisum is None else nisum)
+        self.stats = _native.MCStats()
+
+    def steps(self):
+        P = lambda a, ty=C.c_int: a.ctypes.data_as(C.POINTER(ty))
+        return _native.MCSteps(P(self.wholerot, C.c_float), P(self.wholetrans, C.c_float),
+                               P(self.internrot, C.c_float), P(self.interntrans, C.c_float), P(self.nwhole),
+                               P(self.nmovem), P(self.ninter), P(self.nmovei))
+
+
+def molfit_full(ctx: Context, box: DeviceBox, tables: DeviceTables, ledger: Ledger, mc: MCFullState, ran1_state,
+                xav=None, xav1=None):
+    """One MC cycle of any kind (molfit.molfit): updates coordinates, ledger, mc and
+    ran1_state in place."""
+    c = box.coords()
+    t = tables.struct()
+    st = mc.steps()
+    fad, fadb, nis = C.c_float(float(mc.fadds)), C.c_float(float(mc.faddbs)), C.c_int(int(mc.nisum))
+    assert ran1_state.dtype == np.int32 and ran1_state.size == 35
+    ctx.check(ctx.lib.epsr_molfit_full(ctx.h, box.h, C.byref(c), C.byref(t), _ptr(xav) if xav is not None else None,
+                                       _ptr(xav1) if xav1 is not None else None, ledger.h, C.byref(mc.params),
+                                       C.byref(st), _ptr(mc.nstepreject), ran1_state.ctypes.data_as(C.POINTER(C.c_int32)),
+                                       C.byref(fad), C.byref(fadb), C.byref(nis), C.byref(mc.stats)))
+    mc.fadds, mc.faddbs, mc.nisum = F32(fad.value), F32(fadb.value), int(nis.value)
     return mc
 
 

@@ -908,9 +908,14 @@ def system_from_files(ato, pcof=None, inp=None, name=None):
     by label in either order (read_pcof :1263-1290), with ireset's reset applied;
     rmaxgr <= 0 or beyond half the box becomes half the box (epsr_standalone_rev3.py:2938-2941).
 
-    The GPU path is cubic and rigid: a non-cubic box or Q-type atoms are rejected.  The
-    intramolecular energy of each molecule (en_mole_intra) is outside this package and is set
-    to the same fixed placeholder the synthetic boxes use (it shifts the ledger total only).
+    The box carries the file's intramolecular topology (bonds and angle constraints, image
+    atoms, rotational groups, dihedrals, masses: intra.from_ato) and MC run parameters
+    (stepr/stepm/stepri/stepmi normalised as normalise_coordinates does, ato_read_and_save.py:
+    916-940; tempangwt/tempdihwt; per-template step sizes; per-molecule nstepreject), so the
+    intramolecular energies (en_mole_intra) and moves run on the GPU; eintra is None.
+
+    The GPU path is cubic: a non-cubic box, Q-type atoms, tethered molecules or an
+    intramolecular Coulomb weighting (ref_intra's second word > 0) are rejected.
     """
     from . import synth
 
@@ -921,11 +926,20 @@ def system_from_files(ato, pcof=None, inp=None, name=None):
         raise ValueError("system_from_files: the GPU path handles cubic boxes only")
     if a.qtype().any():
         raise ValueError("system_from_files: Q-type atoms are not handled by the GPU path")
+    if any(m.tethered for m in a.molecules):
+        raise ValueError("system_from_files: tethered molecules are not handled by the GPU path")
+    refine_intra_fac = F32(0.0)
+    if kf is not None and kf.get("ref_intra") is not None:
+        w = kf.get("ref_intra").split()
+        refine_intra_fac = f32(w[0]) if w else F32(0.0)
+        if len(w) > 1 and f32(w[1]) > 0.0:
+            raise NotImplementedError("system_from_files: ref_intra's Coulomb weighting (coul_intra) needs the "
+                                      "intramolecular reference potential xav, which this package does not build")
     dmax = F32(a.dmax)
     sizefactor = kf.real("sizefactor", F32(1.0)) if kf is not None else F32(1.0)
     dmaxmol = F32(dmax * sizefactor)
     itype = a.itype()
-    xa, ya, za, naf, nal, eintra = [], [], [], [], [], []
+    xa, ya, za, naf, nal = [], [], [], [], []
     xadd = np.empty(a.nmol, F32)
     yadd = np.empty(a.nmol, F32)
     zadd = np.empty(a.nmol, F32)
@@ -938,7 +952,6 @@ def system_from_files(ato, pcof=None, inp=None, name=None):
         za.append(off[:, 2])
         naf.append(atom)
         nal.append(atom + m.nato - 1)
-        eintra.append(0.001 * m.nato if m.nato > 1 else 0.0)
         atom += m.nato
     nt = len(a.types)
     eps = np.array([t.eps for t in a.types], F32)
@@ -967,6 +980,9 @@ def system_from_files(ato, pcof=None, inp=None, name=None):
         pcoefp = np.zeros((npairs, ncoeffp + 1), F32)
         rminex = np.zeros(npairs, F32)
         pcoefex = np.zeros(npairs, F32)
+        coulfac = np.ones(npairs, F32)
+        rminfac = pc.head.real("rminfac", F32(0.0))
+        roverlap = pc.head.real("roverlap", F32(0.0))
         for p in range(npairs):
             la, lb = a.types[n1[p]].label, a.types[n2[p]].label
             found = None
@@ -975,8 +991,19 @@ def system_from_files(ato, pcof=None, inp=None, name=None):
                     found = pr
                     break
             if found is None:
+                # a pair the file lacks gets new coefficients and the minimum distance
+                # rminfac * dcor, at least roverlap (read_and_save_files.py:1297-1319)
                 pcoefp[p, 1] = 1.0
+                r0 = F32(F32(rminfac) * dcor[p])
+                rminex[p] = r0 if r0 >= F32(roverlap) else F32(roverlap)
                 continue
+            if any(abs(g[1]) > 0.0 for g in found.gaussians):
+                raise NotImplementedError("system_from_files: Gaussian or modified-Morse terms of the reference "
+                                          "potential (.pcof) are not built by this package")
+            if found.rmaxex > 0.0:
+                raise NotImplementedError("system_from_files: container forces (rmaxex > 0 in the .pcof) are not "
+                                          "handled by this package")
+            coulfac[p] = found.coulfac
             pcoefp[p, 1:] = found.pcoefp[:ncoeffp]
             if ireset != 0:
                 if ireset < 2:
@@ -995,6 +1022,17 @@ def system_from_files(ato, pcof=None, inp=None, name=None):
                 kw[key] = float(v)
         kw["rminex"] = rminex
         kw["pcoefex"] = pcoefex
+        kw["coulfac"] = coulfac
+        for key in ("refpotfac", "expecf"):
+            v = pc.head.real(key, None)
+            if v is not None:
+                kw[key] = float(v)
+        ex = pc.head.get("expecf")
+        if ex is not None and len(ex.split()) > 1:
+            kw["expecfpwr"] = float(f32(ex.split()[1]))
+        rpt = pc.head.get("reppottype")
+        if rpt is not None and rpt.split() and rpt.split()[0].lower().startswith("harm"):
+            raise NotImplementedError("system_from_files: the harmonic repulsive core is not built by this package")
     else:
         pcoefp = np.zeros((npairs, 2), F32)
         pcoefp[:, 1] = 1.0
@@ -1012,15 +1050,33 @@ def system_from_files(ato, pcof=None, inp=None, name=None):
     if rmaxgr > F32(0.5) * dmax or rmaxgr <= 0.0:
         rmaxgr = F32(0.5) * dmax
     sysname = name or (a.templates[0].name if a.templates else "ato")
+    from . import intra as IT
+    amass = np.array([t.mass for t in a.types], F32)
+    topo = IT.from_ato(a, itype, npairref, amass, dmaxmol, refine_intra_fac)
+    stepm = F32(a.stepm) / dmax
+    if stepm > 0.25:
+        stepm = F32(0.25)
+    tpl = a.templates
+    mc_run = dict(stepr=F32(a.stepr), stepm=F32(stepm), stepri=F32(a.stepri), stepmi=F32(a.stepmi) / dmaxmol,
+                  tempangwt=F32(a.tempangwt), tempdihwt=F32(a.tempdihwt), tempangreq=F32(a.tempangreq),
+                  tempdihreq=F32(a.tempdihreq),
+                  wholerot=np.array([t.wholerot for t in tpl], F32), wholetrans=np.array([t.wholetrans for t in tpl], F32),
+                  internrot=np.array([t.internrot for t in tpl], F32),
+                  interntrans=np.array([t.interntrans for t in tpl], F32),
+                  nstepreject=np.array([m.nstepreject for m in a.molecules], np.int32),
+                  intra=kf.int("intra", 100) if kf is not None else 100,
+                  rotfreq=kf.int("rotfreq", 5) if kf is not None else 5,
+                  iinit=kf.int("iinit", 0) if kf is not None else 0)
     return synth.System(
         name=sysname, L=float(dmax), type_names=[t.label.strip() for t in a.types],
         xa=np.ascontiguousarray(np.concatenate(xa)), ya=np.ascontiguousarray(np.concatenate(ya)),
         za=np.ascontiguousarray(np.concatenate(za)), itype=itype, xadd=xadd, yadd=yadd, zadd=zadd,
         naf=np.array(naf, np.int32), nal=np.array(nal, np.int32), mtype=a.mtype(),
-        mol_names=[t.name for t in a.templates], eintra=np.array(eintra, F32), npairref=npairref,
+        mol_names=[t.name for t in a.templates], eintra=None, npairref=npairref,
         n1extr=n1, n2extr=n2, cfrac=cfrac, eps=eps, sig=sig, charge=charge, ecor=ecor, dcor=dcor, qcor=qcor,
         pcoefp=pcoefp, cellst=float(cellst), rmaxgr=float(rmaxgr), qstep=float(qstep), nq=int(nq),
-        simtk=float(a.simtk), **kw)
+        simtk=float(a.simtk), intra=topo, mc_run=mc_run,
+        qradius=np.array([t.qradius for t in a.types], F32), **kw)
 
 
 def update_ato(a: AtoFile, xadd, yadd, zadd, xa, ya, za, ran1=None, sizefactor=F32(1.0)) -> AtoFile:

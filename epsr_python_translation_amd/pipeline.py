@@ -140,7 +140,7 @@ class Refinement:
         npairs = s.npairs
         self.irefpcof_h = np.arange(npairs, dtype=np.int32)
         self.irefpcof = ctx.tensor(self.irefpcof_h)
-        self.rminex = ctx.zeros(npairs)
+        self.rminex = ctx.tensor(np.asarray(s.rminex, F32)) if s.rminex is not None else ctx.zeros(npairs)
         self.pcoefp = np.ascontiguousarray(s.pcoefp.astype(F32).copy())
         # tables
         z = lambda: ctx.zeros(npairs * self.ld)
@@ -154,11 +154,25 @@ class Refinement:
         self.potex = ctx.zeros(npairs)
         self.potexmax = ctx.zeros(npairs)
         self.sofq = ctx.zeros(2 * npairs * self.ldq)
-        self.eintra = ctx.tensor(s.eintra.astype(F32))
         self.ledger = D.Ledger(ctx, self.box)
-        wr = np.full(s.nmoltype, 0.05, F32)
-        wt = np.full(s.nmoltype, F32(0.1) / F32(s.dmax), F32)
-        self.mc = D.MCState(ctx, s.nmol, s.nmoltype, s.simt, rejrate=s.rejrate, wholerot=wr, wholetrans=wt)
+        r = s.mc_run
+        if s.intra is not None:
+            # a box from files: intramolecular energies from its topology, every branch of
+            # molfit, the file's step sizes; fmolec1 sets the angle and dihedral weights to
+            # the requested ones (ato_manipulation_rev4.py:2416-2419), which molfit then uses
+            self.box.set_intra(s.intra)
+            self.eintra = None
+            self.mc = D.MCFullState(ctx, s.nmol, s.nmoltype, s.simt, stepr=r["stepr"], stepm=r["stepm"],
+                                    stepri=r["stepri"], stepmi=r["stepmi"], rejrate=s.rejrate,
+                                    tempangwt=r["tempangreq"], tempdihwt=r["tempdihreq"], intra=r["intra"],
+                                    rotfreq=r["rotfreq"], nisum=r["intra"] if r["iinit"] != 0 else 1,
+                                    wholerot=r["wholerot"], wholetrans=r["wholetrans"], internrot=r["internrot"],
+                                    interntrans=r["interntrans"], nstepreject=r["nstepreject"])
+        else:
+            self.eintra = ctx.tensor(s.eintra.astype(F32))
+            wr = np.full(s.nmoltype, 0.05, F32)
+            wt = np.full(s.nmoltype, F32(0.1) / F32(s.dmax), F32)
+            self.mc = D.MCState(ctx, s.nmol, s.nmoltype, s.simt, rejrate=s.rejrate, wholerot=wr, wholetrans=wt)
         self.pressfac = F32(1.0)
 
     # ---------------------------------------------------------------- setup (inita/initpt)
@@ -174,8 +188,20 @@ class Refinement:
 
     def generate_rp(self):
         ctx, s, g = self.ctx, self.s, self.g
-        self._rp_in = [ctx.tensor(a.astype(F32)) for a in (s.ecor, s.dcor, s.qcor)]
-        p = _native_rp_params(s, self._rp_in)
+        qcor = s.qcor.astype(F32)
+        if s.coulfac is not None:   # qct = qcor * coulfac (epsr_standalone_rev3.py:5739)
+            qcor = (qcor * np.asarray(s.coulfac, F32)).astype(F32)
+        self._rp_in = [ctx.tensor(a.astype(F32)) for a in (s.ecor, s.dcor, qcor)]
+        extra = {}
+        if s.pcoefex is not None and np.any(np.asarray(s.pcoefex) != 0.0):
+            extra["pcoefex"] = ctx.tensor(np.asarray(s.pcoefex, F32))
+            extra["rminex"] = self.rminex
+        if s.qradius is not None and np.any(np.asarray(s.qradius) != 0.0):
+            qr = np.asarray(s.qradius, F32)
+            extra["qradius_i"] = ctx.tensor(qr[s.n1extr])
+            extra["qradius_j"] = ctx.tensor(qr[s.n2extr])
+        self._rp_extra = extra
+        p = _native_rp_params(s, self._rp_in, **extra)
         ctx.check(ctx.lib.epsr_generate_rp(ctx.h, g.npt, g.npt2, D._ptr(self.x), D._ptr(self.xstar),
                                            D._ptr(self.wfac), D._ptr(self.wfacc), C.byref(p), D._ptr(self.epot),
                                            D._ptr(self.ecoul), D._ptr(self.epot1), D._ptr(self.ecoul1), self.ld))
@@ -204,6 +230,11 @@ class Refinement:
     def cells_and_ledger(self):
         """initialise_cellneighbourlist + fmolec1 (:1259-1281).  Returns (ftest, ftestb)."""
         self.ledger.cells_init(self.s.nmolcell, T.sepmax(self.s, self.g))
+        if self.s.intra is not None:
+            r = self.s.mc_run
+            tempdih = F32(F32(r["tempdihreq"]) * F32(self.s.intra.tempb))
+            self.eintra = D.en_mole_intra(self.ctx, self.box, self.tables, self.s.simt, r["tempangreq"], tempdih,
+                                          out=self.eintra)
         return self.ledger.fmolec1(self.tables, self.eintra)
 
     def accumulate(self, nsum: int, nsums: int = 0, nesum: int = 0):
@@ -283,7 +314,11 @@ class Refinement:
                        pressf=pressf, wsum=wsum, nmol=s.nmol)
 
     def molfit(self, tables=None):
-        D.molfit(self.ctx, self.box, tables or self.tables, self.ledger, self.mc, self.ran1)
+        """one molfit cycle (every branch when the box has an intramolecular topology)"""
+        if isinstance(self.mc, D.MCFullState):
+            D.molfit_full(self.ctx, self.box, tables or self.tables, self.ledger, self.mc, self.ran1)
+        else:
+            D.molfit(self.ctx, self.box, tables or self.tables, self.ledger, self.mc, self.ran1)
         return self.mc
 
     def atogen(self, ngrsamples=0):
@@ -330,4 +365,5 @@ class RPParams(C.Structure):
 def _native_rp_params(s, tensors, pcoefex=None, rminex=None, qradius_i=None, qradius_j=None):
     ecor, dcor, qcor = tensors
     return RPParams(s.npairs, D._ptr(ecor), D._ptr(dcor), D._ptr(qcor), D._ptr(qradius_i), D._ptr(qradius_j),
-                    D._ptr(pcoefex), D._ptr(rminex), F32(s.power), F32(1.0), F32(0.3), F32(0.0))
+                    D._ptr(pcoefex), D._ptr(rminex), F32(s.power), F32(s.refpotfac), F32(s.expecf),
+                    F32(s.expecfpwr))

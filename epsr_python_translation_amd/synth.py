@@ -208,6 +208,22 @@ class System:
     pcoefex: np.ndarray = None
     # per type, the coarse-grained atomic fractions (molecom.cfraccg); None means cfrac
     cfraccg: np.ndarray = None
+    # a box read from .ato files: its intramolecular topology (intra.IntraTopology) and the
+    # MC run parameters the file carries (mc_run: stepr, stepm, stepri, stepmi in box units,
+    # tempangwt, tempdihwt, per-type wholerot/wholetrans/internrot/interntrans and per-molecule
+    # nstepreject); None for the synthetic rigid boxes.  With a topology, eintra is None: the
+    # intramolecular energies come from en_mole_intra at the current coordinates.
+    intra: object = None
+    mc_run: dict = None
+    # reference-potential parameters a .pcof / .ato can set: per pair the Coulomb factor
+    # (potential_par.coulfac), per type the charge-sphere radius (molecom.qradius); the
+    # potential factor, the repulsive-core decay and its gradient limit (refpotfac, expecf,
+    # expecfpwr).  None / the defaults reproduce the synthetic boxes.
+    coulfac: np.ndarray = None
+    qradius: np.ndarray = None
+    refpotfac: float = 1.0
+    expecf: float = 0.3
+    expecfpwr: float = 0.0
 
     @property
     def nmol(self):

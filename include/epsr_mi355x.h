@@ -261,6 +261,66 @@ int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *coords,
                 epsr_ledger *l, const epsr_mc_params_t *p, float *wholerot, float *wholetrans, int *nwhole,
                 int *nmovem, int *nstepreject, int32_t *ran1_state, float *fadds, epsr_mc_stats_t *stats);
 
+/* ==== intramolecular energies and moves ========================================
+ * The molecule-internal data of molecom/clustercom/molecom_mod that en_mole_intra,
+ * movei, rotati and centreofmassi read (src/epsr26/common/molecom.py, molecom_mod.py,
+ * clustercom.py: nbond, iabond, tbond, imageatom, qtype, amassr, amassa, nrf/nrl, labaxi,
+ * rotf/rotl, labrot, rotgrp, ndihedralang, labrotdih, dihedralang), host arrays, 0-based:
+ * atom ia has nbond[ia] bonds iabond/tbond[ia*ldb + j] (partner atom; length in the
+ * molecule's box units, negative for an angle constraint); imageatom[ia] = -1 for none;
+ * molecule im owns rotational groups nrf[im]..nrl[im] (none when nrl < nrf); group g turns
+ * labrot[rotf[g]..rotl[g]] about the axis labaxi[2g] -> labaxi[2g+1]; isdih[g] marks a DIH
+ * group with ndihedralang[g] dihedrals: atom pairs labrotdih[(g*ldd + k)*2 + 0/1], angles
+ * dihedralang[g*ldd + k] (radians).  dmaxmol = dmax * sizefactor; tempb the bond weight;
+ * refine_intra / refine_intra_fac the EP weighting of bonded pairs. */
+typedef struct {
+    int ldb;
+    const int *qtype, *imageatom, *nbond, *iabond;
+    const float *tbond, *amassr, *amassa;
+    int ngroup, nlabrot, ldd;
+    const int *nrf, *nrl, *labaxi, *rotf, *rotl, *labrot, *isdih, *ndihedralang, *labrotdih;
+    const float *dihedralang;
+    float dmaxmol, tempb, refine_intra_fac;
+    int refine_intra;
+} epsr_intra_desc_t;
+/* no counterpart call in the reference: uploads the topology above (once per box) */
+int epsr_box_set_intra(epsr_box *box, const epsr_intra_desc_t *desc);
+
+/* ---- en_mole_intra: replaces molfit.en_mole_intra(im, ...) (src/epsr26/library/molfit.py:
+ * 1327-1349, en_atom_intra :922-1047, en_mole_dihedral :1353-1407, en_dihedral :1495-1524)
+ * for every molecule at once: eintra dev [nmol] at the current coordinates.  xav/xav1 dev
+ * [npairs*tables->ld] (potential_par.xav/xav1: the intramolecular reference potential,
+ * zero unless coul_intra; NULL = zero); tempang and tempdih are molecom.tempang and
+ * molecom.tempdih (fmolec1 uses tempangreq and tempdihreq * tempb). */
+int epsr_en_mole_intra(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *coords, const epsr_tables_t *tables,
+                       const float *xav, const float *xav1, float simt, float tempang, float tempdih, float *eintra);
+
+/* ---- molfit, every branch: replaces molfit.molfit(fadds, faddbs, nisum, intra, rotfreq,
+ * ireset) (src/epsr26/library/molfit.py:51-541, movei :545-565, rotati :589-655,
+ * en_change_intra :763-784, centreofmassi ato_manipulation_rev4.py:442-475) for boxes
+ * without tethered molecules, three-body terms or coordination constraints.
+ * When *nisum >= intra: the shake cycle (movei of every molecule judged on its
+ * intramolecular energy, then internal rotations judged on intra + inter energy);
+ * otherwise whole-molecule moves over the cells, preceded, when stepri > 0 and nisum is a
+ * multiple of rotfreq, by internal rotations of every molecule with rotational groups.
+ * Cycles without intramolecular moves take epsr_molfit's parallel resolution; the others
+ * walk the chain in order in one workgroup.  Step sizes host [nmoltype] in/out, counters
+ * host [2*nmoltype] out ([2*mt] attempts, [2*mt+1] rejections), nstepreject dev [nmol],
+ * fadds/faddbs/nisum host in/out, ran1 state host in/out.  Needs epsr_box_set_intra
+ * when the cycle moves atoms inside molecules. */
+typedef struct {
+    float simt, stepr, stepm, stepri, stepmi, rejrate, tempangwt, tempdihwt;
+    int ireset, intra, rotfreq;
+} epsr_mc_full_params_t;
+typedef struct {
+    float *wholerot, *wholetrans, *internrot, *interntrans;
+    int *nwhole, *nmovem, *ninter, *nmovei;
+} epsr_mc_steps_t;
+int epsr_molfit_full(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *coords, const epsr_tables_t *tables,
+                     const float *xav, const float *xav1, epsr_ledger *l, const epsr_mc_full_params_t *p,
+                     epsr_mc_steps_t *steps, int *nstepreject, int32_t *ran1_state, float *fadds, float *faddbs,
+                     int *nisum, epsr_mc_stats_t *stats);
+
 /* ==== learned Delta-E surrogate and delayed-acceptance MC =====================
  * No counterpart in the reference: these entry points add a two-stage Metropolis step
  * beside molfit.  The surrogate is a site-energy network f over radial descriptors of a

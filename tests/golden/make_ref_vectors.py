@@ -22,7 +22,8 @@ and each call's inputs and outputs are recorded:
   fit.<k>       epsr_standalone_rev3.fit_expansion (programs/epsr_standalone_rev3.py:4806)
   genep.<k>     epsr_standalone_rev3.generate_ep (programs/epsr_standalone_rev3.py:5876)
 
-plus the box topology and the potential tables the MC cycles use.  Arrays keep the
+plus the box topology, the potential tables the MC cycles use and the reduced .ato file itself
+(`case.ato`).  Arrays keep the
 reference's 1-based layout with the unused element 0, sliced to the used extent.  The
 transcendentals come from this container's glibc (recorded as `glibc`), the library the
 oracle's glibc flavour calls.
@@ -102,6 +103,8 @@ def main():
 
     rec = {}
     count = {}
+    with open(os.path.join(work, "200DMSO400Water.ato")) as fh:
+        rec["case.ato"] = np.array(fh.read())
 
     def put(prefix, d):
         for k, v in d.items():

@@ -158,7 +158,7 @@ def oracle_fmolec1(name, flavour="glibc", ob=None, cells=None):
     ft, fb = C.c_float(), C.c_float()
     f64 = C.c_double()
     tt = ctables(t)
-    ei = np.ascontiguousarray(s.eintra, dtype=F32)
+    ei = eintra(name, flavour)
     po.lib(flavour).ora_fmolec1(C.byref(ob.box), C.byref(ob.centres), C.byref(tt), C.byref(cl), po.Pf(ei),
                                 C.byref(led), C.byref(ft), C.byref(fb), C.byref(f64))
     return dict(ledger=led, cells=cl, ob=ob, tables=t, ctab=tt, ftest=ft.value, ftestb=fb.value, ftest64=f64.value)
@@ -190,3 +190,55 @@ def oracle_press(name, flavour="glibc", ref=None, tables=None, slots=True):
                               po.Pf(se) if slots else None, po.Pf(sw) if slots else None)
     return dict(energy=F32(e.value), pressc=F32(p.value), presst=w.value, slote=se.reshape(s.nmol, M),
                 slotw=sw.reshape(s.nmol, M), ref=ref)
+
+
+def oracle_intra(s, ld, tempang, tempdih):
+    """po.Intra of a box with an intramolecular topology (s.intra, intra.IntraTopology);
+    xav/xav1 are zero (no intramolecular Coulomb weighting) on tables of leading dimension ld."""
+    it = s.intra
+    c = np.ascontiguousarray
+    keep = dict(qtype=c(it.qtype, np.int32), imageatom=c(it.imageatom, np.int32), mref=c(np.repeat(
+        np.arange(s.nmol, dtype=np.int32), s.nal - s.naf + 1)), nbond=c(it.nbond, np.int32),
+        iabond=c(it.iabond, np.int32).reshape(-1), tbond=c(it.tbond, F32).reshape(-1), amassr=c(it.amassr, F32),
+        amassa=c(it.amassa, F32), ngrot=c(it.ngrot, np.int32), nrf=c(it.nrf, np.int32), nrl=c(it.nrl, np.int32),
+        labaxi=c(it.labaxi, np.int32).reshape(-1), rotf=c(it.rotf, np.int32), rotl=c(it.rotl, np.int32),
+        labrot=c(it.labrot, np.int32), isdih=c(it.isdih, np.int32), ndihedralang=c(it.ndihedralang, np.int32),
+        labrotdih=c(it.labrotdih, np.int32).reshape(-1), dihedralang=c(it.dihedralang, F32).reshape(-1),
+        xav=np.zeros(s.npairs * ld, F32), xav1=np.zeros(s.npairs * ld, F32))
+    for k, v in keep.items():
+        if v.size == 0:
+            keep[k] = np.zeros(2, v.dtype)
+    ldb = int(it.iabond.shape[1])
+    ldd = max(1, int(it.labrotdih.shape[1]))
+    o = po.Intra(po.Pi(keep["qtype"]), po.Pi(keep["imageatom"]), po.Pi(keep["mref"]), po.Pi(keep["nbond"]),
+                 po.Pi(keep["iabond"]), po.Pf(keep["tbond"]), ldb, po.Pf(keep["amassr"]), po.Pf(keep["amassa"]),
+                 po.Pi(keep["ngrot"]), po.Pi(keep["nrf"]), po.Pi(keep["nrl"]), po.Pi(keep["labaxi"]),
+                 po.Pi(keep["rotf"]), po.Pi(keep["rotl"]), po.Pi(keep["labrot"]), po.Pi(keep["isdih"]),
+                 po.Pi(keep["ndihedralang"]), po.Pi(keep["labrotdih"]), po.Pf(keep["dihedralang"]), ldd,
+                 po.Pf(keep["xav"]), po.Pf(keep["xav1"]), F32(it.dmaxmol), F32(it.tempb), F32(s.simt),
+                 F32(tempang), F32(tempdih), F32(it.refine_intra_fac), int(it.refine_intra_fac > 0.0))
+    o._keep = keep
+    return o
+
+
+def fmolec1_weights(s):
+    """(tempang, tempdih) as fmolec1 sets them: tempangreq and tempdihreq * tempb"""
+    r = s.mc_run
+    return F32(r["tempangreq"]), F32(F32(r["tempdihreq"]) * F32(s.intra.tempb))
+
+
+@functools.lru_cache(maxsize=None)
+def eintra(name: str, flavour="glibc"):
+    """The intramolecular energy of every molecule: the synthetic boxes' fixed values, or the
+    oracle's en_mole_intra at the box's coordinates for a box read from files."""
+    s = system(name)
+    if s.eintra is not None:
+        return np.ascontiguousarray(s.eintra, dtype=F32)
+    t = oracle_tables(name)
+    ob = po.OracleBox(s)
+    it = oracle_intra(s, t["ld"], *fmolec1_weights(s))
+    tt = ctables(t)
+    L = po.lib(flavour)
+    out = np.array([L.ora_en_mole_intra(C.byref(ob.box), C.byref(tt), C.byref(it), im, None, None, None)
+                    for im in range(s.nmol)], F32)
+    return out

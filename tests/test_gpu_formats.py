@@ -48,7 +48,7 @@ def test_epsr_test_box_restarts_from_its_ato(ctx, tmp_path):
     tab = D.DeviceTables(ctx, t["ld"], t["npt2"], t["xstar"], t["xavs"], t["xavs1"], t["ecoul"], t["ecoul1"])
     led = D.Ledger(ctx, box)
     led.cells_init(s.nmolcell, T.sepmax(s, g))
-    led.fmolec1(tab, s.eintra)
+    led.fmolec1(tab, H.eintra(name))
     wr2, wt2 = H.initial_steps(s)
     mc = D.MCState(ctx, s.nmol, s.nmoltype, s.simt, wholerot=wr2, wholetrans=wt2)
     st = np.ascontiguousarray(a.ran1.copy())

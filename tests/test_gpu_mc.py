@@ -37,7 +37,7 @@ def _gpu_setup(ctx, name, coords=None):
     tab = D.DeviceTables(ctx, t["ld"], t["npt2"], t["xstar"], t["xavs"], t["xavs1"], t["ecoul"], t["ecoul1"])
     led = D.Ledger(ctx, box)
     led.cells_init(s.nmolcell, T.sepmax(s, g))
-    f64, fb64 = led.fmolec1(tab, s.eintra)
+    f64, fb64 = led.fmolec1(tab, H.eintra(name))
     return box, tab, led, f64, fb64
 
 
@@ -64,7 +64,7 @@ def test_fmolec1_ledger_bit_exact(ctx, name):
     np.testing.assert_array_equal(gd["order"], order)
     # totals: FP64 on the GPU (block partials, any order) against the oracle's FP64 sum
     # of the same entries: both within n u64 sum|e| of the exact sum (u64 = 2^-53)
-    pairs = rd["ergpair"][int(np.count_nonzero(s.eintra)):].astype(np.float64)
+    pairs = rd["ergpair"][int(np.count_nonzero(H.eintra(name))):].astype(np.float64)
     bound64 = 2 * len(pairs) * 2.0 ** -53 * np.abs(pairs).sum() / s.nmol
     assert abs(f64 - ref["ftest64"]) <= bound64 + 1e-300
     # the reference's float32 running sum obeys the recursive-summation bound
@@ -74,7 +74,7 @@ def test_fmolec1_ledger_bit_exact(ctx, name):
     print(f"fmolec1 {name}: ftest fp64 {f64:.9g}, reference fp32 {ref['ftest']:.9g} (rel {rel32:.2e}, "
           f"bound {bound / max(1e-30, abs(f64)):.2e})")
     assert abs(f64 - ref["ftest"]) <= bound + 1e-6 * abs(f64)
-    assert abs(fb64 - float(np.sum(s.eintra, dtype=np.float64)) / s.nmol) < 1e-12
+    assert abs(fb64 - float(np.sum(H.eintra(name), dtype=np.float64)) / s.nmol) < 1e-12
 
 
 def test_en_change_and_en_mole_pair_bit_exact(ctx):

@@ -30,7 +30,7 @@ def test_press_bit_exact(ctx, name):
     tab = D.DeviceTables(ctx, t["ld"], t["npt2"], t["xstar"], t["xavs"], t["xavs1"], t["ecoul"], t["ecoul1"])
     led = D.Ledger(ctx, box)
     led.cells_init(s.nmolcell, T.sepmax(s, g))
-    led.fmolec1(tab, s.eintra)
+    led.fmolec1(tab, H.eintra(name))
     e, p, w, se, sw = led.press(tab, g.npt1, s.simt, H.box_volume(s), slots=True)
     torch.cuda.synchronize()
     print(f"  energy {e} kJ/mol, pressure {p} kbar", flush=True)
@@ -86,7 +86,7 @@ def test_press_after_an_mc_sweep_bit_exact(ctx):
     tab = D.DeviceTables(ctx, t["ld"], t["npt2"], t["xstar"], t["xavs"], t["xavs1"], t["ecoul"], t["ecoul1"])
     led = D.Ledger(ctx, box)
     led.cells_init(s.nmolcell, T.sepmax(s, g))
-    led.fmolec1(tab, s.eintra)
+    led.fmolec1(tab, H.eintra(name))
     wr2, wt2 = H.initial_steps(s)
     mc = D.MCState(ctx, s.nmol, s.nmoltype, s.simt, wholerot=wr2, wholetrans=wt2)
     st = D.ran1_state_array(-31)
