@@ -9,7 +9,7 @@ F32 = np.float32
 
 def reset(n_mol=0, n_moltype=0):
     global xadd, yadd, zadd, naf, nal, nato, mtype, nmol, nmoltype, dmax
-    global wholerot, wholetrans, nwhole, nmovem, nstepreject, ngrot, internrot, interntrans
+    global wholerot, wholetrans, nwhole, nmovem, nstepreject, ngrot, internrot, interntrans, ninter, nmovei
     xadd, yadd, zadd = real(n_mol), real(n_mol), real(n_mol)
     naf, nal, nato, mtype = integer(n_mol), integer(n_mol), integer(n_mol), integer(n_mol)
     ngrot = integer(n_mol)
@@ -19,6 +19,7 @@ def reset(n_mol=0, n_moltype=0):
     wholerot, wholetrans = real(n_moltype), real(n_moltype)
     internrot, interntrans = real(n_moltype), real(n_moltype)
     nwhole, nmovem = integer(2, n_moltype), integer(2, n_moltype)
+    ninter, nmovei = integer(2, n_moltype), integer(2, n_moltype)
 
 
 reset()

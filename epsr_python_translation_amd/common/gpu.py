@@ -19,15 +19,17 @@ _ledger = None
 _version = -1
 version = 0          # bumped by common.setup whenever the topology changes
 eintra = None        # [nmol] intramolecular energy of each molecule (0-based)
+intra = None         # the box's intramolecular topology (intra.IntraTopology), if loaded
 
 
 def reset():
-    global _box, _ledger, _version, version, eintra
+    global _box, _ledger, _version, version, eintra, intra
     _box = None
     _ledger = None
     _version = -1
     version += 1
     eintra = None
+    intra = None
 
 
 def ctx() -> D.Context:
@@ -52,6 +54,8 @@ def box() -> D.DeviceBox:
                            cellst=epsr16.cellst, xa=molecom.xa[1:n + 1], ya=molecom.ya[1:n + 1],
                            za=molecom.za[1:n + 1], xadd=clustercom.xadd[1:m + 1], yadd=clustercom.yadd[1:m + 1],
                            zadd=clustercom.zadd[1:m + 1], nmoltype=clustercom.nmoltype)
+        if intra is not None:
+            _box.set_intra(intra)
         _ledger = None
         _version = version
     return _box

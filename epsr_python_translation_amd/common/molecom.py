@@ -12,6 +12,7 @@ def reset(n_atom=0, n_type=0, n_pairs=0):
     global xa, ya, za, itype, npairref, n1extr, n2extr, cfrac, charge, qradius
     global ecor, dcor, qcor, natom, natomr, ntype, npairs, sizefactor, power, simt
     global stepr, stepm, stepri, stepmi, tol, tolout, xs, ys, zs
+    global tempb, tempangwt, tempdihwt, tempangreq, tempdihreq
     xa, ya, za = real(n_atom), real(n_atom), real(n_atom)
     itype = integer(n_atom)
     npairref = integer(n_type, n_type)
@@ -27,6 +28,9 @@ def reset(n_atom=0, n_type=0, n_pairs=0):
     stepri = stepmi = F32(0.0)
     tol = tolout = F32(0.0)
     xs = ys = zs = F32(0.0)
+    tempb = F32(0.0)
+    tempangwt = tempangreq = F32(1.0)
+    tempdihwt = tempdihreq = F32(0.0)
 
 
 reset()

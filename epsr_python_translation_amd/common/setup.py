@@ -62,5 +62,22 @@ def load(s, idum=-1, nfile=1, grid=None):
     st[1] = idum
     ran1com.set_state(st)
     gpu.reset()
-    gpu.eintra = np.asarray(s.eintra, F32).copy()
+    if s.intra is not None:
+        # a box read from files: its intramolecular topology and the MC run state it carries
+        r = s.mc_run
+        gpu.intra = s.intra
+        gpu.eintra = None
+        molecom.stepr, molecom.stepm = F32(r["stepr"]), F32(r["stepm"])
+        molecom.stepri, molecom.stepmi = F32(r["stepri"]), F32(r["stepmi"])
+        molecom.tempb = F32(s.intra.tempb)
+        molecom.tempangreq, molecom.tempdihreq = F32(r["tempangreq"]), F32(r["tempdihreq"])
+        molecom.tempangwt, molecom.tempdihwt = F32(r["tempangwt"]), F32(r["tempdihwt"])
+        clustercom.ngrot[1:] = s.intra.ngrot
+        clustercom.wholerot[1:] = r["wholerot"]
+        clustercom.wholetrans[1:] = r["wholetrans"]
+        clustercom.internrot[1:] = r["internrot"]
+        clustercom.interntrans[1:] = r["interntrans"]
+        clustercom.nstepreject[1:] = r["nstepreject"]
+    else:
+        gpu.eintra = np.asarray(s.eintra, F32).copy()
     return g

@@ -62,8 +62,14 @@ def fmolec1(ierr, num_threads_input, ftest, ftestb, ftethered, fcoordconstraints
     grid = T.Grid(x=epsr16.x, xstar=epsr16.xstar, ycoh=epsr16.ycoh, r2=None, q=epsr16.q, npt=epsr16.npt,
                   npt1=epsr16.npt1, npt2=epsr16.npt2, nptmin=epsr16.nptmin, nptmax=epsr16.nptmax, nq=epsr16.nq)
     led.cells_init(epsr16.nmolcell, _sepmax(grid))
-    eintra = gpu.eintra if gpu.eintra is not None else np.zeros(clustercom.nmol, F32)
-    f64, fb64 = led.fmolec1(gpu.tables(), c.tensor(eintra))
+    if gpu.intra is not None:
+        # en_mole_intra of every molecule, with the weights fmolec1 sets (:2416-2419)
+        molecom.tempangwt, molecom.tempdihwt = molecom.tempangreq, molecom.tempdihreq
+        ei = D.en_mole_intra(c, gpu.box(), gpu.tables(), molecom.simt, molecom.tempangreq,
+                             F32(molecom.tempdihreq * molecom.tempb))
+    else:
+        ei = c.tensor(gpu.eintra if gpu.eintra is not None else np.zeros(clustercom.nmol, F32))
+    f64, fb64 = led.fmolec1(gpu.tables(), ei)
     return 0, max(1, int(num_threads_input)), F32(f64), F32(fb64), F32(0.0), F32(fcoordconstraints)
 
 

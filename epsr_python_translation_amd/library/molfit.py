@@ -1,6 +1,9 @@
 """molfit.molfit, en_change and en_mole_pair over the COMMON mirror
-(src/epsr26/library/molfit.py:51-541, 805-919, 1527-1578).  molfit covers the
-whole-molecule branch (nisum < intra, no internal rotation groups, no tethering)."""
+(src/epsr26/library/molfit.py:51-541, 805-919, 1527-1578).  With an intramolecular topology
+loaded (common.setup from a box read from files) molfit runs every branch of the
+reference's cycle on the GPU: the shake every `intra` cycles, internal rotations every
+`rotfreq` cycles and the whole-molecule moves.  Without one it runs the whole-molecule
+cycles and refuses the cycles that would move atoms inside molecules."""
 
 from __future__ import annotations
 
@@ -15,25 +18,58 @@ F32 = np.float32
 last_stats = None
 
 
-def molfit(fadds, faddbs, nisum, intra, rotfreq, ireset):
-    """One MC cycle of whole-molecule rotations and translations over the cells, on the
-    ledger of the last fmolec1.  Returns (fadds, faddbs, nisum)."""
-    global last_stats
+def _moves_inside_molecules(nisum, intra, rotfreq):
+    """True when the reference's cycle would move atoms inside molecules (molfit.py:124,295)."""
     if nisum >= intra:
-        raise NotImplementedError("molfit: the intramolecular branch (nisum >= intra) is not part of this package")
+        return True
+    if molecom.stepri > 0.0 and np.any(clustercom.ngrot[1:clustercom.nmol + 1] > 0):
+        if rotfreq == 0:
+            raise ZeroDivisionError("molfit: rotfreq = 0 with internal rotations (nisum / rotfreq)")
+        return nisum == rotfreq * int(nisum / rotfreq)
+    return False
+
+
+def molfit(fadds, faddbs, nisum, intra, rotfreq, ireset):
+    """One MC cycle on the ledger of the last fmolec1.  Returns (fadds, faddbs, nisum)."""
+    global last_stats
+    if gpu.intra is None and _moves_inside_molecules(nisum, intra, rotfreq):
+        raise NotImplementedError("molfit: this cycle moves atoms inside molecules (shake or internal rotations), "
+                                  "which needs the box's intramolecular topology (load it from .ato files)")
     if not gpu.has_ledger():
         raise RuntimeError("molfit: no pair-energy ledger (call fmolec1 first)")
     c = gpu.ctx()
     b = gpu.upload_coords()
     led = gpu.ledger()
     nmol, nmt = clustercom.nmol, clustercom.nmoltype
-    mc = D.MCState(c, nmol, nmt, molecom.simt, stepr=molecom.stepr, stepm=molecom.stepm, rejrate=epsr16.rejrate,
-                   ireset=int(ireset), wholerot=clustercom.wholerot[1:nmt + 1].copy(),
-                   wholetrans=clustercom.wholetrans[1:nmt + 1].copy())
-    mc.nstepreject.copy_(torch.from_numpy(np.ascontiguousarray(clustercom.nstepreject[1:nmol + 1])))
-    mc.fadds = F32(fadds)
     st = ran1com.state()
-    D.molfit(c, b, gpu.tables(), led, mc, st)
+    if gpu.intra is not None:
+        mc = D.MCFullState(c, nmol, nmt, molecom.simt, stepr=molecom.stepr, stepm=molecom.stepm,
+                           stepri=molecom.stepri, stepmi=molecom.stepmi, rejrate=epsr16.rejrate,
+                           tempangwt=molecom.tempangwt, tempdihwt=molecom.tempdihwt, ireset=int(ireset),
+                           intra=int(intra), rotfreq=int(rotfreq), nisum=int(nisum),
+                           wholerot=clustercom.wholerot[1:nmt + 1], wholetrans=clustercom.wholetrans[1:nmt + 1],
+                           internrot=clustercom.internrot[1:nmt + 1], interntrans=clustercom.interntrans[1:nmt + 1],
+                           nstepreject=clustercom.nstepreject[1:nmol + 1])
+        mc.fadds, mc.faddbs = F32(fadds), F32(faddbs)
+        D.molfit_full(c, b, gpu.tables(), led, mc, st)
+        clustercom.internrot[1:nmt + 1] = mc.internrot
+        clustercom.interntrans[1:nmt + 1] = mc.interntrans
+        clustercom.ninter[1, 1:nmt + 1], clustercom.ninter[2, 1:nmt + 1] = mc.ninter[0::2], mc.ninter[1::2]
+        clustercom.nmovei[1, 1:nmt + 1], clustercom.nmovei[2, 1:nmt + 1] = mc.nmovei[0::2], mc.nmovei[1::2]
+        nisum, faddbs = mc.nisum, mc.faddbs
+    else:
+        mc = D.MCState(c, nmol, nmt, molecom.simt, stepr=molecom.stepr, stepm=molecom.stepm, rejrate=epsr16.rejrate,
+                       ireset=int(ireset), wholerot=clustercom.wholerot[1:nmt + 1].copy(),
+                       wholetrans=clustercom.wholetrans[1:nmt + 1].copy())
+        mc.nstepreject.copy_(torch.from_numpy(np.ascontiguousarray(clustercom.nstepreject[1:nmol + 1])))
+        mc.fadds = F32(fadds)
+        D.molfit(c, b, gpu.tables(), led, mc, st)
+        # molecules without rotational groups zero their type's internal rotation step
+        for im in range(1, nmol + 1):
+            if clustercom.ngrot[im] == 0:
+                clustercom.internrot[clustercom.mtype[im]] = 0.0
+        nisum = nisum + 1
+        faddbs = F32(F32(faddbs) + F32(F32(0.0) / F32(nmol)))
     ran1com.set_state(st)
     gpu.download_coords()
     clustercom.wholerot[1:nmt + 1] = mc.wholerot
@@ -44,7 +80,6 @@ def molfit(fadds, faddbs, nisum, intra, rotfreq, ireset):
     clustercom.nmovem[2, 1:nmt + 1] = mc.nmovem[1::2]
     clustercom.nstepreject[1:nmol + 1] = mc.nstepreject.cpu().numpy()
     last_stats = mc.stats
-    nisum = 1 if nisum >= intra else nisum + 1
     return mc.fadds, F32(faddbs), nisum
 
 

@@ -1,0 +1,38 @@
+"""The drop-in molfit without an intramolecular topology refuses every cycle that would
+move atoms inside molecules -- the shake (nisum >= intra, molfit.py:124) and the cycles with
+internal rotations (stepri > 0, rotational groups, nisum a multiple of rotfreq, :295) --
+instead of running a different trajectory; the check runs before any GPU work."""
+
+import numpy as np
+import pytest
+
+F32 = np.float32
+
+
+@pytest.fixture()
+def loaded():
+    from epsr_python_translation_amd import synth
+    from epsr_python_translation_amd.common import setup
+    s = synth.tiny(seed=21, nwater=20, ndmso=4, L=14.0)
+    setup.load(s, idum=-9)
+    return s
+
+
+def test_shake_cycle_raises(loaded):
+    from epsr_python_translation_amd.library import molfit as M
+    with pytest.raises(NotImplementedError):
+        M.molfit(F32(0), F32(0), 100, 100, 5, 0)
+
+
+def test_rotation_cycle_raises(loaded):
+    from epsr_python_translation_amd.common import clustercom, molecom
+    from epsr_python_translation_amd.library import molfit as M
+    molecom.stepri = F32(0.3)
+    clustercom.ngrot[1] = 2
+    with pytest.raises(NotImplementedError):
+        M.molfit(F32(0), F32(0), 5, 100, 5, 0)
+    # not a multiple of rotfreq: a whole-molecule cycle, which needs the ledger first
+    with pytest.raises(RuntimeError, match="ledger"):
+        M.molfit(F32(0), F32(0), 4, 100, 5, 0)
+    with pytest.raises(ZeroDivisionError):
+        M.molfit(F32(0), F32(0), 4, 100, 0, 0)
