@@ -6,6 +6,8 @@
 //          the reference evaluated with a correctly rounded sinf);
 //  mode 1  the same contraction as an FP32 MFMA GEMM (v_mfma_f32_16x16x4_f32) with
 //          FP32 accumulation; the right-hand operand is assembled on the fly.
+#include <mutex>
+
 #include "common.h"
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
@@ -137,6 +139,29 @@ __global__ void gensq_colsum(int npairs, int npt1, int ldw, const float *__restr
     if (threadIdx.x == 0) out[ic] = s[0];
 }
 
+// scratch of mode 1: column sums (256-byte aligned) then GENSQ_KSPLIT partial tiles
+static size_t gensq_scratch_bytes(int nq, int npairs)
+{
+    const int ncolp = 16 * ceil_div(2 * npairs, 16);
+    const size_t csb = (sizeof(float) * npairs + 255) & ~(size_t)255;
+    return csb + sizeof(float) * (size_t)GENSQ_KSPLIT * ncolp * nq;
+}
+
+// mode 1 on stream st with its scratch at sc (gensq_scratch_bytes)
+static void gensq_mfma_launch(hipStream_t st, int nq, int npairs, int npt1, int ldw, const float *wav,
+                              const float *ycoh, const float *foqols, int ldf, const float *natomsphere, float *sofq,
+                              int ldq, char *sc)
+{
+    const int ncolp = 16 * ceil_div(2 * npairs, 16);
+    const size_t csb = (sizeof(float) * npairs + 255) & ~(size_t)255;
+    float *cs = (float *)sc, *part = (float *)(sc + csb);
+    gensq_colsum<<<npairs, 256, 0, st>>>(npairs, npt1, ldw, wav, cs);
+    dim3 grid(ceil_div(nq, 16), ncolp / 16, GENSQ_KSPLIT);
+    gensq_mfma<<<grid, 64, 0, st>>>(nq, npairs, npt1, ldw, wav, ycoh, foqols, ldf, natomsphere, part);
+    gensq_reduce<<<ceil_div(2L * npairs * nq, 256), 256, 0, st>>>(nq, npairs, ncolp, GENSQ_KSPLIT, part, cs, sofq,
+                                                                  ldq);
+}
+
 extern "C" int epsr_gensq(epsr_ctx *ctx, int nq, int npairs, int npt1, int ldw, const float *wav, const float *ycoh,
                           const float *foqols, int ldf, const float *natomsphere, float *sofq, int ldq, int mode)
 {
@@ -152,18 +177,59 @@ extern "C" int epsr_gensq(epsr_ctx *ctx, int nq, int npairs, int npt1, int ldw, 
         int n = nq * 2 * npairs;
         gensq_exact<<<ceil_div(n, 64), 64, 0, st>>>(nq, npairs, npt1, ldw, wav, ycoh, foqols, ldf, natomsphere, sofq, ldq);
     } else {
-        const int ncolp = 16 * ceil_div(2 * npairs, 16);
-        const size_t csb = (sizeof(float) * npairs + 255) & ~(size_t)255;
-        char *sc = (char *)ctx_scratch(ctx, csb + sizeof(float) * (size_t)GENSQ_KSPLIT * ncolp * nq);
+        char *sc = (char *)ctx_scratch(ctx, gensq_scratch_bytes(nq, npairs));
         if (!sc) return EPSR_ERR_HIP;
-        float *cs = (float *)sc, *part = (float *)(sc + csb);
-        gensq_colsum<<<npairs, 256, 0, st>>>(npairs, npt1, ldw, wav, cs);
-        dim3 grid(ceil_div(nq, 16), ncolp / 16, GENSQ_KSPLIT);
-        gensq_mfma<<<grid, 64, 0, st>>>(nq, npairs, npt1, ldw, wav, ycoh, foqols, ldf, natomsphere, part);
-        gensq_reduce<<<ceil_div(2L * npairs * nq, 256), 256, 0, st>>>(nq, npairs, ncolp, GENSQ_KSPLIT, part, cs,
-                                                                      sofq, ldq);
+        gensq_mfma_launch(st, nq, npairs, npt1, ldw, wav, ycoh, foqols, ldf, natomsphere, sofq, ldq, sc);
     }
     timer_end(ctx, "gensq");
+    HIP_CHECK(ctx, hipGetLastError());
+    HIP_CHECK(ctx, ctx_sync(ctx));
+    return EPSR_OK;
+}
+
+// ------------------------------------------------------------------ gensq without allocation
+// The scratch of mode 1 (column sums and the GENSQ_KSPLIT partial tiles) in a file-scope
+// device array, so that a call allocates nothing.  Each GPU holds its own copy at its own
+// address, found once per device with hipGetSymbolAddress; host code uses only that address.
+#define GENSQ_NOALLOC_BYTES ((size_t)8 << 20)
+static __device__ __align__(256) char scratch[GENSQ_NOALLOC_BYTES];
+
+static std::mutex scratch_mu;
+static void *scratch_dev[256];   // the array's address on each device (nullptr: not looked up yet)
+
+static int scratch_address(epsr_ctx *ctx, void **out)
+{
+    std::lock_guard<std::mutex> g(scratch_mu);
+    ARG_CHECK(ctx, ctx->device >= 0 && ctx->device < 256, "epsr_gensq_noalloc: device %d", ctx->device);
+    if (!scratch_dev[ctx->device]) {
+        void *p = nullptr;
+        HIP_CHECK(ctx, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        ARG_CHECK(ctx, p != nullptr, "epsr_gensq_noalloc: no address for the scratch array");
+        scratch_dev[ctx->device] = p;
+    }
+    *out = scratch_dev[ctx->device];
+    return EPSR_OK;
+}
+
+extern "C" int epsr_gensq_noalloc(epsr_ctx *ctx, int nq, int npairs, int npt1, int ldw, const float *wav,
+                                  const float *ycoh, const float *foqols, int ldf, const float *natomsphere,
+                                  float *sofq, int ldq)
+{
+    if (!ctx) return EPSR_ERR_ARG;
+    ARG_CHECK(ctx, wav && ycoh && foqols && sofq, "epsr_gensq_noalloc: null argument");
+    ARG_CHECK(ctx, nq >= 1 && npairs >= 1 && npt1 >= 2 && ldw > npt1 && ldf > npt1 && ldq > nq,
+              "epsr_gensq_noalloc: bad sizes nq=%d npairs=%d npt1=%d ldw=%d ldf=%d ldq=%d", nq, npairs, npt1, ldw,
+              ldf, ldq);
+    const size_t need = gensq_scratch_bytes(nq, npairs);
+    ARG_CHECK(ctx, need <= GENSQ_NOALLOC_BYTES,
+              "epsr_gensq_noalloc: nq=%d x npairs=%d needs %zu bytes of scratch, above its %zu", nq, npairs, need,
+              GENSQ_NOALLOC_BYTES);
+    HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    void *sc = nullptr;
+    int rc = scratch_address(ctx, &sc);
+    if (rc) return rc;
+    // every value the kernels read from the scratch is written first in the same call
+    gensq_mfma_launch(ctx->stream, nq, npairs, npt1, ldw, wav, ycoh, foqols, ldf, natomsphere, sofq, ldq, (char *)sc);
     HIP_CHECK(ctx, hipGetLastError());
     HIP_CHECK(ctx, ctx_sync(ctx));
     return EPSR_OK;

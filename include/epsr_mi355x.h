@@ -123,6 +123,16 @@ int epsr_gensq(epsr_ctx *ctx, int nq, int npairs, int npt1, int ldw, const float
                const float *ycoh, const float *foqols, int ldf, const float *natomsphere,
                float *sofq, int ldq, int mode);
 
+/* ---- gensq without allocation.  No counterpart in the reference: epsr_gensq mode 1 (the
+ * FP32 MFMA GEMM, same kernels, same results) for callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation.  Its scratch (column sums and 16 partial tiles, 4*npairs rounded up to 256
+ * bytes + 64 * ceil(2*npairs/16) * 16 * nq bytes) lives in a fixed 8 MiB device array of the
+ * library, one per GPU, held until the process ends; larger nq x npairs are refused with
+ * EPSR_ERR_ARG.  Calls that overlap on one GPU share that array and race. */
+int epsr_gensq_noalloc(epsr_ctx *ctx, int nq, int npairs, int npt1, int ldw, const float *wav, const float *ycoh,
+                       const float *foqols, int ldf, const float *natomsphere, float *sofq, int ldq);
+
 /* ---- stog: replaces ato_manipulation_rev4.stog(rho, q, sofq, r, gofr, nqmin, nqmax,
  * nr, npairs, nqdim, nrdim, npdim, xwt) (ato_manipulation_rev4.py:4066-4111; Fortran
  * :4154) including lorch (:4114-4124).
