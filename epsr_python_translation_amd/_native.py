@@ -97,6 +97,7 @@ class MCStats(C.Structure):
 
 LAUNCH_ATOGEN_WORKGROUPS = 1
 LAUNCH_PAIR_BLOCK = 2
+LAUNCH_RESOLVE_PASSES = 3
 
 
 class DescParams(C.Structure):

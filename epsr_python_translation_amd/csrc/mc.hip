@@ -1838,10 +1838,11 @@ mc_resolve_finish(const int *__restrict__ order, const int *__restrict__ rowstar
                   const float *__restrict__ delta, const float2 *__restrict__ thr, const float *__restrict__ stream,
                   const int *__restrict__ moff, float temp, const float *__restrict__ shift, int maxc,
                   int *__restrict__ wa, const int *__restrict__ na, int *__restrict__ wb, unsigned char *state,
-                  int *nexact, int *npass, int nmol, unsigned *__restrict__ bits)
+                  int *nexact, int *npass, int nmol, unsigned *__restrict__ bits, int maxpass,
+                  int *__restrict__ undecided)
 {
     extern __shared__ float2 fterm2[];   // [waves][maxc]
-    __shared__ int ncur, nnext;
+    __shared__ int ncur, nnext, nund;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nwv = (int)(blockDim.x >> 6);
     float2 *term = fterm2 + (size_t)wv * maxc;
     if (threadIdx.x == 0) {
@@ -1850,8 +1851,11 @@ mc_resolve_finish(const int *__restrict__ order, const int *__restrict__ rowstar
     }
     __syncthreads();
     int *cur = wa, *nxt = wb;
+    // every pass decides at least the earliest undecided move of the chain, so nmol passes
+    // always suffice; the cap only guards against a fault in that argument, and what is left
+    // undecided is counted for the host
     int passes = 0;
-    while (ncur > 0) {
+    while (ncur > 0 && passes < maxpass) {
         for (int w = wv; w < ncur; w += nwv) {
             const int k = cur[w];
             const int d = mc_pass_move(k, order, rowstart, cnt, pnv, mw, A, Bv, L, delta, thr, stream, moff, temp,
@@ -1873,15 +1877,24 @@ mc_resolve_finish(const int *__restrict__ order, const int *__restrict__ rowstar
         nxt = t;
         __syncthreads();
     }
-    if (threadIdx.x == 0) *npass = passes;
+    if (threadIdx.x == 0) {
+        *npass = passes;
+        nund = 0;
+    }
+    __syncthreads();
     for (int w = threadIdx.x; w < (nmol + 31) / 32; w += blockDim.x) {
         unsigned b = 0u;
+        int und = 0;
         for (int j = 0; j < 32; ++j) {
             const int k = w * 32 + j;
             if (k < nmol && state[k] == 1) b |= 1u << j;
+            if (k < nmol && state[k] == 0) ++und;
         }
         bits[w] = b;
+        if (und) atomicAdd(&nund, und);
     }
+    __syncthreads();
+    if (threadIdx.x == 0) *undecided = nund;
 }
 
 
@@ -2336,7 +2349,8 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
                                                           dpc + 2, dstate, dpc + 3);
             mc_resolve_finish<<<1, 64 * nwf, sizeof(float2) * nwf * (size_t)maxc, st>>>(
                 l->d_order, l->d_rowstart, l->d_cnt, l->d_pn, l->d_mw, dA, dB, dL, ddelta, dthr, stream, moff, temp,
-                dshift, maxc, dW0, dpc + 2, dW1, dstate, dpc + 3, dpc + 4, nmol, dacc);
+                dshift, maxc, dW0, dpc + 2, dW1, dstate, dpc + 3, dpc + 4, nmol, dacc,
+                box->resolve_passes >= 0 ? box->resolve_passes : nmol + 1, dpc + 5);
             timer_end(ctx, "mc_resolve");
             // fadd only reports: the exact fdiff of the accepted moves is formed on the side
             // stream beside the commits, and summed on the host (a sequential float sum is
@@ -2505,6 +2519,11 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     if (fast) {
         nexact = hpc[3];
         npasses = 3 + hpc[4];
+        if (hpc[5] != 0)
+            return set_error(ctx, EPSR_ERR_HIP,
+                             "epsr_molfit: the resolution left %d moves undecided after %d passes; coordinates and "
+                             "ledger are not the chain's",
+                             hpc[5], npasses);
     }
     // counters and the step-size adaptation (molfit.py:465-536)
     for (int mt = 0; mt < nmt; ++mt) {

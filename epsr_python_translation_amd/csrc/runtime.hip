@@ -279,6 +279,10 @@ extern "C" int epsr_box_set_launch(epsr_box *b, int knob, int value)
         ARG_CHECK(ctx, value == 256 || value == 512 || value == 1024, "epsr_box_set_launch: pair block %d", value);
         b->pair_block = value;
         return EPSR_OK;
+    case EPSR_LAUNCH_RESOLVE_PASSES:
+        ARG_CHECK(ctx, value >= 0, "epsr_box_set_launch: resolver passes %d", value);
+        b->resolve_passes = value;
+        return EPSR_OK;
     default:
         return set_error(ctx, EPSR_ERR_ARG, "epsr_box_set_launch: unknown knob %d", knob);
     }

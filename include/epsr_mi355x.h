@@ -75,9 +75,13 @@ int epsr_box_create(epsr_ctx *ctx, const epsr_box_desc_t *desc, epsr_box **out);
  * no launches), set by the autotuner per box size: EPSR_LAUNCH_ATOGEN_WORKGROUPS, the
  * workgroups atogen's pair kernel aims for (partner slices per origin block; default
  * 16384), and EPSR_LAUNCH_PAIR_BLOCK, the workgroup size of mc_pairs and fmolec1's pair
- * kernel (256, 512 or 1024; default 1024).  Results do not depend on them. */
+ * kernel (256, 512 or 1024; default 1024).  Results do not depend on them.
+ * EPSR_LAUNCH_RESOLVE_PASSES caps the passes of molfit's one-workgroup resolution (default
+ * nmol + 1, which always suffices); a cycle that ends with moves undecided returns
+ * EPSR_ERR_HIP.  A smaller cap (>= 0) exists to test that path. */
 #define EPSR_LAUNCH_ATOGEN_WORKGROUPS 1
 #define EPSR_LAUNCH_PAIR_BLOCK 2
+#define EPSR_LAUNCH_RESOLVE_PASSES 3
 int epsr_box_set_launch(epsr_box *box, int knob, int value);
 
 /* HIP graphs of a context's stream (no counterpart in the reference): work issued on

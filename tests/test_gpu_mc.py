@@ -111,11 +111,11 @@ def test_en_change_and_en_mole_pair_bit_exact(ctx):
     assert np.array_equal(got.view(np.int32), want.view(np.int32))
 
 
-def _run_oracle_molfit(name, flavour, nsweep, idum):
+def _run_oracle_molfit(name, flavour, nsweep, idum, steps=None):
     s = H.system(name)
     ref = H.oracle_fmolec1(name, flavour)
     ob = ref["ob"]
-    wr, wt = H.initial_steps(s)
+    wr, wt = H.initial_steps(s) if steps is None else (steps[0].copy(), steps[1].copy())
     nwh = np.zeros(2 * s.nmoltype, np.int32)
     nmv = np.zeros(2 * s.nmoltype, np.int32)
     nsr = np.zeros(s.nmol, np.int32)
@@ -276,3 +276,53 @@ def test_molfit_after_other_ran1_draws(ctx, gap):
     assert np.array_equal(led.export()["ergpair"].view(np.int32),
                           H.ledger_dict(ref["ledger"], s.nmol)["ergpair"].view(np.int32))
     assert F32(mc.fadds).view(np.int32) == F32(fadds.value).view(np.int32)
+
+
+def test_movem_step_doubling_bit_exact(ctx):
+    """A translation step far below the float32 spacing of the centres: movem doubles it
+    until the centre moves (molfit.py:747-758), the doubled step persists for the type, and
+    the GPU's host-resolved redo of the cycle equals the oracle's chain bit for bit."""
+    from epsr_python_translation_amd import device as D
+    name = "tiny_13_12_3"
+    s = H.system(name)
+    wr = np.full(s.nmoltype, 0.05, F32)
+    wt = np.full(s.nmoltype, 1.0e-12, F32)
+    print("molfit with a 1e-12 translation step (movem doubling)", flush=True)
+    ref = _run_oracle_molfit(name, "cr", 1, -77, steps=(wr, wt))
+    box, tab, led, _, _ = _gpu_setup(ctx, name)
+    mc = D.MCState(ctx, s.nmol, s.nmoltype, s.simt, wholerot=wr.copy(), wholetrans=wt.copy())
+    st = D.ran1_state_array(-77)
+    D.molfit(ctx, box, tab, led, mc, st)
+    torch.cuda.synchronize()
+    assert mc.stats.retries > 0, "the step was doubled"
+    co = box.get_coords()
+    for k in ("xa", "ya", "za", "xadd", "yadd", "zadd"):
+        assert np.array_equal(co[k].view(np.int32), getattr(ref["ob"], k).view(np.int32)), k
+    np.testing.assert_array_equal(st, ref["ran1"])
+    assert np.array_equal(mc.wholetrans.view(np.int32), ref["wt"].view(np.int32))
+    assert np.all(mc.wholetrans > 1.0e-12)
+    gd = led.export()
+    assert np.array_equal(gd["ergpair"].view(np.int32), ref["ledger"]["ergpair"].view(np.int32))
+
+
+def test_resolver_pass_cap_reports_undecided_moves(ctx):
+    """The one-workgroup resolution stops at its pass cap (set small from the host) and the
+    cycle fails loudly instead of committing undecided moves."""
+    from epsr_python_translation_amd import _native
+    from epsr_python_translation_amd import device as D
+    name = "c1"
+    s = H.system(name)
+    box, tab, led, _, _ = _gpu_setup(ctx, name)
+    wr, wt = H.initial_steps(s)
+    mc = D.MCState(ctx, s.nmol, s.nmoltype, s.simt, wholerot=wr, wholetrans=wt)
+    st = D.ran1_state_array(-5)
+    D.molfit(ctx, box, tab, led, mc, st)
+    finisher = mc.stats.resolver_passes - 3
+    print(f"resolver: {finisher} finishing passes uncapped", flush=True)
+    assert finisher >= 1
+    box2, tab2, led2, _, _ = _gpu_setup(ctx, name)
+    box2.set_launch(_native.LAUNCH_RESOLVE_PASSES, finisher - 1)
+    mc2 = D.MCState(ctx, s.nmol, s.nmoltype, s.simt, wholerot=wr, wholetrans=wt)
+    st2 = D.ran1_state_array(-5)
+    with pytest.raises(D.EpsrError, match="undecided"):
+        D.molfit(ctx, box2, tab2, led2, mc2, st2)
