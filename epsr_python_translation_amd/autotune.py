@@ -1,5 +1,5 @@
-"""Launch-shape autotuner for a box: picks, by timing on the box itself, the workgroup
-target of atogen's pair kernel and the workgroup size of the pair-energy kernels
+"""Launch-shape autotuner for a box: picks, by timing on the box itself, the resident
+workgroups of atogen's persistent pair kernel and the workgroup size of the pair-energy kernels
 (epsr_box_set_launch).  Results never depend on these shapes, only the time does.  The
 choice is cached per size class (atoms rounded to a quarter power of two, type pairs),
 so an ensemble of similar boxes tunes once per class."""
@@ -14,7 +14,7 @@ import torch
 
 from . import _native
 
-ATOGEN_WORKGROUPS = (1024, 2048, 4096, 8192, 16384)
+ATOGEN_WORKGROUPS = (768, 1024, 1536, 2048)
 PAIR_BLOCKS = (256, 512, 1024)
 
 _CACHE: dict = {}

@@ -15,8 +15,8 @@
 //    normalisation kernel reproduces.
 #include "common.h"
 
-#define ATOGEN_BLOCK 256
-#define ATOGEN_TILE 256
+#define ATOGEN_BLOCK 512       // origin atoms (lanes) per workgroup of the pair kernel
+#define ATOGEN_MIN_SLICE 256   // fewest partner atoms per work item
 
 // absolute positions x2 = xa + xadd (one fp32 add, as :3794-3796), type in .w
 __global__ void atogen_positions(const float *__restrict__ xa, const float *__restrict__ ya,
@@ -102,116 +102,194 @@ __global__ void atogen_rminex(int npairs, const int *__restrict__ irefpcof, cons
     if (v > 0.0f) atomicOr(flag, 1u);
 }
 
-// the tiles of partner atoms p0..p1e-1 against this thread's origin atom (p1), into the
-// block's histograms; every thread of the block takes part in the tile loads
-template <bool POTEX>
-__device__ __forceinline__ void atogen_tiles(const float4 *__restrict__ P, const int *__restrict__ amol, int p0,
-                                             int p1e, bool active, int lo, int mymol, float4 p1, int rowoff, int ldh,
-                                             int npt1, float stepcl, float inv_stepcl, float dmax, const int *snp,
-                                             const float *srm, float4 *tileP, int *tileM, uint32_t *hist,
-                                             uint32_t *spot)
+// partner atoms p0..p1e-1 against this lane's origin atom (p1), into the workgroup's
+// histograms.  Every lane of a wave takes the same partner at the same step, so the
+// partners' positions are wave-uniform (scalar) loads, four at a time.  A lane counts the
+// pair when the partner is outside its own molecule [mlo, mhi] (and, with FULL -- every
+// molecule an origin, partners after the origin's molecule -- at or past lo = mhi + 1);
+// with FULL, from the wave's largest lo (kfree) on every lane counts every partner.
+//  * bin: NINT(fl(sqrtf(dsq)) / stepcl) exactly.  qa = hardware sqrt x rounded 1/stepcl
+//    lies within 3 ulp of the correctly rounded quotient q, so when qa's distance from a
+//    half-integer exceeds 1.2e-6 (npt1 + 2) -- fixed for the launch, and more than 10 ulp
+//    of q wherever the bin can still be counted (rint(qa) <= npt1 - 2 needs
+//    qa < npt1 - 1.5) -- NINT(q) = rint(qa); otherwise the correctly rounded sqrtf and
+//    division decide.  Beyond npt1 - 2 on the first path no pair is counted, and none is
+//    on the second either (q > npt1 - 1.5 - 10 ulp: NINT(q) + 2 > npt1).
+//  * PACK (one group of columns, at most 4 atom types, no close-contact counting): the
+//    byte offset in LDS of bin 2 of each partner type's column for this lane's origin
+//    type sits in two registers, 16 bits each (pk0: types 0, 1; pk1: types 2, 3); otherwise the pair
+//    column comes from the LDS table snp and the column test against [ic0, ic0 + ngrp).
+template <bool POTEX, bool FULL, bool PACK>
+__device__ __forceinline__ void atogen_scan(const float4 *__restrict__ P, int p0, int p1e, int kfree, int lo,
+                                            int mlo, int mhi, float4 p1, int rowoff, uint32_t pk0, uint32_t pk1,
+                                            int ldh, int npt1, float stepcl, float inv_stepcl, float dmax,
+                                            const int *snp, const float *srm, uint32_t *hist, uint32_t *spot, int ic0,
+                                            int ngrp)
 {
-    const int tid = threadIdx.x;
-    const float amb = 1.2e-6f;    // 10 ulp of q; the approximation is within 3 ulp
-    for (int t0 = p0; t0 < p1e; t0 += ATOGEN_TILE) {
-        int n = min(ATOGEN_TILE, p1e - t0);
-        __syncthreads();
-        if (tid < n) {
-            tileP[tid] = P[t0 + tid];
-            tileM[tid] = amol[t0 + tid];
+    const float bnd = 0.5f - 1.2e-6f * (float)(npt1 + 2);
+    const float lim = (float)(npt1 - 2);
+    auto pair = [&](const float4 &p2) {
+        const float xx = mic_sq(p1.x - p2.x), yy = mic_sq(p1.y - p2.y), zz = mic_sq(p1.z - p2.z);
+        const float dsq = xx * xx + yy * yy + zz * zz;
+        const float qa = __builtin_amdgcn_sqrtf(dsq) * inv_stepcl;
+        const float rq = rintf(qa);
+        int iz1;
+        bool in;
+        float dtest = 0.0f;
+        if (fabsf(qa - rq) < bnd) {
+            in = rq <= lim;
+            iz1 = (int)rq + 2;
+            if (POTEX) dtest = sqrtf(dsq);
+        } else {
+            dtest = sqrtf(dsq);
+            iz1 = dev_nint(dtest / stepcl) + 2;
+            in = iz1 > 1 && iz1 <= npt1;
         }
-        __syncthreads();
-        if (!active) continue;
-        const int kstart = max(0, lo - t0);
-#pragma unroll 4
-        for (int k = kstart; k < n; ++k) {
-            float4 p2 = tileP[k];
-            if (tileM[k] == mymol) continue;
-            const float xx = mic_sq(p1.x - p2.x), yy = mic_sq(p1.y - p2.y), zz = mic_sq(p1.z - p2.z);
-            float dsq = xx * xx + yy * yy + zz * zz;
-            const float qa = __builtin_amdgcn_sqrtf(dsq) * inv_stepcl;
-            const float rq = rintf(qa);
-            int iz1;
-            float dtest = 0.0f;
-            // away from half-integers (by more than amb q) rint(q) is NINT of the exact
-            // quotient; the bound is conservative, so fmaf may form it
-            if (fabsf(qa - rq) < fmaf(-amb, qa, 0.5f)) {
-                iz1 = (int)rq + 2;
-                if (POTEX) dtest = sqrtf(dsq);
-            } else {
-                dtest = sqrtf(dsq);
-                iz1 = dev_nint(dtest / stepcl) + 2;
+        const int t2 = __float_as_int(p2.w);
+        if (PACK) {
+            const uint32_t w = t2 < 2 ? pk0 : pk1;
+            const uint32_t base = (w >> ((t2 & 1) * 16)) & 0xffffu;   // byte offset of bin 2 - 2
+            if (in) atomicAdd((uint32_t *)((char *)hist + base + 4 * (iz1 - 2)), 1u);
+        } else {
+            const int g = snp[rowoff + t2] - ic0;   // column within this launch's group
+            if ((unsigned)g < (unsigned)ngrp) {
+                if (in) atomicAdd(&hist[g * ldh + iz1], 1u);
+                if (POTEX && dtest * dmax < srm[g]) atomicAdd(&spot[g], 1u);
             }
-            int icref = snp[rowoff + __float_as_int(p2.w)];
-            if (iz1 > 1 && iz1 <= npt1) atomicAdd(&hist[icref * ldh + iz1], 1u);
-            if (POTEX && dtest * dmax < srm[icref]) atomicAdd(&spot[icref], 1u);
         }
+    };
+    auto take = [&](int k, const float4 &p2) {
+        if (FULL ? k >= lo : (k < mlo || k > mhi)) pair(p2);
+    };
+    const int k1 = FULL ? min(p1e, max(p0, kfree)) : p1e;
+    int k = p0;
+    for (; k + 4 <= k1; k += 4) {
+        const float4 a0 = P[k], a1 = P[k + 1], a2 = P[k + 2], a3 = P[k + 3];
+        take(k, a0);
+        take(k + 1, a1);
+        take(k + 2, a2);
+        take(k + 3, a3);
+    }
+    for (; k < k1; ++k) take(k, P[k]);
+    if (FULL) {
+        for (; k + 4 <= p1e; k += 4) {
+            const float4 a0 = P[k], a1 = P[k + 1], a2 = P[k + 2], a3 = P[k + 3];
+            pair(a0);
+            pair(a1);
+            pair(a2);
+            pair(a3);
+        }
+        for (; k < p1e; ++k) pair(P[k]);
     }
 }
 
+// A persistent grid: gridDim.x workgroups take the work items (origin block bx of
+// ATOGEN_BLOCK origin atoms x partner slice by, item = bx * gy + by) from a queue (one
+// counter per column group, zero at launch) until it runs dry, accumulating into one LDS
+// histogram that is flushed to the global counts once, at the end -- the flush costs
+// gridDim.x histograms, not one per item.
+// blockIdx.y selects the group of pair columns [ic0, ic0 + ngrp) this workgroup counts,
+// so histograms larger than LDS split into groups (each group repeats the distances).
 __global__ void __launch_bounds__(ATOGEN_BLOCK)
-atogen_inter_fast(const float4 *__restrict__ P, const int *__restrict__ amol, const int *__restrict__ nal,
+atogen_inter_fast(const float4 *__restrict__ P, const int *__restrict__ amol, const int *__restrict__ naf,
+                  const int *__restrict__ nal,
                   const int *__restrict__ oatoms, int norigin, int full, int natom,
                   const int *__restrict__ npairref_g, int ntype, int npairs, int npt1, int ldw, float stepcl,
                   float inv_stepcl, float dmax, const float *__restrict__ rminex_abs,
                   const unsigned *__restrict__ potex_flag, uint32_t *__restrict__ counts,
-                  uint32_t *__restrict__ potex_counts)
+                  uint32_t *__restrict__ potex_counts, unsigned *__restrict__ queue, int gx, int gy, int ngrp)
 {
     const bool potex = *potex_flag != 0u;   // some |rminex| is non-zero
     extern __shared__ uint32_t smem[];
     const int ldh = npt1 + 1;
+    const int ic0 = blockIdx.y * ngrp;
+    const int ng = min(ngrp, npairs - ic0);
     uint32_t *hist = smem;
-    uint32_t *spot = hist + (size_t)npairs * ldh;
-    int *snp = (int *)(spot + npairs);
+    uint32_t *spot = hist + (size_t)ngrp * ldh;
+    int *snp = (int *)(spot + ngrp);
     float *srm = (float *)(snp + ntype * ntype);
-    float4 *tileP = (float4 *)(((uintptr_t)(srm + npairs) + 15) & ~(uintptr_t)15);
-    int *tileM = (int *)(tileP + ATOGEN_TILE);
 
     const int tid = threadIdx.x;
-    for (int i = tid; i < npairs * ldh; i += ATOGEN_BLOCK) hist[i] = 0u;
-    for (int i = tid; i < npairs; i += ATOGEN_BLOCK) {
+    for (int i = tid; i < ng * ldh; i += ATOGEN_BLOCK) hist[i] = 0u;
+    for (int i = tid; i < ng; i += ATOGEN_BLOCK) {
         spot[i] = 0u;
-        srm[i] = rminex_abs[i];
+        srm[i] = rminex_abs[ic0 + i];
     }
     for (int i = tid; i < ntype * ntype; i += ATOGEN_BLOCK) snp[i] = npairref_g[i];
+    __syncthreads();
 
-    const int oi = blockIdx.x * ATOGEN_BLOCK + tid;
-    const bool active = oi < norigin;
-    float4 p1 = make_float4(0.f, 0.f, 0.f, 0.f);
-    int mymol = -1, lo = natom, rowoff = 0;
-    if (active) {
-        int ia = oatoms ? oatoms[oi] : oi;
-        p1 = P[ia];
-        mymol = amol[ia];
-        lo = full ? nal[mymol] + 1 : 0;
-        rowoff = __float_as_int(p1.w) * ntype;
+    // packed offsets: one group of columns, at most 4 types, byte offsets below 2^16
+    const bool pack = !potex && ng == npairs && ntype <= 4 && 4 * (npairs * ldh + 2) < 65536;
+    const int nitems = gx * gy;
+    __shared__ int s_item;
+    for (;;) {
+        // the next work item of this group, from its queue: in full mode the items come
+        // heaviest first (origin block 0 has the most partners)
+        if (tid == 0) s_item = (int)atomicAdd(&queue[blockIdx.y], 1u);
+        __syncthreads();
+        const int item = s_item;
+        __syncthreads();
+        if (item >= nitems) break;
+        const int bx = item / gy, by = item - bx * gy;
+        const int oi = bx * ATOGEN_BLOCK + tid;
+        const bool active = oi < norigin;
+        float4 p1 = make_float4(0.f, 0.f, 0.f, 0.f);
+        int lo = natom, mlo = 0, mhi = -1, rowoff = 0;
+        uint32_t pk0 = 0u, pk1 = 0u;
+        if (active) {
+            int ia = oatoms ? oatoms[oi] : oi;
+            p1 = P[ia];
+            const int mymol = amol[ia];
+            mlo = naf[mymol];
+            mhi = nal[mymol];
+            lo = full ? mhi + 1 : 0;
+            rowoff = __float_as_int(p1.w) * ntype;
+            if (pack)
+                for (int t = 0; t < ntype; ++t) {
+                    const uint32_t o = (uint32_t)(4 * (snp[rowoff + t] * ldh + 2)) << ((t & 1) * 16);
+                    if (t < 2) pk0 |= o;
+                    else pk1 |= o;
+                }
+        }
+        const int first = bx * ATOGEN_BLOCK;
+        const int blo = full ? nal[amol[oatoms ? oatoms[first] : first]] + 1 : 0;
+        const int span = natom - blo;
+        const int per = (span + gy - 1) / gy;
+        const int p0 = blo + by * per;
+        const int p1e = min(natom, p0 + per);
+        // the largest lo of this lane's wave (full mode: the origins are atoms 0..norigin-1,
+        // so it is the lo of the wave's last active origin)
+        int kfree = natom;
+        if (full) {
+            const int wlast = min(norigin - 1, (oi | 63));
+            kfree = wlast >= 0 ? nal[amol[wlast]] + 1 : natom;
+            kfree = __builtin_amdgcn_readfirstlane(kfree);   // the same in every lane of the wave
+        }
+        if (active) {
+            // the pair loop, specialised on the launch-wide flags
+            if (potex) {
+                if (full) atogen_scan<true, true, false>(P, p0, p1e, kfree, lo, mlo, mhi, p1, rowoff, 0u, 0u, ldh, npt1, stepcl, inv_stepcl, dmax, snp, srm, hist, spot, ic0, ng);
+                else atogen_scan<true, false, false>(P, p0, p1e, kfree, lo, mlo, mhi, p1, rowoff, 0u, 0u, ldh, npt1, stepcl, inv_stepcl, dmax, snp, srm, hist, spot, ic0, ng);
+            } else if (pack) {
+                if (full) atogen_scan<false, true, true>(P, p0, p1e, kfree, lo, mlo, mhi, p1, rowoff, pk0, pk1, ldh, npt1, stepcl, inv_stepcl, dmax, snp, srm, hist, spot, ic0, ng);
+                else atogen_scan<false, false, true>(P, p0, p1e, kfree, lo, mlo, mhi, p1, rowoff, pk0, pk1, ldh, npt1, stepcl, inv_stepcl, dmax, snp, srm, hist, spot, ic0, ng);
+            } else {
+                if (full) atogen_scan<false, true, false>(P, p0, p1e, kfree, lo, mlo, mhi, p1, rowoff, 0u, 0u, ldh, npt1, stepcl, inv_stepcl, dmax, snp, srm, hist, spot, ic0, ng);
+                else atogen_scan<false, false, false>(P, p0, p1e, kfree, lo, mlo, mhi, p1, rowoff, 0u, 0u, ldh, npt1, stepcl, inv_stepcl, dmax, snp, srm, hist, spot, ic0, ng);
+            }
+        }
     }
-    const int first = blockIdx.x * ATOGEN_BLOCK;
-    const int blo = full ? nal[amol[oatoms ? oatoms[first] : first]] + 1 : 0;
-    const int span = natom - blo;
-    const int per = (span + gridDim.y - 1) / gridDim.y;
-    const int p0 = blo + blockIdx.y * per;
-    const int p1e = min(natom, p0 + per);
     __syncthreads();
-
-    // the pair loop, specialised on the launch-wide flag
-    if (potex)
-        atogen_tiles<true>(P, amol, p0, p1e, active, lo, mymol, p1, rowoff, ldh, npt1, stepcl, inv_stepcl, dmax, snp,
-                           srm, tileP, tileM, hist, spot);
-    else
-        atogen_tiles<false>(P, amol, p0, p1e, active, lo, mymol, p1, rowoff, ldh, npt1, stepcl, inv_stepcl, dmax,
-                            snp, srm, tileP, tileM, hist, spot);
-    __syncthreads();
-    for (int i = tid; i < npairs * ldh; i += ATOGEN_BLOCK) {
+    for (int i = tid; i < ng * ldh; i += ATOGEN_BLOCK) {
         uint32_t v = hist[i];
         if (v) {
-            int ic = i / ldh, iz = i - ic * ldh;
-            atomicAdd(&counts[((size_t)npairs + ic) * ldw + iz], v);
+            int g = i / ldh, iz = i - g * ldh;
+            atomicAdd(&counts[((size_t)npairs + ic0 + g) * ldw + iz], v);
         }
     }
     if (potex)
-        for (int i = tid; i < npairs; i += ATOGEN_BLOCK)
-            if (spot[i]) atomicAdd(&potex_counts[i], spot[i]);
+        for (int i = tid; i < ng; i += ATOGEN_BLOCK)
+            if (spot[i]) atomicAdd(&potex_counts[ic0 + i], spot[i]);
 }
 
 // normalisation (:3859-3897).  norm[ic] = cprod, norm[npairs+ic] = cprod2 (host
@@ -338,6 +416,7 @@ extern "C" int epsr_atogen(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
     size_t oF = take(sizeof(float) * npairs);
     size_t oR = take(sizeof(float) * npairs);
     size_t oMX = take(sizeof(unsigned));
+    size_t oQ = take(sizeof(unsigned) * npairs);
     char *scr = (char *)ctx_scratch(ctx, off);
     if (!scr) return EPSR_ERR_HIP;
     float4 *dP = (float4 *)(scr + oP);
@@ -346,6 +425,7 @@ extern "C" int epsr_atogen(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
     uint32_t *dPC = (uint32_t *)(scr + oPC);
     float *dN = (float *)(scr + oN), *dF = (float *)(scr + oF), *dR = (float *)(scr + oR);
     unsigned *dMX = (unsigned *)(scr + oMX);
+    unsigned *dQ = (unsigned *)(scr + oQ);
 
     // staging of the small host arrays through pinned memory
     const int nlist = (int)oatoms.size();
@@ -368,6 +448,7 @@ extern "C" int epsr_atogen(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
 
     HIP_CHECK(ctx, hipMemsetAsync(dC, 0, sizeof(uint32_t) * 2 * (size_t)npairs * ldw, st));
     HIP_CHECK(ctx, hipMemsetAsync(dPC, 0, sizeof(uint32_t) * npairs, st));
+    HIP_CHECK(ctx, hipMemsetAsync(dQ, 0, sizeof(unsigned) * npairs, st));
     const float stepcl = box->cellst / box->dmax;
 
     atogen_positions<<<ceil_div(natom, 256), 256, 0, st>>>(c->xa, c->ya, c->za, c->xadd, c->yadd, c->zadd,
@@ -381,20 +462,26 @@ extern "C" int epsr_atogen(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
             box->sizefactor, stepcl, npt1, ldw, use_lds, dC);
     }
     if (norigin > 0) {
-        size_t lds = sizeof(uint32_t) * ((size_t)npairs * (npt1 + 1) + npairs) + sizeof(int) * ntype * ntype +
-                     sizeof(float) * npairs + 16 + ATOGEN_TILE * (sizeof(float4) + sizeof(int));
-        if (lds > 160 * 1024)
-            return set_error(ctx, EPSR_ERR_CAPACITY,
-                             "epsr_atogen: histogram of %d pairs x %d bins does not fit in LDS", npairs, npt1);
-        int gx = ceil_div(norigin, ATOGEN_BLOCK);
-        // partner slices so that the launch has the box's target of workgroups (a few per CU)
-        int gy = std::max(1, std::min(256, ceil_div(box->atogen_workgroups, gx)));
-        gy = std::min(gy, std::max(1, natom / ATOGEN_TILE));
+        // pair columns per launch group: the histogram of a group and the tiles within a
+        // workgroup's share of LDS (at least two resident workgroups per CU)
+        const size_t fixed = sizeof(int) * ntype * ntype + 16;
+        const size_t percol = sizeof(uint32_t) * ((size_t)npt1 + 1) + sizeof(uint32_t) + sizeof(float);
+        const size_t budget = 64 * 1024;
+        ARG_CHECK(ctx, fixed + percol <= budget, "epsr_atogen: %d bins do not fit in LDS", npt1);
+        const int ngrp = (int)std::min<size_t>((size_t)npairs, (budget - fixed) / percol);
+        const int ngroups = ceil_div(npairs, ngrp);
+        const size_t lds = fixed + percol * ngrp + 16;
+        const int gx = ceil_div(norigin, ATOGEN_BLOCK);
+        // resident workgroups per group, and partner slices so that each walks several items
+        const int wres = std::max(1, std::min(box->atogen_workgroups / ngroups, 1 << 16));
+        int gy = std::max(1, std::min(256, ceil_div(8L * wres, gx)));
+        gy = std::min(gy, std::max(1, natom / ATOGEN_MIN_SLICE));
+        const int nwg = std::min(wres, gx * gy);
         const float inv_stepcl = 1.0f / stepcl;
         timer_begin(ctx, "atogen_inter");
-        hipLaunchKernelGGL(atogen_inter_fast, dim3(gx, gy), dim3(ATOGEN_BLOCK), lds, st, dP, box->d_amol, box->d_nal,
-                           nlist ? dO : nullptr, norigin, full ? 1 : 0, natom, box->d_npairref, ntype, npairs, npt1,
-                           ldw, stepcl, inv_stepcl, box->dmax, dR, dMX, dC, dPC);
+        hipLaunchKernelGGL(atogen_inter_fast, dim3(nwg, ngroups), dim3(ATOGEN_BLOCK), lds, st, dP, box->d_amol,
+                           box->d_naf, box->d_nal, nlist ? dO : nullptr, norigin, full ? 1 : 0, natom, box->d_npairref, ntype,
+                           npairs, npt1, ldw, stepcl, inv_stepcl, box->dmax, dR, dMX, dC, dPC, dQ, gx, gy, ngrp);
         timer_end(ctx, "atogen_inter");
     }
     int total = 2 * npairs * npt1;

@@ -73,8 +73,8 @@ int epsr_box_create(epsr_ctx *ctx, const epsr_box_desc_t *desc, epsr_box **out);
 
 /* Launch shapes of a box's kernels (no counterpart in the reference: the reference has
  * no launches), set by the autotuner per box size: EPSR_LAUNCH_ATOGEN_WORKGROUPS, the
- * workgroups atogen's pair kernel aims for (partner slices per origin block; default
- * 16384), and EPSR_LAUNCH_PAIR_BLOCK, the workgroup size of mc_pairs and fmolec1's pair
+ * resident workgroups of atogen's persistent pair kernel (each flushes one histogram;
+ * default 1024), and EPSR_LAUNCH_PAIR_BLOCK, the workgroup size of mc_pairs and fmolec1's pair
  * kernel (256, 512 or 1024; default 1024).  Results do not depend on them.
  * EPSR_LAUNCH_RESOLVE_PASSES caps the passes of molfit's one-workgroup resolution (default
  * nmol + 1, which always suffices); a cycle that ends with moves undecided returns

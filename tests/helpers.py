@@ -40,6 +40,10 @@ def system(name: str):
         return synth.config3()
     if name == "c4":
         return synth.config4()
+    if name.startswith("fine"):
+        # 0.004 A bins to 8 A: 21 pairs x 2001 bins, a histogram larger than one workgroup's LDS
+        parts = name.split("_")
+        return synth.tiny(seed=int(parts[1]), nwater=int(parts[2]), ndmso=int(parts[3]), L=16.0, cellst=0.004)
     if name.startswith("tiny"):
         parts = name.split("_")
         seed = int(parts[1]) if len(parts) > 1 else 11

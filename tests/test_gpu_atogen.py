@@ -40,11 +40,16 @@ def _gpu_atogen(ctx, name, ngrsamples, idum, rminex=None):
     ("c0", 150, -31),                 # 3200 atoms, 150 sampled origins
     ("epsr_test", 0, -7),             # the reference's EPSR test box read from its .ato
     ("epsr_test", 200, -3),
+    ("fine_7_30_6", 0, -19),          # 21 pairs x 2001 bins: the pair columns split into LDS-sized groups
+    ("fine_7_30_6", 12, -23),
 ])
-def test_atogen_bit_exact(ctx, name, ngr, idum):
-    print(f"atogen parity {name} ngr={ngr}", flush=True)
+@pytest.mark.parametrize("close", [True, False], ids=["close-contacts", "no-close-contacts"])
+def test_atogen_bit_exact(ctx, name, ngr, idum, close):
+    """With every rminex 2.2 the kernel counts close contacts; with none it takes the path
+    without them (and, at most 4 atom types, the packed histogram offsets)."""
+    print(f"atogen parity {name} ngr={ngr} close={close}", flush=True)
     s = H.system(name)
-    rminex = np.full(s.npairs, 2.2, F32)           # exercise the potex close-contact count
+    rminex = np.full(s.npairs, 2.2 if close else 0.0, F32)
     ref = H.oracle_atogen(name, ngrsamples=ngr, idum=idum, rminex=rminex)
     got = _gpu_atogen(ctx, name, ngr, idum, rminex=rminex)
     assert got["ngr"] == ref["ngr"]
