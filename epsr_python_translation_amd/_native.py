@@ -155,6 +155,8 @@ SIGNATURES = {
     "epsr_graph_destroy": (None, [vp]),
     "epsr_ran1_stream": (C.c_int, [i32p, C.c_int, fp]),
     "epsr_atogen": (C.c_int, [vp, vp, C.POINTER(Coords), C.c_int, C.c_int, vp, vp, vp, ip, i32p, vp, vp, vp, vp]),
+    "epsr_atogen_bin_check": (C.c_int, [vp, C.c_float, C.c_int, C.c_float, C.c_float, C.c_uint32, C.c_uint32,
+                                        vp]),
     "epsr_gensq": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp, C.c_int, vp, vp, C.c_int, C.c_int]),
     "epsr_gensq_noalloc": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, vp, vp, vp, C.c_int, vp, vp,
                                      C.c_int]),

@@ -14,7 +14,7 @@ import torch
 
 from . import _native
 
-ATOGEN_WORKGROUPS = (768, 1024, 1536, 2048)
+ATOGEN_WORKGROUPS = (512, 768, 1024, 2048)
 PAIR_BLOCKS = (256, 512, 1024)
 
 _CACHE: dict = {}

@@ -81,13 +81,7 @@ __global__ void atogen_intra(const float *__restrict__ xa, const float *__restri
 // intermolecular pairs (:3780-3837).
 // origin atoms: oatoms[0..norigin); for origin atom ia the partners are the atoms
 // ja >= lo (lo = nal(mol(ia))+1 when every molecule is an origin, else 0) whose
-// molecule differs from mol(ia).  blockIdx.x: block of 256 origin atoms;
-// blockIdx.y: slice of the partner range.
-//  * bin index: nint(fl(sqrtf(dsq)) / stepcl) is needed exactly.  The hardware square
-//    root (within 1 ulp) times the rounded reciprocal of stepcl is within 4.5 ulp of the
-//    correctly rounded quotient q; NINT(q) can only differ from NINT of the approximation
-//    when q lies within that distance of a half-integer, and those pairs (about 1e-6 * q
-//    of them) take the exact path: correctly rounded sqrtf and division.
+// molecule differs from mol(ia).
 //  * the minimum-image components enter only the squared distance (mic_sq).
 //  * potex (read on the device): close-contact counting only when some rminex is non-zero.
 //  * oatoms null: the origins are atoms 0..norigin-1 (every molecule but the last).
@@ -102,19 +96,112 @@ __global__ void atogen_rminex(int npairs, const int *__restrict__ irefpcof, cons
     if (v > 0.0f) atomicOr(flag, 1u);
 }
 
+// The reference's bin of a squared distance: NINT(fl(sqrtf(dsq)) / stepcl) (:3800-3806),
+// correctly rounded square root and division.
+__device__ __forceinline__ int atogen_exact_bin(float dsq, float stepcl) { return dev_nint(sqrtf(dsq) / stepcl); }
+
+// Both atogen_exact_bin(d) and the close-contact test fl(sqrtf(d) * dmax) < r are monotone
+// in d (correctly rounded operations are), so each is a threshold on the squared distance.
+//   thr[b], 1 <= b < npt1: the smallest float d whose bin is at least b (thr[0] = 0);
+//   pth[ic]: the smallest float d with fl(sqrtf(d) * dmax) >= rabs[ic] -- a pair of column
+//            ic is a close contact exactly when dsq < pth[ic] (0 when rabs is 0).
+// Bisection over the bit patterns of non-negative floats, which are ordered like their values.
+__global__ void atogen_thresholds(float stepcl, int npt1, float *__restrict__ thr, int npairs,
+                                  const float *__restrict__ rabs, float dmax, float *__restrict__ pth)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < npt1) {
+        if (i == 0) {
+            thr[0] = 0.0f;
+            return;
+        }
+        // the bin of 0 is 0 < i; that of (2 (i + 2) stepcl)^2 is about 2 i + 4 >= i
+        float hv = 2.0f * (float)(i + 2) * stepcl;
+        hv = hv * hv;
+        uint32_t lo = 0u, hi = __float_as_uint(hv);
+        while (hi - lo > 1u) {
+            const uint32_t mid = lo + (hi - lo) / 2u;
+            if (atogen_exact_bin(__uint_as_float(mid), stepcl) >= i) hi = mid;
+            else lo = mid;
+        }
+        thr[i] = __uint_as_float(hi);
+        return;
+    }
+    const int ic = i - npt1;
+    if (ic >= npairs) return;
+    const float r = rabs[ic];
+    if (!(r > 0.0f)) {
+        pth[ic] = 0.0f;
+        return;
+    }
+    uint32_t lo = 0u, hi = 0x7f800000u;   // f(0) = 0 < r <= f(+inf) = +inf
+    while (hi - lo > 1u) {
+        const uint32_t mid = lo + (hi - lo) / 2u;
+        if (sqrtf(__uint_as_float(mid)) * dmax >= r) hi = mid;
+        else lo = mid;
+    }
+    pth[ic] = __uint_as_float(hi);
+}
+
+// The bin atogen_exact_bin(dsq, stepcl) from the hardware square root.  qa = v_sqrt(dsq) x
+// fl(1/stepcl) lies within 8 units of roundoff (relative; square root within 2 ulp, two
+// roundings) of the correctly rounded quotient q, itself two roundings off the exact one.
+// Away from half-integers -- |qa - rint(qa)| < bnd = 0.5 - 1.2e-6 (npt1 + 2), a margin of
+// more than 20 units of roundoff of q for every q up to npt1 + 2 -- NINT(q) = rint(qa).  Near
+// the half-integer b - 1/2 the bin is b - 1 or b, and it is b exactly when dsq >= thr[b].
+// A bin past lastb = npt1 - 1 is never counted (the counted bins end at npt1 - 2), so there
+// b needs no threshold; for q past npt1 + 2 both paths give bins past lastb as well.
+__device__ __forceinline__ int atogen_bin(float dsq, float inv_stepcl, float bnd, int lastb, const float *thr)
+{
+    const float qa = __builtin_amdgcn_sqrtf(dsq) * inv_stepcl;
+    const float rq = rintf(qa);
+    const float dq = qa - rq;
+    int n = (int)rq;
+    if (!(fabsf(dq) < bnd)) {
+        const int b = n + (dq > 0.0f ? 1 : 0);   // the nearest half-integer is b - 1/2, b >= 1
+        n = b <= lastb ? b - (dsq < thr[b] ? 1 : 0) : b;
+    }
+    return n;
+}
+
+__device__ __forceinline__ float atogen_bnd(int npt1) { return 0.5f - 1.2e-6f * (float)(npt1 + 2); }
+
+// Every float squared distance lo + i (i < count, as bit patterns) through atogen_bin and
+// the close-contact threshold against the correctly rounded computation.  out[0]: the
+// disagreements (a counted bin that differs, counted on one side only, or a close-contact
+// decision that differs); out[1]: the values checked; out[2]: those that took the
+// threshold branch of atogen_bin (near a bin edge).
+__global__ void atogen_bin_sweep(float stepcl, float inv_stepcl, int npt1, const float *__restrict__ thr,
+                                 float dmax, float rabs, const float *__restrict__ pth, uint32_t lo, uint32_t count,
+                                 unsigned long long *__restrict__ out)
+{
+    const float bnd = atogen_bnd(npt1);
+    const float p = pth[0];
+    unsigned long long nbad = 0, nseen = 0, nedge = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        const float dsq = __uint_as_float(lo + (uint32_t)i);
+        const int n = atogen_bin(dsq, inv_stepcl, bnd, npt1 - 1, thr);
+        const int e = atogen_exact_bin(dsq, stepcl);
+        const bool in = n <= npt1 - 2, ine = e <= npt1 - 2;
+        if (in != ine || (in && n != e)) ++nbad;
+        if ((dsq < p) != (sqrtf(dsq) * dmax < rabs)) ++nbad;
+        const float qa = __builtin_amdgcn_sqrtf(dsq) * inv_stepcl;
+        if (!(fabsf(qa - rintf(qa)) < bnd)) ++nedge;
+        ++nseen;
+    }
+    if (nbad) atomicAdd(&out[0], nbad);
+    atomicAdd(&out[1], nseen);
+    if (nedge) atomicAdd(&out[2], nedge);
+}
+
 // partner atoms p0..p1e-1 against this lane's origin atom (p1), into the workgroup's
 // histograms.  Every lane of a wave takes the same partner at the same step, so the
 // partners' positions are wave-uniform (scalar) loads, four at a time.  A lane counts the
 // pair when the partner is outside its own molecule [mlo, mhi] (and, with FULL -- every
 // molecule an origin, partners after the origin's molecule -- at or past lo = mhi + 1);
 // with FULL, from the wave's largest lo (kfree) on every lane counts every partner.
-//  * bin: NINT(fl(sqrtf(dsq)) / stepcl) exactly.  qa = hardware sqrt x rounded 1/stepcl
-//    lies within 3 ulp of the correctly rounded quotient q, so when qa's distance from a
-//    half-integer exceeds 1.2e-6 (npt1 + 2) -- fixed for the launch, and more than 10 ulp
-//    of q wherever the bin can still be counted (rint(qa) <= npt1 - 2 needs
-//    qa < npt1 - 1.5) -- NINT(q) = rint(qa); otherwise the correctly rounded sqrtf and
-//    division decide.  Beyond npt1 - 2 on the first path no pair is counted, and none is
-//    on the second either (q > npt1 - 1.5 - 10 ulp: NINT(q) + 2 > npt1).
+//  * bin: atogen_bin, exact; close contacts: dsq < pth[g], exact (atogen_thresholds).
 //  * PACK (one group of columns, at most 4 atom types, no close-contact counting): the
 //    byte offset in LDS of bin 2 of each partner type's column for this lane's origin
 //    type sits in two registers, 16 bits each (pk0: types 0, 1; pk1: types 2, 3); otherwise the pair
@@ -122,39 +209,26 @@ __global__ void atogen_rminex(int npairs, const int *__restrict__ irefpcof, cons
 template <bool POTEX, bool FULL, bool PACK>
 __device__ __forceinline__ void atogen_scan(const float4 *__restrict__ P, int p0, int p1e, int kfree, int lo,
                                             int mlo, int mhi, float4 p1, int rowoff, uint32_t pk0, uint32_t pk1,
-                                            int ldh, int npt1, float stepcl, float inv_stepcl, float dmax,
-                                            const int *snp, const float *srm, uint32_t *hist, uint32_t *spot, int ic0,
+                                            int ldh, int npt1, float inv_stepcl, const float *thr,
+                                            const int *snp, const float *pth, uint32_t *hist, uint32_t *spot, int ic0,
                                             int ngrp)
 {
-    const float bnd = 0.5f - 1.2e-6f * (float)(npt1 + 2);
-    const float lim = (float)(npt1 - 2);
+    const float bnd = atogen_bnd(npt1);
     auto pair = [&](const float4 &p2) {
         const float xx = mic_sq(p1.x - p2.x), yy = mic_sq(p1.y - p2.y), zz = mic_sq(p1.z - p2.z);
         const float dsq = xx * xx + yy * yy + zz * zz;
-        const float qa = __builtin_amdgcn_sqrtf(dsq) * inv_stepcl;
-        const float rq = rintf(qa);
-        int iz1;
-        bool in;
-        float dtest = 0.0f;
-        if (fabsf(qa - rq) < bnd) {
-            in = rq <= lim;
-            iz1 = (int)rq + 2;
-            if (POTEX) dtest = sqrtf(dsq);
-        } else {
-            dtest = sqrtf(dsq);
-            iz1 = dev_nint(dtest / stepcl) + 2;
-            in = iz1 > 1 && iz1 <= npt1;
-        }
+        const int n = atogen_bin(dsq, inv_stepcl, bnd, npt1 - 1, thr);
+        const bool in = n <= npt1 - 2;   // iz1 = n + 2 in 2..npt1
         const int t2 = __float_as_int(p2.w);
         if (PACK) {
             const uint32_t w = t2 < 2 ? pk0 : pk1;
-            const uint32_t base = (w >> ((t2 & 1) * 16)) & 0xffffu;   // byte offset of bin 2 - 2
-            if (in) atomicAdd((uint32_t *)((char *)hist + base + 4 * (iz1 - 2)), 1u);
+            const uint32_t base = (w >> ((t2 & 1) * 16)) & 0xffffu;   // byte offset of bin 2
+            if (in) atomicAdd((uint32_t *)((char *)hist + base + 4 * n), 1u);
         } else {
             const int g = snp[rowoff + t2] - ic0;   // column within this launch's group
             if ((unsigned)g < (unsigned)ngrp) {
-                if (in) atomicAdd(&hist[g * ldh + iz1], 1u);
-                if (POTEX && dtest * dmax < srm[g]) atomicAdd(&spot[g], 1u);
+                if (in) atomicAdd(&hist[g * ldh + n + 2], 1u);
+                if (POTEX && dsq < pth[g]) atomicAdd(&spot[g], 1u);
             }
         }
     };
@@ -194,8 +268,8 @@ __global__ void __launch_bounds__(ATOGEN_BLOCK)
 atogen_inter_fast(const float4 *__restrict__ P, const int *__restrict__ amol, const int *__restrict__ naf,
                   const int *__restrict__ nal,
                   const int *__restrict__ oatoms, int norigin, int full, int natom,
-                  const int *__restrict__ npairref_g, int ntype, int npairs, int npt1, int ldw, float stepcl,
-                  float inv_stepcl, float dmax, const float *__restrict__ rminex_abs,
+                  const int *__restrict__ npairref_g, int ntype, int npairs, int npt1, int ldw, float inv_stepcl,
+                  const float *__restrict__ thr_g, const float *__restrict__ pth_g,
                   const unsigned *__restrict__ potex_flag, uint32_t *__restrict__ counts,
                   uint32_t *__restrict__ potex_counts, unsigned *__restrict__ queue, int gx, int gy, int ngrp)
 {
@@ -207,15 +281,17 @@ atogen_inter_fast(const float4 *__restrict__ P, const int *__restrict__ amol, co
     uint32_t *hist = smem;
     uint32_t *spot = hist + (size_t)ngrp * ldh;
     int *snp = (int *)(spot + ngrp);
-    float *srm = (float *)(snp + ntype * ntype);
+    float *pth = (float *)(snp + ntype * ntype);
+    float *thr = pth + ngrp;
 
     const int tid = threadIdx.x;
     for (int i = tid; i < ng * ldh; i += ATOGEN_BLOCK) hist[i] = 0u;
     for (int i = tid; i < ng; i += ATOGEN_BLOCK) {
         spot[i] = 0u;
-        srm[i] = rminex_abs[ic0 + i];
+        pth[i] = pth_g[ic0 + i];
     }
     for (int i = tid; i < ntype * ntype; i += ATOGEN_BLOCK) snp[i] = npairref_g[i];
+    for (int i = tid; i < npt1; i += ATOGEN_BLOCK) thr[i] = thr_g[i];
     __syncthreads();
 
     // packed offsets: one group of columns, at most 4 types, byte offsets below 2^16
@@ -268,14 +344,14 @@ atogen_inter_fast(const float4 *__restrict__ P, const int *__restrict__ amol, co
         if (active) {
             // the pair loop, specialised on the launch-wide flags
             if (potex) {
-                if (full) atogen_scan<true, true, false>(P, p0, p1e, kfree, lo, mlo, mhi, p1, rowoff, 0u, 0u, ldh, npt1, stepcl, inv_stepcl, dmax, snp, srm, hist, spot, ic0, ng);
-                else atogen_scan<true, false, false>(P, p0, p1e, kfree, lo, mlo, mhi, p1, rowoff, 0u, 0u, ldh, npt1, stepcl, inv_stepcl, dmax, snp, srm, hist, spot, ic0, ng);
+                if (full) atogen_scan<true, true, false>(P, p0, p1e, kfree, lo, mlo, mhi, p1, rowoff, 0u, 0u, ldh, npt1, inv_stepcl, thr, snp, pth, hist, spot, ic0, ng);
+                else atogen_scan<true, false, false>(P, p0, p1e, kfree, lo, mlo, mhi, p1, rowoff, 0u, 0u, ldh, npt1, inv_stepcl, thr, snp, pth, hist, spot, ic0, ng);
             } else if (pack) {
-                if (full) atogen_scan<false, true, true>(P, p0, p1e, kfree, lo, mlo, mhi, p1, rowoff, pk0, pk1, ldh, npt1, stepcl, inv_stepcl, dmax, snp, srm, hist, spot, ic0, ng);
-                else atogen_scan<false, false, true>(P, p0, p1e, kfree, lo, mlo, mhi, p1, rowoff, pk0, pk1, ldh, npt1, stepcl, inv_stepcl, dmax, snp, srm, hist, spot, ic0, ng);
+                if (full) atogen_scan<false, true, true>(P, p0, p1e, kfree, lo, mlo, mhi, p1, rowoff, pk0, pk1, ldh, npt1, inv_stepcl, thr, snp, pth, hist, spot, ic0, ng);
+                else atogen_scan<false, false, true>(P, p0, p1e, kfree, lo, mlo, mhi, p1, rowoff, pk0, pk1, ldh, npt1, inv_stepcl, thr, snp, pth, hist, spot, ic0, ng);
             } else {
-                if (full) atogen_scan<false, true, false>(P, p0, p1e, kfree, lo, mlo, mhi, p1, rowoff, 0u, 0u, ldh, npt1, stepcl, inv_stepcl, dmax, snp, srm, hist, spot, ic0, ng);
-                else atogen_scan<false, false, false>(P, p0, p1e, kfree, lo, mlo, mhi, p1, rowoff, 0u, 0u, ldh, npt1, stepcl, inv_stepcl, dmax, snp, srm, hist, spot, ic0, ng);
+                if (full) atogen_scan<false, true, false>(P, p0, p1e, kfree, lo, mlo, mhi, p1, rowoff, 0u, 0u, ldh, npt1, inv_stepcl, thr, snp, pth, hist, spot, ic0, ng);
+                else atogen_scan<false, false, false>(P, p0, p1e, kfree, lo, mlo, mhi, p1, rowoff, 0u, 0u, ldh, npt1, inv_stepcl, thr, snp, pth, hist, spot, ic0, ng);
             }
         }
     }
@@ -415,6 +491,8 @@ extern "C" int epsr_atogen(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
     size_t oN = take(sizeof(float) * 2 * npairs);
     size_t oF = take(sizeof(float) * npairs);
     size_t oR = take(sizeof(float) * npairs);
+    size_t oPT = take(sizeof(float) * npairs);
+    size_t oT = take(sizeof(float) * npt1);
     size_t oMX = take(sizeof(unsigned));
     size_t oQ = take(sizeof(unsigned) * npairs);
     char *scr = (char *)ctx_scratch(ctx, off);
@@ -424,6 +502,7 @@ extern "C" int epsr_atogen(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
     uint32_t *dC = counts_out ? counts_out : (uint32_t *)(scr + oC);
     uint32_t *dPC = (uint32_t *)(scr + oPC);
     float *dN = (float *)(scr + oN), *dF = (float *)(scr + oF), *dR = (float *)(scr + oR);
+    float *dPT = (float *)(scr + oPT), *dT = (float *)(scr + oT);
     unsigned *dMX = (unsigned *)(scr + oMX);
     unsigned *dQ = (unsigned *)(scr + oQ);
 
@@ -464,7 +543,7 @@ extern "C" int epsr_atogen(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
     if (norigin > 0) {
         // pair columns per launch group: the histogram of a group and the tiles within a
         // workgroup's share of LDS (at least two resident workgroups per CU)
-        const size_t fixed = sizeof(int) * ntype * ntype + 16;
+        const size_t fixed = sizeof(int) * ntype * ntype + sizeof(float) * npt1 + 16;
         const size_t percol = sizeof(uint32_t) * ((size_t)npt1 + 1) + sizeof(uint32_t) + sizeof(float);
         const size_t budget = 64 * 1024;
         ARG_CHECK(ctx, fixed + percol <= budget, "epsr_atogen: %d bins do not fit in LDS", npt1);
@@ -478,10 +557,11 @@ extern "C" int epsr_atogen(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
         gy = std::min(gy, std::max(1, natom / ATOGEN_MIN_SLICE));
         const int nwg = std::min(wres, gx * gy);
         const float inv_stepcl = 1.0f / stepcl;
+        atogen_thresholds<<<ceil_div(npt1 + npairs, 64), 64, 0, st>>>(stepcl, npt1, dT, npairs, dR, box->dmax, dPT);
         timer_begin(ctx, "atogen_inter");
         hipLaunchKernelGGL(atogen_inter_fast, dim3(nwg, ngroups), dim3(ATOGEN_BLOCK), lds, st, dP, box->d_amol,
                            box->d_naf, box->d_nal, nlist ? dO : nullptr, norigin, full ? 1 : 0, natom, box->d_npairref, ntype,
-                           npairs, npt1, ldw, stepcl, inv_stepcl, box->dmax, dR, dMX, dC, dPC, dQ, gx, gy, ngrp);
+                           npairs, npt1, ldw, inv_stepcl, dT, dPT, dMX, dC, dPC, dQ, gx, gy, ngrp);
         timer_end(ctx, "atogen_inter");
     }
     int total = 2 * npairs * npt1;
@@ -489,5 +569,36 @@ extern "C" int epsr_atogen(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
                                                                               addwav, wav, potex, potexmax);
     // no synchronisation: the outputs are device arrays, ordered on the stream
     HIP_CHECK(ctx, hipGetLastError());
+    return EPSR_OK;
+}
+
+extern "C" int epsr_atogen_bin_check(epsr_ctx *ctx, float stepcl, int npt1, float dmax, float rabs, uint32_t lo_bits,
+                                     uint32_t hi_bits, unsigned long long *counts)
+{
+    if (!ctx) return EPSR_ERR_ARG;
+    ARG_CHECK(ctx, counts, "epsr_atogen_bin_check: null argument");
+    ARG_CHECK(ctx, npt1 >= 2 && stepcl > 0.0f && dmax > 0.0f && rabs >= 0.0f,
+              "epsr_atogen_bin_check: need npt1 >= 2, stepcl > 0, dmax > 0, rabs >= 0");
+    ARG_CHECK(ctx, lo_bits <= hi_bits && hi_bits <= 0x7f800000u,
+              "epsr_atogen_bin_check: need lo_bits <= hi_bits <= the bits of +inf");
+    HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
+    const size_t oT = take(sizeof(float) * npt1), oR = take(sizeof(float)), oP = take(sizeof(float)),
+                 oB = take(3 * sizeof(unsigned long long));
+    char *scr = (char *)ctx_scratch(ctx, off);
+    if (!scr) return EPSR_ERR_HIP;
+    float *dT = (float *)(scr + oT), *dR = (float *)(scr + oR), *dP = (float *)(scr + oP);
+    unsigned long long *dB = (unsigned long long *)(scr + oB);
+    HIP_CHECK(ctx, hipMemcpyAsync(dR, &rabs, sizeof(float), hipMemcpyHostToDevice, st));
+    HIP_CHECK(ctx, hipMemsetAsync(dB, 0, 3 * sizeof(unsigned long long), st));
+    atogen_thresholds<<<ceil_div(npt1 + 1, 64), 64, 0, st>>>(stepcl, npt1, dT, 1, dR, dmax, dP);
+    const uint32_t count = hi_bits - lo_bits;
+    if (count > 0)
+        atogen_bin_sweep<<<4096, 256, 0, st>>>(stepcl, 1.0f / stepcl, npt1, dT, dmax, rabs, dP, lo_bits, count, dB);
+    HIP_CHECK(ctx, hipGetLastError());
+    HIP_CHECK(ctx, hipMemcpyAsync(counts, dB, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    HIP_CHECK(ctx, hipStreamSynchronize(st));
     return EPSR_OK;
 }

@@ -71,7 +71,7 @@ struct epsr_box {
     int *d_npairref = nullptr, *d_amol = nullptr;
     // launch shapes (epsr_box_set_launch): workgroups atogen's pair kernel aims for, and
     // the workgroup size of the pair-energy kernels
-    int atogen_workgroups = 1024, pair_block = 1024;
+    int atogen_workgroups = 768, pair_block = 1024;
     // the cap on the one-workgroup resolver's passes (-1: nmol + 1); a smaller cap is a test hook
     int resolve_passes = -1;
 };

@@ -202,6 +202,16 @@ def atogen(ctx: Context, box: DeviceBox, npt1: int, ldw: int, ycoh, irefpcof, rm
     return wav, potex, potexmax, ngr.value
 
 
+def atogen_bin_check(ctx: Context, stepcl, npt1, dmax, rabs, lo_bits, hi_bits):
+    """(disagreements, values checked, values near a bin edge) of atogen's pair-kernel binning
+    and close-contact test against the correctly rounded computation, over every float32
+    squared distance with bit pattern in [lo_bits, hi_bits)."""
+    out = (C.c_ulonglong * 3)()
+    ctx.check(ctx.lib.epsr_atogen_bin_check(ctx.h, F32(stepcl), int(npt1), F32(dmax), F32(rabs), int(lo_bits),
+                                            int(hi_bits), out))
+    return int(out[0]), int(out[1]), int(out[2])
+
+
 def gensq(ctx: Context, nq, npairs, npt1, ldw, wav, ycoh, foqols, ldf, sofq, ldq, natomsphere=None, mode=0):
     ctx.check(ctx.lib.epsr_gensq(ctx.h, int(nq), int(npairs), int(npt1), int(ldw), _ptr(wav), _ptr(ycoh),
                                  _ptr(foqols), int(ldf), _ptr(natomsphere), _ptr(sofq), int(ldq), int(mode)))

@@ -74,7 +74,7 @@ int epsr_box_create(epsr_ctx *ctx, const epsr_box_desc_t *desc, epsr_box **out);
 /* Launch shapes of a box's kernels (no counterpart in the reference: the reference has
  * no launches), set by the autotuner per box size: EPSR_LAUNCH_ATOGEN_WORKGROUPS, the
  * resident workgroups of atogen's persistent pair kernel (each flushes one histogram;
- * default 1024), and EPSR_LAUNCH_PAIR_BLOCK, the workgroup size of mc_pairs and fmolec1's pair
+ * default 768), and EPSR_LAUNCH_PAIR_BLOCK, the workgroup size of mc_pairs and fmolec1's pair
  * kernel (256, 512 or 1024; default 1024).  Results do not depend on them.
  * EPSR_LAUNCH_RESOLVE_PASSES caps the passes of molfit's one-workgroup resolution (default
  * nmol + 1, which always suffices); a cycle that ends with moves undecided returns
@@ -115,6 +115,16 @@ int epsr_atogen(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *coords,
                 int ldw, const float *ycoh, const int *irefpcof, const float *rminex,
                 int *ngrsamples, int32_t *ran1_state, float *wav, float *potex, float *potexmax,
                 uint32_t *counts);
+
+/* ---- atogen's binning, checked.  No counterpart in the reference: a check, for callers
+ * and tests, that the pair kernel's bin and close-contact decisions (hardware square root
+ * with exact per-launch thresholds near bin edges) equal the reference's correctly rounded
+ * NINT(sqrtf(dsq) / stepcl) and sqrtf(dsq) * dmax < rabs for every float32 squared distance
+ * whose bit pattern lies in [lo_bits, hi_bits) (hi_bits at most that of +inf).
+ * counts (host [3]) receives the disagreements, the values checked and those near a bin
+ * edge (the threshold branch).  Synchronises the stream. */
+int epsr_atogen_bin_check(epsr_ctx *ctx, float stepcl, int npt1, float dmax, float rabs,
+                          uint32_t lo_bits, uint32_t hi_bits, unsigned long long *counts);
 
 /* ---- gensq: replaces epsr_standalone_rev3.gensq(nq, q, sofq, wav)
  * (src/epsr26/programs/epsr_standalone_rev3.py:4730-4802; Fortran :4737).

@@ -97,3 +97,20 @@ def test_atogen_large_box_full_mode_invariants(ctx):
     wav = got["wav"].reshape(2, s.npairs, -1)
     tail = wav[1, :, g.npt1 - 200:g.npt1 + 1] / g.ycoh[None, g.npt1 - 200:g.npt1 + 1]
     assert np.all(np.abs(tail.mean(axis=1) - 1.0) < 0.02)
+
+
+@pytest.mark.parametrize("name", ["c1", "c0", "epsr_test", "fine_7_30_6"])
+def test_atogen_binning_exhaustive(ctx, name):
+    """Every float32 squared distance from 0 past the last counted bin (about 1e9 values for
+    the 60,000-atom box): the pair kernel's bin (hardware square root, exact thresholds near
+    bin edges) and close-contact test equal the correctly rounded NINT(sqrtf(dsq) / stepcl)
+    and sqrtf(dsq) * dmax < 2.2."""
+    from epsr_python_translation_amd import device as D
+    s, g = H.system(name), H.grid(name)
+    stepcl = F32(F32(s.cellst) / F32(s.dmax))
+    top = F32(F32(g.npt1 + 3) * stepcl)
+    hi = int(np.array(F32(top * top * F32(1.01)), F32).view(np.uint32))
+    print(f"atogen binning sweep {name}: stepcl={stepcl} npt1={g.npt1} floats={hi}", flush=True)
+    bad, seen, edge = D.atogen_bin_check(ctx, stepcl, g.npt1, F32(s.dmax), F32(2.2), 0, hi)
+    print(f"  {seen} values, {edge} near a bin edge, {bad} disagreements", flush=True)
+    assert seen == hi and edge >= 100 * g.npt1 and bad == 0
