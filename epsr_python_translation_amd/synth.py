@@ -293,9 +293,10 @@ def build_system(name, seed, L, molecules, type_names, *, cellst, rmaxgr, qstep,
             naf.append(atom)
             nal.append(atom + len(names) - 1)
             mtype.append(mt)
-            # intramolecular energy (en_mole_intra is outside this package): multi-atom
-            # molecules carry a fixed non-zero value so their self ledger slot is used,
-            # single atoms carry 0 as EPSR computes for them
+            # intramolecular energy: these rigid templates carry no bonds, angle constraints
+            # or dihedrals, so the workload fixes it -- a non-zero value for multi-atom
+            # molecules, so their self ledger slot is used as for molecules with an
+            # intramolecular energy; single atoms carry 0, as EPSR computes for them
             eintra.append(0.001 * len(names) if len(names) > 1 else 0.0)
             atom += len(names)
             im += 1

@@ -163,6 +163,35 @@ static int atogen_select(const ora_box_t *b, int *ngrsamples, ora_ran1_t *rng,
     return 0;
 }
 
+/* inter loop (:3780-3837) of origin molecule im: its atoms against every atom of the
+ * molecules after it (full) or of every other molecule; cnt: the [npairs][ldw] inter
+ * histograms */
+static void atogen_origin(const ora_box_t *b, const ora_centres_t *c, int im, int full, float stepcl, int npt1,
+                          int ldw, const int *irefpcof, const float *rminex, int64_t *cnt, int64_t *potexcnt)
+{
+    float xc1 = c->xadd[im], yc1 = c->yadd[im], zc1 = c->zadd[im];
+    int jm1 = full ? im + 1 : 0;
+    for (int ia = b->naf[im]; ia <= b->nal[im]; ++ia) {
+        float x1 = b->xa[ia] + xc1, y1 = b->ya[ia] + yc1, z1 = b->za[ia] + zc1;
+        int it = b->itype[ia];
+        for (int jm = jm1; jm < b->nmol; ++jm) {
+            if (jm == im) continue;
+            float xc2 = c->xadd[jm], yc2 = c->yadd[jm], zc2 = c->zadd[jm];
+            for (int ja = b->naf[jm]; ja <= b->nal[jm]; ++ja) {
+                int icref = b->npairref[it * b->ntype + b->itype[ja]];
+                float x2 = b->xa[ja] + xc2, y2 = b->ya[ja] + yc2, z2 = b->za[ja] + zc2;
+                float xx = x1 - x2, yy = y1 - y2, zz = z1 - z2;
+                ora_minimage(&xx, &yy, &zz);
+                float dtest = ora_sqrtf(xx * xx + yy * yy + zz * zz);
+                int iz1 = ora_nint(dtest / stepcl) + 2;
+                if (iz1 > 1 && iz1 <= npt1) cnt[(size_t)icref * ldw + iz1] += 1;
+                float dtesttest = dtest * b->dmax;
+                if (dtesttest < 1.0f * fabsf(rminex[irefpcof[icref]])) potexcnt[icref] += 1;
+            }
+        }
+    }
+}
+
 /* intra (:3698-3724) and inter (:3780-3837) pair loops, integer counts */
 static void atogen_loops(const ora_box_t *b, const ora_centres_t *c, int npt1, int ldw,
                          const int *irefpcof, const float *rminex, const int *origin, int full,
@@ -188,30 +217,29 @@ static void atogen_loops(const ora_box_t *b, const ora_centres_t *c, int npt1, i
             }
         }
     }
-    for (int im = 0; im < b->nmol; ++im) {
-        if (!origin[im]) continue;
-        float xc1 = c->xadd[im], yc1 = c->yadd[im], zc1 = c->zadd[im];
-        int jm1 = full ? im + 1 : 0;
-        for (int ia = b->naf[im]; ia <= b->nal[im]; ++ia) {
-            float x1 = b->xa[ia] + xc1, y1 = b->ya[ia] + yc1, z1 = b->za[ia] + zc1;
-            int it = b->itype[ia];
-            for (int jm = jm1; jm < b->nmol; ++jm) {
-                if (jm == im) continue;
-                float xc2 = c->xadd[jm], yc2 = c->yadd[jm], zc2 = c->zadd[jm];
-                for (int ja = b->naf[jm]; ja <= b->nal[jm]; ++ja) {
-                    int icref = b->npairref[it * b->ntype + b->itype[ja]];
-                    float x2 = b->xa[ja] + xc2, y2 = b->ya[ja] + yc2, z2 = b->za[ja] + zc2;
-                    float xx = x1 - x2, yy = y1 - y2, zz = z1 - z2;
-                    ora_minimage(&xx, &yy, &zz);
-                    float dtest = ora_sqrtf(xx * xx + yy * yy + zz * zz);
-                    int iz1 = ora_nint(dtest / stepcl) + 2;
-                    if (iz1 > 1 && iz1 <= npt1) cnt[((size_t)np + icref) * ldw + iz1] += 1;
-                    float dtesttest = dtest * b->dmax;
-                    if (dtesttest < 1.0f * fabsf(rminex[irefpcof[icref]])) potexcnt[icref] += 1;
-                }
-            }
+#ifdef ORACLE_OMP
+    /* the origin molecules across threads, each counting into its own integer
+     * histograms; integer sums do not depend on the order */
+#pragma omp parallel
+    {
+        int64_t *lc = calloc((size_t)np * ldw, sizeof(int64_t));
+        int64_t *lp = calloc((size_t)np, sizeof(int64_t));
+#pragma omp for schedule(dynamic, 8)
+        for (int im = 0; im < b->nmol; ++im)
+            if (origin[im]) atogen_origin(b, c, im, full, stepcl, npt1, ldw, irefpcof, rminex, lc, lp);
+#pragma omp critical
+        {
+            for (size_t k = 0; k < (size_t)np * ldw; ++k) cnt[(size_t)np * ldw + k] += lc[k];
+            for (int k = 0; k < np; ++k) potexcnt[k] += lp[k];
         }
+        free(lc);
+        free(lp);
     }
+#else
+    for (int im = 0; im < b->nmol; ++im)
+        if (origin[im]) atogen_origin(b, c, im, full, stepcl, npt1, ldw, irefpcof, rminex, cnt + (size_t)np * ldw,
+                                      potexcnt);
+#endif
 }
 
 /* The reference accumulates +addwav in float32.  Equal power-of-two increments

@@ -99,6 +99,24 @@ def test_atogen_large_box_full_mode_invariants(ctx):
     assert np.all(np.abs(tail.mean(axis=1) - 1.0) < 0.02)
 
 
+def test_atogen_large_box_full_mode_bit_exact(ctx):
+    """Every pair of the 60,000-atom box (1.8e9 distances, every molecule an origin), with
+    close contacts counted: the histograms, potex and the ran1 state against the all-cores
+    oracle (the sequential oracle's counts, origin molecules spread over threads)."""
+    print("atogen parity c1 full (oracle on all cores)", flush=True)
+    s, g = H.system("c1"), H.grid("c1")
+    rminex = np.full(s.npairs, 2.2, F32)
+    ref = H.oracle_atogen("c1", ngrsamples=s.nmol, idum=-77, rminex=rminex, flavour="omp")
+    got = _gpu_atogen(ctx, "c1", s.nmol, -77, rminex=rminex)
+    assert got["ngr"] == ref["ngr"] == s.nmol
+    np.testing.assert_array_equal(got["ran1"], ref["ran1"])
+    sl = np.s_[:, :, 1:g.npt1 + 1]
+    assert np.array_equal(ref["wav"].reshape(2, s.npairs, -1)[sl].view(np.int32),
+                          got["wav"].reshape(2, s.npairs, -1)[sl].view(np.int32))
+    assert np.array_equal(ref["potex"].view(np.int32), got["potex"].view(np.int32))
+    assert got["potex"].sum() > 0
+
+
 @pytest.mark.parametrize("name", ["c1", "c0", "epsr_test", "fine_7_30_6"])
 def test_atogen_binning_exhaustive(ctx, name):
     """Every float32 squared distance from 0 past the last counted bin (about 1e9 values for

@@ -135,3 +135,81 @@ def test_ensemble_boxes_match_boxes_alone():
         assert np.array_equal(R.sofq.cpu().numpy().view(np.int32), ref.sofq.cpu().numpy().view(np.int32))
         assert np.array_equal(R.gofr.cpu().numpy().view(np.int32), ref.gofr.cpu().numpy().view(np.int32))
 
+
+
+def _large_boxes(n=3):
+    from epsr_python_translation_amd import synth
+    boxes = synth.config2_boxes(nbox=64)
+    boxes.sort(key=lambda s: s.natom)
+    return boxes[-n:]
+
+
+def test_ensemble_largest_boxes_against_oracle():
+    """The three largest boxes of the 64-box draw (about 6,000 molecules each), run together
+    on three streams for one step -- ledger, one MC sweep, atogen with 1000 origin
+    molecules -- each against the oracle from the same start and on the GPU's own tables:
+    the ledger, coordinates, ran1 state, step sizes, counters and histograms, bit for bit."""
+    import ctypes as C
+    from epsr_python_translation_amd import ensemble as E
+    from epsr_python_translation_amd import tables as T
+    from oracle import pyoracle as po
+    systems = _large_boxes(3)
+    print("ensemble largest boxes", [s.natom for s in systems], flush=True)
+    ens = E.Ensemble(systems, nworkers=3, idum0=-400, tune=False)
+    torch.cuda.synchronize()
+    before = []
+    for R in ens.boxes:
+        tabs = {k: getattr(R, k).cpu().numpy().copy() for k in ("xstar", "xavs", "xavs1", "ecoul", "ecoul1")}
+        before.append(dict(ergpair=R.ledger.export()["ergpair"].copy(), tabs=tabs, wr=R.mc.wholerot.copy(),
+                           wt=R.mc.wholetrans.copy(), ran1=R.ran1.copy(), rminex=R.rminex.cpu().numpy().copy()))
+    st = ens.step(sweeps=1, ngrsamples=0)
+    print(f"  step: {st}", flush=True)
+    torch.cuda.synchronize()
+    Lc, Lo = po.lib("cr"), po.lib("omp")
+    for i, (s, R, b) in enumerate(zip(systems, ens.boxes, before)):
+        print(f"  oracle box {i}: {s.nmol} molecules, {s.natom} atoms", flush=True)
+        g = R.g
+        t = b["tabs"]
+        tt = po.Tables(R.ld, g.npt2, po.Pf(t["xstar"]), po.Pf(t["xavs"]), po.Pf(t["xavs1"]), po.Pf(t["ecoul"]),
+                       po.Pf(t["ecoul1"]))
+        ob = po.OracleBox(s)
+        cl = po.Cells()
+        Lc.ora_cells_init(C.byref(cl), s.nmolcell, po.Pf(ob.xadd), po.Pf(ob.yadd), po.Pf(ob.zadd), s.nmol,
+                          T.sepmax(s, g))
+        led = po.Ledger()
+        ft, fb, f64 = C.c_float(), C.c_float(), C.c_double()
+        ei = np.ascontiguousarray(s.eintra, F32)
+        Lc.ora_fmolec1(C.byref(ob.box), C.byref(ob.centres), C.byref(tt), C.byref(cl), po.Pf(ei), C.byref(led),
+                       C.byref(ft), C.byref(fb), C.byref(f64))
+        erg0 = np.ctypeslib.as_array(led.ergpair, shape=(led.nergpair,)).copy()
+        assert np.array_equal(erg0.view(np.int32), b["ergpair"].view(np.int32)), f"box {i} ledger"
+        wr, wt = b["wr"].copy(), b["wt"].copy()
+        nwh = np.zeros(2 * s.nmoltype, np.int32)
+        nmv = np.zeros(2 * s.nmoltype, np.int32)
+        nsr = np.zeros(s.nmol, np.int32)
+        mcs = po.MCState(po.Pf(wr), po.Pf(wt), po.Pi(nwh), po.Pi(nmv), po.Pi(nsr), s.simt, F32(1), F32(1),
+                         F32(s.rejrate), 0)
+        rng = po.Ran1.from_array(b["ran1"])
+        fadds, nacc = C.c_float(0.0), C.c_int()
+        Lc.ora_molfit(C.byref(ob.box), po.Pf(ob.xa), po.Pf(ob.ya), po.Pf(ob.za), C.byref(ob.centres), C.byref(tt),
+                      C.byref(cl), C.byref(led), C.byref(mcs), C.byref(rng), C.byref(fadds), C.byref(nacc))
+        ldw = g.npt1 + 1
+        wav = np.zeros(2 * s.npairs * ldw, F32)
+        potex, potexmax, cs = np.zeros(s.npairs, F32), np.zeros(s.npairs, F32), np.zeros(s.ntype, F32)
+        ngr = Lo.ora_atogen(C.byref(ob.box), C.byref(ob.centres), g.npt1, ldw, po.Pf(g.ycoh),
+                            po.Pi(np.arange(s.npairs, dtype=np.int32)), po.Pf(b["rminex"]), 0, C.byref(rng),
+                            po.Pf(wav), po.Pf(potex), po.Pf(potexmax), po.Pf(cs), None)
+        assert ngr == min(1000, s.nmol)
+        co = R.box.get_coords()
+        for k in ("xa", "ya", "za", "xadd", "yadd", "zadd"):
+            assert np.array_equal(co[k].view(np.int32), getattr(ob, k).view(np.int32)), f"box {i} {k}"
+        np.testing.assert_array_equal(R.ran1, rng.as_array())
+        assert np.array_equal(R.mc.wholerot.view(np.int32), wr.view(np.int32))
+        assert np.array_equal(R.mc.wholetrans.view(np.int32), wt.view(np.int32))
+        np.testing.assert_array_equal(R.mc.nwhole, nwh)
+        np.testing.assert_array_equal(R.mc.nstepreject.cpu().numpy(), nsr)
+        erg1 = np.ctypeslib.as_array(led.ergpair, shape=(led.nergpair,))
+        assert np.array_equal(R.ledger.export()["ergpair"].view(np.int32), erg1.view(np.int32)), f"box {i} ledger"
+        sl = np.s_[:, :, 1:g.npt1 + 1]
+        got = R.wav.cpu().numpy().reshape(2, s.npairs, ldw)[sl]
+        assert np.array_equal(got.view(np.int32), wav.reshape(2, s.npairs, ldw)[sl].view(np.int32)), f"box {i} wav"

@@ -94,6 +94,8 @@ CASES = [
     ("gauss_ncfirst3_nmin5", ("gauss", 12, 90, 10, 0.02, 3.0), 5, 3, 4, 0.2, -19),
     ("poisson_c0_like", ("poisson", 13, 300, 20), 1, 1, 10, 0.01, -101),
     ("exact_fit_nan_path", ("exact", 14, 40, 6), 1, 2, 10, 0.01, -77),
+    # the EPSR test case's size: 116 Poisson terms (FIT_NTMAX is 128) over 600 Q points
+    ("poisson_epsr_test_116", ("poisson", 15, 600, 116), 1, 1, 10, 0.01, -131),
 ]
 
 

@@ -204,7 +204,7 @@ def test_zero_surrogate_is_plain_molfit(ctx):
     assert a[5].view(np.int32) == b[5].view(np.int32)
 
 
-@pytest.mark.parametrize("name,nsweep", [("tiny_13_12_3", 2), ("c0", 2)])
+@pytest.mark.parametrize("name,nsweep", [("tiny_13_12_3", 2), ("c0", 2), ("c3", 1)])
 def test_delayed_acceptance_chain_against_oracle(ctx, name, nsweep):
     from epsr_python_translation_amd import device as D
     from epsr_python_translation_amd import surrogate as SG

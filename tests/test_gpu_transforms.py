@@ -98,11 +98,13 @@ def _oracle_stog(flavour, s, g, sq, ldq, r, nr, ncols, nqmin, nqmax, xwt):
     return out, ldg
 
 
+@pytest.mark.parametrize("name", ["c0", "c1"])
 @pytest.mark.parametrize("mode", [0, 1])
-def test_stog(ctx, mode):
+def test_stog(ctx, mode, name):
+    """c1 is the 60,000-atom box's transform: 1200 Q points x 2100 r bins x 3 partials."""
     from epsr_python_translation_amd import device as D
-    print(f"stog mode{mode} c0", flush=True)
-    s, g, sq, ldq, r, nr, ncols, nqmin, nqmax, xwt = _stog_case("c0")
+    print(f"stog mode{mode} {name}", flush=True)
+    s, g, sq, ldq, r, nr, ncols, nqmin, nqmax, xwt = _stog_case(name)
     ref_cr, ldg = _oracle_stog("cr", s, g, sq, ldq, r, nr, ncols, nqmin, nqmax, xwt)
     ref_gl, _ = _oracle_stog("glibc", s, g, sq, ldq, r, nr, ncols, nqmin, nqmax, xwt)
     gofr = ctx.zeros(ncols * ldg)

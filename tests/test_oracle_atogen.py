@@ -96,3 +96,26 @@ def test_sampled_origins_consume_one_ran1_per_candidate():
     r = H.oracle_atogen(name, ngrsamples=7, idum=-99)
     assert r["ngr"] == 7
     assert r["ran1"][0] == s.nmol        # mcall: one draw per molecule 1..nmol
+
+
+@pytest.mark.parametrize("name,ngr", [("c0", 0), ("c0", 150), ("tiny_14_40_0", 7)])
+def test_oracle_omp_counts_equal_sequential(name, ngr):
+    """The all-cores oracle (origin molecules across OpenMP threads, integer histograms
+    summed at the end) counts exactly what the sequential one does, ran1 included."""
+    s, g = H.system(name), H.grid(name)
+    ldw = g.npt1 + 1
+    rm = np.full(s.npairs, 2.2, F32)
+    out = []
+    for flavour in ("glibc", "omp"):
+        cnt = np.zeros(2 * s.npairs * ldw, np.int64)
+        pc = np.zeros(s.npairs, np.int64)
+        ob = po.OracleBox(s)
+        rng = po.Ran1.seeded(-11)
+        n = po.lib(flavour).ora_atogen_counts(C.byref(ob.box), C.byref(ob.centres), g.npt1, ldw,
+                                              po.Pi(np.arange(s.npairs, dtype=np.int32)), po.Pf(rm), ngr,
+                                              C.byref(rng), cnt.ctypes.data_as(po.i64p), pc.ctypes.data_as(po.i64p))
+        out.append((n, cnt, pc, rng.as_array()))
+    assert out[0][0] == out[1][0]
+    for a, b in zip(out[0][1:], out[1][1:]):
+        np.testing.assert_array_equal(a, b)
+    assert out[0][2].sum() > 0
