@@ -45,20 +45,16 @@ def cpu_model():
     return platform.processor()
 
 
-def cpu_baseline(system, seconds_target=10.0):
-    """The oracle (C restatement of the reference's molfit/en_change, one core) timed on
-    a bounded sample of the same workload: MC sweeps of the same box, scaled to
-    atom-moves/s."""
+def _oracle_setup(s, L):
+    """The CPU path end to end on box s: tables (truncation, generate_rp, generate_ep), cells
+    and the fmolec1 ledger, all from the oracle."""
     import ctypes as C
 
     from epsr_python_translation_amd import tables as T
     from oracle import pyoracle as po
 
-    s = system
     g = T.make_grid(s)
-    L = po.lib("glibc")
     ob = po.OracleBox(s)
-    # tables from the oracle itself (the CPU path end to end)
     ld = g.ld
     wfac = np.array([0.0] + [L.ora_truncate(F32(g.x[j]), F32(s.rminpt), F32(s.rmaxpt)) for j in range(1, g.npt + 1)],
                     F32)
@@ -77,6 +73,7 @@ def cpu_baseline(system, seconds_target=10.0):
                       F32(s.pdmin), F32(s.pdstep), F32(s.rmaxpt), F32(s.psigma2), F32(s.rho), F32(0), F32(s.dmax),
                       F32(10), F32(0), F32(0), po.Pf(ecoul), po.Pf(xavs), po.Pf(xavs1), po.Pf(xt), po.Pf(xt1), ld,
                       C.byref(ea))
+    keep = (xavs, xavs1, ecoul, ecoul1)
     tt = po.Tables(ld, g.npt2, po.Pf(g.xstar), po.Pf(xavs), po.Pf(xavs1), po.Pf(ecoul), po.Pf(ecoul1))
     cl = po.Cells()
     L.ora_cells_init(C.byref(cl), s.nmolcell, po.Pf(ob.xadd), po.Pf(ob.yadd), po.Pf(ob.zadd), s.nmol,
@@ -85,6 +82,17 @@ def cpu_baseline(system, seconds_target=10.0):
     ft, fb, f64 = C.c_float(), C.c_float(), C.c_double()
     L.ora_fmolec1(C.byref(ob.box), C.byref(ob.centres), C.byref(tt), C.byref(cl),
                   po.Pf(np.ascontiguousarray(s.eintra, F32)), C.byref(led), C.byref(ft), C.byref(fb), C.byref(f64))
+    return dict(g=g, ob=ob, tt=tt, cl=cl, led=led, keep=keep)
+
+
+def _cpu_mc_rate(s, L, seconds_target):
+    """Oracle molfit sweeps of box s on one core until seconds_target: (atom-moves/s, sweeps, s)."""
+    import ctypes as C
+
+    from oracle import pyoracle as po
+
+    o = _oracle_setup(s, L)
+    ob = o["ob"]
     wr = np.full(s.nmoltype, 0.05, F32)
     wt = np.full(s.nmoltype, F32(0.1) / F32(s.dmax), F32)
     nwh = np.zeros(2 * s.nmoltype, np.int32)
@@ -96,17 +104,76 @@ def cpu_baseline(system, seconds_target=10.0):
     sweeps = 0
     t0 = time.perf_counter()
     while True:
-        L.ora_molfit(C.byref(ob.box), po.Pf(ob.xa), po.Pf(ob.ya), po.Pf(ob.za), C.byref(ob.centres), C.byref(tt),
-                     C.byref(cl), C.byref(led), C.byref(mc), C.byref(rng), C.byref(fadds), C.byref(nacc))
+        L.ora_molfit(C.byref(ob.box), po.Pf(ob.xa), po.Pf(ob.ya), po.Pf(ob.za), C.byref(ob.centres),
+                     C.byref(o["tt"]), C.byref(o["cl"]), C.byref(o["led"]), C.byref(mc), C.byref(rng),
+                     C.byref(fadds), C.byref(nacc))
         sweeps += 1
         dt = time.perf_counter() - t0
         if dt > seconds_target or sweeps >= 50:
             break
-    rate = sweeps * s.natom / dt
+    return sweeps * s.natom / dt, sweeps, dt
+
+
+def _cpu_pair_rate(s, flavour, norigin_mol):
+    """Oracle atogen of box s with norigin_mol origin molecules (ran1-sampled, the reference's
+    ngrsamples): (pair distances/s, pairs, s)."""
+    import ctypes as C
+
+    from epsr_python_translation_amd import tables as T
+    from oracle import pyoracle as po
+
+    g = T.make_grid(s)
+    ob = po.OracleBox(s)
+    ldw = g.npt1 + 1
+    wav = np.zeros(2 * s.npairs * ldw, F32)
+    px, pm, cs = np.zeros(s.npairs, F32), np.zeros(s.npairs, F32), np.zeros(s.ntype, F32)
+    flags = np.zeros(s.nmol, np.int32)
+    rng = po.Ran1.seeded(-9)
+    t0 = time.perf_counter()
+    po.lib(flavour).ora_atogen(C.byref(ob.box), C.byref(ob.centres), g.npt1, ldw, po.Pf(g.ycoh),
+                               po.Pi(np.arange(s.npairs, dtype=np.int32)), po.Pf(np.zeros(s.npairs, F32)),
+                               int(norigin_mol), C.byref(rng), po.Pf(wav), po.Pf(px), po.Pf(pm), po.Pf(cs),
+                               po.Pi(flags))
+    dt = time.perf_counter() - t0
+    nat = (np.asarray(s.nal) - np.asarray(s.naf) + 1).astype(np.int64)
+    sel = flags != 0
+    pairs = int((nat[sel] * (s.natom - nat[sel])).sum())
+    nintra = int((nat * (nat - 1) // 2).sum())
+    return (pairs + nintra) / dt, pairs + nintra, dt
+
+
+def cpu_baseline(system, seconds_target=8.0):
+    """The oracle (C restatement of the reference's loops, -O2 -fno-tree-vectorize) timed on
+    bounded samples of the workloads on this host:
+      value   molfit sweeps of the same box on one core, atom-moves/s (the MC chain is
+              serial: each move's acceptance depends on the previous ones, so it has no
+              all-cores form);
+      pair_distances_per_s   atogen of the same box with ran1-sampled origin molecules, on
+              one core and on all cores (origin molecules over OpenMP threads);
+      dmso_water_3200   molfit sweeps of the 200 DMSO + 400 water box on one core."""
+    from epsr_python_translation_amd import synth
+    from oracle import pyoracle as po
+
+    s = system
+    L = po.lib("glibc")
+    rate, sweeps, dt = _cpu_mc_rate(s, L, seconds_target)
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    p1, n1, t1 = _cpu_pair_rate(s, "glibc", 100)
+    pn, nn, tn = _cpu_pair_rate(s, "omp", 2000)
+    s0 = synth.config0()
+    r0, sw0, dt0 = _cpu_mc_rate(s0, L, 3.0)
     return {"value": rate, "unit": "atom-moves/s", "cores": 1, "kind": "port",
             "sample": f"oracle molfit (C restatement of the reference MC cycle, -O2 -fno-tree-vectorize) on the "
                       f"same box: {sweeps} sweep(s) of {s.nmol} molecule moves in {dt:.2f} s; "
-                      f"CPU {cpu_model()}, nproc {os.cpu_count()}"}
+                      f"CPU {cpu_model()}, nproc {os.cpu_count()}",
+            "pair_distances_per_s": {
+                "cores_1": p1, "cores_all": pn, "threads": threads,
+                "sample": f"oracle atogen on the same box: 100 origin molecules ({n1:.3g} distances) in {t1:.2f} s "
+                          f"on one core; 2000 origin molecules ({nn:.3g} distances) in {tn:.2f} s on {threads} "
+                          f"OpenMP threads"},
+            "dmso_water_3200": {
+                "value": r0, "unit": "atom-moves/s", "cores": 1,
+                "sample": f"oracle molfit on 200 DMSO + 400 water (3200 atoms): {sw0} sweep(s) in {dt0:.2f} s"}}
 
 
 def main():
@@ -204,10 +271,13 @@ def main():
     npair_entries = R.ledger.nergpair - int(np.count_nonzero(s.eintra))
     apm = float(np.mean(nat))                                     # atoms per molecule (3 for every water)
     lookups_per_sweep = 3.0 * npair_entries * apm * apm          # A for both slots + Bv for the later one
+    # the pair kernels are vector-ALU work (no matrix shape): their peak is MI355X's FP32
+    # vector rate, 157.3 TFLOP/s, which counts an FMA as two operations; without contraction
+    # (the float32 contract of the path) one operation per lane per cycle gives half of it
     work = {
-        "atogen_inter": ("mfma", OPS_PER_DISTANCE * inter_pairs, "TFLOP/s",
+        "atogen_inter": ("valu", OPS_PER_DISTANCE * inter_pairs, "TFLOP/s",
                          f"{OPS_PER_DISTANCE} fp32 ops x {inter_pairs} distances"),
-        "mc_pair": ("mfma", OPS_PER_LOOKUP * lookups_per_sweep, "TFLOP/s",
+        "mc_pair": ("valu", OPS_PER_LOOKUP * lookups_per_sweep, "TFLOP/s",
                     f"{OPS_PER_LOOKUP} fp32 ops x {lookups_per_sweep:.3g} atom-pair lookups"),
         "gensq": ("mfma", 2.0 * g.nq * g.npt1 * 2 * s.npairs, "TFLOP/s", "2 nq npt1 2npairs flop"),
         "stog": ("mfma", 2.0 * nr * g.nq * s.npairs, "TFLOP/s", "2 nr nq npairs flop"),
@@ -224,11 +294,15 @@ def main():
         peak = FP32_PEAK_TFLOPS if unit == "TFLOP/s" else HBM_PEAK_GBS
         kernels[k] = {"bound": bound, "achieved": achieved, "peak": peak, "unit": unit, "frac": achieved / peak,
                       "ms": kavg[k], "work_per_launch": what}
+        if bound == "valu":
+            kernels[k]["peak_unfused"] = FP32_PEAK_TFLOPS / 2.0
+            kernels[k]["frac_unfused"] = achieved / (FP32_PEAK_TFLOPS / 2.0)
     dom = max(ktot, key=ktot.get)
     kd = kernels.get(dom, {})
     traffic = _traffic(dom)
     roof = {"bound": kd.get("bound", "hbm"), "achieved": kd.get("achieved"), "peak": kd.get("peak"),
             "unit": kd.get("unit"), "frac": kd.get("frac"), "traffic": traffic, "kernel": dom,
+            "frac_unfused": kd.get("frac_unfused"),
             "kernel_ms": kavg.get(dom), "work_per_launch": kd.get("work_per_launch"), "per_step_kernel_ms": ktot,
             "kernels": kernels}
     out = {
