@@ -132,6 +132,17 @@ class MCFullParams(C.Structure):
                 ("ireset", C.c_int), ("intra", C.c_int), ("rotfreq", C.c_int)]
 
 
+class DMSDesc(C.Structure):
+    """epsr_dms_desc_t"""
+    _fields_ = [("nfile", C.c_int), ("nq", C.c_int), ("ldq", C.c_int), ("npairs", C.c_int), ("nxfile", C.c_int),
+                ("q", C.c_void_p), ("datai", C.c_void_p), ("soqintra", C.c_void_p), ("soqinter", C.c_void_p),
+                ("weightx", C.c_void_p), ("x", C.c_void_p), ("ciwt", C.c_void_p), ("cwt", C.c_void_p),
+                ("xfile", C.c_void_p), ("nqmind", C.c_void_p), ("nqmaxd", C.c_void_p), ("nqmin", C.c_void_p),
+                ("nqmax", C.c_void_p), ("szeros", C.c_void_p), ("porodamp", C.c_void_p), ("rshmin", C.c_void_p),
+                ("porodq", C.c_float), ("porodpwr", C.c_float), ("nsmoop", C.c_int), ("npt1", C.c_int),
+                ("rho", C.c_float)]
+
+
 class MCSteps(C.Structure):
     _fields_ = [("wholerot", fp), ("wholetrans", fp), ("internrot", fp), ("interntrans", fp), ("nwhole", ip),
                 ("nmovem", ip), ("ninter", ip), ("nmovei", ip)]
@@ -205,7 +216,8 @@ SIGNATURES = {
                                      fp, fp, fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, i32p, C.c_int,
                                      ip]),
     "epsr_ptgnsq_diff": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, vp, C.c_int, vp, fp, fp, ip, ip, C.c_float,
-                                   vp]),
+                                   vp, vp, ip, C.c_int]),
+    "epsr_data_minus_sim": (C.c_int, [vp, C.POINTER(DMSDesc), vp, vp, vp, vp]),
     "epsr_ptgnsq_rfactor": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, fp]),
     "epsr_ptgnsq_update": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, fp, ip, fp, fp, C.c_int, C.c_float, fp, fp,
                                      C.c_int]),

@@ -1,13 +1,13 @@
-"""Fill the COMMON mirror from a synthetic box (epsr_python_translation_amd.synth), the
-way the reference's readers fill it from .ato/.EPSR.inp files (those formats are not
-part of this package): 1-based atoms, molecules, types and pairs, column-major tables."""
+"""Fill the COMMON mirror from a box (epsr_python_translation_amd.synth, or one read from
+.ato/.pcof/.EPSR.inp files by formats.system_from_files), the way the reference's readers
+fill it: 1-based atoms, molecules, types and pairs, column-major tables."""
 
 from __future__ import annotations
 
 import numpy as np
 
 from .. import tables as T
-from . import clustercom, epsr16, gpu, molecom, potential_par, ran1com, weights
+from . import clustercom, epsr16, gpu, molecom, potential_par, ran1com, weights, x_rayf0
 
 F32 = np.float32
 
@@ -57,6 +57,7 @@ def load(s, idum=-1, nfile=1, grid=None):
     potential_par.pdmin, potential_par.pdstep = F32(s.pdmin), F32(s.pdstep)
     potential_par.pdmax, potential_par.psigma2 = F32(s.rmaxpt), F32(s.psigma2)
     weights.reset(nfile, npairs)
+    x_rayf0.reset(g.nq, npairs)
     ran1com.reset()
     st = np.zeros(35, np.int32)
     st[1] = idum

@@ -18,6 +18,8 @@
 #include <vector>
 
 #include "common.h"
+#include <cstring>
+#include <vector>
 
 #define FIT_NTMAX 128
 #define FIT_NSTEP 100
@@ -503,9 +505,15 @@ extern "C" int epsr_fit_expansion(epsr_ctx *ctx, int nchain, const float *yin, i
 // qtemp(i, id): data minus the cwt-weighted sum of the intermolecular partials, with the
 // extrapolation to Q = 0 below qmin (epsr_standalone_rev3.py:3827-3867); one thread per
 // (Q point, dataset)
+// sum = f32(sum + w * s): the reference's weights are float64 arrays (weights.cwt, and
+// x_rayf0.weightx for X-ray data), so each term of a weighted sum is added in double and
+// the sum rounded to float
+__device__ __forceinline__ float wsum_step(float sum, double w, float s) { return (float)((double)sum + w * (double)s); }
+
 __global__ void ptgnsq_diff_kernel(int nfile, int nq, int npairs, const float *__restrict__ q,
                                    const float *__restrict__ data, int ldq, const float *__restrict__ qwind,
-                                   const float *__restrict__ cwt, const float *__restrict__ szeros,
+                                   const float *__restrict__ cwt, const double *__restrict__ weightx,
+                                   const int *__restrict__ xfile, const float *__restrict__ szeros,
                                    const int *__restrict__ nqmin, const int *__restrict__ nqmax, float qminptemp,
                                    float *__restrict__ qtemp)
 {
@@ -514,13 +522,19 @@ __global__ void ptgnsq_diff_kernel(int nfile, int nq, int npairs, const float *_
     const int id = t / nq, i = t - id * nq + 1;
     const float *dat = data + (size_t)id * ldq;
     const int nqmn = nqmin[id];
+    const int ix = xfile ? xfile[id] : 0;
     float grads0 = 0.0f, conss0 = 0.0f;
     if (szeros[id] != 0.0f && nqmn > 1) {
         grads0 = (dat[nqmn] - szeros[id]) / (q[nqmn] - q[1]);
         conss0 = szeros[id];
     }
     float sum = 0.0f;
-    for (int ic = 0; ic < npairs; ++ic) sum = sum + cwt[id * npairs + ic] * qwind[(size_t)ic * ldq + i];
+    if (ix > 0) {
+        const double *w = weightx + ((size_t)(ix - 1) * ldq + i) * npairs;
+        for (int ic = 0; ic < npairs; ++ic) sum = wsum_step(sum, w[ic], qwind[(size_t)ic * ldq + i]);
+    } else {
+        for (int ic = 0; ic < npairs; ++ic) sum = wsum_step(sum, (double)cwt[id * npairs + ic], qwind[(size_t)ic * ldq + i]);
+    }
     float v;
     if (q[i] < fmaxf(qminptemp, q[nqmn])) {
         v = szeros[id] != 0.0f ? conss0 + grads0 * (q[i] - q[1]) - sum : 0.0f;
@@ -534,30 +548,219 @@ __global__ void ptgnsq_diff_kernel(int nfile, int nq, int npairs, const float *_
 
 extern "C" int epsr_ptgnsq_diff(epsr_ctx *ctx, int nfile, int nq, int npairs, const float *q, const float *data,
                                 int ldq, const float *qwind, const float *cwt, const float *szeros, const int *nqmin,
-                                const int *nqmax, float qminptemp, float *qtemp)
+                                const int *nqmax, float qminptemp, float *qtemp, const double *weightx,
+                                const int *xfile, int nxfile)
 {
     if (!ctx) return EPSR_ERR_ARG;
     ARG_CHECK(ctx, nfile >= 1 && nq >= 1 && npairs >= 1 && ldq >= nq + 1 && q && data && qwind && cwt && szeros &&
                        nqmin && nqmax && qtemp,
               "epsr_ptgnsq_diff: bad argument");
-    for (int id = 0; id < nfile; ++id)
+    for (int id = 0; id < nfile; ++id) {
         ARG_CHECK(ctx, nqmin[id] >= 1 && nqmin[id] <= nq && nqmax[id] <= nq,
                   "epsr_ptgnsq_diff: dataset %d: Q range %d..%d outside 1..%d", id, nqmin[id], nqmax[id], nq);
+        ARG_CHECK(ctx, !xfile || (xfile[id] >= 0 && xfile[id] <= nxfile && (xfile[id] == 0 || weightx)),
+                  "epsr_ptgnsq_diff: dataset %d: X-ray file %d outside 0..%d (or no weightx)", id,
+                  xfile ? xfile[id] : 0, nxfile);
+    }
     HIP_CHECK(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     StreamTemp tmp(ctx);
     float *dcwt = (float *)tmp.alloc(sizeof(float) * nfile * npairs);
     float *dsz = (float *)tmp.alloc(sizeof(float) * nfile);
     int *dmn = (int *)tmp.alloc(sizeof(int) * nfile), *dmx = (int *)tmp.alloc(sizeof(int) * nfile);
-    if (!dcwt || !dsz || !dmn || !dmx) return EPSR_ERR_HIP;
+    int *dxf = xfile ? (int *)tmp.alloc(sizeof(int) * nfile) : nullptr;
+    if (!dcwt || !dsz || !dmn || !dmx || (xfile && !dxf)) return EPSR_ERR_HIP;
     HIP_CHECK(ctx, hipMemcpyAsync(dcwt, cwt, sizeof(float) * nfile * npairs, hipMemcpyHostToDevice, st));
     HIP_CHECK(ctx, hipMemcpyAsync(dsz, szeros, sizeof(float) * nfile, hipMemcpyHostToDevice, st));
     HIP_CHECK(ctx, hipMemcpyAsync(dmn, nqmin, sizeof(int) * nfile, hipMemcpyHostToDevice, st));
     HIP_CHECK(ctx, hipMemcpyAsync(dmx, nqmax, sizeof(int) * nfile, hipMemcpyHostToDevice, st));
-    ptgnsq_diff_kernel<<<ceil_div(nfile * nq, 256), 256, 0, st>>>(nfile, nq, npairs, q, data, ldq, qwind, dcwt, dsz,
-                                                                   dmn, dmx, qminptemp, qtemp);
+    if (dxf) HIP_CHECK(ctx, hipMemcpyAsync(dxf, xfile, sizeof(int) * nfile, hipMemcpyHostToDevice, st));
+    ptgnsq_diff_kernel<<<ceil_div(nfile * nq, 256), 256, 0, st>>>(nfile, nq, npairs, q, data, ldq, qwind, dcwt, weightx,
+                                                                   dxf, dsz, dmn, dmx, qminptemp, qtemp);
     HIP_CHECK(ctx, hipGetLastError());
     HIP_CHECK(ctx, hipStreamSynchronize(st));   // the host arguments are released on return
+    return EPSR_OK;
+}
+
+// ------------------------------------------------------------------ data minus simulation
+// The block of the epsr program between the running averages and ptgnsq
+// (epsr_standalone_rev3.py:1703-1830), one workgroup, the datasets in order:
+//  1. datat = datai - sum_ip w soqintra (w: ciwt, or weightx for X-ray data) on the data's
+//     Q range, szerostemp = szeros - the same sum at Q index 1 (when szeros is non-zero);
+//  2. per dataset dts = datat - sum_ic w soqinter on the fitted range (plus the Porod term
+//     porodamp (porodq / q)^porodpwr when porodamp > 0), extrapolated below it from
+//     szerostemp when szeros is non-zero, zero elsewhere;
+//  3. with nsmoop > 0, ftback (:5330-5389): the part of dts from r < rshmin (a sine
+//     transform to g(r) on r < rshmin and back), subtracted from datat on the fitted range.
+// grt is the program's one r-indexed work array: entries a call does not write keep what
+// earlier calls left.  sinf and powf are the correctly rounded values (float of the
+// double functions).
+__device__ __forceinline__ float cr_sinf(float x) { return (float)sin((double)x); }
+__device__ __forceinline__ float cr_powf(float x, float y) { return (float)pow((double)x, (double)y); }
+
+__device__ __forceinline__ float dms_pair_sum(const epsr_dms_desc_t &d, const float *__restrict__ ciw,
+                                              const float *__restrict__ cw, const int *__restrict__ xf, int id,
+                                              int iq, const float *__restrict__ soq, bool intra)
+{
+    const int ix = xf[id];
+    float sum = 0.0f;
+    if (ix > 0) {
+        const double *w = d.weightx + ((size_t)(ix - 1) * d.ldq + iq) * d.npairs;
+        for (int ip = 0; ip < d.npairs; ++ip) sum = wsum_step(sum, w[ip], soq[(size_t)ip * d.ldq + iq]);
+    } else {
+        const float *w = (intra ? ciw : cw) + (size_t)id * d.npairs;
+        for (int ip = 0; ip < d.npairs; ++ip) sum = wsum_step(sum, (double)w[ip], soq[(size_t)ip * d.ldq + iq]);
+    }
+    return sum;
+}
+
+// per-dataset arrays staged on the device: [0] xfile, [1..4] nqmind, nqmaxd, nqmin, nqmax
+// (int), then szeros, porodamp, rshmin (float), then ciwt, cwt [nfile*npairs]
+__global__ void __launch_bounds__(1024)
+data_minus_sim_kernel(epsr_dms_desc_t d, const int *__restrict__ iv, const float *__restrict__ fv,
+                      float *__restrict__ datat, float *__restrict__ szerostemp, float *__restrict__ dts,
+                      float *__restrict__ grt)
+{
+    const int nf = d.nfile, nq = d.nq, ldq = d.ldq;
+    const int *xf = iv, *nqmind = iv + nf, *nqmaxd = iv + 2 * nf, *nqmin = iv + 3 * nf, *nqmax = iv + 4 * nf;
+    const float *szeros = fv, *porodamp = fv + nf, *rshmin = fv + 2 * nf;
+    const float *ciw = fv + 3 * nf, *cw = ciw + (size_t)nf * d.npairs;
+    const float *q = d.q;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    __shared__ int s_jfirst;
+    // 1. the intramolecular terms out of the data
+    for (int t = tid; t < nf * nq; t += nt) {
+        const int id = t / nq, iq = t - id * nq + 1;
+        float v = 0.0f;
+        if (iq >= nqmind[id] && iq <= nqmaxd[id])
+            v = d.datai[(size_t)id * ldq + iq] - dms_pair_sum(d, ciw, cw, xf, id, iq, d.soqintra, true);
+        datat[(size_t)id * ldq + iq] = v;
+    }
+    for (int id = tid; id < nf; id += nt)
+        szerostemp[id] = szeros[id] != 0.0f ? szeros[id] - dms_pair_sum(d, ciw, cw, xf, id, 1, d.soqintra, true) : 0.0f;
+    __syncthreads();
+    const float pi = 3.141592653f;
+    for (int id = 0; id < nf; ++id) {
+        float *dt = datat + (size_t)id * ldq;
+        const int nmn = nqmin[id], nmx = nqmax[id];
+        float grads0, conss0;
+        if (szeros[id] != 0.0f && nmn > 1) {
+            grads0 = (dt[nmn] - szerostemp[id]) / (q[nmn] - q[1]);
+            conss0 = szerostemp[id];
+        } else {
+            grads0 = 0.0f;
+            conss0 = dt[nmn];
+        }
+        // 2. data - simulation
+        for (int i = 1 + tid; i <= nq; i += nt) {
+            const float sum = dms_pair_sum(d, ciw, cw, xf, id, i, d.soqinter, false);
+            float v;
+            if (i < nmn) {
+                v = szeros[id] != 0.0f ? conss0 + grads0 * (q[i] - q[1]) - sum : 0.0f;
+            } else if (i <= nmx) {
+                v = dt[i] - sum;
+                if (porodamp[id] > 0.0f) v = v + porodamp[id] * cr_powf(d.porodq / q[i], d.porodpwr);
+            } else {
+                v = 0.0f;
+            }
+            // ftback starts from sofq * q on the fitted range
+            if (d.nsmoop > 0 && i >= nmn && i <= nmx) v = v * q[i];
+            dts[i] = v;
+        }
+        if (d.nsmoop <= 0) {
+            __syncthreads();
+            continue;
+        }
+        // 3. ftback: g(r) on r < rshmin from dts q, then the S(Q) of that truncated g(r)
+        if (tid == 0) {
+            int j = 2;
+            while (j <= d.npt1 && d.x[j] < rshmin[id]) ++j;
+            s_jfirst = j;
+        }
+        __syncthreads();
+        const int jfirst = s_jfirst;
+        const int jsave = jfirst > 2 ? jfirst - 1 : 2;
+        const float qstep = q[4] - q[3];
+        const float sfac = 0.5f * qstep / pi / pi / d.rho;
+        for (int j = 2 + tid; j < jfirst; j += nt) {
+            const float rr = d.x[j];
+            float sum = 0.0f;
+            for (int i = nmn; i <= nmx; ++i) sum = sum + dts[i] * cr_sinf(q[i] * rr);
+            grt[j] = sfac * sum;
+        }
+        __syncthreads();
+        const float gfac = 4.0f * pi * d.rho;
+        for (int i = nmn + tid; i <= nmx; i += nt) {
+            const float qq = q[i];
+            float r1 = 0.0f, sum = 0.0f;
+            if (qq > 0.0f) {
+                for (int j = 2; j <= jsave; ++j) {
+                    const float rr = d.x[j];
+                    const float r2 = 2.0f * rr - r1;
+                    const float rstep = r2 - r1;
+                    sum = sum + grt[j] * cr_sinf(qq * rr) * rstep;
+                    r1 = r2;
+                }
+                dt[i] = dt[i] - gfac * sum / qq;
+            } else {
+                for (int j = 2; j <= jsave; ++j) {
+                    const float rr = d.x[j];
+                    const float r2 = 2.0f * rr - r1;
+                    const float rstep = r2 - r1;
+                    sum = sum + grt[j] * rr * rstep;
+                    r1 = r2;
+                }
+                dt[i] = dt[i] - gfac * sum;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+extern "C" int epsr_data_minus_sim(epsr_ctx *ctx, const epsr_dms_desc_t *d, float *datat, float *szerostemp,
+                                   float *dts, float *grt)
+{
+    if (!ctx) return EPSR_ERR_ARG;
+    ARG_CHECK(ctx, d && datat && szerostemp && dts && grt, "epsr_data_minus_sim: null argument");
+    ARG_CHECK(ctx, d->nfile >= 1 && d->nq >= 4 && d->npairs >= 1 && d->ldq >= d->nq + 1 && d->q && d->datai &&
+                       d->soqintra && d->soqinter && d->ciwt && d->cwt && d->xfile && d->nqmind && d->nqmaxd &&
+                       d->nqmin && d->nqmax && d->szeros && d->porodamp && d->rshmin && d->x && d->npt1 >= 2,
+              "epsr_data_minus_sim: bad descriptor");
+    const int nf = d->nfile;
+    for (int id = 0; id < nf; ++id) {
+        ARG_CHECK(ctx, d->nqmin[id] >= 1 && d->nqmin[id] <= d->nqmax[id] && d->nqmax[id] <= d->nq,
+                  "epsr_data_minus_sim: dataset %d: fitted Q range %d..%d outside 1..%d", id, d->nqmin[id],
+                  d->nqmax[id], d->nq);
+        ARG_CHECK(ctx, d->xfile[id] >= 0 && d->xfile[id] <= d->nxfile && (d->xfile[id] == 0 || d->weightx),
+                  "epsr_data_minus_sim: dataset %d: X-ray file %d outside 0..%d (or no weightx)", id, d->xfile[id],
+                  d->nxfile);
+    }
+    HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    StreamTemp tmp(ctx);
+    const size_t ni = 5 * (size_t)nf, nfl = 3 * (size_t)nf + 2 * (size_t)nf * d->npairs;
+    int *div = (int *)tmp.alloc(sizeof(int) * ni);
+    float *dfv = (float *)tmp.alloc(sizeof(float) * nfl);
+    if (!div || !dfv) return EPSR_ERR_HIP;
+    std::vector<int> hi(ni);
+    std::vector<float> hf(nfl);
+    for (int id = 0; id < nf; ++id) {
+        hi[id] = d->xfile[id];
+        hi[nf + id] = d->nqmind[id];
+        hi[2 * nf + id] = d->nqmaxd[id];
+        hi[3 * nf + id] = d->nqmin[id];
+        hi[4 * nf + id] = d->nqmax[id];
+        hf[id] = d->szeros[id];
+        hf[nf + id] = d->porodamp[id];
+        hf[2 * nf + id] = d->rshmin[id];
+    }
+    std::memcpy(hf.data() + 3 * nf, d->ciwt, sizeof(float) * nf * d->npairs);
+    std::memcpy(hf.data() + 3 * nf + (size_t)nf * d->npairs, d->cwt, sizeof(float) * nf * d->npairs);
+    HIP_CHECK(ctx, hipMemcpyAsync(div, hi.data(), sizeof(int) * ni, hipMemcpyHostToDevice, st));
+    HIP_CHECK(ctx, hipMemcpyAsync(dfv, hf.data(), sizeof(float) * nfl, hipMemcpyHostToDevice, st));
+    data_minus_sim_kernel<<<1, 1024, 0, st>>>(*d, div, dfv, datat, szerostemp, dts, grt);
+    HIP_CHECK(ctx, hipGetLastError());
+    HIP_CHECK(ctx, hipStreamSynchronize(st));   // the staged host arrays are released on return
     return EPSR_OK;
 }
 

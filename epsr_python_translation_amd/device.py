@@ -607,16 +607,54 @@ def calc_pwrexpq(ctx: Context, lpt, nterms, dmin, dstep, dmax, sigmab, rho, q, e
 
 
 def ptgnsq_diff(ctx: Context, nfile, nq, npairs, q, data, ldq, qwind, cwt, szeros, nqmin, nqmax, qminptemp,
-                qtemp=None):
+                qtemp=None, weightx=None, xfile=None):
+    """ptgnsq's differences; weightx (device float64 [nxfile*ldq*npairs]) and xfile (host
+    int [nfile], 0 for neutron data) for X-ray datasets."""
     if qtemp is None:
         qtemp = ctx.zeros(nfile * ldq)
     cwt_, cwtp = _fp(cwt)
     sz_, szp = _fp(szeros)
     mn_, mnp = _ip(nqmin)
     mx_, mxp = _ip(nqmax)
+    if xfile is not None:
+        xf_, xfp = _ip(xfile)
+        nx = int(np.max(xf_)) if len(xf_) else 0
+    else:
+        xfp, nx = None, 0
     ctx.check(ctx.lib.epsr_ptgnsq_diff(ctx.h, int(nfile), int(nq), int(npairs), _ptr(q), _ptr(data), int(ldq),
-                                       _ptr(qwind), cwtp, szp, mnp, mxp, F32(qminptemp), _ptr(qtemp)))
+                                       _ptr(qwind), cwtp, szp, mnp, mxp, F32(qminptemp), _ptr(qtemp), _ptr(weightx),
+                                       xfp, nx))
     return qtemp
+
+
+def data_minus_sim(ctx: Context, q, datai, ldq, nq, soqintra, soqinter, ciwt, cwt, nqmind, nqmaxd, nqmin, nqmax,
+                   szeros, porodamp, rshmin, porodq, porodpwr, nsmoop, x, npt1, rho, grt, weightx=None, xfile=None,
+                   datat=None, szerostemp=None):
+    """The data-minus-simulation block of the epsr program (epsr_data_minus_sim).  Device:
+    q [ldq], datai [nfile*ldq], soqintra/soqinter [npairs*ldq], x [npt1+1], grt [npt1+1]
+    (kept between calls), weightx float64 [nxfile*ldq*npairs]; host: ciwt/cwt [nfile,
+    npairs], the per-dataset ranges and scalars.  Returns (datat, szerostemp) on the device."""
+    ciwt = np.ascontiguousarray(ciwt, F32)
+    cwt = np.ascontiguousarray(cwt, F32)
+    nfile, npairs = ciwt.shape
+    xf = np.zeros(nfile, np.int32) if xfile is None else np.ascontiguousarray(xfile, np.int32)
+    keep = dict(ciwt=ciwt, cwt=cwt, xfile=xf, nqmind=np.ascontiguousarray(nqmind, np.int32),
+                nqmaxd=np.ascontiguousarray(nqmaxd, np.int32), nqmin=np.ascontiguousarray(nqmin, np.int32),
+                nqmax=np.ascontiguousarray(nqmax, np.int32), szeros=np.ascontiguousarray(szeros, F32),
+                porodamp=np.ascontiguousarray(porodamp, F32), rshmin=np.ascontiguousarray(rshmin, F32))
+    if datat is None:
+        datat = ctx.zeros(nfile * ldq)
+    if szerostemp is None:
+        szerostemp = ctx.zeros(nfile)
+    dts = ctx.zeros(ldq)
+    h = lambda k: keep[k].ctypes.data
+    dv = lambda t: t.data_ptr() if t is not None else None
+    d = _native.DMSDesc(nfile, int(nq), int(ldq), npairs, int(xf.max()) if nfile else 0, dv(q), dv(datai),
+                        dv(soqintra), dv(soqinter), dv(weightx), dv(x), h("ciwt"), h("cwt"), h("xfile"),
+                        h("nqmind"), h("nqmaxd"), h("nqmin"), h("nqmax"), h("szeros"), h("porodamp"), h("rshmin"),
+                        F32(porodq), F32(porodpwr), int(nsmoop), int(npt1), F32(rho))
+    ctx.check(ctx.lib.epsr_data_minus_sim(ctx.h, C.byref(d), _ptr(datat), _ptr(szerostemp), _ptr(dts), _ptr(grt)))
+    return datat, szerostemp
 
 
 def ptgnsq_rfactor(ctx: Context, i0, i1, fit, qtemp):

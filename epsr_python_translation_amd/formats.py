@@ -866,19 +866,78 @@ def write_pcof(path, p: PcofFile):
 
 
 @dataclasses.dataclass
+class XrayRecord:
+    """One atom type's X-ray form factor: cxt + sum_i axt_i exp(-bxt_i (Q/4 pi)^2), with the
+    modification alphaxt, deltaxt (get_wts/_read_xray_coeffs, epsr_standalone_rev3.py:
+    4285-4296, 4517-4537); a1 the type label (character*3), a2 the element read beside it."""
+
+    a1: str
+    a2: str
+    cxt: float
+    axt: tuple                 # 5 doubles
+    bxt: tuple                 # 5 doubles
+    alphaxt: np.float32
+    deltaxt: np.float32
+
+
+@dataclasses.dataclass
 class WtsFile:
-    """A neutron weights file (the neutron branch of get_wts, epsr_standalone_rev3.py:
-    4308-4318): the density factor, then per type pair its labels and the intramolecular
-    and intermolecular weights (the rest of each value record is not read)."""
+    """A weights file (get_wts, epsr_standalone_rev3.py:4180-4330).  Neutron: the density
+    factor, then per type pair its labels and the intramolecular and intermolecular weights
+    (the rest of each value record is not read).  X-ray (a file name containing
+    `X-ray.wts`, `x-ray.wts` or `XWTS.wts`): the normalisation line -- rhofac, normx,
+    iafformaff, or one old-style value they are deduced from (:4240-4266) -- then per atom
+    type its labels and form-factor coefficients."""
 
     rhofac: np.float32
     pairs: list                # [(a1, a2, w_intra, w_inter)] in file order, labels character*3
+    xray: bool = False
+    normx: int = 0
+    iafformaff: int = 0
+    records: list = dataclasses.field(default_factory=list)   # [XrayRecord]
+
+
+def is_xray_wts(path) -> bool:
+    return any(k in str(path) for k in ("X-ray.wts", "x-ray.wts", "XWTS.wts"))
+
+
+def _read_xray_wts(L: _Lines) -> WtsFile:
+    first = L.line()
+    w = words(first, 5)
+    if len(w) == 3:
+        rhofac, normx, iafformaff = f32(w[0]), fint(w[1]), fint(w[2])
+    else:
+        # the old single value: its sign gives per-atom (1) or per-molecule (2) data, its
+        # size the form factors and the normalisation (normx 1 <-> 2 in the new numbering)
+        rhofac = f32(w[0])
+        nrho = 2 if rhofac < 0 else 1
+        iafformaff = int(np.trunc(F32(F32(abs(rhofac)) + F32(1)) / F32(2)))
+        normx = int(np.trunc(F32(F32(abs(rhofac)) - F32(2 * (iafformaff - 1)))))
+        normx = 2 if normx == 1 else 1
+        rhofac = f32(nrho)
+    recs = []
+    while not L.at_end():
+        lab = L.line()
+        if not lab.strip():
+            continue
+        a1 = fa(lab[1:4], 3)                               # (1x,a3,1x,a4): a2 keeps the
+        a2 = fa((lab[5:9] + "    ")[:4][1:], 3)            # rightmost 3 of its 4 characters
+        try:
+            v = L.items(13)
+        except EOFError:
+            break
+        if len(v) < 13:
+            break
+        num = [float(t.replace("d", "e").replace("D", "e")) for t in v[:11]]
+        recs.append(XrayRecord(a1=a1, a2=a2, cxt=num[0], axt=tuple(num[1:11:2]), bxt=tuple(num[2:11:2]),
+                               alphaxt=f32(v[11]), deltaxt=f32(v[12])))
+    return WtsFile(rhofac=rhofac, pairs=[], xray=True, normx=normx, iafformaff=iafformaff, records=recs)
 
 
 def read_wts(path) -> WtsFile:
     L = _Lines(_read_text(path))
-    if any(k in str(path) for k in ("X-ray.wts", "x-ray.wts", "XWTS.wts")):
-        raise NotImplementedError("read_wts: X-ray weights files are not handled (neutron weights only)")
+    if is_xray_wts(path):
+        return _read_xray_wts(L)
     rhofac = f32(L.items(1)[0])
     pairs = []
     while not L.at_end():

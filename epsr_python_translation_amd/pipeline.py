@@ -58,54 +58,196 @@ def invert_cwt(ctx, cwt, cwtt, feedback, efacm, ran1_state):
     return carray, carrayold, it.value
 
 
-def get_wts(ctx, s, wts, feedback, efacm, ran1_state, cwtt=None, recompute=True):
-    """The neutron branch of get_wts (epsr_standalone_rev3.py:4180-4479) for box `s` and the
-    parsed weights files `wts` (formats.read_wts, one per dataset): per dataset and type
-    pair the weights ciwt (intra) and cwt (inter), wsum = sum |cwt| and xwt = sum cwt, and,
-    when `recompute` (ireset = 1 or feedback/efacm changed), the inverted matrix through
-    invert_cwt from cwtt = [0 | I] (else from the given cwtt; with neither, cwtpot is None).  Returns a dict with ciwt, cwt [nfile, npairs] (float32),
-    wsum, xwt [nfile], cwtt [npairs, npairs + nfile] and cwtpot [npairs, nfile] (float64)."""
+_LIBM = None
+
+
+def _expf(x):
+    """glibc expf, the function the reference's fmath.expf calls (fortran/fmath.py:46-145)"""
+    global _LIBM
+    if _LIBM is None:
+        import ctypes.util
+        _LIBM = C.CDLL(ctypes.util.find_library("m") or "libm.so.6")
+        _LIBM.expf.restype = C.c_float
+        _LIBM.expf.argtypes = [C.c_float]
+    return F32(_LIBM.expf(C.c_float(x)))
+
+
+def _ifix(x):
+    return int(np.trunc(F32(x)))
+
+
+def _qtypea(s):
+    """molecom.qtypea: per atom type, the Q-type flag of its first atom (1 for labels
+    starting with Q, ato_manipulation_rev4.py:269)"""
+    out = np.zeros(s.ntype, np.int32)
+    if getattr(s, "intra", None) is not None:
+        seen = set()
+        for ia, it in enumerate(np.asarray(s.itype)):
+            if int(it) not in seen:
+                seen.add(int(it))
+                out[int(it)] = int(s.intra.qtype[ia])
+    return out
+
+
+def xray_form_factors(q, nq, w, labels, ntype):
+    """wxtype (epsr_standalone_rev3.py:4571-4610) for every record of X-ray weights file w:
+    f0x, fm0x float64 [nq + 1, ntype] (Q index 1..nq; types without a record stay 0)."""
+    f0x = np.zeros((nq + 1, ntype))
+    fm0x = np.zeros((nq + 1, ntype))
+    pi4 = 16.0 * float(np.arctan(1.0))
+    import math
+    for r in w.records:
+        n1 = max([k for k, lab in enumerate(labels) if lab == r.a1], default=-1)
+        if n1 < 0:
+            continue
+        for iq in range(1, nq + 1):
+            qx = float(q[iq]) / pi4
+            qx2 = qx * qx
+            sm = r.cxt
+            for a, b in zip(r.axt, r.bxt):
+                term = -b * qx2
+                if -40.0 < term < 40.0:
+                    sm = sm + a * math.exp(term)
+            f0x[iq, n1] = sm
+            fac = F32(1.0)
+            if r.alphaxt != 0.0 and r.deltaxt > 0.0:
+                expon = F32(F32(q[iq] * q[iq]) / F32(F32(F32(2.0) * r.deltaxt) * r.deltaxt))
+                if expon < 20.0:
+                    fac = F32(1.0 - (float(r.alphaxt) / f0x[1, n1]) * float(_expf(-expon)))
+            fm0x[iq, n1] = float(fac) * sm
+    return f0x, fm0x
+
+
+def xray_weights(nq, w, f0x, fm0x, s, qtypea, cfraccg):
+    """wxcreate (epsr_standalone_rev3.py:4613-4705) for one X-ray file: the Q-dependent pair
+    weights weightx float64 [nq + 1, npairs] and the self-scattering shift backx [nq + 1]
+    (before get_wts scales weightx by the concentrations)."""
+    nrho, normx, iafformaff = _ifix(w.rhofac), int(w.normx), int(w.iafformaff)
+    amolrat = F32(F32(s.natomr) / F32(s.nmol))
+    weightx = np.zeros((nq + 1, s.npairs))
+    backx = np.zeros(nq + 1)
+    for iq in range(1, nq + 1):
+        singlem = F32(0.0)
+        singlei = F32(0.0)
+        bbar = 0.0
+        for it in range(s.ntype):
+            if qtypea[it] != 0:
+                continue
+            cf = float(cfraccg[it])
+            fm, f0 = fm0x[iq, it], f0x[iq, it]
+            singlem = F32(float(singlem) + cf * (fm * fm))
+            singlei = F32(float(singlei) + cf * (f0 * f0))
+            f = fm if iafformaff == 2 else f0
+            if normx == 0:
+                bbar = bbar + cf
+            elif normx == 1:
+                bbar = bbar + cf * f
+            elif normx == 2:
+                bbar = bbar + cf * (f * f)
+        if normx == 1:
+            bbar = bbar * bbar
+        for ip in range(s.npairs):
+            n1, n2 = int(s.n1extr[ip]), int(s.n2extr[ip])
+            weightx[iq, ip] = fm0x[iq, n1] * fm0x[iq, n2] / bbar
+            if nrho == 2 and normx == 1:
+                weightx[iq, ip] = weightx[iq, ip] / float(amolrat)
+        backx[iq] = float(F32(singlem - singlei)) / bbar if iafformaff == 1 else 0.0
+        if nrho == 2 and normx == 1:
+            backx[iq] = backx[iq] / float(amolrat)
+    return weightx, backx
+
+
+def get_wts(ctx, s, wts, feedback, efacm, ran1_state, cwtt=None, recompute=True, q=None, nq=None):
+    """get_wts (epsr_standalone_rev3.py:4180-4479) for box `s` and the parsed weights files
+    `wts` (formats.read_wts, one per dataset, neutron or X-ray): per dataset and type pair
+    the weights ciwt (intra) and cwt (inter) -- for X-ray data the Q = 0 values of the
+    Q-dependent weights -- wsum = sum |cwt| and xwt = sum cwt, and, when `recompute`
+    (ireset = 1 or feedback/efacm changed), the inverted matrix through invert_cwt from
+    cwtt = [0 | I] (else from the given cwtt; with neither, cwtpot is None): at one Q, or
+    with X-ray data at every 2/qstep-th Q point (at most 15), continuing from the previous
+    point's matrix.  X-ray data need the Q grid q [nq + 1] (float32).
+    Returns a dict with ciwt, cwt [nfile, npairs] (float32), wsum, xwt [nfile], cwtt
+    [npairs, npairs + nfile] and cwtpot [npairs, nfile] (float64), and for X-ray data
+    xfile [nfile] (0 neutron, else the 1-based X-ray file), weightx float64 [nxfile, nq + 1,
+    npairs] (scaled by the concentrations), backx [nxfile, nq + 1], f0x/fm0x [nxfile,
+    nq + 1, ntype], qinv (1-based Q points of the inversion) and cwttq [nqinv, npairs,
+    npairs + nfile]."""
     nfile, npairs = len(wts), s.npairs
     labels = [fa3(t) for t in s.type_names]
     weight = np.zeros((2, nfile, npairs), F32)
+    cfraccg = s.cfraccg if s.cfraccg is not None else s.cfrac
+    xfile = np.zeros(nfile, np.int32)
+    xr = []                                             # per X-ray file: (weightx, backx, f0x, fm0x)
     for i, w in enumerate(wts):
+        if getattr(w, "xray", False):
+            if q is None or nq is None:
+                raise ValueError("get_wts: X-ray weights need the Q grid (q, nq)")
+            if len(xr) >= 2:                            # x_rayf0.mxfile: further X-ray files get no weights
+                continue
+            f0x, fm0x = xray_form_factors(q, nq, w, labels, s.ntype)
+            wx, bx = xray_weights(nq, w, f0x, fm0x, s, _qtypea(s), cfraccg)
+            xr.append((wx, bx, f0x, fm0x))
+            xfile[i] = len(xr)
+            weight[0, i, :] = wx[1, :].astype(F32)     # the Q = 0 weights (wxcreate :4690-4693)
+            weight[1, i, :] = weight[0, i, :]
+            continue
         for a1, a2, w1, w2 in w.pairs:                 # wtype (:4540-4558)
             n1 = max([k + 1 for k, lab in enumerate(labels) if lab == a1], default=0)
             n2 = max([k + 1 for k, lab in enumerate(labels) if lab == a2], default=0)
             if n1 > 0 and n2 > 0:
                 ic = int(s.npairref[n1 - 1, n2 - 1])
                 weight[0, i, ic], weight[1, i, ic] = w1, w2
-    cfraccg = s.cfraccg if s.cfraccg is not None else s.cfrac
     ciwt = np.zeros((nfile, npairs), F32)
     cwt = np.zeros((nfile, npairs), F32)
     wsum = np.zeros(nfile, F32)
     xwt = np.zeros(nfile, F32)
+    cprods = np.zeros(npairs, F32)
+    for ic in range(npairs):
+        it, jt = int(s.n1extr[ic]), int(s.n2extr[ic])
+        cprod = F32(cfraccg[it]) * F32(cfraccg[jt])
+        cprods[ic] = F32(2.0) * cprod if it != jt else cprod
     for i in range(nfile):
         for ic in range(npairs):
-            it, jt = int(s.n1extr[ic]), int(s.n2extr[ic])
-            cprod = F32(cfraccg[it]) * F32(cfraccg[jt])
-            if it != jt:
-                cprod = F32(2.0) * cprod
-            ciwt[i, ic] = cprod * weight[0, i, ic]
-            cwt[i, ic] = cprod * weight[1, i, ic]
+            ciwt[i, ic] = cprods[ic] * weight[0, i, ic]
+            cwt[i, ic] = cprods[ic] * weight[1, i, ic]
             wsum[i] = F32(float(wsum[i]) + float(abs(cwt[i, ic])))
             xwt[i] = F32(float(xwt[i]) + float(cwt[i, ic]))
-    out = dict(ciwt=ciwt, cwt=cwt, wsum=wsum, xwt=xwt)
+    for wx, _, _, _ in xr:
+        wx *= cprods.astype(np.float64)[None, :]
+    out = dict(ciwt=ciwt, cwt=cwt, wsum=wsum, xwt=xwt, xfile=xfile)
+    if xr:
+        out["weightx"] = np.stack([x[0] for x in xr])
+        out["backx"] = np.stack([x[1] for x in xr])
+        out["f0x"] = np.stack([x[2] for x in xr])
+        out["fm0x"] = np.stack([x[3] for x in xr])
+        nqinvstep = _ifix(F32(2.0) / F32(q[2] - q[1]))
+    else:
+        nqinvstep = nq if nq is not None else 1
     nnew = npairs + nfile
     if recompute:
         cwtt = np.zeros((npairs, nnew))
         for ip in range(npairs):
             cwtt[ip, nfile + ip] = 1.0
-        cwtpot = np.zeros((nfile, npairs))
-        for i in range(nfile):
-            for ic in range(npairs):
-                it, jt = int(s.n1extr[ic]), int(s.n2extr[ic])
-                if cfraccg[it] > 0.0 and cfraccg[jt] > 0.0:
-                    rat = F32(s.cfrac[it]) * F32(s.cfrac[jt]) / (F32(cfraccg[it]) * F32(cfraccg[jt]))
-                else:
-                    rat = F32(0.0)
-                cwtpot[i, ic] = float(rat) * float(cwt[i, ic])
-        invert_cwt(ctx, cwtpot, cwtt, feedback, efacm, ran1_state)
+        rat = np.zeros(npairs, F32)
+        for ic in range(npairs):
+            it, jt = int(s.n1extr[ic]), int(s.n2extr[ic])
+            if cfraccg[it] > 0.0 and cfraccg[jt] > 0.0:
+                rat[ic] = F32(s.cfrac[it]) * F32(s.cfrac[jt]) / (F32(cfraccg[it]) * F32(cfraccg[jt]))
+        qinv, cwttq = [], []
+        for iqinv in range(1, (nq if nq is not None else 1) + 1, max(1, nqinvstep)):
+            if len(qinv) >= 15:                         # weights.mqinv
+                break
+            qinv.append(iqinv)
+            cwtpot = np.zeros((nfile, npairs))
+            for i in range(nfile):
+                for ic in range(npairs):
+                    wv = out["weightx"][xfile[i] - 1, iqinv, ic] if xfile[i] > 0 else float(cwt[i, ic])
+                    cwtpot[i, ic] = float(rat[ic]) * wv
+            invert_cwt(ctx, cwtpot, cwtt, feedback, efacm, ran1_state)
+            cwttq.append(cwtt.copy())
+        out["qinv"] = np.array(qinv, np.int32)
+        out["cwttq"] = np.stack(cwttq)
+        cwtt = cwttq[0]
     out["cwtt"] = cwtt
     if cwtt is None:           # no inversion asked for and none given
         out["cwtpot"] = None

@@ -2,7 +2,7 @@
 (src/epsr26/programs/epsr_standalone_rev3.py): inita's transform matrix (:3025-3071),
 initpt's truncation functions (:3755-3767), gensq (:4730-4802), expand (:4924-4939),
 fit_expansion (:4806-4920), calc_pwrexpq (:5100-5153), generate_rp (:5675-5872),
-generate_ep (:5875-6023) and ptgnsq (:3789-4021, neutron weights)."""
+generate_ep (:5875-6023) and ptgnsq (:3789-4021, neutron and X-ray weights)."""
 
 from __future__ import annotations
 
@@ -14,7 +14,7 @@ from .. import _native
 from .. import device as D
 from .. import pipeline as PL
 from .. import tables as T
-from ..common import clustercom, epsr16, gpu, molecom, potential_par, ran1com, weights
+from ..common import clustercom, epsr16, gpu, molecom, potential_par, ran1com, weights, x_rayf0
 
 F32 = np.float32
 SUMMAX = D.SUMMAX
@@ -164,11 +164,11 @@ def ptgnsq(nsum, nq, q, wav, data, qwind, rminpt, rmaxpt, szeros, overallfac, gf
     """Refine the empirical potential from data minus the weighted intermolecular
     partials qwind(i, ic): per dataset the difference, a Poisson-basis fit of it
     (fit_expansion), the update of pcoefd/pcoefp, then generate_ep.  Neutron weights
-    only (weights.xfile = 0).  potnoi(id) receives chi^2.  Returns the r-factors."""
+    for X-ray datasets (weights.xfile(id) = ix > 0) the Q-dependent weights
+    x_rayf0.weightx(i, ic, ix).  potnoi(id) receives chi^2.  Returns the r-factors."""
     c = gpu.ctx()
     nfile, npairs = epsr16.nfile, molecom.npairs
-    if np.any(weights.xfile[1:nfile + 1] > 0):
-        raise NotImplementedError("ptgnsq: X-ray weights are not part of this package")
+    xfile = np.ascontiguousarray(weights.xfile[1:nfile + 1], np.int32)
     ldq = int(nq) + 1
     qminptemp = epsr16.qminm if epsr16.qminp <= 0.0 else epsr16.qminp
     dq = c.tensor(np.asarray(q[:ldq], F32))
@@ -176,8 +176,14 @@ def ptgnsq(nsum, nq, q, wav, data, qwind, rminpt, rmaxpt, szeros, overallfac, gf
     dqw = c.tensor(_cols(qwind, npairs))
     cwt = np.ascontiguousarray(weights.cwt[1:nfile + 1, 1:npairs + 1], F32).reshape(-1)
     szr = np.asarray(szeros[1:nfile + 1], F32)
+    wx = None
+    if np.any(xfile > 0):
+        nx = int(xfile.max())
+        w = np.zeros((nx, ldq, npairs))
+        w[:, :, :] = np.transpose(x_rayf0.weightx[:ldq, 1:npairs + 1, 1:nx + 1], (2, 0, 1))
+        wx = c.tensor(np.ascontiguousarray(w).reshape(-1))
     qt = D.ptgnsq_diff(c, nfile, nq, npairs, dq, dd, ldq, dqw, cwt, szr, epsr16.nqmin[1:nfile + 1],
-                       epsr16.nqmax[1:nfile + 1], qminptemp)
+                       epsr16.nqmax[1:nfile + 1], qminptemp, weightx=wx, xfile=xfile)
     rfac = np.zeros(nfile + 1, F32)
     if not (potential_par.feedback > 0.0 and potential_par.potfac > 0.0):
         potential_par.npits = potential_par.npitss

@@ -501,14 +501,49 @@ int epsr_fit_expansion(epsr_ctx *ctx, int nchain, const float *yin, int ldy, con
                        float *rejrat, float *chisqnew, int nmin, int npt, int ncfirst, int nterms, int nits,
                        int nfreq, int32_t *ran1_state, int mode, int *ntries);
 
-/* ---- ptgnsq (epsr_standalone_rev3.py:3789-4021), neutron weights (cwt):
+/* ---- ptgnsq (epsr_standalone_rev3.py:3789-4021):
  * epsr_ptgnsq_diff forms qtemp(i, id) for every dataset (:3827-3867): q dev [nq+1],
  * data dev [nfile*ldq] (data[id*ldq+i]), qwind dev [npairs*ldq] (intermolecular
  * partials, qwind[ic*ldq+i]), cwt host [nfile*npairs], szeros host [nfile], nqmin/nqmax
- * host [nfile] (1-based), qtemp dev out [nfile*ldq]. */
+ * host [nfile] (1-based), qtemp dev out [nfile*ldq].  X-ray datasets (xfile host
+ * [nfile], 0 for neutron data, else the 1-based X-ray file, at most nxfile; NULL: none)
+ * take the Q-dependent weights weightx dev [nxfile*ldq*npairs] (double,
+ * weightx[((ix-1)*ldq + i)*npairs + ic], the reference's x_rayf0.weightx).  Each term of
+ * a weighted sum is added in double and the sum rounded to float, as the reference's
+ * float64 weight arrays make it. */
 int epsr_ptgnsq_diff(epsr_ctx *ctx, int nfile, int nq, int npairs, const float *q, const float *data, int ldq,
                      const float *qwind, const float *cwt, const float *szeros, const int *nqmin, const int *nqmax,
-                     float qminptemp, float *qtemp);
+                     float qminptemp, float *qtemp, const double *weightx, const int *xfile, int nxfile);
+/* ---- the data-minus-simulation block of the epsr program between the running averages
+ * and ptgnsq (epsr_standalone_rev3.py:1703-1830, with ftback :5330-5389), which the
+ * reference keeps inline in runepsr17.  Per dataset: the intramolecular partials (ciwt, or
+ * weightx for X-ray data) out of the data datai on its Q range nqmind..nqmaxd and out of
+ * the zero limit szeros; then data - intermolecular simulation (cwt or weightx) on the
+ * fitted range nqmin..nqmax with the Porod term (porodamp > 0) and the extrapolation to
+ * Q = 0 (szeros non-zero); with nsmoop > 0 the part of that difference from r < rshmin
+ * (ftback) is subtracted from the data.  Outputs datat dev [nfile*ldq] and szerostemp dev
+ * [nfile] (what the program hands ptgnsq as data and szeros); dts dev [ldq] is work
+ * space; grt dev [npt1+1] is the program's r-indexed work array, which keeps what earlier
+ * calls left in the entries a call does not write (zero it once).  sinf and powf are
+ * correctly rounded.  Synchronises the stream. */
+typedef struct {
+    int nfile, nq, ldq, npairs, nxfile;
+    const float *q;                  /* dev [ldq] */
+    const float *datai;              /* dev [nfile*ldq] */
+    const float *soqintra, *soqinter;/* dev [npairs*ldq], the running averages */
+    const double *weightx;           /* dev [nxfile*ldq*npairs] or NULL */
+    const float *x;                  /* dev [npt1+1], the r grid */
+    const float *ciwt, *cwt;         /* host [nfile*npairs] */
+    const int *xfile;                /* host [nfile] */
+    const int *nqmind, *nqmaxd, *nqmin, *nqmax;   /* host [nfile], 1-based */
+    const float *szeros, *porodamp, *rshmin;       /* host [nfile] */
+    float porodq, porodpwr;
+    int nsmoop, npt1;
+    float rho;
+} epsr_dms_desc_t;
+int epsr_data_minus_sim(epsr_ctx *ctx, const epsr_dms_desc_t *desc, float *datat, float *szerostemp, float *dts,
+                        float *grt);
+
 /* rfactorep of one dataset over i0..i1 (:3977-3991); fit, qtemp dev; rfactor host out. */
 int epsr_ptgnsq_rfactor(epsr_ctx *ctx, int i0, int i1, const float *fit, const float *qtemp, float *rfactor);
 /* the coefficient update of dataset id (:3962-3975), host arrays: pcoefd[id*ldp+ic],

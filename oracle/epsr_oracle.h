@@ -305,6 +305,30 @@ void  ora_ptgnsq_diff(int nfile, int nq, int npairs, const float *q, const float
                       const int *nqmax, float qminptemp, float *qtemp);
 float ora_ptgnsq_rfactor(int i0, int i1, const float *fit, const float *qt);
 
+/* the data-minus-simulation block of the epsr program (epsr_standalone_rev3.py:1703-1830)
+ * and ftback (:5330-5389).  Datasets 0..nfile-1; Q index 1..nq in rows of ldq; partial ip
+ * of soqintra/soqinter at [ip*ldq + iq] (the running averages); ciwt/cwt [nfile*npairs]
+ * (neutron datasets); weightx [nx*ldq*npairs] double, [((ix-1)*ldq + iq)*npairs + ip];
+ * xfile[id]: 0 neutron, else the 1-based X-ray index; x [npt1+1] the r grid. */
+typedef struct {
+    int nfile, nq, ldq, npairs;
+    const float *q, *datai, *soqintra, *soqinter, *ciwt, *cwt;
+    const double *weightx;
+    const int *xfile, *nqmind, *nqmaxd, *nqmin, *nqmax;
+    const float *szeros, *porodamp, *rshmin;
+    float porodq, porodpwr;
+    int nsmoop;
+    const float *x;
+    int npt1;
+    float rho;
+} ora_dms_t;
+void ora_ftback(int nqmin, int nqmax, const float *q, const float *x, int npt1, float rho, float rmin, float *sofq,
+                float *grt);
+void ora_data_minus_sim(const ora_dms_t *d, float *datat, float *szerostemp, float *dts, float *grt);
+void ora_ptgnsq_diff_x(int nfile, int nq, int npairs, const float *q, const float *data, int ldq, const float *qwind,
+                       const float *cwt, const double *weightx, const int *xfile, const float *szeros,
+                       const int *nqmin, const int *nqmax, float qminptemp, float *qtemp);
+
 #ifdef __cplusplus
 }
 #endif

@@ -598,34 +598,10 @@ void ora_ptgnsq_diff(int nfile, int nq, int npairs, const float *q, const float 
                      const float *qwind, const float *cwt, const float *szeros, const int *nqmin,
                      const int *nqmax, float qminptemp, float *qtemp)
 {
-    for (int id = 0; id < nfile; ++id) {
-        const float *dat = data + (size_t)id * ldq;
-        int nqmn = nqmin[id];
-        float grads0 = 0.0f, conss0 = 0.0f;
-        if (szeros[id] != 0.0f && nqmn > 1) {
-            grads0 = (dat[nqmn] - szeros[id]) / (q[nqmn] - q[1]);
-            conss0 = szeros[id];
-        }
-        for (int i = 1; i <= nq; ++i) {
-            float sum = 0.0f;
-            for (int ic = 0; ic < npairs; ++ic)
-                sum = sum + cwt[id * npairs + ic] * qwind[(size_t)ic * ldq + i];
-            float qlim = qminptemp > q[nqmn] ? qminptemp : q[nqmn];
-            float v;
-            if (q[i] < qlim) {
-                if (szeros[id] != 0.0f) v = conss0 + grads0 * (q[i] - q[1]) - sum;
-                else v = 0.0f;
-            } else if (i >= nqmn && i <= nqmax[id]) {
-                v = dat[i] - sum;
-            } else {
-                v = 0.0f;
-            }
-            qtemp[(size_t)id * ldq + i] = v;
-        }
-    }
+    ora_ptgnsq_diff_x(nfile, nq, npairs, q, data, ldq, qwind, cwt, NULL, NULL, szeros, nqmin, nqmax, qminptemp,
+                      qtemp);
 }
 
-/* epsr_standalone_rev3.py:3977-3991 */
 float ora_ptgnsq_rfactor(int i0, int i1, const float *fit, const float *qt)
 {
     float r = 0.0f;

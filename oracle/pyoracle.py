@@ -118,6 +118,14 @@ class MCFull(C.Structure):
                 ("rejrate", C.c_float), ("tempangwt", C.c_float), ("tempdihwt", C.c_float), ("ireset", C.c_int)]
 
 
+class DMS(C.Structure):
+    _fields_ = [("nfile", C.c_int), ("nq", C.c_int), ("ldq", C.c_int), ("npairs", C.c_int), ("q", fp),
+                ("datai", fp), ("soqintra", fp), ("soqinter", fp), ("ciwt", fp), ("cwt", fp), ("weightx", dp),
+                ("xfile", ip), ("nqmind", ip), ("nqmaxd", ip), ("nqmin", ip), ("nqmax", ip), ("szeros", fp),
+                ("porodamp", fp), ("rshmin", fp), ("porodq", C.c_float), ("porodpwr", C.c_float),
+                ("nsmoop", C.c_int), ("x", fp), ("npt1", C.c_int), ("rho", C.c_float)]
+
+
 class Grid(C.Structure):
     _fields_ = [("npt", C.c_int), ("npt1", C.c_int), ("npt2", C.c_int), ("nptmin", C.c_int),
                 ("nptmax", C.c_int)]
@@ -191,6 +199,10 @@ def _declare(L):
         "ora_ptgnsq_diff": (None, [C.c_int, C.c_int, C.c_int, fp, fp, C.c_int, fp, fp, fp, ip, ip, C.c_float,
                                    fp]),
         "ora_ptgnsq_rfactor": (C.c_float, [C.c_int, C.c_int, fp, fp]),
+        "ora_ptgnsq_diff_x": (None, [C.c_int, C.c_int, C.c_int, fp, fp, C.c_int, fp, fp, dp, ip, fp, ip, ip,
+                                     C.c_float, fp]),
+        "ora_ftback": (None, [C.c_int, C.c_int, fp, fp, C.c_int, C.c_float, C.c_float, fp, fp]),
+        "ora_data_minus_sim": (None, [P(DMS), fp, fp, fp, fp]),
         "ora_da_uniform": (C.c_float, [C.c_ulonglong, C.c_int]),
         "ora_molfit_da": (C.c_int, [P(Box), fp, fp, fp, P(Centres), P(Tables), P(Cells), P(Ledger),
                                     P(MCState), P(Ran1), fp, ip, fp, C.c_ulonglong, ip, dp, fp]),
