@@ -74,24 +74,124 @@ __device__ __forceinline__ float atom_pair(float x1, float y1, float z1, float x
     return sume;
 }
 
-// en_mole_pair (molfit.py:1527-1578): sequential sum, molecule 1 outer, molecule 2 inner
-__device__ __forceinline__ float mol_pair(const float4 *__restrict__ P1, int na1, const float4 *__restrict__ P2,
-                                          int nb2, const PairTab &t)
+// The same arithmetic as atom_pair, split in three steps so that a group of pairs has
+// all its table reads in flight at once: the bins (no branch but a rare fix-up), the
+// table reads, then the energies.
+// pair_bin_fast: dtest (sqrt_rn's correction, branch-free) and the bin from the rounded
+// reciprocal; *ok is false when the reciprocal cannot settle the bin or dsq lies outside
+// sqrt_rn's fast range -- those pairs take pair_bin_exact.
+__device__ __forceinline__ int pair_bin_fast(float dsq, const PairTab &t, float &dtest, bool &ok)
 {
-    float sume = 0.0f;
-    for (int ia = 0; ia < na1; ++ia) {
-        float4 p = P1[ia];
-        int it = __float_as_int(p.w) * t.ntype;
-        for (int ja = 0; ja < nb2; ++ja) {
-            float4 q = P2[ja];
-            sume = sume + atom_pair(p.x, p.y, p.z, q.x, q.y, q.z, t.npr[it + __float_as_int(q.w)], t);
+    const float s = __builtin_amdgcn_sqrtf(dsq);
+    const float sdn = __int_as_float(__float_as_int(s) - 1), sup = __int_as_float(__float_as_int(s) + 1);
+    const float rdn = fmaf(-sdn, s, dsq), rup = fmaf(-sup, s, dsq);
+    const float r = rdn <= 0.0f ? sdn : s;
+    dtest = rup > 0.0f ? sup : r;
+    const float q = dtest * t.inv_stepcl;
+    const float tq = truncf(q), fq = q - tq, mq = 1.2e-6f * q;
+    ok = dsq >= 0x1p-96f && dsq < INFINITY && fq > mq && fq < 1.0f - mq && q < 4194304.0f;
+    return (int)tq + 2;
+}
+
+// atom_pair's division path: dtest and the bin, INT_MAX (no table entry) where atom_pair
+// gives 0 outright
+struct PairBin {
+    float dtest;
+    int iz;
+};
+__device__ __forceinline__ PairBin pair_bin_exact(float dsq, const PairTab &t)
+{
+    const float d = sqrt_rn(dsq);
+    const float rat = d / t.stepcl;
+    return {d, rat < 2.0e9f ? (int)truncf(rat) + 2 : 0x7fffffff};
+}
+
+// G atom pairs, member g pairing p[g / NQ] with q[g % NQ] (valid: bit g), added to sume
+// in the order of g
+template <int G, int NQ>
+__device__ __forceinline__ float pair_block(float sume, const float4 *p, const float4 *q, unsigned valid,
+                                            const PairTab &t)
+{
+    float dsq[G], dt[G];
+    int iz[G], ic[G];
+    bool fix = false;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        dsq[g] = 0.0f;
+        dt[g] = 0.0f;
+        iz[g] = 0x7fffffff;
+        ic[g] = 0;
+        if ((valid >> g) & 1u) {
+            const float4 &pa = p[g / NQ], &qa = q[g % NQ];
+            ic[g] = t.npr[__float_as_int(pa.w) * t.ntype + __float_as_int(qa.w)];
+            const float xd = mic_sq(qa.x - pa.x), yd = mic_sq(qa.y - pa.y), zd = mic_sq(qa.z - pa.z);
+            dsq[g] = xd * xd + yd * yd + zd * zd;
+            bool ok;
+            iz[g] = pair_bin_fast(dsq[g], t, dt[g], ok);
+            if (!ok) {
+                iz[g] = -1;   // marks the pair for the exact bin below
+                fix = true;
+            }
+        }
+    }
+    if (fix) {
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+            if (iz[g] < 0) {
+                const PairBin b = pair_bin_exact(dsq[g], t);
+                dt[g] = b.dtest;
+                iz[g] = b.iz;
+            }
+    }
+    float4 v[G];
+    float xs[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        const bool in = iz[g] <= t.npt2 && iz[g] < t.nte;
+        const bool near = iz[g] < t.nnear;
+        v[g] = t.T[in && near ? ic[g] * t.nnear + iz[g] : 0];
+        xs[g] = t.xs[in ? iz[g] : 0];
+        if (in && !near) {
+            const float2 f = t.F[ic[g] * (t.nte - t.nnear) + (iz[g] - t.nnear)];
+            v[g] = make_float4(0.0f, 0.0f, f.x, f.y);   // the stored +0 of xavs, xavs1
+        }
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        if ((valid >> g) & 1u) {
+            float e = 0.0f;
+            if (iz[g] <= t.npt2 && iz[g] < t.nte) {
+                const float dtdif = dt[g] - xs[g];
+                e = v[g].x + v[g].y * dtdif;
+                e = e + v[g].z + v[g].w * dtdif;
+            }
+            sume = sume + e;
         }
     }
     return sume;
 }
 
-// mol_pair for molecules of at most three atoms: the six positions are loaded
-// before any pair is evaluated; the sum runs in the same order
+// en_mole_pair (molfit.py:1527-1578): sequential sum, molecule 1 outer, molecule 2 inner;
+// each atom of molecule 1 against molecule 2's atoms four at a time (pair_block)
+__device__ __forceinline__ float mol_pair(const float4 *__restrict__ P1, int na1, const float4 *__restrict__ P2,
+                                          int nb2, const PairTab &t)
+{
+    float sume = 0.0f;
+    for (int ia = 0; ia < na1; ++ia) {
+        const float4 p = P1[ia];
+        for (int j0 = 0; j0 < nb2; j0 += 4) {
+            float4 q[4];
+            const int ng = min(4, nb2 - j0);
+#pragma unroll
+            for (int g = 0; g < 4; ++g) q[g] = g < ng ? P2[j0 + g] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            sume = pair_block<4, 4>(sume, &p, q, (1u << ng) - 1u, t);
+        }
+    }
+    return sume;
+}
+
+// mol_pair for molecules of at most three atoms: the six positions are loaded before any
+// pair is evaluated, and each atom of molecule 1 meets the three of molecule 2 as a group
 __device__ __forceinline__ float mol_pair3(const float4 *__restrict__ P1, int n1, const float4 *__restrict__ P2,
                                            int n2, const PairTab &t)
 {
@@ -101,17 +201,12 @@ __device__ __forceinline__ float mol_pair3(const float4 *__restrict__ P1, int n1
         p[j] = j < n1 ? P1[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         q[j] = j < n2 ? P2[j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
+    const unsigned rowm = (1u << n2) - 1u;
     float sume = 0.0f;
 #pragma unroll
     for (int ia = 0; ia < 3; ++ia) {
         if (ia >= n1) break;
-        const int it = __float_as_int(p[ia].w) * t.ntype;
-#pragma unroll
-        for (int ja = 0; ja < 3; ++ja) {
-            if (ja >= n2) break;
-            sume = sume + atom_pair(p[ia].x, p[ia].y, p[ia].z, q[ja].x, q[ja].y, q[ja].z,
-                                    t.npr[it + __float_as_int(q[ja].w)], t);
-        }
+        sume = pair_block<3, 3>(sume, &p[ia], q, rowm, t);
     }
     return sume;
 }
