@@ -393,8 +393,32 @@ class Refinement:
         D.sumsq(ctx, s.npairs, g.nq, self.ldq, self.sofq, fc1, fc2, self.soq, self.sq2)
         D.sumgr(ctx, s.npairs, g.npt1, self.ldw, self.wav, self.ycoh, fc1, fc2, self.gor, self.gr2, self.entsig)
 
+    def data_minus_sim(self, datai, wts, nqmind, nqmaxd, nqmin, nqmax, szeros, porodamp, rshmin, porodq=0.0,
+                       porodpwr=0.0, nsmoop=1):
+        """The data-minus-simulation block of the epsr program (epsr_standalone_rev3.py:
+        1703-1830) on the running averages (accumulate first): datai float32 [nfile, nq + 1]
+        (the data on this box's Q grid), wts = get_wts(...) (ciwt, cwt, xfile, weightx),
+        the per-dataset Q ranges (1-based), zero limits, Porod amplitudes and ftback radii.
+        Returns (datat, szerostemp) on the device: the data and zero limits ptgnsq takes."""
+        ctx, s, g = self.ctx, self.s, self.g
+        nfile, npairs, ldq = datai.shape[0], s.npairs, self.ldq
+        dd = np.zeros((nfile, ldq), F32)
+        dd[:, :min(ldq, datai.shape[1])] = np.asarray(datai, F32)[:, :ldq]
+        if getattr(self, "grt", None) is None:
+            self.grt = ctx.zeros(g.npt1 + 1)     # the program's r-indexed work array of ftback
+        wx = wts.get("weightx")
+        dwx = None
+        if wx is not None:
+            w = np.zeros((wx.shape[0], ldq, npairs))
+            w[:, :wx.shape[1], :] = wx
+            dwx = ctx.tensor(w.reshape(-1))
+        return D.data_minus_sim(ctx, self.q, ctx.tensor(dd.reshape(-1)), ldq, g.nq, self.soq[:npairs * ldq],
+                                self.soq[npairs * ldq:2 * npairs * ldq], wts["ciwt"], wts["cwt"], nqmind, nqmaxd,
+                                nqmin, nqmax, szeros, porodamp, rshmin, porodq, porodpwr, nsmoop, self.x, g.npt1,
+                                s.rho, self.grt, weightx=dwx, xfile=wts.get("xfile"))
+
     def ptgnsq(self, data, cwt, cwtpot, nqmin=None, szeros=None, potfac=1.0, fit_mode=None, npits=1000,
-               paccept=5.0e-4, pstepd=0.002):
+               paccept=5.0e-4, pstepd=0.002, weightx=None, xfile=None, nqmax=None):
         """One refinement of the empirical potential from the running intermolecular S(Q)
         (accumulate first), as ptgnsq does it (epsr_standalone_rev3.py:3789-4021, neutron
         weights): per dataset the difference data - sum_ic cwt soqinter, a Poisson-basis fit
@@ -402,19 +426,30 @@ class Refinement:
         data: float32 [nfile, nq + 1] (Q points 1..nq); cwt float32 [nfile, npairs];
         cwtpot float64/32 [npairs, nfile] (get_wts).  Returns (r-factors, chi^2) per dataset."""
         ctx, s, g = self.ctx, self.s, self.g
-        nfile, npairs, nq, ldq = data.shape[0], s.npairs, g.nq, self.ldq
+        nfile, npairs, nq, ldq = np.asarray(cwt).shape[0], s.npairs, g.nq, self.ldq
         fit_mode = _native.FIT_MC if fit_mode is None else fit_mode
         nterms = s.nterms
         if getattr(self, "pcoefd", None) is None:
             self.pcoefd = np.zeros((nfile + 1, nterms + 1), F32)
             self.pstepd = np.full(nfile + 1, F32(pstepd), F32)
         nqmin = np.full(nfile, 2, np.int32) if nqmin is None else np.asarray(nqmin, np.int32)
-        nqmax = np.full(nfile, nq, np.int32)
+        nqmax = np.full(nfile, nq, np.int32) if nqmax is None else np.asarray(nqmax, np.int32)
+        if isinstance(szeros, torch.Tensor):
+            szeros = szeros.cpu().numpy()
         szeros = np.zeros(nfile, F32) if szeros is None else np.asarray(szeros, F32)
-        dd = ctx.tensor(np.ascontiguousarray(np.asarray(data, F32)[:, :ldq]).reshape(-1))
+        if isinstance(data, torch.Tensor):       # data_minus_sim's datat, [nfile * ldq] on the device
+            dd = data.reshape(-1)
+        else:
+            dd = ctx.tensor(np.ascontiguousarray(np.asarray(data, F32)[:, :ldq]).reshape(-1))
         qwind = self.soq[npairs * ldq:2 * npairs * ldq]
         cw = np.ascontiguousarray(np.asarray(cwt, F32)).reshape(-1)
-        qt = D.ptgnsq_diff(ctx, nfile, nq, npairs, self.q, dd, ldq, qwind, cw, szeros, nqmin, nqmax, F32(0.0))
+        dwx = None
+        if weightx is not None:
+            w = np.zeros((weightx.shape[0], ldq, npairs))
+            w[:, :weightx.shape[1], :] = weightx
+            dwx = ctx.tensor(w.reshape(-1))
+        qt = D.ptgnsq_diff(ctx, nfile, nq, npairs, self.q, dd, ldq, qwind, cw, szeros, nqmin, nqmax, F32(0.0),
+                           weightx=dwx, xfile=xfile)
         pcp = np.zeros((npairs + 1, nterms + 1), F32)
         pcp[1:] = self.pcoefp
         irefp = np.arange(1, npairs + 1, dtype=np.int32)
