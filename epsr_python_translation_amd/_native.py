@@ -156,6 +156,7 @@ SIGNATURES = {
     "epsr_ctx_set_stream": (C.c_int, [vp, vp]),
     "epsr_ctx_synchronize": (C.c_int, [vp]),
     "epsr_ctx_last_kernel_ms": (C.c_double, [vp, C.c_char_p]),
+    "epsr_ctx_kernel_ms_sum": (C.c_double, [vp, C.c_char_p, C.POINTER(C.c_longlong)]),
     "epsr_ctx_enable_timing": (C.c_int, [vp, C.c_int]),
     "epsr_box_create": (C.c_int, [vp, C.POINTER(BoxDesc), C.POINTER(vp)]),
     "epsr_box_destroy": (None, [vp]),
@@ -188,6 +189,8 @@ SIGNATURES = {
                              C.POINTER(C.c_double), C.c_int, fp, fp]),
     "epsr_molfit": (C.c_int, [vp, vp, C.POINTER(Coords), C.POINTER(Tables), vp, C.POINTER(MCParams), fp, fp, ip, ip,
                               vp, i32p, fp, C.POINTER(MCStats)]),
+    "epsr_molfit_cycles": (C.c_int, [vp, vp, C.POINTER(Coords), C.POINTER(Tables), vp, C.POINTER(MCParams), C.c_int,
+                                     fp, fp, ip, ip, vp, i32p, fp, C.POINTER(MCStats)]),
     "epsr_molfit_da": (C.c_int, [vp, vp, C.POINTER(Coords), C.POINTER(Tables), vp, C.POINTER(MCParams), fp, fp, ip,
                                  ip, vp, i32p, fp, C.POINTER(DA), C.POINTER(MCRecord), C.POINTER(MCStats)]),
     "epsr_box_set_intra": (C.c_int, [vp, C.POINTER(IntraDesc)]),

@@ -14,10 +14,14 @@
 
 // ------------------------------------------------------------------ context
 
+// event pairs of the launches of one kernel family not yet read (n of them), the
+// duration of the last one read, and the sum and count of those read since the last sum
 struct KernelTimer {
-    hipEvent_t start = nullptr, stop = nullptr;
-    bool pending = false;
+    std::vector<hipEvent_t> start, stop;
+    int n = 0;
     double last_ms = -1.0;
+    double sum_ms = 0.0;
+    long long count = 0;
 };
 
 struct epsr_ctx {
@@ -119,9 +123,15 @@ struct epsr_ledger {
     // a host thread drawing window gen_buf (gen_len values) for the cycle after next
     std::thread gen;
     int gen_buf = -1, gen_len = 0;
+    hipError_t gen_err = hipSuccess;   // of the thread's upload, when it uploads
+    hipStream_t up_stream = nullptr;   // the stream of such uploads
     // pinned block of a cycle's small transfers (results down, step sizes up)
     int *hio = nullptr;
     size_t hio_cap = 0;
+    // cycles run back to back (epsr_molfit_cycles): the device block of constants, step
+    // sizes and per-cycle results
+    int *d_bres = nullptr;
+    size_t d_bres_cap = 0;
     bool ledger_ready = false;
     int maxcnt = 0;   // the longest list
     // table extents measured for the arrays in tab_ptr (xstar, xavs, xavs1, ecoul, ecoul1)

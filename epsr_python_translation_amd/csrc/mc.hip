@@ -65,12 +65,12 @@ extern "C" void epsr_ledger_destroy(epsr_ledger *l)
     hipFree(l->d_ergpair);
     hipFree(l->d_roff); hipFree(l->d_toff); hipFree(l->d_moff);
     gen_join(l);
+    // no upload may still read the pinned windows
+    if (l->up_stream) hipStreamSynchronize(l->up_stream);
+    if (l->ev_up) hipEventSynchronize(l->ev_up);
     for (float *q : l->pin)
         if (q) hipHostFree(q);
-    if (l->ev_up) {
-        hipEventSynchronize(l->ev_up);
-        hipEventDestroy(l->ev_up);
-    }
+    if (l->ev_up) hipEventDestroy(l->ev_up);
     if (l->hio) hipHostFree(l->hio);
     for (float *q : l->d_stream) hipFree(q);
     if (l->hstage) hipHostFree(l->hstage);
@@ -83,6 +83,8 @@ extern "C" void epsr_ledger_destroy(epsr_ledger *l)
     hipFree(l->d_tab);
     if (l->ev_stage) hipEventDestroy(l->ev_stage);
     for (void *w : l->wbuf) hipFree(w);
+    hipFree(l->d_bres);
+    if (l->up_stream) hipStreamDestroy(l->up_stream);
     delete l;
 }
 
@@ -1693,11 +1695,14 @@ mc_fdiff_accepted(int nmol, const int *__restrict__ order, const int *__restrict
                   const int *__restrict__ cnt, const int *__restrict__ pnv, const int *__restrict__ mw,
                   const float *__restrict__ A, const float *__restrict__ Bv,
                   const float *__restrict__ L, const unsigned *__restrict__ bits, int stride,
-                  float *__restrict__ fdiff_out)
+                  float *__restrict__ fdiff_out, unsigned *__restrict__ bits_out, int nwords)
 {
     extern __shared__ float2 fterm[];   // [4 waves][stride >= longest list]
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int k = blockIdx.x * 4 + wv;
+    // a copy of the acceptance bits beside the fdiffs, when asked for (nwords <= nmol)
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (bits_out && g < nwords) bits_out[g] = bits[g];
     if (k >= nmol) return;
     if (!((bits[k >> 5] >> (k & 31)) & 1u)) return;
     const int m = order[k];
@@ -1938,7 +1943,7 @@ mc_record_target(int nmol, const int *__restrict__ order, const int *__restrict_
     if (lane == 0) target[k] = (shift && isnan(shift[k])) ? __int_as_float(0x7fc00000) : (float)s;
 }
 
-static float adapt_ratio(float rejrate, int att, int rej)
+__host__ __device__ static float adapt_ratio(float rejrate, int att, int rej)
 {
     float rat;
     if (rej > 0) {
@@ -1988,22 +1993,93 @@ static int window_offset(const epsr_ledger *l, int b, const int32_t *state, int 
 }
 
 // a host thread draws window buf (len values from `from`); the window is usable once
-// gen_finish has joined it
-static void gen_start(epsr_ledger *l, int buf, const Ran1 &from, int len)
+// gen_finish has joined it.  With upload, the thread also copies it to d_stream[buf] on
+// the ledger's upload stream and records ev_up there (both exist before the call).
+static void gen_start(epsr_ledger *l, int buf, const Ran1 &from, int len, bool upload = false)
 {
     l->gen_buf = buf;
     l->gen_len = len;
+    l->gen_err = hipSuccess;
     l->win_len[buf] = 0;
     float *out = l->pin[buf];
     std::vector<int32_t> *ck = &l->win_ck[buf];
-    l->gen = std::thread([from, out, len, ck]() { fill_window(from, out, len, *ck); });
+    float *dst = upload ? l->d_stream[buf] : nullptr;
+    hipStream_t us = l->up_stream;
+    hipEvent_t ev = l->ev_up;
+    hipError_t *err = &l->gen_err;
+    const int dev = l->ctx->device;
+    l->gen = std::thread([from, out, len, ck, dst, us, ev, err, dev]() {
+        fill_window(from, out, len, *ck);
+        if (!dst) return;
+        hipError_t e = hipSetDevice(dev);
+        if (e == hipSuccess) e = hipMemcpyAsync(dst, out, sizeof(float) * (size_t)len, hipMemcpyHostToDevice, us);
+        if (e == hipSuccess) e = hipEventRecord(ev, us);
+        *err = e;
+    });
 }
 
 static void gen_finish(epsr_ledger *l)
 {
     gen_join(l);
-    if (l->gen_buf >= 0) l->win_len[l->gen_buf] = l->gen_len;
+    if (l->gen_buf >= 0) {
+        l->win_len[l->gen_buf] = l->gen_len;
+        if (l->gen_err != hipSuccess) {
+            // its upload failed: the window is not used
+            l->win_len[l->gen_buf] = 0;
+            if (l->pin_ahead == l->gen_buf) l->pin_ahead = -1;
+        }
+    }
+    l->gen_err = hipSuccess;
     l->gen_buf = -1;
+}
+
+// Step-size adaptation after a cycle on the device (molfit.py:465-536, the host loop of a
+// single cycle restated), so that cycles follow each other without the host.  blk: the
+// cycle's result block (16 + 2 nmt ints; [9] the no-change flag, [16..] rejections per
+// type); att: attempts per type; w: the step sizes the cycle used, replaced by the next
+// cycle's (a zero step of a type that moves set to the minimum, molfit.py:375-392).  The
+// block and the adapted steps go to slot (16 + 2 nmt ints, then 2 nmt floats).  A cycle
+// whose trial moves did not all change committed nothing, and neither do the ones after
+// it: their steps are left alone.
+__global__ void mc_adapt(int nmt, const int *__restrict__ att, const int *__restrict__ type_nat,
+                         const int *__restrict__ blk, float *__restrict__ w, int *__restrict__ slot, float rejrate,
+                         int ireset, int stepr_on, int stepm_on, float rotstepmax, float rotstepmin,
+                         float trstepmax, float trstepmin)
+{
+    const int nblk = 16 + 2 * nmt;
+    for (int i = threadIdx.x; i < nblk; i += blockDim.x) slot[i] = blk[i];
+    if (blk[9] != 0) return;
+    float *after = (float *)(slot + nblk);
+    const int *rej = blk + 16;
+    for (int mt = threadIdx.x; mt < nmt; mt += blockDim.x) {
+        float wr = w[2 * mt], wt = w[2 * mt + 1];
+        if (ireset != 1) {
+            float rat = adapt_ratio(rejrate, att[2 * mt], rej[2 * mt]);
+            wr = wr * rat;
+            if (wr > rotstepmax) wr = rotstepmax;
+            if (wr < rotstepmin && wr != 0.0f) wr = rotstepmin;
+            rat = adapt_ratio(rejrate, att[2 * mt + 1], rej[2 * mt + 1]);
+            wt = wt * rat;
+            if (wt > trstepmax) wt = trstepmax;
+            if (wt < trstepmin && wt != 0.0f) wt = trstepmin;
+        }
+        after[2 * mt] = wr;
+        after[2 * mt + 1] = wt;
+        if (type_nat[mt] != 0) {
+            if (stepr_on && type_nat[mt] > 1) {
+                if (wr <= 0.0f) wr = rotstepmin;
+            } else {
+                wr = 0.0f;
+            }
+            if (stepm_on) {
+                if (wt <= 0.0f) wt = trstepmin;
+            } else {
+                wt = 0.0f;
+            }
+        }
+        w[2 * mt] = wr;
+        w[2 * mt + 1] = wt;
+    }
 }
 
 // the plan of the chain (molfit.py:367-403) on the device.  flags: 1 stepr > 0, 2 stepm > 0.
@@ -2036,16 +2112,37 @@ __global__ void mc_plan_offsets(int nmol, const int *__restrict__ order, const i
     base_moff[k] = t + ((flags & 2) ? 3 : 0);
 }
 
+// the most cycles one pass of epsr_molfit_cycles keeps on the device (more are run in turns)
+static const int MC_BATCH_MAX = 64;
+
 static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *c, const epsr_tables_t *tb,
-                       epsr_ledger *l, const epsr_mc_params_t *p, float *wholerot, float *wholetrans, int *nwhole,
-                       int *nmovem, int *nstepreject, int32_t *ran1_state, float *fadds, const epsr_da_t *da,
-                       const epsr_mc_record_t *rec, epsr_mc_stats_t *stats)
+                       epsr_ledger *l, const epsr_mc_params_t *p, int ncycles, float *wholerot, float *wholetrans,
+                       int *nwhole, int *nmovem, int *nstepreject, int32_t *ran1_state, float *fadds,
+                       const epsr_da_t *da, const epsr_mc_record_t *rec, epsr_mc_stats_t *stats)
 {
     if (!ctx) return EPSR_ERR_ARG;
     ARG_CHECK(ctx, box && c && tb && l && p && wholerot && wholetrans && nwhole && nmovem && nstepreject &&
                        ran1_state && fadds,
               "epsr_molfit: null argument");
     ARG_CHECK(ctx, l->ledger_ready, "epsr_molfit: no ledger (call epsr_fmolec1)");
+    ARG_CHECK(ctx, ncycles >= 1, "epsr_molfit_cycles: ncycles=%d", ncycles);
+    if (ncycles > 1) {
+        // cycles back to back need the parallel resolution and no surrogate stage or
+        // recording; otherwise, and beyond MC_BATCH_MAX cycles, they are run in turns
+        const bool par = sizeof(float2) * 4 * (size_t)std::max(1, l->maxcnt) <= 160 * 1024;
+        const int first = (da || rec || !par) ? 1 : std::min(ncycles, MC_BATCH_MAX);
+        if (first < ncycles) {
+            int rc0 = molfit_impl(ctx, box, c, tb, l, p, first, wholerot, wholetrans, nwhole, nmovem, nstepreject,
+                                  ran1_state, fadds, da, rec, stats);
+            if (rc0) return rc0;
+            return molfit_impl(ctx, box, c, tb, l, p, ncycles - first, wholerot, wholetrans, nwhole, nmovem,
+                               nstepreject, ran1_state, fadds, da, rec, stats);
+        }
+    }
+    const bool batch = ncycles > 1;
+    // the window the previous call left (drawn and uploaded by a host thread) is ready once
+    // that thread is joined
+    if (l->gen_buf >= 0 && l->gen_buf == l->pin_ahead) gen_finish(l);
     // delayed acceptance and recording: descriptor rows of width ldx
     const bool use_net = da && !da->des;
     ARG_CHECK(ctx, !da || da->des || da->mlp, "epsr_molfit_da: neither a network nor des");
@@ -2112,7 +2209,12 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     // (16 + 2 nmt ints), the no-change flag, the step sizes per type (2 nmt floats).  The
     // previous call ended with a synchronisation, so nothing reads it now.
     const int nwords = (nmol + 31) / 32;
-    const size_t hio_need = sizeof(int) * (20 + 4 * (size_t)nmt) + sizeof(float) * (size_t)nmol + sizeof(int) * nwords;
+    // back-to-back cycles: per cycle its result block and adapted steps (hw words), fdiffs
+    // and acceptance bits; before them the constants and first steps (5 nmt words)
+    const int hw = 16 + 4 * nmt;
+    const size_t bres_words = batch ? 5 * (size_t)nmt + (size_t)ncycles * ((size_t)hw + nmol + nwords) : 0;
+    const size_t hio_need = sizeof(int) * (20 + 4 * (size_t)nmt) + sizeof(float) * (size_t)nmol + sizeof(int) * nwords +
+                            sizeof(int) * bres_words;
     if (l->hio_cap < hio_need) {
         if (l->hio) HIP_CHECK(ctx, hipHostFree(l->hio));
         l->hio = nullptr;
@@ -2128,6 +2230,7 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     // the exact fdiff of every move and the acceptance bits, for fadd on the host
     float *hfd = (float *)(hblk + 20 + 4 * nmt);
     const unsigned *hbits = (const unsigned *)(hfd + nmol);
+    int *hbres = (int *)(hbits + nwords);   // the batch block (bres_words)
     std::vector<int> att(l->att);
     std::vector<float> wr(wholerot, wholerot + nmt), wt(wholetrans, wholetrans + nmt);
     for (int mt = 0; mt < nmt; ++mt) {
@@ -2148,9 +2251,9 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     int total = l->plan_total;
     // the ran1 stream of the cycle: drawn ahead by the previous call when it starts from
     // this state, else drawn now; both in pinned memory
-    // windows of total + extra values: room for atogen's origin draws (one per molecule)
-    // between two cycles
-    const int wlen = total + nmol + 256;
+    // windows of the call's values plus room for atogen's origin draws (one per molecule)
+    // before the next call
+    const int wlen = ncycles * total + nmol + 256;
     if (l->pin_cap < (size_t)wlen + 2 * nmt + 8) {
         gen_join(l);
         l->gen_buf = -1;
@@ -2174,7 +2277,7 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     int use = -1, off = 0;
     bool uploaded = false;   // the window is in l->d_stream[use], behind the event ev_up
     {
-        const int o = window_offset(l, l->pin_ahead, ran1_state, total);
+        const int o = window_offset(l, l->pin_ahead, ran1_state, ncycles * total);
         if (o >= 0) {
             use = l->pin_ahead;
             off = o;
@@ -2190,9 +2293,10 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     }
     l->pin_ahead = -1;
     l->ahead_uploaded = false;
-    window_state(l, use, off + total, rng);   // the state after the cycle's stream
+    window_state(l, use, off + ncycles * total, rng);   // the state after the call's stream
     // the device copies of the windows
     if (l->dstream_cap < (size_t)wlen + 8) {
+        gen_finish(l);
         HIP_CHECK(ctx, hipStreamSynchronize(st));
         if (l->ev_up) HIP_CHECK(ctx, hipEventSynchronize(l->ev_up));
         const size_t cap = (size_t)wlen + wlen / 8 + 64;
@@ -2202,6 +2306,25 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
         }
         l->dstream_cap = cap;
         uploaded = false;
+    }
+    // back-to-back cycles: every cycle's values are in window `use`; while they run, a host
+    // thread draws the window of the next call (nb) from this call's end state
+    int nb = -1;
+    int *dbres = nullptr;
+    if (batch) {
+        gen_finish(l);
+        if (l->ev_up) HIP_CHECK(ctx, hipEventSynchronize(l->ev_up));
+        if (l->d_bres_cap < bres_words) {
+            l->d_bres_cap = 0;
+            int rc0;
+            if ((rc0 = dev_alloc(ctx, &l->d_bres, bres_words))) return rc0;
+            l->d_bres_cap = bres_words;
+        }
+        dbres = l->d_bres;
+        if (!l->up_stream) HIP_CHECK(ctx, hipStreamCreateWithFlags(&l->up_stream, hipStreamNonBlocking));
+        if (!l->ev_up) HIP_CHECK(ctx, hipEventCreateWithFlags(&l->ev_up, hipEventDisableTiming));
+        nb = (use + 1) % 3;
+        gen_start(l, nb, rng, wlen, true);
     }
 
     // device buffers.  The parallel resolution keeps the terms of an exact sum (the
@@ -2235,6 +2358,7 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     // types, the step sizes of every type (else they go by value to mc_trial)
     float *dstream = l->d_stream[use] + off;
     float *dwtype = S.take<float>(2 * nmt);
+    if (batch) dwtype = (float *)(dbres + 3 * nmt);   // the steps the device adapts between cycles
     float *txa = S.take<float>(natom), *tya = S.take<float>(natom), *tza = S.take<float>(natom);
     float *txadd = S.take<float>(nmol), *tyadd = S.take<float>(nmol), *tzadd = S.take<float>(nmol);
     float *dA = S.take<float>(nslots), *dB = S.take<float>(nslots), *dL = S.take<float>(nslots);
@@ -2276,18 +2400,30 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
         tab_remember(l, tb, ext[0], ext[1]);
         l->tab_packed = true;
     }
-    const bool steps_by_value = nmt <= STEP_TYPES;
+    const bool steps_by_value = !batch && nmt <= STEP_TYPES;
     StepArgs sa{};
     if (steps_by_value)
         for (int j = 0; j < 2 * nmt; ++j) sa.w[j] = wtype[j];
     if (uploaded) {
         HIP_CHECK(ctx, hipStreamWaitEvent(st, l->ev_up, 0));
     } else {
-        HIP_CHECK(ctx, hipMemcpyAsync(dstream, l->pin[use] + off, sizeof(float) * total, hipMemcpyHostToDevice, st));
+        HIP_CHECK(ctx, hipMemcpyAsync(dstream, l->pin[use] + off, sizeof(float) * (size_t)ncycles * total,
+                                      hipMemcpyHostToDevice, st));
     }
-    if (!steps_by_value) {
-        std::memcpy(l->pin[use] + wlen, wtype, sizeof(float) * 2 * nmt);
-        HIP_CHECK(ctx, hipMemcpyAsync(dwtype, l->pin[use] + wlen, sizeof(float) * 2 * nmt, hipMemcpyHostToDevice, st));
+    if (batch) {
+        // attempts and molecules per type, and the first cycle's steps (a type without
+        // molecules keeps its given steps, which the adaptation still clamps)
+        float *w0 = (float *)(hbres + 3 * nmt);
+        for (int mt = 0; mt < nmt; ++mt) {
+            hbres[2 * mt] = att[2 * mt];
+            hbres[2 * mt + 1] = att[2 * mt + 1];
+            hbres[2 * nmt + mt] = l->type_nat[mt];
+            w0[2 * mt] = wr[mt];
+            w0[2 * mt + 1] = wt[mt];
+        }
+        HIP_CHECK(ctx, hipMemcpyAsync(dbres, hbres, sizeof(int) * 5 * nmt, hipMemcpyHostToDevice, st));
+    } else if (!steps_by_value) {
+        HIP_CHECK(ctx, hipMemcpyAsync(dwtype, wtype, sizeof(float) * 2 * nmt, hipMemcpyHostToDevice, st));
     }
     HIP_CHECK(ctx, hipMemsetAsync(dblk + 8, 0, 4 * sizeof(int), st));
     mc_trial<<<ceil_div(nmol, 128), 128, 0, st>>>(nmol, l->d_order, box->d_naf, box->d_nal, c->xa, c->ya, c->za, c->xadd,
@@ -2298,7 +2434,9 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     int nexact = nmol, npasses = 0;
     // B..D of the cycle from trial positions in txa..tzadd and the stream at stream; the
     // commits are skipped when *redo is set
-    auto run = [&](const float *stream, const int *moff, const int *roff, const int *toff, const int *redo) -> int {
+    // slot >= 0: cycle `slot` of a batch, whose fdiffs and acceptance bits stay on the device
+    auto run = [&](const float *stream, const int *moff, const int *roff, const int *toff, const int *redo,
+                   int slot) -> int {
         int rc1;
         bool joined = false;
         abs_positions2<<<ceil_div(2 * (long)natom, 256), 256, 0, st>>>(
@@ -2359,10 +2497,19 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
             if ((rc1 = ctx_side(ctx, &side))) return rc1;
             HIP_CHECK(ctx, hipEventRecord(ctx->ev_fork, st));
             HIP_CHECK(ctx, hipStreamWaitEvent(side, ctx->ev_fork, 0));
+            float *fdo = dfdiff;
+            unsigned *bo = nullptr;
+            if (slot >= 0) {
+                fdo = (float *)(dbres + 5 * nmt + (size_t)ncycles * hw) + (size_t)slot * nmol;
+                bo = (unsigned *)(dbres + 5 * nmt + (size_t)ncycles * (hw + nmol)) + (size_t)slot * nwords;
+            }
             mc_fdiff_accepted<<<ceil_div(nmol, 4), 256, sizeof(float2) * 4 * (size_t)maxc, side>>>(
-                nmol, l->d_order, l->d_rowstart, l->d_cnt, l->d_pn, l->d_mw, dA, dB, dL, dacc, maxc, dfdiff);
-            HIP_CHECK(ctx, hipMemcpyAsync(hfd, dfdiff, sizeof(float) * nmol, hipMemcpyDeviceToHost, side));
-            HIP_CHECK(ctx, hipMemcpyAsync((void *)hbits, dacc, sizeof(unsigned) * nwords, hipMemcpyDeviceToHost, side));
+                nmol, l->d_order, l->d_rowstart, l->d_cnt, l->d_pn, l->d_mw, dA, dB, dL, dacc, maxc, fdo, bo, nwords);
+            if (slot < 0) {
+                HIP_CHECK(ctx, hipMemcpyAsync(hfd, dfdiff, sizeof(float) * nmol, hipMemcpyDeviceToHost, side));
+                HIP_CHECK(ctx,
+                          hipMemcpyAsync((void *)hbits, dacc, sizeof(unsigned) * nwords, hipMemcpyDeviceToHost, side));
+            }
             HIP_CHECK(ctx, hipEventRecord(ctx->ev_join, side));
             joined = true;
         } else {
@@ -2381,11 +2528,100 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
                                                             c->ya, c->za, c->xadd, c->yadd, c->zadd, nstepreject,
                                                             p->stepr > 0.0f ? 1 : 0, redo, drej);
         if (joined) HIP_CHECK(ctx, hipStreamWaitEvent(st, ctx->ev_join, 0));
-        HIP_CHECK(ctx, hipMemcpyAsync(hblk, dblk, sizeof(int) * (16 + 2 * (size_t)nmt), hipMemcpyDeviceToHost, st));
+        if (slot < 0)
+            HIP_CHECK(ctx, hipMemcpyAsync(hblk, dblk, sizeof(int) * (16 + 2 * (size_t)nmt), hipMemcpyDeviceToHost, st));
         HIP_CHECK(ctx, hipGetLastError());
         return EPSR_OK;
     };
-    if ((rc = run(dstream, l->d_moff, l->d_roff, l->d_toff, dnochange))) return rc;
+    if ((rc = run(dstream, l->d_moff, l->d_roff, l->d_toff, dnochange, batch ? 0 : -1))) return rc;
+    if (batch) {
+        // cycles 1..n-1, the steps adapted on the device after every cycle; one wait at the end
+        auto adapt = [&](int k) {
+            mc_adapt<<<1, 64, 0, st>>>(nmt, dbres, dbres + 2 * nmt, dblk, dwtype, dbres + 5 * nmt + (size_t)k * hw,
+                                       p->rejrate, p->ireset, p->stepr > 0.0f ? 1 : 0, p->stepm > 0.0f ? 1 : 0,
+                                       rotstepmax, rotstepmin, trstepmax, trstepmin);
+        };
+        adapt(0);
+        for (int k = 1; k < ncycles; ++k) {
+            const float *sk = dstream + (size_t)k * total;
+            mc_trial<<<ceil_div(nmol, 128), 128, 0, st>>>(nmol, l->d_order, box->d_naf, box->d_nal, c->xa, c->ya,
+                                                          c->za, c->xadd, c->yadd, c->zadd, sk, l->d_roff, l->d_toff,
+                                                          nullptr, nullptr, box->d_mtype, dwtype, sa, txa, tya, tza,
+                                                          txadd, tyadd, tzadd, dnochange, tc, drej, 2 * nmt);
+            if ((rc = run(sk, l->d_moff, l->d_roff, l->d_toff, dnochange, k))) return rc;
+            adapt(k);
+        }
+        HIP_CHECK(ctx, hipMemcpyAsync(hbres + 5 * nmt, dbres + 5 * nmt, sizeof(int) * (bres_words - 5 * (size_t)nmt),
+                                      hipMemcpyDeviceToHost, st));
+        if ((rc = ctx_wait(ctx))) return rc;
+        // the next call's window: its thread goes on drawing and uploading after the return
+        // (the next call joins it)
+        l->pin_ahead = nb;
+        l->ahead_uploaded = true;
+        // the cycles in order.  The first whose trial moves did not all change (or that
+        // found the tables changed in place) committed nothing, and neither did any after it
+        const int *slots = hbres + 5 * nmt;
+        const float *fds = (const float *)(slots + (size_t)ncycles * hw);
+        const unsigned *bts = (const unsigned *)(fds + (size_t)ncycles * nmol);
+        int done = 0, nacc = 0;
+        for (; done < ncycles; ++done) {
+            const int *b = slots + (size_t)done * hw;
+            if (b[9] != 0) break;
+            if (b[5] != 0)
+                return set_error(ctx, EPSR_ERR_HIP,
+                                 "epsr_molfit_cycles: cycle %d left %d moves undecided after %d passes; coordinates "
+                                 "and ledger are not the chain's",
+                                 done, b[5], 3 + b[4]);
+            const float *fd = fds + (size_t)done * nmol;
+            const unsigned *bw = bts + (size_t)done * nwords;
+            float fadd = 0.0f;
+            nacc = 0;
+            for (int w = 0; w < nwords; ++w)
+                for (unsigned bb = bw[w]; bb; bb &= bb - 1u) {   // the accepted moves of the word, in order
+                    fadd = fadd + fd[32 * w + __builtin_ctz(bb)];
+                    ++nacc;
+                }
+            *fadds = *fadds + fadd / (float)nmol;
+        }
+        if (done > 0) {
+            const int *b = slots + (size_t)(done - 1) * hw;
+            const float *after = (const float *)(b + 16 + 2 * nmt);
+            for (int mt = 0; mt < nmt; ++mt) {
+                wholerot[mt] = after[2 * mt];
+                wholetrans[mt] = after[2 * mt + 1];
+                nwhole[2 * mt] = att[2 * mt];
+                nwhole[2 * mt + 1] = b[16 + 2 * mt];
+                nmovem[2 * mt] = att[2 * mt + 1];
+                nmovem[2 * mt + 1] = b[16 + 2 * mt + 1];
+            }
+            Ran1 r;
+            window_state(l, use, off + done * total, r);   // the state at the start of cycle `done`
+            ran1_store(r, ran1_state);
+            if (stats) {
+                stats->moves = nmol;
+                stats->accepted = nacc;
+                stats->exact_resolutions = b[3];
+                stats->pushed_entries = 0;
+                stats->resolver_passes = 3 + b[4];
+                stats->resolver_bytes = (double)nslots * 13.0 + (double)nmol * (sizeof(float2) + 1);
+                stats->retries = 0;
+                stats->ms_pairs = ctx->timing ? epsr_ctx_last_kernel_ms(ctx, "mc_pair") : -1.0;
+                stats->ms_resolve = ctx->timing ? epsr_ctx_last_kernel_ms(ctx, "mc_resolve") : -1.0;
+                stats->stage1_rejected = b[8];
+            }
+        }
+        if (done < ncycles) {
+            // that cycle on its own (it doubles movem's step, or measures the tables again),
+            // then the rest
+            if ((rc = molfit_impl(ctx, box, c, tb, l, p, 1, wholerot, wholetrans, nwhole, nmovem, nstepreject,
+                                  ran1_state, fadds, da, rec, stats)))
+                return rc;
+            if (done + 1 < ncycles)
+                return molfit_impl(ctx, box, c, tb, l, p, ncycles - done - 1, wholerot, wholetrans, nwhole, nmovem,
+                                   nstepreject, ran1_state, fadds, da, rec, stats);
+        }
+        return EPSR_OK;
+    }
     // while the GPU runs the cycle: the next cycle's stream, from this one's end state
     // (the window the thread of the previous call drew, when this cycle's end state lies
     // in it), and a thread for the window after, from the next cycle's expected end
@@ -2428,8 +2664,8 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
         // the tables changed in place since their extents were measured; nothing was
         // committed and no state was updated: measure them and run the cycle again
         l->tab_ok = false;
-        return molfit_impl(ctx, box, c, tb, l, p, wholerot, wholetrans, nwhole, nmovem, nstepreject, ran1_state, fadds,
-                           da, rec, stats);
+        return molfit_impl(ctx, box, c, tb, l, p, 1, wholerot, wholetrans, nwhole, nmovem, nstepreject, ran1_state,
+                           fadds, da, rec, stats);
     }
     int retries = 0;
     if (*hflag) {
@@ -2513,7 +2749,7 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
                                                       c->xadd, c->yadd, c->zadd, dstream2, droff, dtoff, drstep,
                                                       dtstep, box->d_mtype, steps_by_value ? nullptr : dwtype, sa, txa,
                                                       tya, tza, txadd, tyadd, tzadd, dtmp + 2, none, drej, 2 * nmt);
-        if ((rc = run(dstream2, dmoff, droff, dtoff, dtmp + 3))) return rc;
+        if ((rc = run(dstream2, dmoff, droff, dtoff, dtmp + 3, -1))) return rc;
         HIP_CHECK(ctx, hipStreamSynchronize(st));
     }
     if (fast) {
@@ -2581,8 +2817,17 @@ extern "C" int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords
                            epsr_ledger *l, const epsr_mc_params_t *p, float *wholerot, float *wholetrans, int *nwhole,
                            int *nmovem, int *nstepreject, int32_t *ran1_state, float *fadds, epsr_mc_stats_t *stats)
 {
-    return molfit_impl(ctx, box, c, tb, l, p, wholerot, wholetrans, nwhole, nmovem, nstepreject, ran1_state, fadds,
+    return molfit_impl(ctx, box, c, tb, l, p, 1, wholerot, wholetrans, nwhole, nmovem, nstepreject, ran1_state, fadds,
                        nullptr, nullptr, stats);
+}
+
+extern "C" int epsr_molfit_cycles(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *c,
+                                  const epsr_tables_t *tb, epsr_ledger *l, const epsr_mc_params_t *p, int ncycles,
+                                  float *wholerot, float *wholetrans, int *nwhole, int *nmovem, int *nstepreject,
+                                  int32_t *ran1_state, float *fadds, epsr_mc_stats_t *stats)
+{
+    return molfit_impl(ctx, box, c, tb, l, p, ncycles, wholerot, wholetrans, nwhole, nmovem, nstepreject, ran1_state,
+                       fadds, nullptr, nullptr, stats);
 }
 
 extern "C" int epsr_molfit_da(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *c, const epsr_tables_t *tb,
@@ -2590,7 +2835,7 @@ extern "C" int epsr_molfit_da(epsr_ctx *ctx, const epsr_box *box, const epsr_coo
                               int *nwhole, int *nmovem, int *nstepreject, int32_t *ran1_state, float *fadds,
                               const epsr_da_t *da, const epsr_mc_record_t *rec, epsr_mc_stats_t *stats)
 {
-    return molfit_impl(ctx, box, c, tb, l, p, wholerot, wholetrans, nwhole, nmovem, nstepreject, ran1_state, fadds,
+    return molfit_impl(ctx, box, c, tb, l, p, 1, wholerot, wholetrans, nwhole, nmovem, nstepreject, ran1_state, fadds,
                        da, rec, stats);
 }
 

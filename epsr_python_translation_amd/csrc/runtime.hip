@@ -39,8 +39,8 @@ extern "C" void epsr_ctx_destroy(epsr_ctx *ctx)
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
     for (auto &kv : ctx->timers) {
-        if (kv.second.start) hipEventDestroy(kv.second.start);
-        if (kv.second.stop) hipEventDestroy(kv.second.stop);
+        for (hipEvent_t e : kv.second.start) hipEventDestroy(e);
+        for (hipEvent_t e : kv.second.stop) hipEventDestroy(e);
     }
     if (ctx->side) {
         hipStreamSynchronize(ctx->side);
@@ -105,23 +105,46 @@ bool ctx_capturing(epsr_ctx *ctx) { return ctx->capturing; }
 
 hipError_t ctx_sync(epsr_ctx *ctx) { return ctx_capturing(ctx) ? hipSuccess : hipStreamSynchronize(ctx->stream); }
 
+// read the durations of every recorded launch of t (waits for the last one)
+static void timer_flush(KernelTimer &t)
+{
+    if (t.n == 0) return;
+    hipEventSynchronize(t.stop[t.n - 1]);
+    for (int i = 0; i < t.n; ++i) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, t.start[i], t.stop[i]);
+        t.sum_ms += ms;
+        t.count += 1;
+        t.last_ms = ms;
+    }
+    t.n = 0;
+}
+
+// at most this many launches of a family are kept unread; the next one reads them first
+static const int TIMER_RECORDS = 1024;
+
 void timer_begin(epsr_ctx *ctx, const char *family)
 {
     if (!ctx->timing || ctx_capturing(ctx)) return;
     KernelTimer &t = ctx->timers[family];
-    if (!t.start) {
-        hipEventCreate(&t.start);
-        hipEventCreate(&t.stop);
+    if (t.n == TIMER_RECORDS) timer_flush(t);
+    if (t.n == (int)t.start.size()) {
+        hipEvent_t a = nullptr, b = nullptr;
+        hipEventCreate(&a);
+        hipEventCreate(&b);
+        t.start.push_back(a);
+        t.stop.push_back(b);
     }
-    hipEventRecord(t.start, ctx->stream);
+    hipEventRecord(t.start[t.n], ctx->stream);
 }
 
 void timer_end(epsr_ctx *ctx, const char *family)
 {
     if (!ctx->timing || ctx_capturing(ctx)) return;
     KernelTimer &t = ctx->timers[family];
-    hipEventRecord(t.stop, ctx->stream);
-    t.pending = true;
+    if (t.n == (int)t.start.size()) return;   // no begin
+    hipEventRecord(t.stop[t.n], ctx->stream);
+    t.n += 1;
 }
 
 extern "C" double epsr_ctx_last_kernel_ms(const epsr_ctx *cctx, const char *family)
@@ -131,14 +154,23 @@ extern "C" double epsr_ctx_last_kernel_ms(const epsr_ctx *cctx, const char *fami
     auto it = ctx->timers.find(family);
     if (it == ctx->timers.end()) return -1.0;
     KernelTimer &t = it->second;
-    if (t.pending) {
-        hipEventSynchronize(t.stop);
-        float ms = 0.f;
-        hipEventElapsedTime(&ms, t.start, t.stop);
-        t.last_ms = ms;
-        t.pending = false;
-    }
+    timer_flush(t);
     return t.last_ms;
+}
+
+extern "C" double epsr_ctx_kernel_ms_sum(epsr_ctx *ctx, const char *family, long long *count)
+{
+    if (count) *count = 0;
+    if (!ctx || !family) return -1.0;
+    auto it = ctx->timers.find(family);
+    if (it == ctx->timers.end()) return -1.0;
+    KernelTimer &t = it->second;
+    timer_flush(t);
+    if (count) *count = t.count;
+    const double s = t.count ? t.sum_ms : -1.0;
+    t.sum_ms = 0.0;
+    t.count = 0;
+    return s;
 }
 
 void *ctx_scratch(epsr_ctx *ctx, size_t bytes)

@@ -62,6 +62,12 @@ class Context:
     def kernel_ms(self, family: str) -> float:
         return float(self.lib.epsr_ctx_last_kernel_ms(self.h, family.encode()))
 
+    def kernel_ms_sum(self, family: str):
+        """(summed ms, launches) of the family since the previous call; (-1.0, 0) for none."""
+        n = C.c_longlong(0)
+        t = float(self.lib.epsr_ctx_kernel_ms_sum(self.h, family.encode(), C.byref(n)))
+        return t, int(n.value)
+
     def enable_timing(self, on=True):
         self.check(self.lib.epsr_ctx_enable_timing(self.h, 1 if on else 0))
 
@@ -444,13 +450,19 @@ def _mc_args(mc: MCState, ran1_state):
     return cached[1]
 
 
-def molfit(ctx: Context, box: DeviceBox, tables: DeviceTables, ledger: Ledger, mc: MCState, ran1_state):
-    """One MC cycle on the GPU; updates coordinates, ledger, mc and ran1_state in place."""
+def molfit(ctx: Context, box: DeviceBox, tables: DeviceTables, ledger: Ledger, mc: MCState, ran1_state, ncycles=1):
+    """ncycles MC cycles on the GPU (one call of epsr_molfit, or epsr_molfit_cycles for
+    several back to back); updates coordinates, ledger, mc and ran1_state in place."""
     c = box.coords()
     t = tables.struct()
     fad = C.c_float(float(mc.fadds))
-    ctx.check(ctx.lib.epsr_molfit(ctx.h, box.h, C.byref(c), C.byref(t), ledger.h, C.byref(mc.params),
-                                  *_mc_args(mc, ran1_state), C.byref(fad), C.byref(mc.stats)))
+    if ncycles == 1:
+        ctx.check(ctx.lib.epsr_molfit(ctx.h, box.h, C.byref(c), C.byref(t), ledger.h, C.byref(mc.params),
+                                      *_mc_args(mc, ran1_state), C.byref(fad), C.byref(mc.stats)))
+    else:
+        ctx.check(ctx.lib.epsr_molfit_cycles(ctx.h, box.h, C.byref(c), C.byref(t), ledger.h, C.byref(mc.params),
+                                             int(ncycles), *_mc_args(mc, ran1_state), C.byref(fad),
+                                             C.byref(mc.stats)))
     mc.fadds = F32(fad.value)
     return mc
 

@@ -490,12 +490,14 @@ class Refinement:
         return P.press(self.ledger, self.tables, self.g.npt1, s.simt, vol, pcoefp=self.pcoefp, energf=energf,
                        pressf=pressf, wsum=wsum, nmol=s.nmol)
 
-    def molfit(self, tables=None):
-        """one molfit cycle (every branch when the box has an intramolecular topology)"""
+    def molfit(self, tables=None, ncycles=1):
+        """ncycles molfit cycles (every branch when the box has an intramolecular topology,
+        one call per cycle; rigid boxes run them back to back on the GPU)"""
         if isinstance(self.mc, D.MCFullState):
-            D.molfit_full(self.ctx, self.box, tables or self.tables, self.ledger, self.mc, self.ran1)
+            for _ in range(ncycles):
+                D.molfit_full(self.ctx, self.box, tables or self.tables, self.ledger, self.mc, self.ran1)
         else:
-            D.molfit(self.ctx, self.box, tables or self.tables, self.ledger, self.mc, self.ran1)
+            D.molfit(self.ctx, self.box, tables or self.tables, self.ledger, self.mc, self.ran1, ncycles=ncycles)
         return self.mc
 
     def atogen(self, ngrsamples=0):

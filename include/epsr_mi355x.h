@@ -48,6 +48,11 @@ int epsr_ctx_synchronize(epsr_ctx *ctx);
  * events on the launch stream; families: "atogen_inter", "gensq", "stog",
  * "mc_pair", "mc_resolve", "fmolec1_pair".  Returns -1 if never launched. */
 double epsr_ctx_last_kernel_ms(const epsr_ctx *ctx, const char *family);
+/* the summed GPU time (ms) and the number (*count) of the named family's launches since
+ * the previous call of this function (those read by epsr_ctx_last_kernel_ms included);
+ * -1 with *count = 0 when there were none.  Several launches of a family in one call
+ * (epsr_molfit_cycles) are each counted. */
+double epsr_ctx_kernel_ms_sum(epsr_ctx *ctx, const char *family, long long *count);
 int epsr_ctx_enable_timing(epsr_ctx *ctx, int on);
 
 /* ---- host ran1 stream: n deviates of auxiliary_routines.ran1
@@ -284,6 +289,20 @@ typedef struct {
 int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *coords, const epsr_tables_t *tables,
                 epsr_ledger *l, const epsr_mc_params_t *p, float *wholerot, float *wholetrans, int *nwhole,
                 int *nmovem, int *nstepreject, int32_t *ran1_state, float *fadds, epsr_mc_stats_t *stats);
+
+/* ncycles rigid-molecule molfit cycles in a row: the program's MC loop over itimes
+ * (src/epsr26/programs/epsr_standalone_rev3.py:1313-1319) for boxes without intramolecular
+ * moves.  Every output is what ncycles calls of epsr_molfit leave (coordinates, ledger,
+ * step sizes, the last cycle's counters and stats, nstepreject, ran1 state, fadds).  The
+ * cycles run back to back on the GPU: the step sizes adapt on the device between cycles, a
+ * host thread draws each cycle's ran1 values while earlier cycles run, and the host waits
+ * once.  A cycle in which movem must double a step (or that finds the tables changed)
+ * commits nothing, nor do the cycles queued after it; the call then runs it alone and goes
+ * on.  Up to 64 cycles are queued at a time. */
+int epsr_molfit_cycles(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *coords, const epsr_tables_t *tables,
+                       epsr_ledger *l, const epsr_mc_params_t *p, int ncycles, float *wholerot, float *wholetrans,
+                       int *nwhole, int *nmovem, int *nstepreject, int32_t *ran1_state, float *fadds,
+                       epsr_mc_stats_t *stats);
 
 /* ==== intramolecular energies and moves ========================================
  * The molecule-internal data of molecom/clustercom/molecom_mod that en_mole_intra,
