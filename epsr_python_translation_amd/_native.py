@@ -92,7 +92,8 @@ class MCParams(C.Structure):
 class MCStats(C.Structure):
     _fields_ = [("moves", C.c_int), ("accepted", C.c_int), ("exact_resolutions", C.c_int), ("retries", C.c_int),
                 ("ms_pairs", C.c_double), ("ms_resolve", C.c_double), ("pushed_entries", C.c_longlong),
-                ("resolver_bytes", C.c_double), ("stage1_rejected", C.c_int), ("resolver_passes", C.c_int)]
+                ("resolver_bytes", C.c_double), ("stage1_rejected", C.c_int), ("resolver_passes", C.c_int),
+                ("cycles", C.c_int), ("accepted_total", C.c_longlong)]
 
 
 LAUNCH_ATOGEN_WORKGROUPS = 1

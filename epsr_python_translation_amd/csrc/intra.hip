@@ -931,6 +931,8 @@ extern "C" int epsr_molfit_full(epsr_ctx *ctx, const epsr_box *box, const epsr_c
             *stats = epsr_mc_stats_t{};
             stats->moves = ntests;
             stats->accepted = nacc;
+            stats->cycles = 1;
+            stats->accepted_total = nacc;
             stats->ms_resolve = ctx->timing ? epsr_ctx_last_kernel_ms(ctx, "mc_serial") : -1.0;
         }
     }

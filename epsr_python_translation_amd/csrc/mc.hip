@@ -2132,11 +2132,17 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
         const bool par = sizeof(float2) * 4 * (size_t)std::max(1, l->maxcnt) <= 160 * 1024;
         const int first = (da || rec || !par) ? 1 : std::min(ncycles, MC_BATCH_MAX);
         if (first < ncycles) {
+            epsr_mc_stats_t s0{};
             int rc0 = molfit_impl(ctx, box, c, tb, l, p, first, wholerot, wholetrans, nwhole, nmovem, nstepreject,
-                                  ran1_state, fadds, da, rec, stats);
+                                  ran1_state, fadds, da, rec, &s0);
             if (rc0) return rc0;
-            return molfit_impl(ctx, box, c, tb, l, p, ncycles - first, wholerot, wholetrans, nwhole, nmovem,
-                               nstepreject, ran1_state, fadds, da, rec, stats);
+            rc0 = molfit_impl(ctx, box, c, tb, l, p, ncycles - first, wholerot, wholetrans, nwhole, nmovem,
+                              nstepreject, ran1_state, fadds, da, rec, stats);
+            if (!rc0 && stats) {
+                stats->cycles += s0.cycles;
+                stats->accepted_total += s0.accepted_total;
+            }
+            return rc0;
         }
     }
     const bool batch = ncycles > 1;
@@ -2564,6 +2570,7 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
         const float *fds = (const float *)(slots + (size_t)ncycles * hw);
         const unsigned *bts = (const unsigned *)(fds + (size_t)ncycles * nmol);
         int done = 0, nacc = 0;
+        long long nacc_all = 0;
         for (; done < ncycles; ++done) {
             const int *b = slots + (size_t)done * hw;
             if (b[9] != 0) break;
@@ -2582,6 +2589,7 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
                     ++nacc;
                 }
             *fadds = *fadds + fadd / (float)nmol;
+            nacc_all += nacc;
         }
         if (done > 0) {
             const int *b = slots + (size_t)(done - 1) * hw;
@@ -2608,17 +2616,30 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
                 stats->ms_pairs = ctx->timing ? epsr_ctx_last_kernel_ms(ctx, "mc_pair") : -1.0;
                 stats->ms_resolve = ctx->timing ? epsr_ctx_last_kernel_ms(ctx, "mc_resolve") : -1.0;
                 stats->stage1_rejected = b[8];
+                stats->cycles = done;
+                stats->accepted_total = nacc_all;
             }
         }
         if (done < ncycles) {
             // that cycle on its own (it doubles movem's step, or measures the tables again),
             // then the rest
+            epsr_mc_stats_t s1{};
             if ((rc = molfit_impl(ctx, box, c, tb, l, p, 1, wholerot, wholetrans, nwhole, nmovem, nstepreject,
-                                  ran1_state, fadds, da, rec, stats)))
+                                  ran1_state, fadds, da, rec, &s1)))
                 return rc;
-            if (done + 1 < ncycles)
-                return molfit_impl(ctx, box, c, tb, l, p, ncycles - done - 1, wholerot, wholetrans, nwhole, nmovem,
-                                   nstepreject, ran1_state, fadds, da, rec, stats);
+            if (done + 1 < ncycles) {
+                if ((rc = molfit_impl(ctx, box, c, tb, l, p, ncycles - done - 1, wholerot, wholetrans, nwhole, nmovem,
+                                      nstepreject, ran1_state, fadds, da, rec, stats)))
+                    return rc;
+                if (stats) {
+                    stats->cycles += done + 1;
+                    stats->accepted_total += nacc_all + s1.accepted_total;
+                }
+            } else if (stats) {
+                *stats = s1;
+                stats->cycles = done + 1;
+                stats->accepted_total = nacc_all + s1.accepted_total;
+            }
         }
         return EPSR_OK;
     }
@@ -2809,6 +2830,8 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
         stats->ms_pairs = ctx->timing ? epsr_ctx_last_kernel_ms(ctx, "mc_pair") : -1.0;
         stats->ms_resolve = ctx->timing ? epsr_ctx_last_kernel_ms(ctx, "mc_resolve") : -1.0;
         stats->stage1_rejected = hblk[8];
+        stats->cycles = 1;
+        stats->accepted_total = nacc;
     }
     return EPSR_OK;
 }

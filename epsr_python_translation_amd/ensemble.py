@@ -7,8 +7,9 @@ side on the GPU while every box keeps the exact per-box semantics of
 as when the box runs alone).  Launch shapes come from the autotuner, once per box size
 class.  The transform stage of a worker's boxes (gensq and stog, no host reads) is
 captured into one HIP graph when the ensemble is built and replayed by every step: one
-launch per worker instead of four per box.  The MC cycles and atogen read results back to the host
-(acceptance counters, origin selection with ran1), so they run as eager calls.
+launch per worker instead of four per box.  A box's MC cycles of a step run back to back in
+one call (the step sizes adapt on the device; the host waits once per box and step), and
+atogen selects its origins with ran1 on the host, so both run as eager calls.
 """
 
 from __future__ import annotations
@@ -125,11 +126,9 @@ class Ensemble:
             acc = mov = 0
             for R in wk.boxes:
                 self._ledger(R)
-                tables = R.tables
-                for _ in range(sweeps):
-                    R.molfit(tables)
-                    acc += R.mc.stats.accepted
-                    mov += R.mc.stats.moves
+                R.molfit(R.tables, ncycles=sweeps)
+                acc += R.mc.stats.accepted_total
+                mov += R.mc.stats.moves * R.mc.stats.cycles
                 R.atogen(ngrsamples=ngrsamples)
             wk.graph.launch()
             stats[id(wk)] = (acc, mov)

@@ -284,6 +284,8 @@ typedef struct {
     double resolver_bytes;
     int stage1_rejected;   /* delayed acceptance: moves the surrogate stage rejected */
     int resolver_passes;   /* passes of the parallel resolution */
+    int cycles;                /* cycles the call ran (the fields above are the last one's) */
+    long long accepted_total;  /* moves accepted over those cycles */
 } epsr_mc_stats_t;
 
 int epsr_molfit(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *coords, const epsr_tables_t *tables,

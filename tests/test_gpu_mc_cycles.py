@@ -73,10 +73,11 @@ def test_cycles_equal_single_calls(ctx, name, ncyc, kind, ireset):
     steps = _steps(H.system(name), kind)
     s, box, tab, led, mc = _setup(ctx, name, steps, ireset)
     st = D.ran1_state_array(idum)
-    halts = 0
+    halts = acc = 0
     for _ in range(ncyc):
         D.molfit(ctx, box, tab, led, mc, st)
         halts += mc.stats.retries > 0
+        acc += mc.stats.accepted
     one = _state(box, led, mc, st)
     s, box, tab, led, mc = _setup(ctx, name, steps, ireset)
     st = D.ran1_state_array(idum)
@@ -84,6 +85,7 @@ def test_cycles_equal_single_calls(ctx, name, ncyc, kind, ireset):
     many = _state(box, led, mc, st)
     print(f"  cycles with movem doubling: {halts}; last cycle accepted {many['accepted']}", flush=True)
     _same(one, many, f"{name} {ncyc} cycles")
+    assert mc.stats.cycles == ncyc and mc.stats.accepted_total == acc
     if kind == "halting":
         assert halts >= 1
     if name == "c0":
