@@ -417,6 +417,47 @@ class Refinement:
                                 nqmin, nqmax, szeros, porodamp, rshmin, porodq, porodpwr, nsmoop, self.x, g.npt1,
                                 s.rho, self.grt, weightx=dwx, xfile=wts.get("xfile"))
 
+    def rfactor_data(self, datai, wts, nqmind=None, nqmaxd=None, nqmin=None, nqmax=None, porodamp=None,
+                     porodq=0.0, porodpwr=0.0):
+        """The fit to the data that the epsr program reports after each iteration, its
+        "R-factor for data file" (epsr_standalone_rev3.py:2070-2131), on the running averages
+        (accumulate first): per dataset the mean over Q points nqmin..nqmax of
+        (data - sim)^2, where sim = sum over pairs of ciwt S_intra + cwt S_inter (X-ray datasets:
+        weightx (S_intra + S_inter) + backx) plus the Porod term, and data - sim is taken only
+        inside the data's range nqmind..nqmaxd; and the mean over datasets (rfactot).  A
+        report, formed on the host in FP64.  Returns (rfacd [nfile], rfactot)."""
+        s, g = self.s, self.g
+        nq, npairs, ldq = g.nq, s.npairs, self.ldq
+        datai = np.asarray(datai, np.float64)
+        nfile = datai.shape[0]
+        soq = self.soq.cpu().numpy().astype(np.float64).reshape(2, npairs, ldq)
+        intra, inter = soq[0][:, 1:nq + 1], soq[1][:, 1:nq + 1]
+        one = lambda v, d: np.full(nfile, d, np.int32) if v is None else np.asarray(v, np.int32)
+        nqmind, nqmaxd = one(nqmind, 1), one(nqmaxd, nq)
+        nqmin, nqmax = one(nqmin, 1), one(nqmax, nq)
+        porodamp = np.zeros(nfile) if porodamp is None else np.asarray(porodamp, np.float64)
+        q = np.asarray(g.q, np.float64)[1:nq + 1]
+        ciwt = np.asarray(wts.get("ciwt", np.zeros((nfile, npairs))), np.float64)
+        cwt = np.asarray(wts["cwt"], np.float64)
+        xfile = wts.get("xfile")
+        idx = np.arange(1, nq + 1)
+        rfacd = np.zeros(nfile)
+        for f in range(nfile):
+            ix = int(xfile[f]) if xfile is not None else 0
+            if ix > 0:
+                wx = np.asarray(wts["weightx"], np.float64)[ix - 1][1:nq + 1]          # [nq, npairs]
+                sim = (wx * (intra + inter).T).sum(1) + np.asarray(wts["backx"], np.float64)[ix - 1][1:nq + 1]
+            else:
+                sim = ciwt[f] @ intra + cwt[f] @ inter
+            if porodamp[f] > 0.0:
+                sim = sim + np.where(q > 0.0, porodamp[f] * np.power(porodq / np.where(q > 0, q, 1.0), porodpwr), 0.0)
+            inside = (idx >= nqmind[f]) & (idx <= nqmaxd[f])
+            diff = np.where(inside, datai[f, 1:nq + 1] - sim, 0.0)
+            fit = (idx >= nqmin[f]) & (idx <= nqmax[f])
+            npts = int(nqmax[f] - nqmin[f] + 1)
+            rfacd[f] = float((diff[fit] ** 2).sum() / npts) if npts > 0 else 0.0
+        return rfacd, float(rfacd.mean())
+
     def ptgnsq(self, data, cwt, cwtpot, nqmin=None, szeros=None, potfac=1.0, fit_mode=None, npits=1000,
                paccept=5.0e-4, pstepd=0.002, weightx=None, xfile=None, nqmax=None):
         """One refinement of the empirical potential from the running intermolecular S(Q)

@@ -3,11 +3,13 @@
 
 Every replica is a full `pipeline.Refinement` with its own seed and ran1 stream; one
 iteration of a replica is what the reference's runepsr17 loop does between potential
-refinements, restricted to this package's path: the ledger, `sweeps` MC cycles, atogen,
-gensq, the running averages (sumsq/sumgr) and ptgnsq (difference from the data, the
-Poisson-basis fit, the EP update and generate_ep).  Replicas run on the ensemble's worker
-streams and threads, so one replica's serial host steps overlap the others' kernels; a
-replica's results do not depend on which worker runs it or on the other replicas.
+refinements, restricted to this package's path: the ledger, `sweeps` MC cycles (plain, back
+to back on the GPU, or two-stage with a learned surrogate of the energy change), atogen,
+gensq, stog, the running averages (sumsq/sumgr), the fit to the data the program reports
+(its R-factor per dataset) and ptgnsq (difference from the data, the Poisson-basis fit,
+the EP update and generate_ep).  Replicas run on the ensemble's worker streams and threads,
+so one replica's serial host steps overlap the others' kernels; a replica's results do not
+depend on which worker runs it or on the other replicas.
 """
 
 from __future__ import annotations
@@ -18,6 +20,7 @@ import torch
 from . import _native
 from . import formats as FM
 from . import pipeline as PL
+from . import surrogate as SG
 from . import synth
 from .ensemble import Ensemble
 
@@ -44,16 +47,23 @@ def target_weights(type_names, fractions):
     return out
 
 
-def synthetic_targets(ctx, seed=4, target_seed=999, ndata=3):
+def synthetic_targets(ctx, seed=4, target_seed=999, ndata=3, relax_sweeps=100):
     """Isotope fractions U(0, 1) (seeded), the weights they give, and the data: the weighted
-    intermolecular S(Q) of another SiO2 configuration (target_seed), so the refinement has
-    a structure to move towards.  Returns (wts, data [ndata, nq + 1], fractions)."""
+    intermolecular S(Q) of another SiO2 configuration (target_seed) after relax_sweeps MC
+    sweeps under the reference potential, so the refinement has a structure to move
+    towards.  Returns (wts, data [ndata, nq + 1], fractions)."""
     rng = np.random.default_rng(seed)
     fractions = rng.uniform(0.0, 1.0, ndata)
     s = synth.config4(seed=target_seed)
     wts = target_weights(s.type_names, fractions)
     R = PL.Refinement(ctx, s, idum=-target_seed).build_tables()
     R.cells_and_ledger()
+    left = relax_sweeps
+    while left > 0:
+        n = min(left, 10)
+        R.molfit(ncycles=n)
+        R.cells_and_ledger()   # cells and ledger follow the moved atoms
+        left -= n
     R.atogen()
     R.gensq()
     torch.cuda.synchronize()
@@ -78,6 +88,29 @@ class ReplicaRefinement(Ensemble):
         self.feedback = F32(feedback)
         self.nsum = 0
         self.weights = None
+        self.net = None
+        self.da = None
+
+    def train_surrogate(self, ctx, nsamples=200_000, train_seed=777, epochs=3, batch=4096, lr=2e-3, seed=1):
+        """The learned energy-change surrogate for delayed acceptance: plain GPU MC on a
+        separate SiO2 box (train_seed; 10 adapting sweeps first) records (descriptor, exact
+        later-partner dE) pairs, the network trains on them on the GPU, and every replica
+        gets a two-stage cycle driven by it (stage-1 uniforms seeded per replica).  Returns
+        the training losses per epoch."""
+        s = synth.config4(seed=train_seed)
+        R = PL.Refinement(ctx, s, idum=-train_seed).build_tables()
+        R.cells_and_ledger()
+        R.molfit(ncycles=10)
+        self.spec = SG.DescriptorSpec()
+        ldx = self.spec.width_for(s.npairs)
+        nsweep = max(1, int(np.ceil(nsamples / s.nmol)))
+        X, T = SG.collect(ctx, R.box, R.tables, R.ledger, R.mc, R.ran1, self.spec, nsweep, ldx)
+        self.net = SG.Network(ctx, ldx, seed=seed)
+        hist = self.net.train(X, T, ldx, epochs=epochs, batch=batch, lr=lr, tclip=25.0 * float(s.simt))
+        torch.cuda.synchronize()
+        del X, T, R
+        self.da = [SG.DelayedAcceptance(self.net, self.spec, seed=1000 + 7919 * i) for i in range(len(self.boxes))]
+        return hist
 
     def set_targets(self, wts, data):
         """Weights (formats.WtsFile per dataset) and data [ndata, nq + 1]; the inverted weights
@@ -88,10 +121,12 @@ class ReplicaRefinement(Ensemble):
             ctx = R.ctx
             self.weights.append(PL.get_wts(ctx, R.s, wts, self.feedback, np.ones(len(wts), F32), R.ran1))
 
-    def iterate(self, sweeps=1, fit_mode=_native.FIT_NORMAL):
-        """One iteration of every replica; returns per replica (r-factors, chi^2) and the
-        attempted moves."""
+    def iterate(self, sweeps=1, fit_mode=_native.FIT_NORMAL, da=False):
+        """One iteration of every replica; returns per replica (the EP fit's r-factors per
+        dataset, chi^2, the fit to the data per dataset).  da: MC cycles with delayed
+        acceptance (train_surrogate first), else plain cycles back to back."""
         assert self.weights is not None, "set_targets first"
+        assert not da or self.da is not None, "train_surrogate first"
         self.nsum += 1
         out = [None] * len(self.boxes)
 
@@ -100,12 +135,18 @@ class ReplicaRefinement(Ensemble):
                 i = self.boxes.index(R)
                 self._ledger(R)
                 tables = R.tables
-                for _ in range(sweeps):
-                    R.molfit(tables)
+                if da:
+                    for _ in range(sweeps):
+                        self.da[i].sweep(R.ctx, R.box, tables, R.ledger, R.mc, R.ran1)
+                else:
+                    R.molfit(tables, ncycles=sweeps)
                 R.atogen()
                 R.gensq()
+                R.stog()
                 R.accumulate(self.nsum)
                 w = self.weights[i]
-                out[i] = R.ptgnsq(self.data, w["cwt"], w["cwtpot"], fit_mode=fit_mode)
+                rfacd, _ = R.rfactor_data(self.data, w)
+                rf, chi = R.ptgnsq(self.data, w["cwt"], w["cwtpot"], fit_mode=fit_mode)
+                out[i] = (rf, chi, rfacd)
         self._parallel(work)
         return out
