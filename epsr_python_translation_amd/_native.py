@@ -18,7 +18,7 @@ LIB_PATH = HERE / "libepsr_mi355x.so"
 INCLUDE = HERE.parent / "include"
 
 HIPCC_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # the reference's float32 contract: no contraction to FMA, correctly rounded
     # division and square root, no fast-math
     "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt", "-fno-fast-math",
@@ -32,16 +32,31 @@ def sources():
     return sorted(CSRC.glob("*.hip"))
 
 
-def build(verbose: bool = False) -> pathlib.Path:
+def build(verbose: bool = False, jobs: int | None = None) -> pathlib.Path:
+    """Compile every source to an object (in parallel, each its own translation unit as in
+    a one-command build) and link the shared library."""
+    from concurrent.futures import ThreadPoolExecutor
+
     hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
     tmp = LIB_PATH.with_suffix(".so.tmp")
-    cmd = [hipcc, *HIPCC_FLAGS, f"-I{INCLUDE}", "-o", str(tmp), *map(str, sources())]
-    if verbose:
-        print(" ".join(cmd), flush=True)
+    objdir = HERE / "build"
+    objdir.mkdir(exist_ok=True)
     env = dict(os.environ)
-    r = subprocess.run(cmd, capture_output=True, text=True, env=env)
-    if r.returncode != 0:
-        raise RuntimeError(f"hipcc failed ({r.returncode}):\n{r.stderr[-6000:]}")
+
+    def run(cmd):
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True, env=env)
+        if r.returncode != 0:
+            raise RuntimeError(f"hipcc failed ({r.returncode}):\n{r.stderr[-6000:]}")
+
+    srcs = sources()
+    objs = [objdir / (p.stem + ".o") for p in srcs]
+    n = jobs or min(len(srcs), max(1, min(8, os.cpu_count() or 1)))
+    with ThreadPoolExecutor(n) as ex:
+        list(ex.map(run, [[hipcc, *HIPCC_FLAGS, f"-I{INCLUDE}", "-c", "-o", str(o), str(p)]
+                          for p, o in zip(srcs, objs)]))
+    run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
     os.replace(tmp, LIB_PATH)
     return LIB_PATH
 

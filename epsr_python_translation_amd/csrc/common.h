@@ -279,3 +279,26 @@ __device__ __forceinline__ float mic_sq(float x) { return x - rintf(x); }
 __device__ __forceinline__ int dev_ifix_pos(float x) { return (int)truncf(x); }
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+
+// XCD-aware work split.  MI355X dispatches workgroup b of a grid to XCD b % 8, and each XCD
+// has an L2 of its own.  For a grid whose size is a multiple of 8 (xcd_grid), xcd_block()
+// renumbers the workgroups so that every XCD holds one contiguous range of logical blocks,
+// and xcd_range() gives every XCD one contiguous eighth of n items, walked by its
+// workgroups in order: work that touches neighbouring data (chain-ordered ledger lists and
+// the lists of their partners) then finds it in the L2 of the XCD that touched it last.
+inline int xcd_grid(long long blocks) { return (int)(((blocks < 8 ? 8 : blocks) + 7) / 8 * 8); }
+__device__ __forceinline__ int xcd_block() { return (int)((blockIdx.x & 7u) * (gridDim.x >> 3) + (blockIdx.x >> 3)); }
+struct XcdRange {
+    long long lo, hi;   // this XCD's items [lo, hi)
+    long long first, stride;   // this thread's first item and the stride of its loop
+};
+__device__ __forceinline__ XcdRange xcd_range(long long n)
+{
+    const long long x = blockIdx.x & 7u, nj = gridDim.x >> 3, j = blockIdx.x >> 3;
+    XcdRange r;
+    r.lo = n * x / 8;
+    r.hi = n * (x + 1) / 8;
+    r.first = r.lo + j * blockDim.x + threadIdx.x;
+    r.stride = nj * blockDim.x;
+    return r;
+}

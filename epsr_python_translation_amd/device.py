@@ -426,9 +426,10 @@ class MCState:
     def __init__(self, ctx: Context, nmol, nmoltype, simt, stepr=1.0, stepm=1.0, rejrate=0.75, ireset=0,
                  wholerot=None, wholetrans=None):
         self.params = _native.MCParams(F32(simt), F32(stepr), F32(stepm), F32(rejrate), int(ireset))
-        self.wholerot = np.ascontiguousarray(np.zeros(nmoltype, F32) if wholerot is None else wholerot, dtype=F32)
-        self.wholetrans = np.ascontiguousarray(np.zeros(nmoltype, F32) if wholetrans is None else wholetrans,
-                                               dtype=F32)
+        # copies: the cycles adapt the step sizes in place, and the caller's arrays stay as given
+        self.wholerot = np.array(np.zeros(nmoltype, F32) if wholerot is None else wholerot, dtype=F32, copy=True)
+        self.wholetrans = np.array(np.zeros(nmoltype, F32) if wholetrans is None else wholetrans, dtype=F32,
+                                   copy=True)
         self.nwhole = np.zeros(2 * nmoltype, np.int32)
         self.nmovem = np.zeros(2 * nmoltype, np.int32)
         self.nstepreject = ctx.zeros(nmol, torch.int32)

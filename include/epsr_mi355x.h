@@ -81,9 +81,11 @@ int epsr_box_create(epsr_ctx *ctx, const epsr_box_desc_t *desc, epsr_box **out);
  * resident workgroups of atogen's persistent pair kernel (each flushes one histogram;
  * default 768), and EPSR_LAUNCH_PAIR_BLOCK, the workgroup size of mc_pairs and fmolec1's pair
  * kernel (256, 512 or 1024; default 1024).  Results do not depend on them.
- * EPSR_LAUNCH_RESOLVE_PASSES caps the passes of molfit's one-workgroup resolution (default
- * nmol + 1, which always suffices); a cycle that ends with moves undecided returns
- * EPSR_ERR_HIP.  A smaller cap (>= 0) exists to test that path. */
+ * EPSR_LAUNCH_RESOLVE_PASSES caps the passes of molfit's one-workgroup resolution (default:
+ * no cap, nmol + 1 passes, which always suffice; a cap also leaves the grid-wide passes at
+ * one, so that the finisher has work); a cycle that ends with moves undecided returns
+ * EPSR_ERR_HIP.  The cap (>= 0) exists to test that path: results do not depend on it when
+ * it is large enough. */
 #define EPSR_LAUNCH_ATOGEN_WORKGROUPS 1
 #define EPSR_LAUNCH_PAIR_BLOCK 2
 #define EPSR_LAUNCH_RESOLVE_PASSES 3

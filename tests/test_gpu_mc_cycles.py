@@ -48,7 +48,10 @@ def _state(box, led, mc, st):
     out.update(ran1=st.copy(), wr=mc.wholerot.view(np.int32).copy(), wt=mc.wholetrans.view(np.int32).copy(),
                nwhole=mc.nwhole.copy(), nmovem=mc.nmovem.copy(), nsr=mc.nstepreject.cpu().numpy().copy(),
                fadds=F32(mc.fadds).view(np.int32), erg=led.export()["ergpair"].view(np.int32).copy(),
-               accepted=mc.stats.accepted, exact=mc.stats.exact_resolutions, passes=mc.stats.resolver_passes)
+               accepted=mc.stats.accepted)
+    # (the resolver's pass and exact-sum counts are left out: moves decided within a grid-wide
+    # pass depend on which partners other workgroups have decided by then, so those counts
+    # vary from run to run while every decision is the chain's)
     return out
 
 
