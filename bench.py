@@ -545,41 +545,60 @@ def surrogate_leg(D, PL, synth, ctx, nsamples, timed_sweeps=20, gr_sweeps=20):
         acc = mov = rej = 0
         torch.cuda.synchronize()
         t = time.perf_counter()
-        fwd = []
+        fwd, desc, pairs, res = [], [], [], []
         for _ in range(n):
             if da is None:
                 R.molfit(tables)
             else:
                 da.sweep(ctx, R.box, tables, R.ledger, R.mc, R.ran1)
                 fwd.append(ctx.kernel_ms("mlp_forward"))
+                desc.append(ctx.kernel_ms("surr_descriptors"))
+            pairs.append(ctx.kernel_ms("mc_pair"))
+            res.append(ctx.kernel_ms("mc_resolve"))
             acc += R.mc.stats.accepted
             mov += R.mc.stats.moves
             rej += R.mc.stats.stage1_rejected
         torch.cuda.synchronize()
-        return acc, mov, rej, time.perf_counter() - t, fwd
+        kms = {"mc_pair": float(np.mean(pairs)), "mc_resolve": float(np.mean(res))}
+        if desc:
+            kms.update(surr_descriptors=float(np.mean(desc)), mlp_forward=float(np.mean(fwd)))
+        return acc, mov, rej, time.perf_counter() - t, fwd, kms
 
     dam = SG.DelayedAcceptance(net, spec, seed=4242)
     run(None, 2)
     run(dam, 2)
-    ap_, mp, _, tp, _ = run(None, timed_sweeps)
-    ad, md, rd, td, fwd = run(dam, timed_sweeps)
+    ap_, mp, _, tp, _, kp = run(None, timed_sweeps)
+    ad, md, rd, td, fwd, kd = run(dam, timed_sweeps)
 
     def hist_segment(da):
-        acc = None
+        rows = []
         for _ in range(gr_sweeps):
             if da is None:
                 R.molfit(tables)
             else:
                 da.sweep(ctx, R.box, tables, R.ledger, R.mc, R.ran1)
             R.atogen(ngrsamples=s.nmol)
-            w = R.wav.view(2, s.npairs, R.ldw)[1].double().clone()
-            acc = w if acc is None else acc + w
-        return acc.cpu().numpy()
+            rows.append(R.wav.view(2, s.npairs, R.ldw)[1].double().cpu().numpy().copy())
+        return np.stack(rows)
 
-    hA = hist_segment(None)
-    hB = hist_segment(dam)
-    hC = hist_segment(None)
+    sA = hist_segment(None)
+    sB = hist_segment(dam)
+    sC = hist_segment(None)
+    hA, hB, hC = sA.sum(0), sB.sum(0), sC.sum(0)
     dist = lambda a, b: float(np.sum((a - b) ** 2) / np.sum(((a + b) / 2) ** 2))
+
+    def chi2(a, b):
+        """chi^2 per degree of freedom of the difference of two segments' mean intermolecular
+        g(r) histograms, each bin's variance of the mean from its sweep-to-sweep spread"""
+        va = a.var(0, ddof=1) / a.shape[0]
+        vb = b.var(0, ddof=1) / b.shape[0]
+        v = va + vb
+        ok = v > 0
+        d = (a.mean(0) - b.mean(0))[ok]
+        return float(np.sum(d * d / v[ok]) / max(1, int(ok.sum()))), int(ok.sum())
+
+    c_da, dof = chi2(sB, np.concatenate([sA, sC]))
+    c_pp, _ = chi2(sA, sC)
     plain_rate = ap_ / tp
     da_rate = ad / td
     flops_row = 2.0 * (ldx * 128 + 128 * 128 + 128)
@@ -609,6 +628,11 @@ def surrogate_leg(D, PL, synth, ctx, nsamples, timed_sweeps=20, gr_sweeps=20):
         "histogram_distance": {"plain_vs_da": dist(hB, (hA + hC) / 2), "plain_vs_plain": dist(hA, hC),
                                "definition": "sum (a-b)^2 / sum ((a+b)/2)^2 over the intermolecular pair histograms "
                                              f"summed over {gr_sweeps} sweeps per segment"},
+        "gr_chi2": {"da_vs_plain_per_dof": c_da, "plain_vs_plain_per_dof": c_pp, "dof": dof,
+                    "definition": "sum over bins of (mean_a - mean_b)^2 / (var_a/n_a + var_b/n_b) / dof, the "
+                                  "intermolecular g(r) histograms of segments plain | DA | plain, "
+                                  f"{gr_sweeps} sweeps each (DA against both plain segments)"},
+        "kernel_ms_per_sweep": {"plain": kp, "delayed_acceptance": kd},
     }
 
 

@@ -346,8 +346,8 @@ __global__ void pack_tables(const float *__restrict__ xavs, const float *__restr
     ((float2 *)(T + (size_t)npairs * nnear))[t] = make_float2(ecoul[o], ecoul1[o]);
 }
 
-// out[0]: the last bin (up to lim) where any of the four tables has a non-zero bit
-// pattern; out[1]: the same for xavs and xavs1
+// out[0]: the last bin (up to lim) where any of the four tables has a non-zero value;
+// out[1]: the same for xavs and xavs1
 __global__ void last_nonzero(const float *__restrict__ a, const float *__restrict__ b, const float *__restrict__ c,
                              const float *__restrict__ d, int ld, int npairs, int lim, int *out)
 {
@@ -355,9 +355,14 @@ __global__ void last_nonzero(const float *__restrict__ a, const float *__restric
     if (t >= npairs * ld) return;
     int iz = t % ld;
     if (iz > lim) return;
-    const int ab = __float_as_int(a[t]) | __float_as_int(b[t]);
+    // by value: a bin whose four entries are all +0 or -0 adds exactly nothing to a pair sum
+    // (the lookup gives +0 or -0 there, and a sum that starts at +0 is never -0, so adding
+    // either leaves it unchanged), the same as a bin beyond the packed tables; likewise
+    // a -0 of xavs or xavs1 read as the +0 of the far part only changes the sign of a
+    // zero lookup
+    const bool ab = a[t] != 0.0f || b[t] != 0.0f;
     if (ab) atomicMax(out + 1, iz);
-    if (ab | __float_as_int(c[t]) | __float_as_int(d[t])) atomicMax(out, iz);
+    if (ab || c[t] != 0.0f || d[t] != 0.0f) atomicMax(out, iz);
 }
 
 // absolute atom positions (x = xa + xadd, one fp32 add) with the type in .w
@@ -515,8 +520,9 @@ struct TabCheck {
     int ld, np, lim, nte, nnear;
 };
 
-// any bit of the arrays (bins up to lim) that the packing does not reproduce sets
-// TABLES_STALE in *flag: a changed value, or a non-zero value beyond the packed extents
+// any entry of the arrays (bins up to lim) that the packing does not reproduce sets
+// TABLES_STALE in *flag: a changed bit pattern within the packed extents, or a non-zero
+// value where the packing keeps none
 __device__ void tables_verify(const TabCheck &tc, int *flag)
 {
     if (!tc.T) return;
@@ -533,10 +539,11 @@ __device__ void tables_verify(const TabCheck &tc, int *flag)
             bad |= a != __float_as_int(v.x) || b != __float_as_int(v.y) || c != __float_as_int(v.z) ||
                    d != __float_as_int(v.w);
         } else if (iz < tc.nte) {
+            // (zeros of either sign stand for the +0 of the far part, as the packing measures)
             const float2 f = F[ic * nfar + (iz - tc.nnear)];
-            bad |= (a | b) != 0 || c != __float_as_int(f.x) || d != __float_as_int(f.y);
+            bad |= tc.a[t] != 0.0f || tc.b[t] != 0.0f || c != __float_as_int(f.x) || d != __float_as_int(f.y);
         } else {
-            bad |= (a | b | c | d) != 0;
+            bad |= tc.a[t] != 0.0f || tc.b[t] != 0.0f || tc.c[t] != 0.0f || tc.d[t] != 0.0f;
         }
     }
     if (bad) atomicOr(flag, TABLES_STALE);

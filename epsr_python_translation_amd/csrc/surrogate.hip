@@ -12,7 +12,6 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 #define MLP_H 128
-#define DESC_CAP 4096    // in-range atom pairs staged in LDS per flush
 #define DESC_MAXD 2048   // channels * Gaussians
 
 // float -> bf16, round to nearest even (NaN stays NaN)
@@ -33,79 +32,164 @@ __device__ __forceinline__ float dsilu(float z)
 }
 
 // ------------------------------------------------------------------ descriptors
-// One workgroup per row (chain position k, current or trial).  The in-range atom pairs
-// (distance, channel) of the molecule and its later partners are staged in LDS, then
-// each half-wave expands one pair into its ng Gaussians (lane g) and adds them, in 2^-32
-// fixed point, to the row held in LDS: integer sums do not depend on the order in which
-// the pairs arrive, so a row is a deterministic function of the coordinates.
-__global__ void __launch_bounds__(256)
+// One wave per row (chain position k, current or trial).  The later partners' atoms are
+// gathered into LDS a batch at a time, and every (atom of the molecule, partner atom) pair
+// of a batch with r < rc is staged in the wave's LDS list (compacted by ballot) with its
+// cutoff factor, formed once per pair.  At a flush the list is sorted by channel (a
+// counting sort); then, channel by channel, lane g of each half-wave sums the g-th
+// Gaussian of every other pair in a register, in 2^-32 fixed point; the two halves' sums
+// are added and go to the row's accumulators in LDS once per channel.  Integer sums do not depend
+// on the order of the pairs, so a row is a deterministic function of the coordinates.
+#define DESC_WCAP 192    // staged pairs per wave
+#define DESC_PCAP 128    // partner atoms gathered per batch
+#define DESC_MAXCH 256   // channels (type pairs)
+// LDS of the wave, in bytes: D accumulators (8 bytes each), the staged pairs (r, factor)
+// and their channels, the sorted pairs, the partner atoms, channel counts and starts
+#define DESC_LDS(D) (8 * ((size_t)(D) + 1) + 20 * DESC_WCAP + 16 * DESC_PCAP + 8 * (DESC_MAXCH + 2))
+__global__ void __launch_bounds__(64)
 surr_descriptors(int nmol, const int *__restrict__ order, const int *__restrict__ pos,
                  const int *__restrict__ rowstart, const int *__restrict__ cnt, const int *__restrict__ partner,
                  const int *__restrict__ naf, const int *__restrict__ nal, const int *__restrict__ npr, int ntype,
                  const float4 *__restrict__ Pold, const float4 *__restrict__ Pnew, float dmax, DescDev dp, int nch,
                  int maxat, int ldx, uint16_t *__restrict__ X, float *__restrict__ Xf)
 {
-    __shared__ unsigned long long acc[DESC_MAXD];
-    __shared__ float2 list[DESC_CAP];
-    __shared__ int count;
+    extern __shared__ unsigned long long dsm[];
+    const int lane = threadIdx.x;
     const int row = blockIdx.x;
+    const int D = nch * dp.ng;
+    unsigned long long *acc = dsm;                              // [D]
+    float4 *pq = (float4 *)(acc + D + (D & 1));                 // [DESC_PCAP] partner atoms
+    float2 *lrf = (float2 *)(pq + DESC_PCAP);                   // [DESC_WCAP] staged (r, factor)
+    float2 *srt = lrf + DESC_WCAP;                              // [DESC_WCAP] sorted by channel
+    int *lch = (int *)(srt + DESC_WCAP);                        // [DESC_WCAP] staged channels
+    int *chn = lch + DESC_WCAP;                                 // [DESC_MAXCH + 1] counts, cursors
+    int *chs = chn + DESC_MAXCH + 1;                            // [DESC_MAXCH + 1] starts
+    for (int i = lane; i < D; i += 64) acc[i] = 0ull;
     const int k = row >> 1;
     const int m = order[k];
     const int a0 = naf[m], na = nal[m] - a0 + 1;
     const float4 *Pm = ((row & 1) ? Pnew : Pold) + a0;
-    const int D = nch * dp.ng;
-    for (int i = threadIdx.x; i < D; i += blockDim.x) acc[i] = 0ull;
-    if (threadIdx.x == 0) count = 0;
     const int rs = rowstart[m], c = cnt[m];
-    const int nitems = (c - 1) * na;
-    const int per_round = min((int)blockDim.x, DESC_CAP / maxat);
+    const int spc = min(64, DESC_PCAP / maxat);   // slots per partner chunk: a chunk always fits pq
     const float rcf = dp.rc;
-    const float pi = 3.14159265358979323846f;
-    __syncthreads();
+    const float pi_rc = 3.14159265358979323846f / rcf;
+    const int h = lane >> 5, g = lane & 31;
+    const float mu = dp.r0 + (float)g * dp.dmu;
+    int nl = 0;   // staged pairs (wave-uniform)
 
+    auto wave_sync = []() {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
     auto flush = [&]() {
-        const int n = count;
-        const int g = threadIdx.x & 31;
-        for (int e = threadIdx.x >> 5; e < n; e += blockDim.x >> 5) {
-            if (g >= dp.ng) continue;
-            const float2 v = list[e];
-            const float r = v.x;
-            const int ch = __float_as_int(v.y);
-            const float t = (r - (dp.r0 + (float)g * dp.dmu)) * dp.inv_w;
-            const float fc = 0.5f * (cosf(pi * r / rcf) + 1.0f);
-            const float G = expf(-t * t) * fc;
-            atomicAdd(&acc[ch * dp.ng + g], (unsigned long long)(G * 4294967296.0f));
+        for (int i = lane; i <= nch; i += 64) chn[i] = 0;
+        wave_sync();
+        for (int e = lane; e < nl; e += 64) atomicAdd(&chn[lch[e]], 1);
+        wave_sync();
+        if (lane == 0) {
+            int run = 0;
+            for (int ch = 0; ch < nch; ++ch) {
+                chs[ch] = run;
+                run += chn[ch];
+                chn[ch] = chs[ch];
+            }
+            chs[nch] = run;
         }
-        __syncthreads();
-        if (threadIdx.x == 0) count = 0;
-        __syncthreads();
+        wave_sync();
+        for (int e = lane; e < nl; e += 64) srt[atomicAdd(&chn[lch[e]], 1)] = lrf[e];
+        wave_sync();
+        for (int ch = 0; ch < nch; ++ch) {
+            const int e0 = chs[ch], e1 = chs[ch + 1];
+            if (e0 == e1) continue;
+            unsigned long long sum = 0ull;
+#pragma unroll 4
+            for (int e = e0 + h; e < e1; e += 2) {
+                const float2 v = srt[e];
+                const float t = (v.x - mu) * dp.inv_w;
+                // the hardware exponential: within a few 1e-7 relative of exp, far inside
+                // the descriptors' tolerance against the FP64 definition
+                const float G = __expf(-t * t) * v.y;   // < 1, so G * 2^32 fits 32 bits
+                sum += (unsigned long long)(unsigned)(G * 4294967296.0f);
+            }
+            sum += __shfl_xor(sum, 32);   // the two half-waves' sums, then one writer per Gaussian
+            if (h == 0 && g < dp.ng) acc[ch * dp.ng + g] += sum;
+        }
+        wave_sync();
+        nl = 0;
     };
 
-    for (int base = 0; base < nitems; base += per_round) {
-        const int i = base + (int)threadIdx.x;
-        if ((int)threadIdx.x < per_round && i < nitems) {
-            const int s = 1 + i / na, a = i - (s - 1) * na;
-            const int n = partner[rs + s];
-            if (pos[n] > k) {
-                const float4 p = Pm[a];
-                const int ta = __float_as_int(p.w) * ntype;
-                for (int b = naf[n]; b <= nal[n]; ++b) {
-                    const float4 q = Pold[b];
-                    const float dx = dev_mic(q.x - p.x), dy = dev_mic(q.y - p.y), dz = dev_mic(q.z - p.z);
-                    const float r = sqrtf(dx * dx + dy * dy + dz * dz) * dmax;
-                    if (r < rcf) {
-                        const int slot = atomicAdd(&count, 1);
-                        list[slot] = make_float2(r, __int_as_float(npr[ta + __float_as_int(q.w)]));
-                    }
+    // the later partners' atoms, DESC_PCAP at a time, in slot order, then every (atom of the
+    // molecule, partner atom) pair of the batch, 64 at a time
+    int s0 = 1;
+    while (s0 < c) {
+        int np = 0;
+        for (; s0 < c; s0 += spc) {
+            const int s = s0 + lane;
+            int nat = 0, b0 = 0;
+            if (lane < spc && s < c) {
+                const int n = partner[rs + s];
+                if (pos[n] > k) {
+                    b0 = naf[n];
+                    nat = nal[n] - b0 + 1;
                 }
             }
+            int pre = nat;   // inclusive prefix of the atom counts over the lanes
+            for (int off = 1; off < 64; off <<= 1) {
+                const int t = __shfl_up(pre, off);
+                if (lane >= off) pre += t;
+            }
+            const int tot = __shfl(pre, 63);
+            if (np + tot > DESC_PCAP) break;   // (then np > 0: a chunk alone fits, spc * maxat <= DESC_PCAP)
+            for (int j = 0; j < nat; ++j) pq[np + pre - nat + j] = Pold[b0 + j];
+            np += tot;
         }
-        __syncthreads();
-        if (count > DESC_CAP - per_round * maxat) flush();
+        wave_sync();
+        const int npairs = na * np;
+        // lane's pair i = a * np + t, advanced by 64 each step without a division
+        int a = 0, t = lane;
+        while (np > 0 && t >= np) {
+            t -= np;
+            ++a;
+        }
+        for (int base = 0; base < npairs; base += 64) {
+            const int i = base + lane;
+            bool in = false;
+            float r = 0.0f;
+            int ch = 0;
+            if (i < npairs) {
+                const float4 p = Pm[a], q = pq[t];
+                // only the distance is used: x - rint(x) and the hardware square root
+                // (the descriptors are checked against the FP64 definition, not bit for bit)
+                const float dx = mic_sq(q.x - p.x), dy = mic_sq(q.y - p.y), dz = mic_sq(q.z - p.z);
+                r = __builtin_amdgcn_sqrtf(dx * dx + dy * dy + dz * dz) * dmax;
+                in = r < rcf;
+                ch = npr[__float_as_int(p.w) * ntype + __float_as_int(q.w)];
+            }
+            t += 64;
+            while (t >= np) {
+                t -= np;
+                ++a;
+            }
+            const unsigned long long mask = __ballot(in);
+            if (in) {
+                const int at = nl + __popcll(mask & ((1ull << lane) - 1ull));
+                lrf[at] = make_float2(r, 0.5f * (__cosf(pi_rc * r) + 1.0f));
+                lch[at] = ch;
+            }
+            nl += __popcll(mask);
+            if (nl > DESC_WCAP - 64) {
+                wave_sync();
+                flush();
+            }
+        }
+        wave_sync();
     }
-    __syncthreads();
-    if (count > 0) flush();
-    for (int i = threadIdx.x; i < ldx; i += blockDim.x) {
+    if (nl > 0) {
+        wave_sync();
+        flush();
+    }
+    for (int i = lane; i < ldx; i += 64) {
         float v = i < D ? (float)((double)acc[i] * 2.3283064365386963e-10) * dp.xscale : 0.0f;
         X[(size_t)row * ldx + i] = f2bf(v);
         if (Xf) Xf[(size_t)row * ldx + i] = v;
@@ -121,13 +205,16 @@ int surr_descriptors_launch(epsr_ctx *ctx, const epsr_box *box, const epsr_ledge
     ARG_CHECK(ctx, D <= DESC_MAXD, "descriptors: %d channels x %d Gaussians exceed %d", box->npairs, dp->ng,
               DESC_MAXD);
     ARG_CHECK(ctx, ldx >= D && ldx % 32 == 0, "descriptors: ldx=%d must be >= %d and a multiple of 32", ldx, D);
-    ARG_CHECK(ctx, box->max_atoms_per_mol <= DESC_CAP / 64, "descriptors: molecules of %d atoms are too large",
+    ARG_CHECK(ctx, box->max_atoms_per_mol <= DESC_PCAP, "descriptors: molecules of %d atoms are too large",
               box->max_atoms_per_mol);
+    ARG_CHECK(ctx, box->npairs <= DESC_MAXCH, "descriptors: %d channels exceed %d", box->npairs, DESC_MAXCH);
     DescDev d{dp->ng, dp->rc, dp->r0, dp->dmu, 1.0f / dp->width, dp->xscale};
-    surr_descriptors<<<2 * box->nmol, 256, 0, ctx->stream>>>(
+    timer_begin(ctx, "surr_descriptors");
+    surr_descriptors<<<2 * box->nmol, 64, DESC_LDS(D), ctx->stream>>>(
         box->nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt, l->d_partner, box->d_naf, box->d_nal,
         box->d_npairref, box->ntype, Pold, Pnew, box->dmax, d, box->npairs, std::max(1, box->max_atoms_per_mol), ldx,
         x, xf);
+    timer_end(ctx, "surr_descriptors");
     HIP_CHECK(ctx, hipGetLastError());
     return EPSR_OK;
 }
