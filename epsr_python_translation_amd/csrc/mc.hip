@@ -594,17 +594,21 @@ __device__ __forceinline__ int nth_set_bit(unsigned long long m, int r)
 }
 
 // (8 waves per SIMD: two 1024-lane workgroups per CU when their tables fit LDS twice)
-template <bool LDS>
-__global__ void __attribute__((amdgpu_flat_work_group_size(1, 1024), amdgpu_waves_per_eu(8))) fmolec1_pairs(const long long *__restrict__ rowoff, int nmol, const int *__restrict__ order,
-                              const int *__restrict__ mcell, const int *__restrict__ cstart,
-                              const int *__restrict__ nbflat, const float4 *__restrict__ P,
-                              const int *__restrict__ naf, const int *__restrict__ nal, PairTab gt, int npairs,
-                              float *__restrict__ rowE, int *__restrict__ rowJ, int *__restrict__ rowcnt)
+// NQ: candidate queues, one per partner molecule type when the box has at most NQ types
+// (NQ = 1: one queue for all)
+template <bool LDS, int NQ>
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, 1024), amdgpu_waves_per_eu(NQ == 1 ? 8 : 4)))
+fmolec1_pairs(const long long *__restrict__ rowoff, int nmol, const int *__restrict__ order,
+              const int *__restrict__ mcell, const int *__restrict__ cstart, const int *__restrict__ nbflat,
+              const float4 *__restrict__ P, const int *__restrict__ naf, const int *__restrict__ nal,
+              const int *__restrict__ mtype, PairTab gt, int npairs, float *__restrict__ rowE,
+              int *__restrict__ rowJ, int *__restrict__ rowcnt)
 {
     extern __shared__ char smem[];
     PairTab t = LDS ? tables_to_lds(gt, npairs, smem) : gt;
     const int lane = threadIdx.x & 63;
     const int nw = gridDim.x * (blockDim.x >> 6);
+    const unsigned long long below = (1ull << lane) - 1ull;   // lanes before this one
     for (int k = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); k < nmol; k += nw) {
         const long long r0 = rowoff[k];
         const int len = (int)(rowoff[k + 1] - r0);
@@ -612,23 +616,38 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, 1024), amdgpu_wave
         const int *nb = nbflat + cstart[mcell[im]];
         const float4 *P1 = P + naf[im];
         const int n1 = nal[im] - naf[im] + 1;
-        // Only candidates later in the numbering are evaluated (half of them).  They are
-        // gathered, in candidate order, into a queue of one partner per lane and evaluated
-        // 64 at a time, so no lane idles on an earlier partner; the stored energies keep the
-        // candidate order.
-        int base = 0, qj = 0, qn = 0;   // qj: this lane's queued partner, valid for lane < qn
-        auto flush = [&](int nvalid) {
+        // Only candidates later in the numbering are evaluated (half of them).  Each gets its
+        // index ci among the evaluated candidates, in candidate order, and joins the queue of
+        // its molecule type (one partner per lane); a full queue is evaluated 64 at a time,
+        // so the lanes of a batch meet partners of one size and none idles on an earlier
+        // partner.  The energies are staged at their index ci, then the non-zero ones are
+        // compacted in candidate order.
+        int qj[NQ], qi[NQ], qn[NQ];   // lane's queued partner and its index; queue lengths
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) {
+            qj[c] = 0;
+            qi[c] = 0;
+            qn[c] = 0;
+        }
+        int nev = 0;    // evaluated candidates so far (wave-uniform)
+        int base = 0;   // stored entries (one queue: written in order at once)
+        auto flush = [&](int c, int nvalid) {
             float e = 0.0f;
-            if (lane < nvalid) e = mol_pair_any(P1, n1, P + naf[qj], nal[qj] - naf[qj] + 1, t);
-            const bool keep = e != 0.0f;
-            const unsigned long long mask = __ballot(keep);
-            if (keep) {
-                const int at = base + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
-                                                                     __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
-                rowE[r0 + at] = e;
-                rowJ[r0 + at] = qj;
+            const int j = qj[c];
+            if (lane < nvalid) e = mol_pair_any(P1, n1, P + naf[j], nal[j] - naf[j] + 1, t);
+            if constexpr (NQ == 1) {
+                const bool keep = e != 0.0f;
+                const unsigned long long mask = __ballot(keep);
+                if (keep) {
+                    const int at = base + __popcll(mask & below);
+                    rowE[r0 + at] = e;
+                    rowJ[r0 + at] = j;
+                }
+                base += __popcll(mask);
+            } else if (lane < nvalid) {
+                rowE[r0 + qi[c]] = e;
+                rowJ[r0 + qi[c]] = j;
             }
-            base += __popcll(mask);
         };
         for (int i0 = 0; i0 < len; i0 += 64) {   // wave-uniform trip count
             const int inn = i0 + lane;
@@ -639,24 +658,64 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, 1024), amdgpu_wave
                 work = inn >= 1 && jm > im;
             }
             const unsigned long long wm = __ballot(work);
-            const int cnt = __popcll(wm);
-            // queue lanes qn .. qn + cnt - 1 (those below 64) take the working candidates in order
-            const int r = lane - qn;
-            const bool take = r >= 0 && r < cnt;
-            const int v = __shfl(jm, take ? nth_set_bit(wm, r) : lane);
-            if (take) qj = v;
-            if (qn + cnt >= 64) {
-                flush(64);
-                const int r2 = lane + 64 - qn;   // the rest: ranks 64 - qn .. cnt - 1
-                const bool rest = r2 < cnt;
-                const int v2 = __shfl(jm, rest ? nth_set_bit(wm, r2) : lane);
-                qj = rest ? v2 : 0;
-                qn = qn + cnt - 64;
-            } else {
-                qn += cnt;
+            const int ci = NQ == 1 ? 0 : nev + __popcll(wm & below);
+            nev += __popcll(wm);
+            const int cls = (NQ == 1 || !work) ? 0 : mtype[jm];
+#pragma unroll
+            for (int c = 0; c < NQ; ++c) {
+                const unsigned long long cm = NQ == 1 ? wm : __ballot(work && cls == c);
+                const int cnt = __popcll(cm);
+                if (cnt == 0) continue;
+                // queue lanes qn .. qn + cnt - 1 (those below 64) take the class's candidates in order
+                const int r = lane - qn[c];
+                const bool take = r >= 0 && r < cnt;
+                const int src = take ? nth_set_bit(cm, r) : lane;
+                const int vj = __shfl(jm, src), vi = NQ == 1 ? 0 : __shfl(ci, src);
+                if (take) {
+                    qj[c] = vj;
+                    qi[c] = vi;
+                }
+                if (qn[c] + cnt >= 64) {
+                    flush(c, 64);
+                    const int r2 = lane + 64 - qn[c];   // the rest: ranks 64 - qn .. cnt - 1
+                    const bool rest = r2 < cnt;
+                    const int src2 = rest ? nth_set_bit(cm, r2) : lane;
+                    const int vj2 = __shfl(jm, src2), vi2 = NQ == 1 ? 0 : __shfl(ci, src2);
+                    qj[c] = rest ? vj2 : 0;
+                    qi[c] = rest ? vi2 : 0;
+                    qn[c] = qn[c] + cnt - 64;
+                } else {
+                    qn[c] += cnt;
+                }
             }
         }
-        if (qn > 0) flush(qn);
+#pragma unroll
+        for (int c = 0; c < NQ; ++c)
+            if (qn[c] > 0) flush(c, qn[c]);
+        if constexpr (NQ > 1) {
+        // the staged entries written by other lanes of the wave, then compacted in order
+        // (a chunk is read whole before it is written to entries at or before it)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        for (int i0 = 0; i0 < nev; i0 += 64) {
+            const int idx = i0 + lane;
+            float e = 0.0f;
+            int j = 0;
+            if (idx < nev) {
+                e = rowE[r0 + idx];
+                j = rowJ[r0 + idx];
+            }
+            const bool keep = idx < nev && e != 0.0f;
+            const unsigned long long mask = __ballot(keep);
+            if (keep) {
+                const int at = base + __popcll(mask & below);
+                rowE[r0 + at] = e;
+                rowJ[r0 + at] = j;
+            }
+            base += __popcll(mask);
+        }
+        }
         if (lane == 0) rowcnt[k] = base;
     }
 }
@@ -1046,16 +1105,23 @@ extern "C" int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coord
     {
         const int B = box->pair_block;
         timer_begin(ctx, "fmolec1_pair");
+        // one candidate queue per molecule type (up to 4 types; more share one queue)
+        const int nq = box->nmoltype <= 1 ? 1 : box->nmoltype <= 2 ? 2 : box->nmoltype <= 4 ? 4 : 1;
+        const size_t lds = pt.in_lds ? pt.lds_bytes : 0;
+        const int g = persistent_blocks(lds, (long long)nmol * 64, B);
+#define FM1_LAUNCH(L, Q)                                                                                               \
+    fmolec1_pairs<L, Q><<<g, B, lds, st>>>(drow, nmol, l->d_order, dmcell, dcstart, dnb, dP, box->d_naf, box->d_nal,  \
+                                           box->d_mtype, pt.tab, box->npairs, rowE, rowJ, cat + nmol)
         if (pt.in_lds) {
-            int g = persistent_blocks(pt.lds_bytes, (long long)nmol * 64, B);
-            fmolec1_pairs<true><<<g, B, pt.lds_bytes, st>>>(drow, nmol, l->d_order, dmcell, dcstart, dnb, dP,
-                                                            box->d_naf, box->d_nal, pt.tab, box->npairs, rowE, rowJ,
-                                                            cat + nmol);
+            if (nq == 1) FM1_LAUNCH(true, 1);
+            else if (nq == 2) FM1_LAUNCH(true, 2);
+            else FM1_LAUNCH(true, 4);
         } else {
-            int g = persistent_blocks(0, (long long)nmol * 64, B);
-            fmolec1_pairs<false><<<g, B, 0, st>>>(drow, nmol, l->d_order, dmcell, dcstart, dnb, dP, box->d_naf,
-                                                  box->d_nal, pt.tab, box->npairs, rowE, rowJ, cat + nmol);
+            if (nq == 1) FM1_LAUNCH(false, 1);
+            else if (nq == 2) FM1_LAUNCH(false, 2);
+            else FM1_LAUNCH(false, 4);
         }
+#undef FM1_LAUNCH
         timer_end(ctx, "fmolec1_pair");
     }
     fmolec1_self_flags<<<ceil_div(nmol + 1, 256), 256, 0, st>>>(eintra, nmol, cat);
