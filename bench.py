@@ -488,6 +488,7 @@ def replicas_leg(ctx, nrep=16, workers=8, iters=50, sweeps=5):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / iters
     atoms = sum(s.natom for s in ens.systems)
+    tb = transforms_batch_rate(ens.boxes)
     del ens
     pick = sorted(set([0, 1, 2, 5, 10, 20, 30, 40, len(rfd) - 1]))
     return {"workload": f"{nrep} replicas x SiO2 24,000 atoms, 3 datasets (isotope fractions "
@@ -501,7 +502,44 @@ def replicas_leg(ctx, nrep=16, workers=8, iters=50, sweeps=5):
             "rfactor_definition": "rfactor_data: the program's 'R-factor for data file' (mean over Q of (data - "
                                   "weighted simulated S(Q))^2 on the running averages), mean over datasets and "
                                   "replicas; rfactor_ep_fit: ptgnsq's relative error of the EP fit to the "
-                                  "difference"}
+                                  "difference",
+            "transforms_batched": tb}
+
+
+def transforms_batch_rate(boxes, reps=5):
+    """gensq and stog of the replicas as one FP32 MFMA GEMM each (the iteration's path),
+    timed with HIP events on a context of its own, beside the same transforms box by box."""
+    import torch
+
+    from epsr_python_translation_amd import device as D
+    from epsr_python_translation_amd import pipeline as PL
+    tctx = D.Context(0, timing=True)
+    R0 = boxes[0]
+    g, npairs, nb = R0.g, R0.s.npairs, len(boxes)
+    gs, ss = [], []
+    for _ in range(reps + 1):
+        PL.transforms_batch(tctx, boxes)
+        gs.append(tctx.kernel_ms("gensq"))
+        ss.append(tctx.kernel_ms("stog"))
+    gb, sb = float(np.median(gs[1:])), float(np.median(ss[1:]))
+    one_g = one_s = 0.0
+    for R in boxes:
+        R.gensq(mode=1, ctx=tctx)
+        one_g += tctx.kernel_ms("gensq")
+        R.stog(mode=1, ctx=tctx)
+        one_s += tctx.kernel_ms("stog")
+    torch.cuda.synchronize()
+    fg = 2.0 * g.nq * g.npt1 * 2 * npairs * nb
+    fs = 2.0 * R0.nr * g.nq * npairs * nb
+    return {"boxes": nb, "gensq": {"gemm": f"[{g.nq} x {g.npt1}] . [{g.npt1} x {2 * npairs * nb}]", "ms": gb,
+                                   "tflops": fg / (gb * 1e-3) / 1e12,
+                                   "frac_fp32_mfma": fg / (gb * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
+                                   "box_by_box_ms": one_g},
+            "stog": {"gemm": f"[{R0.nr} x {g.nq}] . [{g.nq} x {npairs * nb}]", "ms": sb,
+                     "tflops": fs / (sb * 1e-3) / 1e12, "frac_fp32_mfma": fs / (sb * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
+                     "box_by_box_ms": one_s},
+            "note": "FP32 MFMA (v_mfma_f32_16x16x4_f32) against MI355X's 157.3 TFLOP/s; each box's outputs equal "
+                    "its own transforms bit for bit"}
 
 
 def surrogate_leg(D, PL, synth, ctx, nsamples, timed_sweeps=20, gr_sweeps=20):

@@ -90,6 +90,16 @@ class BoxDesc(C.Structure):
                 ("cellst", C.c_float)]
 
 
+class GensqBox(C.Structure):
+    _fields_ = [("npairs", C.c_int), ("ldw", C.c_int), ("ldq", C.c_int), ("wav", vp), ("natomsphere", vp),
+                ("sofq", vp)]
+
+
+class StogBox(C.Structure):
+    _fields_ = [("sofq", vp), ("gofr", vp), ("nqmin", vp), ("nqmax", vp), ("xwt", vp), ("ncols", C.c_int),
+                ("ldg", C.c_int)]
+
+
 class Coords(C.Structure):
     _fields_ = [("xa", vp), ("ya", vp), ("za", vp), ("xadd", vp), ("yadd", vp), ("zadd", vp)]
 
@@ -190,6 +200,8 @@ SIGNATURES = {
                                      C.c_int]),
     "epsr_stog": (C.c_int, [vp, C.c_float, vp, vp, C.c_int, vp, vp, C.c_int, vp, vp, C.c_int, C.c_int, vp,
                             C.c_int]),
+    "epsr_gensq_batch": (C.c_int, [vp, C.c_int, C.c_int, vp, vp, C.c_int, C.c_int, vp]),
+    "epsr_stog_batch": (C.c_int, [vp, C.c_float, vp, C.c_int, vp, C.c_int, C.c_int, vp]),
     "epsr_ledger_create": (C.c_int, [vp, vp, C.POINTER(vp)]),
     "epsr_ledger_destroy": (None, [vp]),
     "epsr_cells_init": (C.c_int, [vp, vp, vp, C.POINTER(Coords), C.c_int, C.c_float]),

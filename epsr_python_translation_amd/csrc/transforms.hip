@@ -6,7 +6,9 @@
 //          the reference evaluated with a correctly rounded sinf);
 //  mode 1  the same contraction as an FP32 MFMA GEMM (v_mfma_f32_16x16x4_f32) with
 //          FP32 accumulation; the right-hand operand is assembled on the fly.
+#include <cstring>
 #include <mutex>
+#include <vector>
 
 #include "common.h"
 
@@ -375,6 +377,236 @@ extern "C" int epsr_stog(epsr_ctx *ctx, float rho, const float *q, const float *
         stog_reduce<<<ceil_div((long)ncols * nr, 256), 256, 0, st>>>(rho, q, nr, ncols, ncolp, STOG_KSPLIT, part, r,
                                                                      gofr, ldg, xwt);
     }
+    timer_end(ctx, "stog");
+    HIP_CHECK(ctx, hipGetLastError());
+    HIP_CHECK(ctx, ctx_sync(ctx));
+    return EPSR_OK;
+}
+
+// ------------------------------------------------------------------ batched transforms
+// gensq and stog of many boxes on the same grids (replicas of one system) as one GEMM each:
+// the boxes' columns side by side in the right-hand operand, so the transform matrix (A)
+// is read once for all of them.  A result element depends only on its row of A, its
+// column of B and the k-slices, which are the single-box mode 1's, so every box's output
+// equals what epsr_gensq / epsr_stog mode 1 give it alone, bit for bit.
+
+struct GensqBoxDev {
+    const float *wav, *natomsphere;
+    float *sofq, *colsum;
+    int npairs, ldw, ldq, col0;   // col0: the box's first column in the batch
+};
+
+__global__ void __launch_bounds__(64) gensq_mfma_batch(int nq, int npt1, const float *__restrict__ ycoh,
+                                                       const float *__restrict__ foqols, int ldf,
+                                                       const GensqBoxDev *__restrict__ bx,
+                                                       const int *__restrict__ colbox, int ncol,
+                                                       float *__restrict__ part)
+{
+    const int ncolp = (int)gridDim.y * 16;
+    const int m0 = blockIdx.x * 16, n0 = blockIdx.y * 16, kz = blockIdx.z;
+    auto bfun = [&](int j, int col) -> float {
+        if (col >= ncol) return 0.0f;
+        const GensqBoxDev &b = bx[colbox[col]];
+        const int c = col - b.col0;
+        if (c < b.npairs) return b.wav[(size_t)c * b.ldw + j];
+        const int ic = c - b.npairs;
+        const float nas = b.natomsphere ? b.natomsphere[ic] : 1.0f;
+        return (b.wav[((size_t)b.npairs + ic) * b.ldw + j] - ycoh[j]) * nas;
+    };
+    const int K0 = 2, K1 = npt1 + 1;
+    const int span = (((K1 - K0) + (int)gridDim.z - 1) / (int)gridDim.z + 3) & ~3;
+    const int kb = K0 + kz * span, ke = min(K1, kb + span);
+    floatx4 acc = mfma_tile_16x16(foqols + ldf, ldf, nq, m0, n0, kb, ke, bfun);
+    const int lane = threadIdx.x & 63;
+    const int col = n0 + (lane & 15);
+    for (int v = 0; v < 4; ++v) {
+        int m = m0 + 4 * (lane >> 4) + v;
+        if (m < nq) part[((size_t)kz * ncolp + col) * nq + m] = acc[v];
+    }
+}
+
+// gensq_colsum over every box of a batch: one workgroup per intermolecular column
+__global__ void gensq_colsum_batch(int npt1, const GensqBoxDev *__restrict__ bx, const int *__restrict__ colbox)
+{
+    __shared__ float s[256];
+    const int col = blockIdx.x;
+    const GensqBoxDev &b = bx[colbox[col]];
+    const int c = col - b.col0;
+    if (c < b.npairs) return;   // (uniform over the workgroup)
+    const int ic = c - b.npairs;
+    const float *w2 = b.wav + ((size_t)b.npairs + ic) * b.ldw;
+    float m = 0.0f;
+    for (int j = 2 + threadIdx.x; j <= npt1; j += blockDim.x) m = fmaxf(m, w2[j]);
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) s[threadIdx.x] = fmaxf(s[threadIdx.x], s[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) b.colsum[ic] = s[0];
+}
+
+__global__ void gensq_reduce_batch(int nq, int ncol, int ncolp, int ksplit, const float *__restrict__ part,
+                                   const GensqBoxDev *__restrict__ bx, const int *__restrict__ colbox)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)ncol * nq) return;
+    const int col = (int)(t / nq), m = (int)(t - (long)col * nq);
+    float sum = 0.0f;
+    for (int kz = 0; kz < ksplit; ++kz) sum = sum + part[((size_t)kz * ncolp + col) * nq + m];
+    const GensqBoxDev &b = bx[colbox[col]];
+    const int c = col - b.col0;
+    const bool zero = c >= b.npairs && b.colsum[c - b.npairs] <= 0.0f;
+    b.sofq[(size_t)c * b.ldq + m + 1] = zero ? 0.0f : sum;
+}
+
+// the small per-box tables of a batch in the context's scratch: the descriptors, then the
+// box of every column (uploaded from pinned memory on the context's stream)
+static int batch_tables(epsr_ctx *ctx, const void *desc, size_t desc_bytes, const std::vector<int> &colbox,
+                        char *dev)
+{
+    const size_t cb = sizeof(int) * colbox.size();
+    char *h = (char *)ctx_pinned(ctx, desc_bytes + cb);
+    if (!h) return EPSR_ERR_HIP;
+    std::memcpy(h, desc, desc_bytes);
+    std::memcpy(h + desc_bytes, colbox.data(), cb);
+    HIP_CHECK(ctx, hipMemcpyAsync(dev, h, desc_bytes + cb, hipMemcpyHostToDevice, ctx->stream));
+    return EPSR_OK;
+}
+
+extern "C" int epsr_gensq_batch(epsr_ctx *ctx, int nq, int npt1, const float *ycoh, const float *foqols, int ldf,
+                                int nbox, const epsr_gensq_box_t *boxes)
+{
+    if (!ctx) return EPSR_ERR_ARG;
+    ARG_CHECK(ctx, ycoh && foqols && boxes && nbox >= 1, "epsr_gensq_batch: null argument or no box");
+    ARG_CHECK(ctx, !ctx_capturing(ctx), "epsr_gensq_batch: not for graph capture (it uploads its box tables from host memory)");
+    ARG_CHECK(ctx, nq >= 1 && npt1 >= 2 && ldf > npt1, "epsr_gensq_batch: bad sizes nq=%d npt1=%d ldf=%d", nq, npt1,
+              ldf);
+    std::vector<GensqBoxDev> bd(nbox);
+    std::vector<int> colbox;
+    int ncol = 0;
+    for (int b = 0; b < nbox; ++b) {
+        const epsr_gensq_box_t &x = boxes[b];
+        ARG_CHECK(ctx, x.wav && x.sofq, "epsr_gensq_batch: box %d has a null array", b);
+        ARG_CHECK(ctx, x.npairs >= 1 && x.ldw > npt1 && x.ldq > nq, "epsr_gensq_batch: box %d: npairs=%d ldw=%d ldq=%d",
+                  b, x.npairs, x.ldw, x.ldq);
+        bd[b] = GensqBoxDev{x.wav, x.natomsphere, x.sofq, nullptr, x.npairs, x.ldw, x.ldq, ncol};
+        for (int c = 0; c < 2 * x.npairs; ++c) colbox.push_back(b);
+        ncol += 2 * x.npairs;
+    }
+    HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int ncolp = 16 * ceil_div(ncol, 16);
+    // scratch: column sums of every box (256-byte aligned each), the tables, the partial tiles
+    size_t off = 0;
+    std::vector<size_t> csoff(nbox);
+    for (int b = 0; b < nbox; ++b) {
+        csoff[b] = off;
+        off += (sizeof(float) * bd[b].npairs + 255) & ~(size_t)255;
+    }
+    const size_t tab = off, descb = sizeof(GensqBoxDev) * nbox;
+    off += (descb + sizeof(int) * ncol + 255) & ~(size_t)255;
+    const size_t partoff = off;
+    off += sizeof(float) * (size_t)GENSQ_KSPLIT * ncolp * nq;
+    char *sc = (char *)ctx_scratch(ctx, off);
+    if (!sc) return EPSR_ERR_HIP;
+    for (int b = 0; b < nbox; ++b) bd[b].colsum = (float *)(sc + csoff[b]);
+    int rc = batch_tables(ctx, bd.data(), descb, colbox, sc + tab);
+    if (rc) return rc;
+    const GensqBoxDev *dbx = (const GensqBoxDev *)(sc + tab);
+    const int *dcol = (const int *)(sc + tab + descb);
+    timer_begin(ctx, "gensq");
+    gensq_colsum_batch<<<ncol, 256, 0, st>>>(npt1, dbx, dcol);
+    dim3 grid(ceil_div(nq, 16), ncolp / 16, GENSQ_KSPLIT);
+    gensq_mfma_batch<<<grid, 64, 0, st>>>(nq, npt1, ycoh, foqols, ldf, dbx, dcol, ncol, (float *)(sc + partoff));
+    gensq_reduce_batch<<<ceil_div((long)ncol * nq, 256), 256, 0, st>>>(nq, ncol, ncolp, GENSQ_KSPLIT,
+                                                                      (const float *)(sc + partoff), dbx, dcol);
+    timer_end(ctx, "gensq");
+    HIP_CHECK(ctx, hipGetLastError());
+    HIP_CHECK(ctx, ctx_sync(ctx));
+    return EPSR_OK;
+}
+
+struct StogBoxDev {
+    const float *sofq, *xwt;
+    const int *nqmin, *nqmax;
+    float *gofr;
+    int ldg, ncols, col0;
+};
+
+__global__ void stog_prepare_b_batch(const float *__restrict__ q, int nqpad, const StogBoxDev *__restrict__ bx,
+                                     const int *__restrict__ colbox, int ncol, float *__restrict__ B)
+{
+    const float pi = 3.14159265358979323846f;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nqpad * ncol) return;
+    const int col = t / nqpad, i = t - col * nqpad;
+    const StogBoxDev &b = bx[colbox[col]];
+    const int in = col - b.col0;
+    float v = 0.0f;
+    if (i >= b.nqmin[in] && i <= b.nqmax[in]) {
+        const float qmax = pi / q[b.nqmax[in]];
+        v = b.sofq[(size_t)in * nqpad + i] * dev_lorch(qmax * q[i], pi);
+    }
+    B[(size_t)col * nqpad + i] = v;
+}
+
+__global__ void stog_reduce_batch(float rho, const float *__restrict__ q, int nr, int ncol, int ncolp, int ksplit,
+                                  const float *__restrict__ part, const float *__restrict__ r,
+                                  const StogBoxDev *__restrict__ bx, const int *__restrict__ colbox)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)ncol * nr) return;
+    const int col = (int)(t / nr), j0 = (int)(t - (long)col * nr);
+    float sum = 0.0f;
+    for (int kz = 0; kz < ksplit; ++kz) sum = sum + part[((size_t)kz * ncolp + col) * nr + j0];
+    const float sfac = stog_sfac(q, rho);
+    const int j = j0 + 1;
+    const float rj = r[j];
+    const StogBoxDev &b = bx[colbox[col]];
+    const int in = col - b.col0;
+    b.gofr[(size_t)in * b.ldg + j] = rj > 0.0f ? b.xwt[in] + sfac * sum / rj : b.xwt[in] + sfac * sum;
+}
+
+extern "C" int epsr_stog_batch(epsr_ctx *ctx, float rho, const float *q, int ldq, const float *r, int nr, int nbox,
+                               const epsr_stog_box_t *boxes)
+{
+    if (!ctx) return EPSR_ERR_ARG;
+    ARG_CHECK(ctx, q && r && boxes && nbox >= 1, "epsr_stog_batch: null argument or no box");
+    ARG_CHECK(ctx, !ctx_capturing(ctx), "epsr_stog_batch: not for graph capture (it uploads its box tables from host memory)");
+    ARG_CHECK(ctx, nr >= 1 && ldq >= 2, "epsr_stog_batch: bad sizes nr=%d ldq=%d", nr, ldq);
+    std::vector<StogBoxDev> bd(nbox);
+    std::vector<int> colbox;
+    int ncol = 0;
+    for (int b = 0; b < nbox; ++b) {
+        const epsr_stog_box_t &x = boxes[b];
+        ARG_CHECK(ctx, x.sofq && x.gofr && x.nqmin && x.nqmax && x.xwt, "epsr_stog_batch: box %d has a null array",
+                  b);
+        ARG_CHECK(ctx, x.ncols >= 1 && x.ldg > nr, "epsr_stog_batch: box %d: ncols=%d ldg=%d", b, x.ncols, x.ldg);
+        bd[b] = StogBoxDev{x.sofq, x.xwt, x.nqmin, x.nqmax, x.gofr, x.ldg, x.ncols, ncol};
+        for (int c = 0; c < x.ncols; ++c) colbox.push_back(b);
+        ncol += x.ncols;
+    }
+    HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const int nqpad = ldq;   // the Q range used is 1..ldq-1, as epsr_stog mode 1
+    const int ncolp = 16 * ceil_div(ncol, 16);
+    const size_t descb = sizeof(StogBoxDev) * nbox;
+    const size_t tabb = (descb + sizeof(int) * ncol + 255) & ~(size_t)255;
+    const size_t bb = (sizeof(float) * (size_t)nqpad * ncol + 255) & ~(size_t)255;
+    char *sc = (char *)ctx_scratch(ctx, tabb + bb + sizeof(float) * (size_t)STOG_KSPLIT * ncolp * nr);
+    if (!sc) return EPSR_ERR_HIP;
+    int rc = batch_tables(ctx, bd.data(), descb, colbox, sc);
+    if (rc) return rc;
+    const StogBoxDev *dbx = (const StogBoxDev *)sc;
+    const int *dcol = (const int *)(sc + descb);
+    float *B = (float *)(sc + tabb), *part = (float *)(sc + tabb + bb);
+    timer_begin(ctx, "stog");
+    stog_prepare_b_batch<<<ceil_div((long)nqpad * ncol, 256), 256, 0, st>>>(q, nqpad, dbx, dcol, ncol, B);
+    dim3 grid(ceil_div(nr, 16), ncolp / 16, STOG_KSPLIT);
+    stog_mfma<<<grid, 64, 0, st>>>(q, nqpad, B, r, nr, ncol, part);
+    stog_reduce_batch<<<ceil_div((long)ncol * nr, 256), 256, 0, st>>>(rho, q, nr, ncol, ncolp, STOG_KSPLIT, part, r,
+                                                                      dbx, dcol);
     timer_end(ctx, "stog");
     HIP_CHECK(ctx, hipGetLastError());
     HIP_CHECK(ctx, ctx_sync(ctx));

@@ -28,6 +28,10 @@ def _ptr(t):
     return C.c_void_p(t.data_ptr()) if t is not None else None
 
 
+def _addr(t):
+    return t.data_ptr() if t is not None else None
+
+
 class Context:
     """One epsr_ctx on one GPU, running on torch's current stream of that device."""
 
@@ -228,6 +232,31 @@ def stog(ctx: Context, rho, q, sofq, ldq, r, gofr, ldg, nqmin, nqmax, nr, ncols,
     ctx.check(ctx.lib.epsr_stog(ctx.h, F32(rho), _ptr(q), _ptr(sofq), int(ldq), _ptr(r), _ptr(gofr), int(ldg),
                                 _ptr(nqmin), _ptr(nqmax), int(nr), int(ncols), _ptr(xwt), int(mode)))
     return gofr
+
+
+def gensq_batch(ctx: Context, nq, npt1, ycoh, foqols, ldf, boxes):
+    """gensq mode 1 of many boxes on one grid as one GEMM (epsr_gensq_batch): boxes is a
+    list of (npairs, wav, ldw, sofq, ldq[, natomsphere]); each box's sofq equals its
+    gensq(..., mode=1) bit for bit."""
+    arr = (_native.GensqBox * len(boxes))()
+    for a, b in zip(arr, boxes):
+        npairs, wav, ldw, sofq, ldq = b[:5]
+        nas = b[5] if len(b) > 5 else None
+        a.npairs, a.ldw, a.ldq = int(npairs), int(ldw), int(ldq)
+        a.wav, a.natomsphere, a.sofq = _addr(wav), _addr(nas), _addr(sofq)
+    ctx.check(ctx.lib.epsr_gensq_batch(ctx.h, int(nq), int(npt1), _ptr(ycoh), _ptr(foqols), int(ldf), len(boxes),
+                                       arr))
+
+
+def stog_batch(ctx: Context, rho, q, ldq, r, nr, boxes):
+    """stog mode 1 of many boxes on one grid as one GEMM (epsr_stog_batch): boxes is a list
+    of (sofq, gofr, ldg, nqmin, nqmax, ncols, xwt); each box's gofr equals its
+    stog(..., mode=1) bit for bit."""
+    arr = (_native.StogBox * len(boxes))()
+    for a, (sofq, gofr, ldg, nqmin, nqmax, ncols, xwt) in zip(arr, boxes):
+        a.sofq, a.gofr, a.nqmin, a.nqmax, a.xwt = (_addr(t) for t in (sofq, gofr, nqmin, nqmax, xwt))
+        a.ncols, a.ldg = int(ncols), int(ldg)
+    ctx.check(ctx.lib.epsr_stog_batch(ctx.h, F32(rho), _ptr(q), int(ldq), _ptr(r), int(nr), len(boxes), arr))
 
 
 def foqols(ctx: Context, grid, rho, rmaxgr, fwhm=0.0, fwhmq=0.0, lorchqwidth=0.0, lorchqwidthq=0.0):

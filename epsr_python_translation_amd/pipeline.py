@@ -555,6 +555,15 @@ class Refinement:
         """g(r) of the intermolecular partials on the r grid 1..npt1 (stog, Q points 2..nq, no
         weights), into self.gofr [npairs][npt1 + 1].  ctx: another context on the same
         device (a graph's), else the box's own."""
+        s = self.s
+        ctx = ctx or self.ctx
+        self.stog_setup(ctx)
+        D.stog(ctx, s.rho, self.q, self.sofq[s.npairs * self.ldq:], self.ldq, self.r, self.gofr, self.ldg,
+               self.nqmin, self.nqmax, self.nr, s.npairs, self.xwt, mode=mode)
+        return self.gofr
+
+    def stog_setup(self, ctx=None):
+        """the r grid, g(r) array and Q limits of stog (once)"""
         s, g = self.s, self.g
         ctx = ctx or self.ctx
         if getattr(self, "gofr", None) is None:
@@ -565,9 +574,22 @@ class Refinement:
             self.nqmin = ctx.tensor(np.full(s.npairs, 2, np.int32))
             self.nqmax = ctx.tensor(np.full(s.npairs, g.nq, np.int32))
             self.xwt = ctx.tensor(np.zeros(s.npairs, F32))
-        D.stog(ctx, s.rho, self.q, self.sofq[s.npairs * self.ldq:], self.ldq, self.r, self.gofr, self.ldg,
-               self.nqmin, self.nqmax, self.nr, s.npairs, self.xwt, mode=mode)
-        return self.gofr
+
+
+def transforms_batch(ctx, boxes):
+    """gensq and stog (mode 1) of refinements on one grid (replicas of one system) as one
+    GEMM each (epsr_gensq_batch, epsr_stog_batch): every box's S(Q) and g(r) equal its own
+    gensq(mode=1) and stog(mode=1) bit for bit.  The boxes' inputs must be complete (their
+    streams synchronised) before the call; the outputs are complete when it returns."""
+    R0 = boxes[0]
+    g = R0.g
+    for R in boxes:
+        assert (R.g.nq, R.g.npt1, R.ldf, R.ldq) == (g.nq, g.npt1, R0.ldf, R0.ldq), "boxes on different grids"
+        R.stog_setup(R.ctx)
+    D.gensq_batch(ctx, g.nq, g.npt1, R0.ycoh, R0.foqols, R0.ldf,
+                  [(R.s.npairs, R.wav, R.ldw, R.sofq, R.ldq) for R in boxes])
+    D.stog_batch(ctx, R0.s.rho, R0.q, R0.ldq, R0.r, R0.nr,
+                 [(R.sofq[R.s.npairs * R.ldq:], R.gofr, R.ldg, R.nqmin, R.nqmax, R.s.npairs, R.xwt) for R in boxes])
 
 
 class EPParams(C.Structure):

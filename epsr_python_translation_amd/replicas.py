@@ -121,32 +121,48 @@ class ReplicaRefinement(Ensemble):
             ctx = R.ctx
             self.weights.append(PL.get_wts(ctx, R.s, wts, self.feedback, np.ones(len(wts), F32), R.ran1))
 
-    def iterate(self, sweeps=1, fit_mode=_native.FIT_NORMAL, da=False):
+    def iterate(self, sweeps=1, fit_mode=_native.FIT_NORMAL, da=False, batch_transforms=True):
         """One iteration of every replica; returns per replica (the EP fit's r-factors per
         dataset, chi^2, the fit to the data per dataset).  da: MC cycles with delayed
-        acceptance (train_surrogate first), else plain cycles back to back."""
+        acceptance (train_surrogate first), else plain cycles back to back.
+        batch_transforms: gensq and stog of all replicas as one GEMM each between the MC
+        and the fit phases (the replicas share their grids; results are the same bits as
+        each replica's own transforms)."""
         assert self.weights is not None, "set_targets first"
         assert not da or self.da is not None, "train_surrogate first"
         self.nsum += 1
         out = [None] * len(self.boxes)
 
-        def work(wk):
-            for R in wk.boxes:
-                i = self.boxes.index(R)
-                self._ledger(R)
-                tables = R.tables
-                if da:
-                    for _ in range(sweeps):
-                        self.da[i].sweep(R.ctx, R.box, tables, R.ledger, R.mc, R.ran1)
-                else:
-                    R.molfit(tables, ncycles=sweeps)
-                R.atogen()
-                R.gensq()
-                R.stog()
-                R.accumulate(self.nsum)
-                w = self.weights[i]
-                rfacd, _ = R.rfactor_data(self.data, w)
-                rf, chi = R.ptgnsq(self.data, w["cwt"], w["cwtpot"], fit_mode=fit_mode)
-                out[i] = (rf, chi, rfacd)
-        self._parallel(work)
+        def sample(R):
+            i = self.boxes.index(R)
+            self._ledger(R)
+            tables = R.tables
+            if da:
+                for _ in range(sweeps):
+                    self.da[i].sweep(R.ctx, R.box, tables, R.ledger, R.mc, R.ran1)
+            else:
+                R.molfit(tables, ncycles=sweeps)
+            R.atogen()
+
+        def refine(R):
+            i = self.boxes.index(R)
+            R.accumulate(self.nsum)
+            w = self.weights[i]
+            rfacd, _ = R.rfactor_data(self.data, w)
+            rf, chi = R.ptgnsq(self.data, w["cwt"], w["cwtpot"], fit_mode=fit_mode)
+            out[i] = (rf, chi, rfacd)
+
+        if batch_transforms:
+            self._parallel(lambda wk: [sample(R) for R in wk.boxes])
+            wk0 = self.workers[0]
+            wk0.run(lambda: PL.transforms_batch(wk0.ctx, self.boxes))
+            self._parallel(lambda wk: [refine(R) for R in wk.boxes])
+        else:
+            def work(wk):
+                for R in wk.boxes:
+                    sample(R)
+                    R.gensq()
+                    R.stog()
+                    refine(R)
+            self._parallel(work)
         return out

@@ -154,6 +154,19 @@ int epsr_gensq(epsr_ctx *ctx, int nq, int npairs, int npt1, int ldw, const float
 int epsr_gensq_noalloc(epsr_ctx *ctx, int nq, int npairs, int npt1, int ldw, const float *wav, const float *ycoh,
                        const float *foqols, int ldf, const float *natomsphere, float *sofq, int ldq);
 
+/* ---- gensq of many boxes on one grid (replicas of one system).  No counterpart in the
+ * reference: epsr_gensq mode 1 of every box as one GEMM, the boxes' columns side by side
+ * (the transform matrix foqols is read once); each box's sofq equals its epsr_gensq mode 1
+ * result bit for bit.  nq, npt1, ycoh, foqols, ldf are shared; boxes is a host array. */
+typedef struct {
+    int npairs, ldw, ldq;
+    const float *wav;          /* dev [2*npairs*ldw] */
+    const float *natomsphere;  /* dev [npairs] or NULL */
+    float *sofq;               /* dev [2*npairs*ldq] */
+} epsr_gensq_box_t;
+int epsr_gensq_batch(epsr_ctx *ctx, int nq, int npt1, const float *ycoh, const float *foqols, int ldf, int nbox,
+                     const epsr_gensq_box_t *boxes);
+
 /* ---- stog: replaces ato_manipulation_rev4.stog(rho, q, sofq, r, gofr, nqmin, nqmax,
  * nr, npairs, nqdim, nrdim, npdim, xwt) (ato_manipulation_rev4.py:4066-4111; Fortran
  * :4154) including lorch (:4114-4124).
@@ -166,6 +179,20 @@ int epsr_gensq_noalloc(epsr_ctx *ctx, int nq, int npairs, int npt1, int ldw, con
 int epsr_stog(epsr_ctx *ctx, float rho, const float *q, const float *sofq, int ldq,
               const float *r, float *gofr, int ldg, const int *nqmin, const int *nqmax,
               int nr, int ncols, const float *xwt, int mode);
+
+/* ---- stog of many boxes on one grid (replicas of one system).  No counterpart in the
+ * reference: epsr_stog mode 1 of every box as one GEMM (q sin(qr) generated once for all
+ * columns); each box's gofr equals its epsr_stog mode 1 result bit for bit.  rho, q, ldq,
+ * r and nr are shared; boxes is a host array. */
+typedef struct {
+    const float *sofq;            /* dev [ncols*ldq] */
+    float *gofr;                  /* dev [ncols*ldg] */
+    const int *nqmin, *nqmax;     /* dev [ncols] */
+    const float *xwt;             /* dev [ncols] */
+    int ncols, ldg;
+} epsr_stog_box_t;
+int epsr_stog_batch(epsr_ctx *ctx, float rho, const float *q, int ldq, const float *r, int nr, int nbox,
+                    const epsr_stog_box_t *boxes);
 
 /* ---- potential tables, dev arrays [npairs*ld] (table[ic*ld+iz] = reference
  * table(iz, ic+1)) and xstar dev [ld]: potential_par.xavs/xavs1 (empirical potential
