@@ -94,6 +94,9 @@ struct epsr_ledger {
     // ledger (device)
     int nself = 0, npairent = 0, nslots = 0, nentries = 0;
     int *d_rowstart = nullptr, *d_cnt = nullptr, *d_order = nullptr, *d_pos = nullptr;
+    // the lists lie in chain order: d_rowk[k] is the first slot of chain position k's list
+    // (d_rowk[nmol] = the slot count), so the list of position k is [rowk[k], rowk[k + 1])
+    int *d_rowk = nullptr;
     int *d_partner = nullptr, *d_entry = nullptr, *d_mirror = nullptr, *d_slotmol = nullptr;
     // per slot: partner's chain position, mirror unless the partner's first slot (else -1),
     // and the packed record of the MC pair kernel

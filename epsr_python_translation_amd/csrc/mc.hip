@@ -49,7 +49,8 @@ extern "C" int epsr_ledger_create(epsr_ctx *ctx, const epsr_box *box, epsr_ledge
     l->nmol = box->nmol;
     int rc;
     if ((rc = dev_alloc(ctx, &l->d_order, box->nmol)) || (rc = dev_alloc(ctx, &l->d_pos, box->nmol)) ||
-        (rc = dev_alloc(ctx, &l->d_rowstart, box->nmol + 1)) || (rc = dev_alloc(ctx, &l->d_cnt, box->nmol))) {
+        (rc = dev_alloc(ctx, &l->d_rowstart, box->nmol + 1)) || (rc = dev_alloc(ctx, &l->d_cnt, box->nmol)) ||
+        (rc = dev_alloc(ctx, &l->d_rowk, box->nmol + 1))) {
         epsr_ledger_destroy(l);
         return rc;
     }
@@ -60,7 +61,7 @@ extern "C" int epsr_ledger_create(epsr_ctx *ctx, const epsr_box *box, epsr_ledge
 extern "C" void epsr_ledger_destroy(epsr_ledger *l)
 {
     if (!l) return;
-    hipFree(l->d_rowstart); hipFree(l->d_cnt); hipFree(l->d_order); hipFree(l->d_pos);
+    hipFree(l->d_rowstart); hipFree(l->d_cnt); hipFree(l->d_order); hipFree(l->d_pos); hipFree(l->d_rowk);
     hipFree(l->d_partner); hipFree(l->d_entry); hipFree(l->d_mirror); hipFree(l->d_slotmol);
     hipFree(l->d_ergpair);
     hipFree(l->d_roff); hipFree(l->d_toff); hipFree(l->d_moff);
@@ -1152,8 +1153,8 @@ extern "C" int epsr_fmolec1(epsr_ctx *ctx, const epsr_box *box, const epsr_coord
     }
     int *key = nullptr;
     const size_t nr1 = (size_t)nrec + 1;
-    if ((rc = ledger_work(ctx, l, LW_REC, sizeof(int) * (5 * nr1 + nmol + 1), (void **)&key))) return rc;
-    int *val = key + nr1, *skey = val + nr1, *sval = skey + nr1, *spos = sval + nr1, *rowk = spos + nr1;
+    if ((rc = ledger_work(ctx, l, LW_REC, sizeof(int) * 5 * nr1, (void **)&key))) return rc;
+    int *val = key + nr1, *skey = val + nr1, *sval = skey + nr1, *spos = sval + nr1, *rowk = l->d_rowk;
     {
         int g = persistent_blocks(0, (long long)nmol * 64, 256);
         fmolec1_entries<<<g, 256, 0, st>>>(nmol, l->d_order, l->d_pos, drow, rowE, rowJ, ent, eintra, key, val,
@@ -1389,12 +1390,15 @@ __global__ void mc_trial(int nmol, const int *__restrict__ order, const int *__r
                          float *__restrict__ txa,
                          float *__restrict__ tya, float *__restrict__ tza, float *__restrict__ txadd,
                          float *__restrict__ tyadd, float *__restrict__ tzadd, int *__restrict__ nochange,
-                         TabCheck tc, int *__restrict__ zero, int nzero)
+                         TabCheck tc, int *__restrict__ zero, int nzero, int *__restrict__ zero2, int nzero2)
 {
     // the cycle's set-up rides along: verify the packed tables, clear the counters in zero
+    // and zero2
     tables_verify(tc, nochange);
-    if (blockIdx.x == 0)
+    if (blockIdx.x == 0) {
         for (int i = threadIdx.x; i < nzero; i += blockDim.x) zero[i] = 0;
+        for (int i = threadIdx.x; i < nzero2; i += blockDim.x) zero2[i] = 0;
+    }
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nmol) return;
     const float pi = 3.14159265358979323846f;
@@ -1683,11 +1687,11 @@ __device__ void mc_thresholds(double b, double bnd, double tempd, double nlr, fl
 // chain position)
 #define PREP_CH 4
 __global__ void __launch_bounds__(256)
-mc_prepare(int nmol, const int *__restrict__ order, const int *__restrict__ rowstart, const int *__restrict__ cnt,
-           const int *__restrict__ pnv, const int *__restrict__ mw, const float *__restrict__ A,
-           const float *__restrict__ Bv, const float *__restrict__ L, const float *__restrict__ stream,
-           const int *__restrict__ moff, float temp, const float *__restrict__ shift, float *__restrict__ delta,
-           float2 *__restrict__ thr, unsigned char *__restrict__ state, int *__restrict__ counters)
+mc_prepare(int nmol, const int *__restrict__ rowk, const int *__restrict__ pnv, const int *__restrict__ mw,
+           const float *__restrict__ A, const float *__restrict__ Bv, const float *__restrict__ L,
+           const float *__restrict__ stream, const int *__restrict__ moff, float temp,
+           const float *__restrict__ shift, float *__restrict__ delta, float2 *__restrict__ thr,
+           unsigned char *__restrict__ state, int *__restrict__ wout, int *__restrict__ nout)
 {
     // Each rounding of the sequential sum errs by at most u |partial sum|.  With
     // P_s = |sum_{s'<=s} e0| + sum_{s'<=s, earlier partner} |delta| the partial sums obey
@@ -1696,21 +1700,25 @@ mc_prepare(int nmol, const int *__restrict__ order, const int *__restrict__ rows
     // molecule, taken in chain order (each XCD a contiguous part of the chain, whose lists
     // and their partners' lists lie together); the slots are scanned in order, 64 at a
     // time, with the loads of PREP_CH chunks issued together.
+    //
+    // The first pass of the resolution rides along: with every earlier partner still
+    // undecided, a move whose pushed sum lies on one side of its thresholds for every
+    // outcome (the sum of its negative deltas at or above fr: rejected; of its positive ones
+    // at or below fa: accepted) is decided here; the others go to the work list wout.
     const int pm = xcd_block() * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    // the resolver's counters and every move's state start at zero
-    if (blockIdx.x == 0 && threadIdx.x < 8) counters[threadIdx.x] = 0;
     if (pm >= nmol) return;
-    const int m = order[pm];
-    const int row = rowstart[m], c = cnt[m];
-    if (lane == 0) state[pm] = 0;
+    const int row = rowk[pm], c = rowk[pm + 1] - row;
     const float sh = shift ? shift[pm] : 0.0f;
     if (isnan(sh)) {
         // rejected by the surrogate stage: never accepted, receives no pushes
-        if (lane == 0) thr[pm] = make_float2(-INFINITY, -INFINITY);
+        if (lane == 0) {
+            thr[pm] = make_float2(-INFINITY, -INFINITY);
+            state[pm] = 2;
+        }
         return;
     }
-    double carry_e = 0.0, carry_d = 0.0, pprev = 0.0, w = 0.0;
+    double carry_e = 0.0, carry_d = 0.0, pprev = 0.0, w = 0.0, dneg = 0.0, dpos = 0.0;
     int npb = 0;
     for (int g0 = 1; g0 < c; g0 += 64 * PREP_CH) {
         double e0[PREP_CH], ad[PREP_CH];
@@ -1735,6 +1743,8 @@ mc_prepare(int nmol, const int *__restrict__ order, const int *__restrict__ rows
                     dl = (float)(((double)bv - (double)lw) - e0[j]);
                     maxnew[j] = fmaxf(maxnew[j], fabsf(bv));
                     ad[j] = fabs((double)dl);
+                    if (dl < 0.0f) dneg += (double)dl;
+                    else dpos += (double)dl;
                     npb++;
                 }
                 delta[i] = dl;
@@ -1765,6 +1775,8 @@ mc_prepare(int nmol, const int *__restrict__ order, const int *__restrict__ rows
     for (int off = 32; off > 0; off >>= 1) {
         w += __shfl_down(w, off);
         npb += __shfl_down(npb, off);
+        dneg += __shfl_down(dneg, off);
+        dpos += __shfl_down(dpos, off);
     }
     if (lane == 0) {
         const double u = 5.9604644775390625e-08;   // 2^-24
@@ -1773,14 +1785,16 @@ mc_prepare(int nmol, const int *__restrict__ order, const int *__restrict__ rows
         float fa, fr;
         mc_thresholds(carry_e - (double)sh, bnd, (double)temp, nlr, fa, fr);
         thr[pm] = make_float2(fa, fr);
+        const int d = dpos <= (double)fa ? 1 : (dneg >= (double)fr ? 2 : 0);
+        state[pm] = (unsigned char)d;
+        if (!d) wout[atomicAdd(nout, 1)] = pm;
     }
 }
 
 // exact fdiff of every accepted move (for fadd): one wave per chain position gathers
 // the selected variant of every slot into LDS, lane 0 sums them in the reference's order
 __global__ void __launch_bounds__(256)
-mc_fdiff_accepted(int nmol, const int *__restrict__ order, const int *__restrict__ rowstart,
-                  const int *__restrict__ cnt, const int *__restrict__ pnv, const int *__restrict__ mw,
+mc_fdiff_accepted(int nmol, const int *__restrict__ rowk, const int *__restrict__ pnv, const int *__restrict__ mw,
                   const float *__restrict__ A, const float *__restrict__ Bv,
                   const float *__restrict__ L, const unsigned *__restrict__ bits, int stride,
                   float *__restrict__ fdiff_out, unsigned *__restrict__ bits_out, int nwords)
@@ -1793,8 +1807,7 @@ mc_fdiff_accepted(int nmol, const int *__restrict__ order, const int *__restrict
     if (bits_out && g < nwords) bits_out[g] = bits[g];
     if (k >= nmol) return;
     if (!((bits[k >> 5] >> (k & 31)) & 1u)) return;
-    const int m = order[k];
-    const int row = rowstart[m], c = cnt[m];
+    const int row = rowk[k], c = rowk[k + 1] - row;
     float2 *t = fterm + (size_t)wv * stride;
     for (int s = 1 + lane; s < c; s += 64) {
         int i = row + s;
@@ -1832,8 +1845,8 @@ mc_fdiff_accepted(int nmol, const int *__restrict__ order, const int *__restrict
 // one wave decides move k if it can; returns the decision (0: still undecided)
 // pnv[i]: chain position of slot i's partner; mw[i]: the mirror slot, -1 when it is the
 // partner's first slot (a slot that no move rewrites)
-__device__ int mc_pass_move(int k, const int *__restrict__ order, const int *__restrict__ rowstart,
-                            const int *__restrict__ cnt, const int *__restrict__ pnv, const int *__restrict__ mw,
+__device__ int mc_pass_move(int k, const int *__restrict__ rowk, const int *__restrict__ pnv,
+                            const int *__restrict__ mw,
                             const float *__restrict__ A, const float *__restrict__ Bv, const float *__restrict__ L,
                             const float *__restrict__ delta, const float2 *__restrict__ thr,
                             const float *__restrict__ stream, const int *__restrict__ moff, float temp,
@@ -1841,8 +1854,7 @@ __device__ int mc_pass_move(int k, const int *__restrict__ order, const int *__r
                             int *nexact)
 {
     const int lane = threadIdx.x & 63;
-    const int m = order[k];
-    const int row = rowstart[m], c = cnt[m];
+    const int row = rowk[k], c = rowk[k + 1] - row;
     double sure = 0.0, lo = 0.0, hi = 0.0;
     int und = 0;
     for (int s = 1 + lane; s < c; s += 64) {
@@ -1896,8 +1908,7 @@ __device__ int mc_pass_move(int k, const int *__restrict__ order, const int *__r
 // one pass over a work list (the whole chain when first): decided moves write state,
 // the others go to the next list
 __global__ void __launch_bounds__(256)
-mc_resolve_pass(int nmol, const int *__restrict__ order, const int *__restrict__ rowstart,
-                const int *__restrict__ cnt, const int *__restrict__ pnv, const int *__restrict__ mw,
+mc_resolve_pass(int nmol, const int *__restrict__ rowk, const int *__restrict__ pnv, const int *__restrict__ mw,
                 const float *__restrict__ A, const float *__restrict__ Bv,
                 const float *__restrict__ L, const float *__restrict__ delta, const float2 *__restrict__ thr,
                 const float *__restrict__ stream, const int *__restrict__ moff, float temp,
@@ -1911,8 +1922,8 @@ mc_resolve_pass(int nmol, const int *__restrict__ order, const int *__restrict__
     const int n = first ? nmol : *nin;
     for (int w = xcd_block() * 4 + wv; w < n; w += gridDim.x * 4) {
         const int k = first ? w : win[w];
-        const int d = mc_pass_move(k, order, rowstart, cnt, pnv, mw, A, Bv, L, delta, thr, stream, moff, temp,
-                                   shift, state, term, nexact);
+        const int d = mc_pass_move(k, rowk, pnv, mw, A, Bv, L, delta, thr, stream, moff, temp, shift, state, term,
+                                   nexact);
         if (lane == 0) {
             if (d) state[k] = (unsigned char)d;
             else wout[atomicAdd(nout, 1)] = k;
@@ -1925,8 +1936,7 @@ mc_resolve_pass(int nmol, const int *__restrict__ order, const int *__restrict__
 // undecided; lists ping-pong between wa and wb.  Then the acceptance bits by chain
 // position from the decisions.
 __global__ void __launch_bounds__(1024)
-mc_resolve_finish(const int *__restrict__ order, const int *__restrict__ rowstart, const int *__restrict__ cnt,
-                  const int *__restrict__ pnv, const int *__restrict__ mw,
+mc_resolve_finish(const int *__restrict__ rowk, const int *__restrict__ pnv, const int *__restrict__ mw,
                   const float *__restrict__ A, const float *__restrict__ Bv, const float *__restrict__ L,
                   const float *__restrict__ delta, const float2 *__restrict__ thr, const float *__restrict__ stream,
                   const int *__restrict__ moff, float temp, const float *__restrict__ shift, int maxc,
@@ -1951,8 +1961,8 @@ mc_resolve_finish(const int *__restrict__ order, const int *__restrict__ rowstar
     while (ncur > 0 && passes < maxpass) {
         for (int w = wv; w < ncur; w += nwv) {
             const int k = cur[w];
-            const int d = mc_pass_move(k, order, rowstart, cnt, pnv, mw, A, Bv, L, delta, thr, stream, moff, temp,
-                                       shift, state, term, nexact);
+            const int d = mc_pass_move(k, rowk, pnv, mw, A, Bv, L, delta, thr, stream, moff, temp, shift, state,
+                                       term, nexact);
             if (lane == 0) {
                 if (d) state[k] = (unsigned char)d;
                 else nxt[atomicAdd(&nnext, 1)] = k;
@@ -2523,7 +2533,7 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
     mc_trial<<<ceil_div(nmol, 128), 128, 0, st>>>(nmol, l->d_order, box->d_naf, box->d_nal, c->xa, c->ya, c->za, c->xadd,
                                                   c->yadd, c->zadd, dstream, l->d_roff, l->d_toff, nullptr, nullptr,
                                                   box->d_mtype, steps_by_value ? nullptr : dwtype, sa, txa, tya, tza,
-                                                  txadd, tyadd, tzadd, dnochange, tc, drej, 2 * nmt);
+                                                  txadd, tyadd, tzadd, dnochange, tc, drej, 2 * nmt, dpc, 8);
 
     int nexact = nmol, npasses = 0;
     // B..D of the cycle from trial positions in txa..tzadd and the stream at stream; the
@@ -2561,32 +2571,28 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
             mc_record_target<<<nmol, 64, 0, st>>>(nmol, l->d_order, l->d_pos, l->d_rowstart, l->d_cnt,
                                                   l->d_partner, dA, dL, dshift, rec->target);
         if (fast) {
-            mc_prepare<<<xcd_grid(ceil_div(nmol, 4)), 256, 0, st>>>(nmol, l->d_order, l->d_rowstart, l->d_cnt,
-                                                                    l->d_pn, l->d_mw,
-                                                          dA, dB, dL, stream, moff, temp, dshift, ddelta, dthr, dstate,
-                                                          dpc);
+            // the deltas, the thresholds and the first decisions (every earlier partner
+            // undecided); the moves left go to dW0 (count dpc[0])
+            mc_prepare<<<xcd_grid(ceil_div(nmol, 4)), 256, 0, st>>>(nmol, l->d_rowk, l->d_pn, l->d_mw, dA, dB, dL,
+                                                                    stream, moff, temp, dshift, ddelta, dthr, dstate,
+                                                                    dW0, dpc + 0);
             const int nwf = std::max(1, std::min(16, (int)((150 * 1024) / (sizeof(float2) * (size_t)maxc))));
             const int gpass = xcd_grid(std::min(ceil_div(nmol, 4), 2048));
             timer_begin(ctx, "mc_resolve");
-            mc_resolve_pass<<<gpass, 256, lds_pass, st>>>(nmol, l->d_order, l->d_rowstart, l->d_cnt, l->d_pn,
-                                                          l->d_mw, dA, dB, dL, ddelta, dthr,
-                                                          stream, moff, temp, dshift, maxc, 1, nullptr, nullptr, dW0,
-                                                          dpc + 0, dstate, dpc + 3);
-            // (a capped finisher, the test hook, follows the first grid pass alone)
+            // grid-wide passes over the lists (dW0 -> dW1 -> dW0), then the finisher; a capped
+            // finisher (the test hook) follows the first decisions alone
             const bool capped = box->resolve_passes >= 0;
             if (!capped) {
-                mc_resolve_pass<<<gpass, 256, lds_pass, st>>>(nmol, l->d_order, l->d_rowstart, l->d_cnt, l->d_pn,
-                                                              l->d_mw, dA, dB, dL, ddelta, dthr,
-                                                              stream, moff, temp, dshift, maxc, 0, dW0, dpc + 0, dW1,
-                                                              dpc + 1, dstate, dpc + 3);
-                mc_resolve_pass<<<gpass, 256, lds_pass, st>>>(nmol, l->d_order, l->d_rowstart, l->d_cnt, l->d_pn,
-                                                              l->d_mw, dA, dB, dL, ddelta, dthr,
-                                                              stream, moff, temp, dshift, maxc, 0, dW1, dpc + 1, dW0,
-                                                              dpc + 2, dstate, dpc + 3);
+                mc_resolve_pass<<<gpass, 256, lds_pass, st>>>(nmol, l->d_rowk, l->d_pn, l->d_mw, dA, dB, dL, ddelta,
+                                                              dthr, stream, moff, temp, dshift, maxc, 0, dW0, dpc + 0,
+                                                              dW1, dpc + 1, dstate, dpc + 3);
+                mc_resolve_pass<<<gpass, 256, lds_pass, st>>>(nmol, l->d_rowk, l->d_pn, l->d_mw, dA, dB, dL, ddelta,
+                                                              dthr, stream, moff, temp, dshift, maxc, 0, dW1, dpc + 1,
+                                                              dW0, dpc + 2, dstate, dpc + 3);
             }
             mc_resolve_finish<<<1, 64 * nwf, sizeof(float2) * nwf * (size_t)maxc, st>>>(
-                l->d_order, l->d_rowstart, l->d_cnt, l->d_pn, l->d_mw, dA, dB, dL, ddelta, dthr, stream, moff, temp,
-                dshift, maxc, dW0, dpc + (capped ? 0 : 2), dW1, dstate, dpc + 3, dpc + 4, nmol, dacc,
+                l->d_rowk, l->d_pn, l->d_mw, dA, dB, dL, ddelta, dthr, stream, moff, temp, dshift, maxc, dW0,
+                dpc + (capped ? 0 : 2), dW1, dstate, dpc + 3, dpc + 4, nmol, dacc,
                 capped ? box->resolve_passes : nmol + 1, dpc + 5);
             timer_end(ctx, "mc_resolve");
             // fadd only reports: the exact fdiff of the accepted moves is formed on the side
@@ -2603,7 +2609,7 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
                 bo = (unsigned *)(dbres + 5 * nmt + (size_t)ncycles * (hw + nmol)) + (size_t)slot * nwords;
             }
             mc_fdiff_accepted<<<xcd_grid(ceil_div(nmol, 4)), 256, sizeof(float2) * 4 * (size_t)maxc, side>>>(
-                nmol, l->d_order, l->d_rowstart, l->d_cnt, l->d_pn, l->d_mw, dA, dB, dL, dacc, maxc, fdo, bo, nwords);
+                nmol, l->d_rowk, l->d_pn, l->d_mw, dA, dB, dL, dacc, maxc, fdo, bo, nwords);
             if (slot < 0) {
                 HIP_CHECK(ctx, hipMemcpyAsync(hfd, dfdiff, sizeof(float) * nmol, hipMemcpyDeviceToHost, side));
                 HIP_CHECK(ctx,
@@ -2646,7 +2652,7 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
             mc_trial<<<ceil_div(nmol, 128), 128, 0, st>>>(nmol, l->d_order, box->d_naf, box->d_nal, c->xa, c->ya,
                                                           c->za, c->xadd, c->yadd, c->zadd, sk, l->d_roff, l->d_toff,
                                                           nullptr, nullptr, box->d_mtype, dwtype, sa, txa, tya, tza,
-                                                          txadd, tyadd, tzadd, dnochange, tc, drej, 2 * nmt);
+                                                          txadd, tyadd, tzadd, dnochange, tc, drej, 2 * nmt, dpc, 8);
             if ((rc = run(sk, l->d_moff, l->d_roff, l->d_toff, dnochange, k))) return rc;
             adapt(k);
         }
@@ -2703,7 +2709,7 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
                 stats->accepted = nacc;
                 stats->exact_resolutions = b[3];
                 stats->pushed_entries = 0;
-                stats->resolver_passes = (box->resolve_passes >= 0 ? 1 : 3) + b[4];
+                stats->resolver_passes = (box->resolve_passes >= 0 ? 1 : 3) + b[4];   // (the first in mc_prepare)
                 stats->resolver_bytes = (double)nslots * 13.0 + (double)nmol * (sizeof(float2) + 1);
                 stats->retries = 0;
                 stats->ms_pairs = ctx->timing ? epsr_ctx_last_kernel_ms(ctx, "mc_pair") : -1.0;
@@ -2862,7 +2868,7 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
         mc_trial<<<ceil_div(nmol, 128), 128, 0, st>>>(nmol, l->d_order, box->d_naf, box->d_nal, c->xa, c->ya, c->za,
                                                       c->xadd, c->yadd, c->zadd, dstream2, droff, dtoff, drstep,
                                                       dtstep, box->d_mtype, steps_by_value ? nullptr : dwtype, sa, txa,
-                                                      tya, tza, txadd, tyadd, tzadd, dtmp + 2, none, drej, 2 * nmt);
+                                                      tya, tza, txadd, tyadd, tzadd, dtmp + 2, none, drej, 2 * nmt, dpc, 8);
         if ((rc = run(dstream2, dmoff, droff, dtoff, dtmp + 3, -1))) return rc;
         HIP_CHECK(ctx, hipStreamSynchronize(st));
     }
