@@ -1448,15 +1448,22 @@ __global__ void mc_trial(int nmol, const int *__restrict__ order, const int *__r
 // A = E(m_trial, n_current), Bv = E(m_trial, n_trial) when n moves before m, and the
 // ledger value at the start of the cycle.
 // (8 waves per SIMD: two 1024-lane workgroups per CU when their tables fit LDS twice)
-template <bool LDS>
-__global__ void __attribute__((amdgpu_flat_work_group_size(1, 1024), amdgpu_waves_per_eu(8))) mc_pairs(int nslots, int nbv, const int *__restrict__ bvslot, const int4 *__restrict__ srec,
-                         const int *__restrict__ entry, const int *__restrict__ pos, const float *__restrict__ ergpair,
-                         const float4 *__restrict__ Pold, const float4 *__restrict__ Pnew, PairTab gt, int npairs,
-                         const float *__restrict__ shift, float *__restrict__ A, float *__restrict__ Bv,
-                         float *__restrict__ L)
+// COOP (boxes of molecules of at most three atoms): the wave loads its 64 partners' atoms together,
+// consecutive lanes fetching consecutive atoms (about a third of the cache lines of three
+// separate loads per lane), through a wave-private LDS area of 192 float4 at coop_off;
+// the arithmetic is unchanged
+template <bool LDS, bool COOP>
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, 1024), amdgpu_waves_per_eu(8)))
+mc_pairs(int nslots, int nbv, const int *__restrict__ bvslot, const int4 *__restrict__ srec,
+         const int *__restrict__ entry, const int *__restrict__ pos, const float *__restrict__ ergpair,
+         const float4 *__restrict__ Pold, const float4 *__restrict__ Pnew, PairTab gt, int npairs,
+         const float *__restrict__ shift, float *__restrict__ A, float *__restrict__ Bv, float *__restrict__ L,
+         int coop_off)
 {
     extern __shared__ char smem[];
     PairTab t = LDS ? tables_to_lds(gt, npairs, smem) : gt;
+    const int lane = threadIdx.x & 63;
+    float4 *stage = COOP ? (float4 *)(smem + coop_off) + (threadIdx.x >> 6) * 192 : nullptr;
     // items [0, nslots): A (and L) of every slot; items [nslots, nslots + nbv): Bv of the
     // slots whose partner moves earlier in the chain (bvslot, from the ledger), so every
     // lane of a wave evaluates one molecule pair.  srec[i] holds the slot's molecule (bit
@@ -1467,19 +1474,50 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, 1024), amdgpu_wave
     for (int part = 0; part < 2; ++part) {
         const bool bv = part == 1;
         const XcdRange xr = xcd_range(bv ? nbv : nslots);
-        for (long long it = xr.first; it < xr.hi; it += xr.stride) {
-            const int i = bv ? bvslot[it] : (int)it;
-            const int4 r = srec[i];
-            if (!bv) {
-                L[i] = ergpair[entry[i]];
-                if (r.x < 0) continue;
+        const float4 *P2a = bv ? Pnew : Pold;
+        // the wave's items are w0 .. w0 + 63 (w0 is the same in every lane)
+        for (long long w0 = xr.first - lane; w0 < xr.hi; w0 += xr.stride) {
+            const long long it = w0 + lane;
+            int i = 0;
+            int4 r = make_int4(0, 0, 0, 0);
+            bool doit = false;
+            if (it < xr.hi) {
+                i = bv ? bvslot[it] : (int)it;
+                r = srec[i];
+                if (!bv) L[i] = ergpair[entry[i]];
+                doit = bv || r.x >= 0;
+                // rejected by the surrogate stage: no energy
+                if (doit && shift && isnan(shift[pos[r.x & 0x7fffffff]])) doit = false;
             }
-            if (shift && isnan(shift[pos[r.x & 0x7fffffff]])) continue;   // rejected by the surrogate stage
-            const float4 *P1 = Pnew + (r.z & 0xffffff), *P2 = (bv ? Pnew : Pold) + (r.w & 0xffffff);
+            const float4 *P1 = Pnew + (r.z & 0xffffff);
             const int n1 = (int)((unsigned)r.z >> 24), n2 = (int)((unsigned)r.w >> 24);
-            const float e = mol_pair_any(P1, n1, P2, n2, t);
-            if (bv) Bv[i] = e;
-            else A[i] = e;
+            float e = 0.0f;
+            if constexpr (COOP) {
+                // every molecule has at most three atoms: lane f of load k fetches atom f % 3
+                // of item f / 3's partner (when the partner has it)
+                const int b2 = r.w & 0xffffff;
+                const int nd = doit ? n2 : 0;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const int f = 64 * k + lane, src = f / 3, atom = f - 3 * src;
+                    const int bb = __shfl(b2, src), na = __shfl(nd, src);
+                    stage[f] = atom < na ? P2a[bb + atom] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                const float4 q[3] = {stage[3 * lane], stage[3 * lane + 1], stage[3 * lane + 2]};
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                if (doit) e = mol_pair3_q(P1, n1, q, n2, t);
+            } else if (doit) {
+                e = mol_pair_any(P1, n1, P2a + (r.w & 0xffffff), n2, t);
+            }
+            if (doit) {
+                if (bv) Bv[i] = e;
+                else A[i] = e;
+            }
         }
     }
 }
@@ -2555,16 +2593,25 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
             da_stage1<<<ceil_div(nmol, 256), 256, 0, st>>>(nmol, l->d_order, dy, da->des, temp, da->seed, dshift,
                                                           rec ? rec->des : nullptr, rec ? rec->pass : nullptr, dnrej);
         timer_begin(ctx, "mc_pair");
-        if (pt.in_lds) {
-            int g = xcd_grid(persistent_blocks(pt.lds_bytes, (long long)nslots + l->nbv, box->pair_block));
-            mc_pairs<true><<<g, box->pair_block, pt.lds_bytes, st>>>(
+        // three-atom molecules: the partners' atoms loaded by the wave together, staged in
+        // 192 float4 of LDS per wave after the tables
+        const size_t coop_off = (pt.lds_bytes + 15) & ~(size_t)15;
+        const size_t coop_lds = coop_off + (size_t)(box->pair_block / 64) * 192 * sizeof(float4);
+        if (pt.in_lds && box->max_atoms_per_mol <= 3 && coop_lds <= 160 * 1024) {
+            int g = xcd_grid(persistent_blocks(coop_lds, (long long)nslots + l->nbv, box->pair_block));
+            mc_pairs<true, true><<<g, box->pair_block, coop_lds, st>>>(
                 nslots, l->nbv, l->d_bvslot, l->d_srec, l->d_entry, l->d_pos, l->d_ergpair, Pold, Pnew, pt.tab,
-                box->npairs, dshift, dA, dB, dL);
+                box->npairs, dshift, dA, dB, dL, (int)coop_off);
+        } else if (pt.in_lds) {
+            int g = xcd_grid(persistent_blocks(pt.lds_bytes, (long long)nslots + l->nbv, box->pair_block));
+            mc_pairs<true, false><<<g, box->pair_block, pt.lds_bytes, st>>>(
+                nslots, l->nbv, l->d_bvslot, l->d_srec, l->d_entry, l->d_pos, l->d_ergpair, Pold, Pnew, pt.tab,
+                box->npairs, dshift, dA, dB, dL, 0);
         } else {
             int g = xcd_grid(persistent_blocks(0, (long long)nslots + l->nbv, box->pair_block));
-            mc_pairs<false><<<g, box->pair_block, 0, st>>>(
+            mc_pairs<false, false><<<g, box->pair_block, 0, st>>>(
                 nslots, l->nbv, l->d_bvslot, l->d_srec, l->d_entry, l->d_pos, l->d_ergpair, Pold, Pnew, pt.tab,
-                box->npairs, dshift, dA, dB, dL);
+                box->npairs, dshift, dA, dB, dL, 0);
         }
         timer_end(ctx, "mc_pair");
         if (rec && rec->target)

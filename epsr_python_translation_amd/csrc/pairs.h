@@ -211,6 +211,21 @@ __device__ __forceinline__ float mol_pair3(const float4 *__restrict__ P1, int n1
     return sume;
 }
 
+// mol_pair3 with molecule 2's three atoms given (loaded by the caller)
+__device__ __forceinline__ float mol_pair3_q(const float4 *__restrict__ P1, int n1, const float4 (&q)[3], int n2,
+                                             const PairTab &t)
+{
+    const unsigned rowm = (1u << n2) - 1u;
+    float sume = 0.0f;
+#pragma unroll
+    for (int ia = 0; ia < 3; ++ia) {
+        if (ia >= n1) break;
+        const float4 p = P1[ia];   // (one atom at a time: fewer registers live)
+        sume = pair_block<3, 3>(sume, &p, q, rowm, t);
+    }
+    return sume;
+}
+
 // the pair energy of two molecules, through mol_pair3 when neither has more than three atoms
 __device__ __forceinline__ float mol_pair_any(const float4 *__restrict__ P1, int na1, const float4 *__restrict__ P2,
                                               int nb2, const PairTab &t)
