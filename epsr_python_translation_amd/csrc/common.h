@@ -45,6 +45,8 @@ struct epsr_ctx {
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // marks the end of a call's work for ctx_wait
     hipEvent_t ev_done = nullptr;
+    // mark the chunks of a copy that the host consumes as they land (epsr_press)
+    hipEvent_t ev_chunk[8] = {};
 };
 
 // intramolecular topology of a box (epsr_box_set_intra), device copies

@@ -4,11 +4,19 @@
 // Work split: one workgroup per molecule m, one lane per ledger slot of m.  A lane whose
 // partner n comes later in the numbering forms the pair's float32 energy and virial over
 // every atom pair (atoms of m outer, atoms of n inner) exactly as the reference orders
-// them; other lanes leave 0.  The two per-slot values go to host memory and the host
-// adds them in the reference's order (molecule, then ledger slot): the energy as a
-// float32 running sum, the virial as a double one.  Those two sums are sequential by
-// definition, and a host loop over a few million values is cheaper than the same chain
-// of dependent adds on one GPU lane.
+// them; other lanes leave 0.  The host adds the per-slot values in the reference's order
+// (molecule, then ledger slot): the energy as a float32 running sum, the virial as a
+// double one.  Those two sums are sequential by definition, and a host loop is cheaper
+// than the same chain of dependent adds on one GPU lane.  Two things shorten it:
+//  - a slot whose energy and virial are both zero is left out.  Adding a zero changes a
+//    running sum only when the sum is -0, and neither sum can be: each starts at +0, and
+//    a round-to-nearest sum that is exactly zero is +0 unless both terms are -0.  About
+//    half the slots (the partners numbered lower) are such zeros;
+//  - the remaining (energy, virial) pairs are gathered on the GPU into molecule order, one
+//    contiguous array, and copied to the host in chunks, each summed as soon as it lands
+//    while the next is in flight.
+#include <algorithm>
+
 #include "common.h"
 
 namespace {
@@ -20,8 +28,13 @@ press_slots(int nmol, const int *__restrict__ rowstart, const int *__restrict__ 
             const float *__restrict__ za, const float *__restrict__ xadd, const float *__restrict__ yadd,
             const float *__restrict__ zadd, const float *__restrict__ xstar, const float *__restrict__ xavs,
             const float *__restrict__ xavs1, const float *__restrict__ ecoul, const float *__restrict__ ecoul1,
-            int ld, int npt1, float stepcl, float *__restrict__ slote, float *__restrict__ slotw)
+            int ld, int npt1, float stepcl, float *__restrict__ slote, float *__restrict__ slotw,
+            int *__restrict__ nnz)
 {
+    __shared__ int nzs;
+    if (threadIdx.x == 0) nzs = 0;
+    __syncthreads();
+    int mine = 0;
     const int im = blockIdx.x;
     const int row = rowstart[im], c = cnt[im];
     const float xc1 = xadd[im], yc1 = yadd[im], zc1 = zadd[im];
@@ -62,6 +75,58 @@ press_slots(int nmol, const int *__restrict__ rowstart, const int *__restrict__ 
         }
         slote[row + s] = ergold;
         slotw[row + s] = ergnew;
+        mine += (ergold != 0.0f || ergnew != 0.0f) ? 1 : 0;
+    }
+    if (mine) atomicAdd(&nzs, mine);
+    __syncthreads();
+    if (threadIdx.x == 0) nnz[im] = nzs;
+}
+
+// off[m] = nnz[0] + ... + nnz[m-1] for m <= n (one workgroup of 1024 lanes, each over a
+// contiguous run of molecules)
+__global__ void __launch_bounds__(1024) press_scan(int n, const int *__restrict__ nnz, int *__restrict__ off)
+{
+    __shared__ int part[1024];
+    const int per = (n + 1023) / 1024, lo = min(n, (int)threadIdx.x * per), hi = min(n, lo + per);
+    int t = 0;
+    for (int m = lo; m < hi; ++m) t += nnz[m];
+    part[threadIdx.x] = t;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {   // inclusive scan of the run totals
+        const int v = threadIdx.x >= (unsigned)d ? part[threadIdx.x - d] : 0;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int base = part[threadIdx.x] - t;
+    for (int m = lo; m < hi; ++m) {
+        off[m] = base;
+        base += nnz[m];
+    }
+    if (threadIdx.x == 1023) off[n] = part[1023];
+}
+
+// one wave per molecule m < n: its non-zero slots, in slot order, to out[off[m]...]
+__global__ void __launch_bounds__(256)
+press_gather(int n, const int *__restrict__ rowstart, const int *__restrict__ cnt, const int *__restrict__ off,
+             const float *__restrict__ slote, const float *__restrict__ slotw, float2 *__restrict__ out)
+{
+    const int lane = threadIdx.x & 63, m = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (m >= n) return;
+    const int row = rowstart[m], c = cnt[m];
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int base = off[m];
+    for (int s0 = 0; s0 < c; s0 += 64) {
+        const int s = s0 + lane;
+        float e = 0.0f, w = 0.0f;
+        if (s < c) {
+            e = slote[row + s];
+            w = slotw[row + s];
+        }
+        const bool keep = e != 0.0f || w != 0.0f;
+        const unsigned long long mask = __ballot(keep);
+        if (keep) out[base + __popcll(mask & below)] = make_float2(e, w);
+        base += __popcll(mask);
     }
 }
 
@@ -84,32 +149,65 @@ extern "C" int epsr_press(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_
     float sumc = 0.0f;
     double pt = 0.0;
     if (nmol >= 2 && ns > 0) {
-        float *dE = (float *)ctx_scratch(ctx, sizeof(float) * 2 * (size_t)ns);
-        if (!dE) return EPSR_ERR_HIP;
-        float *dW = dE + ns;
+        const int nm = nmol - 1;
+        // scratch: slot energies and virials, non-zero counts and offsets, the gathered pairs
+        const size_t oE = 0, oW = oE + sizeof(float) * (size_t)ns, oN = oW + sizeof(float) * (size_t)ns,
+                     oO = oN + sizeof(int) * (size_t)nm, oV = (oO + sizeof(int) * ((size_t)nm + 1) + 15) & ~(size_t)15,
+                     sb = oV + sizeof(float2) * (size_t)ns;
+        char *scr = (char *)ctx_scratch(ctx, sb);
+        if (!scr) return EPSR_ERR_HIP;
+        float *dE = (float *)(scr + oE), *dW = (float *)(scr + oW);
+        int *dN = (int *)(scr + oN), *dO = (int *)(scr + oO);
+        float2 *dV = (float2 *)(scr + oV);
         const float stepcl = box->cellst / box->dmax;
-        press_slots<<<nmol - 1, 128, 0, st>>>(nmol, l->d_rowstart, l->d_cnt, l->d_partner, box->d_naf, box->d_nal,
-                                               box->d_itype, box->d_npairref, box->ntype, c->xa, c->ya, c->za,
-                                               c->xadd, c->yadd, c->zadd, tb->xstar, tb->xavs, tb->xavs1, tb->ecoul,
-                                               tb->ecoul1, tb->ld, npt1, stepcl, dE, dW);
+        timer_begin(ctx, "press_slots");
+        press_slots<<<nm, 128, 0, st>>>(nmol, l->d_rowstart, l->d_cnt, l->d_partner, box->d_naf, box->d_nal,
+                                        box->d_itype, box->d_npairref, box->ntype, c->xa, c->ya, c->za, c->xadd,
+                                        c->yadd, c->zadd, tb->xstar, tb->xavs, tb->xavs1, tb->ecoul, tb->ecoul1,
+                                        tb->ld, npt1, stepcl, dE, dW, dN);
         HIP_CHECK(ctx, hipGetLastError());
-        const size_t hb = sizeof(float) * 2 * (size_t)ns + sizeof(int) * (2 * (size_t)nmol + 1);
+        timer_end(ctx, "press_slots");
+        timer_begin(ctx, "press_gather");
+        press_scan<<<1, 1024, 0, st>>>(nm, dN, dO);
+        HIP_CHECK(ctx, hipGetLastError());
+        press_gather<<<(nm + 3) / 4, 256, 0, st>>>(nm, l->d_rowstart, l->d_cnt, dO, dE, dW, dV);
+        HIP_CHECK(ctx, hipGetLastError());
+        timer_end(ctx, "press_gather");
+        const bool want = slote_h || slotw_h;
+        const size_t hs = want ? sizeof(float) * 2 * (size_t)ns + sizeof(int) * (2 * (size_t)nmol + 1) : 0;
+        const size_t hb = 16 + sizeof(float2) * (size_t)ns + hs;
         char *h = (char *)ctx_pinned(ctx, hb);
         if (!h) return EPSR_ERR_HIP;
-        float *hE = (float *)h, *hW = hE + ns;
-        int *hrow = (int *)(hW + ns), *hcnt = hrow + nmol + 1;
-        HIP_CHECK(ctx, hipMemcpyAsync(hE, dE, sizeof(float) * 2 * (size_t)ns, hipMemcpyDeviceToHost, st));
-        HIP_CHECK(ctx, hipMemcpyAsync(hrow, l->d_rowstart, sizeof(int) * (nmol + 1), hipMemcpyDeviceToHost, st));
-        HIP_CHECK(ctx, hipMemcpyAsync(hcnt, l->d_cnt, sizeof(int) * nmol, hipMemcpyDeviceToHost, st));
+        int *htot = (int *)h;
+        float2 *hV = (float2 *)(h + 16);
+        HIP_CHECK(ctx, hipMemcpyAsync(htot, dO + nm, sizeof(int), hipMemcpyDeviceToHost, st));
         HIP_CHECK(ctx, hipStreamSynchronize(st));
-        for (int m = 0; m < nmol - 1; ++m) {
-            const float *e = hE + hrow[m], *w = hW + hrow[m];
-            for (int s = 0; s < hcnt[m]; ++s) {
-                sumc = sumc + e[s];
-                pt = pt + (double)w[s];
+        const long long tot = *htot;
+        ARG_CHECK(ctx, tot >= 0 && tot <= ns, "epsr_press: %lld non-zero slots of %d", tot, ns);
+        // chunks of at least 4096 pairs, at most 8 of them
+        const long long chunk = std::max(4096LL, (tot + 7) / 8);
+        int nch = 0;
+        for (long long lo = 0; lo < tot; lo += chunk, ++nch) {
+            if (!ctx->ev_chunk[nch]) HIP_CHECK(ctx, hipEventCreateWithFlags(&ctx->ev_chunk[nch], hipEventDisableTiming));
+            const long long n = std::min(chunk, tot - lo);
+            HIP_CHECK(ctx, hipMemcpyAsync(hV + lo, dV + lo, sizeof(float2) * n, hipMemcpyDeviceToHost, st));
+            HIP_CHECK(ctx, hipEventRecord(ctx->ev_chunk[nch], st));
+        }
+        for (int k = 0; k < nch; ++k) {
+            HIP_CHECK(ctx, hipEventSynchronize(ctx->ev_chunk[k]));
+            const long long lo = k * chunk, hi = std::min(tot, lo + chunk);
+            for (long long i = lo; i < hi; ++i) {
+                sumc = sumc + hV[i].x;
+                pt = pt + (double)hV[i].y;
             }
         }
-        if (slote_h || slotw_h) {
+        if (want) {
+            float *hE = (float *)(h + 16 + sizeof(float2) * (size_t)ns), *hW = hE + ns;
+            int *hrow = (int *)(hW + ns), *hcnt = hrow + nmol + 1;
+            HIP_CHECK(ctx, hipMemcpyAsync(hE, dE, sizeof(float) * 2 * (size_t)ns, hipMemcpyDeviceToHost, st));
+            HIP_CHECK(ctx, hipMemcpyAsync(hrow, l->d_rowstart, sizeof(int) * (nmol + 1), hipMemcpyDeviceToHost, st));
+            HIP_CHECK(ctx, hipMemcpyAsync(hcnt, l->d_cnt, sizeof(int) * nmol, hipMemcpyDeviceToHost, st));
+            HIP_CHECK(ctx, hipStreamSynchronize(st));
             for (int m = 0; m < nmol; ++m)
                 for (int s = 0; s < hcnt[m]; ++s) {
                     const bool last = m == nmol - 1;

@@ -49,6 +49,8 @@ extern "C" void epsr_ctx_destroy(epsr_ctx *ctx)
     if (ctx->ev_fork) hipEventDestroy(ctx->ev_fork);
     if (ctx->ev_done) hipEventDestroy(ctx->ev_done);
     if (ctx->ev_join) hipEventDestroy(ctx->ev_join);
+    for (hipEvent_t e : ctx->ev_chunk)
+        if (e) hipEventDestroy(e);
     if (ctx->scratch) hipFree(ctx->scratch);
     if (ctx->pinned) hipHostFree(ctx->pinned);
     if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
