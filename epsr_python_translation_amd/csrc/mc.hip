@@ -1626,33 +1626,30 @@ __global__ void mc_commit_mols(int nmol, const int *__restrict__ order, const in
     }
 }
 
-// each pair entry is finalised by its slot in the row of the molecule that moves later
-// ledger write-back (molfit.py:417-447): the entry of a pair takes the energy of the
-// later of its two molecules' accepted moves.  Items: the slots whose partner comes
-// earlier (bvslot; they hold the later molecule's side of every pair but the first slots),
-// then the first slot of every molecule, which may also be a pair slot.
-__global__ void mc_commit_ledger(int nbv, int nmol, const int *__restrict__ bvslot, const int *__restrict__ rowstart,
-                                 const int *__restrict__ cnt, const int4 *__restrict__ srec, const int *__restrict__ pos,
-                                 const int *__restrict__ pnv, const int *__restrict__ mw,
-                                 const int *__restrict__ entry, const unsigned *__restrict__ accb,
-                                 const float *__restrict__ A, const float *__restrict__ Bv, const int *__restrict__ redo,
-                                 float *__restrict__ ergpair)
+// ledger write-back (molfit.py:417-447): an accepted move rewrites the entries of its list's
+// slots s >= 1 with their new energies, in chain order, so an entry ends with the value of
+// the later of its two molecules' writes.  One wave per accepted chain position k writes
+// the entries whose last write is its own: for a partner earlier in the chain, always
+// (the energy against the partner's final position: Bv when the partner moved, else A);
+// for a later partner, unless the partner was accepted and the pair is not in slot 0 of
+// the partner's list (mw[i] >= 0), when the partner's move writes last.  Exactly one move
+// writes each rewritten entry.
+__global__ void __launch_bounds__(256)
+mc_commit_ledger(int nmol, const int *__restrict__ rowk, const int *__restrict__ pnv, const int *__restrict__ mw,
+                 const int *__restrict__ entry, const unsigned *__restrict__ accb, const float *__restrict__ A,
+                 const float *__restrict__ Bv, const int *__restrict__ redo, float *__restrict__ ergpair)
 {
-    const int t = xcd_block() * blockDim.x + threadIdx.x;   // chain order, each XCD a contiguous part
-    if (t >= nbv + nmol || *redo) return;
-    const bool first = t >= nbv;
-    if (first && cnt[t - nbv] == 0) return;   // an empty list
-    const int i = first ? rowstart[t - nbv] : bvslot[t];
-    const int4 r = srec[i];
-    const int m = r.x & 0x7fffffff;
-    if (r.y == m) return;   // the molecule's own entry
-    const int pm = pos[m], pn = pnv[i];
-    if (pm < pn) return;
-    const bool am = (accb[pm >> 5] >> (pm & 31)) & 1u;
-    const bool an = (accb[pn >> 5] >> (pn & 31)) & 1u;
-    const int mi = mw[i];
-    if (am && !first) ergpair[entry[i]] = an ? Bv[i] : A[i];
-    else if (an && mi >= 0) ergpair[entry[i]] = A[mi];
+    const int lane = threadIdx.x & 63;
+    const int k = xcd_block() * 4 + (threadIdx.x >> 6);   // chain order, each XCD a contiguous part
+    if (k >= nmol || *redo) return;
+    if (!((accb[k >> 5] >> (k & 31)) & 1u)) return;
+    const int row = rowk[k], c = rowk[k + 1] - row;
+    for (int s = 1 + lane; s < c; s += 64) {
+        const int i = row + s, pn = pnv[i];
+        const bool an = (accb[pn >> 5] >> (pn & 31)) & 1u;
+        if (pn < k) ergpair[entry[i]] = an ? Bv[i] : A[i];
+        else if (!an || mw[i] < 0) ergpair[entry[i]] = A[i];
+    }
 }
 
 // ------------------------------------------------------------------ fast resolution
@@ -1829,6 +1826,23 @@ mc_prepare(int nmol, const int *__restrict__ rowk, const int *__restrict__ pnv, 
     }
 }
 
+// fdiff of molfit.py:397-405: the float32 sum fdiff = fdiff + new_s - old_s over the slots
+// s in [1, c) in order, from (new, old) pairs in LDS read eight at a time ahead of the adds
+__device__ __forceinline__ float serial_fdiff(const float2 *term, int c)
+{
+    float fdiff = 0.0f;
+    int s = 1;
+    for (; s + 8 <= c; s += 8) {
+        float2 v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = term[s + j];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) fdiff = fdiff + v[j].x - v[j].y;
+    }
+    for (; s < c; ++s) fdiff = fdiff + term[s].x - term[s].y;
+    return fdiff;
+}
+
 // exact fdiff of every accepted move (for fadd): one wave per chain position gathers
 // the selected variant of every slot into LDS, lane 0 sums them in the reference's order
 __global__ void __launch_bounds__(256)
@@ -1847,24 +1861,28 @@ mc_fdiff_accepted(int nmol, const int *__restrict__ rowk, const int *__restrict_
     if (!((bits[k >> 5] >> (k & 31)) & 1u)) return;
     const int row = rowk[k], c = rowk[k + 1] - row;
     float2 *t = fterm + (size_t)wv * stride;
-    for (int s = 1 + lane; s < c; s += 64) {
-        int i = row + s;
-        int pn = pnv[i];
-        bool a = pn < k && ((bits[pn >> 5] >> (pn & 31)) & 1u);
-        int mi = mw[i];
-        t[s] = make_float2(a ? Bv[i] : A[i], (a && mi >= 0) ? A[mi] : L[i]);
+    // the terms, four chunks of 64 slots with their loads in flight together
+    for (int g0 = 1; g0 < c; g0 += 256) {
+        float2 v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int s = g0 + 64 * j + lane;
+            if (s < c) {
+                const int i = row + s;
+                const int pn = pnv[i];
+                const bool a = pn < k && ((bits[pn >> 5] >> (pn & 31)) & 1u);
+                const int mi = mw[i];
+                v[j] = make_float2(a ? Bv[i] : A[i], (a && mi >= 0) ? A[mi] : L[i]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (g0 + 64 * j + lane < c) t[g0 + 64 * j + lane] = v[j];
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    if (lane == 0) {
-        float fdiff = 0.0f;
-        for (int s = 1; s < c; ++s) {
-            float2 v = t[s];
-            fdiff = fdiff + v.x - v.y;
-        }
-        fdiff_out[k] = fdiff;
-    }
+    if (lane == 0) fdiff_out[k] = serial_fdiff(t, c);
 }
 
 // ------------------------------------------------------------------ parallel resolution
@@ -1933,8 +1951,7 @@ __device__ int mc_pass_move(int k, const int *__restrict__ rowk, const int *__re
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     int dd = 0;
     if (lane == 0) {
-        float fdiff = 0.0f;
-        for (int s = 1; s < c; ++s) fdiff = fdiff + term[s].x - term[s].y;
+        const float fdiff = serial_fdiff(term, c);
         const float ediff = (shift ? fdiff - shift[k] : fdiff + 0.0f) * temp;
         dd = stream[moff[k]] < etest_of(ediff) ? 1 : 2;
         atomicAdd(nexact, 1);
@@ -2672,9 +2689,8 @@ static int molfit_impl(epsr_ctx *ctx, const epsr_box *box, const epsr_coords_t *
                                             l->d_mirror, dA, dB, dL, stream, moff, temp, maxc, dacc, dfdiff, dres);
             timer_end(ctx, "mc_resolve");
         }
-        mc_commit_ledger<<<xcd_grid(ceil_div((long)l->nbv + nmol, 256)), 256, 0, st>>>(
-            l->nbv, nmol, l->d_bvslot, l->d_rowstart, l->d_cnt, l->d_srec, l->d_pos, l->d_pn, l->d_mw, l->d_entry, dacc, dA, dB,
-            redo, l->d_ergpair);
+        mc_commit_ledger<<<xcd_grid(ceil_div(nmol, 4)), 256, 0, st>>>(nmol, l->d_rowk, l->d_pn, l->d_mw, l->d_entry,
+                                                                   dacc, dA, dB, redo, l->d_ergpair);
         mc_commit_mols<<<ceil_div(nmol, 128), 128, 0, st>>>(nmol, l->d_order, box->d_naf, box->d_nal, box->d_mtype,
                                                             dacc, roff, toff, txa, tya, tza, txadd, tyadd, tzadd, c->xa,
                                                             c->ya, c->za, c->xadd, c->yadd, c->zadd, nstepreject,
