@@ -383,7 +383,7 @@ def iteration_extras(R, s, reps=5):
     elems = 2 * npairs * g.nq + 2 * npairs * (g.npt1 - 1)
     ms_a = float(np.median(ta[1:])) * 1e3
     return {"press": {"ms": float(np.median(tp[1:])) * 1e3, "energy_kj_mol": float(e), "pressure_kbar": float(p),
-                      "ledger_slots": int(R.ledger.nergpair)},
+                      "ledger_entries": int(R.ledger.nergpair)},
             "sumsq_sumgr": {"ms": ms_a, "elements": elems, "bytes": 20 * elems,
                             "GB_per_s_wall": 20 * elems / (ms_a * 1e-3) / 1e9}}
 
