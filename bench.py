@@ -230,41 +230,35 @@ def main():
     ktime = {}
     mc_bytes = []
 
-    def add(name, v):
-        if v >= 0:
-            ktime.setdefault(name, []).append(v)
+    families = ("fmolec1_pair", "mc_pair", "mc_resolve", "atogen_inter", "gensq", "stog")
 
     def step(record):
+        # every launch is timed by HIP events on its stream; they are read after the timed
+        # steps, so that no read waits on the GPU inside the step
         R.cells_and_ledger()
-        if record:
-            add("fmolec1_pair", ctx.kernel_ms("fmolec1_pair"))
-        # the sweeps back to back on the GPU (one host wait); every launch is timed
+        # the sweeps back to back on the GPU (one host wait)
         R.molfit(ncycles=args.sweeps)
-        for fam in ("mc_pair", "mc_resolve"):
-            t, n = ctx.kernel_ms_sum(fam)
-            if record and n:
-                ktime.setdefault(fam, []).extend([t / n] * n)
         if record:
             mc_bytes.append(R.mc.stats.resolver_bytes)
         R.atogen(ngrsamples=s.nmol)
-        if record:
-            add("atogen_inter", ctx.kernel_ms("atogen_inter"))
         R.gensq(mode=1)
-        if record:
-            add("gensq", ctx.kernel_ms("gensq"))
         D.stog(ctx, s.rho, R.q, R.sofq[s.npairs * R.ldq:], R.ldq, r, gofr, ldg, nqmin, nqmax, nr, s.npairs, xwt,
                mode=1)
-        if record:
-            add("stog", ctx.kernel_ms("stog"))
 
     for _ in range(args.warmup):
         step(False)
     torch.cuda.synchronize()
+    for fam in families:   # the warm-up launches' events, read and dropped
+        ctx.kernel_ms_sum(fam)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step(True)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    for fam in families:
+        t, n = ctx.kernel_ms_sum(fam)
+        if n:
+            ktime[fam] = [t / n] * n
     ms_per_step = 1000.0 * elapsed / args.steps
     ktot = {k: float(np.sum(v)) / args.steps for k, v in ktime.items()}
     kavg = {k: float(np.mean(v)) for k, v in ktime.items()}
