@@ -1,6 +1,8 @@
 """The reference's EPSR test box, read from its own .ato/.pcof/.EPSR.inp, through the GPU
-path: one MC sweep restarted from the ran1 state saved in the .ato against the oracle
-from the same state, bit for bit, then written back as an .ato and read again."""
+path: one MC sweep restarted from the ran1 state, the per-template step sizes and the
+per-molecule rejection counts saved in the .ato, against the oracle from the same state, bit
+for bit (coordinates, ran1, fadds, adapted steps, counters), then written back as an .ato and
+read again."""
 
 import ctypes as C
 
@@ -32,10 +34,13 @@ def test_epsr_test_box_restarts_from_its_ato(ctx, tmp_path):
     # oracle: ledger, then one molfit sweep from the saved state
     ref = H.oracle_fmolec1(name, "cr")
     ob = ref["ob"]
-    wr, wt = H.initial_steps(s)
+    # the run state the .ato saved: template step sizes (clustercom.wholerot/wholetrans as
+    # read, ato_read_and_save.py:682-692) and each molecule's rejection count
+    wr, wt = s.mc_run["wholerot"].copy(), s.mc_run["wholetrans"].copy()
+    assert not np.array_equal(wr, H.initial_steps(s)[0])
     nwh = np.zeros(2 * s.nmoltype, np.int32)
     nmv = np.zeros(2 * s.nmoltype, np.int32)
-    nsr = np.zeros(s.nmol, np.int32)
+    nsr = s.mc_run["nstepreject"].copy()
     mcs = po.MCState(po.Pf(wr), po.Pf(wt), po.Pi(nwh), po.Pi(nmv), po.Pi(nsr), s.simt, F32(1), F32(1), F32(0.75), 0)
     rng = po.Ran1.from_array(a.ran1)
     fadds, nacc = C.c_float(0.0), C.c_int()
@@ -49,8 +54,9 @@ def test_epsr_test_box_restarts_from_its_ato(ctx, tmp_path):
     led = D.Ledger(ctx, box)
     led.cells_init(s.nmolcell, T.sepmax(s, g))
     led.fmolec1(tab, H.eintra(name))
-    wr2, wt2 = H.initial_steps(s)
+    wr2, wt2 = s.mc_run["wholerot"].copy(), s.mc_run["wholetrans"].copy()
     mc = D.MCState(ctx, s.nmol, s.nmoltype, s.simt, wholerot=wr2, wholetrans=wt2)
+    mc.nstepreject.copy_(torch.from_numpy(s.mc_run["nstepreject"].copy()))
     st = np.ascontiguousarray(a.ran1.copy())
     D.molfit(ctx, box, tab, led, mc, st)
     torch.cuda.synchronize()
@@ -60,6 +66,9 @@ def test_epsr_test_box_restarts_from_its_ato(ctx, tmp_path):
         assert np.array_equal(co[k].view(np.int32), getattr(ob, k).view(np.int32)), k
     np.testing.assert_array_equal(st, rng.as_array())
     assert F32(mc.fadds).view(np.int32) == F32(fadds.value).view(np.int32)
+    np.testing.assert_array_equal(mc.wholerot.view(np.int32), wr.view(np.int32))
+    np.testing.assert_array_equal(mc.wholetrans.view(np.int32), wt.view(np.int32))
+    np.testing.assert_array_equal(mc.nstepreject.cpu().numpy(), nsr)
     # write the result as an .ato and read it back: the ran1 state exactly, the
     # coordinates to the file's precision
     b = FM.update_ato(a, co["xadd"], co["yadd"], co["zadd"], co["xa"], co["ya"], co["za"], ran1=st)
