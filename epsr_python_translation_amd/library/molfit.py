@@ -32,6 +32,10 @@ def _moves_inside_molecules(nisum, intra, rotfreq):
 def molfit(fadds, faddbs, nisum, intra, rotfreq, ireset):
     """One MC cycle on the ledger of the last fmolec1.  Returns (fadds, faddbs, nisum)."""
     global last_stats
+    if np.any(clustercom.tethered[1:clustercom.nmol + 1]):
+        # the reference's moves keep a tethered molecule's anchor atom in place
+        # (check_tethered, molfit.py:219, :317, :397); the GPU cycle has no such moves
+        raise NotImplementedError("molfit: tethered molecules are not handled by the GPU path")
     if gpu.intra is None and _moves_inside_molecules(nisum, intra, rotfreq):
         raise NotImplementedError("molfit: this cycle moves atoms inside molecules (shake or internal rotations), "
                                   "which needs the box's intramolecular topology (load it from .ato files)")

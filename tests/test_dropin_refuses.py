@@ -1,7 +1,8 @@
 """The drop-in molfit without an intramolecular topology refuses every cycle that would
 move atoms inside molecules -- the shake (nisum >= intra, molfit.py:124) and the cycles with
 internal rotations (stepri > 0, rotational groups, nisum a multiple of rotfreq, :295) --
-instead of running a different trajectory; the check runs before any GPU work."""
+instead of running a different trajectory, and refuses a box with tethered molecules; the
+checks run before any GPU work."""
 
 import numpy as np
 import pytest
@@ -36,3 +37,15 @@ def test_rotation_cycle_raises(loaded):
         M.molfit(F32(0), F32(0), 4, 100, 5, 0)
     with pytest.raises(ZeroDivisionError):
         M.molfit(F32(0), F32(0), 4, 100, 0, 0)
+
+
+def test_tethered_molecule_raises(loaded):
+    from epsr_python_translation_amd.common import clustercom
+    from epsr_python_translation_amd.library import molfit as M
+    clustercom.tethered[3] = True
+    clustercom.tetheredatom[3] = 1
+    with pytest.raises(NotImplementedError, match="tethered"):
+        M.molfit(F32(0), F32(0), 4, 100, 5, 0)
+    clustercom.tethered[3] = False
+    with pytest.raises(RuntimeError, match="ledger"):
+        M.molfit(F32(0), F32(0), 4, 100, 5, 0)
